@@ -1,0 +1,120 @@
+"""ctypes binding of ``libjaxiga_hip.so`` (the C ABI in ``include/jaxiga_hip.h``).
+
+The library is required: every compute entry point of the package goes
+through it, and a missing or unloadable library raises immediately -- there is
+no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libjaxiga_hip.so"
+
+c_int = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+P_int = ctypes.POINTER(ctypes.c_int32)
+P_dbl = ctypes.POINTER(ctypes.c_double)
+
+IGA_POISSON, IGA_ELASTICITY = 0, 1
+IGA_COEF_CONST, IGA_COEF_POINTS = 0, 1
+IGA_SRC_NONE, IGA_SRC_CONST, IGA_SRC_POINTS = 0, 1, 2
+IGA_FP64, IGA_FP32, IGA_BF16 = 0, 1, 2
+
+STATUS = {-1: "inconsistent shapes", -2: "unsupported degree/dimension/quadrature",
+          -3: "HIP runtime error", -4: "null pointer or invalid argument"}
+
+
+class IgaPatch(ctypes.Structure):
+    _fields_ = [
+        ("dim", c_int), ("degree", c_int), ("n_elem", c_int * 3), ("n_cp", c_int * 3),
+        ("ext", c_void_p * 3), ("first", c_void_p * 3), ("lo", c_void_p * 3), ("hi", c_void_p * 3),
+        ("cpts", c_void_p), ("wgts", c_void_p),
+    ]
+
+
+class IgaRule(ctypes.Structure):
+    _fields_ = [("n_q1", c_int), ("pts", c_void_p), ("wts", c_void_p)]
+
+
+class IgaAsmArgs(ctypes.Structure):
+    _fields_ = [
+        ("problem", c_int), ("vec", c_int), ("precision", c_int), ("coef_kind", c_int),
+        ("coef_const", c_double), ("coef_points", c_void_p), ("cmat", c_double * 36),
+        ("src_kind", c_int), ("src_const", c_double * 3), ("src_points", c_void_p),
+        ("full_to_free", c_void_p), ("gfull", c_void_p), ("indptr", c_void_p), ("cols", c_void_p),
+        ("vals", c_void_p), ("rhs", c_void_p), ("diag", c_void_p),
+    ]
+
+
+_SIGNATURES = {
+    "iga_assemble": (c_int, [ctypes.POINTER(IgaPatch), ctypes.POINTER(IgaRule),
+                             ctypes.POINTER(IgaAsmArgs), c_void_p]),
+    "iga_basis_eval": (c_int, [ctypes.POINTER(IgaPatch), c_int, c_void_p, c_int, c_void_p, c_int,
+                               c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p]),
+    "iga_load_points": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_csr_count": (c_int, [c_int, P_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                              c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_csr_fill": (c_int, [c_int, P_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                             c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p]),
+    "iga_scan_workspace": (c_i64, [c_int]),
+    "iga_scan_exclusive": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_csr_diag_inv": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_spmv": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_cg_workspace": (c_i64, [c_int]),
+    "iga_cg_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_residual_norms": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+    "iga_rff_eval": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_double,
+                             c_void_p, c_void_p]),
+}
+
+_LIB = None
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def load():
+    """Load and type the library; raises ``RuntimeError`` if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = Path(os.environ.get("JAXIGA_HIP_LIB", LIB_PATH))
+    if not path.exists():
+        raise RuntimeError(
+            f"jaxiga_amd: HIP library {path} is missing; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)"
+        )
+    lib = ctypes.CDLL(str(path))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {STATUS.get(rc, 'status')} ({rc})")
+
+
+def ptr(t):
+    """Device (or host) address of a tensor, or None."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

@@ -1,0 +1,196 @@
+// Shared device helpers for the gfx950 Galerkin kernels: compile-time shape
+// arithmetic, Bernstein evaluation, wave reductions, the NURBS quotient with
+// the FP64 push-forward, and the deterministic last-block reduction used by
+// the Krylov kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/jaxiga_hip.h"
+
+namespace iga {
+
+constexpr int kWave = 64;
+
+__host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+__host__ __device__ constexpr int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+#define IGA_CHECK_LAUNCH()                              \
+  do {                                                  \
+    if (hipGetLastError() != hipSuccess) return (int)IGA_ERR_HIP; \
+  } while (0)
+
+// Degree-P Bernstein polynomials on [-1,1] and their derivatives at u, built
+// by the two-term recurrence B^j_k = s B^{j-1}_k + t B^{j-1}_{k-1} with
+// s = (1-u)/2, t = (1+u)/2; dB_k = P/2 (B^{P-1}_{k-1} - B^{P-1}_k).
+template <int P>
+__device__ __forceinline__ void bernstein(double u, double* B, double* dB) {
+  const double s = 0.5 * (1.0 - u), t = 0.5 * (1.0 + u);
+  double low[P + 1];
+  low[0] = 1.0;
+#pragma unroll
+  for (int j = 1; j < P; ++j) {
+    low[j] = t * low[j - 1];
+#pragma unroll
+    for (int k = j - 1; k >= 1; --k) low[k] = s * low[k] + t * low[k - 1];
+    low[0] = s * low[0];
+  }
+#pragma unroll
+  for (int k = 0; k <= P; ++k) {
+    const double a = (k >= 1) ? low[k - 1] : 0.0;
+    const double b = (k < P) ? low[k] : 0.0;
+    B[k] = s * b + t * a;
+    dB[k] = 0.5 * P * (a - b);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = W / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, W);
+  return v;
+}
+
+// Rational basis and physical gradients at one point for the lane holding
+// basis function i of a group of NLP lanes (lanes >= n_l pass active=false).
+// J[a][c] = d x_c / d xi_a; dR_phys solves J dR_phys = dR_param.
+template <int DIM, int NLP>
+struct PointGeo {
+  double R;
+  double dR[DIM];
+  double x[DIM];
+  double J[DIM][DIM];
+  double detJ;
+};
+
+template <int DIM, int NLP>
+__device__ __forceinline__ void rational_pushforward(bool active, double N, const double* dN,
+                                                     double wgt, const double* P,
+                                                     PointGeo<DIM, NLP>& g) {
+  const double M = active ? N * wgt : 0.0;
+  double dM[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) dM[k] = active ? dN[k] * wgt : 0.0;
+  double S0 = group_sum<NLP>(M);
+  double S1[DIM], X[DIM], Y[DIM][DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) X[c] = group_sum<NLP>(M * P[c]);
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    S1[k] = group_sum<NLP>(dM[k]);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) Y[k][c] = group_sum<NLP>(dM[k] * P[c]);
+  }
+  const double iW = 1.0 / S0;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) g.x[c] = X[c] * iW;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) g.J[k][c] = (Y[k][c] - S1[k] * g.x[c]) * iW;
+  g.R = M * iW;
+  double dRp[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) dRp[k] = (dM[k] - g.R * S1[k]) * iW;
+  const auto& J = g.J;
+  if constexpr (DIM == 2) {
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double id = 1.0 / det;
+    g.detJ = det;
+    g.dR[0] = (J[1][1] * dRp[0] - J[0][1] * dRp[1]) * id;
+    g.dR[1] = (-J[1][0] * dRp[0] + J[0][0] * dRp[1]) * id;
+  } else {
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double id = 1.0 / det;
+    g.detJ = det;
+    // inverse = adj(J)^T / det ; inv[c][k]
+    const double i00 = c00 * id, i10 = c01 * id, i20 = c02 * id;
+    const double i01 = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    const double i11 = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    const double i21 = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    const double i02 = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    const double i12 = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    const double i22 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    g.dR[0] = i00 * dRp[0] + i01 * dRp[1] + i02 * dRp[2];
+    g.dR[1] = i10 * dRp[0] + i11 * dRp[1] + i12 * dRp[2];
+    g.dR[2] = i20 * dRp[0] + i21 * dRp[1] + i22 * dRp[2];
+  }
+}
+
+// Block-wide FP64 sum (blockDim.x a multiple of 64, <= 1024); result valid in
+// thread 0.
+__device__ __forceinline__ double block_sum(double v, double* scratch32) {
+  v = group_sum<kWave>(v);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) scratch32[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x / kWave;
+    for (int i = 0; i < nw; ++i) s += scratch32[i];
+  }
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Deterministic grid reduction of NV values: each block stores its partials,
+// the last block to arrive sums all partials in block order and calls fin()
+// from thread 0 with the totals. `counter` must start at 0; it is reset.
+template <int NV, typename Fin>
+__device__ __forceinline__ void grid_reduce(const double (&v)[NV], double* partials,
+                                            unsigned int* counter, Fin fin) {
+  __shared__ double scratch[16];
+  __shared__ bool is_last;
+  double bs[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) bs[k] = block_sum(v[k], scratch);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(&partials[k * gridDim.x + blockIdx.x], bs[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double tot[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
+      s += load_coherent(&partials[k * gridDim.x + i]);
+    tot[k] = block_sum(s, scratch);
+  }
+  if (threadIdx.x == 0) {
+    fin(tot);
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ __forceinline__ int lower_bound(const int32_t* a, int lo, int hi, int key) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace iga

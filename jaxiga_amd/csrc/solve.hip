@@ -1,0 +1,530 @@
+// CSR structure, sparse matrix-vector products and the device-resident
+// Jacobi-preconditioned conjugate-gradient solver.
+//
+// CG keeps every scalar on the device (no host round trip per iteration):
+// each iteration is three kernels -- SpMV fused with p.Ap, the x/r/z update
+// fused with r.z and r.r, and the direction update -- whose dot products are
+// reduced deterministically (fixed block order) by the last block to finish.
+// Kernels read a device `done` flag, so the host enqueues iterations in
+// batches and synchronises only once per batch.
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+
+namespace iga {
+
+// ---------------------------------------------------------------------------
+// CSR pattern of the free-free block on a tensor patch
+// ---------------------------------------------------------------------------
+
+struct NbInfo {
+  int dim, vec;
+  int ncp[3];
+  const int32_t* lo[3];
+  const int32_t* hi[3];
+};
+
+template <bool FILL>
+__global__ void k_csr_rows(NbInfo nb, int n_free, const int32_t* __restrict__ freed,
+                           const int32_t* __restrict__ f2f, int32_t* counts_or_indptr,
+                           int32_t* cols, int32_t* rows) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_free) return;
+  const int g = freed[r];
+  const int s = g / nb.vec;
+  int ii[3] = {0, 0, 0};
+  {
+    int rem = s;
+    for (int d = 0; d < nb.dim; ++d) {
+      ii[d] = rem % nb.ncp[d];
+      rem /= nb.ncp[d];
+    }
+  }
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < nb.dim; ++d) {
+    lo[d] = nb.lo[d][ii[d]];
+    hi[d] = nb.hi[d][ii[d]];
+  }
+  const int s0 = nb.ncp[0], s1 = nb.ncp[0] * (nb.dim > 1 ? nb.ncp[1] : 1);
+  int pos = FILL ? counts_or_indptr[r] : 0;
+  for (int j2 = lo[2]; j2 <= hi[2]; ++j2)
+    for (int j1 = lo[1]; j1 <= hi[1]; ++j1)
+      for (int j0 = lo[0]; j0 <= hi[0]; ++j0) {
+        const int sj = j0 + j1 * s0 + j2 * s1;
+        for (int c = 0; c < nb.vec; ++c) {
+          const int f = f2f[(size_t)nb.vec * sj + c];
+          if (f < 0) continue;
+          if (FILL) {
+            cols[pos] = f;
+            if (rows) rows[pos] = r;
+          }
+          ++pos;
+        }
+      }
+  if (!FILL) counts_or_indptr[r] = pos;
+}
+
+static int make_nb(int dim, const int32_t* n_cp, const int32_t* const* lo,
+                   const int32_t* const* hi, int vec, NbInfo& nb) {
+  if (!n_cp || !lo || !hi || dim < 1 || dim > 3 || vec < 1) return IGA_ERR_ARG;
+  nb.dim = dim;
+  nb.vec = vec;
+  for (int d = 0; d < 3; ++d) {
+    nb.ncp[d] = d < dim ? n_cp[d] : 1;
+    nb.lo[d] = d < dim ? lo[d] : nullptr;
+    nb.hi[d] = d < dim ? hi[d] : nullptr;
+  }
+  return IGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (int32), recursive over 1024-element blocks
+// ---------------------------------------------------------------------------
+
+constexpr int kScanBlock = 1024;
+
+__global__ void k_scan_block(int n, const int32_t* __restrict__ in, int32_t* out_plus1,
+                             int32_t* block_sums) {
+  __shared__ int32_t buf[kScanBlock];
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  buf[threadIdx.x] = i < n ? in[i] : 0;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const int v = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+    __syncthreads();
+    buf[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (i < n) out_plus1[i] = buf[threadIdx.x];
+  if (threadIdx.x == kScanBlock - 1 && block_sums) block_sums[blockIdx.x] = buf[threadIdx.x];
+}
+
+__global__ void k_scan_add(int n, int32_t* out_plus1, const int32_t* __restrict__ offs) {
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  if (i < n && blockIdx.x > 0) out_plus1[i] += offs[blockIdx.x - 1];
+}
+
+// inclusive scan of in[0..n) into out[0..n)
+static int scan_inclusive(int n, const int32_t* in, int32_t* out, int32_t* ws, hipStream_t st) {
+  const int nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb <= 1) {
+    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kScanBlock), 0, st, n, in, out, nullptr);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
+  }
+  int32_t* sums = ws;
+  int32_t* sums_scan = ws + nb;
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, st, n, in, out, sums);
+  IGA_CHECK_LAUNCH();
+  int rc = scan_inclusive(nb, sums, sums_scan, ws + 2 * nb, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, st, n, out, sums_scan);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+static int64_t scan_ws(int64_t n) {
+  const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb <= 1) return 0;
+  return 2 * nb + scan_ws(nb);
+}
+
+// ---------------------------------------------------------------------------
+// SpMV and Jacobi
+// ---------------------------------------------------------------------------
+
+__global__ void k_diag_inv(int n, const int32_t* __restrict__ indptr,
+                           const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                           double* inv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int lo = indptr[r], hi = indptr[r + 1];
+  const int k = lower_bound(cols, lo, hi, r);
+  const double d = (k < hi && cols[k] == r) ? vals[k] : 0.0;
+  inv[r] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// LPR lanes per row; returns the row's value in every lane of the group.
+template <int LPR>
+__device__ __forceinline__ double row_dot(int r, const int32_t* __restrict__ indptr,
+                                          const int32_t* __restrict__ cols,
+                                          const double* __restrict__ vals,
+                                          const double* __restrict__ x, int l) {
+  const int lo = indptr[r], hi = indptr[r + 1];
+  double s = 0.0;
+  for (int k = lo + l; k < hi; k += LPR) s += vals[k] * x[cols[k]];
+  return group_sum<LPR>(s);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(256)
+    k_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+           const double* __restrict__ vals, const double* __restrict__ x, double* y) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = gid / LPR, l = gid % LPR;
+  if (r >= n) return;
+  const double s = row_dot<LPR>(r, indptr, cols, vals, x, l);
+  if (l == 0) y[r] = s;
+}
+
+static int pick_lpr(int n, int nnz) {
+  const double avg = n > 0 ? double(nnz) / n : 0.0;
+  if (avg > 96) return 64;
+  if (avg > 40) return 32;
+  if (avg > 12) return 16;
+  return 8;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi-PCG
+// ---------------------------------------------------------------------------
+
+struct CGState {
+  double rz, pq, alpha, beta, rr, bb, tol2, pad;
+  int iter, maxiter, done, pad2;
+  unsigned int cnt[4];
+};
+
+constexpr int kVecThreads = 256;
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_init(int n, const double* __restrict__ b, const double* __restrict__ dinv, double* x,
+              double* r, double* z, double* p, CGState* st, double* partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, bb = 0.0;
+  if (i < n) {
+    const double bi = b[i];
+    const double zi = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    z[i] = zi;
+    p[i] = zi;
+    rz = bi * zi;
+    bb = bi * bi;
+  }
+  const double v[2] = {rz, bb};
+  grid_reduce<2>(v, partials, &st->cnt[0], [&](const double* t) {
+    st->rz = t[0];
+    st->bb = t[1];
+    st->rr = t[1];
+    st->iter = 0;
+    st->done = (!(t[1] > st->tol2 * t[1]) || st->maxiter <= 0) ? 1 : 0;
+  });
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+              const double* __restrict__ vals, const double* __restrict__ p, double* q,
+              CGState* st, double* partials) {
+  if (st->done) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = gid / LPR, l = gid % LPR;
+  double pq = 0.0;
+  if (r < n) {
+    const double s = row_dot<LPR>(r, indptr, cols, vals, p, l);
+    if (l == 0) {
+      q[r] = s;
+      pq = s * p[r];
+    }
+  }
+  const double v[1] = {pq};
+  grid_reduce<1>(v, partials, &st->cnt[1], [&](const double* t) {
+    st->pq = t[0];
+    st->alpha = st->rz / t[0];
+  });
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_update(int n, const double* __restrict__ dinv, const double* __restrict__ p,
+                const double* __restrict__ q, double* x, double* r, double* z, CGState* st,
+                double* partials) {
+  if (st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double alpha = st->alpha;
+  double rz = 0.0, rr = 0.0;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    const double zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    rz = ri * zi;
+    rr = ri * ri;
+  }
+  const double v[2] = {rz, rr};
+  grid_reduce<2>(v, partials, &st->cnt[2], [&](const double* t) {
+    st->beta = t[0] / st->rz;
+    st->rz = t[0];
+    st->rr = t[1];
+    st->iter += 1;
+    st->done = (!(t[1] > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
+  });
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_dir(int n, const double* __restrict__ z, double* p, const CGState* st) {
+  if (st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + st->beta * p[i];
+}
+
+__global__ void k_cg_setup(CGState* st, double tol2, int maxiter) {
+  if (threadIdx.x == 0) {
+    st->tol2 = tol2;
+    st->maxiter = maxiter;
+    st->done = 0;
+    st->iter = 0;
+    for (int k = 0; k < 4; ++k) st->cnt[k] = 0u;
+  }
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(kVecThreads)
+    k_residual(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+               const double* __restrict__ vals, const double* __restrict__ u,
+               const double* __restrict__ b, double* out, unsigned int* cnt, double* partials) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = gid / LPR, l = gid % LPR;
+  double rr = 0.0, bb = 0.0;
+  if (r < n) {
+    const double s = row_dot<LPR>(r, indptr, cols, vals, u, l);
+    if (l == 0) {
+      const double d = s - b[r];
+      rr = d * d;
+      bb = b[r] * b[r];
+    }
+  }
+  const double v[2] = {rr, bb};
+  grid_reduce<2>(v, partials, cnt, [&](const double* t) {
+    out[0] = t[0];
+    out[1] = t[1];
+  });
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// random Fourier features
+// ---------------------------------------------------------------------------
+
+constexpr int kRffTile = 256;
+
+__global__ void __launch_bounds__(256)
+    k_rff(int n, int dim, const double* __restrict__ pts, int m, const double* __restrict__ omega,
+          const double* __restrict__ phase, double amp, double* out) {
+  __shared__ double om[kRffTile * 3];
+  __shared__ double ph[kRffTile];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xv[3] = {0.0, 0.0, 0.0};
+  if (i < n)
+    for (int d = 0; d < dim; ++d) xv[d] = pts[(size_t)i * dim + d];
+  double s = 0.0;
+  for (int m0 = 0; m0 < m; m0 += kRffTile) {
+    const int cnt = min(kRffTile, m - m0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      ph[t] = phase[m0 + t];
+      for (int d = 0; d < dim; ++d) om[t * 3 + d] = omega[(size_t)(m0 + t) * dim + d];
+    }
+    __syncthreads();
+    for (int t = 0; t < cnt; ++t) {
+      double arg = ph[t];
+      for (int d = 0; d < dim; ++d) arg += om[t * 3 + d] * xv[d];
+      s += cos(arg);
+    }
+  }
+  if (i < n) out[i] = amp * s;
+}
+
+}  // namespace iga
+
+using namespace iga;
+
+extern "C" int iga_csr_count(int32_t dim, const int32_t* n_cp, const int32_t* const* nb_lo,
+                             const int32_t* const* nb_hi, int32_t vec, int32_t n_free,
+                             const int32_t* freed, const int32_t* full_to_free, int32_t* counts,
+                             void* stream) {
+  NbInfo nb;
+  if (int rc = make_nb(dim, n_cp, nb_lo, nb_hi, vec, nb)) return rc;
+  if (n_free == 0) return IGA_OK;
+  hipLaunchKernelGGL(k_csr_rows<false>, dim3((n_free + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, nb, n_free, freed, full_to_free, counts, nullptr,
+                     nullptr);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int iga_csr_fill(int32_t dim, const int32_t* n_cp, const int32_t* const* nb_lo,
+                            const int32_t* const* nb_hi, int32_t vec, int32_t n_free,
+                            const int32_t* freed, const int32_t* full_to_free,
+                            const int32_t* indptr, int32_t* cols, int32_t* rows, void* stream) {
+  NbInfo nb;
+  if (int rc = make_nb(dim, n_cp, nb_lo, nb_hi, vec, nb)) return rc;
+  if (n_free == 0) return IGA_OK;
+  hipLaunchKernelGGL(k_csr_rows<true>, dim3((n_free + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, nb, n_free, freed, full_to_free,
+                     const_cast<int32_t*>(indptr), cols, rows);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int64_t iga_scan_workspace(int32_t n) { return scan_ws(n) + 1; }
+
+extern "C" int iga_scan_exclusive(int32_t n, const int32_t* in, int32_t* out, int32_t* ws,
+                                  void* stream) {
+  if (!in || !out) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned int*>(out), 1);
+  IGA_CHECK_LAUNCH();
+  if (n == 0) return IGA_OK;
+  return scan_inclusive(n, in, out + 1, ws, st);
+}
+
+extern "C" int iga_csr_diag_inv(int32_t n, const int32_t* indptr, const int32_t* cols,
+                                const double* vals, double* inv, void* stream) {
+  if (n == 0) return IGA_OK;
+  hipLaunchKernelGGL(k_diag_inv, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                     indptr, cols, vals, inv);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+template <typename F>
+static int by_lpr(int lpr, F f) {
+  switch (lpr) {
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    default: return f(std::integral_constant<int, 64>{});
+  }
+}
+
+static int nnz_of(int n, const int32_t* indptr, hipStream_t st, int* nnz) {
+  int32_t v = 0;
+  if (hipMemcpyAsync(&v, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return IGA_ERR_HIP;
+  if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+  *nnz = v;
+  return IGA_OK;
+}
+
+extern "C" int iga_spmv(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,
+                        const double* x, double* y, void* stream) {
+  if (n == 0) return IGA_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int nnz = 0;
+  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  return by_lpr(pick_lpr(n, nnz), [&](auto L) {
+    constexpr int LPR = decltype(L)::value;
+    const long long threads = (long long)n * LPR;
+    hipLaunchKernelGGL(k_spmv<LPR>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, n,
+                       indptr, cols, vals, x, y);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+// workspace layout (doubles): state (8 doubles + 8 ints = 12 doubles, padded to
+// 16), r, z, p, q (4n), partials (2 * max grid)
+static int64_t cg_grid_max(int64_t n) { return (n * 64 + kVecThreads - 1) / kVecThreads + 1; }
+
+extern "C" int64_t iga_cg_workspace(int32_t n) { return 16 + 4 * (int64_t)n + 2 * cg_grid_max(n); }
+
+extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
+                            const double* vals, const double* dinv, const double* b, double* x,
+                            double tol, int32_t maxiter, double* work, int32_t* out_iters,
+                            double* out_rr, double* out_bb, void* stream) {
+  if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  CGState* S = reinterpret_cast<CGState*>(work);
+  double* r = work + 16;
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  double* partials = q + n;
+  if (n == 0) {
+    if (out_iters) *out_iters = 0;
+    if (out_rr) *out_rr = 0.0;
+    if (out_bb) *out_bb = 0.0;
+    return IGA_OK;
+  }
+  int nnz = 0;
+  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  const int lpr = pick_lpr(n, nnz);
+  const int vblocks = (n + kVecThreads - 1) / kVecThreads;
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, st, S, tol * tol, maxiter);
+  hipLaunchKernelGGL(k_cg_init, dim3(vblocks), dim3(kVecThreads), 0, st, n, b, dinv, x, r, z, p,
+                     S, partials);
+  IGA_CHECK_LAUNCH();
+  CGState host{};
+  const int batch = 16;
+  int done_iters = 0;
+  for (;;) {
+    if (hipMemcpyAsync(&host, S, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess)
+      return IGA_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+    if (host.done) break;
+    for (int k = 0; k < batch; ++k) {
+      int rc = by_lpr(lpr, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        const long long threads = (long long)n * LPR;
+        hipLaunchKernelGGL(k_cg_spmv<LPR>, dim3((unsigned)((threads + kVecThreads - 1) / kVecThreads)),
+                           dim3(kVecThreads), 0, st, n, indptr, cols, vals, p, q, S, partials);
+        IGA_CHECK_LAUNCH();
+        return (int)IGA_OK;
+      });
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_cg_update, dim3(vblocks), dim3(kVecThreads), 0, st, n, dinv, p, q, x,
+                         r, z, S, partials);
+      hipLaunchKernelGGL(k_cg_dir, dim3(vblocks), dim3(kVecThreads), 0, st, n, z, p, S);
+      IGA_CHECK_LAUNCH();
+    }
+    done_iters += batch;
+  }
+  if (out_iters) *out_iters = host.iter;
+  if (out_rr) *out_rr = host.rr;
+  if (out_bb) *out_bb = host.bb;
+  return IGA_OK;
+}
+
+extern "C" int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols,
+                                  const double* vals, const double* u, const double* b,
+                                  double* out, double* work, void* stream) {
+  if (!indptr || !cols || !vals || !u || !b || !out || !work) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) {
+    const double z[2] = {0.0, 0.0};
+    return hipMemcpyAsync(out, z, sizeof(z), hipMemcpyHostToDevice, st) == hipSuccess
+               ? IGA_OK
+               : IGA_ERR_HIP;
+  }
+  int nnz = 0;
+  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  unsigned int* cnt = reinterpret_cast<unsigned int*>(work);
+  double* partials = work + 2;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, cnt, 2);
+  return by_lpr(pick_lpr(n, nnz), [&](auto L) {
+    constexpr int LPR = decltype(L)::value;
+    const long long threads = (long long)n * LPR;
+    hipLaunchKernelGGL(k_residual<LPR>, dim3((unsigned)((threads + kVecThreads - 1) / kVecThreads)),
+                       dim3(kVecThreads), 0, st, n, indptr, cols, vals, u, b, out, cnt, partials);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+extern "C" int iga_rff_eval(int32_t n, int32_t dim, const double* pts, int32_t m,
+                            const double* omega, const double* phase, double amp, double* out,
+                            void* stream) {
+  if (!pts || !omega || !phase || !out || dim < 1 || dim > 3) return IGA_ERR_ARG;
+  if (n == 0) return IGA_OK;
+  hipLaunchKernelGGL(k_rff, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, dim, pts,
+                     m, omega, phase, amp, out);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}

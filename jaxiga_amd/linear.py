@@ -1,0 +1,249 @@
+"""Linear-solver options, the device CSR pattern, and GPU Krylov solves.
+
+Mirrors ``solvers/linear.py`` of the reference: ``CSRPattern`` :38-68,
+``LinearOptions`` :76-127 (same fields, defaults and validation),
+``_iterative_csr`` :324-351 (Jacobi-preconditioned CG from zero, stop at
+``||r|| <= tol ||b||``), ``sparse_solve`` :392-396, ``matrix_free_solve``
+:414-442; and ``solvers/diagnostics.py`` :8-34.
+
+The CSR pattern of the free-free block is generated on the GPU from the
+tensor structure (row ``r`` lists the free dofs of the tensor neighbourhood of
+its scalar basis function, in increasing order), which is the same sorted,
+unique pattern the reference builds with ``np.unique`` over element keys.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import warnings
+
+import numpy as np
+
+from jaxiga_amd import _lib
+from jaxiga_amd.runtime import device, lib, stream
+
+SOLVE_COUNTER = {"forward": 0, "adjoint": 0}
+
+
+def reset_solve_counter():
+    SOLVE_COUNTER["forward"] = 0
+    SOLVE_COUNTER["adjoint"] = 0
+
+
+@dataclasses.dataclass(frozen=True)
+class LinearOptions:
+    """Linear-solver configuration (reference field names and checks).
+
+    On this backend ``"cg"`` runs the device-resident Jacobi-PCG;
+    ``"dense"`` and ``"scipy"`` (a direct solve) run a dense LU on the GPU for
+    systems up to ``DIRECT_LIMIT`` free dofs.
+    """
+
+    method: str = "scipy"
+    matrix_free: bool = False
+    preconditioner: str = "jacobi"
+    tol: float = 1e-10
+    maxiter: int | None = None
+    superlu_ordering: str = "auto"
+
+    def __post_init__(self):
+        if not np.isfinite(self.tol) or self.tol <= 0:
+            raise ValueError("tol must be finite and positive")
+        if self.maxiter is not None and (
+            isinstance(self.maxiter, (bool, np.bool_))
+            or not isinstance(self.maxiter, (int, np.integer)) or self.maxiter <= 0
+        ):
+            raise ValueError("maxiter must be a positive integer or None")
+        allowed = {"cg", "bicgstab", "gmres", "dense", "scipy"}
+        if self.method not in allowed:
+            raise ValueError(f"unknown linear method {self.method!r}; expected one of {allowed}")
+        if self.preconditioner not in {"none", "jacobi"}:
+            raise ValueError(f"unknown preconditioner {self.preconditioner!r}; expected 'none' or 'jacobi'")
+        orderings = {"auto", "NATURAL", "MMD_ATA", "MMD_AT_PLUS_A", "COLAMD"}
+        if self.superlu_ordering not in orderings:
+            raise ValueError(f"unknown SuperLU ordering {self.superlu_ordering!r}; expected one of {orderings}")
+        if self.matrix_free and self.method in {"dense", "scipy"}:
+            raise ValueError(f"matrix_free=True needs an iterative method, not {self.method!r}; "
+                             "there is no matrix for a direct solver to factor")
+
+
+DIRECT_LIMIT = 20000
+
+
+class CSRPattern:
+    """Static CSR structure of the free-free block, resident on the GPU."""
+
+    def __init__(self, indptr, cols, rows, n):
+        self.indptr_d = indptr
+        self.cols_d = cols
+        self.rows_d = rows
+        self.shape = (n, n)
+        self.symmetric = True
+        self._host = None
+
+    @property
+    def nnz(self) -> int:
+        return int(self.cols_d.shape[0])
+
+    def _h(self):
+        if self._host is None:
+            self._host = (self.indptr_d.cpu().numpy(), self.cols_d.cpu().numpy(),
+                          self.rows_d.cpu().numpy())
+        return self._host
+
+    @property
+    def indptr(self):
+        return self._h()[0]
+
+    @property
+    def column_indices(self):
+        return self._h()[1]
+
+    @property
+    def row_indices(self):
+        return self._h()[2]
+
+    @property
+    def coo_indices(self):
+        return np.stack([self.row_indices, self.column_indices], axis=1)
+
+
+def build_pattern(space, free_dev, f2f_dev, n_free) -> CSRPattern:
+    """Generate the CSR pattern on the GPU (count, scan, fill)."""
+    import torch
+
+    ds = space.device()
+    L = lib()
+    st = stream()
+    dev = ds.dev
+    counts = torch.empty(max(n_free, 1), dtype=torch.int32, device=dev)
+    ncp = (_lib.c_int * 3)(*space.patch.n_cp_per_dir, *([1] * (3 - space.dim)))
+    lo = (_lib.c_void_p * 3)(*[t.data_ptr() for t in ds.nb_lo], *([None] * (3 - space.dim)))
+    hi = (_lib.c_void_p * 3)(*[t.data_ptr() for t in ds.nb_hi], *([None] * (3 - space.dim)))
+    _lib.check(L.iga_csr_count(space.dim, ncp, lo, hi, space.vec, n_free, _lib.ptr(free_dev),
+                               _lib.ptr(f2f_dev), _lib.ptr(counts), st), "iga_csr_count")
+    indptr = torch.empty(n_free + 1, dtype=torch.int32, device=dev)
+    ws = torch.empty(int(L.iga_scan_workspace(n_free)), dtype=torch.int32, device=dev)
+    _lib.check(L.iga_scan_exclusive(n_free, _lib.ptr(counts), _lib.ptr(indptr), _lib.ptr(ws), st),
+               "iga_scan_exclusive")
+    nnz = int(indptr[-1].item())
+    cols = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)[:nnz]
+    rows = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)[:nnz]
+    _lib.check(L.iga_csr_fill(space.dim, ncp, lo, hi, space.vec, n_free, _lib.ptr(free_dev),
+                              _lib.ptr(f2f_dev), _lib.ptr(indptr), _lib.ptr(cols), _lib.ptr(rows), st),
+               "iga_csr_fill")
+    return CSRPattern(indptr, cols, rows, n_free)
+
+
+# ---------------------------------------------------------------------------
+# operators and solves
+# ---------------------------------------------------------------------------
+
+
+def spmv(pattern: CSRPattern, data, x):
+    import torch
+
+    y = torch.empty_like(x)
+    n = pattern.shape[0]
+    _lib.check(lib().iga_spmv(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d), _lib.ptr(data),
+                              _lib.ptr(x), _lib.ptr(y), stream()), "iga_spmv")
+    return y
+
+
+def inverse_diagonal(pattern: CSRPattern, data):
+    import torch
+
+    n = pattern.shape[0]
+    inv = torch.empty(n, dtype=torch.float64, device=data.device)
+    _lib.check(lib().iga_csr_diag_inv(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d),
+                                      _lib.ptr(data), _lib.ptr(inv), stream()), "iga_csr_diag_inv")
+    return inv
+
+
+@dataclasses.dataclass
+class CGResult:
+    x: object
+    iterations: int
+    rr: float
+    bb: float
+
+
+def cg_csr(pattern: CSRPattern, data, b, tol=1e-10, maxiter=None, dinv=None, workspace=None):
+    """Jacobi-PCG on the device; returns :class:`CGResult`."""
+    import torch
+
+    L = lib()
+    n = pattern.shape[0]
+    if dinv is None:
+        dinv = inverse_diagonal(pattern, data)
+    if workspace is None:
+        workspace = torch.empty(int(L.iga_cg_workspace(n)), dtype=torch.float64, device=b.device)
+    x = torch.empty(n, dtype=torch.float64, device=b.device)
+    it = _lib.c_int(0)
+    rr = _lib.c_double(0.0)
+    bb = _lib.c_double(0.0)
+    maxiter = 10 * n if maxiter is None else int(maxiter)
+    _lib.check(L.iga_cg_solve(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d), _lib.ptr(data),
+                              _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), float(tol), maxiter,
+                              _lib.ptr(workspace), _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
+                              _lib.ctypes.byref(bb), stream()), "iga_cg_solve")
+    return CGResult(x, it.value, rr.value, bb.value)
+
+
+def residual_norms(pattern: CSRPattern, data, u, b):
+    """``(||A u - b||, ||b||)`` in FP64 on the device."""
+    import torch
+
+    L = lib()
+    n = pattern.shape[0]
+    out = torch.zeros(2, dtype=torch.float64, device=b.device)
+    work = torch.empty(int(L.iga_cg_workspace(n)), dtype=torch.float64, device=b.device)
+    _lib.check(L.iga_residual_norms(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d),
+                                    _lib.ptr(data), _lib.ptr(u), _lib.ptr(b), _lib.ptr(out),
+                                    _lib.ptr(work), stream()), "iga_residual_norms")
+    o = out.cpu().numpy()
+    return float(np.sqrt(o[0])), float(np.sqrt(o[1]))
+
+
+def dense_from_csr(pattern: CSRPattern, data):
+    import torch
+
+    n = pattern.shape[0]
+    A = torch.zeros((n, n), dtype=data.dtype, device=data.device)
+    A[pattern.rows_d.long(), pattern.cols_d.long()] = data
+    return A
+
+
+def sparse_solve(opts: LinearOptions, pattern: CSRPattern, data, b):
+    """Solve ``A u = b`` on the GPU; returns ``(u, info)``."""
+    import torch
+
+    SOLVE_COUNTER["forward"] += 1
+    if opts.method == "cg":
+        if opts.preconditioner == "jacobi":
+            res = cg_csr(pattern, data, b, opts.tol, opts.maxiter)
+        else:
+            ones = torch.ones(pattern.shape[0], dtype=torch.float64, device=b.device)
+            res = cg_csr(pattern, data, b, opts.tol, opts.maxiter, dinv=ones)
+        return res.x, {"iterations": res.iterations}
+    if opts.method in ("dense", "scipy"):
+        if pattern.shape[0] > DIRECT_LIMIT:
+            raise ValueError(f"direct solves are limited to {DIRECT_LIMIT} free dofs on this "
+                             "backend; use LinearOptions(method='cg')")
+        return torch.linalg.solve(dense_from_csr(pattern, data), b), {"iterations": None}
+    raise NotImplementedError(f"linear method {opts.method!r} is not provided on this backend; use 'cg'")
+
+
+def linear_diagnostics(residual_norm, b_norm, u, tol):
+    import torch
+
+    rel = residual_norm / (b_norm if b_norm > 0 else 1.0)
+    finite = bool(np.isfinite(residual_norm) and bool(torch.isfinite(u).all()))
+    return {"residual": residual_norm, "relative_residual": rel,
+            "converged": finite and residual_norm <= tol * b_norm, "finite": finite}
+
+
+def warn_if_failed(name, converged, residual):
+    if not converged:
+        warnings.warn(f"{name} did not converge (residual {residual:.3e}); "
+                      "do not use this result for implicit sensitivities", RuntimeWarning, stacklevel=3)

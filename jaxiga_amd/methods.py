@@ -1,0 +1,311 @@
+"""The Galerkin driver: context, fused GPU assembly, right-hand side, solve.
+
+Mirrors ``methods/_common.py`` (``build_context`` :163-178, the CSR pattern
+cache :181-264) and ``methods/galerkin.py`` (``assemble_csr_values`` :145-148,
+``tangent_diagonal`` :221-237, ``rhs`` :240-245, ``CG_CROSSOVER`` and
+``default_linear`` :248-272, ``solve_galerkin`` :275-357) plus the
+``solve`` dispatcher of ``methods/__init__.py`` :11-64.
+
+The element stiffness, the source work and the Dirichlet lifting ``-K_fp g``
+come out of one fused kernel (``iga_assemble``) that scatters straight into
+the CSR values and the free right-hand side; Neumann work is a boundary
+basis evaluation plus a point-load scatter; the linear solve is the device
+Jacobi-PCG. ``stats`` carries the reference's keys.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as np
+
+from jaxiga_amd import _lib
+from jaxiga_amd import linear as LIN
+from jaxiga_amd.evaluation import evaluate
+from jaxiga_amd.forms import PointField, Problem
+from jaxiga_amd.pointset import boundary_gauss, gauss, n_points, tensor_rule
+from jaxiga_amd.runtime import device, lib, stream
+from jaxiga_amd.space import make_dofmap
+
+CG_CROSSOVER = 3000
+
+
+@dataclasses.dataclass
+class MethodContext:
+    problem: object
+    dofmap: object
+    n_q1: int
+    free_d: object
+    f2f_d: object
+    gfull_d: object
+    rule_pts: object
+    rule_wts: object
+    _pattern: object = None
+    _xq: object = None
+
+    @property
+    def space(self):
+        return self.problem.space
+
+    @property
+    def n_free(self):
+        return len(self.dofmap.free)
+
+    @property
+    def assembly(self) -> LIN.CSRPattern:
+        if self._pattern is None:
+            self._pattern = csr_pattern(self.space, self.dofmap, self.free_d, self.f2f_d)
+        return self._pattern
+
+    def rule(self):
+        r = _lib.IgaRule()
+        r.n_q1 = self.n_q1
+        r.pts = self.rule_pts.data_ptr()
+        r.wts = self.rule_wts.data_ptr()
+        return r
+
+    def gauss_points(self):
+        """Physical interior Gauss points ``(n_e, n_q, dim)`` (cached)."""
+        if self._xq is None:
+            self._xq = evaluate(self.space, gauss(self.space, self.n_q1), outputs=("x",)).x
+        return self._xq
+
+
+def csr_pattern(space, dofmap, free_d, f2f_d):
+    key = (space.vec, dofmap.free.tobytes())
+    pat = space._patterns.get(key)
+    if pat is None:
+        pat = LIN.build_pattern(space, free_d, f2f_d, len(dofmap.free))
+        space._patterns[key] = pat
+    return pat
+
+
+def build_context(problem, params=None, quadrature=None, chunk=None) -> MethodContext:
+    import torch
+
+    space = problem.space
+    dev = device()
+    n_q = quadrature if quadrature is not None else problem.quadrature
+    per_dir = n_points(space, n_q)
+    if len(set(per_dir)) != 1:
+        raise NotImplementedError("the GPU path uses the same number of Gauss points per direction")
+    prescribed, values = problem.resolve_dirichlet(params)
+    dofmap = make_dofmap(space.n_dofs, prescribed, values)
+    f2f = np.full(space.n_dofs, -1, dtype=np.int32)
+    f2f[dofmap.free] = np.arange(len(dofmap.free), dtype=np.int32)
+    gfull = np.zeros(space.n_dofs)
+    gfull[dofmap.prescribed] = dofmap.values
+    pts, wts = np.polynomial.legendre.leggauss(per_dir[0])
+    f64 = dict(dtype=torch.float64, device=dev)
+    return MethodContext(
+        problem=problem, dofmap=dofmap, n_q1=per_dir[0],
+        free_d=torch.as_tensor(dofmap.free.astype(np.int32), device=dev),
+        f2f_d=torch.as_tensor(f2f, device=dev), gfull_d=torch.as_tensor(gfull, **f64),
+        rule_pts=torch.as_tensor(pts, **f64), rule_wts=torch.as_tensor(wts, **f64))
+
+
+# ---------------------------------------------------------------------------
+# point callables
+# ---------------------------------------------------------------------------
+
+
+def call_points(fn, x, *extra, params=None, vec=1, with_params=True):
+    """Evaluate a vectorised callable at points ``x (..., dim)`` (GPU tensor):
+    ``fn(x_T, *extra_T, params)`` with ``(dim, n)`` arguments. Returns a
+    ``(..., vec)`` float64 tensor on the same device."""
+    import torch
+
+    lead = x.shape[:-1]
+    flat = x.reshape(-1, x.shape[-1]).T
+    args = [e.reshape(-1, e.shape[-1]).T for e in extra]
+    out = fn(flat, *args, params) if with_params else fn(flat, *args)
+    out = torch.as_tensor(out, dtype=torch.float64, device=x.device)
+    n = flat.shape[1]
+    if out.dim() == 0:
+        out = out.expand(n)[None, :]
+    elif out.dim() == 1:
+        out = out[None, :] if out.shape[0] == n else out[:, None].expand(-1, n)
+    out = out.T.reshape(*lead, out.shape[0])
+    if out.shape[-1] == 1 and vec > 1:
+        out = out.expand(*lead, vec)
+    return out.contiguous()
+
+
+def _coefficient_args(ctx, params, args):
+    import torch
+
+    value = None if params is None else params.get("a0") if isinstance(params, dict) else None
+    if value is None:
+        args.coef_kind, args.coef_const = _lib.IGA_COEF_CONST, 1.0
+        return None
+    if isinstance(value, PointField):
+        vals = torch.as_tensor(value.values, dtype=torch.float64, device=ctx.f2f_d.device).contiguous()
+        space = ctx.space
+        if tuple(vals.shape) != (space.n_elems, ctx.n_q1 ** space.dim):
+            raise ValueError(f"PointField a0 has shape {tuple(vals.shape)}, expected "
+                             f"{(space.n_elems, ctx.n_q1 ** space.dim)}")
+        args.coef_kind, args.coef_points = _lib.IGA_COEF_POINTS, vals.data_ptr()
+        return vals
+    if callable(value):
+        vals = call_points(value, ctx.gauss_points(), params=params)[..., 0].contiguous()
+        args.coef_kind, args.coef_points = _lib.IGA_COEF_POINTS, vals.data_ptr()
+        return vals
+    args.coef_kind, args.coef_const = _lib.IGA_COEF_CONST, float(value)
+    return None
+
+
+def _source_args(ctx, params, args):
+    src = ctx.problem.source
+    vec = ctx.space.vec
+    if src is None:
+        args.src_kind = _lib.IGA_SRC_NONE
+        return None
+    if not callable(src):
+        v = np.broadcast_to(np.asarray(src, float), (vec,))
+        args.src_kind = _lib.IGA_SRC_CONST
+        for c in range(vec):
+            args.src_const[c] = float(v[c])
+        return None
+    vals = call_points(src, ctx.gauss_points(), params=params, vec=vec).contiguous()
+    args.src_kind, args.src_points = _lib.IGA_SRC_POINTS, vals.data_ptr()
+    return vals
+
+
+def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagonal=False,
+             precision=_lib.IGA_FP64):
+    """One fused pass: CSR values, free load (source work + lifting) and/or the
+    diagonal. Returns ``(vals, rhs, diag)`` (tensors or None)."""
+    import torch
+
+    space = ctx.space
+    problem = ctx.problem
+    dev = ctx.f2f_d.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    args = _lib.IgaAsmArgs()
+    args.vec = space.vec
+    args.precision = precision
+    keep = []
+    if problem.kernel == "poisson":
+        args.problem = _lib.IGA_POISSON
+        keep.append(_coefficient_args(ctx, params, args))
+    elif problem.kernel == "elasticity":
+        args.problem = _lib.IGA_ELASTICITY
+        C = np.asarray(problem.C(params), float).ravel()
+        for i, c in enumerate(C):
+            args.cmat[i] = c
+    else:
+        raise NotImplementedError(
+            f"{type(problem).__name__} has no GPU element kernel; the Galerkin hot path covers "
+            "Poisson and LinearElasticity")
+    pattern = ctx.assembly if values else None
+    vals = torch.zeros(pattern.nnz, **f64) if values else None
+    rhs = torch.zeros(ctx.n_free, **f64) if load else None
+    diag = torch.zeros(ctx.n_free, **f64) if diagonal else None
+    if load:
+        keep.append(_source_args(ctx, params, args))
+    args.full_to_free = ctx.f2f_d.data_ptr()
+    args.gfull = ctx.gfull_d.data_ptr() if load else None
+    if values:
+        args.indptr = pattern.indptr_d.data_ptr()
+        args.cols = pattern.cols_d.data_ptr() if pattern.nnz else None
+        args.vals = vals.data_ptr() if pattern.nnz else None
+    args.rhs = rhs.data_ptr() if load and ctx.n_free else None
+    args.diag = diag.data_ptr() if diagonal and ctx.n_free else None
+    rule = ctx.rule()
+    _lib.check(lib().iga_assemble(space.device().struct, rule, args, stream()), "iga_assemble")
+    if load:
+        neumann_load(ctx, params, rhs)
+    del keep
+    return vals, rhs, diag
+
+
+def neumann_load(ctx, params, rhs):
+    import torch
+
+    space = ctx.space
+    for bc, bset in ctx.problem.resolve_neumann():
+        ps = boundary_gauss(space, bset, ctx.n_q1)
+        if ps.n_elems == 0:
+            continue
+        basis = evaluate(space, ps, outputs=("R", "x", "w", "normal"))
+        t = call_points(bc.value, basis.x, basis.normal, params=params, vec=space.vec)
+        edofs = torch.as_tensor(basis.dofs.astype(np.int32), device=rhs.device)
+        _lib.check(lib().iga_load_points(ps.n_elems, ps.n_q, space.n_local, space.vec,
+                                         _lib.ptr(basis.R), _lib.ptr(basis.w), _lib.ptr(t),
+                                         _lib.ptr(edofs), _lib.ptr(ctx.f2f_d), _lib.ptr(rhs), stream()),
+                   "iga_load_points")
+
+
+def assemble_csr_values(ctx, params=None):
+    return assemble(ctx, params, values=True, load=False)[0]
+
+
+def assemble_triplets(ctx, params=None):
+    return ctx.assembly.coo_indices, assemble_csr_values(ctx, params)
+
+
+def rhs(problem, params=None, ctx=None):
+    if ctx is None:
+        ctx = build_context(problem, params)
+    return assemble(ctx, params, values=False, load=True)[1]
+
+
+def tangent_diagonal(ctx, params=None):
+    return assemble(ctx, params, values=False, load=False, diagonal=True)[2]
+
+
+def default_linear(problem, n_free=None) -> LIN.LinearOptions:
+    """CG with Jacobi at ``tol=1e-12`` for declared SPD linear problems (the
+    reference's accelerator choice); a direct solve otherwise."""
+    if problem.is_linear and problem.is_positive_definite:
+        return LIN.LinearOptions(method="cg", preconditioner="jacobi", tol=1e-12)
+    return LIN.LinearOptions(method="dense")
+
+
+def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadrature=None,
+                   chunk=None, **kwargs):
+    import torch
+
+    if not isinstance(problem, Problem) or problem.kernel is None:
+        raise TypeError("Galerkin requires a problem with an element kernel "
+                        "(Poisson or LinearElasticity)")
+    if not problem.is_linear:
+        raise NotImplementedError("nonlinear Galerkin (Newton) is outside the supported hot path")
+    ctx = build_context(problem, params, quadrature, chunk)
+    if linear is None:
+        linear = default_linear(problem, ctx.n_free)
+    if linear.matrix_free:
+        from jaxiga_amd.matfree import matrix_free_solve_ctx
+
+        return matrix_free_solve_ctx(ctx, params, linear)
+    t0 = time.perf_counter()
+    vals, b, _ = assemble(ctx, params)
+    u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
+    rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
+    torch.cuda.synchronize()
+    stats = {"method": "galerkin", "linear": linear.method, "matrix_free": False,
+             "n_free": ctx.n_free, "nnz": ctx.assembly.nnz, **info,
+             "wall_time": time.perf_counter() - t0}
+    stats.update(LIN.linear_diagnostics(rn, bn, u_free, linear.tol))
+    LIN.warn_if_failed("linear Galerkin solve", stats["converged"], stats["residual"])
+    return ctx.dofmap.lift(u_free), ctx, stats
+
+
+_IMPLEMENTED = {"galerkin"}
+
+
+def solve(problem, *, method: str = "galerkin", params=None, linear=None, initial_guess=None,
+          **method_opts):
+    """Solve a problem and return a :class:`~jaxiga_amd.post.Solution`."""
+    from jaxiga_amd.post import Solution
+
+    if method in {"energy", "collocation"}:
+        raise NotImplementedError(f"method={method!r} is outside the supported hot path; "
+                                  "use method='galerkin'")
+    if method not in _IMPLEMENTED:
+        raise ValueError(f"unknown method {method!r}; expected one of "
+                         f"{sorted(_IMPLEMENTED | {'energy', 'collocation'})}")
+    u, ctx, stats = solve_galerkin(problem, params, linear=linear, initial_guess=initial_guess,
+                                   **method_opts)
+    return Solution(space=problem.space, u=u, params=params, problem=problem, stats=stats)

@@ -1,0 +1,206 @@
+"""GPU parity of the Galerkin hot path against the FP64 CPU oracle.
+
+Bars: integer structure (free/prescribed sets, CSR indptr/columns) bit-exact;
+FP64 basis, matrix entries and load within 1e-12 relative to their scale
+(summation order differs: the GPU scatters with atomics, like the reference's
+XLA scatter-add); CG iterates within 1e-9 of the oracle's solution.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import jaxiga_amd as jx
+from jaxiga_amd import methods as M
+from oracle import assembly as OA
+from oracle import forms as OF
+from oracle import pipeline as OP
+from oracle import space as OS
+from oracle import solver as OSV
+from tests import cases as C
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+
+
+def rel(a, b):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+BASIS_CASES = [("square", 3, 2), ("square", 2, 3), ("annulus", 3, 2), ("annulus", 4, 1),
+               ("cube", 2, 2), ("cube", 1, 2), ("square", 1, 2)]
+
+
+@pytest.mark.parametrize("geom,deg,ref", BASIS_CASES)
+def test_basis_eval_matches_oracle(gpu, geom, deg, ref):
+    print(f"[basis] {geom} p={deg} r={ref}", flush=True)
+    P, O = C.patches(geom, deg, ref)
+    V, VO = jx.FunctionSpace(P), OS.function_space(O)
+    b = jx.evaluate(V, jx.gauss(V))
+    ob = OS.gauss_basis(VO)
+    assert rel(b.R.cpu(), ob.R) < TOL_FP64
+    assert rel(b.dR.cpu(), ob.dR) < 1e-11
+    assert rel(b.x.cpu(), ob.x) < TOL_FP64
+    assert rel(b.w.cpu(), ob.w) < TOL_FP64
+    np.testing.assert_array_equal(b.dofs, ob.dofs)
+
+
+@pytest.mark.parametrize("geom,deg,ref,label", [("square", 3, 2, "left"), ("annulus", 3, 2, "outer"),
+                                                  ("annulus", 2, 2, "sym_x"), ("cube", 2, 1, "front")])
+def test_boundary_basis_matches_oracle(gpu, geom, deg, ref, label):
+    P, O = C.patches(geom, deg, ref)
+    V, VO = jx.FunctionSpace(P), OS.function_space(O)
+    b = jx.evaluate(V, jx.boundary_gauss(V, label))
+    ob = OS.boundary_basis(VO, VO.boundary(label))
+    assert rel(b.R.cpu(), ob.R) < TOL_FP64
+    assert rel(b.x.cpu(), ob.x) < TOL_FP64
+    assert rel(b.w.cpu(), ob.w) < TOL_FP64
+    assert rel(b.normal.cpu(), ob.normal) < TOL_FP64
+
+
+def _oracle_system(case):
+    VO = case.ospace
+    presc, vals = OF.resolve_dirichlet(VO, case.odirichlet, case.params)
+    dm = OS.make_dofmap(VO.n_dofs, presc, vals)
+    basis = OS.gauss_basis(VO)
+    Ke = OP.element_tangents(case.kind, VO, basis, case.params, case.plane)
+    pat = OA.csr_pattern(VO.elem_dofs, VO.n_dofs, VO.vec, dm.free)
+    data = OA.assemble_csr(pat, Ke)
+    gd = OF.gdofs(VO)
+    load = np.zeros(VO.n_dofs)
+    if case.source is not None:
+        load += OA.global_load(VO.n_dofs, gd, OF.source_load(basis, case.source, case.params, VO.vec))
+    for bc in case.oneumann:
+        bs = OF.resolve_neumann(VO, bc.where)
+        bb = OS.boundary_basis(VO, bs)
+        load += OA.global_load(VO.n_dofs, OF.gdofs(VO, bs.elems), OF.neumann_load(bb, bc.value, case.params, VO.vec))
+    load -= OA.lifting(VO.n_dofs, gd, Ke, dm)
+    return dm, pat, data, load[dm.free]
+
+
+SYSTEM_CASES = [
+    lambda: C.poisson_case("square", 3, 3),
+    lambda: C.poisson_case("square", 2, 3, a0=2.5),
+    lambda: C.poisson_case("annulus", 3, 2),
+    lambda: C.poisson_case("cube", 2, 2, source=lambda x, p: 1.0 + 0 * x[0]),
+    lambda: C.poisson_case("square", 3, 2, g=lambda x, p: 1.0 + 2.0 * x[0] - 3.0 * x[1]),
+    lambda: C.elasticity_case(3, 2, "strain"),
+    lambda: C.elasticity_case(2, 2, "stress"),
+]
+
+
+@pytest.mark.parametrize("make", SYSTEM_CASES)
+def test_assembled_system_matches_oracle(gpu, make):
+    case = make()
+    print(f"[system] {case.name}", flush=True)
+    dm, pat, data, b = _oracle_system(case)
+    ctx = M.build_context(case.problem, case.params)
+    np.testing.assert_array_equal(ctx.dofmap.free, dm.free)
+    np.testing.assert_array_equal(ctx.dofmap.prescribed, dm.prescribed)
+    np.testing.assert_allclose(ctx.dofmap.values, dm.values, rtol=0, atol=1e-14)
+    gp = ctx.assembly
+    np.testing.assert_array_equal(gp.indptr, pat.indptr)
+    np.testing.assert_array_equal(gp.column_indices, pat.column_indices)
+    np.testing.assert_array_equal(gp.row_indices, pat.row_indices)
+    vals, rhs, diag = M.assemble(ctx, case.params, diagonal=True)
+    assert rel(vals.cpu(), data) < TOL_FP64
+    assert rel(rhs.cpu(), b) < TOL_FP64
+    K = OA.to_scipy(pat, data)
+    assert rel(diag.cpu(), K.diagonal()) < TOL_FP64
+
+
+@pytest.mark.parametrize("make", SYSTEM_CASES[:3] + SYSTEM_CASES[5:6])
+def test_cg_matches_oracle(gpu, make):
+    case = make()
+    print(f"[cg] {case.name}", flush=True)
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    u_ref, it_ref = OSV.jacobi_pcg(lambda v: K @ v, b, K.diagonal(), tol=1e-10)
+    ctx = M.build_context(case.problem, case.params)
+    vals, rhs, _ = M.assemble(ctx, case.params)
+    res = jx.linear.cg_csr(ctx.assembly, vals, rhs, tol=1e-10)
+    assert abs(res.iterations - it_ref) <= 1, (res.iterations, it_ref)
+    u = res.x.cpu().numpy()
+    assert rel(u, u_ref) < 1e-8
+    # the independent FP64 residual agrees with the recurrence
+    rn, bn = jx.linear.residual_norms(ctx.assembly, vals, res.x, rhs)
+    assert rn <= 1.01e-10 * bn
+
+
+def test_truncated_cg_reports_non_convergence(gpu):
+    case = C.poisson_case("square", 3, 3)
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        sol = jx.solve(case.problem, params=case.params,
+                       linear=jx.LinearOptions(method="cg", maxiter=1))
+    assert not sol.stats["converged"]
+    assert sol.stats["iterations"] == 1
+
+
+def test_cfg1_solve_and_l2_error_match_oracle(gpu):
+    """32x32 bicubic unit square (1225 dofs): the oracle's L2 error to 6 digits."""
+    case = C.poisson_case("square", 3, 5)
+    sol = jx.solve(case.problem, params=case.params, linear=jx.LinearOptions(method="cg", tol=1e-10))
+    assert sol.stats["n_free"] == 1089 and sol.stats["nnz"] == 47961
+    assert sol.stats["converged"] and sol.stats["relative_residual"] <= 1e-10
+    ref = OP.solve("poisson", case.ospace, case.odirichlet, source=case.source, params=case.params,
+                   tol=1e-10)
+    e_gpu = jx.errornorm(sol, C.sin_exact, "L2")
+    e_ref = OSV.errornorm(case.ospace, ref["u"], C.sin_exact, "L2")
+    assert abs(e_gpu - e_ref) <= 1e-6 * e_ref, (e_gpu, e_ref)
+    assert sol.stats["iterations"] == ref["stats"]["iterations"]
+
+
+def test_patch_test_linear_field_exact(gpu):
+    g = lambda x, p: 1.0 + 2.0 * x[0] - 3.0 * x[1]  # noqa: E731
+    for deg in (2, 3):
+        case = C.poisson_case("square", deg, 2, source=lambda x, p: 0.0 * x[0], g=g)
+        sol = jx.solve(case.problem, params=case.params,
+                       linear=jx.LinearOptions(method="cg", tol=1e-14, maxiter=2000))
+        assert jx.errornorm(sol, lambda x: 1.0 + 2.0 * x[0] - 3.0 * x[1], "L2") < 1e-12
+
+
+def test_poisson_convergence_rate(gpu):
+    for deg in (2, 3):
+        errs = []
+        for r in (2, 3, 4):
+            case = C.poisson_case("square", deg, r)
+            sol = jx.solve(case.problem, params=case.params,
+                           linear=jx.LinearOptions(method="cg", tol=1e-13))
+            errs.append(jx.errornorm(sol, C.sin_exact, "L2"))
+        assert errs[0] > errs[1] > errs[2]
+        assert np.log2(errs[-2] / errs[-1]) > deg + 0.5
+
+
+def test_lame_annulus(gpu):
+    """Quarter annulus under internal pressure vs the Lame solution
+    (reference tests/test_examples.py:60-66 thresholds)."""
+    a2, b2, pr, E, nu = 1.0, 16.0, 10.0, 1e5, 0.3
+
+    def disp(x):
+        r = torch.hypot(x[0], x[1])
+        th = torch.atan2(x[1], x[0])
+        ur = pr * a2 * r / (E * (b2 - a2)) * ((1 - nu) + (1 + nu) * b2 / r**2)
+        return torch.stack([ur * torch.cos(th), ur * torch.sin(th)])
+
+    def stress(x):
+        r = torch.hypot(x[0], x[1])
+        th = torch.atan2(x[1], x[0])
+        srr = pr * a2 / (b2 - a2) * (1 - b2 / r**2)
+        stt = pr * a2 / (b2 - a2) * (1 + b2 / r**2)
+        c, s = torch.cos(th), torch.sin(th)
+        return torch.stack([srr * c**2 + stt * s**2, srr * s**2 + stt * c**2, (srr - stt) * s * c])
+
+    V = jx.FunctionSpace(jx.primitives.quarter_annulus(1.0, 4.0).elevate(4).refine(3), vec=2)
+    prob = jx.LinearElasticity(
+        V, plane="stress",
+        dirichlet=[jx.DirichletBC(0.0, where="sym_y", component=1),
+                   jx.DirichletBC(0.0, where="sym_x", component=0)],
+        neumann=[jx.Neumann(lambda x, n, p: -pr * n, where="inner")])
+    sol = jx.solve(prob, params={"E": E, "nu": nu}, linear=jx.LinearOptions(method="cg", tol=1e-13))
+    assert sol.stats["converged"]
+    assert jx.errornorm(sol, disp, "L2") < 5e-5
+    assert jx.errornorm(sol, disp, "energy", exact_grad=stress) < 5e-4
