@@ -21,8 +21,17 @@
  *   iga_spmv                  solvers/linear.py:331-332    (segment-sum matvec)
  *   iga_cg_solve              solvers/linear.py:324-351    (jax cg, Jacobi M)
  *   iga_residual_norms        solvers/diagnostics.py:8-19  (independent ||Au-b||)
- *   iga_mf_apply / iga_mf_diag methods/galerkin.py:207-237 (tangent_matvec /
- *                             tangent_diagonal, sum-factorised)
+ *   iga_mf_setup/iga_mf_apply methods/galerkin.py:207-218 (tangent_matvec,
+ *                             sum-factorised instead of a JVP)
+ *   iga_cg_solve_mf           solvers/linear.py:414-442    (matrix_free_solve, cg)
+ *   iga_diff_norms            solvers/diagnostics.py:8-19  (matrix-free diagnostics)
+ *   iga_inv_guard             solvers/linear.py:426-430    (Jacobi inverse, zero guard)
+ *   iga_residual_vec/iga_axpy solvers/diagnostics.py:8-19 (FP64 outer residual of the
+ *                             mixed-precision refinement; no reference counterpart)
+ *   iga_net_* / iga_nmse / iga_adam / iga_fcg_learned
+ *                             (no reference counterpart: the learned preconditioner;
+ *                             the reference allows only "none"/"jacobi",
+ *                             solvers/linear.py:113-116)
  *   iga_rff_eval              (no reference counterpart: seeded GRF inputs)
  */
 #ifndef JAXIGA_HIP_H
@@ -58,6 +67,8 @@ typedef struct iga_patch {
   const double* hi[3];
   const double* cpts;
   const double* wgts;
+  const int32_t* elo[3]; /* (n_cp[d]) first span supporting each 1-D basis function */
+  const int32_t* ehi[3]; /* (n_cp[d]) last span supporting each 1-D basis function */
 } iga_patch_t;
 
 /* Gauss rule: the same n_q1 points per direction, tensor order first fastest. */
@@ -72,17 +83,23 @@ enum iga_coef { IGA_COEF_CONST = 0, IGA_COEF_POINTS = 1 };
 enum iga_src { IGA_SRC_NONE = 0, IGA_SRC_CONST = 1, IGA_SRC_POINTS = 2 };
 enum iga_prec { IGA_FP64 = 0, IGA_FP32 = 1, IGA_BF16 = 2 };
 
-/* Element stiffness + load, fused with the CSR scatter-add.
+/* Element stiffness + load, then the CSR values of the free-free block.
  * vec        : components per scalar basis (1 Poisson, dim elasticity)
+ * precision  : IGA_FP64 or IGA_FP32 (element contraction on the matrix cores
+ *              in that type; vals and the element-matrix workspace in that
+ *              type; basis, Jacobian and weights always FP64)
  * coef       : Poisson a0 as constant (coef_const) or per Gauss point (n_e, n_q)
  * cmat       : elasticity constitutive matrix, host, (n_s, n_s) row-major
  * src        : none / constant per component (src_const[vec], host) /
  *              per Gauss point (n_e, n_q, vec) device
- * full_to_free: (n_dofs) free index or -1
+ * full_to_free: (n_dofs) free index or -1;  freed: (n_free) free dof list
  * gfull      : (n_dofs) prescribed values (0 on free dofs), or null
- * indptr/cols: CSR pattern of the free-free block; vals (nnz) += K_e
+ * rows/cols  : (nnz) COO view of the CSR pattern; vals (nnz) is overwritten
+ * ke_work    : element-matrix buffer of iga_assemble_workspace() bytes
  * rhs        : (n_free) += source work - K_fp g ; diag (n_free) += diag(K_e)
- * Any of vals/rhs/diag may be null.                                       */
+ * passes     : bit 1 element pass, bit 2 gather pass (0 = both); the
+ *              gather reads the element matrices of an earlier element pass
+ * vals, rhs and diag may each be null.                                    */
 typedef struct iga_asm_args {
   int32_t problem;
   int32_t vec;
@@ -95,14 +112,21 @@ typedef struct iga_asm_args {
   double src_const[3];
   const double* src_points;
   const int32_t* full_to_free;
+  const int32_t* freed;
   const double* gfull;
-  const int32_t* indptr;
+  int64_t nnz;
+  const int32_t* rows;
   const int32_t* cols;
-  double* vals;
+  void* vals;
+  void* ke_work;
+  int64_t ke_work_bytes;
   double* rhs;
   double* diag;
+  int32_t passes;
 } iga_asm_args_t;
 
+/* bytes of element-matrix workspace iga_assemble needs for vals, or -1 */
+int64_t iga_assemble_workspace(const iga_patch_t* patch, int32_t problem, int32_t precision);
 int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule, const iga_asm_args_t* args,
                  void* stream);
 
@@ -145,16 +169,107 @@ int iga_spmv(int32_t n, const int32_t* indptr, const int32_t* cols, const double
              const double* x, double* y, void* stream);
 
 /* Jacobi-PCG from x0 = 0 with the accelerator semantics of jax cg:
- * stop when ||r||_2 <= tol ||b||_2 or after maxiter iterations.
- * work: iga_cg_workspace(n) doubles. out: iterations, final ||r||^2, ||b||^2. */
+ * stop when ||r||_2 <= tol ||b||_2 or after maxiter iterations. The CSR
+ * values are FP64 or FP32 (`precision`); vectors, dot products and all
+ * recurrences are FP64. work: iga_cg_workspace(n) doubles.
+ * out: iterations, final recursive ||r||^2, ||b||^2.                          */
 int64_t iga_cg_workspace(int32_t n);
-int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,
-                 const double* dinv, const double* b, double* x, double tol, int32_t maxiter,
-                 double* work, int32_t* out_iters, double* out_rr, double* out_bb, void* stream);
+int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
+                 int32_t precision, const double* dinv, const double* b, double* x, double tol,
+                 int32_t maxiter, double* work, int32_t* out_iters, double* out_rr, double* out_bb,
+                 void* stream);
+
+/* r = b - A u (FP64) and ||r||^2, ||b||^2 into out[0], out[1]; work 2048 doubles */
+int iga_residual_vec(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,
+                     const double* u, const double* b, double* r, double* out, double* work,
+                     void* stream);
+/* y += a x */
+int iga_axpy(int32_t n, double a, const double* x, double* y, void* stream);
 
 /* ||A u - b||_2^2 and ||b||_2^2 in FP64 into out[0], out[1] (device). */
 int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,
                        const double* u, const double* b, double* out, double* work, void* stream);
+
+/* Matrix-free sum-factorised Poisson operator (scalar, vec = 1).
+ * iga_mf_workspace : bytes of the operator workspace (geometric factors, 1-D
+ *                    tables, element-local results) at the given precision
+ * iga_mf_setup     : fill the workspace once (FP64 geometry, stored in
+ *                    `precision`); coefficient constant or per Gauss point
+ * iga_mf_apply     : y = K x over the free dofs, x and y FP64 (n_free),
+ *                    internal arithmetic in `precision`; rational != 0 when
+ *                    the patch weights are not all one                      */
+int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t rational);
+int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
+                 int32_t rational, int32_t coef_kind, double coef_const,
+                 const double* coef_points, void* work, int64_t work_bytes, void* stream);
+int iga_mf_apply(const iga_patch_t* patch, int32_t precision, int32_t rational, const void* work,
+                 int64_t work_bytes, int32_t n_free, const int32_t* freed,
+                 const int32_t* full_to_free, const void* x, void* y, void* stream);
+/* Jacobi-PCG (semantics of iga_cg_solve) on the matrix-free operator. */
+int iga_cg_solve_mf(const iga_patch_t* patch, int32_t precision, int32_t rational,
+                    const void* mf_work, int64_t mf_work_bytes, int32_t n_free,
+                    const int32_t* freed, const int32_t* full_to_free, const double* dinv,
+                    const double* b, double* x, double tol, int32_t maxiter, double* work,
+                    int32_t* out_iters, double* out_rr, double* out_bb, void* stream);
+
+/* ||y - b||^2 and ||b||^2 in FP64 into out[0], out[1] (device); work: 2048 doubles */
+int iga_diff_norms(int32_t n, const double* y, const double* b, double* out, double* work,
+                   void* stream);
+/* inv[i] = 1/d[i], or 1 where d[i] == 0 (Jacobi zero guard) */
+int iga_inv_guard(int32_t n, const double* d, double* inv, void* stream);
+
+/* Learned preconditioner network: a U-Net of 3x3 convolutions on the
+ * control-point grid, float32 NCHW, no bias, no activation.
+ * kind[l]   : 0 forward convolution (weights (cout, cin, 3, 3)),
+ *             1 transposed convolution + skip add (weights (cin, cout, 3, 3))
+ * src[l]    : index of the input layer (-1 = network input, 1 channel)
+ * skip[l]   : layer added to a transposed convolution's output (or -1)
+ * woff[l]   : offset (floats) of the layer's weights in the parameter buffer
+ * H/W[0]    : input size; H/W[l+1]: output size of layer l                 */
+typedef struct iga_net {
+  int32_t n_layers;
+  int32_t kind[16], cin[16], cout[16], stride[16], src[16], skip[16];
+  int64_t woff[16];
+  int32_t H[17], W[17];
+} iga_net_t;
+
+/* floats of activation buffer for `batch` samples (input at offset 0, the
+ * output is the last layer's slot), of weight-gradient scratch */
+int64_t iga_net_act_floats(const iga_net_t* net, int32_t batch);
+int64_t iga_net_act_offset(const iga_net_t* net, int32_t batch, int32_t layer);
+int64_t iga_net_scratch_floats(const iga_net_t* net, int32_t batch);
+/* forward: reads the input slot of `act`, writes every layer's slot */
+int iga_net_forward(const iga_net_t* net, int32_t batch, const float* params, float* act,
+                    void* stream);
+/* backward: `gact` (same layout as act) holds d loss/d output in the output
+ * slot on entry; on exit `grads` (parameter layout) and the input slot hold
+ * d loss / d weights and d loss / d input */
+int iga_net_backward(const iga_net_t* net, int32_t batch, const float* params,
+                     const float* act, float* gact, float* grads, float* scratch, void* stream);
+/* per-sample normalised MSE mean_n ||p_n - t_n||^2 / ||t_n||^2, its gradient,
+ * sums (2*batch doubles) and loss (1 double) on the device */
+int iga_nmse(int32_t batch, int64_t per_sample, const float* pred, const float* target,
+             float* grad, double* sums, double* loss, void* stream);
+/* one Adam step (bias-corrected, step >= 1) over n parameters */
+int iga_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2,
+             float eps, int32_t step, void* stream);
+
+/* Flexible CG with the learned two-level preconditioner
+ *   x1 = w D^-1 r, r1 = r - Kpc x1, x2 = x1 + D^-1/2 U(D^-1/2 r1), r2 = r - Kpc x2,
+ *   z = x2 + w D^-1 r2
+ * (U: the network, batch 1, activations in `act`; Kpc: FP32 CSR values;
+ * D: the diagonal, dinv = 1/D, dsq = D^-1/2 on free dofs; cell[i]: grid cell
+ * of free dof i). FCG with one stored direction on the FP64 operator `vals`;
+ * stops at ||r|| <= tol ||b|| or maxiter. graph_iters > 0 captures that many
+ * iterations (rounded up to even) into a hipGraph that is replayed.
+ * work: iga_fcg_workspace(n) doubles.                                       */
+int64_t iga_fcg_workspace(int32_t n);
+int iga_fcg_learned(const iga_net_t* net, const float* params, float* act, int32_t n,
+                    const int32_t* indptr, const int32_t* cols, const double* vals,
+                    const float* vals_pc, const double* dinv, const double* dsq,
+                    const int32_t* cell, double omega, const double* b, double* x, double tol,
+                    int32_t maxiter, int32_t graph_iters, double* work, int32_t* out_iters,
+                    double* out_rr, double* out_bb, void* stream);
 
 /* Seeded random-Fourier-feature field g(x) = amp * sum_m cos(omega_m . x + phase_m)
  * at n points (n, dim) (device, FP64 throughout). */

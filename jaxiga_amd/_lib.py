@@ -33,7 +33,7 @@ class IgaPatch(ctypes.Structure):
     _fields_ = [
         ("dim", c_int), ("degree", c_int), ("n_elem", c_int * 3), ("n_cp", c_int * 3),
         ("ext", c_void_p * 3), ("first", c_void_p * 3), ("lo", c_void_p * 3), ("hi", c_void_p * 3),
-        ("cpts", c_void_p), ("wgts", c_void_p),
+        ("cpts", c_void_p), ("wgts", c_void_p), ("elo", c_void_p * 3), ("ehi", c_void_p * 3),
     ]
 
 
@@ -46,14 +46,24 @@ class IgaAsmArgs(ctypes.Structure):
         ("problem", c_int), ("vec", c_int), ("precision", c_int), ("coef_kind", c_int),
         ("coef_const", c_double), ("coef_points", c_void_p), ("cmat", c_double * 36),
         ("src_kind", c_int), ("src_const", c_double * 3), ("src_points", c_void_p),
-        ("full_to_free", c_void_p), ("gfull", c_void_p), ("indptr", c_void_p), ("cols", c_void_p),
-        ("vals", c_void_p), ("rhs", c_void_p), ("diag", c_void_p),
+        ("full_to_free", c_void_p), ("freed", c_void_p), ("gfull", c_void_p), ("nnz", c_i64),
+        ("rows", c_void_p), ("cols", c_void_p), ("vals", c_void_p), ("ke_work", c_void_p),
+        ("ke_work_bytes", c_i64), ("rhs", c_void_p), ("diag", c_void_p), ("passes", c_int),
+    ]
+
+
+class IgaNet(ctypes.Structure):
+    _fields_ = [
+        ("n_layers", c_int), ("kind", c_int * 16), ("cin", c_int * 16), ("cout", c_int * 16),
+        ("stride", c_int * 16), ("src", c_int * 16), ("skip", c_int * 16), ("woff", c_i64 * 16),
+        ("H", c_int * 17), ("W", c_int * 17),
     ]
 
 
 _SIGNATURES = {
     "iga_assemble": (c_int, [ctypes.POINTER(IgaPatch), ctypes.POINTER(IgaRule),
                              ctypes.POINTER(IgaAsmArgs), c_void_p]),
+    "iga_assemble_workspace": (c_i64, [ctypes.POINTER(IgaPatch), c_int, c_int]),
     "iga_basis_eval": (c_int, [ctypes.POINTER(IgaPatch), c_int, c_void_p, c_int, c_void_p, c_int,
                                c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p]),
@@ -69,10 +79,37 @@ _SIGNATURES = {
     "iga_csr_diag_inv": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_spmv": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_cg_workspace": (c_i64, [c_int]),
-    "iga_cg_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                             c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_cg_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                             c_void_p, c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_residual_vec": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
+    "iga_axpy": (c_int, [c_int, c_double, c_void_p, c_void_p, c_void_p]),
     "iga_residual_norms": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p]),
+    "iga_mf_workspace": (c_i64, [ctypes.POINTER(IgaPatch), c_int, c_int]),
+    "iga_mf_setup": (c_int, [ctypes.POINTER(IgaPatch), ctypes.POINTER(IgaRule), c_int, c_int, c_int,
+                             c_double, c_void_p, c_void_p, c_i64, c_void_p]),
+    "iga_mf_apply": (c_int, [ctypes.POINTER(IgaPatch), c_int, c_int, c_void_p, c_i64, c_int, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_cg_solve_mf": (c_int, [ctypes.POINTER(IgaPatch), c_int, c_int, c_void_p, c_i64, c_int,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
+                                c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_diff_norms": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_inv_guard": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_net_act_floats": (c_i64, [ctypes.POINTER(IgaNet), c_int]),
+    "iga_net_act_offset": (c_i64, [ctypes.POINTER(IgaNet), c_int, c_int]),
+    "iga_net_scratch_floats": (c_i64, [ctypes.POINTER(IgaNet), c_int]),
+    "iga_net_forward": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_net_backward": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p]),
+    "iga_nmse": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_adam": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, ctypes.c_float, c_int, c_void_p]),
+    "iga_fcg_workspace": (c_i64, [c_int]),
+    "iga_fcg_learned": (c_int, [ctypes.POINTER(IgaNet), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                c_void_p, c_double, c_int, c_int, c_void_p, P_int, P_dbl, P_dbl,
+                                c_void_p]),
     "iga_rff_eval": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_double,
                              c_void_p, c_void_p]),
 }

@@ -1,20 +1,28 @@
-// Fused Galerkin assembly for a single tensor-product NURBS patch on gfx950.
+// Galerkin assembly for a single tensor-product NURBS patch on gfx950, in two
+// streaming passes with no atomics on the matrix values.
 //
-// One workgroup per element. The element's 1-D Bezier-extracted tables
-// (N1 = C_d B, dN1 = C_d dB * 2/h for each direction) are staged in LDS; the
-// rational basis, the FP64 Jacobian/determinant and the physical gradients
-// are formed with one lane per (Gauss point, local basis function) and
-// group-of-lanes reductions; the element matrix is contracted from the LDS
-// gradient table and scattered straight into the CSR values of the free-free
-// block with hardware FP64 atomics (slot found by a search of the row's
-// sorted column list). The load (source work) and the Dirichlet lifting
-// -K_fp g go to the free right-hand side in the same pass.
+// Pass 1 (k_element) -- one workgroup per element. The element's 1-D
+// Bezier-extracted tables (N1 = C_d B, dN1 = C_d dB * 2/h) are staged in LDS;
+// the rational basis, the FP64 Jacobian/determinant and the physical
+// gradients come from one lane per (Gauss point, local function) with
+// group-of-lanes reductions. The element matrix K_e = Bm^T (W D) Bm is then
+// contracted on the matrix cores (v_mfma_f64_16x16x4 in FP64,
+// v_mfma_f32_16x16x4 in FP32) from LDS strain tables and written to an
+// element-matrix buffer. The load (source work) and the Dirichlet lifting
+// -K_fp g, and optionally the diagonal, are accumulated in the same pass.
+//
+// Pass 2 (k_gather) -- one lane per CSR nonzero of the free-free block. The
+// elements that share the row's and the column's basis functions form a
+// small tensor box; the lane sums their element-matrix entries in a fixed
+// order and stores the value once. This is the reference's scatter-add into
+// unique CSR slots, realised as a deterministic gather: the assembled values
+// are bitwise reproducible run to run.
 #include "common.hpp"
 
 namespace iga {
 
 struct AsmDev {
-  int problem, vec;
+  int problem, vec, precision;
   int coef_kind;
   double coef_const;
   const double* coef_points;
@@ -24,11 +32,30 @@ struct AsmDev {
   const double* src_points;
   const int32_t* f2f;
   const double* gfull;
-  const int32_t* indptr;
-  const int32_t* cols;
-  double* vals;
   double* rhs;
   double* diag;
+};
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  static __device__ __forceinline__ acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+  static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
 };
 
 // Voigt pair (a, b) of strain component s: 2-D (xx, yy, xy), 3-D
@@ -46,41 +73,53 @@ __device__ __forceinline__ void voigt(int s, int& a, int& b) {
   }
 }
 
-// Strain vector of basis function i in component c: B[s] = sum over the
-// Voigt pair of the matching gradient entries.
+// Strain component s of basis function i moving in component c.
 template <int DIM>
-__device__ __forceinline__ void strain_col(const double* g, int c, double* Bs) {
-  constexpr int NS = DIM == 2 ? 3 : 6;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    int a, b;
-    voigt<DIM>(s, a, b);
-    double v = 0.0;
-    if (c == a) v += g[b];
-    if (a != b && c == b) v += g[a];
-    Bs[s] = v;
-  }
+__device__ __forceinline__ double strain(const double* g, int c, int s) {
+  int a, b;
+  voigt<DIM>(s, a, b);
+  double v = 0.0;
+  if (c == a) v += g[b];
+  if (a != b && c == b) v += g[a];
+  return v;
 }
 
-template <int DIM, int P>
+template <int DIM, int P, int PROB>
+struct Shape {
+  static constexpr int N1 = P + 1;
+  static constexpr int NL = ipow(N1, DIM);
+  static constexpr int NLP = pow2ceil(NL);
+  static constexpr int VEC = PROB == IGA_ELASTICITY ? DIM : 1;
+  static constexpr int NS = PROB == IGA_ELASTICITY ? (DIM == 2 ? 3 : 6) : DIM;
+  static constexpr int NLV = NL * VEC;
+  static constexpr int TI = (NLV + 15) / 16;
+};
+
+template <int DIM, int P, int PROB, typename T>
 __global__ void __launch_bounds__(1024)
-    k_assemble(iga_patch_t pt, int nq1, const double* __restrict__ qp,
-               const double* __restrict__ qw, AsmDev a) {
-  constexpr int N1 = P + 1;
-  constexpr int NL = ipow(N1, DIM);
-  constexpr int NLP = pow2ceil(NL);
-  constexpr int NS = DIM == 2 ? 3 : 6;
+    k_element(iga_patch_t pt, int nq1, const double* __restrict__ qp,
+              const double* __restrict__ qw, AsmDev a, T* __restrict__ ke) {
+  using S = Shape<DIM, P, PROB>;
+  constexpr int N1 = S::N1, NL = S::NL, NLP = S::NLP, NS = S::NS, NLV = S::NLV, TI = S::TI;
   const int nq = DIM == 2 ? nq1 * nq1 : nq1 * nq1 * nq1;
+  const int KK = nq * NS;
+  const int KKp = (KK + 3) & ~3;
 
   extern __shared__ double smem[];
-  double* N1t = smem;                         // [DIM][nq1][N1]
-  double* dN1t = N1t + DIM * nq1 * N1;        // [DIM][nq1][N1]
-  double* G = dN1t + DIM * nq1 * N1;          // [nq][DIM][NL]
-  double* Rs = G + nq * DIM * NL;             // [nq][NL]
-  double* Wk = Rs + nq * NL;                  // [nq] weight * coefficient
-  double* Wf = Wk + nq;                       // [nq] weight
-  int* fdof = reinterpret_cast<int*>(Wf + nq);  // [NL*vec] free index or -1
-  int* gdof = fdof + NL * 3;                  // [NL] scalar dofs
+  double* N1t = smem;                   // [DIM][nq1][N1]
+  double* dN1t = N1t + DIM * nq1 * N1;  // [DIM][nq1][N1]
+  // G holds the physical gradients [(q*DIM + d)][NL], zero-padded to KKp rows
+  // for the Poisson contraction, which uses it directly as its strain table.
+  const int Grows = PROB == IGA_POISSON ? KKp : nq * DIM;
+  double* G = dN1t + DIM * nq1 * N1;    // [Grows][NL]
+  double* Rs = G + Grows * NL;          // [nq][NL]
+  double* Wk = Rs + nq * NL;            // [nq] weight * coefficient
+  double* Wf = Wk + nq;                 // [nq] weight
+  double* Ks = Wf + nq;                 // [NLV][NLV]
+  T* Hm = reinterpret_cast<T*>(Ks + NLV * NLV);      // [KKp][NLV]
+  T* Bm = Hm + KKp * NLV;                            // [KKp][NLV], elasticity only
+  int* fdof = reinterpret_cast<int*>(Bm + (PROB == IGA_POISSON ? 0 : KKp * NLV) + 2);  // [NL*3]
+  int* gdof = fdof + NL * 3;                         // [NL] scalar dofs
 
   const int e = blockIdx.x;
   int ei[3] = {0, 0, 0};
@@ -149,7 +188,7 @@ __global__ void __launch_bounds__(1024)
   for (int d = 0; d < DIM; ++d) jac_ref *= 0.5 * (pt.hi[d][ei[d]] - pt.lo[d][ei[d]]);
   __syncthreads();
 
-  // ---- rational basis and push-forward per Gauss point --------------------
+  // ---- rational basis and push-forward per Gauss point (FP64) -------------
   for (int q = slot; q < nq; q += nslots) {
     int qi[3] = {q % nq1, (q / nq1) % nq1, q / (nq1 * nq1)};
     double N = 1.0, dN[DIM];
@@ -182,99 +221,96 @@ __global__ void __launch_bounds__(1024)
   }
   __syncthreads();
 
-  // ---- element matrix, scatter, lifting -----------------------------------
-  const bool want_lift = a.rhs != nullptr && a.gfull != nullptr;
-  for (int idx = threadIdx.x; idx < NL * NL; idx += blockDim.x) {
-    const int i = idx / NL, j = idx % NL;
-    if (a.problem == IGA_POISSON) {
-      double k = 0.0;
-      for (int q = 0; q < nq; ++q) {
-        double gg = 0.0;
+  // ---- strain tables: Bm[(q,s)][col], Hm = (W D) Bm -------------------------
+  if constexpr (PROB == IGA_POISSON)
+    for (int idx = KK * NL + threadIdx.x; idx < KKp * NL; idx += blockDim.x) G[idx] = 0.0;
+  for (int idx = threadIdx.x; idx < KKp * NLV; idx += blockDim.x) {
+    const int k = idx / NLV, col = idx % NLV;
+    double bv = 0.0, hv = 0.0;
+    if (k < KK) {
+      const int q = k / NS, s = k % NS;
+      if constexpr (PROB == IGA_POISSON) {
+        hv = Wk[q] * G[(q * DIM + s) * NL + col];
+      } else {
+        const int i = col / DIM, c = col % DIM;
+        double gi[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) gg += G[(q * DIM + d) * NL + i] * G[(q * DIM + d) * NL + j];
-        k += Wk[q] * gg;
+        for (int d = 0; d < DIM; ++d) gi[d] = G[(q * DIM + d) * NL + i];
+        bv = strain<DIM>(gi, c, s);
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < NS; ++t) acc += a.cmat[s * NS + t] * strain<DIM>(gi, c, t);
+        hv = Wf[q] * acc;
       }
+    }
+    if constexpr (PROB != IGA_POISSON) Bm[idx] = (T)bv;
+    Hm[idx] = (T)hv;
+  }
+  __syncthreads();
+
+  // ---- K_e = Bm^T Hm on the matrix cores, one 16x16 tile per wave -----------
+  {
+    const int wave = threadIdx.x / kWave, wl = threadIdx.x % kWave;
+    const int nwaves = blockDim.x / kWave;
+    const int r16 = wl % 16, k4 = wl / 16;
+    for (int tile = wave; tile < TI * TI; tile += nwaves) {
+      const int i0 = (tile / TI) * 16, j0 = (tile % TI) * 16;
+      const int ii = i0 + r16, jj = j0 + r16;
+      auto acc = Mfma<T>::zero();
+      for (int k0 = 0; k0 < KKp; k0 += 4) {
+        const int k = k0 + k4;
+        T av = (T)0;
+        if (ii < NLV) {
+          if constexpr (PROB == IGA_POISSON)
+            av = (T)G[k * NL + ii];
+          else
+            av = Bm[k * NLV + ii];
+        }
+        const T bv = jj < NLV ? Hm[k * NLV + jj] : (T)0;
+        acc = Mfma<T>::step(av, bv, acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + 4 * k4 + r, j = j0 + r16;
+        if (i < NLV && j < NLV) Ks[i * NLV + j] = (double)acc[r];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- element-matrix buffer, lifting, diagonal ----------------------------
+  const bool lift = a.rhs != nullptr && a.gfull != nullptr;
+  T* kout = ke ? ke + (size_t)e * NLV * NLV : nullptr;
+  for (int idx = threadIdx.x; idx < NLV * NLV; idx += blockDim.x) {
+    const double v = Ks[idx];
+    if (kout) kout[idx] = (T)v;
+    const int i = idx / NLV, j = idx % NLV;
+    if constexpr (PROB == IGA_POISSON) {
       for (int c = 0; c < a.vec; ++c) {
         const int r = fdof[i * a.vec + c];
         if (r < 0) continue;
         const int cc = fdof[j * a.vec + c];
         if (cc >= 0) {
-          if (a.vals) {
-            const int hi = a.indptr[r + 1];
-            const int s = lower_bound(a.cols, a.indptr[r], hi, cc);
-            if (s < hi && a.cols[s] == cc) unsafeAtomicAdd(a.vals + s, k);
-          }
-          if (a.diag && i == j) unsafeAtomicAdd(a.diag + r, k);
-        } else if (want_lift) {
+          if (a.diag && i == j) unsafeAtomicAdd(a.diag + r, v);
+        } else if (lift) {
           const double gv = a.gfull[(size_t)a.vec * gdof[j] + c];
-          if (gv != 0.0) unsafeAtomicAdd(a.rhs + r, -k * gv);
+          if (gv != 0.0) unsafeAtomicAdd(a.rhs + r, -v * gv);
         }
       }
     } else {
-      double kb[DIM][DIM];
-#pragma unroll
-      for (int c = 0; c < DIM; ++c)
-#pragma unroll
-        for (int cp = 0; cp < DIM; ++cp) kb[c][cp] = 0.0;
-      for (int q = 0; q < nq; ++q) {
-        double gi[DIM], gj[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          gi[d] = G[(q * DIM + d) * NL + i];
-          gj[d] = G[(q * DIM + d) * NL + j];
-        }
-        const double w = Wf[q];
-#pragma unroll
-        for (int cp = 0; cp < DIM; ++cp) {
-          double bj[NS], cb[NS];
-          strain_col<DIM>(gj, cp, bj);
-#pragma unroll
-          for (int s = 0; s < NS; ++s) {
-            double acc = 0.0;
-#pragma unroll
-            for (int t = 0; t < NS; ++t) acc += a.cmat[s * NS + t] * bj[t];
-            cb[s] = acc;
-          }
-#pragma unroll
-          for (int c = 0; c < DIM; ++c) {
-            double bi[NS];
-            strain_col<DIM>(gi, c, bi);
-            double acc = 0.0;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) acc += bi[s] * cb[s];
-            kb[c][cp] += w * acc;
-          }
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        const int r = fdof[i * DIM + c];
-        if (r < 0) continue;
-        int row_lo = 0, row_hi = 0;
-        if (a.vals) {
-          row_lo = a.indptr[r];
-          row_hi = a.indptr[r + 1];
-        }
-#pragma unroll
-        for (int cp = 0; cp < DIM; ++cp) {
-          const int cc = fdof[j * DIM + cp];
-          const double k = kb[c][cp];
-          if (cc >= 0) {
-            if (a.vals) {
-              const int s = lower_bound(a.cols, row_lo, row_hi, cc);
-              if (s < row_hi && a.cols[s] == cc) unsafeAtomicAdd(a.vals + s, k);
-            }
-            if (a.diag && i == j && c == cp) unsafeAtomicAdd(a.diag + r, k);
-          } else if (want_lift) {
-            const double gv = a.gfull[(size_t)DIM * gdof[j] + cp];
-            if (gv != 0.0) unsafeAtomicAdd(a.rhs + r, -k * gv);
-          }
-        }
+      const int r = fdof[i];
+      if (r < 0) continue;
+      const int cc = fdof[j];
+      if (cc >= 0) {
+        if (a.diag && i == j) unsafeAtomicAdd(a.diag + r, v);
+      } else if (lift) {
+        const double gv = a.gfull[(size_t)DIM * gdof[j / DIM] + (j % DIM)];
+        if (gv != 0.0) unsafeAtomicAdd(a.rhs + r, -v * gv);
       }
     }
   }
 
-  // ---- source work --------------------------------------------------------
+  // ---- source work ----------------------------------------------------------
   if (a.rhs && a.src_kind != IGA_SRC_NONE) {
     for (int idx = threadIdx.x; idx < NL * a.vec; idx += blockDim.x) {
       const int i = idx / a.vec, c = idx % a.vec;
@@ -292,30 +328,147 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-template <int DIM, int P>
+// Pass 2: vals[k] = sum over the elements shared by row and column of the
+// matching element-matrix entries (fixed order -> bitwise reproducible).
+template <int DIM, int P, int PROB, typename T>
+__global__ void __launch_bounds__(256)
+    k_gather(iga_patch_t pt, int vec, int64_t nnz, const int32_t* __restrict__ rows,
+             const int32_t* __restrict__ cols, const int32_t* __restrict__ freed,
+             const T* __restrict__ ke, T* __restrict__ vals) {
+  using S = Shape<DIM, P, PROB>;
+  constexpr int N1 = S::N1, NL = S::NL, NLV = S::NLV;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int gi = freed[rows[k]], gj = freed[cols[k]];
+    const int si = gi / vec, ci = gi - si * vec;
+    const int sj = gj / vec, cj = gj - sj * vec;
+    if (PROB == IGA_POISSON && ci != cj) {
+      vals[k] = (T)0;
+      continue;
+    }
+    int id[3] = {0, 0, 0}, jd[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    {
+      int ri = si, rj = sj;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        id[d] = ri % pt.n_cp[d];
+        ri /= pt.n_cp[d];
+        jd[d] = rj % pt.n_cp[d];
+        rj /= pt.n_cp[d];
+        lo[d] = max(pt.elo[d][id[d]], pt.elo[d][jd[d]]);
+        hi[d] = min(pt.ehi[d][id[d]], pt.ehi[d][jd[d]]);
+      }
+    }
+    double sum = 0.0;
+    for (int e2 = lo[2]; e2 <= hi[2]; ++e2)
+      for (int e1 = lo[1]; e1 <= hi[1]; ++e1)
+        for (int e0 = lo[0]; e0 <= hi[0]; ++e0) {
+          const int ee[3] = {e0, e1, e2};
+          int li = 0, lj = 0, stride = 1, e = 0, estride = 1;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            const int f = pt.first[d][ee[d]];
+            li += (id[d] - f) * stride;
+            lj += (jd[d] - f) * stride;
+            stride *= N1;
+            e += ee[d] * estride;
+            estride *= pt.n_elem[d];
+          }
+          const size_t base = (size_t)e * NLV * NLV;
+          if constexpr (PROB == IGA_POISSON)
+            sum += (double)ke[base + (size_t)li * NL + lj];
+          else
+            sum += (double)ke[base + (size_t)(li * DIM + ci) * NLV + lj * DIM + cj];
+        }
+    vals[k] = (T)sum;
+  }
+}
+
+template <int DIM, int P, int PROB, typename T>
 static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const AsmDev& a,
-                           hipStream_t st) {
-  constexpr int N1 = P + 1;
-  constexpr int NL = ipow(N1, DIM);
-  constexpr int NLP = pow2ceil(NL);
+                           const iga_asm_args_t& args, hipStream_t st) {
+  using S = Shape<DIM, P, PROB>;
+  constexpr int NL = S::NL, NLP = S::NLP, NLV = S::NLV, N1 = S::N1, NS = S::NS;
   const int nq1 = rule.n_q1;
   const int nq = DIM == 2 ? nq1 * nq1 : nq1 * nq1 * nq1;
+  const int KKp = (nq * NS + 3) & ~3;
   int threads = nq * NLP;
   threads = ((threads + kWave - 1) / kWave) * kWave;
   if (threads > 1024) threads = 1024;
-  const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + nq * DIM * NL + nq * NL + 2 * nq) +
+  if (threads < 256) threads = 256;
+  const int Grows = PROB == IGA_POISSON ? KKp : nq * DIM;
+  const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + Grows * NL + nq * NL + 2 * nq +
+                                       NLV * NLV) +
+                     sizeof(T) * ((PROB == IGA_POISSON ? 1 : 2) * KKp * NLV + 2) +
                      sizeof(int) * (NL * 3 + NL);
   if (lds > 64 * 1024) return IGA_ERR_UNSUPPORTED;
   int n_e = 1;
   for (int d = 0; d < DIM; ++d) n_e *= pt.n_elem[d];
   if (n_e == 0) return IGA_OK;
-  hipLaunchKernelGGL((k_assemble<DIM, P>), dim3(n_e), dim3(threads), lds, st, pt, nq1, rule.pts,
-                     rule.wts, a);
-  IGA_CHECK_LAUNCH();
+  T* ke = reinterpret_cast<T*>(args.ke_work);
+  if (args.vals) {
+    const int64_t need = (int64_t)n_e * NLV * NLV * sizeof(T);
+    if (!ke || args.ke_work_bytes < need) return IGA_ERR_SHAPE;
+    if (!args.rows || !args.cols || !args.freed) return IGA_ERR_ARG;
+  } else {
+    ke = nullptr;
+  }
+  const int passes = args.passes ? args.passes : 3;
+  if (passes & 1) {
+    hipLaunchKernelGGL((k_element<DIM, P, PROB, T>), dim3(n_e), dim3(threads), lds, st, pt, nq1,
+                       rule.pts, rule.wts, a, ke);
+    IGA_CHECK_LAUNCH();
+  }
+  if ((passes & 2) && args.vals && args.nnz > 0) {
+    const int64_t blocks = std::min<int64_t>((args.nnz + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL((k_gather<DIM, P, PROB, T>), dim3((unsigned)blocks), dim3(256), 0, st, pt,
+                       args.vec, args.nnz, args.rows, args.cols, args.freed, ke,
+                       reinterpret_cast<T*>(args.vals));
+    IGA_CHECK_LAUNCH();
+  }
   return IGA_OK;
 }
 
+template <int DIM, int P>
+static int dispatch_prob(const iga_patch_t& pt, const iga_rule_t& rule, const AsmDev& a,
+                         const iga_asm_args_t& args, hipStream_t st) {
+  const bool f32 = args.precision == IGA_FP32;
+  if (a.problem == IGA_POISSON)
+    return f32 ? launch_assemble<DIM, P, IGA_POISSON, float>(pt, rule, a, args, st)
+               : launch_assemble<DIM, P, IGA_POISSON, double>(pt, rule, a, args, st);
+  return f32 ? launch_assemble<DIM, P, IGA_ELASTICITY, float>(pt, rule, a, args, st)
+             : launch_assemble<DIM, P, IGA_ELASTICITY, double>(pt, rule, a, args, st);
+}
+
+template <int DIM, int P, int PROB>
+static int64_t ke_bytes(int64_t n_e, int precision) {
+  constexpr int NLV = Shape<DIM, P, PROB>::NLV;
+  return n_e * NLV * NLV * (precision == IGA_FP32 ? 4 : 8);
+}
+
 }  // namespace iga
+
+extern "C" int64_t iga_assemble_workspace(const iga_patch_t* patch, int32_t problem,
+                                          int32_t precision) {
+  using namespace iga;
+  if (!patch) return -1;
+  int64_t n_e = 1;
+  for (int d = 0; d < patch->dim; ++d) n_e *= patch->n_elem[d];
+  const int key = patch->dim * 10 + patch->degree;
+  const bool el = problem == IGA_ELASTICITY;
+#define IGA_W(D, P) \
+  return el ? ke_bytes<D, P, IGA_ELASTICITY>(n_e, precision) : ke_bytes<D, P, IGA_POISSON>(n_e, precision)
+  switch (key) {
+    case 21: IGA_W(2, 1);
+    case 22: IGA_W(2, 2);
+    case 23: IGA_W(2, 3);
+    case 24: IGA_W(2, 4);
+    case 31: IGA_W(3, 1);
+    case 32: IGA_W(3, 2);
+    default: return -1;
+  }
+#undef IGA_W
+}
 
 extern "C" int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule,
                             const iga_asm_args_t* args, void* stream) {
@@ -324,6 +477,7 @@ extern "C" int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule,
   AsmDev a;
   a.problem = args->problem;
   a.vec = args->vec;
+  a.precision = args->precision;
   a.coef_kind = args->coef_kind;
   a.coef_const = args->coef_const;
   a.coef_points = args->coef_points;
@@ -333,28 +487,25 @@ extern "C" int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule,
   a.src_points = args->src_points;
   a.f2f = args->full_to_free;
   a.gfull = args->gfull;
-  a.indptr = args->indptr;
-  a.cols = args->cols;
-  a.vals = args->vals;
   a.rhs = args->rhs;
   a.diag = args->diag;
   if (!a.f2f) return IGA_ERR_ARG;
-  if (a.vals && (!a.indptr || !a.cols)) return IGA_ERR_ARG;
+  if (a.problem != IGA_POISSON && a.problem != IGA_ELASTICITY) return IGA_ERR_ARG;
   if (a.problem == IGA_ELASTICITY && a.vec != patch->dim) return IGA_ERR_SHAPE;
   if (a.vec < 1 || a.vec > 3) return IGA_ERR_SHAPE;
-  if (args->precision != IGA_FP64) return IGA_ERR_UNSUPPORTED;
+  if (args->precision != IGA_FP64 && args->precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
   if (a.coef_kind == IGA_COEF_POINTS && !a.coef_points) return IGA_ERR_ARG;
   if (a.src_kind == IGA_SRC_POINTS && !a.src_points) return IGA_ERR_ARG;
   if (rule->n_q1 < 1 || rule->n_q1 > 8) return IGA_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
   const int key = patch->dim * 10 + patch->degree;
   switch (key) {
-    case 21: return launch_assemble<2, 1>(*patch, *rule, a, st);
-    case 22: return launch_assemble<2, 2>(*patch, *rule, a, st);
-    case 23: return launch_assemble<2, 3>(*patch, *rule, a, st);
-    case 24: return launch_assemble<2, 4>(*patch, *rule, a, st);
-    case 31: return launch_assemble<3, 1>(*patch, *rule, a, st);
-    case 32: return launch_assemble<3, 2>(*patch, *rule, a, st);
+    case 21: return dispatch_prob<2, 1>(*patch, *rule, a, *args, st);
+    case 22: return dispatch_prob<2, 2>(*patch, *rule, a, *args, st);
+    case 23: return dispatch_prob<2, 3>(*patch, *rule, a, *args, st);
+    case 24: return dispatch_prob<2, 4>(*patch, *rule, a, *args, st);
+    case 31: return dispatch_prob<3, 1>(*patch, *rule, a, *args, st);
+    case 32: return dispatch_prob<3, 2>(*patch, *rule, a, *args, st);
     default: return IGA_ERR_UNSUPPORTED;
   }
 }

@@ -1,11 +1,14 @@
 // Shared device helpers for the gfx950 Galerkin kernels: compile-time shape
 // arithmetic, Bernstein evaluation, wave reductions, the NURBS quotient with
-// the FP64 push-forward, and the deterministic last-block reduction used by
-// the Krylov kernels.
+// the FP64 push-forward, and the Krylov building blocks (CSR row products,
+// deterministic per-block partial reductions, device-resident solver state).
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
+#include <type_traits>
 
 #include "../../include/jaxiga_hip.h"
 
@@ -141,47 +144,6 @@ __device__ __forceinline__ double block_sum(double v, double* scratch32) {
   return s;
 }
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Deterministic grid reduction of NV values: each block stores its partials,
-// the last block to arrive sums all partials in block order and calls fin()
-// from thread 0 with the totals. `counter` must start at 0; it is reset.
-template <int NV, typename Fin>
-__device__ __forceinline__ void grid_reduce(const double (&v)[NV], double* partials,
-                                            unsigned int* counter, Fin fin) {
-  __shared__ double scratch[16];
-  __shared__ bool is_last;
-  double bs[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) bs[k] = block_sum(v[k], scratch);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&partials[k * gridDim.x + blockIdx.x], bs[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double tot[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
-      s += load_coherent(&partials[k * gridDim.x + i]);
-    tot[k] = block_sum(s, scratch);
-  }
-  if (threadIdx.x == 0) {
-    fin(tot);
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
 __device__ __forceinline__ int lower_bound(const int32_t* a, int lo, int hi, int key) {
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -191,6 +153,88 @@ __device__ __forceinline__ int lower_bound(const int32_t* a, int lo, int hi, int
       hi = mid;
   }
   return lo;
+}
+
+
+// ---------------------------------------------------------------------------
+// Krylov building blocks shared by the CG / FCG drivers
+// ---------------------------------------------------------------------------
+
+// LPR lanes per row; returns the row's value in every lane of the group.
+template <int LPR, typename TV = double>
+__device__ __forceinline__ double row_dot(int r, const int32_t* __restrict__ indptr,
+                                          const int32_t* __restrict__ cols,
+                                          const TV* __restrict__ vals,
+                                          const double* __restrict__ x, int l) {
+  const int lo = indptr[r], hi = indptr[r + 1];
+  double s = 0.0;
+  for (int k = lo + l; k < hi; k += LPR) s += (double)vals[k] * x[cols[k]];
+  return group_sum<LPR>(s);
+}
+
+inline int pick_lpr(int n, int nnz) {
+  const double avg = n > 0 ? double(nnz) / n : 0.0;
+  if (avg > 96) return 64;
+  if (avg > 40) return 32;
+  if (avg > 12) return 16;
+  return 8;
+}
+
+
+template <typename F>
+inline int by_lpr(int lpr, F f) {
+  switch (lpr) {
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    default: return f(std::integral_constant<int, 64>{});
+  }
+}
+
+struct CGState {
+  double rr, bb, tol2, pad;
+  int iter, maxiter, done, pad2;
+};
+
+constexpr int kVecThreads = 256;
+constexpr int kRedBlocks = 1024;  // upper bound on blocks of any Krylov kernel
+
+struct CGBufs {
+  CGState* st;
+  double* P_pq;  // [kRedBlocks]
+  double* P_rz;  // [2][kRedBlocks]
+  double* P_rr;  // [kRedBlocks]
+  double* P_bb;  // [kRedBlocks]
+};
+
+// Sum of n partials, identical in every block; valid in all threads.
+__device__ __forceinline__ double reduce_partials(const double* parts, int n) {
+  __shared__ double scratch[16];
+  __shared__ double result;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[i];
+  s = block_sum(s, scratch);
+  if (threadIdx.x == 0) result = s;
+  __syncthreads();
+  const double r = result;
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void store_partial(double* parts, double v) {
+  __shared__ double scratch[16];
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v;
+}
+
+inline int vec_blocks(int64_t n) {
+  const int64_t b = (n + kVecThreads - 1) / kVecThreads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
+}
+
+inline int row_blocks(int64_t n, int lpr) {
+  const int64_t b = (n * lpr + kVecThreads - 1) / kVecThreads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
 }
 
 }  // namespace iga

@@ -1,0 +1,18 @@
+// Internal launcher of the matrix-free operator, shared by the Krylov drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/jaxiga_hip.h"
+
+namespace iga {
+
+// y = K x over the free dofs (x, y FP64); a nonzero *skip makes every launch a
+// no-op (used by the batched Krylov loops once they have converged).
+int mf_apply_launch(const iga_patch_t* patch, int precision, int rational, const void* work,
+                    int64_t work_bytes, int n_free, const int32_t* freed,
+                    const int32_t* full_to_free, const double* x, double* y, const int* skip,
+                    hipStream_t st);
+
+}  // namespace iga

@@ -7,10 +7,12 @@
 // reduced deterministically (fixed block order) by the last block to finish.
 // Kernels read a device `done` flag, so the host enqueues iterations in
 // batches and synchronises only once per batch.
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
 #include "common.hpp"
+#include "matfree.hpp"
 
 namespace iga {
 
@@ -145,18 +147,6 @@ __global__ void k_diag_inv(int n, const int32_t* __restrict__ indptr,
   inv[r] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
-// LPR lanes per row; returns the row's value in every lane of the group.
-template <int LPR>
-__device__ __forceinline__ double row_dot(int r, const int32_t* __restrict__ indptr,
-                                          const int32_t* __restrict__ cols,
-                                          const double* __restrict__ vals,
-                                          const double* __restrict__ x, int l) {
-  const int lo = indptr[r], hi = indptr[r + 1];
-  double s = 0.0;
-  for (int k = lo + l; k < hi; k += LPR) s += vals[k] * x[cols[k]];
-  return group_sum<LPR>(s);
-}
-
 template <int LPR>
 __global__ void __launch_bounds__(256)
     k_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
@@ -168,115 +158,101 @@ __global__ void __launch_bounds__(256)
   if (l == 0) y[r] = s;
 }
 
-static int pick_lpr(int n, int nnz) {
-  const double avg = n > 0 ? double(nnz) / n : 0.0;
-  if (avg > 96) return 64;
-  if (avg > 40) return 32;
-  if (avg > 12) return 16;
-  return 8;
-}
-
 // ---------------------------------------------------------------------------
-// Jacobi-PCG
+// Jacobi-PCG with device-resident scalars
 // ---------------------------------------------------------------------------
-
-struct CGState {
-  double rz, pq, alpha, beta, rr, bb, tol2, pad;
-  int iter, maxiter, done, pad2;
-  unsigned int cnt[4];
-};
-
-constexpr int kVecThreads = 256;
+//
+// Every reduction writes one partial per block; the kernel that consumes the
+// scalar re-reduces the partials in every block, in the same fixed order, so
+// all blocks agree bit for bit and no atomics or grid barriers are needed.
 
 __global__ void __launch_bounds__(kVecThreads)
     k_cg_init(int n, const double* __restrict__ b, const double* __restrict__ dinv, double* x,
-              double* r, double* z, double* p, CGState* st, double* partials) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+              double* r, double* z, double* p, CGBufs B) {
   double rz = 0.0, bb = 0.0;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double bi = b[i];
     const double zi = dinv[i] * bi;
     x[i] = 0.0;
     r[i] = bi;
     z[i] = zi;
     p[i] = zi;
-    rz = bi * zi;
-    bb = bi * bi;
+    rz += bi * zi;
+    bb += bi * bi;
   }
-  const double v[2] = {rz, bb};
-  grid_reduce<2>(v, partials, &st->cnt[0], [&](const double* t) {
-    st->rz = t[0];
-    st->bb = t[1];
-    st->rr = t[1];
-    st->iter = 0;
-    st->done = (!(t[1] > st->tol2 * t[1]) || st->maxiter <= 0) ? 1 : 0;
-  });
+  store_partial(B.P_rz, rz);
+  store_partial(B.P_bb, bb);
 }
 
-template <int LPR>
+__global__ void k_cg_init_fin(int nblk, CGBufs B, double tol2, int maxiter) {
+  const double bb = reduce_partials(B.P_bb, nblk);
+  if (threadIdx.x == 0) {
+    CGState* st = B.st;
+    st->bb = bb;
+    st->rr = bb;
+    st->tol2 = tol2;
+    st->iter = 0;
+    st->maxiter = maxiter;
+    st->done = (!(bb > tol2 * bb) || maxiter <= 0) ? 1 : 0;
+  }
+}
+
+template <int LPR, typename TV>
 __global__ void __launch_bounds__(kVecThreads)
     k_cg_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
-              const double* __restrict__ vals, const double* __restrict__ p, double* q,
-              CGState* st, double* partials) {
-  if (st->done) return;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = gid / LPR, l = gid % LPR;
+              const TV* __restrict__ vals, const double* __restrict__ p, double* q, CGBufs B) {
+  if (B.st->done) return;
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
   double pq = 0.0;
-  if (r < n) {
-    const double s = row_dot<LPR>(r, indptr, cols, vals, p, l);
+  for (int r = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < n; r += groups) {
+    const double s = row_dot<LPR, TV>(r, indptr, cols, vals, p, l);
     if (l == 0) {
       q[r] = s;
-      pq = s * p[r];
+      pq += s * p[r];
     }
   }
-  const double v[1] = {pq};
-  grid_reduce<1>(v, partials, &st->cnt[1], [&](const double* t) {
-    st->pq = t[0];
-    st->alpha = st->rz / t[0];
-  });
+  store_partial(B.P_pq, pq);
 }
 
 __global__ void __launch_bounds__(kVecThreads)
-    k_cg_update(int n, const double* __restrict__ dinv, const double* __restrict__ p,
-                const double* __restrict__ q, double* x, double* r, double* z, CGState* st,
-                double* partials) {
-  if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double alpha = st->alpha;
+    k_cg_update(int n, int nblk_pq, int nblk, int par, const double* __restrict__ dinv,
+                const double* __restrict__ p, const double* __restrict__ q, double* x, double* r,
+                double* z, CGBufs B) {
+  if (B.st->done) return;
+  const double pq = reduce_partials(B.P_pq, nblk_pq);
+  const double rz_old = reduce_partials(B.P_rz + par * kRedBlocks, nblk);
+  const double alpha = rz_old / pq;
   double rz = 0.0, rr = 0.0;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     const double zi = dinv[i] * ri;
     r[i] = ri;
     z[i] = zi;
-    rz = ri * zi;
-    rr = ri * ri;
+    rz += ri * zi;
+    rr += ri * ri;
   }
-  const double v[2] = {rz, rr};
-  grid_reduce<2>(v, partials, &st->cnt[2], [&](const double* t) {
-    st->beta = t[0] / st->rz;
-    st->rz = t[0];
-    st->rr = t[1];
-    st->iter += 1;
-    st->done = (!(t[1] > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
-  });
+  store_partial(B.P_rz + (1 - par) * kRedBlocks, rz);
+  store_partial(B.P_rr, rr);
 }
 
 __global__ void __launch_bounds__(kVecThreads)
-    k_cg_dir(int n, const double* __restrict__ z, double* p, const CGState* st) {
+    k_cg_dir(int n, int nblk, int par, const double* __restrict__ z, double* p, CGBufs B) {
+  CGState* st = B.st;
   if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + st->beta * p[i];
-}
-
-__global__ void k_cg_setup(CGState* st, double tol2, int maxiter) {
-  if (threadIdx.x == 0) {
-    st->tol2 = tol2;
-    st->maxiter = maxiter;
-    st->done = 0;
-    st->iter = 0;
-    for (int k = 0; k < 4; ++k) st->cnt[k] = 0u;
+  const double rz_new = reduce_partials(B.P_rz + (1 - par) * kRedBlocks, nblk);
+  const double rz_old = reduce_partials(B.P_rz + par * kRedBlocks, nblk);
+  const double beta = rz_new / rz_old;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0) {
+    const double rr = reduce_partials(B.P_rr, nblk);
+    if (threadIdx.x == 0) {
+      st->rr = rr;
+      st->iter += 1;
+      st->done = (!(rr > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
+    }
   }
 }
 
@@ -284,23 +260,29 @@ template <int LPR>
 __global__ void __launch_bounds__(kVecThreads)
     k_residual(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
                const double* __restrict__ vals, const double* __restrict__ u,
-               const double* __restrict__ b, double* out, unsigned int* cnt, double* partials) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = gid / LPR, l = gid % LPR;
+               const double* __restrict__ b, double* P_rr, double* P_bb) {
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
   double rr = 0.0, bb = 0.0;
-  if (r < n) {
+  for (int r = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < n; r += groups) {
     const double s = row_dot<LPR>(r, indptr, cols, vals, u, l);
     if (l == 0) {
       const double d = s - b[r];
-      rr = d * d;
-      bb = b[r] * b[r];
+      rr += d * d;
+      bb += b[r] * b[r];
     }
   }
-  const double v[2] = {rr, bb};
-  grid_reduce<2>(v, partials, cnt, [&](const double* t) {
-    out[0] = t[0];
-    out[1] = t[1];
-  });
+  store_partial(P_rr, rr);
+  store_partial(P_bb, bb);
+}
+
+__global__ void k_residual_fin(int nblk, const double* P_rr, const double* P_bb, double* out) {
+  const double rr = reduce_partials(P_rr, nblk);
+  const double bb = reduce_partials(P_bb, nblk);
+  if (threadIdx.x == 0) {
+    out[0] = rr;
+    out[1] = bb;
+  }
 }
 
 __global__ void k_zero_u32(unsigned int* p, int n) {
@@ -394,16 +376,6 @@ extern "C" int iga_csr_diag_inv(int32_t n, const int32_t* indptr, const int32_t*
   return IGA_OK;
 }
 
-template <typename F>
-static int by_lpr(int lpr, F f) {
-  switch (lpr) {
-    case 8: return f(std::integral_constant<int, 8>{});
-    case 16: return f(std::integral_constant<int, 16>{});
-    case 32: return f(std::integral_constant<int, 32>{});
-    default: return f(std::integral_constant<int, 64>{});
-  }
-}
-
 static int nnz_of(int n, const int32_t* indptr, hipStream_t st, int* nnz) {
   int32_t v = 0;
   if (hipMemcpyAsync(&v, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -429,67 +401,125 @@ extern "C" int iga_spmv(int32_t n, const int32_t* indptr, const int32_t* cols, c
   });
 }
 
-// workspace layout (doubles): state (8 doubles + 8 ints = 12 doubles, padded to
-// 16), r, z, p, q (4n), partials (2 * max grid)
-static int64_t cg_grid_max(int64_t n) { return (n * 64 + kVecThreads - 1) / kVecThreads + 1; }
+// workspace layout (doubles): state (8), r, z, p, q (4n), partials (5 * kRedBlocks)
+extern "C" int64_t iga_cg_workspace(int32_t n) { return 8 + 4 * (int64_t)n + 5 * kRedBlocks; }
 
-extern "C" int64_t iga_cg_workspace(int32_t n) { return 16 + 4 * (int64_t)n + 2 * cg_grid_max(n); }
+static CGBufs cg_bufs(double* work, int32_t n) {
+  CGBufs B;
+  B.st = reinterpret_cast<CGState*>(work);
+  double* parts = work + 8 + 4 * (int64_t)n;
+  B.P_pq = parts;
+  B.P_rz = parts + kRedBlocks;
+  B.P_rr = parts + 3 * kRedBlocks;
+  B.P_bb = parts + 4 * kRedBlocks;
+  return B;
+}
 
-extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
-                            const double* vals, const double* dinv, const double* b, double* x,
-                            double tol, int32_t maxiter, double* work, int32_t* out_iters,
-                            double* out_rr, double* out_bb, void* stream) {
-  if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
-  CGState* S = reinterpret_cast<CGState*>(work);
-  double* r = work + 16;
-  double* z = r + n;
-  double* p = z + n;
-  double* q = p + n;
-  double* partials = q + n;
+// Dot-product partials p.q for an operator applied outside the fused SpMV.
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_dot(int n, const double* __restrict__ p, const double* __restrict__ q, CGBufs B) {
+  if (B.st->done) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += p[i] * q[i];
+  store_partial(B.P_pq, s);
+}
+
+// Jacobi-PCG loop; `apply(p, q, B)` must produce q = A p and the p.q
+// partials (returns the number of partial blocks, or a negative status).
+template <typename Apply>
+static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, double tol,
+                   int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
+                   double* out_bb, hipStream_t st, Apply apply) {
   if (n == 0) {
     if (out_iters) *out_iters = 0;
     if (out_rr) *out_rr = 0.0;
     if (out_bb) *out_bb = 0.0;
     return IGA_OK;
   }
-  int nnz = 0;
-  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
-  const int lpr = pick_lpr(n, nnz);
-  const int vblocks = (n + kVecThreads - 1) / kVecThreads;
-  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, st, S, tol * tol, maxiter);
-  hipLaunchKernelGGL(k_cg_init, dim3(vblocks), dim3(kVecThreads), 0, st, n, b, dinv, x, r, z, p,
-                     S, partials);
+  CGBufs B = cg_bufs(work, n);
+  double* r = work + 8;
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  const int vb = vec_blocks(n);
+  hipLaunchKernelGGL(k_cg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, dinv, x, r, z, p, B);
+  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kVecThreads), 0, st, vb, B, tol * tol, maxiter);
   IGA_CHECK_LAUNCH();
   CGState host{};
   const int batch = 16;
-  int done_iters = 0;
+  int launched = 0;
   for (;;) {
-    if (hipMemcpyAsync(&host, S, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess)
+    if (hipMemcpyAsync(&host, B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess)
       return IGA_ERR_HIP;
     if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
     if (host.done) break;
-    for (int k = 0; k < batch; ++k) {
-      int rc = by_lpr(lpr, [&](auto L) {
-        constexpr int LPR = decltype(L)::value;
-        const long long threads = (long long)n * LPR;
-        hipLaunchKernelGGL(k_cg_spmv<LPR>, dim3((unsigned)((threads + kVecThreads - 1) / kVecThreads)),
-                           dim3(kVecThreads), 0, st, n, indptr, cols, vals, p, q, S, partials);
-        IGA_CHECK_LAUNCH();
-        return (int)IGA_OK;
-      });
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_cg_update, dim3(vblocks), dim3(kVecThreads), 0, st, n, dinv, p, q, x,
-                         r, z, S, partials);
-      hipLaunchKernelGGL(k_cg_dir, dim3(vblocks), dim3(kVecThreads), 0, st, n, z, p, S);
+    for (int k = 0; k < batch; ++k, ++launched) {
+      const int par = launched & 1;
+      const int pb = apply(p, q, B);
+      if (pb < 0) return pb;
+      hipLaunchKernelGGL(k_cg_update, dim3(vb), dim3(kVecThreads), 0, st, n, pb, vb, par, dinv, p,
+                         q, x, r, z, B);
+      hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, par, z, p, B);
       IGA_CHECK_LAUNCH();
     }
-    done_iters += batch;
   }
   if (out_iters) *out_iters = host.iter;
   if (out_rr) *out_rr = host.rr;
   if (out_bb) *out_bb = host.bb;
   return IGA_OK;
+}
+
+extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
+                            const void* vals, int32_t precision, const double* dinv,
+                            const double* b, double* x, double tol, int32_t maxiter, double* work,
+                            int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
+  if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
+  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
+  hipStream_t st = (hipStream_t)stream;
+  int nnz = 0;
+  if (n > 0)
+    if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  const int lpr = pick_lpr(n, nnz);
+  const int rb = row_blocks(n, lpr);
+  return cg_loop(n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
+                 [&](const double* p, double* q, CGBufs B) -> int {
+                   int rc = by_lpr(lpr, [&](auto L) {
+                     constexpr int LPR = decltype(L)::value;
+                     if (precision == IGA_FP32)
+                       hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kVecThreads), 0,
+                                          st, n, indptr, cols, reinterpret_cast<const float*>(vals),
+                                          p, q, B);
+                     else
+                       hipLaunchKernelGGL((k_cg_spmv<LPR, double>), dim3(rb), dim3(kVecThreads), 0,
+                                          st, n, indptr, cols,
+                                          reinterpret_cast<const double*>(vals), p, q, B);
+                     IGA_CHECK_LAUNCH();
+                     return (int)IGA_OK;
+                   });
+                   return rc ? rc : rb;
+                 });
+}
+
+extern "C" int iga_cg_solve_mf(const iga_patch_t* patch, int32_t precision, int32_t rational,
+                               const void* mf_work, int64_t mf_work_bytes, int32_t n_free,
+                               const int32_t* freed, const int32_t* full_to_free,
+                               const double* dinv, const double* b, double* x, double tol,
+                               int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
+                               double* out_bb, void* stream) {
+  if (!patch || !mf_work || !dinv || !b || !x || !work) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int vb = vec_blocks(n_free);
+  return cg_loop(n_free, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
+                 [&](const double* p, double* q, CGBufs B) -> int {
+                   int rc = mf_apply_launch(patch, precision, rational, mf_work, mf_work_bytes,
+                                            n_free, freed, full_to_free, p, q, &B.st->done, st);
+                   if (rc) return rc;
+                   hipLaunchKernelGGL(k_cg_dot, dim3(vb), dim3(kVecThreads), 0, st, n_free, p, q,
+                                      B);
+                   IGA_CHECK_LAUNCH();
+                   return vb;
+                 });
 }
 
 extern "C" int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols,
@@ -498,24 +528,125 @@ extern "C" int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_
   if (!indptr || !cols || !vals || !u || !b || !out || !work) return IGA_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) {
-    const double z[2] = {0.0, 0.0};
-    return hipMemcpyAsync(out, z, sizeof(z), hipMemcpyHostToDevice, st) == hipSuccess
-               ? IGA_OK
-               : IGA_ERR_HIP;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned int*>(out), 4);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
   }
   int nnz = 0;
   if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
-  unsigned int* cnt = reinterpret_cast<unsigned int*>(work);
-  double* partials = work + 2;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, cnt, 2);
-  return by_lpr(pick_lpr(n, nnz), [&](auto L) {
+  double* P_rr = work;
+  double* P_bb = work + kRedBlocks;
+  const int lpr = pick_lpr(n, nnz);
+  const int rb = row_blocks(n, lpr);
+  int rc = by_lpr(lpr, [&](auto L) {
     constexpr int LPR = decltype(L)::value;
-    const long long threads = (long long)n * LPR;
-    hipLaunchKernelGGL(k_residual<LPR>, dim3((unsigned)((threads + kVecThreads - 1) / kVecThreads)),
-                       dim3(kVecThreads), 0, st, n, indptr, cols, vals, u, b, out, cnt, partials);
+    hipLaunchKernelGGL(k_residual<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols, vals,
+                       u, b, P_rr, P_bb);
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
   });
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_residual_fin, dim3(1), dim3(kVecThreads), 0, st, rb, P_rr, P_bb, out);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_diff_norms(int n, const double* __restrict__ y, const double* __restrict__ b, double* P_rr,
+                 double* P_bb) {
+  double rr = 0.0, bb = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double d = y[i] - b[i];
+    rr += d * d;
+    bb += b[i] * b[i];
+  }
+  store_partial(P_rr, rr);
+  store_partial(P_bb, bb);
+}
+
+__global__ void k_inv_guard(int n, const double* __restrict__ d, double* inv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
+}
+
+extern "C" int iga_diff_norms(int32_t n, const double* y, const double* b, double* out,
+                              double* work, void* stream) {
+  if (!y || !b || !out || !work) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int vb = vec_blocks(n);
+  hipLaunchKernelGGL(k_diff_norms, dim3(vb), dim3(kVecThreads), 0, st, n, y, b, work,
+                     work + kRedBlocks);
+  hipLaunchKernelGGL(k_residual_fin, dim3(1), dim3(kVecThreads), 0, st, vb, work,
+                     work + kRedBlocks, out);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int iga_inv_guard(int32_t n, const double* d, double* inv, void* stream) {
+  if (!d || !inv) return IGA_ERR_ARG;
+  if (n == 0) return IGA_OK;
+  hipLaunchKernelGGL(k_inv_guard, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, d,
+                     inv);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(kVecThreads)
+    k_residual_vec(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                   const double* __restrict__ vals, const double* __restrict__ u,
+                   const double* __restrict__ b, double* rvec, double* P_rr, double* P_bb) {
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  double rr = 0.0, bb = 0.0;
+  for (int r = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < n; r += groups) {
+    const double s = row_dot<LPR>(r, indptr, cols, vals, u, l);
+    if (l == 0) {
+      const double d = b[r] - s;
+      rvec[r] = d;
+      rr += d * d;
+      bb += b[r] * b[r];
+    }
+  }
+  store_partial(P_rr, rr);
+  store_partial(P_bb, bb);
+}
+
+__global__ void k_axpy(int n, double a, const double* __restrict__ x, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+extern "C" int iga_residual_vec(int32_t n, const int32_t* indptr, const int32_t* cols,
+                                const double* vals, const double* u, const double* b, double* r,
+                                double* out, double* work, void* stream) {
+  if (!indptr || !cols || !vals || !u || !b || !r || !out || !work) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) return IGA_OK;
+  int nnz = 0;
+  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  const int lpr = pick_lpr(n, nnz);
+  const int rb = row_blocks(n, lpr);
+  int rc = by_lpr(lpr, [&](auto L) {
+    constexpr int LPR = decltype(L)::value;
+    hipLaunchKernelGGL(k_residual_vec<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
+                       vals, u, b, r, work, work + kRedBlocks);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_residual_fin, dim3(1), dim3(kVecThreads), 0, st, rb, work, work + kRedBlocks,
+                     out);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int iga_axpy(int32_t n, double a, const double* x, double* y, void* stream) {
+  if (!x || !y) return IGA_ERR_ARG;
+  if (n == 0) return IGA_OK;
+  hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, a, x, y);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
 }
 
 extern "C" int iga_rff_eval(int32_t n, int32_t dim, const double* pts, int32_t m,

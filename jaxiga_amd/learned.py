@@ -1,0 +1,322 @@
+"""Learned preconditioner: a convolutional U-Net on the control-point grid,
+trained on the GPU with this build's own forward/backward/Adam kernels and
+applied inside flexible CG.
+
+The reference has no learned preconditioner -- ``LinearOptions`` allows only
+``"none"`` and ``"jacobi"`` (``src/jaxiga/solvers/linear.py:113-116``) -- so
+this is an addition; plain Jacobi-CG stays the fallback and the judge of
+convergence (the final FP64 residual is recomputed independently and, if the
+flexible solve has not reached the tolerance, Jacobi-CG finishes the job).
+
+Preconditioner (two-level, the network as the coarse correction):
+    x1 = w D^-1 r;  r1 = r - K x1;  x2 = x1 + D^-1/2 U(D^-1/2 r1);
+    r2 = r - K x2;  z = x2 + w D^-1 r2
+with ``D`` the operator diagonal, ``w = 1/lambda_max(D^-1 K)`` (power
+iteration), ``K`` the FP32 CSR values, and ``U`` a bias-free, activation-free
+4-level U-Net (about 180k FP32 weights): a linear, coefficient-agnostic
+multilevel filter made coefficient-aware by the diagonal scaling.
+
+Training data are residual/error pairs produced by the solver: for each
+training coefficient field, truncated Jacobi-CG solves ``u_k`` (several
+iteration counts) against the converged ``u*`` give ``e_k = u* - u_k`` and
+``r_k = b - K u_k``; the network learns ``D^-1/2 r1 -> D^1/2 e1`` (the smoothed
+pair) under a per-sample normalised squared error.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from jaxiga_amd import _lib
+from jaxiga_amd import linear as LIN
+from jaxiga_amd.runtime import lib, stream
+
+# (name, kind, c_in, c_out, stride, input, skip); kind 0 = conv, 1 = up (+skip)
+ARCH = (
+    ("a0", 0, 1, 8, 1, "x", None),
+    ("a1", 0, 8, 16, 2, "a0", None),
+    ("a2", 0, 16, 32, 2, "a1", None),
+    ("a3", 0, 32, 64, 2, "a2", None),
+    ("b3", 0, 64, 64, 1, "a3", None),
+    ("c3", 0, 64, 64, 1, "b3", None),
+    ("d3", 0, 64, 64, 1, "c3", None),
+    ("u2", 1, 64, 32, 2, "d3", "a2"),
+    ("b2", 0, 32, 32, 1, "u2", None),
+    ("c2", 0, 32, 32, 1, "b2", None),
+    ("u1", 1, 32, 16, 2, "c2", "a1"),
+    ("b1", 0, 16, 16, 1, "u1", None),
+    ("u0", 1, 16, 8, 2, "b1", "a0"),
+    ("y", 0, 8, 1, 1, "u0", None),
+)
+
+
+def _wshape(layer):
+    _, kind, cin, cout, _, _, _ = layer
+    return (cout, cin, 3, 3) if kind == 0 else (cin, cout, 3, 3)
+
+
+def n_params():
+    return int(sum(np.prod(_wshape(l)) for l in ARCH))
+
+
+class UNet:
+    """Parameters (one flat FP32 buffer), Adam state and the C descriptor."""
+
+    def __init__(self, seed=0, device="cuda"):
+        import torch
+
+        self.offsets = {}
+        off = 0
+        for layer in ARCH:
+            self.offsets[layer[0]] = (off, _wshape(layer))
+            off += int(np.prod(_wshape(layer)))
+        self.n = off
+        rng = np.random.default_rng(seed)
+        w = np.empty(off, np.float32)
+        for layer in ARCH:
+            o, shape = self.offsets[layer[0]]
+            fan = layer[2] * 9
+            w[o : o + int(np.prod(shape))] = rng.normal(size=int(np.prod(shape))) * math.sqrt(1.0 / fan)
+        self.params = torch.as_tensor(w, device=device)
+        self.grads = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+        self._structs = {}
+
+    def weights(self, name):
+        o, shape = self.offsets[name]
+        return self.params[o : o + int(np.prod(shape))].view(shape)
+
+    def load_numpy(self, params: dict):
+        import torch
+
+        for name, arr in params.items():
+            self.weights(name).copy_(torch.as_tensor(np.asarray(arr, np.float32)))
+
+    def struct(self, H, W):
+        key = (H, W)
+        if key in self._structs:
+            return self._structs[key]
+        s = _lib.IgaNet()
+        s.n_layers = len(ARCH)
+        names = [l[0] for l in ARCH]
+        sizes = {"x": (H, W)}
+        s.H[0], s.W[0] = H, W
+        for i, (name, kind, cin, cout, stride, src, skip) in enumerate(ARCH):
+            s.kind[i], s.cin[i], s.cout[i], s.stride[i] = kind, cin, cout, stride
+            s.src[i] = -1 if src == "x" else names.index(src)
+            s.skip[i] = -1 if skip is None else names.index(skip)
+            s.woff[i] = self.offsets[name][0]
+            h, w = sizes[src]
+            if kind == 0:
+                oh, ow = (h - 1) // stride + 1, (w - 1) // stride + 1
+            else:
+                oh, ow = sizes[skip]
+            sizes[name] = (oh, ow)
+            s.H[i + 1], s.W[i + 1] = oh, ow
+        self._structs[key] = s
+        return s
+
+    def buffers(self, H, W, batch, device="cuda"):
+        import torch
+
+        st = self.struct(H, W)
+        L = lib()
+        na = int(L.iga_net_act_floats(st, batch))
+        act = torch.empty(na, dtype=torch.float32, device=device)
+        out_off = int(L.iga_net_act_offset(st, batch, len(ARCH) - 1))
+        return act, out_off
+
+    def forward(self, x):
+        """``x (B, 1, H, W)`` float32 on the GPU -> output of the same shape."""
+        import torch
+
+        B, _, H, W = x.shape
+        st = self.struct(H, W)
+        act, out_off = self.buffers(H, W, B, x.device)
+        act[: B * H * W].copy_(x.reshape(-1))
+        _lib.check(lib().iga_net_forward(st, B, _lib.ptr(self.params), _lib.ptr(act), stream()),
+                   "iga_net_forward")
+        return act[out_off : out_off + B * H * W].view(B, 1, H, W).clone(), act
+
+    def backward(self, act, gy, B, H, W):
+        """Gradients for an activation buffer from :meth:`forward`; returns d/dx."""
+        import torch
+
+        st = self.struct(H, W)
+        L = lib()
+        gact = torch.empty_like(act)
+        out_off = int(L.iga_net_act_offset(st, B, len(ARCH) - 1))
+        gact[out_off : out_off + B * H * W].copy_(gy.reshape(-1))
+        scratch = torch.empty(max(1, int(L.iga_net_scratch_floats(st, B))), dtype=torch.float32,
+                              device=act.device)
+        _lib.check(L.iga_net_backward(st, B, _lib.ptr(self.params), _lib.ptr(act), _lib.ptr(gact),
+                                      _lib.ptr(self.grads), _lib.ptr(scratch), stream()),
+                   "iga_net_backward")
+        return gact[: B * H * W].view(B, 1, H, W)
+
+    def nmse(self, pred, target):
+        """Loss and gradient of the per-sample normalised squared error."""
+        import torch
+
+        B = pred.shape[0]
+        per = pred[0].numel()
+        grad = torch.empty_like(pred)
+        sums = torch.empty(2 * B, dtype=torch.float64, device=pred.device)
+        loss = torch.empty(1, dtype=torch.float64, device=pred.device)
+        _lib.check(lib().iga_nmse(B, per, _lib.ptr(pred), _lib.ptr(target), _lib.ptr(grad),
+                                  _lib.ptr(sums), _lib.ptr(loss), stream()), "iga_nmse")
+        return loss, grad
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.step += 1
+        _lib.check(lib().iga_adam(self.n, _lib.ptr(self.params), _lib.ptr(self.grads), _lib.ptr(self.m),
+                                  _lib.ptr(self.v), lr, b1, b2, eps, self.step, stream()), "iga_adam")
+
+    def train_step(self, x, t, lr):
+        B, _, H, W = x.shape
+        y, act = self.forward(x)
+        loss, gy = self.nmse(y, t)
+        self.backward(act, gy, B, H, W)
+        self.adam(lr)
+        return loss
+
+
+# ---------------------------------------------------------------------------
+# per-operator data and the preconditioned solve
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class PCData:
+    """Everything the preconditioned solve needs for one assembled operator."""
+
+    pattern: object
+    vals64: object
+    vals32: object
+    dinv: object
+    dsq: object
+    cell: object
+    omega: float
+    H: int
+    W: int
+
+
+def power_lambda_max(pattern, vals, dinv, iters=30, seed=0):
+    """Largest eigenvalue of D^-1 K (power iteration on the device operator)."""
+    import torch
+
+    g = torch.Generator(device=vals.device).manual_seed(seed)
+    v = torch.rand(pattern.shape[0], dtype=torch.float64, device=vals.device, generator=g)
+    lam = 1.0
+    for _ in range(iters):
+        w = dinv * LIN.spmv(pattern, vals, v)
+        lam = float(w.norm() / v.norm())
+        v = w / w.norm()
+    return lam
+
+
+def pc_data(ctx, vals64, vals32=None):
+    """Grid map, diagonal scalings and the smoothing weight of a scalar 2-D
+    operator on a tensor patch."""
+    import torch
+
+    space = ctx.space
+    if space.dim != 2 or space.vec != 1:
+        raise NotImplementedError("the learned preconditioner works on scalar 2-D control grids")
+    n0, n1 = space.patch.n_cp_per_dir
+    # the control-point grid is stored first direction fastest, i.e. row-major
+    # (H, W) = (n1, n0): a scalar dof index is its own grid cell
+    cell = torch.as_tensor(ctx.dofmap.free.astype(np.int32), device=vals64.device)
+    dinv = LIN.inverse_diagonal(ctx.assembly, vals64)
+    dsq = dinv.sqrt()
+    if vals32 is None:
+        vals32 = vals64.float()
+    lam = power_lambda_max(ctx.assembly, vals64, dinv)
+    return PCData(ctx.assembly, vals64, vals32, dinv, dsq, cell, 1.0 / lam, n1, n0)
+
+
+def smooth_pair(d: PCData, r, e):
+    """Network input/target of a residual/error pair after pre-smoothing:
+    ``(D^-1/2 r1, D^1/2 e1)`` with ``x1 = w D^-1 r``, ``r1 = r - K x1``,
+    ``e1 = e - x1``; returned as grids ``(1, H, W)`` float32."""
+    import torch
+
+    x1 = d.omega * d.dinv * r
+    r1 = r - LIN.spmv(d.pattern, d.vals64, x1)
+    e1 = e - x1
+    gi = torch.zeros(d.H * d.W, dtype=torch.float32, device=r.device)
+    gt = torch.zeros_like(gi)
+    gi[d.cell.long()] = (d.dsq * r1).float()
+    gt[d.cell.long()] = (e1 / d.dsq).float()
+    return gi.view(1, d.H, d.W), gt.view(1, d.H, d.W)
+
+
+def solver_pairs(d: PCData, b, iters=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048)):
+    """Residual/error pairs from truncated Jacobi-CG solves against the
+    converged solution."""
+    import torch
+
+    work = torch.empty(int(lib().iga_cg_workspace(d.pattern.shape[0])), dtype=torch.float64,
+                       device=b.device)
+    ustar = LIN.cg_csr(d.pattern, d.vals64, b, 1e-12, dinv=d.dinv, workspace=work).x
+    out = []
+    for k in iters:
+        uk = torch.zeros_like(b) if k == 0 else LIN.cg_csr(d.pattern, d.vals64, b, 1e-14, maxiter=k,
+                                                           dinv=d.dinv, workspace=work).x
+        r = b - LIN.spmv(d.pattern, d.vals64, uk)
+        if float(r.norm()) <= 1e-13 * float(b.norm()):
+            break
+        out.append(smooth_pair(d, r, ustar - uk))
+    return out
+
+
+@dataclasses.dataclass
+class SolveResult:
+    x: object
+    iterations: int
+    fallback_iterations: int
+    relative_residual: float
+    seconds: float
+
+
+def fcg_solve(net: UNet, d: PCData, b, tol=1e-10, maxiter=2000, graph_iters=8):
+    """Flexible CG with the learned preconditioner, Jacobi-CG as the fallback
+    and judge: the FP64 residual is recomputed independently at the end."""
+    import torch
+
+    from jaxiga_amd import precision as PR
+
+    t0 = time.perf_counter()
+    L = lib()
+    n = d.pattern.shape[0]
+    st = net.struct(d.H, d.W)
+    act = torch.empty(int(L.iga_net_act_floats(st, 1)), dtype=torch.float32, device=b.device)
+    work = torch.empty(int(L.iga_fcg_workspace(n)), dtype=torch.float64, device=b.device)
+    x = torch.empty(n, dtype=torch.float64, device=b.device)
+    it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
+    _lib.check(L.iga_fcg_learned(st, _lib.ptr(net.params), _lib.ptr(act), n, _lib.ptr(d.pattern.indptr_d),
+                                 _lib.ptr(d.pattern.cols_d), _lib.ptr(d.vals64), _lib.ptr(d.vals32),
+                                 _lib.ptr(d.dinv), _lib.ptr(d.dsq), _lib.ptr(d.cell), float(d.omega),
+                                 _lib.ptr(b), _lib.ptr(x), float(tol), int(maxiter), int(graph_iters),
+                                 _lib.ptr(work), _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
+                                 _lib.ctypes.byref(bb), stream()), "iga_fcg_learned")
+    # judge: independent FP64 residual; Jacobi-CG finishes if needed
+    r = torch.empty_like(b)
+    scratch = torch.empty(2048, dtype=torch.float64, device=b.device)
+    rn, bn = PR.csr_residual(d.pattern, d.vals64, x, b, r, scratch)
+    extra = 0
+    while rn > tol * bn and extra < 10 * n:
+        res = LIN.cg_csr(d.pattern, d.vals64, r, 0.5 * tol * bn / rn, dinv=d.dinv)
+        x += res.x
+        extra += res.iterations
+        rn, bn = PR.csr_residual(d.pattern, d.vals64, x, b, r, scratch)
+        if res.iterations == 0:
+            break
+    torch.cuda.synchronize()
+    return SolveResult(x, it.value, extra, rn / bn if bn > 0 else 0.0, time.perf_counter() - t0)

@@ -183,8 +183,9 @@ def cg_csr(pattern: CSRPattern, data, b, tol=1e-10, maxiter=None, dinv=None, wor
     rr = _lib.c_double(0.0)
     bb = _lib.c_double(0.0)
     maxiter = 10 * n if maxiter is None else int(maxiter)
+    prec = _lib.IGA_FP32 if data.dtype == torch.float32 else _lib.IGA_FP64
     _lib.check(L.iga_cg_solve(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d), _lib.ptr(data),
-                              _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), float(tol), maxiter,
+                              prec, _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), float(tol), maxiter,
                               _lib.ptr(workspace), _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
                               _lib.ctypes.byref(bb), stream()), "iga_cg_solve")
     return CGResult(x, it.value, rr.value, bb.value)

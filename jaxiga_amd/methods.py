@@ -43,6 +43,7 @@ class MethodContext:
     rule_wts: object
     _pattern: object = None
     _xq: object = None
+    _ke: object = None
 
     @property
     def space(self):
@@ -172,10 +173,36 @@ def _source_args(ctx, params, args):
     return vals
 
 
+@dataclasses.dataclass
+class AssemblyLaunch:
+    """Prepared arguments and output buffers of one ``iga_assemble`` call."""
+
+    args: object
+    vals: object
+    rhs: object
+    diag: object
+    keep: list
+
+    def launch(self, ctx, passes=0):
+        self.args.passes = passes
+        _lib.check(lib().iga_assemble(ctx.space.device().struct, ctx.rule(), self.args, stream()),
+                   "iga_assemble")
+
+
 def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagonal=False,
              precision=_lib.IGA_FP64):
-    """One fused pass: CSR values, free load (source work + lifting) and/or the
+    """CSR values, free load (source work + lifting + Neumann work) and/or the
     diagonal. Returns ``(vals, rhs, diag)`` (tensors or None)."""
+    job = prepare_assembly(ctx, params, values=values, load=load, diagonal=diagonal,
+                           precision=precision)
+    job.launch(ctx)
+    if load:
+        neumann_load(ctx, params, job.rhs)
+    return job.vals, job.rhs, job.diag
+
+
+def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True, diagonal=False,
+                     precision=_lib.IGA_FP64) -> AssemblyLaunch:
     import torch
 
     space = ctx.space
@@ -199,25 +226,44 @@ def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagona
             f"{type(problem).__name__} has no GPU element kernel; the Galerkin hot path covers "
             "Poisson and LinearElasticity")
     pattern = ctx.assembly if values else None
-    vals = torch.zeros(pattern.nnz, **f64) if values else None
+    vdtype = torch.float32 if precision == _lib.IGA_FP32 else torch.float64
+    vals = torch.empty(pattern.nnz, dtype=vdtype, device=dev) if values else None
     rhs = torch.zeros(ctx.n_free, **f64) if load else None
     diag = torch.zeros(ctx.n_free, **f64) if diagonal else None
     if load:
         keep.append(_source_args(ctx, params, args))
     args.full_to_free = ctx.f2f_d.data_ptr()
+    args.freed = ctx.free_d.data_ptr() if ctx.n_free else None
     args.gfull = ctx.gfull_d.data_ptr() if load else None
-    if values:
-        args.indptr = pattern.indptr_d.data_ptr()
-        args.cols = pattern.cols_d.data_ptr() if pattern.nnz else None
-        args.vals = vals.data_ptr() if pattern.nnz else None
+    if values and pattern.nnz:
+        ws = element_workspace(ctx, precision)
+        args.nnz = pattern.nnz
+        args.rows = pattern.rows_d.data_ptr()
+        args.cols = pattern.cols_d.data_ptr()
+        args.vals = vals.data_ptr()
+        args.ke_work = ws.data_ptr()
+        args.ke_work_bytes = ws.numel()
     args.rhs = rhs.data_ptr() if load and ctx.n_free else None
     args.diag = diag.data_ptr() if diagonal and ctx.n_free else None
-    rule = ctx.rule()
-    _lib.check(lib().iga_assemble(space.device().struct, rule, args, stream()), "iga_assemble")
-    if load:
-        neumann_load(ctx, params, rhs)
-    del keep
-    return vals, rhs, diag
+    return AssemblyLaunch(args, vals, rhs, diag, keep)
+
+
+def element_workspace(ctx, precision=_lib.IGA_FP64):
+    """Element-matrix buffer for the two-pass assembly (cached on the context)."""
+    import torch
+
+    key = (ctx.problem.kernel, precision)
+    ws = ctx._ke.get(key) if ctx._ke is not None else None
+    if ws is None:
+        kind = _lib.IGA_ELASTICITY if ctx.problem.kernel == "elasticity" else _lib.IGA_POISSON
+        nbytes = int(lib().iga_assemble_workspace(ctx.space.device().struct, kind, precision))
+        if nbytes < 0:
+            raise NotImplementedError(f"no GPU assembly for dim={ctx.space.dim}, degree={ctx.space.degree}")
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=ctx.f2f_d.device)
+        if ctx._ke is None:
+            ctx._ke = {}
+        ctx._ke[key] = ws
+    return ws
 
 
 def neumann_load(ctx, params, rhs):

@@ -212,6 +212,23 @@ class FunctionSpace:
             out.append((lo, hi))
         return out
 
+    def support_spans(self):
+        """Per direction, the first and last span on which each 1-D basis
+        function is supported: ``[(elo (n_cp_d,), ehi (n_cp_d,)), ...]``."""
+        out = []
+        for d in range(self.dim):
+            p = self.degree[d]
+            first = self._spans[d][2]
+            n = self.patch.n_cp_per_dir[d]
+            elo = np.full(n, np.iinfo(np.int64).max)
+            ehi = np.full(n, -1)
+            span = np.arange(len(first))
+            for a in range(p + 1):
+                np.minimum.at(elo, first + a, span)
+                np.maximum.at(ehi, first + a, span)
+            out.append((elo, ehi))
+        return out
+
     # -- device tables ------------------------------------------------------------
     def device(self):
         """Lazily built :class:`DeviceSpace` (tables on the current GPU)."""
@@ -247,6 +264,9 @@ class DeviceSpace:
         nb = space.neighbour_ranges()
         self.nb_lo = [torch.as_tensor(lo, **i32) for lo, _ in nb]
         self.nb_hi = [torch.as_tensor(hi, **i32) for _, hi in nb]
+        sup = space.support_spans()
+        self.elo = [torch.as_tensor(lo, **i32) for lo, _ in sup]
+        self.ehi = [torch.as_tensor(hi, **i32) for _, hi in sup]
         self.elem_dofs = torch.as_tensor(space.elem_dofs, **i32)
         s = _lib.IgaPatch()
         s.dim = space.dim
@@ -259,6 +279,8 @@ class DeviceSpace:
                 s.first[d] = self.first[d].data_ptr()
                 s.lo[d] = self.lo[d].data_ptr()
                 s.hi[d] = self.hi[d].data_ptr()
+                s.elo[d] = self.elo[d].data_ptr()
+                s.ehi[d] = self.ehi[d].data_ptr()
             else:
                 s.n_elem[d] = 1
                 s.n_cp[d] = 1

@@ -106,7 +106,8 @@ def coefficient(params, name, default, x):
 
 def poisson_tangents(basis, params, vec=1):
     a0 = coefficient(params, "a0", 1.0, basis.x)
-    K = np.einsum("eq,eqdi,eqdj->eij", basis.w * a0, basis.dR, basis.dR)
+    K = np.einsum("eqdi,eqdj->eij", basis.dR * (basis.w * a0)[:, :, None, None], basis.dR,
+                  optimize=True)
     if vec == 1:
         return K
     n_e, n_l, _ = K.shape
@@ -117,7 +118,8 @@ def poisson_tangents(basis, params, vec=1):
 def elasticity_tangents(basis, params, dim, plane="stress"):
     C = constitutive(params["E"], params["nu"], dim, plane)
     B = strain_displacement(basis.dR)
-    return np.einsum("eq,eqsi,st,eqtj->eij", basis.w, B, C, B)
+    CB = np.einsum("st,eqtj->eqsj", C, B, optimize=True) * basis.w[:, :, None, None]
+    return np.einsum("eqsi,eqsj->eij", B, CB, optimize=True)
 
 
 def _call_points(fn, x, *extra):
