@@ -32,6 +32,8 @@
  *                             (no reference counterpart: the learned preconditioner;
  *                             the reference allows only "none"/"jacobi",
  *                             solvers/linear.py:113-116)
+ *   iga_batched_cg            (no reference counterpart: the reference batches by
+ *                             vmap over params, tests/test_advanced.py:337-344)
  *   iga_rff_eval              (no reference counterpart: seeded GRF inputs)
  */
 #ifndef JAXIGA_HIP_H
@@ -99,6 +101,10 @@ enum iga_prec { IGA_FP64 = 0, IGA_FP32 = 1, IGA_BF16 = 2 };
  * rhs        : (n_free) += source work - K_fp g ; diag (n_free) += diag(K_e)
  * passes     : bit 1 element pass, bit 2 gather pass (0 = both); the
  *              gather reads the element matrices of an earlier element pass
+ * batch      : number of coefficient/source instances sharing the patch and
+ *              pattern (0 or 1 = one); instance b uses coef_points/src_points
+ *              + b*n_e*n_q(*vec), rhs/diag + b*n_free, vals + b*nnz and its
+ *              own slice of ke_work (the workspace scales with batch)
  * vals, rhs and diag may each be null.                                    */
 typedef struct iga_asm_args {
   int32_t problem;
@@ -123,6 +129,8 @@ typedef struct iga_asm_args {
   double* rhs;
   double* diag;
   int32_t passes;
+  int32_t batch;
+  int32_t n_free;
 } iga_asm_args_t;
 
 /* bytes of element-matrix workspace iga_assemble needs for vals, or -1 */
@@ -270,6 +278,17 @@ int iga_fcg_learned(const iga_net_t* net, const float* params, float* act, int32
                     const int32_t* cell, double omega, const double* b, double* x, double tol,
                     int32_t maxiter, int32_t graph_iters, double* work, int32_t* out_iters,
                     double* out_rr, double* out_bb, void* stream);
+
+/* Batched Jacobi-PCG: one workgroup per instance runs the whole solve in one
+ * persistent launch. insts: device array of n_inst records
+ *   { int32 n, pad; const int32* indptr, *cols; const double* vals, *dinv,
+ *     *b; double* x; double* work (4n) }
+ * order: device permutation (largest first); lanes_per_row in {8,16,32,64};
+ * threads a multiple of 64 up to 1024. Outputs per instance: iterations and
+ * the recursive relative residual. No host synchronisation (graph-capturable). */
+int iga_batched_cg(int32_t n_inst, const void* insts, const int32_t* order,
+                   int32_t lanes_per_row, int32_t threads, double tol, int32_t maxiter,
+                   int32_t* out_iters, double* out_rel, void* stream);
 
 /* Seeded random-Fourier-feature field g(x) = amp * sum_m cos(omega_m . x + phase_m)
  * at n points (n, dim) (device, FP64 throughout). */

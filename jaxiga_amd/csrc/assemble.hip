@@ -7,8 +7,9 @@
 // gradients come from one lane per (Gauss point, local function) with
 // group-of-lanes reductions. The element matrix K_e = Bm^T (W D) Bm is then
 // contracted on the matrix cores (v_mfma_f64_16x16x4 in FP64,
-// v_mfma_f32_16x16x4 in FP32) from LDS strain tables and written to an
-// element-matrix buffer. The load (source work) and the Dirichlet lifting
+// v_mfma_f32_16x16x4 in FP32) -- (W D) Bm staged in LDS, the strain operand Bm
+// formed on the fly from the gradients -- and each lane streams its
+// accumulators to an element-matrix buffer. The load (source work) and the Dirichlet lifting
 // -K_fp g, and optionally the diagonal, are accumulated in the same pass.
 //
 // Pass 2 (k_gather) -- one lane per CSR nonzero of the free-free block. The
@@ -34,6 +35,7 @@ struct AsmDev {
   const double* gfull;
   double* rhs;
   double* diag;
+  int n_free;  // rhs/diag stride between batch instances
 };
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -41,9 +43,14 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// Output mapping of the 16x16x4 MFMA accumulators: lane l holds column l%16;
+// register r holds row l/16 + 4r for the f64 instruction and 4*(l/16) + r for
+// the f32 instruction. The A operand of lane l is A[l%16][l/16], the B
+// operand B[l/16][l%16], for both.
 template <>
 struct Mfma<double> {
   using acc_t = d4;
+  static __device__ __forceinline__ int row(int k4, int r) { return k4 + 4 * r; }
   static __device__ __forceinline__ acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
   static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -52,6 +59,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   using acc_t = f4;
+  static __device__ __forceinline__ int row(int k4, int r) { return 4 * k4 + r; }
   static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
   static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -115,13 +123,21 @@ __global__ void __launch_bounds__(1024)
   double* Rs = G + Grows * NL;          // [nq][NL]
   double* Wk = Rs + nq * NL;            // [nq] weight * coefficient
   double* Wf = Wk + nq;                 // [nq] weight
-  double* Ks = Wf + nq;                 // [NLV][NLV]
-  T* Hm = reinterpret_cast<T*>(Ks + NLV * NLV);      // [KKp][NLV]
-  T* Bm = Hm + KKp * NLV;                            // [KKp][NLV], elasticity only
-  int* fdof = reinterpret_cast<int*>(Bm + (PROB == IGA_POISSON ? 0 : KKp * NLV) + 2);  // [NL*3]
+  T* Hm = reinterpret_cast<T*>(Wf + nq);             // [KKp][NLV] = (W D) Bm
+  int* fdof = reinterpret_cast<int*>(Hm + KKp * NLV + 2);  // [NL*3] free index or -1
   int* gdof = fdof + NL * 3;                         // [NL] scalar dofs
 
   const int e = blockIdx.x;
+  {
+    // batch instance: shift the per-instance inputs and outputs
+    const int b = blockIdx.y;
+    const int64_t n_e = (int64_t)gridDim.x;
+    if (a.coef_points) a.coef_points += (size_t)b * n_e * nq;
+    if (a.src_points) a.src_points += (size_t)b * n_e * nq * a.vec;
+    if (a.rhs) a.rhs += (size_t)b * a.n_free;
+    if (a.diag) a.diag += (size_t)b * a.n_free;
+    if (ke) ke += (size_t)b * n_e * S::NLV * S::NLV;
+  }
   int ei[3] = {0, 0, 0};
   {
     int rem = e;
@@ -221,12 +237,12 @@ __global__ void __launch_bounds__(1024)
   }
   __syncthreads();
 
-  // ---- strain tables: Bm[(q,s)][col], Hm = (W D) Bm -------------------------
+  // ---- H = (W D) Bm in LDS; Bm itself is formed on the fly from G ---------
   if constexpr (PROB == IGA_POISSON)
     for (int idx = KK * NL + threadIdx.x; idx < KKp * NL; idx += blockDim.x) G[idx] = 0.0;
   for (int idx = threadIdx.x; idx < KKp * NLV; idx += blockDim.x) {
     const int k = idx / NLV, col = idx % NLV;
-    double bv = 0.0, hv = 0.0;
+    double hv = 0.0;
     if (k < KK) {
       const int q = k / NS, s = k % NS;
       if constexpr (PROB == IGA_POISSON) {
@@ -236,20 +252,22 @@ __global__ void __launch_bounds__(1024)
         double gi[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) gi[d] = G[(q * DIM + d) * NL + i];
-        bv = strain<DIM>(gi, c, s);
         double acc = 0.0;
 #pragma unroll
         for (int t = 0; t < NS; ++t) acc += a.cmat[s * NS + t] * strain<DIM>(gi, c, t);
         hv = Wf[q] * acc;
       }
     }
-    if constexpr (PROB != IGA_POISSON) Bm[idx] = (T)bv;
     Hm[idx] = (T)hv;
   }
   __syncthreads();
 
-  // ---- K_e = Bm^T Hm on the matrix cores, one 16x16 tile per wave -----------
+  // ---- K_e = Bm^T H on the matrix cores, one 16x16 tile per wave; each
+  // ---- lane streams its entries to the element-matrix buffer and applies the
+  // ---- lifting and the diagonal from its accumulators
   {
+    const bool lift = a.rhs != nullptr && a.gfull != nullptr;
+    T* kout = ke ? ke + (size_t)e * NLV * NLV : nullptr;
     const int wave = threadIdx.x / kWave, wl = threadIdx.x % kWave;
     const int nwaves = blockDim.x / kWave;
     const int r16 = wl % 16, k4 = wl / 16;
@@ -261,51 +279,48 @@ __global__ void __launch_bounds__(1024)
         const int k = k0 + k4;
         T av = (T)0;
         if (ii < NLV) {
-          if constexpr (PROB == IGA_POISSON)
+          if constexpr (PROB == IGA_POISSON) {
             av = (T)G[k * NL + ii];
-          else
-            av = Bm[k * NLV + ii];
+          } else if (k < KK) {
+            const int q = k / NS, s = k % NS, il = ii / DIM, c = ii % DIM;
+            double gi[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) gi[d] = G[(q * DIM + d) * NL + il];
+            av = (T)strain<DIM>(gi, c, s);
+          }
         }
         const T bv = jj < NLV ? Hm[k * NLV + jj] : (T)0;
         acc = Mfma<T>::step(av, bv, acc);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + 4 * k4 + r, j = j0 + r16;
-        if (i < NLV && j < NLV) Ks[i * NLV + j] = (double)acc[r];
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---- element-matrix buffer, lifting, diagonal ----------------------------
-  const bool lift = a.rhs != nullptr && a.gfull != nullptr;
-  T* kout = ke ? ke + (size_t)e * NLV * NLV : nullptr;
-  for (int idx = threadIdx.x; idx < NLV * NLV; idx += blockDim.x) {
-    const double v = Ks[idx];
-    if (kout) kout[idx] = (T)v;
-    const int i = idx / NLV, j = idx % NLV;
-    if constexpr (PROB == IGA_POISSON) {
-      for (int c = 0; c < a.vec; ++c) {
-        const int r = fdof[i * a.vec + c];
-        if (r < 0) continue;
-        const int cc = fdof[j * a.vec + c];
-        if (cc >= 0) {
-          if (a.diag && i == j) unsafeAtomicAdd(a.diag + r, v);
-        } else if (lift) {
-          const double gv = a.gfull[(size_t)a.vec * gdof[j] + c];
-          if (gv != 0.0) unsafeAtomicAdd(a.rhs + r, -v * gv);
+        const int i = i0 + Mfma<T>::row(k4, r), j = j0 + r16;
+        if (i >= NLV || j >= NLV) continue;
+        const double v = (double)acc[r];
+        if (kout) kout[i * NLV + j] = acc[r];
+        if constexpr (PROB == IGA_POISSON) {
+          for (int c = 0; c < a.vec; ++c) {
+            const int rr = fdof[i * a.vec + c];
+            if (rr < 0) continue;
+            const int cc = fdof[j * a.vec + c];
+            if (cc >= 0) {
+              if (a.diag && i == j) unsafeAtomicAdd(a.diag + rr, v);
+            } else if (lift) {
+              const double gv = a.gfull[(size_t)a.vec * gdof[j] + c];
+              if (gv != 0.0) unsafeAtomicAdd(a.rhs + rr, -v * gv);
+            }
+          }
+        } else {
+          const int rr = fdof[i];
+          if (rr < 0) continue;
+          const int cc = fdof[j];
+          if (cc >= 0) {
+            if (a.diag && i == j) unsafeAtomicAdd(a.diag + rr, v);
+          } else if (lift) {
+            const double gv = a.gfull[(size_t)DIM * gdof[j / DIM] + (j % DIM)];
+            if (gv != 0.0) unsafeAtomicAdd(a.rhs + rr, -v * gv);
+          }
         }
-      }
-    } else {
-      const int r = fdof[i];
-      if (r < 0) continue;
-      const int cc = fdof[j];
-      if (cc >= 0) {
-        if (a.diag && i == j) unsafeAtomicAdd(a.diag + r, v);
-      } else if (lift) {
-        const double gv = a.gfull[(size_t)DIM * gdof[j / DIM] + (j % DIM)];
-        if (gv != 0.0) unsafeAtomicAdd(a.rhs + r, -v * gv);
       }
     }
   }
@@ -334,11 +349,14 @@ template <int DIM, int P, int PROB, typename T>
 __global__ void __launch_bounds__(256)
     k_gather(iga_patch_t pt, int vec, int64_t nnz, const int32_t* __restrict__ rows,
              const int32_t* __restrict__ cols, const int32_t* __restrict__ freed,
-             const T* __restrict__ ke, T* __restrict__ vals) {
+             const T* __restrict__ ke_all, T* __restrict__ vals_all, int batch, int64_t n_e) {
   using S = Shape<DIM, P, PROB>;
   constexpr int N1 = S::N1, NL = S::NL, NLV = S::NLV;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
-       k += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < nnz * batch;
+       kk += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t bi = kk / nnz, k = kk - bi * nnz;
+    const T* ke = ke_all + (size_t)bi * n_e * NLV * NLV;
+    T* vals = vals_all + (size_t)bi * nnz;
     const int gi = freed[rows[k]], gj = freed[cols[k]];
     const int si = gi / vec, ci = gi - si * vec;
     const int sj = gj / vec, cj = gj - sj * vec;
@@ -397,17 +415,17 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
   if (threads > 1024) threads = 1024;
   if (threads < 256) threads = 256;
   const int Grows = PROB == IGA_POISSON ? KKp : nq * DIM;
-  const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + Grows * NL + nq * NL + 2 * nq +
-                                       NLV * NLV) +
-                     sizeof(T) * ((PROB == IGA_POISSON ? 1 : 2) * KKp * NLV + 2) +
-                     sizeof(int) * (NL * 3 + NL);
+  const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + Grows * NL + nq * NL + 2 * nq) +
+                     sizeof(T) * (KKp * NLV + 2) + sizeof(int) * (NL * 3 + NL);
   if (lds > 64 * 1024) return IGA_ERR_UNSUPPORTED;
   int n_e = 1;
   for (int d = 0; d < DIM; ++d) n_e *= pt.n_elem[d];
   if (n_e == 0) return IGA_OK;
+  const int batch = args.batch > 1 ? args.batch : 1;
+  if (batch > 1 && (a.rhs || a.diag) && args.n_free <= 0) return IGA_ERR_ARG;
   T* ke = reinterpret_cast<T*>(args.ke_work);
   if (args.vals) {
-    const int64_t need = (int64_t)n_e * NLV * NLV * sizeof(T);
+    const int64_t need = (int64_t)batch * n_e * NLV * NLV * sizeof(T);
     if (!ke || args.ke_work_bytes < need) return IGA_ERR_SHAPE;
     if (!args.rows || !args.cols || !args.freed) return IGA_ERR_ARG;
   } else {
@@ -415,15 +433,15 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
   }
   const int passes = args.passes ? args.passes : 3;
   if (passes & 1) {
-    hipLaunchKernelGGL((k_element<DIM, P, PROB, T>), dim3(n_e), dim3(threads), lds, st, pt, nq1,
-                       rule.pts, rule.wts, a, ke);
+    hipLaunchKernelGGL((k_element<DIM, P, PROB, T>), dim3(n_e, batch), dim3(threads), lds, st, pt,
+                       nq1, rule.pts, rule.wts, a, ke);
     IGA_CHECK_LAUNCH();
   }
   if ((passes & 2) && args.vals && args.nnz > 0) {
-    const int64_t blocks = std::min<int64_t>((args.nnz + 255) / 256, 256 * 64);
+    const int64_t blocks = std::min<int64_t>((args.nnz * batch + 255) / 256, 256 * 64);
     hipLaunchKernelGGL((k_gather<DIM, P, PROB, T>), dim3((unsigned)blocks), dim3(256), 0, st, pt,
                        args.vec, args.nnz, args.rows, args.cols, args.freed, ke,
-                       reinterpret_cast<T*>(args.vals));
+                       reinterpret_cast<T*>(args.vals), batch, (int64_t)n_e);
     IGA_CHECK_LAUNCH();
   }
   return IGA_OK;
@@ -489,6 +507,7 @@ extern "C" int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule,
   a.gfull = args->gfull;
   a.rhs = args->rhs;
   a.diag = args->diag;
+  a.n_free = args->n_free;
   if (!a.f2f) return IGA_ERR_ARG;
   if (a.problem != IGA_POISSON && a.problem != IGA_ELASTICITY) return IGA_ERR_ARG;
   if (a.problem == IGA_ELASTICITY && a.vec != patch->dim) return IGA_ERR_SHAPE;

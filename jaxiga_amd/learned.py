@@ -300,12 +300,17 @@ def fcg_solve(net: UNet, d: PCData, b, tol=1e-10, maxiter=2000, graph_iters=8):
     work = torch.empty(int(L.iga_fcg_workspace(n)), dtype=torch.float64, device=b.device)
     x = torch.empty(n, dtype=torch.float64, device=b.device)
     it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
-    _lib.check(L.iga_fcg_learned(st, _lib.ptr(net.params), _lib.ptr(act), n, _lib.ptr(d.pattern.indptr_d),
-                                 _lib.ptr(d.pattern.cols_d), _lib.ptr(d.vals64), _lib.ptr(d.vals32),
-                                 _lib.ptr(d.dinv), _lib.ptr(d.dsq), _lib.ptr(d.cell), float(d.omega),
-                                 _lib.ptr(b), _lib.ptr(x), float(tol), int(maxiter), int(graph_iters),
-                                 _lib.ptr(work), _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
-                                 _lib.ctypes.byref(bb), stream()), "iga_fcg_learned")
+    # graph capture needs a stream other than the legacy default one
+    side = torch.cuda.Stream() if graph_iters > 0 else torch.cuda.current_stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        _lib.check(L.iga_fcg_learned(
+            st, _lib.ptr(net.params), _lib.ptr(act), n, _lib.ptr(d.pattern.indptr_d),
+            _lib.ptr(d.pattern.cols_d), _lib.ptr(d.vals64), _lib.ptr(d.vals32), _lib.ptr(d.dinv),
+            _lib.ptr(d.dsq), _lib.ptr(d.cell), float(d.omega), _lib.ptr(b), _lib.ptr(x), float(tol),
+            int(maxiter), int(graph_iters), _lib.ptr(work), _lib.ctypes.byref(it),
+            _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()), "iga_fcg_learned")
+    torch.cuda.current_stream().wait_stream(side)
     # judge: independent FP64 residual; Jacobi-CG finishes if needed
     r = torch.empty_like(b)
     scratch = torch.empty(2048, dtype=torch.float64, device=b.device)

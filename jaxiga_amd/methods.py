@@ -234,6 +234,8 @@ def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True,
         keep.append(_source_args(ctx, params, args))
     args.full_to_free = ctx.f2f_d.data_ptr()
     args.freed = ctx.free_d.data_ptr() if ctx.n_free else None
+    args.n_free = ctx.n_free
+    args.batch = 1
     args.gfull = ctx.gfull_d.data_ptr() if load else None
     if values and pattern.nnz:
         ws = element_workspace(ctx, precision)
