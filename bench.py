@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solve measurement")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 256 MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -108,6 +110,14 @@ def main():
     from jaxiga_amd import methods as M
 
     torch.cuda.init()
+    if args.copy_calibration:
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] copied 5 x 256 MiB", flush=True)
+        return
     lib = _lib.load()
     precision = _lib.IGA_FP32 if args.precision == "fp32" else _lib.IGA_FP64
     dtype = "f32" if args.precision == "fp32" else "f64"

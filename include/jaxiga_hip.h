@@ -105,6 +105,9 @@ enum iga_prec { IGA_FP64 = 0, IGA_FP32 = 1, IGA_BF16 = 2 };
  *              pattern (0 or 1 = one); instance b uses coef_points/src_points
  *              + b*n_e*n_q(*vec), rhs/diag + b*n_free, vals + b*nnz and its
  *              own slice of ke_work (the workspace scales with batch)
+ * indptr     : (n_free + 1) CSR row pointers, optional: with it (and n_free)
+ *              the gather pass runs one wavefront per row over coalesced
+ *              element-matrix rows; without it, one thread per nonzero
  * vals, rhs and diag may each be null.                                    */
 typedef struct iga_asm_args {
   int32_t problem;
@@ -131,6 +134,7 @@ typedef struct iga_asm_args {
   int32_t passes;
   int32_t batch;
   int32_t n_free;
+  const int32_t* indptr;
 } iga_asm_args_t;
 
 /* bytes of element-matrix workspace iga_assemble needs for vals, or -1 */

@@ -89,6 +89,7 @@ class _Group:
         args.full_to_free, args.freed = self.ctx.f2f_d.data_ptr(), self.ctx.free_d.data_ptr()
         args.gfull = self.ctx.gfull_d.data_ptr()
         args.nnz, args.rows, args.cols = nnz, self.pattern.rows_d.data_ptr(), self.pattern.cols_d.data_ptr()
+        args.indptr = self.pattern.indptr_d.data_ptr()
         args.vals, args.ke_work, args.ke_work_bytes = self.vals.data_ptr(), self.ke.data_ptr(), self.ke.numel()
         args.rhs, args.diag = self.rhs.data_ptr(), self.diag.data_ptr()
         args.batch, args.n_free = B, nf

@@ -106,7 +106,7 @@ struct Shape {
 template <int DIM, int P, int PROB, typename T>
 __global__ void __launch_bounds__(1024)
     k_element(iga_patch_t pt, int nq1, const double* __restrict__ qp,
-              const double* __restrict__ qw, AsmDev a, T* __restrict__ ke) {
+              const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all) {
   using S = Shape<DIM, P, PROB>;
   constexpr int N1 = S::N1, NL = S::NL, NLP = S::NLP, NS = S::NS, NLV = S::NLV, TI = S::TI;
   const int nq = DIM == 2 ? nq1 * nq1 : nq1 * nq1 * nq1;
@@ -126,18 +126,21 @@ __global__ void __launch_bounds__(1024)
   T* Hm = reinterpret_cast<T*>(Wf + nq);             // [KKp][NLV] = (W D) Bm
   int* fdof = reinterpret_cast<int*>(Hm + KKp * NLV + 2);  // [NL*3] free index or -1
   int* gdof = fdof + NL * 3;                         // [NL] scalar dofs
+  // constitutive matrix and constant source staged in LDS: indexing the
+  // kernel-argument copies with runtime indices would place the whole
+  // argument block in per-lane scratch memory
+  __shared__ double cm[36], srcc[3];
+  if (threadIdx.x < 36) cm[threadIdx.x] = a.cmat[threadIdx.x];
+  if (threadIdx.x < 3) srcc[threadIdx.x] = a.src_const[threadIdx.x];
 
   const int e = blockIdx.x;
-  {
-    // batch instance: shift the per-instance inputs and outputs
-    const int b = blockIdx.y;
-    const int64_t n_e = (int64_t)gridDim.x;
-    if (a.coef_points) a.coef_points += (size_t)b * n_e * nq;
-    if (a.src_points) a.src_points += (size_t)b * n_e * nq * a.vec;
-    if (a.rhs) a.rhs += (size_t)b * a.n_free;
-    if (a.diag) a.diag += (size_t)b * a.n_free;
-    if (ke) ke += (size_t)b * n_e * S::NLV * S::NLV;
-  }
+  // batch instance: per-instance inputs and outputs
+  const int64_t bi = blockIdx.y, n_e = gridDim.x;
+  const double* coefp = a.coef_points ? a.coef_points + bi * n_e * nq : nullptr;
+  const double* srcp = a.src_points ? a.src_points + bi * n_e * nq * a.vec : nullptr;
+  double* rhsp = a.rhs ? a.rhs + bi * a.n_free : nullptr;
+  double* diagp = a.diag ? a.diag + bi * a.n_free : nullptr;
+  T* ke = ke_all ? ke_all + bi * n_e * S::NLV * S::NLV : nullptr;
   int ei[3] = {0, 0, 0};
   {
     int rem = e;
@@ -230,7 +233,7 @@ __global__ void __launch_bounds__(1024)
     }
     if (lane == 0) {
       const double coef =
-          a.coef_kind == IGA_COEF_POINTS ? a.coef_points[(size_t)e * nq + q] : a.coef_const;
+          a.coef_kind == IGA_COEF_POINTS ? coefp[(size_t)e * nq + q] : a.coef_const;
       Wk[q] = wq * coef;
       Wf[q] = wq;
     }
@@ -254,7 +257,7 @@ __global__ void __launch_bounds__(1024)
         for (int d = 0; d < DIM; ++d) gi[d] = G[(q * DIM + d) * NL + i];
         double acc = 0.0;
 #pragma unroll
-        for (int t = 0; t < NS; ++t) acc += a.cmat[s * NS + t] * strain<DIM>(gi, c, t);
+        for (int t = 0; t < NS; ++t) acc += cm[s * NS + t] * strain<DIM>(gi, c, t);
         hv = Wf[q] * acc;
       }
     }
@@ -266,7 +269,7 @@ __global__ void __launch_bounds__(1024)
   // ---- lane streams its entries to the element-matrix buffer and applies the
   // ---- lifting and the diagonal from its accumulators
   {
-    const bool lift = a.rhs != nullptr && a.gfull != nullptr;
+    const bool lift = rhsp != nullptr && a.gfull != nullptr;
     T* kout = ke ? ke + (size_t)e * NLV * NLV : nullptr;
     const int wave = threadIdx.x / kWave, wl = threadIdx.x % kWave;
     const int nwaves = blockDim.x / kWave;
@@ -304,10 +307,10 @@ __global__ void __launch_bounds__(1024)
             if (rr < 0) continue;
             const int cc = fdof[j * a.vec + c];
             if (cc >= 0) {
-              if (a.diag && i == j) unsafeAtomicAdd(a.diag + rr, v);
+              if (diagp && i == j) unsafeAtomicAdd(diagp + rr, v);
             } else if (lift) {
               const double gv = a.gfull[(size_t)a.vec * gdof[j] + c];
-              if (gv != 0.0) unsafeAtomicAdd(a.rhs + rr, -v * gv);
+              if (gv != 0.0) unsafeAtomicAdd(rhsp + rr, -v * gv);
             }
           }
         } else {
@@ -315,10 +318,10 @@ __global__ void __launch_bounds__(1024)
           if (rr < 0) continue;
           const int cc = fdof[j];
           if (cc >= 0) {
-            if (a.diag && i == j) unsafeAtomicAdd(a.diag + rr, v);
+            if (diagp && i == j) unsafeAtomicAdd(diagp + rr, v);
           } else if (lift) {
             const double gv = a.gfull[(size_t)DIM * gdof[j / DIM] + (j % DIM)];
-            if (gv != 0.0) unsafeAtomicAdd(a.rhs + rr, -v * gv);
+            if (gv != 0.0) unsafeAtomicAdd(rhsp + rr, -v * gv);
           }
         }
       }
@@ -326,7 +329,7 @@ __global__ void __launch_bounds__(1024)
   }
 
   // ---- source work ----------------------------------------------------------
-  if (a.rhs && a.src_kind != IGA_SRC_NONE) {
+  if (rhsp && a.src_kind != IGA_SRC_NONE) {
     for (int idx = threadIdx.x; idx < NL * a.vec; idx += blockDim.x) {
       const int i = idx / a.vec, c = idx % a.vec;
       const int r = fdof[i * a.vec + c];
@@ -334,11 +337,11 @@ __global__ void __launch_bounds__(1024)
       double f = 0.0;
       for (int q = 0; q < nq; ++q) {
         const double fv = a.src_kind == IGA_SRC_CONST
-                              ? a.src_const[c]
-                              : a.src_points[((size_t)e * nq + q) * a.vec + c];
+                              ? srcc[c]
+                              : srcp[((size_t)e * nq + q) * a.vec + c];
         f += Wf[q] * fv * Rs[q * NL + i];
       }
-      unsafeAtomicAdd(a.rhs + r, f);
+      unsafeAtomicAdd(rhsp + r, f);
     }
   }
 }
@@ -402,6 +405,126 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Pass 2 (row form): one wavefront per CSR row. For each element supporting
+// the row's basis function (fixed order) the wave reads that element's
+// matrix row -- NLV contiguous values, a coalesced read -- and accumulates it
+// into a dense stencil box of the row held in LDS; the box is then written
+// out in CSR order. Lane groups take different elements into separate boxes
+// that are summed in a fixed order at the end: bitwise reproducible.
+template <int DIM, int P, int PROB, typename T>
+__global__ void __launch_bounds__(256)
+    k_gather_rows(iga_patch_t pt, int vec, int32_t n_rows, const int32_t* __restrict__ indptr,
+                  const int32_t* __restrict__ cols, const int32_t* __restrict__ freed,
+                  const T* __restrict__ ke_all, T* __restrict__ vals_all, int64_t nnz,
+                  int64_t n_e) {
+  using S = Shape<DIM, P, PROB>;
+  constexpr int N1 = S::N1, NL = S::NL, NLV = S::NLV;
+  constexpr int VEC = S::VEC;
+  constexpr int BW = 2 * P + 1;                           // stencil box width per direction
+  constexpr int BOX = DIM == 2 ? BW * BW : BW * BW * BW;  // control points in the box
+  constexpr int BOXV = BOX * VEC;
+  constexpr int NLVP = NLV <= 8 ? 8 : NLV <= 16 ? 16 : NLV <= 32 ? 32 : 64;  // lanes per element
+  constexpr int GROUPS = kWave / NLVP;                                       // elements per step
+  constexpr int WAVES = 4;
+  __shared__ double box[WAVES][GROUPS][BOXV];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int row = blockIdx.x * WAVES + wave;
+  if (row >= n_rows) return;
+  const int64_t b = blockIdx.y;
+  const T* ke = ke_all + b * n_e * NLV * NLV;
+  T* vals = vals_all + b * nnz;
+  double* mybox = &box[wave][0][0];
+  // the box is private to the wave; LDS operations of one wave execute in
+  // order, so wave barriers (which stop compiler reordering) separate phases
+  for (int t = lane; t < GROUPS * BOXV; t += kWave) mybox[t] = 0.0;
+  __builtin_amdgcn_wave_barrier();
+
+  const int gi = freed[row];
+  const int si = gi / vec, ci = gi - si * vec;
+  int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, ne[3] = {1, 1, 1};
+  {
+    int rem = si;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      id[d] = rem % pt.n_cp[d];
+      rem /= pt.n_cp[d];
+      lo[d] = pt.elo[d][id[d]];
+      ne[d] = pt.ehi[d][id[d]] - lo[d] + 1;
+    }
+  }
+  const int n_sup = ne[0] * ne[1] * ne[2];
+  const int grp = lane / NLVP, gl = lane % NLVP;
+  for (int s0 = 0; s0 < n_sup; s0 += GROUPS) {
+    const int sidx = s0 + grp;
+    if (sidx >= n_sup) {
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
+    int es[3] = {0, 0, 0};
+    {
+      int rem = sidx;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        es[d] = lo[d] + rem % ne[d];
+        rem /= ne[d];
+      }
+    }
+    int e = 0, estride = 1, li = 0, lstride = 1, fo[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int f = pt.first[d][es[d]];
+      e += es[d] * estride;
+      estride *= pt.n_elem[d];
+      li += (id[d] - f) * lstride;
+      lstride *= N1;
+      fo[d] = f - id[d] + P;  // box offset of the element's first function
+    }
+    const int lrow = PROB == IGA_POISSON ? li : li * VEC + ci;
+    const T* krow = ke + (size_t)e * NLV * NLV + (size_t)lrow * NLV;
+    for (int c = gl; c < NLV; c += NLVP) {
+      const int f = PROB == IGA_POISSON ? c : c / VEC;
+      const int cj = PROB == IGA_POISSON ? 0 : c - f * VEC;
+      int rem = f, o = 0, ostride = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        o += (fo[d] + rem % N1) * ostride;
+        rem /= N1;
+        ostride *= BW;
+      }
+      mybox[grp * BOXV + o * VEC + cj] += (double)krow[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // fixed-order sum of the group boxes
+  if (GROUPS > 1)
+    for (int t = lane; t < BOXV; t += kWave) {
+      double acc = mybox[t];
+#pragma unroll
+      for (int g = 1; g < GROUPS; ++g) acc += mybox[g * BOXV + t];
+      mybox[t] = acc;
+    }
+  __builtin_amdgcn_wave_barrier();
+  const int k0 = indptr[row], k1 = indptr[row + 1];
+  for (int k = k0 + lane; k < k1; k += kWave) {
+    const int gj = freed[cols[k]];
+    const int sj = gj / vec, cjj = gj - sj * vec;
+    int rem = sj, o = 0, ostride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int jd = rem % pt.n_cp[d];
+      rem /= pt.n_cp[d];
+      o += (jd - id[d] + P) * ostride;
+      ostride *= BW;
+    }
+    double v;
+    if constexpr (PROB == IGA_POISSON)
+      v = cjj == ci ? mybox[o] : 0.0;
+    else
+      v = mybox[o * VEC + cjj];
+    vals[k] = (T)v;
+  }
+}
+
 template <int DIM, int P, int PROB, typename T>
 static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const AsmDev& a,
                            const iga_asm_args_t& args, hipStream_t st) {
@@ -438,10 +561,17 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
     IGA_CHECK_LAUNCH();
   }
   if ((passes & 2) && args.vals && args.nnz > 0) {
-    const int64_t blocks = std::min<int64_t>((args.nnz * batch + 255) / 256, 256 * 64);
-    hipLaunchKernelGGL((k_gather<DIM, P, PROB, T>), dim3((unsigned)blocks), dim3(256), 0, st, pt,
-                       args.vec, args.nnz, args.rows, args.cols, args.freed, ke,
-                       reinterpret_cast<T*>(args.vals), batch, (int64_t)n_e);
+    if (args.indptr && args.n_free > 0) {
+      const unsigned blocks = (unsigned)((args.n_free + 3) / 4);
+      hipLaunchKernelGGL((k_gather_rows<DIM, P, PROB, T>), dim3(blocks, batch), dim3(256), 0, st,
+                         pt, args.vec, args.n_free, args.indptr, args.cols, args.freed, ke,
+                         reinterpret_cast<T*>(args.vals), args.nnz, (int64_t)n_e);
+    } else {
+      const int64_t blocks = std::min<int64_t>((args.nnz * batch + 255) / 256, 256 * 64);
+      hipLaunchKernelGGL((k_gather<DIM, P, PROB, T>), dim3((unsigned)blocks), dim3(256), 0, st,
+                         pt, args.vec, args.nnz, args.rows, args.cols, args.freed, ke,
+                         reinterpret_cast<T*>(args.vals), batch, (int64_t)n_e);
+    }
     IGA_CHECK_LAUNCH();
   }
   return IGA_OK;

@@ -242,6 +242,7 @@ def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True,
         args.nnz = pattern.nnz
         args.rows = pattern.rows_d.data_ptr()
         args.cols = pattern.cols_d.data_ptr()
+        args.indptr = pattern.indptr_d.data_ptr()
         args.vals = vals.data_ptr()
         args.ke_work = ws.data_ptr()
         args.ke_work_bytes = ws.numel()
