@@ -28,9 +28,11 @@
  *   iga_inv_guard             solvers/linear.py:426-430    (Jacobi inverse, zero guard)
  *   iga_residual_vec/iga_axpy solvers/diagnostics.py:8-19 (FP64 outer residual of the
  *                             mixed-precision refinement; no reference counterpart)
- *   iga_net_* / iga_nmse / iga_adam / iga_fcg_learned
- *                             (no reference counterpart: the learned preconditioner;
- *                             the reference allows only "none"/"jacobi",
+ *   iga_net_* / iga_nmse / iga_adam / iga_mg_* / iga_masked_spgemm /
+ *   iga_stencil_* / iga_dense_inverse / iga_csr_f32 / iga_energy_loss / iga_fcg_mg
+ *                             (no reference counterpart: the learned preconditioner,
+ *                             a Galerkin multigrid with learned smoothers in flexible
+ *                             CG; the reference allows only "none"/"jacobi",
  *                             solvers/linear.py:113-116)
  *   iga_batched_cg            (no reference counterpart: the reference batches by
  *                             vmap over params, tests/test_advanced.py:337-344)
@@ -230,19 +232,23 @@ int iga_diff_norms(int32_t n, const double* y, const double* b, double* out, dou
 /* inv[i] = 1/d[i], or 1 where d[i] == 0 (Jacobi zero guard) */
 int iga_inv_guard(int32_t n, const double* d, double* inv, void* stream);
 
-/* Learned preconditioner network: a U-Net of 3x3 convolutions on the
- * control-point grid, float32 NCHW, no bias, no activation.
+/* Learned networks on the control-point grid: 3x3 convolutions, float32 NCHW.
  * kind[l]   : 0 forward convolution (weights (cout, cin, 3, 3)),
  *             1 transposed convolution + skip add (weights (cin, cout, 3, 3))
- * src[l]    : index of the input layer (-1 = network input, 1 channel)
+ * src[l]    : index of the input layer (-1 = network input, c_in channels)
  * skip[l]   : layer added to a transposed convolution's output (or -1)
  * woff[l]   : offset (floats) of the layer's weights in the parameter buffer
+ * boff[l]   : offset of the layer's bias (cout floats), or -1 for none
+ * relu[l]   : 1 applies max(0, .) after the bias (and after the skip add)
  * H/W[0]    : input size; H/W[l+1]: output size of layer l                 */
 typedef struct iga_net {
   int32_t n_layers;
   int32_t kind[16], cin[16], cout[16], stride[16], src[16], skip[16];
   int64_t woff[16];
   int32_t H[17], W[17];
+  int32_t c_in;
+  int32_t relu[16];
+  int64_t boff[16];
 } iga_net_t;
 
 /* floats of activation buffer for `batch` samples (input at offset 0, the
@@ -255,7 +261,7 @@ int iga_net_forward(const iga_net_t* net, int32_t batch, const float* params, fl
                     void* stream);
 /* backward: `gact` (same layout as act) holds d loss/d output in the output
  * slot on entry; on exit `grads` (parameter layout) and the input slot hold
- * d loss / d weights and d loss / d input */
+ * d loss / d parameters (weights and biases) and d loss / d input */
 int iga_net_backward(const iga_net_t* net, int32_t batch, const float* params,
                      const float* act, float* gact, float* grads, float* scratch, void* stream);
 /* per-sample normalised MSE mean_n ||p_n - t_n||^2 / ||t_n||^2, its gradient,
@@ -266,22 +272,79 @@ int iga_nmse(int32_t batch, int64_t per_sample, const float* pred, const float* 
 int iga_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2,
              float eps, int32_t step, void* stream);
 
-/* Flexible CG with the learned two-level preconditioner
- *   x1 = w D^-1 r, r1 = r - Kpc x1, x2 = x1 + D^-1/2 U(D^-1/2 r1), r2 = r - Kpc x2,
- *   z = x2 + w D^-1 r2
- * (U: the network, batch 1, activations in `act`; Kpc: FP32 CSR values;
- * D: the diagonal, dinv = 1/D, dsq = D^-1/2 on free dofs; cell[i]: grid cell
- * of free dof i). FCG with one stored direction on the FP64 operator `vals`;
- * stops at ||r|| <= tol ||b|| or maxiter. graph_iters > 0 captures that many
- * iterations (rounded up to even) into a hipGraph that is replayed.
- * work: iga_fcg_workspace(n) doubles.                                       */
+/* Geometric multigrid with learned smoothers (no reference counterpart).
+ * Level l has n free dofs on an h x w grid (row-major, direction 0 fastest),
+ * the FP32 operator in CSR, dsq = D^-1/2, and a smoother: per-dof stencil
+ * weights (ks*ks, n) acting as S v = D^-1/2 T_W(D^-1/2 v),
+ * (T_W u)_i = sum_o W[o][i] u[i + off(o)], or, when stencil is null, damped
+ * Jacobi omega D^-1. P (n x n_next, CSR) maps the next coarser level's
+ * coefficients to this level's; pt is its transpose. The coarsest level is
+ * solved with the dense FP32 inverse coarse_inv. One V-cycle: pre-smooth S,
+ * residual, restrict, recurse, prolong, residual, post-smooth S^T (FP32).
+ * work: iga_mg_work_floats() floats.                                       */
+typedef struct iga_mg_level {
+  int32_t n, h, w, nnz;
+  const int32_t* indptr;
+  const int32_t* cols;
+  const float* vals;
+  const float* dsq;
+  const float* stencil;
+  float omega;
+  int32_t p_nnz, pt_nnz;
+  const int32_t* p_ptr;
+  const int32_t* p_cols;
+  const float* p_vals;
+  const int32_t* pt_ptr;
+  const int32_t* pt_cols;
+  const float* pt_vals;
+} iga_mg_level_t;
+
+typedef struct iga_mg {
+  int32_t n_levels; /* 1..16 */
+  int32_t ks;       /* stencil width: 3, 5, 7 or 9 */
+  iga_mg_level_t level[16];
+  const float* coarse_inv;
+  float* work;
+} iga_mg_t;
+
+int64_t iga_mg_work_floats(const iga_mg_t* mg);
+/* x = V(b) on level `level` (FP32 vectors of that level's size) */
+int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, float* x, void* stream);
+/* C = A B on C's pattern (COO c_rows/c_cols, nnz_c entries): A in CSR, B as the
+ * CSR of B^T, sorted columns; FP64. Two calls form a Galerkin product P^T K P. */
+int iga_masked_spgemm(int64_t nnz_c, const int32_t* c_rows, const int32_t* c_cols,
+                      const int32_t* a_ptr, const int32_t* a_cols, const double* a_vals,
+                      const int32_t* bt_ptr, const int32_t* bt_cols, const double* bt_vals,
+                      double* c_vals, void* stream);
+/* feats[o][i] = K[i, i+off(o)] / K[i,i], o over (2 radius + 1)^2 grid offsets */
+int iga_stencil_features(int32_t n, int32_t w, int32_t radius, const int32_t* indptr,
+                         const int32_t* cols, const double* vals, float* feats, void* stream);
+/* FP32 inverse of a small SPD CSR matrix (n <= 4096); work: 2 n^2 doubles */
+int iga_dense_inverse(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,
+                      double* work, float* inv, void* stream);
+/* y = A x (mode 0), b - A x (1), y + A x (2) for `batch` vectors (strides in floats) */
+int iga_csr_f32(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* cols,
+                const float* vals, const float* x, const float* b, float* y, int32_t mode,
+                int32_t batch, int64_t x_stride, int64_t y_stride, void* stream);
+/* out (=|+=) D^-1/2 T (D^-1/2 v) for `batch` vectors, T = T_W or T_W^T */
+int iga_stencil_apply(int32_t h, int32_t w, int32_t ks, const float* stencil, const float* dsq,
+                      const float* v, float* out, int32_t transpose, int32_t accumulate,
+                      int32_t batch, void* stream);
+/* dW = d(sum_b g_b . y_b)/dW for y_b = D^-1/2 T_W^T (D^-1/2 v_b) */
+int iga_stencil_wgrad_t(int32_t h, int32_t w, int32_t ks, int32_t batch, const float* dsq,
+                        const float* v, const float* g, float* dW, void* stream);
+/* loss = mean_b z_b . kz_b (FP64), per-sample values, grad = -2 kz / batch */
+int iga_energy_loss(int32_t batch, int64_t n, const float* z, const float* kz, float* grad,
+                    double* per_sample, double* loss, void* stream);
+/* Flexible CG (one stored direction) on the FP64 operator with one FP32
+ * V-cycle as the preconditioner; stops at ||r|| <= tol ||b|| or maxiter.
+ * graph_iters > 0 captures that many iterations (rounded up to even) into a
+ * hipGraph that is replayed. work: iga_fcg_workspace(n) doubles; work32: 2n floats. */
 int64_t iga_fcg_workspace(int32_t n);
-int iga_fcg_learned(const iga_net_t* net, const float* params, float* act, int32_t n,
-                    const int32_t* indptr, const int32_t* cols, const double* vals,
-                    const float* vals_pc, const double* dinv, const double* dsq,
-                    const int32_t* cell, double omega, const double* b, double* x, double tol,
-                    int32_t maxiter, int32_t graph_iters, double* work, int32_t* out_iters,
-                    double* out_rr, double* out_bb, void* stream);
+int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32_t* cols,
+               const double* vals, const double* b, double* x, double tol, int32_t maxiter,
+               int32_t graph_iters, double* work, float* work32, int32_t* out_iters,
+               double* out_rr, double* out_bb, void* stream);
 
 /* Batched Jacobi-PCG: one workgroup per instance runs the whole solve in one
  * persistent launch. insts: device array of n_inst records

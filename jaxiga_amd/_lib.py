@@ -57,8 +57,24 @@ class IgaNet(ctypes.Structure):
     _fields_ = [
         ("n_layers", c_int), ("kind", c_int * 16), ("cin", c_int * 16), ("cout", c_int * 16),
         ("stride", c_int * 16), ("src", c_int * 16), ("skip", c_int * 16), ("woff", c_i64 * 16),
-        ("H", c_int * 17), ("W", c_int * 17),
+        ("H", c_int * 17), ("W", c_int * 17), ("c_in", c_int), ("relu", c_int * 16),
+        ("boff", c_i64 * 16),
     ]
+
+
+class IgaMgLevel(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("h", c_int), ("w", c_int), ("nnz", c_int), ("indptr", c_void_p),
+        ("cols", c_void_p), ("vals", c_void_p), ("dsq", c_void_p), ("stencil", c_void_p),
+        ("omega", ctypes.c_float), ("p_nnz", c_int), ("pt_nnz", c_int), ("p_ptr", c_void_p),
+        ("p_cols", c_void_p), ("p_vals", c_void_p), ("pt_ptr", c_void_p), ("pt_cols", c_void_p),
+        ("pt_vals", c_void_p),
+    ]
+
+
+class IgaMg(ctypes.Structure):
+    _fields_ = [("n_levels", c_int), ("ks", c_int), ("level", IgaMgLevel * 16),
+                ("coarse_inv", c_void_p), ("work", c_void_p)]
 
 
 _SIGNATURES = {
@@ -106,11 +122,25 @@ _SIGNATURES = {
     "iga_nmse": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_adam": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
                          ctypes.c_float, ctypes.c_float, c_int, c_void_p]),
-    "iga_fcg_workspace": (c_i64, [c_int]),
-    "iga_fcg_learned": (c_int, [ctypes.POINTER(IgaNet), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
-                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
-                                c_void_p, c_double, c_int, c_int, c_void_p, P_int, P_dbl, P_dbl,
+    "iga_mg_work_floats": (c_i64, [ctypes.POINTER(IgaMg)]),
+    "iga_mg_vcycle": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_masked_spgemm": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_stencil_features": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "iga_dense_inverse": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "iga_csr_f32": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_int, c_int, c_i64, c_i64, c_void_p]),
+    "iga_stencil_apply": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                  c_int, c_int, c_void_p]),
+    "iga_stencil_wgrad_t": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
+    "iga_energy_loss": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p]),
+    "iga_fcg_workspace": (c_i64, [c_int]),
+    "iga_fcg_mg": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_void_p, c_double, c_int, c_int, c_void_p, c_void_p, P_int, P_dbl, P_dbl,
+                           c_void_p]),
     "iga_batched_cg": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_double, c_int, c_void_p,
                                c_void_p, c_void_p]),
     "iga_rff_eval": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_double,

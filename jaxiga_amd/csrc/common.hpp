@@ -172,6 +172,20 @@ __device__ __forceinline__ double row_dot(int r, const int32_t* __restrict__ ind
   return group_sum<LPR>(s);
 }
 
+// FP32 row product (FP32 values and vector, FP32 accumulation)
+template <int LPR>
+__device__ __forceinline__ float row_dot_f(int r, const int32_t* __restrict__ indptr,
+                                           const int32_t* __restrict__ cols,
+                                           const float* __restrict__ vals,
+                                           const float* __restrict__ x, int l) {
+  const int lo = indptr[r], hi = indptr[r + 1];
+  float s = 0.f;
+  for (int k = lo + l; k < hi; k += LPR) s += vals[k] * x[cols[k]];
+#pragma unroll
+  for (int off = LPR / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LPR);
+  return s;
+}
+
 inline int pick_lpr(int n, int nnz) {
   const double avg = n > 0 ? double(nnz) / n : 0.0;
   if (avg > 96) return 64;

@@ -1,12 +1,13 @@
-// Learned preconditioner: a bias-free, activation-free convolutional U-Net on
-// the control-point grid, with hand-written FP32 forward, backward and Adam
-// kernels for gfx950, and the flexible-CG solve that applies it.
+// Convolutional networks on the control-point grid with hand-written FP32
+// forward, backward and Adam kernels for gfx950 (the smoother generator of the
+// learned multigrid preconditioner is one).
 //
 // Tensors are NCHW float32. A stride-s 3x3 convolution with zero padding 1
 // maps H -> (H-1)/s + 1 and has weights (C_out, C_in, 3, 3); the transposed
 // ("up") convolution has weights (C_in, C_out, 3, 3), an explicit output size,
 // and is the exact adjoint of the forward convolution with the same weights.
-// An up layer adds its skip tensor: out = conv_t(in) + skip.
+// An up layer adds its skip tensor: out = conv_t(in) + skip. Each layer may
+// add a per-channel bias and apply a ReLU, in that order, after the skip add.
 #include <algorithm>
 #include <type_traits>
 
@@ -202,6 +203,37 @@ __global__ void k_adam(int64_t n, float* __restrict__ w, const float* __restrict
   w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// out[n][c][p] = act(out[n][c][p] + bias[c]) over C channels of HW pixels
+__global__ void k_bias_act(int64_t total, int C, int64_t HW, const float* __restrict__ bias,
+                           int relu, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  float v = out[i];
+  if (bias) v += bias[(i / HW) % C];
+  if (relu) v = fmaxf(v, 0.f);
+  out[i] = v;
+}
+
+// g *= (a > 0): the ReLU derivative read from the stored post-activation output
+__global__ void k_relu_mask(int64_t total, const float* __restrict__ a, float* __restrict__ g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total && !(a[i] > 0.f)) g[i] = 0.f;
+}
+
+// gb[c] = sum over batch and pixels of g[n][c][p]; one block per channel
+__global__ void k_bias_grad(int B, int C, int64_t HW, const float* __restrict__ g,
+                            float* __restrict__ gb) {
+  __shared__ double scratch[16];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t t = threadIdx.x; t < (int64_t)B * HW; t += blockDim.x) {
+    const int64_t n = t / HW, p = t - n * HW;
+    s += g[(n * C + c) * HW + p];
+  }
+  s = block_sum(s, scratch);
+  if (threadIdx.x == 0) gb[c] = (float)s;
+}
+
 __global__ void k_copy_f(int64_t n, const float* __restrict__ a, float* __restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
@@ -227,9 +259,13 @@ struct NetPlan {
   int64_t total;
 };
 
+static int in_C(const iga_net_t& net, int src) {
+  return src < 0 ? (net.c_in > 0 ? net.c_in : 1) : net.cout[src];
+}
+
 static int64_t act_size(const iga_net_t& net, int l, int batch) {
-  // l = -1: network input (1 channel at H[0] x W[0]); else layer l output
-  if (l < 0) return (int64_t)batch * net.H[0] * net.W[0];
+  // l = -1: network input (c_in channels at H[0] x W[0]); else layer l output
+  if (l < 0) return (int64_t)batch * in_C(net, -1) * net.H[0] * net.W[0];
   return (int64_t)batch * net.cout[l] * net.H[l + 1] * net.W[l + 1];
 }
 
@@ -248,7 +284,6 @@ static NetPlan plan(const iga_net_t& net, int batch) {
 
 static int in_H(const iga_net_t& net, int src) { return net.H[src + 1]; }
 static int in_W(const iga_net_t& net, int src) { return net.W[src + 1]; }
-static int in_C(const iga_net_t& net, int src) { return src < 0 ? 1 : net.cout[src]; }
 
 static void launch_conv(int B, int Cin, int H, int W, int Cout, int Ho, int Wo, int s,
                         const float* in, const float* w, float* out, int acc, hipStream_t st) {
@@ -276,14 +311,18 @@ static int net_forward(const iga_net_t& net, int batch, const float* params, flo
     const int Co = net.cout[l], Ho = net.H[l + 1], Wo = net.W[l + 1], s = net.stride[l];
     const float* w = params + net.woff[l];
     if (Ci != net.cin[l]) return IGA_ERR_SHAPE;
+    const int64_t n = act_size(net, l, batch);
     if (net.kind[l] == 0) {
       launch_conv(batch, Ci, Hi, Wi, Co, Ho, Wo, s, A(src), w, A(l), 0, st);
     } else {
       const int sk = net.skip[l];
-      const int64_t n = act_size(net, l, batch);
       hipLaunchKernelGGL(k_copy_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, A(sk), A(l));
       launch_conv_t(batch, Ci, Hi, Wi, Co, Ho, Wo, s, A(src), w, A(l), 1, st);
     }
+    if (net.boff[l] >= 0 || net.relu[l])
+      hipLaunchKernelGGL(k_bias_act, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, Co,
+                         (int64_t)Ho * Wo, net.boff[l] >= 0 ? params + net.boff[l] : nullptr,
+                         net.relu[l], A(l));
     IGA_CHECK_LAUNCH();
   }
   return IGA_OK;
@@ -314,6 +353,13 @@ static int net_backward(const iga_net_t& net, int batch, const float* params, co
     const int Co = net.cout[l], Ho = net.H[l + 1], Wo = net.W[l + 1], s = net.stride[l];
     const float* w = params + net.woff[l];
     float* gw = grads + net.woff[l];
+    const int64_t nout = act_size(net, l, batch);
+    if (net.relu[l])
+      hipLaunchKernelGGL(k_relu_mask, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st, nout,
+                         A(l), G(l));
+    if (net.boff[l] >= 0)
+      hipLaunchKernelGGL(k_bias_grad, dim3(Co), dim3(256), 0, st, batch, Co, (int64_t)Ho * Wo, G(l),
+                         grads + net.boff[l]);
     if (net.kind[l] == 0) {
       // weight grad: (Co, Ci) = corr(g_out (Co, small), in (Ci, big))
       const int pairs = Co * Ci;
@@ -334,9 +380,8 @@ static int net_backward(const iga_net_t& net, int batch, const float* params, co
       hipLaunchKernelGGL(k_wgrad_sum, dim3((pairs * 9 + 255) / 256), dim3(256), 0, st, Ci, Co, ns,
                          part, gw, 0);
       launch_conv(batch, Co, Ho, Wo, Ci, Hi, Wi, s, G(l), w, G(src), 1, st);
-      const int64_t n = act_size(net, l, batch);
-      hipLaunchKernelGGL(k_add_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, G(l),
-                         G(net.skip[l]));
+      hipLaunchKernelGGL(k_add_f, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st, nout,
+                         G(l), G(net.skip[l]));
     }
     IGA_CHECK_LAUNCH();
   }
@@ -404,320 +449,4 @@ extern "C" int iga_adam(int64_t n, float* w, const float* g, float* m, float* v,
                      w, g, m, v, lr, b1, b2, eps, c1, c2);
   IGA_CHECK_LAUNCH();
   return IGA_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Flexible CG (Notay's FCG with one stored direction) with the learned
-// two-level preconditioner
-//   x1 = w D^-1 r ; r1 = r - K x1 ; x2 = x1 + D^-1/2 U(D^-1/2 r1) ; r2 = r - K x2 ;
-//   z  = x2 + w D^-1 r2
-// (U: the network on the control-point grid, K: FP32 CSR values). Every
-// scalar stays on the device; one iteration is captured into a hipGraph and
-// replayed, and the host reads the convergence flag once per replay batch.
-// ---------------------------------------------------------------------------
-
-namespace iga {
-
-struct FState {
-  double rr, bb, tol2, pq_prev;
-  int iter, maxiter, done, pad;
-};
-
-struct FBufs {
-  FState* st;
-  double* P_a;  // [kRedBlocks] z.q_prev
-  double* P_b;  // [kRedBlocks] p.r
-  double* P_c;  // [kRedBlocks] p.q
-  double* P_d;  // [kRedBlocks] r.r
-};
-
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_init(int n, const double* __restrict__ b, double* x, double* r, double* p1, double* q1,
-               FBufs F) {
-  double bb = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] = 0.0;
-    r[i] = b[i];
-    p1[i] = 0.0;  // the "previous" direction of the first iteration
-    q1[i] = 0.0;
-    bb += b[i] * b[i];
-  }
-  store_partial(F.P_d, bb);
-}
-
-__global__ void k_fcg_check(int nblk, FBufs F, int init, double tol2, int maxiter) {
-  FState* st = F.st;
-  if (!init && st->done) return;
-  const double rr = reduce_partials(F.P_d, nblk);
-  if (threadIdx.x == 0) {
-    if (init) {
-      st->bb = rr;
-      st->tol2 = tol2;
-      st->maxiter = maxiter;
-      st->iter = 0;
-      st->pq_prev = 1.0;
-    } else {
-      st->iter += 1;
-    }
-    st->rr = rr;
-    st->done = (!(rr > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
-  }
-}
-
-__global__ void k_pc_pre(int n, double omega, const double* __restrict__ dinv,
-                         const double* __restrict__ r, double* x1, const FState* st) {
-  if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x1[i] = omega * dinv[i] * r[i];
-}
-
-template <int LPR, typename TV>
-__global__ void __launch_bounds__(kVecThreads)
-    k_resid(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
-            const TV* __restrict__ vals, const double* __restrict__ r, const double* __restrict__ x,
-            double* out, const FState* st) {
-  if (st->done) return;
-  const int l = threadIdx.x % LPR;
-  const int groups = gridDim.x * (blockDim.x / LPR);
-  for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
-    const double s = row_dot<LPR, TV>(row, indptr, cols, vals, x, l);
-    if (l == 0) out[row] = r[row] - s;
-  }
-}
-
-__global__ void k_to_grid(int n, const int32_t* __restrict__ cell, const double* __restrict__ dsq,
-                          const double* __restrict__ v, float* grid, const FState* st) {
-  if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) grid[cell[i]] = (float)(dsq[i] * v[i]);
-}
-
-__global__ void k_from_grid(int n, const int32_t* __restrict__ cell, const double* __restrict__ dsq,
-                            const float* __restrict__ grid, const double* __restrict__ x1,
-                            double* x2, const FState* st) {
-  if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x2[i] = x1[i] + dsq[i] * (double)grid[cell[i]];
-}
-
-__global__ void k_pc_post(int n, double omega, const double* __restrict__ dinv,
-                          const double* __restrict__ x2, const double* __restrict__ r2, double* z,
-                          const FState* st) {
-  if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) z[i] = x2[i] + omega * dinv[i] * r2[i];
-}
-
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_zq(int n, const double* __restrict__ z, const double* __restrict__ qp, FBufs F) {
-  if (F.st->done) return;
-  double s = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    s += z[i] * qp[i];
-  store_partial(F.P_a, s);
-}
-
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_dir(int n, int nblk, const double* __restrict__ z, const double* __restrict__ pp,
-              double* p, const double* __restrict__ r, FBufs F) {
-  const FState* st = F.st;
-  if (st->done) return;
-  double beta = 0.0;
-  if (st->iter > 0) beta = -reduce_partials(F.P_a, nblk) / st->pq_prev;
-  double pr = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double pi = z[i] + beta * pp[i];
-    p[i] = pi;
-    pr += pi * r[i];
-  }
-  store_partial(F.P_b, pr);
-}
-
-template <int LPR>
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
-               const double* __restrict__ vals, const double* __restrict__ p, double* q, FBufs F) {
-  if (F.st->done) return;
-  const int l = threadIdx.x % LPR;
-  const int groups = gridDim.x * (blockDim.x / LPR);
-  double pq = 0.0;
-  for (int r = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < n; r += groups) {
-    const double s = row_dot<LPR>(r, indptr, cols, vals, p, l);
-    if (l == 0) {
-      q[r] = s;
-      pq += s * p[r];
-    }
-  }
-  store_partial(F.P_c, pq);
-}
-
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_update(int n, int nblk_c, int nblk, const double* __restrict__ p,
-                 const double* __restrict__ q, double* x, double* r, FBufs F) {
-  FState* st = F.st;
-  if (st->done) return;
-  const double pq = reduce_partials(F.P_c, nblk_c);
-  const double pr = reduce_partials(F.P_b, nblk);
-  const double alpha = pr / pq;
-  double rr = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    rr += ri * ri;
-  }
-  store_partial(F.P_d, rr);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->pq_prev = pq;
-}
-
-}  // namespace iga
-
-extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 10 * (int64_t)n + 4 * kRedBlocks; }
-
-extern "C" int iga_fcg_learned(const iga_net_t* net, const float* params, float* act, int32_t n,
-                               const int32_t* indptr, const int32_t* cols, const double* vals,
-                               const float* vals_pc, const double* dinv, const double* dsq,
-                               const int32_t* cell, double omega, const double* b, double* x,
-                               double tol, int32_t maxiter, int32_t graph_iters, double* work,
-                               int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
-  if (!net || !params || !act || !indptr || !cols || !vals || !vals_pc || !dinv || !dsq || !cell ||
-      !b || !x || !work)
-    return IGA_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
-  if (n == 0) {
-    if (out_iters) *out_iters = 0;
-    return IGA_OK;
-  }
-  FBufs F;
-  F.st = reinterpret_cast<FState*>(work);
-  double* r = work + 8;
-  double* z = r + n;
-  double* P[2] = {z + n, z + 2 * n};
-  double* Q[2] = {z + 3 * n, z + 4 * n};
-  double* x1 = z + 5 * n;
-  double* x2 = z + 6 * n;
-  double* t1 = z + 7 * n;
-  double* t2 = z + 8 * n;
-  double* parts = z + 9 * n;
-  F.P_a = parts;
-  F.P_b = parts + kRedBlocks;
-  F.P_c = parts + 2 * kRedBlocks;
-  F.P_d = parts + 3 * kRedBlocks;
-  int nnz = 0;
-  {
-    int32_t v = 0;
-    if (hipMemcpyAsync(&v, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
-      return IGA_ERR_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
-    nnz = v;
-  }
-  const int lpr = pick_lpr(n, nnz);
-  const int vb = vec_blocks(n);
-  const int rb = row_blocks(n, lpr);
-  const int eb = (n + 255) / 256;
-  const NetPlan plan_ = plan(*net, 1);
-  float* grid_in = act + plan_.in_off;
-  const float* grid_out = act + plan_.off[net->n_layers - 1];
-  {
-    // boundary cells of the network input stay zero
-    const int64_t cells = (int64_t)net->H[0] * net->W[0];
-    hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, cells,
-                       grid_in);
-  }
-  hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, r, P[1], Q[1], F);
-  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol, maxiter);
-  IGA_CHECK_LAUNCH();
-
-  auto iteration = [&](int par) -> int {
-    double* p = P[par];
-    double* pp = P[1 - par];
-    double* q = Q[par];
-    double* qp = Q[1 - par];
-    hipLaunchKernelGGL(k_pc_pre, dim3(eb), dim3(256), 0, st, n, omega, dinv, r, x1, F.st);
-    int rc = by_lpr(lpr, [&](auto L) {
-      constexpr int LPR = decltype(L)::value;
-      hipLaunchKernelGGL((k_resid<LPR, float>), dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
-                         vals_pc, r, x1, t1, F.st);
-      return 0;
-    });
-    hipLaunchKernelGGL(k_to_grid, dim3(eb), dim3(256), 0, st, n, cell, dsq, t1, grid_in, F.st);
-    rc = net_forward(*net, 1, params, act, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_from_grid, dim3(eb), dim3(256), 0, st, n, cell, dsq, grid_out, x1, x2, F.st);
-    by_lpr(lpr, [&](auto L) {
-      constexpr int LPR = decltype(L)::value;
-      hipLaunchKernelGGL((k_resid<LPR, float>), dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
-                         vals_pc, r, x2, t2, F.st);
-      return 0;
-    });
-    hipLaunchKernelGGL(k_pc_post, dim3(eb), dim3(256), 0, st, n, omega, dinv, x2, t2, z, F.st);
-    hipLaunchKernelGGL(k_fcg_zq, dim3(vb), dim3(kVecThreads), 0, st, n, z, qp, F);
-    hipLaunchKernelGGL(k_fcg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, z, pp, p, r, F);
-    by_lpr(lpr, [&](auto L) {
-      constexpr int LPR = decltype(L)::value;
-      hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols, vals,
-                         p, q, F);
-      return 0;
-    });
-    hipLaunchKernelGGL(k_fcg_update, dim3(vb), dim3(kVecThreads), 0, st, n, rb, vb, p, q, x, r, F);
-    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 0, 0.0, 0);
-    IGA_CHECK_LAUNCH();
-    return (int)IGA_OK;
-  };
-
-  const int G = graph_iters > 0 ? (graph_iters + 1) & ~1 : 0;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (G > 0) {
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
-    int rc = 0;
-    for (int k = 0; k < G && !rc; ++k) rc = iteration(k & 1);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(st, &g);
-    if (rc || e != hipSuccess) {
-      if (g) hipGraphDestroy(g);
-      return rc ? rc : IGA_ERR_HIP;
-    }
-    graph = g;
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-      hipGraphDestroy(graph);
-      return IGA_ERR_HIP;
-    }
-  }
-  FState host{};
-  int launched = 0;
-  int status = IGA_OK;
-  for (;;) {
-    if (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) != hipSuccess) {
-      status = IGA_ERR_HIP;
-      break;
-    }
-    if (hipStreamSynchronize(st) != hipSuccess) {
-      status = IGA_ERR_HIP;
-      break;
-    }
-    if (host.done) break;
-    if (exec) {
-      if (hipGraphLaunch(exec, st) != hipSuccess) {
-        status = IGA_ERR_HIP;
-        break;
-      }
-      launched += G;
-    } else {
-      for (int k = 0; k < 2; ++k, ++launched) {
-        const int rc = iteration(launched & 1);
-        if (rc) {
-          status = rc;
-          break;
-        }
-      }
-      if (status) break;
-    }
-  }
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
-  if (out_iters) *out_iters = host.iter;
-  if (out_rr) *out_rr = host.rr;
-  if (out_bb) *out_bb = host.bb;
-  return status;
 }

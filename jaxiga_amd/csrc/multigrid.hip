@@ -1,0 +1,756 @@
+// Geometric multigrid on tensor-patch IGA operators with learned stencil
+// smoothers, and the flexible CG that takes one V-cycle as its preconditioner.
+//
+// The levels come from dyadic coarsening of the knot vectors. The coarse
+// spline spaces are nested in the fine one, so the transfer P (coarse ->
+// fine coefficients) is exact and every coarse operator is the Galerkin
+// product P^T K P, formed on the device by two masked sparse products.
+//
+// On every level but the coarsest the smoother is a per-control-point stencil
+// of width ks with weights W[o][i], acting in diagonally scaled variables:
+//     S v = D^-1/2 T_W (D^-1/2 v),   (T_W u)_i = sum_o W[o][i] u[i + off(o)].
+// It runs once before the coarse correction and, transposed, once after it,
+// so the V-cycle is symmetric. The free dofs of a level form an h x w grid
+// (row-major, direction 0 fastest); the stencil runs on that grid. The
+// coarsest level is solved with a dense inverse. The V-cycle runs in FP32; the
+// outer flexible CG keeps FP64 vectors, residuals and dot products.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace iga {
+
+// ---------------------------------------------------------------------------
+// set-up kernels
+// ---------------------------------------------------------------------------
+
+// C = A B over C's pattern (COO rows/cols). A in CSR, B as the CSR of B^T;
+// both with sorted column indices. One thread per entry, sorted merge.
+__global__ void __launch_bounds__(256)
+    k_masked_spgemm(int64_t nnz, const int32_t* __restrict__ c_rows,
+                    const int32_t* __restrict__ c_cols, const int32_t* __restrict__ a_ptr,
+                    const int32_t* __restrict__ a_cols, const double* __restrict__ a_vals,
+                    const int32_t* __restrict__ b_ptr, const int32_t* __restrict__ b_cols,
+                    const double* __restrict__ b_vals, double* __restrict__ c_vals) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int i = c_rows[k], j = c_cols[k];
+    int pa = a_ptr[i], pb = b_ptr[j];
+    const int ea = a_ptr[i + 1], eb = b_ptr[j + 1];
+    double acc = 0.0;
+    while (pa < ea && pb < eb) {
+      const int ca = a_cols[pa], cb = b_cols[pb];
+      if (ca == cb) {
+        acc += a_vals[pa] * b_vals[pb];
+        ++pa;
+        ++pb;
+      } else if (ca < cb) {
+        ++pa;
+      } else {
+        ++pb;
+      }
+    }
+    c_vals[k] = acc;
+  }
+}
+
+// feats[o][i] = K[i, i + off(o)] / K[i, i] on the h x w grid, o over the
+// (2R+1)^2 offsets (feats cleared beforehand).
+__global__ void k_stencil_features(int n, int w, int R, const int32_t* __restrict__ indptr,
+                                   const int32_t* __restrict__ cols,
+                                   const double* __restrict__ vals, float* __restrict__ feats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int lo = indptr[i], hi = indptr[i + 1];
+  double d = 1.0;
+  for (int k = lo; k < hi; ++k)
+    if (cols[k] == i) d = vals[k];
+  if (d == 0.0) d = 1.0;
+  const int ri = i / w, ci = i - ri * w, B = 2 * R + 1;
+  for (int k = lo; k < hi; ++k) {
+    const int j = cols[k];
+    const int dr = j / w - ri, dc = j % w - ci;
+    if (dr < -R || dr > R || dc < -R || dc > R) continue;
+    feats[(size_t)((dr + R) * B + dc + R) * n + i] = (float)(vals[k] / d);
+  }
+}
+
+__global__ void k_zero_range(int64_t n, float* __restrict__ a) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.f;
+}
+
+// dense augmented [A | I] (n x 2n) of a CSR matrix
+__global__ void k_densify(int n, const int32_t* __restrict__ indptr,
+                          const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                          double* __restrict__ A) {
+  const int i = blockIdx.x;
+  double* row = A + (size_t)i * 2 * n;
+  for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) row[j] = j == n + i ? 1.0 : 0.0;
+  __syncthreads();
+  for (int k = indptr[i] + threadIdx.x; k < indptr[i + 1]; k += blockDim.x) row[cols[k]] = vals[k];
+}
+
+// Gauss-Jordan elimination of [A | I] without pivoting (A is SPD), one block.
+__global__ void __launch_bounds__(1024) k_gauss_jordan(int n, double* __restrict__ A) {
+  extern __shared__ double fac[];  // [n]
+  const int n2 = 2 * n;
+  for (int k = 0; k < n; ++k) {
+    const double piv = A[(size_t)k * n2 + k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) A[(size_t)k * n2 + j] /= piv;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = A[(size_t)i * n2 + k];
+    __syncthreads();
+    for (int64_t t = threadIdx.x; t < (int64_t)n * n2; t += blockDim.x) {
+      const int i = t / n2, j = t - (int64_t)i * n2;
+      if (i != k) A[t] -= fac[i] * A[(size_t)k * n2 + j];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_inverse_f32(int n, const double* __restrict__ A, float* __restrict__ inv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  const int i = t / n, j = t - (int64_t)i * n;
+  inv[t] = (float)A[(size_t)i * 2 * n + n + j];
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels (every kernel returns at once when *skip is set)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip; }
+
+// out (=|+=) D^-1/2 T (D^-1/2 v), T = T_W or its transpose; batch over blockIdx.y
+template <int KS, bool TR>
+__global__ void __launch_bounds__(256)
+    k_stencil(int h, int w, const float* __restrict__ W, const float* __restrict__ dsq,
+              const float* __restrict__ v, float* __restrict__ out, int accumulate,
+              const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  constexpr int R = KS / 2;
+  const int n = h * w;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* vb = v + (size_t)blockIdx.y * n;
+  const int r = i / w, c = i - r * w;
+  float acc = 0.f;
+#pragma unroll
+  for (int dy = -R; dy <= R; ++dy) {
+    const int rr = TR ? r - dy : r + dy;
+    if (rr < 0 || rr >= h) continue;
+#pragma unroll
+    for (int dx = -R; dx <= R; ++dx) {
+      const int cc = TR ? c - dx : c + dx;
+      if (cc < 0 || cc >= w) continue;
+      const int o = (dy + R) * KS + (dx + R);
+      const int j = rr * w + cc;
+      const float wt = W[(size_t)o * n + (TR ? j : i)];
+      acc += wt * dsq[j] * vb[j];
+    }
+  }
+  float* ob = out + (size_t)blockIdx.y * n;
+  const float val = dsq[i] * acc;
+  ob[i] = accumulate ? ob[i] + val : val;
+}
+
+// damped Jacobi (used where a level has no stencil): out (=|+=) omega D^-1 v
+__global__ void k_jacobi_f(int n, float omega, const float* __restrict__ dsq,
+                           const float* __restrict__ v, float* __restrict__ out, int accumulate,
+                           const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float val = omega * dsq[i] * dsq[i] * v[i];
+  out[i] = accumulate ? out[i] + val : val;
+}
+
+// y = A x (mode 0), y = b - A x (mode 1), y += A x (mode 2); batch over blockIdx.y
+template <int LPR>
+__global__ void __launch_bounds__(256)
+    k_csr_f(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+            const float* __restrict__ vals, const float* __restrict__ x,
+            const float* __restrict__ b, float* __restrict__ y, int mode, int64_t xs, int64_t ys,
+            const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  const float* xb = x + (size_t)blockIdx.y * xs;
+  float* yb = y + (size_t)blockIdx.y * ys;
+  const float* bb = b ? b + (size_t)blockIdx.y * ys : nullptr;
+  for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
+    const float s = row_dot_f<LPR>(row, indptr, cols, vals, xb, l);
+    if (l == 0) {
+      if (mode == 0)
+        yb[row] = s;
+      else if (mode == 1)
+        yb[row] = bb[row] - s;
+      else
+        yb[row] += s;
+    }
+  }
+}
+
+// x = M b for a dense n x n matrix (coarsest level)
+__global__ void k_dense_gemv(int n, const float* __restrict__ M, const float* __restrict__ b,
+                             float* __restrict__ x, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* row = M + (size_t)i * n;
+  float s = 0.f;
+  for (int j = 0; j < n; ++j) s += row[j] * b[j];
+  x[i] = s;
+}
+
+__global__ void k_d2f(int n, const double* __restrict__ a, float* __restrict__ b,
+                      const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_f2d(int n, const float* __restrict__ a, double* __restrict__ b,
+                      const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// training kernels
+// ---------------------------------------------------------------------------
+
+// Gradient of y = D^-1/2 T_W^T (D^-1/2 v) with respect to W, summed over the
+// batch: dW[o][j] = sum_b (D^-1/2 g_b)[j + off(o)] (D^-1/2 v_b)[j].
+template <int KS>
+__global__ void __launch_bounds__(256)
+    k_stencil_wgrad_t(int h, int w, int B, const float* __restrict__ dsq,
+                      const float* __restrict__ v, const float* __restrict__ g,
+                      float* __restrict__ dW) {
+  constexpr int R = KS / 2;
+  const int n = h * w;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int r = j / w, c = j - r * w;
+  const float dj = dsq[j];
+#pragma unroll
+  for (int dy = -R; dy <= R; ++dy) {
+#pragma unroll
+    for (int dx = -R; dx <= R; ++dx) {
+      const int o = (dy + R) * KS + (dx + R);
+      const int rr = r + dy, cc = c + dx;
+      float acc = 0.f;
+      if (rr >= 0 && rr < h && cc >= 0 && cc < w) {
+        const int i = rr * w + cc;
+        const float di = dsq[i];
+        for (int b = 0; b < B; ++b) acc += g[(size_t)b * n + i] * v[(size_t)b * n + j];
+        acc *= di * dj;
+      }
+      dW[(size_t)o * n + j] = acc;
+    }
+  }
+}
+
+// per-sample z_b . kz_b (FP64), their mean, and grad = -2 kz / B
+__global__ void k_energy_sums(int64_t n, const float* __restrict__ z, const float* __restrict__ kz,
+                              double* __restrict__ per) {
+  __shared__ double scratch[16];
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+    s += (double)z[(size_t)b * n + i] * kz[(size_t)b * n + i];
+  s = block_sum(s, scratch);
+  if (threadIdx.x == 0) per[b] = s;
+}
+
+__global__ void k_energy_grad(int B, int64_t n, const float* __restrict__ kz,
+                              const double* __restrict__ per, float* __restrict__ grad,
+                              double* loss) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    double l = 0.0;
+    for (int b = 0; b < B; ++b) l += per[b];
+    *loss = l / B;
+  }
+  if (t < (int64_t)B * n) grad[t] = -2.f * kz[t] / (float)B;
+}
+
+// ---------------------------------------------------------------------------
+// host-side V-cycle
+// ---------------------------------------------------------------------------
+
+template <typename F>
+static int by_ks(int ks, F f) {
+  switch (ks) {
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    case 7: return f(std::integral_constant<int, 7>{});
+    case 9: return f(std::integral_constant<int, 9>{});
+    default: return IGA_ERR_UNSUPPORTED;
+  }
+}
+
+static inline int blocks_for(int64_t n, int t = 256) { return (int)std::max<int64_t>(1, (n + t - 1) / t); }
+
+static int csr_f(int n, int nnz, const int32_t* ptr, const int32_t* cols, const float* vals,
+                 const float* x, const float* b, float* y, int mode, int batch, int64_t xs,
+                 int64_t ys, const int* skip, hipStream_t st) {
+  const int lpr = pick_lpr(n, nnz);
+  const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)n * lpr + 255) / 256, 4096));
+  return by_lpr(lpr, [&](auto L) {
+    constexpr int LPR = decltype(L)::value;
+    hipLaunchKernelGGL(k_csr_f<LPR>, dim3(rb, batch), dim3(256), 0, st, n, ptr, cols, vals, x, b,
+                       y, mode, xs, ys, skip);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+static int smooth(const iga_mg_t& mg, int l, const float* v, float* out, bool transpose,
+                  int accumulate, const int* skip, hipStream_t st) {
+  const iga_mg_level_t& L = mg.level[l];
+  if (!L.stencil) {
+    hipLaunchKernelGGL(k_jacobi_f, dim3(blocks_for(L.n)), dim3(256), 0, st, L.n, L.omega, L.dsq, v,
+                       out, accumulate, skip);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
+  }
+  return by_ks(mg.ks, [&](auto K) {
+    constexpr int KS = decltype(K)::value;
+    if (transpose)
+      hipLaunchKernelGGL((k_stencil<KS, true>), dim3(blocks_for(L.n)), dim3(256), 0, st, L.h, L.w,
+                         L.stencil, L.dsq, v, out, accumulate, skip);
+    else
+      hipLaunchKernelGGL((k_stencil<KS, false>), dim3(blocks_for(L.n)), dim3(256), 0, st, L.h,
+                         L.w, L.stencil, L.dsq, v, out, accumulate, skip);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+static int64_t level_offset(const iga_mg_t& mg, int l) {
+  int64_t o = 0;
+  for (int k = 0; k < l; ++k) o += 3 * (int64_t)mg.level[k].n;
+  return o;
+}
+
+// x = V(b) starting at level l0 (b, x of level l0's size)
+static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const int* skip,
+                  hipStream_t st) {
+  const int Lc = mg.n_levels - 1;
+  const float* B[16];
+  float* X[16];
+  float* Rr[16];
+  for (int l = l0; l <= Lc; ++l) {
+    float* base = mg.work + level_offset(mg, l);
+    X[l] = base;
+    B[l] = base + mg.level[l].n;
+    Rr[l] = base + 2 * mg.level[l].n;
+  }
+  X[l0] = x0;
+  B[l0] = b0;
+  int rc;
+  for (int l = l0; l < Lc; ++l) {
+    const iga_mg_level_t& L = mg.level[l];
+    if ((rc = smooth(mg, l, B[l], X[l], false, 0, skip, st))) return rc;
+    if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
+      return rc;
+    const iga_mg_level_t& C = mg.level[l + 1];
+    if ((rc = csr_f(C.n, L.pt_nnz, L.pt_ptr, L.pt_cols, L.pt_vals, Rr[l], nullptr,
+                    const_cast<float*>(B[l + 1]), 0, 1, 0, 0, skip, st)))
+      return rc;
+  }
+  {
+    const iga_mg_level_t& C = mg.level[Lc];
+    hipLaunchKernelGGL(k_dense_gemv, dim3(blocks_for(C.n)), dim3(256), 0, st, C.n, mg.coarse_inv,
+                       B[Lc], X[Lc], skip);
+    IGA_CHECK_LAUNCH();
+  }
+  for (int l = Lc - 1; l >= l0; --l) {
+    const iga_mg_level_t& L = mg.level[l];
+    if ((rc = csr_f(L.n, L.p_nnz, L.p_ptr, L.p_cols, L.p_vals, X[l + 1], nullptr, X[l], 2, 1, 0, 0,
+                    skip, st)))
+      return rc;
+    if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
+      return rc;
+    if ((rc = smooth(mg, l, Rr[l], X[l], true, 1, skip, st))) return rc;
+  }
+  return IGA_OK;
+}
+
+static int check_mg(const iga_mg_t* mg) {
+  if (!mg || mg->n_levels < 1 || mg->n_levels > 16 || !mg->work || !mg->coarse_inv)
+    return IGA_ERR_ARG;
+  for (int l = 0; l < mg->n_levels; ++l) {
+    const iga_mg_level_t& L = mg->level[l];
+    if (L.n <= 0 || (int64_t)L.h * L.w != L.n) return IGA_ERR_SHAPE;
+    if (l + 1 < mg->n_levels) {
+      if (!L.indptr || !L.cols || !L.vals || !L.dsq || !L.p_ptr || !L.p_cols || !L.p_vals ||
+          !L.pt_ptr || !L.pt_cols || !L.pt_vals)
+        return IGA_ERR_ARG;
+      if (mg->level[l + 1].n <= 0) return IGA_ERR_SHAPE;
+      if (L.stencil && (mg->ks < 3 || mg->ks > 9 || mg->ks % 2 == 0)) return IGA_ERR_UNSUPPORTED;
+    }
+  }
+  return IGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// flexible CG (Notay's FCG with one stored direction): device-resident
+// scalars; G iterations are captured into a hipGraph and replayed, and the
+// host reads the convergence flag once per replay
+// ---------------------------------------------------------------------------
+
+struct FState {
+  double rr, bb, tol2, pq_prev;
+  int iter, maxiter, done, pad;
+};
+
+struct FBufs {
+  FState* st;
+  double* P_a;  // [kRedBlocks] z.q_prev
+  double* P_b;  // [kRedBlocks] p.r
+  double* P_c;  // [kRedBlocks] p.q
+  double* P_d;  // [kRedBlocks] r.r
+};
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_fcg_init(int n, const double* __restrict__ b, double* x, double* r, double* p1, double* q1,
+               FBufs F) {
+  double bb = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    p1[i] = 0.0;  // the "previous" direction of the first iteration
+    q1[i] = 0.0;
+    bb += b[i] * b[i];
+  }
+  store_partial(F.P_d, bb);
+}
+
+__global__ void k_fcg_check(int nblk, FBufs F, int init, double tol2, int maxiter) {
+  FState* st = F.st;
+  if (!init && st->done) return;
+  const double rr = reduce_partials(F.P_d, nblk);
+  if (threadIdx.x == 0) {
+    if (init) {
+      st->bb = rr;
+      st->tol2 = tol2;
+      st->maxiter = maxiter;
+      st->iter = 0;
+      st->pq_prev = 1.0;
+    } else {
+      st->iter += 1;
+    }
+    st->rr = rr;
+    st->done = (!(rr > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_fcg_zq(int n, const double* __restrict__ z, const double* __restrict__ qp, FBufs F) {
+  if (F.st->done) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += z[i] * qp[i];
+  store_partial(F.P_a, s);
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_fcg_dir(int n, int nblk, const double* __restrict__ z, const double* __restrict__ pp,
+              double* p, const double* __restrict__ r, FBufs F) {
+  const FState* st = F.st;
+  if (st->done) return;
+  double beta = 0.0;
+  if (st->iter > 0) beta = -reduce_partials(F.P_a, nblk) / st->pq_prev;
+  double pr = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double pi = z[i] + beta * pp[i];
+    p[i] = pi;
+    pr += pi * r[i];
+  }
+  store_partial(F.P_b, pr);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(kVecThreads)
+    k_fcg_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+               const double* __restrict__ vals, const double* __restrict__ p, double* q, FBufs F) {
+  if (F.st->done) return;
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  double pq = 0.0;
+  for (int r = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < n; r += groups) {
+    const double s = row_dot<LPR>(r, indptr, cols, vals, p, l);
+    if (l == 0) {
+      q[r] = s;
+      pq += s * p[r];
+    }
+  }
+  store_partial(F.P_c, pq);
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_fcg_update(int n, int nblk_c, int nblk, const double* __restrict__ p,
+                 const double* __restrict__ q, double* x, double* r, FBufs F) {
+  FState* st = F.st;
+  if (st->done) return;
+  const double pq = reduce_partials(F.P_c, nblk_c);
+  const double pr = reduce_partials(F.P_b, nblk);
+  const double alpha = pr / pq;
+  double rr = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  store_partial(F.P_d, rr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->pq_prev = pq;
+}
+
+}  // namespace iga
+
+using namespace iga;
+
+extern "C" int iga_masked_spgemm(int64_t nnz_c, const int32_t* c_rows, const int32_t* c_cols,
+                                 const int32_t* a_ptr, const int32_t* a_cols, const double* a_vals,
+                                 const int32_t* bt_ptr, const int32_t* bt_cols,
+                                 const double* bt_vals, double* c_vals, void* stream) {
+  if (nnz_c == 0) return IGA_OK;
+  if (!c_rows || !c_cols || !a_ptr || !a_cols || !a_vals || !bt_ptr || !bt_cols || !bt_vals ||
+      !c_vals)
+    return IGA_ERR_ARG;
+  const int blocks = (int)std::min<int64_t>((nnz_c + 255) / 256, 256 * 32);
+  hipLaunchKernelGGL(k_masked_spgemm, dim3(blocks), dim3(256), 0, (hipStream_t)stream, nnz_c,
+                     c_rows, c_cols, a_ptr, a_cols, a_vals, bt_ptr, bt_cols, bt_vals, c_vals);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int iga_stencil_features(int32_t n, int32_t w, int32_t radius, const int32_t* indptr,
+                                    const int32_t* cols, const double* vals, float* feats,
+                                    void* stream) {
+  if (n == 0) return IGA_OK;
+  if (!indptr || !cols || !vals || !feats || w <= 0 || n % w || radius < 0) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t total = (int64_t)(2 * radius + 1) * (2 * radius + 1) * n;
+  hipLaunchKernelGGL(k_zero_range, dim3((int)std::min<int64_t>(blocks_for(total), 4096)), dim3(256),
+                     0, st, total, feats);
+  hipLaunchKernelGGL(k_stencil_features, dim3(blocks_for(n)), dim3(256), 0, st, n, w, radius,
+                     indptr, cols, vals, feats);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int iga_dense_inverse(int32_t n, const int32_t* indptr, const int32_t* cols,
+                                 const double* vals, double* work, float* inv, void* stream) {
+  if (n <= 0 || n > 4096 || !indptr || !cols || !vals || !work || !inv) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_densify, dim3(n), dim3(256), 0, st, n, indptr, cols, vals, work);
+  hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), n * sizeof(double), st, n, work);
+  hipLaunchKernelGGL(k_inverse_f32, dim3(blocks_for((int64_t)n * n)), dim3(256), 0, st, n, work,
+                     inv);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int64_t iga_mg_work_floats(const iga_mg_t* mg) {
+  if (!mg || mg->n_levels < 1 || mg->n_levels > 16) return -1;
+  return level_offset(*mg, mg->n_levels);
+}
+
+extern "C" int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, float* x,
+                             void* stream) {
+  int rc = check_mg(mg);
+  if (rc) return rc;
+  if (level < 0 || level >= mg->n_levels || !b || !x) return IGA_ERR_ARG;
+  return vcycle(*mg, level, b, x, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int iga_csr_f32(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* cols,
+                           const float* vals, const float* x, const float* b, float* y,
+                           int32_t mode, int32_t batch, int64_t x_stride, int64_t y_stride,
+                           void* stream) {
+  if (n == 0 || batch == 0) return IGA_OK;
+  if (!indptr || !cols || !vals || !x || !y || mode < 0 || mode > 2 || (mode == 1 && !b))
+    return IGA_ERR_ARG;
+  return csr_f(n, nnz, indptr, cols, vals, x, b, y, mode, batch, x_stride, y_stride, nullptr,
+               (hipStream_t)stream);
+}
+
+extern "C" int iga_stencil_apply(int32_t h, int32_t w, int32_t ks, const float* stencil,
+                                 const float* dsq, const float* v, float* out, int32_t transpose,
+                                 int32_t accumulate, int32_t batch, void* stream) {
+  if ((int64_t)h * w == 0 || batch == 0) return IGA_OK;
+  if (!stencil || !dsq || !v || !out || h < 0 || w < 0) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int n = h * w;
+  return by_ks(ks, [&](auto K) {
+    constexpr int KS = decltype(K)::value;
+    if (transpose)
+      hipLaunchKernelGGL((k_stencil<KS, true>), dim3(blocks_for(n), batch), dim3(256), 0, st, h, w,
+                         stencil, dsq, v, out, accumulate, nullptr);
+    else
+      hipLaunchKernelGGL((k_stencil<KS, false>), dim3(blocks_for(n), batch), dim3(256), 0, st, h,
+                         w, stencil, dsq, v, out, accumulate, nullptr);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+extern "C" int iga_stencil_wgrad_t(int32_t h, int32_t w, int32_t ks, int32_t batch,
+                                   const float* dsq, const float* v, const float* g, float* dW,
+                                   void* stream) {
+  if ((int64_t)h * w == 0) return IGA_OK;
+  if (!dsq || !v || !g || !dW || batch < 1) return IGA_ERR_ARG;
+  const int n = h * w;
+  return by_ks(ks, [&](auto K) {
+    constexpr int KS = decltype(K)::value;
+    hipLaunchKernelGGL(k_stencil_wgrad_t<KS>, dim3(blocks_for(n)), dim3(256), 0,
+                       (hipStream_t)stream, h, w, batch, dsq, v, g, dW);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+extern "C" int iga_energy_loss(int32_t batch, int64_t n, const float* z, const float* kz,
+                               float* grad, double* per_sample, double* loss, void* stream) {
+  if (batch < 1 || !z || !kz || !grad || !per_sample || !loss) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_energy_sums, dim3(batch), dim3(1024), 0, st, n, z, kz, per_sample);
+  hipLaunchKernelGGL(k_energy_grad, dim3(blocks_for((int64_t)batch * n)), dim3(256), 0, st, batch,
+                     n, kz, per_sample, grad, loss);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 6 * (int64_t)n + 4 * kRedBlocks; }
+
+extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
+                          const int32_t* cols, const double* vals, const double* b, double* x,
+                          double tol, int32_t maxiter, int32_t graph_iters, double* work,
+                          float* work32, int32_t* out_iters, double* out_rr, double* out_bb,
+                          void* stream) {
+  int rc0 = check_mg(mg);
+  if (rc0) return rc0;
+  if (!indptr || !cols || !vals || !b || !x || !work || !work32) return IGA_ERR_ARG;
+  if (mg->level[0].n != n) return IGA_ERR_SHAPE;
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) {
+    if (out_iters) *out_iters = 0;
+    return IGA_OK;
+  }
+  FBufs F;
+  F.st = reinterpret_cast<FState*>(work);
+  double* r = work + 8;
+  double* z = r + n;
+  double* P[2] = {z + n, z + 2 * n};
+  double* Q[2] = {z + 3 * n, z + 4 * n};
+  double* parts = z + 5 * n;
+  F.P_a = parts;
+  F.P_b = parts + kRedBlocks;
+  F.P_c = parts + 2 * kRedBlocks;
+  F.P_d = parts + 3 * kRedBlocks;
+  float* r32 = work32;
+  float* z32 = work32 + n;
+  int nnz = 0;
+  {
+    int32_t v = 0;
+    if (hipMemcpyAsync(&v, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
+      return IGA_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+    nnz = v;
+  }
+  const int lpr = pick_lpr(n, nnz);
+  const int vb = vec_blocks(n);
+  const int rb = row_blocks(n, lpr);
+  const int eb = blocks_for(n);
+  const int* skip = &F.st->done;
+  hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, r, P[1], Q[1], F);
+  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol, maxiter);
+  IGA_CHECK_LAUNCH();
+
+  auto iteration = [&](int par) -> int {
+    double* p = P[par];
+    double* pp = P[1 - par];
+    double* q = Q[par];
+    double* qp = Q[1 - par];
+    hipLaunchKernelGGL(k_d2f, dim3(eb), dim3(256), 0, st, n, r, r32, skip);
+    int rc = vcycle(*mg, 0, r32, z32, skip, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_f2d, dim3(eb), dim3(256), 0, st, n, z32, z, skip);
+    hipLaunchKernelGGL(k_fcg_zq, dim3(vb), dim3(kVecThreads), 0, st, n, z, qp, F);
+    hipLaunchKernelGGL(k_fcg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, z, pp, p, r, F);
+    by_lpr(lpr, [&](auto L) {
+      constexpr int LPR = decltype(L)::value;
+      hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols, vals,
+                         p, q, F);
+      return 0;
+    });
+    hipLaunchKernelGGL(k_fcg_update, dim3(vb), dim3(kVecThreads), 0, st, n, rb, vb, p, q, x, r, F);
+    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 0, 0.0, 0);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  };
+
+  const int G = graph_iters > 0 ? (graph_iters + 1) & ~1 : 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (G > 0) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
+    int rc = 0;
+    for (int k = 0; k < G && !rc; ++k) rc = iteration(k & 1);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : IGA_ERR_HIP;
+    }
+    graph = g;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return IGA_ERR_HIP;
+    }
+  }
+  FState host{};
+  int launched = 0;
+  int status = IGA_OK;
+  for (;;) {
+    if (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) != hipSuccess) {
+      status = IGA_ERR_HIP;
+      break;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      status = IGA_ERR_HIP;
+      break;
+    }
+    if (host.done) break;
+    if (exec) {
+      if (hipGraphLaunch(exec, st) != hipSuccess) {
+        status = IGA_ERR_HIP;
+        break;
+      }
+      launched += G;
+    } else {
+      for (int k = 0; k < 2; ++k, ++launched) {
+        const int rc = iteration(launched & 1);
+        if (rc) {
+          status = rc;
+          break;
+        }
+      }
+      if (status) break;
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (out_iters) *out_iters = host.iter;
+  if (out_rr) *out_rr = host.rr;
+  if (out_bb) *out_bb = host.bb;
+  return status;
+}

@@ -47,3 +47,30 @@ def gauss_point_field(ctx, seed, ell, sigma, M=4096):
     """``g`` at the interior Gauss points of a method context, ``(n_e, n_q)``."""
     omega, phase = rff_params(seed, ctx.space.dim, ell, M)
     return rff_eval(ctx.gauss_points(), omega, phase, sigma)
+
+
+def poisson_instance(refine, seed, ell=0.1, sigma=1.5, degree=3, source_ell=0.25, source_sigma=1.0):
+    """A seeded variable-coefficient Poisson problem on the unit square:
+    ``-div(a grad u) = f``, ``u = 0`` on the boundary, ``a = exp(g)`` with
+    ``g`` a GRF (``ell``, ``sigma``) sampled at the Gauss points and ``f`` a
+    smooth random source (a second GRF). The patch is the bilinear unit square
+    elevated to ``degree`` and refined to ``2**refine`` elements per side.
+    Returns ``(ctx, vals, rhs)``: the method context, FP64 CSR values and the
+    free load vector, assembled on the GPU."""
+    import jaxiga_amd as jx
+    from jaxiga_amd import methods as M
+
+    V = jx.FunctionSpace(jx.primitives.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]])
+                         .elevate(degree).refine(refine))
+    om_s, ph_s = rff_params(10_000 + seed, 2, source_ell)
+
+    def source(x, p=None):
+        return rff_eval(x.T.contiguous(), om_s, ph_s, source_sigma)
+
+    prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=lambda x: np.ones(len(x), bool))],
+                      source=source)
+    ctx = M.build_context(prob)
+    a = gauss_point_field(ctx, seed, ell, sigma)
+    a.exp_()
+    vals, rhs, _ = M.assemble(ctx, {"a0": jx.PointField(a)})
+    return ctx, vals, rhs

@@ -1,6 +1,4 @@
-"""Learned preconditioner: a convolutional U-Net on the control-point grid,
-trained on the GPU with this build's own forward/backward/Adam kernels and
-applied inside flexible CG.
+"""Learned preconditioner: multigrid with learned smoothers in flexible CG.
 
 The reference has no learned preconditioner -- ``LinearOptions`` allows only
 ``"none"`` and ``"jacobi"`` (``src/jaxiga/solvers/linear.py:113-116``) -- so
@@ -8,19 +6,28 @@ this is an addition; plain Jacobi-CG stays the fallback and the judge of
 convergence (the final FP64 residual is recomputed independently and, if the
 flexible solve has not reached the tolerance, Jacobi-CG finishes the job).
 
-Preconditioner (two-level, the network as the coarse correction):
-    x1 = w D^-1 r;  r1 = r - K x1;  x2 = x1 + D^-1/2 U(D^-1/2 r1);
-    r2 = r - K x2;  z = x2 + w D^-1 r2
-with ``D`` the operator diagonal, ``w = 1/lambda_max(D^-1 K)`` (power
-iteration), ``K`` the FP32 CSR values, and ``U`` a bias-free, activation-free
-4-level U-Net (about 180k FP32 weights): a linear, coefficient-agnostic
-multilevel filter made coefficient-aware by the diagonal scaling.
+Preconditioner: one FP32 V-cycle of the Galerkin multigrid of
+:mod:`jaxiga_amd.multigrid`, whose smoother on every level is a stencil of
+width ``ks`` per control point, ``S v = D^-1/2 T_W(D^-1/2 v)``, applied once
+before and (transposed) once after the coarse correction. The stencil weights
+``W`` come from a small convolutional network on the control-point grid -- the
+generator -- whose input is the operator's own stencil at every control point
+(``K[i, i+o] / K[i, i]``), so the smoother follows the coefficient field; the
+generator runs once per level when an operator is set up, not per iteration.
 
-Training data are residual/error pairs produced by the solver: for each
-training coefficient field, truncated Jacobi-CG solves ``u_k`` (several
-iteration counts) against the converged ``u*`` give ``e_k = u* - u_k`` and
-``r_k = b - K u_k``; the network learns ``D^-1/2 r1 -> D^1/2 e1`` (the smoothed
-pair) under a per-sample normalised squared error.
+Training (on the GPU, with this package's own forward/backward/Adam kernels):
+the solver generates residual/error pairs -- errors of truncated Jacobi-CG
+and multigrid-FCG iterates against the converged solution, carried to the
+coarser levels by restriction, and errors of damped-Jacobi sweeps from random
+starts on every level (the rough components) -- which are coarse-corrected
+(``e_c = e - P V(P^T K e)``) and normalised in the energy norm. The loss is the energy of the error left after the
+post-smoother, ``mean ||e_c - S^T K e_c||_K^2``, i.e. exactly what the
+smoother contributes to the V-cycle's error reduction. The untrained
+generator (zero-initialised weights of its last layer) yields damped Jacobi,
+which is the method without the network. The generator provides the
+smoother of the finest level; the coarser levels keep damped Jacobi (learned
+smoothers there, trained on the few points of the small grids, generalised
+worse to unseen coefficient fields and raised the iteration count).
 """
 
 from __future__ import annotations
@@ -33,54 +40,56 @@ import numpy as np
 
 from jaxiga_amd import _lib
 from jaxiga_amd import linear as LIN
+from jaxiga_amd import multigrid as MG
 from jaxiga_amd.runtime import lib, stream
 
-# (name, kind, c_in, c_out, stride, input, skip); kind 0 = conv, 1 = up (+skip)
-ARCH = (
-    ("a0", 0, 1, 8, 1, "x", None),
-    ("a1", 0, 8, 16, 2, "a0", None),
-    ("a2", 0, 16, 32, 2, "a1", None),
-    ("a3", 0, 32, 64, 2, "a2", None),
-    ("b3", 0, 64, 64, 1, "a3", None),
-    ("c3", 0, 64, 64, 1, "b3", None),
-    ("d3", 0, 64, 64, 1, "c3", None),
-    ("u2", 1, 64, 32, 2, "d3", "a2"),
-    ("b2", 0, 32, 32, 1, "u2", None),
-    ("c2", 0, 32, 32, 1, "b2", None),
-    ("u1", 1, 32, 16, 2, "c2", "a1"),
-    ("b1", 0, 16, 16, 1, "u1", None),
-    ("u0", 1, 16, 8, 2, "b1", "a0"),
-    ("y", 0, 8, 1, 1, "u0", None),
-)
+KS = 7  # smoother stencil width
+OMEGA0 = 0.5  # damped-Jacobi weight of the untrained generator
+LEARNED_LEVELS = 1  # levels (from the finest) whose smoother the generator provides
+
+
+def generator_arch(c_in, ks=KS, width=32):
+    """(name, kind, c_in, c_out, stride, input, skip, bias, relu); kind 0 = conv."""
+    return (
+        ("g0", 0, c_in, width, 1, "x", None, True, True),
+        ("g1", 0, width, width, 1, "g0", None, True, True),
+        ("w", 0, width, ks * ks, 1, "g1", None, True, False),
+    )
 
 
 def _wshape(layer):
-    _, kind, cin, cout, _, _, _ = layer
+    _, kind, cin, cout = layer[:4]
     return (cout, cin, 3, 3) if kind == 0 else (cin, cout, 3, 3)
 
 
-def n_params():
-    return int(sum(np.prod(_wshape(l)) for l in ARCH))
+class ConvNet:
+    """A convolutional network: flat FP32 parameters (each layer's weights then
+    bias), Adam state and the C descriptor per input size."""
 
-
-class UNet:
-    """Parameters (one flat FP32 buffer), Adam state and the C descriptor."""
-
-    def __init__(self, seed=0, device="cuda"):
+    def __init__(self, arch, seed=0, device="cuda", last_scale=0.0, center_bias=OMEGA0):
         import torch
 
-        self.offsets = {}
+        self.arch = tuple(arch)
+        self.c_in = self.arch[0][2]
+        self.offsets, self.bias_offsets = {}, {}
         off = 0
-        for layer in ARCH:
+        for layer in self.arch:
             self.offsets[layer[0]] = (off, _wshape(layer))
             off += int(np.prod(_wshape(layer)))
+            if layer[7]:
+                self.bias_offsets[layer[0]] = off
+                off += layer[3]
         self.n = off
         rng = np.random.default_rng(seed)
-        w = np.empty(off, np.float32)
-        for layer in ARCH:
+        w = np.zeros(off, np.float32)
+        for i, layer in enumerate(self.arch):
             o, shape = self.offsets[layer[0]]
             fan = layer[2] * 9
-            w[o : o + int(np.prod(shape))] = rng.normal(size=int(np.prod(shape))) * math.sqrt(1.0 / fan)
+            scale = math.sqrt(2.0 / fan) * (last_scale if i == len(self.arch) - 1 else 1.0)
+            w[o: o + int(np.prod(shape))] = rng.normal(size=int(np.prod(shape))) * scale
+        last = self.arch[-1]
+        if last[7] and center_bias:
+            w[self.bias_offsets[last[0]] + last[3] // 2] = center_bias
         self.params = torch.as_tensor(w, device=device)
         self.grads = torch.zeros_like(self.params)
         self.m = torch.zeros_like(self.params)
@@ -90,28 +99,39 @@ class UNet:
 
     def weights(self, name):
         o, shape = self.offsets[name]
-        return self.params[o : o + int(np.prod(shape))].view(shape)
+        return self.params[o: o + int(np.prod(shape))].view(shape)
+
+    def bias(self, name):
+        o = self.bias_offsets[name]
+        cout = [l for l in self.arch if l[0] == name][0][3]
+        return self.params[o: o + cout]
 
     def load_numpy(self, params: dict):
         import torch
 
         for name, arr in params.items():
-            self.weights(name).copy_(torch.as_tensor(np.asarray(arr, np.float32)))
+            if name.endswith(".b"):
+                self.bias(name[:-2]).copy_(torch.as_tensor(np.asarray(arr, np.float32)))
+            else:
+                self.weights(name).copy_(torch.as_tensor(np.asarray(arr, np.float32)))
 
     def struct(self, H, W):
         key = (H, W)
         if key in self._structs:
             return self._structs[key]
         s = _lib.IgaNet()
-        s.n_layers = len(ARCH)
-        names = [l[0] for l in ARCH]
+        s.n_layers = len(self.arch)
+        s.c_in = self.c_in
+        names = [l[0] for l in self.arch]
         sizes = {"x": (H, W)}
         s.H[0], s.W[0] = H, W
-        for i, (name, kind, cin, cout, stride, src, skip) in enumerate(ARCH):
+        for i, (name, kind, cin, cout, stride, src, skip, bias, relu) in enumerate(self.arch):
             s.kind[i], s.cin[i], s.cout[i], s.stride[i] = kind, cin, cout, stride
             s.src[i] = -1 if src == "x" else names.index(src)
             s.skip[i] = -1 if skip is None else names.index(skip)
             s.woff[i] = self.offsets[name][0]
+            s.boff[i] = self.bias_offsets.get(name, -1)
+            s.relu[i] = 1 if relu else 0
             h, w = sizes[src]
             if kind == 0:
                 oh, ow = (h - 1) // stride + 1, (w - 1) // stride + 1
@@ -122,158 +142,131 @@ class UNet:
         self._structs[key] = s
         return s
 
-    def buffers(self, H, W, batch, device="cuda"):
+    def forward(self, x):
+        """``x (B, c_in, H, W)`` float32 on the GPU -> ``(output view, activations)``."""
         import torch
 
+        B, C, H, W = x.shape
+        if C != self.c_in:
+            raise ValueError(f"network expects {self.c_in} input channels, got {C}")
         st = self.struct(H, W)
         L = lib()
-        na = int(L.iga_net_act_floats(st, batch))
-        act = torch.empty(na, dtype=torch.float32, device=device)
-        out_off = int(L.iga_net_act_offset(st, batch, len(ARCH) - 1))
-        return act, out_off
-
-    def forward(self, x):
-        """``x (B, 1, H, W)`` float32 on the GPU -> output of the same shape."""
-        import torch
-
-        B, _, H, W = x.shape
-        st = self.struct(H, W)
-        act, out_off = self.buffers(H, W, B, x.device)
-        act[: B * H * W].copy_(x.reshape(-1))
-        _lib.check(lib().iga_net_forward(st, B, _lib.ptr(self.params), _lib.ptr(act), stream()),
+        act = torch.empty(int(L.iga_net_act_floats(st, B)), dtype=torch.float32, device=x.device)
+        act[: x.numel()].copy_(x.reshape(-1))
+        _lib.check(L.iga_net_forward(st, B, _lib.ptr(self.params), _lib.ptr(act), stream()),
                    "iga_net_forward")
-        return act[out_off : out_off + B * H * W].view(B, 1, H, W).clone(), act
+        out_off = int(L.iga_net_act_offset(st, B, len(self.arch) - 1))
+        cout = self.arch[-1][3]
+        Ho, Wo = st.H[len(self.arch)], st.W[len(self.arch)]
+        return act[out_off: out_off + B * cout * Ho * Wo].view(B, cout, Ho, Wo), act
 
     def backward(self, act, gy, B, H, W):
-        """Gradients for an activation buffer from :meth:`forward`; returns d/dx."""
+        """Parameter gradients into ``self.grads``; returns d/d input."""
         import torch
 
         st = self.struct(H, W)
         L = lib()
         gact = torch.empty_like(act)
-        out_off = int(L.iga_net_act_offset(st, B, len(ARCH) - 1))
-        gact[out_off : out_off + B * H * W].copy_(gy.reshape(-1))
+        out_off = int(L.iga_net_act_offset(st, B, len(self.arch) - 1))
+        gact[out_off: out_off + gy.numel()].copy_(gy.reshape(-1))
         scratch = torch.empty(max(1, int(L.iga_net_scratch_floats(st, B))), dtype=torch.float32,
                               device=act.device)
         _lib.check(L.iga_net_backward(st, B, _lib.ptr(self.params), _lib.ptr(act), _lib.ptr(gact),
                                       _lib.ptr(self.grads), _lib.ptr(scratch), stream()),
                    "iga_net_backward")
-        return gact[: B * H * W].view(B, 1, H, W)
-
-    def nmse(self, pred, target):
-        """Loss and gradient of the per-sample normalised squared error."""
-        import torch
-
-        B = pred.shape[0]
-        per = pred[0].numel()
-        grad = torch.empty_like(pred)
-        sums = torch.empty(2 * B, dtype=torch.float64, device=pred.device)
-        loss = torch.empty(1, dtype=torch.float64, device=pred.device)
-        _lib.check(lib().iga_nmse(B, per, _lib.ptr(pred), _lib.ptr(target), _lib.ptr(grad),
-                                  _lib.ptr(sums), _lib.ptr(loss), stream()), "iga_nmse")
-        return loss, grad
+        return gact[: B * self.c_in * H * W].view(B, self.c_in, H, W)
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step += 1
         _lib.check(lib().iga_adam(self.n, _lib.ptr(self.params), _lib.ptr(self.grads), _lib.ptr(self.m),
                                   _lib.ptr(self.v), lr, b1, b2, eps, self.step, stream()), "iga_adam")
 
-    def train_step(self, x, t, lr):
-        B, _, H, W = x.shape
-        y, act = self.forward(x)
-        loss, gy = self.nmse(y, t)
-        self.backward(act, gy, B, H, W)
-        self.adam(lr)
-        return loss
+
+def generator(p, seed=0, ks=KS, width=32, device="cuda"):
+    """Untrained smoother generator for degree-``p`` operators (its stencils
+    are damped Jacobi until it is trained)."""
+    return ConvNet(generator_arch((2 * p + 1) ** 2, ks, width), seed=seed, device=device)
 
 
 # ---------------------------------------------------------------------------
-# per-operator data and the preconditioned solve
+# device helpers
 # ---------------------------------------------------------------------------
 
 
-@dataclasses.dataclass
-class PCData:
-    """Everything the preconditioned solve needs for one assembled operator."""
-
-    pattern: object
-    vals64: object
-    vals32: object
-    dinv: object
-    dsq: object
-    cell: object
-    omega: float
-    H: int
-    W: int
-
-
-def power_lambda_max(pattern, vals, dinv, iters=30, seed=0):
-    """Largest eigenvalue of D^-1 K (power iteration on the device operator)."""
+def features(op: MG.MGOperator, level):
+    """``((2p+1)^2, h, w)`` operator-stencil features of one level."""
     import torch
 
-    g = torch.Generator(device=vals.device).manual_seed(seed)
-    v = torch.rand(pattern.shape[0], dtype=torch.float64, device=vals.device, generator=g)
-    lam = 1.0
-    for _ in range(iters):
-        w = dinv * LIN.spmv(pattern, vals, v)
-        lam = float(w.norm() / v.norm())
-        v = w / w.norm()
-    return lam
+    lv = op.hier.levels[level]
+    R = op.hier.p
+    f = torch.empty(((2 * R + 1) ** 2, lv.n), dtype=torch.float32, device=op.vals64[0].device)
+    _lib.check(lib().iga_stencil_features(lv.n, lv.w, R, _lib.ptr(lv.indptr), _lib.ptr(lv.cols),
+                                          _lib.ptr(op.vals64[level]), _lib.ptr(f), stream()),
+               "iga_stencil_features")
+    return f.view(1, -1, lv.h, lv.w)
 
 
-def pc_data(ctx, vals64, vals32=None):
-    """Grid map, diagonal scalings and the smoothing weight of a scalar 2-D
-    operator on a tensor patch."""
+def csr_f32(n, indptr, cols, nnz, vals, x, mode=0, b=None, out=None, rows_out=None):
+    """Batched FP32 CSR product over the rows of ``x (B, n_in)``."""
     import torch
 
-    space = ctx.space
-    if space.dim != 2 or space.vec != 1:
-        raise NotImplementedError("the learned preconditioner works on scalar 2-D control grids")
-    n0, n1 = space.patch.n_cp_per_dir
-    # the control-point grid is stored first direction fastest, i.e. row-major
-    # (H, W) = (n1, n0): a scalar dof index is its own grid cell
-    cell = torch.as_tensor(ctx.dofmap.free.astype(np.int32), device=vals64.device)
-    dinv = LIN.inverse_diagonal(ctx.assembly, vals64)
-    dsq = dinv.sqrt()
-    if vals32 is None:
-        vals32 = vals64.float()
-    lam = power_lambda_max(ctx.assembly, vals64, dinv)
-    return PCData(ctx.assembly, vals64, vals32, dinv, dsq, cell, 1.0 / lam, n1, n0)
+    B = x.shape[0]
+    y = torch.empty((B, rows_out or n), dtype=torch.float32, device=x.device) if out is None else out
+    _lib.check(lib().iga_csr_f32(rows_out or n, nnz, _lib.ptr(indptr), _lib.ptr(cols), _lib.ptr(vals),
+                                 _lib.ptr(x), _lib.ptr(b), _lib.ptr(y), mode, B, x.shape[1], y.shape[1],
+                                 stream()), "iga_csr_f32")
+    return y
 
 
-def smooth_pair(d: PCData, r, e):
-    """Network input/target of a residual/error pair after pre-smoothing:
-    ``(D^-1/2 r1, D^1/2 e1)`` with ``x1 = w D^-1 r``, ``r1 = r - K x1``,
-    ``e1 = e - x1``; returned as grids ``(1, H, W)`` float32."""
+def _op_apply(op, l, x, mode=0, b=None):
+    lv = op.hier.levels[l]
+    return csr_f32(lv.n, lv.indptr, lv.cols, lv.nnz, op.vals32[l], x, mode, b)
+
+
+def _transfer(op, l, x, transpose):
+    lv = op.hier.levels[l]
+    ip, cc, _, v32 = lv.pt_dev if transpose else lv.p_dev
+    rows = op.hier.levels[l + 1].n if transpose else lv.n
+    return csr_f32(rows, ip, cc, cc.shape[0], v32, x, rows_out=rows)
+
+
+def stencil_apply(op, l, W, v, transpose=True, accumulate=False, out=None):
     import torch
 
-    x1 = d.omega * d.dinv * r
-    r1 = r - LIN.spmv(d.pattern, d.vals64, x1)
-    e1 = e - x1
-    gi = torch.zeros(d.H * d.W, dtype=torch.float32, device=r.device)
-    gt = torch.zeros_like(gi)
-    gi[d.cell.long()] = (d.dsq * r1).float()
-    gt[d.cell.long()] = (e1 / d.dsq).float()
-    return gi.view(1, d.H, d.W), gt.view(1, d.H, d.W)
+    lv = op.hier.levels[l]
+    y = torch.empty_like(v) if out is None else out
+    _lib.check(lib().iga_stencil_apply(lv.h, lv.w, op.ks, _lib.ptr(W), _lib.ptr(op.dsq[l]), _lib.ptr(v),
+                                       _lib.ptr(y), int(transpose), int(accumulate), v.shape[0], stream()),
+               "iga_stencil_apply")
+    return y
 
 
-def solver_pairs(d: PCData, b, iters=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048)):
-    """Residual/error pairs from truncated Jacobi-CG solves against the
-    converged solution."""
-    import torch
+# ---------------------------------------------------------------------------
+# set-up and solve
+# ---------------------------------------------------------------------------
 
-    work = torch.empty(int(lib().iga_cg_workspace(d.pattern.shape[0])), dtype=torch.float64,
-                       device=b.device)
-    ustar = LIN.cg_csr(d.pattern, d.vals64, b, 1e-12, dinv=d.dinv, workspace=work).x
+
+def smoother_stencils(gen: ConvNet, op: MG.MGOperator, levels=LEARNED_LEVELS):
+    """Generator output ``(ks*ks, n_l)`` for the ``levels`` finest smoothing
+    levels; None (damped Jacobi) below them."""
     out = []
-    for k in iters:
-        uk = torch.zeros_like(b) if k == 0 else LIN.cg_csr(d.pattern, d.vals64, b, 1e-14, maxiter=k,
-                                                           dinv=d.dinv, workspace=work).x
-        r = b - LIN.spmv(d.pattern, d.vals64, uk)
-        if float(r.norm()) <= 1e-13 * float(b.norm()):
-            break
-        out.append(smooth_pair(d, r, ustar - uk))
+    for l in range(op.hier.n_levels - 1):
+        if l < levels:
+            y, _ = gen.forward(features(op, l))
+            out.append(y[0].reshape(op.ks * op.ks, -1).contiguous())
+        else:
+            out.append(None)
     return out
+
+
+def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
+    """Multigrid operator with the generator's smoothers on the ``levels``
+    finest levels and damped Jacobi elsewhere (everywhere when ``gen`` is None)."""
+    hier = MG.hierarchy(ctx)
+    op = MG.MGOperator(hier, vals64, omega=OMEGA0, ks=KS)
+    if gen is not None:
+        op.set_stencils(smoother_stencils(gen, op, levels))
+    return op
 
 
 @dataclasses.dataclass
@@ -283,45 +276,175 @@ class SolveResult:
     fallback_iterations: int
     relative_residual: float
     seconds: float
+    setup_seconds: float
 
 
-def fcg_solve(net: UNet, d: PCData, b, tol=1e-10, maxiter=2000, graph_iters=8):
-    """Flexible CG with the learned preconditioner, Jacobi-CG as the fallback
-    and judge: the FP64 residual is recomputed independently at the end."""
+def solve(ctx, vals64, b, gen: ConvNet = None, tol=1e-10, maxiter=1000, graph_iters=8,
+          levels=LEARNED_LEVELS):
+    """Flexible CG with the learned multigrid preconditioner; Jacobi-CG is the
+    fallback and the independent FP64 residual the judge."""
     import torch
 
     from jaxiga_amd import precision as PR
 
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    L = lib()
-    n = d.pattern.shape[0]
-    st = net.struct(d.H, d.W)
-    act = torch.empty(int(L.iga_net_act_floats(st, 1)), dtype=torch.float32, device=b.device)
-    work = torch.empty(int(L.iga_fcg_workspace(n)), dtype=torch.float64, device=b.device)
-    x = torch.empty(n, dtype=torch.float64, device=b.device)
-    it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
-    # graph capture needs a stream other than the legacy default one
-    side = torch.cuda.Stream() if graph_iters > 0 else torch.cuda.current_stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        _lib.check(L.iga_fcg_learned(
-            st, _lib.ptr(net.params), _lib.ptr(act), n, _lib.ptr(d.pattern.indptr_d),
-            _lib.ptr(d.pattern.cols_d), _lib.ptr(d.vals64), _lib.ptr(d.vals32), _lib.ptr(d.dinv),
-            _lib.ptr(d.dsq), _lib.ptr(d.cell), float(d.omega), _lib.ptr(b), _lib.ptr(x), float(tol),
-            int(maxiter), int(graph_iters), _lib.ptr(work), _lib.ctypes.byref(it),
-            _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()), "iga_fcg_learned")
-    torch.cuda.current_stream().wait_stream(side)
-    # judge: independent FP64 residual; Jacobi-CG finishes if needed
+    op = setup(ctx, vals64, gen, levels)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    x, it, rr, bb = op.fcg(b, tol, maxiter, graph_iters)
+    pattern = ctx.assembly
     r = torch.empty_like(b)
     scratch = torch.empty(2048, dtype=torch.float64, device=b.device)
-    rn, bn = PR.csr_residual(d.pattern, d.vals64, x, b, r, scratch)
+    rn, bn = PR.csr_residual(pattern, vals64, x, b, r, scratch)
     extra = 0
-    while rn > tol * bn and extra < 10 * n:
-        res = LIN.cg_csr(d.pattern, d.vals64, r, 0.5 * tol * bn / rn, dinv=d.dinv)
+    while rn > tol * bn and extra < 10 * pattern.shape[0]:
+        res = LIN.cg_csr(pattern, vals64, r, 0.5 * tol * bn / rn, dinv=op.dinv[0])
         x += res.x
         extra += res.iterations
-        rn, bn = PR.csr_residual(d.pattern, d.vals64, x, b, r, scratch)
+        rn, bn = PR.csr_residual(pattern, vals64, x, b, r, scratch)
         if res.iterations == 0:
             break
     torch.cuda.synchronize()
-    return SolveResult(x, it.value, extra, rn / bn if bn > 0 else 0.0, time.perf_counter() - t0)
+    t2 = time.perf_counter()
+    return SolveResult(x, it, extra, rn / bn if bn > 0 else 0.0, t2 - t0, t1 - t0)
+
+
+# ---------------------------------------------------------------------------
+# training
+# ---------------------------------------------------------------------------
+
+
+def solver_errors(op: MG.MGOperator, pattern, vals64, b, jacobi_iters=(0, 1, 2, 4, 8, 16, 32, 64),
+                  mg_iters=(1, 2, 3)):
+    """Errors ``u* - u_k`` (FP64, ``(S, n)``) of truncated Jacobi-CG and
+    multigrid-FCG iterates against the converged solution."""
+    import torch
+
+    xs = op.fcg(b, 1e-13, 1000)[0]
+    out = []
+    for k in jacobi_iters:
+        xk = torch.zeros_like(b) if k == 0 else LIN.cg_csr(pattern, vals64, b, 1e-30, maxiter=k,
+                                                           dinv=op.dinv[0]).x
+        out.append(xs - xk)
+    for k in mg_iters:
+        out.append(xs - op.fcg(b, 1e-30, k, graph_iters=0)[0])
+    return torch.stack(out)
+
+
+def jacobi_errors(op: MG.MGOperator, level, n_start=4, sweeps=(0, 1, 2), seed=0):
+    """Errors of damped-Jacobi sweeps ``e <- (I - w D^-1 K) e`` on one level,
+    started from random vectors (FP32, ``(n_start * len(sweeps), n_l)``): the
+    rough components a smoother must handle."""
+    import torch
+
+    lv = op.hier.levels[level]
+    g = torch.Generator(device=op.vals32[level].device).manual_seed(seed * 7919 + level)
+    E = torch.randn((n_start, lv.n), generator=g, device=g.device, dtype=torch.float32)
+    d = op.dsq[level] * op.dsq[level]
+    out = []
+    for k in range(max(sweeps) + 1):
+        if k in sweeps:
+            out.append(E.clone())
+        E = E - OMEGA0 * d * _op_apply(op, level, E)
+    return torch.cat(out)
+
+
+def level_pairs(op: MG.MGOperator, errors64, n_random=4, seed=0, levels=LEARNED_LEVELS):
+    """Per smoothing level (the ``levels`` finest), energy-normalised
+    coarse-corrected errors and their residuals ``(E_c, K E_c)`` (FP32,
+    ``(S, n_l)``): the solver's errors carried down by restriction, plus
+    random-start Jacobi errors of the level."""
+    import torch
+
+    pairs = []
+    E = errors64.float().contiguous()
+    for l in range(min(levels, op.hier.n_levels - 1)):
+        El = torch.cat([E, jacobi_errors(op, l, n_random, seed=seed)]) if n_random else E
+        R = _op_apply(op, l, El)
+        Bc = _transfer(op, l, R, transpose=True)
+        Xc = torch.stack([op.vcycle(Bc[s].contiguous(), level=l + 1) for s in range(Bc.shape[0])])
+        Ec = El - _transfer(op, l, Xc, transpose=False)
+        KEc = _op_apply(op, l, Ec)
+        nrm = (Ec.double() * KEc.double()).sum(1).clamp_min(1e-300).sqrt().float()[:, None]
+        keep = torch.isfinite(nrm[:, 0]) & (nrm[:, 0] > 0)
+        pairs.append(((Ec / nrm)[keep].contiguous(), (KEc / nrm)[keep].contiguous()))
+        E = _transfer(op, l, E, transpose=True)
+    return pairs
+
+
+class TrainingSet:
+    """Operators, their per-level features and residual/error pairs."""
+
+    def __init__(self):
+        self.items = []  # (op, [features], [(E_c, R_c)])
+
+    def add(self, ctx, vals64, b, n_random=4, levels=LEARNED_LEVELS):
+        op = setup(ctx, vals64, None)
+        errs = solver_errors(op, ctx.assembly, vals64, b)
+        pairs = level_pairs(op, errs, n_random, seed=len(self.items), levels=levels)
+        feats = [features(op, l) for l in range(len(pairs))]
+        self.items.append((op, feats, pairs))
+
+    @property
+    def n_pairs(self):
+        return sum(E.shape[0] for _, _, pairs in self.items for E, _ in pairs)
+
+
+def energy_loss(op, l, W, Ec, Rc):
+    """Loss ``mean ||E_c - S^T R_c||_K^2`` of one level and its gradient with
+    respect to the stencil weights ``W (ks*ks, n)``."""
+    import torch
+
+    lv = op.hier.levels[l]
+    L = lib()
+    y = stencil_apply(op, l, W, Rc, transpose=True)
+    Z = Ec - y
+    KZ = _op_apply(op, l, Z)
+    B = Z.shape[0]
+    gy = torch.empty_like(Z)
+    per = torch.empty(B, dtype=torch.float64, device=Z.device)
+    loss = torch.empty(1, dtype=torch.float64, device=Z.device)
+    _lib.check(L.iga_energy_loss(B, lv.n, _lib.ptr(Z), _lib.ptr(KZ), _lib.ptr(gy), _lib.ptr(per),
+                                 _lib.ptr(loss), stream()), "iga_energy_loss")
+    # gy = d loss / d y (y = S^T R_c enters the loss through Z = E_c - y)
+    dW = torch.empty_like(W)
+    _lib.check(L.iga_stencil_wgrad_t(lv.h, lv.w, op.ks, B, _lib.ptr(op.dsq[l]), _lib.ptr(Rc), _lib.ptr(gy),
+                                     _lib.ptr(dW), stream()), "iga_stencil_wgrad_t")
+    return loss, dW
+
+
+def train(gen: ConvNet, data: TrainingSet, steps=200, lr=1e-3, batch=None, seed=0, log_every=0):
+    """Adam on the mean smoothing loss over (operator, level) pairs of the
+    training set -- all operators per step, or ``batch`` of them drawn at
+    random; returns the loss history (host floats)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    history = []
+    total = torch.zeros_like(gen.grads)
+    n_items = len(data.items)
+    for step in range(steps):
+        total.zero_()
+        losses = []
+        count = 0
+        chosen = range(n_items) if not batch or batch >= n_items else rng.choice(n_items, batch, replace=False)
+        for i in chosen:
+            op, feats, pairs = data.items[i]
+            for l, (Ec, Rc) in enumerate(pairs):
+                if Ec.shape[0] == 0:
+                    continue
+                W4, act = gen.forward(feats[l])
+                W = W4[0].reshape(op.ks * op.ks, -1)
+                loss, dW = energy_loss(op, l, W, Ec, Rc)
+                lv = op.hier.levels[l]
+                gen.backward(act, dW, 1, lv.h, lv.w)
+                total += gen.grads
+                losses.append(loss)
+                count += 1
+        gen.grads.copy_(total / max(count, 1))
+        gen.adam(lr)
+        history.append(float(torch.cat(losses).mean()))
+        if log_every and step % log_every == 0:
+            print(f"[train] step {step} loss {history[-1]:.4f}", flush=True)
+    return history

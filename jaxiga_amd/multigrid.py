@@ -1,0 +1,296 @@
+"""Geometric multigrid for scalar 2-D tensor-patch operators.
+
+The reference has no multigrid (its preconditioners are ``"none"`` and
+``"jacobi"``, ``src/jaxiga/solvers/linear.py:113-116``); this is the backbone
+of the learned preconditioner.
+
+Levels come from dyadic coarsening of the knot vectors: every other distinct
+interior knot is removed, so each coarse spline space is nested in the finer
+one and the transfer ``P`` (coarse -> fine coefficients, the knot-insertion
+matrix) is exact. The coarse operators are the Galerkin products
+``P^T K P``, formed on the device by two masked sparse products over
+patterns computed once per space on the host (like the reference's cached CSR
+pattern, ``src/jaxiga/methods/_common.py:181-264``). The prescribed dofs must
+be whole sides of the patch, so the free dofs of every level form a
+rectangle of the control grid -- the ``h x w`` grid the stencil smoothers
+run on (row-major, direction 0 fastest). The coarsest level is solved with a
+dense inverse.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import scipy.sparse as sp
+
+from jaxiga_amd import _lib
+from jaxiga_amd.geometry import change_of_basis
+from jaxiga_amd.runtime import lib, stream
+
+
+def coarsen_knots(knots, p):
+    """Remove every other distinct interior knot (with its multiplicity);
+    ``None`` when the number of spans is odd or below 4."""
+    t = np.asarray(knots, float)
+    u, mult = np.unique(t, return_counts=True)
+    m = len(u) - 1
+    if m % 2 or m < 4:
+        return None
+    keep = np.ones(len(u), bool)
+    keep[1:-1:2] = False
+    return np.repeat(u[keep], mult[keep])
+
+
+def free_rectangle(n_cp, prescribed):
+    """``(lo, hi)`` per direction of the free control points when the
+    prescribed set is a union of whole sides; raises otherwise."""
+    n0, n1 = n_cp
+    pres = np.zeros(n0 * n1, bool)
+    pres[np.asarray(prescribed, np.int64)] = True
+    grid = pres.reshape(n1, n0)
+    fixed = {(0, 0): grid[:, 0].all(), (0, 1): grid[:, -1].all(),
+             (1, 0): grid[0, :].all(), (1, 1): grid[-1, :].all()}
+    lo = [1 if fixed[(d, 0)] else 0 for d in range(2)]
+    hi = [n_cp[d] - 2 if fixed[(d, 1)] else n_cp[d] - 1 for d in range(2)]
+    rect = np.zeros_like(grid)
+    rect[lo[1]: hi[1] + 1, lo[0]: hi[0] + 1] = True
+    if not np.array_equal(~grid, rect):
+        raise NotImplementedError("multigrid needs the prescribed dofs to be whole sides of the patch")
+    return lo, hi, fixed
+
+
+def _csr_dev(A, dev):
+    import torch
+
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    i32 = dict(dtype=torch.int32, device=dev)
+    indptr = torch.as_tensor(A.indptr.astype(np.int32), **i32)
+    cols = torch.as_tensor(A.indices.astype(np.int32), **i32)
+    rows = torch.as_tensor(np.repeat(np.arange(A.shape[0], dtype=np.int32), np.diff(A.indptr)), **i32)
+    return indptr, cols, rows
+
+
+@dataclasses.dataclass
+class LevelStructure:
+    """Static structure of one level (and its transfer to the next coarser)."""
+
+    n: int
+    h: int
+    w: int
+    knots: tuple
+    indptr: object
+    cols: object
+    rows: object
+    P: object = None  # host CSR (n x n_coarse)
+    p_dev: tuple = None  # (indptr, cols, vals64, vals32)
+    pt_dev: tuple = None
+    t_pat: tuple = None  # pattern of P^T K (n_coarse x n): (indptr, cols, rows)
+
+    @property
+    def nnz(self):
+        return int(self.cols.shape[0])
+
+
+class Hierarchy:
+    """Levels, transfers and Galerkin patterns of one scalar 2-D space and
+    free set (cached on the space)."""
+
+    def __init__(self, ctx, max_coarse=64, max_levels=16):
+        space = ctx.space
+        if space.dim != 2 or space.vec != 1:
+            raise NotImplementedError("multigrid works on scalar 2-D tensor patches")
+        if len(set(space.degree)) != 1:
+            raise NotImplementedError("multigrid needs one degree in both directions")
+        self.p = p = space.degree[0]
+        dev = ctx.f2f_d.device
+        lo, hi, fixed = free_rectangle(space.patch.n_cp_per_dir, ctx.dofmap.prescribed)
+        knots = tuple(np.asarray(k, float) for k in space.patch.knot_arrays())
+        pat = ctx.assembly
+        h, w = hi[1] - lo[1] + 1, hi[0] - lo[0] + 1
+        levels = [LevelStructure(pat.shape[0], h, w, knots, pat.indptr_d, pat.cols_d, pat.rows_d)]
+        if h * w != pat.shape[0]:
+            raise NotImplementedError("free dofs do not form the expected grid")
+        K = sp.csr_matrix((np.ones(pat.nnz), pat.column_indices, pat.indptr), shape=pat.shape)
+        while len(levels) < max_levels and levels[-1].n > max_coarse:
+            fine = levels[-1]
+            ck = tuple(coarsen_knots(t, p) for t in fine.knots)
+            if any(c is None for c in ck):
+                break
+            P1 = []
+            for d in range(2):
+                T = change_of_basis(ck[d], p, fine.knots[d], p)  # fine coefs = T @ coarse coefs
+                T[np.abs(T) < 1e-14] = 0.0
+                nf, nc = T.shape
+                rlo, rhi = (1 if fixed[(d, 0)] else 0), (nf - 2 if fixed[(d, 1)] else nf - 1)
+                clo, chi = (1 if fixed[(d, 0)] else 0), (nc - 2 if fixed[(d, 1)] else nc - 1)
+                P1.append(T[rlo: rhi + 1, clo: chi + 1])
+            P = sp.csr_matrix(sp.kron(sp.csr_matrix(P1[1]), sp.csr_matrix(P1[0])))
+            P.eliminate_zeros()
+            P.sort_indices()
+            PT = sp.csr_matrix(P.T)
+            PT.sort_indices()
+            Tpat = sp.csr_matrix(PT @ K)
+            Tpat.sort_indices()
+            Kc = sp.csr_matrix(Tpat @ P)
+            Kc.data[:] = 1.0
+            Kc.sort_indices()
+            fine.P = P
+            fine.p_dev = self._transfer_dev(P, dev)
+            fine.pt_dev = self._transfer_dev(PT, dev)
+            fine.t_pat = _csr_dev(Tpat, dev)
+            hc, wc = P1[1].shape[1], P1[0].shape[1]
+            ip, cc, rr = _csr_dev(Kc, dev)
+            levels.append(LevelStructure(Kc.shape[0], hc, wc, ck, ip, cc, rr))
+            K = Kc
+        self.levels = levels
+
+    @staticmethod
+    def _transfer_dev(M, dev):
+        import torch
+
+        ip, cc, _ = _csr_dev(M, dev)
+        v64 = torch.as_tensor(M.data, dtype=torch.float64, device=dev)
+        return ip, cc, v64, v64.float()
+
+    @property
+    def n_levels(self):
+        return len(self.levels)
+
+
+def hierarchy(ctx, max_coarse=64):
+    """Cached :class:`Hierarchy` of a method context's space and free set."""
+    key = ("mg", ctx.dofmap.free.tobytes(), max_coarse)
+    cache = ctx.space._patterns
+    if key not in cache:
+        cache[key] = Hierarchy(ctx, max_coarse)
+    return cache[key]
+
+
+def galerkin(hier: Hierarchy, vals64):
+    """FP64 operator values of every level: ``K_{l+1} = P_l^T K_l P_l``."""
+    import torch
+
+    L = lib()
+    st = stream()
+    out = [vals64]
+    for l in range(hier.n_levels - 1):
+        fine, coarse = hier.levels[l], hier.levels[l + 1]
+        tp_ptr, tp_cols, tp_rows = fine.t_pat
+        pt_ptr, pt_cols, pt_v64, _ = fine.pt_dev
+        tvals = torch.empty(tp_cols.shape[0], dtype=torch.float64, device=vals64.device)
+        # T = P^T K (K symmetric: its CSR is the CSR of K^T)
+        _lib.check(L.iga_masked_spgemm(tvals.shape[0], _lib.ptr(tp_rows), _lib.ptr(tp_cols),
+                                       _lib.ptr(pt_ptr), _lib.ptr(pt_cols), _lib.ptr(pt_v64),
+                                       _lib.ptr(fine.indptr), _lib.ptr(fine.cols), _lib.ptr(out[-1]),
+                                       _lib.ptr(tvals), st), "iga_masked_spgemm")
+        kc = torch.empty(coarse.nnz, dtype=torch.float64, device=vals64.device)
+        # K_c = T P (the CSR of P^T is the CSR of (P)^T)
+        _lib.check(L.iga_masked_spgemm(kc.shape[0], _lib.ptr(coarse.rows), _lib.ptr(coarse.cols),
+                                       _lib.ptr(tp_ptr), _lib.ptr(tp_cols), _lib.ptr(tvals),
+                                       _lib.ptr(pt_ptr), _lib.ptr(pt_cols), _lib.ptr(pt_v64),
+                                       _lib.ptr(kc), st), "iga_masked_spgemm")
+        out.append(kc)
+    return out
+
+
+class MGOperator:
+    """Numeric multigrid for one operator: Galerkin levels in FP64 and FP32,
+    diagonal scalings, the coarsest inverse and the smoothers (stencils per
+    level, or damped Jacobi with ``omega`` where a level has none)."""
+
+    def __init__(self, hier: Hierarchy, vals64, stencils=None, omega=0.5, ks=7):
+        import torch
+
+        from jaxiga_amd import linear as LIN
+
+        self.hier = hier
+        self.ks = ks
+        self.omega = float(omega)
+        dev = vals64.device
+        self.vals64 = galerkin(hier, vals64)
+        self.vals32 = [v.float() for v in self.vals64]
+        self.dinv = []
+        self.dsq = []
+        for lv, v in zip(hier.levels, self.vals64):
+            pat = LIN.CSRPattern(lv.indptr, lv.cols, lv.rows, lv.n)
+            d = LIN.inverse_diagonal(pat, v)
+            self.dinv.append(d)
+            self.dsq.append(d.sqrt().float())
+        c = hier.levels[-1]
+        work = torch.empty(2 * c.n * c.n, dtype=torch.float64, device=dev)
+        self.coarse_inv = torch.empty(c.n * c.n, dtype=torch.float32, device=dev)
+        _lib.check(lib().iga_dense_inverse(c.n, _lib.ptr(c.indptr), _lib.ptr(c.cols),
+                                           _lib.ptr(self.vals64[-1]), _lib.ptr(work),
+                                           _lib.ptr(self.coarse_inv), stream()), "iga_dense_inverse")
+        self.stencils = [None] * hier.n_levels
+        if stencils is not None:
+            self.set_stencils(stencils)
+        self._work = None
+        self._struct = None
+
+    def set_stencils(self, stencils):
+        for l, s in enumerate(stencils):
+            self.stencils[l] = s
+        self._struct = None
+
+    def struct(self):
+        import torch
+
+        if self._struct is not None:
+            return self._struct
+        h = self.hier
+        m = _lib.IgaMg()
+        m.n_levels = h.n_levels
+        m.ks = self.ks
+        for l, lv in enumerate(h.levels):
+            L = m.level[l]
+            L.n, L.h, L.w, L.nnz = lv.n, lv.h, lv.w, lv.nnz
+            L.indptr, L.cols = lv.indptr.data_ptr(), lv.cols.data_ptr()
+            L.vals, L.dsq = self.vals32[l].data_ptr(), self.dsq[l].data_ptr()
+            L.stencil = self.stencils[l].data_ptr() if self.stencils[l] is not None else None
+            L.omega = self.omega
+            if lv.p_dev is not None:
+                ip, cc, _, v32 = lv.p_dev
+                L.p_nnz, L.p_ptr, L.p_cols, L.p_vals = cc.shape[0], ip.data_ptr(), cc.data_ptr(), v32.data_ptr()
+                ip, cc, _, v32 = lv.pt_dev
+                L.pt_nnz, L.pt_ptr, L.pt_cols, L.pt_vals = cc.shape[0], ip.data_ptr(), cc.data_ptr(), v32.data_ptr()
+        m.coarse_inv = self.coarse_inv.data_ptr()
+        if self._work is None:
+            nw = int(lib().iga_mg_work_floats(m))
+            self._work = torch.empty(max(nw, 1), dtype=torch.float32, device=self.coarse_inv.device)
+        m.work = self._work.data_ptr()
+        self._struct = m
+        return m
+
+    def vcycle(self, b32, level=0, out=None):
+        import torch
+
+        x = torch.empty_like(b32) if out is None else out
+        _lib.check(lib().iga_mg_vcycle(self.struct(), level, _lib.ptr(b32), _lib.ptr(x), stream()),
+                   "iga_mg_vcycle")
+        return x
+
+    def fcg(self, b, tol=1e-10, maxiter=1000, graph_iters=8):
+        """Flexible CG on the level-0 FP64 operator with one V-cycle per
+        iteration; returns ``(x, iterations, rr, bb)``."""
+        import torch
+
+        L = lib()
+        lv = self.hier.levels[0]
+        n = lv.n
+        work = torch.empty(int(L.iga_fcg_workspace(n)), dtype=torch.float64, device=b.device)
+        work32 = torch.empty(2 * n, dtype=torch.float32, device=b.device)
+        x = torch.empty(n, dtype=torch.float64, device=b.device)
+        it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
+        m = self.struct()
+        side = torch.cuda.Stream() if graph_iters > 0 else torch.cuda.current_stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # graph capture needs a non-default stream
+            _lib.check(L.iga_fcg_mg(m, n, _lib.ptr(lv.indptr), _lib.ptr(lv.cols), _lib.ptr(self.vals64[0]),
+                                    _lib.ptr(b), _lib.ptr(x), float(tol), int(maxiter), int(graph_iters),
+                                    _lib.ptr(work), _lib.ptr(work32), _lib.ctypes.byref(it),
+                                    _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()), "iga_fcg_mg")
+        torch.cuda.current_stream().wait_stream(side)
+        return x, it.value, rr.value, bb.value

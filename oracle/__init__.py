@@ -19,6 +19,8 @@ assembly   CSR pattern, scatter-add assembly, load vector
 solver     SciPy-faithful Jacobi-CG, textbook PCG, diagnostics, error norms
 pipeline   end-to-end ``solve`` for the configurations used by the tests
 grf        seeded random-Fourier-feature Gaussian random fields (inputs)
+learned    networks of the learned preconditioner (forward, exact gradients)
+multigrid  knot coarsening, Galerkin levels, stencil smoothers, V-cycle, FCG
 
 Pinning status of every function is listed in ``PINNING`` below; each entry
 names the test module that pins it against closed forms, SciPy, or brute
@@ -42,4 +44,10 @@ PINNING = {
     "solver.errornorm": "tests/test_oracle_solver.py (||sin sin||_L2 = 1/2, convergence rates)",
     "pipeline.solve": "tests/test_oracle_pipeline.py (patch test, convergence rate, Lame annulus)",
     "grf.rff_field": "tests/test_oracle_grf.py (sample covariance vs squared-exponential kernel)",
+    "learned.forward/backward": "tests/test_oracle_learned.py (torch-CPU autograd, finite differences)",
+    "multigrid.refinement_matrix": "tests/test_oracle_multigrid.py (coarse B-splines reproduced exactly)",
+    "multigrid.hierarchy": "tests/test_oracle_multigrid.py (Galerkin = coarse stiffness assembled directly)",
+    "multigrid.stencil_apply/wgrad_t": "tests/test_oracle_multigrid.py (adjointness, finite differences, dense stencil)",
+    "multigrid.stencil_features": "tests/test_oracle_multigrid.py (dense-matrix restatement)",
+    "multigrid.vcycle/fcg": "tests/test_oracle_multigrid.py (symmetry, exact two-level solve, dense solve)",
 }

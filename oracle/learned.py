@@ -1,51 +1,59 @@
-"""Oracle for the learned preconditioner network (test infrastructure only).
+"""Oracle for the learned-preconditioner networks (test infrastructure only).
 
 The reference has no learned preconditioner (``src/jaxiga/solvers/linear.py:
-113-116`` allows only ``"none"`` and ``"jacobi"``); the network is defined by
+113-116`` allows only ``"none"`` and ``"jacobi"``); the networks are defined by
 this project. This module is the CPU FP64 statement of that definition --
-forward pass and exact gradients of a 4-level, bias-free, activation-free
-convolutional U-Net on the control-point grid -- against which the GPU
-forward/backward kernels are checked. It is itself pinned by torch-CPU
-``conv2d``/``conv_transpose2d`` autograd and by finite differences
+forward pass and exact gradients of networks of 3x3 convolutions with optional
+transposed-convolution + skip layers, per-channel biases and ReLUs -- against
+which the GPU forward/backward kernels are checked. It is itself pinned by
+torch-CPU ``conv2d``/``conv_transpose2d`` autograd and by finite differences
 (``tests/test_oracle_learned.py``).
 
 Conventions (shared with ``jaxiga_amd.learned``): tensors are NCHW; a
 stride-``s`` 3x3 convolution with zero padding 1 maps ``H -> (H - 1)//s + 1``
 and has weights ``(C_out, C_in, 3, 3)``; the transposed convolution has
 weights ``(C_in, C_out, 3, 3)`` and an explicit output size, and is the exact
-adjoint of the forward convolution with the same weights.
+adjoint of the forward convolution with the same weights. A layer computes
+``act(conv(x) [+ skip] + bias)``. Layers are
+``(name, kind, c_in, c_out, stride, input, skip, bias, relu)``; parameters
+are a dict of weights ``name`` and biases ``name + ".b"``.
 """
 
 from __future__ import annotations
 
 import numpy as np
 
-# (name, kind, c_in, c_out, stride, input, skip) ; kind: "conv" | "up"
-ARCH = (
-    ("a0", "conv", 1, 8, 1, "x", None),
-    ("a1", "conv", 8, 16, 2, "a0", None),
-    ("a2", "conv", 16, 32, 2, "a1", None),
-    ("a3", "conv", 32, 64, 2, "a2", None),
-    ("b3", "conv", 64, 64, 1, "a3", None),
-    ("c3", "conv", 64, 64, 1, "b3", None),
-    ("d3", "conv", 64, 64, 1, "c3", None),
-    ("u2", "up", 64, 32, 2, "d3", "a2"),
-    ("b2", "conv", 32, 32, 1, "u2", None),
-    ("c2", "conv", 32, 32, 1, "b2", None),
-    ("u1", "up", 32, 16, 2, "c2", "a1"),
-    ("b1", "conv", 16, 16, 1, "u1", None),
-    ("u0", "up", 16, 8, 2, "b1", "a0"),
-    ("y", "conv", 8, 1, 1, "u0", None),
+
+def generator_arch(c_in, ks=7, width=32):
+    """The smoother generator: operator stencil features -> ks*ks stencil weights."""
+    return (
+        ("g0", "conv", c_in, width, 1, "x", None, True, True),
+        ("g1", "conv", width, width, 1, "g0", None, True, True),
+        ("w", "conv", width, ks * ks, 1, "g1", None, True, False),
+    )
+
+
+# a 4-level U-Net exercising strided and transposed layers, skips, biases and ReLUs
+UNET_ARCH = (
+    ("a0", "conv", 2, 8, 1, "x", None, True, True),
+    ("a1", "conv", 8, 16, 2, "a0", None, False, True),
+    ("a2", "conv", 16, 16, 2, "a1", None, True, False),
+    ("a3", "conv", 16, 16, 2, "a2", None, True, True),
+    ("b3", "conv", 16, 16, 1, "a3", None, False, False),
+    ("u2", "up", 16, 16, 2, "b3", "a2", True, True),
+    ("u1", "up", 16, 16, 2, "u2", "a1", False, True),
+    ("u0", "up", 16, 8, 2, "u1", "a0", True, False),
+    ("y", "conv", 8, 3, 1, "u0", None, True, False),
 )
 
 
 def weight_shape(layer):
-    _, kind, cin, cout, _, _, _ = layer
+    _, kind, cin, cout = layer[:4]
     return (cout, cin, 3, 3) if kind == "conv" else (cin, cout, 3, 3)
 
 
-def n_params():
-    return int(sum(np.prod(weight_shape(l)) for l in ARCH))
+def n_params(arch):
+    return int(sum(np.prod(weight_shape(l)) + (l[3] if l[7] else 0) for l in arch))
 
 
 def out_size(h, s):
@@ -92,28 +100,34 @@ def conv_wgrad(x, gy, s):
     return g
 
 
-def forward(params, x):
-    """Returns the output and the activations of every layer."""
+def forward(arch, params, x):
+    """Returns the output and the (post-activation) output of every layer."""
     act = {"x": x}
-    for layer in ARCH:
-        name, kind, cin, cout, s, src, skip = layer
+    for name, kind, cin, cout, s, src, skip, bias, relu in arch:
         inp = act[src]
         if kind == "conv":
             out = conv(inp, params[name], s)
         else:
             H, W = act[skip].shape[2:]
             out = conv_t(inp, params[name], s, H, W) + act[skip]
+        if bias:
+            out = out + params[name + ".b"][None, :, None, None]
+        if relu:
+            out = np.maximum(out, 0.0)
         act[name] = out
-    return act["y"], act
+    return act[arch[-1][0]], act
 
 
-def backward(params, act, gy):
-    """Gradients w.r.t. every weight and the input, given d loss / d y."""
+def backward(arch, params, act, gy):
+    """Gradients w.r.t. every parameter and the input, given d loss / d output."""
     grads = {}
-    gact = {"y": gy}
-    for layer in reversed(ARCH):
-        name, kind, cin, cout, s, src, skip = layer
+    gact = {arch[-1][0]: gy}
+    for name, kind, cin, cout, s, src, skip, bias, relu in reversed(arch):
         g = gact.pop(name)
+        if relu:
+            g = g * (act[name] > 0)
+        if bias:
+            grads[name + ".b"] = g.sum((0, 2, 3))
         inp = act[src]
         if kind == "conv":
             grads[name] = conv_wgrad(inp, g, s)
@@ -128,21 +142,12 @@ def backward(params, act, gy):
     return grads, gact["x"]
 
 
-def init_params(seed=0):
+def init_params(arch, seed=0):
     rng = np.random.default_rng(seed)
     params = {}
-    for layer in ARCH:
+    for layer in arch:
         shape = weight_shape(layer)
-        fan = layer[2] * 9
-        params[layer[0]] = rng.normal(size=shape) * np.sqrt(1.0 / fan)
+        params[layer[0]] = rng.normal(size=shape) * np.sqrt(2.0 / (layer[2] * 9))
+        if layer[7]:
+            params[layer[0] + ".b"] = rng.normal(size=layer[3]) * 0.1
     return params
-
-
-def normalized_mse(pred, target):
-    """Per-sample ||pred - target||^2 / ||target||^2, averaged; and its gradient."""
-    B = pred.shape[0]
-    d = pred - target
-    den = (target**2).reshape(B, -1).sum(1)
-    loss = float(((d**2).reshape(B, -1).sum(1) / den).mean())
-    grad = 2.0 * d / den[:, None, None, None] / B
-    return loss, grad

@@ -1,19 +1,32 @@
-"""Learned preconditioner: GPU forward/backward kernels against the CPU oracle
-(FP32 kernels vs FP64 oracle: forward within 1e-5, gradients within 1e-4
-relative to each tensor's largest entry), a short training run that reduces
-the loss, and the trained network inside flexible CG beating the untrained
-one and Jacobi-CG in iterations while reaching the FP64 tolerance."""
+"""Learned multigrid preconditioner on the GPU against the CPU oracle.
+
+* network forward/backward kernels (biases, ReLUs, strided and transposed
+  layers, skips) vs the FP64 oracle: FP32 kernels within 1e-5 (forward) and
+  1e-4 (gradients) relative to each tensor's largest entry;
+* Galerkin levels (FP64, 1e-12), transfers, operator-stencil features and
+  the coarsest inverse vs the oracle hierarchy built by Boehm insertion;
+* stencil smoother, its transpose, its weight gradient and the energy loss
+  vs the oracle (FP32, 1e-5);
+* one FP32 V-cycle vs the FP64 oracle V-cycle (1e-4);
+* multigrid-FCG reaches 1e-10 in FP64 and matches Jacobi-CG;
+* training on solver-generated pairs lowers the loss, and the trained
+  smoothers need fewer FCG iterations than the untrained ones (damped Jacobi)
+  and than Jacobi-CG, with no fallback iterations.
+"""
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import torch
 
 import jaxiga_amd as jx
-from jaxiga_amd import _lib
+from jaxiga_amd import grf
 from jaxiga_amd import learned as LN
 from jaxiga_amd import linear as LIN
 from jaxiga_amd import methods as M
+from jaxiga_amd import multigrid as MGP
 from oracle import learned as OL
+from oracle import multigrid as OM
 
 pytestmark = pytest.mark.gpu
 
@@ -22,76 +35,155 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def test_architecture_matches_the_oracle():
-    assert [(l[0], l[2], l[3], l[4]) for l in LN.ARCH] == [(l[0], l[2], l[3], l[4]) for l in OL.ARCH]
-    assert LN.n_params() == OL.n_params()
+def _product_arch(arch):
+    return tuple((n, 0 if k == "conv" else 1, ci, co, s, src, sk, b, r)
+                 for n, k, ci, co, s, src, sk, b, r in arch)
 
 
-def test_forward_backward_match_oracle(gpu):
+@pytest.mark.parametrize("kind", ["generator", "unet"])
+def test_network_forward_backward_match_oracle(gpu, kind):
+    arch = OL.generator_arch(49, 7, 16) if kind == "generator" else OL.UNET_ARCH
     rng = np.random.default_rng(5)
-    params = OL.init_params(7)
-    x = rng.normal(size=(2, 1, 19, 21))
-    t = rng.normal(size=(2, 1, 19, 21))
-    y_ref, act = OL.forward(params, x)
-    loss_ref, gy = OL.normalized_mse(y_ref, t)
-    grads_ref, gx_ref = OL.backward(params, act, gy)
-
-    net = LN.UNet()
+    params = OL.init_params(arch, 7)
+    x = rng.normal(size=(2, arch[0][2], 19, 21))
+    y_ref, act = OL.forward(arch, params, x)
+    t = rng.normal(size=y_ref.shape)
+    grads_ref, gx_ref = OL.backward(arch, params, act, 2.0 * (y_ref - t))
+    net = LN.ConvNet(_product_arch(arch))
     net.load_numpy(params)
-    xt = torch.as_tensor(x, dtype=torch.float32, device="cuda")
-    tt = torch.as_tensor(t, dtype=torch.float32, device="cuda")
-    y, a = net.forward(xt)
-    assert _rel(y.double().cpu().numpy(), y_ref) < 1e-5
-    loss, g = net.nmse(y, tt)
-    assert abs(float(loss) - loss_ref) < 1e-5 * abs(loss_ref)
+    y, a = net.forward(torch.as_tensor(x, dtype=torch.float32, device="cuda"))
+    y_np = y.double().cpu().numpy()
+    assert _rel(y_np, y_ref) < 1e-5
+    gy = torch.as_tensor(2.0 * (y_np - t), dtype=torch.float32, device="cuda")
     net.grads.zero_()
-    gx = net.backward(a, g, 2, 19, 21)
-    for name in params:
-        got = net.grads[net.offsets[name][0]: net.offsets[name][0] + params[name].size].view(params[name].shape)
-        assert _rel(got.double().cpu().numpy(), grads_ref[name]) < 1e-4, name
+    gx = net.backward(a, gy, 2, 19, 21)
+    for name, ref in grads_ref.items():
+        if name.endswith(".b"):
+            got = net.grads[net.bias_offsets[name[:-2]]: net.bias_offsets[name[:-2]] + ref.size]
+        else:
+            o = net.offsets[name][0]
+            got = net.grads[o: o + ref.size].view(ref.shape)
+        assert _rel(got.double().cpu().numpy(), ref) < 1e-4, name
     assert _rel(gx.double().cpu().numpy(), gx_ref) < 1e-4
 
 
-def _grf_problem(seed, refine=5):
-    from jaxiga_amd import grf
-
+def _problem(refine, seed, sigma=1.5, ell=0.1):
     V = jx.FunctionSpace(jx.primitives.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]]).elevate(3).refine(refine))
     prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=lambda x: np.ones(len(x), bool))],
                       source=lambda x, p: 1.0 + 0 * x[0])
     ctx = M.build_context(prob)
-    a = grf.gauss_point_field(ctx, seed=seed, ell=0.15, sigma=1.0)
-    params = {"a0": jx.PointField(torch.exp(a))}
-    vals, rhs, _ = M.assemble(ctx, params)
+    a = torch.exp(grf.gauss_point_field(ctx, seed=seed, ell=ell, sigma=sigma))
+    vals, rhs, _ = M.assemble(ctx, {"a0": jx.PointField(a)})
     return ctx, vals, rhs
 
 
-def test_training_reduces_loss_and_iterations(gpu):
+def _scipy(indptr, cols, vals, n, m=None):
+    return sp.csr_matrix((vals.double().cpu().numpy(), cols.cpu().numpy(), indptr.cpu().numpy()),
+                         shape=(n, m or n))
+
+
+def _oracle_levels(ctx, vals):
+    pat = ctx.assembly
+    K = _scipy(pat.indptr_d, pat.cols_d, vals, pat.shape[0])
+    return OM.hierarchy(K, ctx.space.patch.knot_arrays(), 3, ctx.dofmap.prescribed)
+
+
+def test_levels_transfers_features_match_oracle(gpu):
+    ctx, vals, _ = _problem(4, 3)
+    op = LN.setup(ctx, vals)
+    ref = _oracle_levels(ctx, vals)
+    hier = op.hier
+    assert hier.n_levels == len(ref) == 3
+    for l, (lv, rl) in enumerate(zip(hier.levels, ref)):
+        print(f"[mg] level {l} n={lv.n} h={lv.h} w={lv.w}", flush=True)
+        assert (lv.h, lv.w) == (rl["h"], rl["w"])
+        K = _scipy(lv.indptr, lv.cols, op.vals64[l], lv.n)
+        Kr = rl["K"]
+        assert abs(K - Kr).max() <= 1e-12 * abs(Kr).max()
+        if l + 1 < len(ref):
+            ip, cc, v64, _ = lv.p_dev
+            P = _scipy(ip, cc, v64, lv.n, hier.levels[l + 1].n)
+            assert abs(P - rl["P"]).max() <= 1e-14
+            f = LN.features(op, l)[0].reshape((7 * 7, -1)).double().cpu().numpy()
+            assert _rel(f, OM.stencil_features(Kr, lv.h, lv.w, 3)) < 1e-6
+    inv = op.coarse_inv.view(ref[-1]["K"].shape).double().cpu().numpy()
+    assert _rel(inv, np.linalg.inv(ref[-1]["K"].toarray())) < 1e-5
+
+
+def test_stencil_kernels_and_loss_match_oracle(gpu):
+    ctx, vals, _ = _problem(4, 4)
+    op = LN.setup(ctx, vals)
+    ref = _oracle_levels(ctx, vals)
+    rng = np.random.default_rng(8)
+    l = 1
+    lv = op.hier.levels[l]
+    Kr = ref[l]["K"]
+    dsq = 1 / np.sqrt(Kr.diagonal())
+    W = rng.normal(size=(49, lv.n)) * 0.1
+    W[24] += 0.5
+    E = rng.normal(size=(3, lv.n))
+    R = (Kr @ E.T).T
+    f32 = dict(dtype=torch.float32, device="cuda")
+    Wd, Ed, Rd = (torch.as_tensor(a, **f32).contiguous() for a in (W, E, R))
+    for tr in (False, True):
+        y = LN.stencil_apply(op, l, Wd, Ed, transpose=tr).double().cpu().numpy()
+        assert _rel(y, OM.stencil_apply(W, dsq, E, lv.h, lv.w, 7, transpose=tr)) < 1e-5
+    loss, dW = LN.energy_loss(op, l, Wd, Ed, Rd)
+    loss_ref, dW_ref = OM.energy_loss(Kr, W, dsq, E, R, lv.h, lv.w, 7)
+    assert abs(float(loss) - loss_ref) <= 1e-4 * abs(loss_ref)
+    assert _rel(dW.double().cpu().numpy(), dW_ref) < 1e-4
+
+
+def test_vcycle_matches_oracle(gpu):
+    ctx, vals, rhs = _problem(5, 6)
+    ref = _oracle_levels(ctx, vals)
+    rng = np.random.default_rng(9)
+    stencils = [rng.normal(size=(49, lv["K"].shape[0])) * 0.05 for lv in ref[:-1]]
+    for s in stencils:
+        s[24] += 0.5
+    b = rng.normal(size=ref[0]["K"].shape[0])
+    for st in (None, stencils):
+        op = LN.setup(ctx, vals)
+        if st is not None:
+            op.set_stencils([torch.as_tensor(s, dtype=torch.float32, device="cuda") for s in st])
+        x = op.vcycle(torch.as_tensor(b, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+        xr = OM.vcycle(ref, st, b)
+        assert _rel(x, xr) < 1e-4
+
+
+def test_multigrid_fcg_converges_and_matches_jacobi_cg(gpu):
+    ctx, vals, rhs = _problem(6, 11)
+    op = LN.setup(ctx, vals)
+    x, it, rr, bb = op.fcg(rhs, 1e-10, 200)
+    jac = LIN.cg_csr(ctx.assembly, vals, rhs, 1e-10)
+    rn, bn = LIN.residual_norms(ctx.assembly, vals, x, rhs)
+    print(f"[mg] fcg(jacobi smoothers) {it} it, jacobi-cg {jac.iterations} it", flush=True)
+    assert rn <= 1.01e-10 * bn
+    assert it < jac.iterations / 2
+    ref = jac.x.cpu().numpy()
+    assert np.abs(x.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_training_gains_over_untrained_smoothers(gpu):
     torch.manual_seed(0)
-    data, pairs = [], []
-    for seed in range(6):
-        ctx, vals, rhs = _grf_problem(seed)
-        d = LN.pc_data(ctx, vals)
-        data.append((d, rhs))
-        pairs += LN.solver_pairs(d, rhs, iters=(0, 2, 8, 32, 128))
-    X = torch.stack([p[0] for p in pairs])
-    T = torch.stack([p[1] for p in pairs])
-    net = LN.UNet(seed=1)
-    d0, b0 = data[0]
-    before = LN.fcg_solve(net, d0, b0, tol=1e-10)
-    losses = []
-    g = torch.Generator(device="cuda").manual_seed(0)
-    for step in range(300):
-        idx = torch.randint(0, X.shape[0], (8,), device="cuda", generator=g)
-        losses.append(float(net.train_step(X[idx].contiguous(), T[idx].contiguous(), 2e-3)))
-    assert np.mean(losses[-20:]) < 0.5 * np.mean(losses[:20])
-    ctx, vals, rhs = _grf_problem(99)  # unseen coefficient field
-    d = LN.pc_data(ctx, vals)
-    after = LN.fcg_solve(net, d, rhs, tol=1e-10)
-    jac = LIN.cg_csr(d.pattern, vals, rhs, 1e-10, dinv=d.dinv)
-    assert after.relative_residual <= 1e-10
-    assert after.fallback_iterations == 0
+    data = LN.TrainingSet()
+    for seed in range(3):
+        ctx, vals, rhs = _problem(6, seed)
+        data.add(ctx, vals, rhs)
+    assert data.n_pairs > 60
+    gen = LN.generator(3, seed=1)
+    hist = LN.train(gen, data, steps=150, lr=2e-3)
+    assert np.mean(hist[-10:]) < 0.8 * np.mean(hist[:3])
+    ctx, vals, rhs = _problem(6, 99)  # unseen coefficient field
+    before = LN.solve(ctx, vals, rhs, None, tol=1e-10)
+    after = LN.solve(ctx, vals, rhs, gen, tol=1e-10)
+    jac = LIN.cg_csr(ctx.assembly, vals, rhs, 1e-10)
+    print(f"[learned] loss {hist[0]:.3f} -> {hist[-1]:.3f}; jacobi-cg {jac.iterations} it, "
+          f"untrained mg-fcg {before.iterations} it, trained mg-fcg {after.iterations} it", flush=True)
+    for res in (before, after):
+        assert res.relative_residual <= 1e-10
+        assert res.fallback_iterations == 0
+    assert after.iterations < before.iterations
     assert after.iterations < jac.iterations
     ref = jac.x.cpu().numpy()
-    assert np.abs(after.x.cpu().numpy() - ref).max() <= 1e-7 * np.abs(ref).max()
-    print(f"[learned] jacobi {jac.iterations} it, untrained FCG {before.iterations} it, "
-          f"trained FCG {after.iterations} it", flush=True)
+    assert np.abs(after.x.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
