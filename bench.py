@@ -86,6 +86,91 @@ def cpu_baseline(sample_refine=7):
                       f"median of 3 runs of {t:.2f} s"}
 
 
+def learned_pc_bench(refine=8, n_train=48, steps=300, batch=4, test_seeds=(1000, 1001, 1002), reps=3,
+                     log=print):
+    """Time to a 1e-10 relative residual on the 256x256 cubic variable-coefficient
+    Poisson problem (a = exp(GRF), l = 0.1, sigma = 1.5, smooth random source):
+    FP64 Jacobi-CG versus flexible CG with the learned multigrid preconditioner
+    (FP32 V-cycle), on coefficient seeds the generator never saw. The generator
+    is trained first, on the GPU, from pairs the solver generates on
+    ``n_train`` training seeds (training time reported, not counted)."""
+    import torch
+
+    import jaxiga_amd as jx
+    from jaxiga_amd import grf
+    from jaxiga_amd import learned as LN
+    from jaxiga_amd import linear as LIN
+
+    V = jx.FunctionSpace(jx.primitives.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]]).elevate(3).refine(refine))
+    t0 = time.perf_counter()
+    data = LN.TrainingSet()
+    for seed in range(n_train):
+        ctx, vals, rhs = grf.poisson_instance(refine, seed, space=V)
+        data.add(ctx, vals, rhs)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    log(f"[bench] learned PC: {data.n_pairs} residual/error pairs from {n_train} fields in {t_data:.1f} s")
+    gen = LN.generator(3, seed=1)
+    t0 = time.perf_counter()
+    hist = LN.train(gen, data, steps=steps, lr=1e-3, batch=batch)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"[bench] trained {steps} steps in {t_train:.1f} s, loss {hist[0]:.3f} -> {np.mean(hist[-10:]):.3f}")
+    del data
+    rows = []
+    for seed in test_seeds:
+        ctx, vals, rhs = grf.poisson_instance(refine, seed, space=V)
+        pattern = ctx.assembly
+        work = torch.empty(int(_lib_mod().iga_cg_workspace(ctx.n_free)), dtype=torch.float64, device="cuda")
+        # warm every path once (first-use plan capture is reported separately)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        LN.solve(ctx, vals, rhs, gen)
+        torch.cuda.synchronize()
+        t_first = time.perf_counter() - t0
+        LN.solve(ctx, vals, rhs, None)
+        LIN.cg_csr(pattern, vals, rhs, 1e-10, workspace=work)
+        tj, tu, tl = [], [], []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            jac = LIN.cg_csr(pattern, vals, rhs, 1e-10, workspace=work)
+            torch.cuda.synchronize()
+            tj.append(time.perf_counter() - t0)
+            un = LN.solve(ctx, vals, rhs, None)
+            tu.append(un.seconds)
+            le = LN.solve(ctx, vals, rhs, gen)
+            tl.append(le.seconds)
+        rn, bn = LIN.residual_norms(pattern, vals, jac.x, rhs)
+        diff = float((le.x - jac.x).abs().max() / jac.x.abs().max())
+        row = {"seed": seed, "jacobi_cg_fp64": {"seconds": statistics.median(tj), "iterations": jac.iterations,
+                                                "relative_residual": rn / bn},
+               "mg_fcg_untrained": {"seconds": statistics.median(tu), "iterations": un.iterations,
+                                    "relative_residual": un.relative_residual},
+               "learned_mg_fcg": {"seconds": statistics.median(tl), "setup_seconds": le.setup_seconds,
+                                  "iterations": le.iterations, "fallback_iterations": le.fallback_iterations,
+                                  "relative_residual": le.relative_residual, "first_call_seconds": t_first,
+                                  "max_rel_diff_vs_jacobi_cg": diff}}
+        row["speedup_vs_jacobi_cg"] = row["jacobi_cg_fp64"]["seconds"] / row["learned_mg_fcg"]["seconds"]
+        log(f"[bench] seed {seed}: jacobi-cg {jac.iterations} it {row['jacobi_cg_fp64']['seconds'] * 1e3:.2f} ms | "
+            f"mg-fcg untrained {un.iterations} it {row['mg_fcg_untrained']['seconds'] * 1e3:.2f} ms | "
+            f"learned {le.iterations} it {row['learned_mg_fcg']['seconds'] * 1e3:.2f} ms "
+            f"(setup {le.setup_seconds * 1e3:.2f} ms) -> {row['speedup_vs_jacobi_cg']:.2f}x")
+        rows.append(row)
+    return {"workload": f"unit-square Poisson, p=3, {2**refine}x{2**refine} elements, a=exp(GRF l=0.1 sigma=1.5), "
+                        "smooth random source, rel. residual 1e-10, unseen coefficient seeds",
+            "training": {"fields": n_train, "pairs": None, "steps": steps, "seconds": t_train,
+                         "data_seconds": t_data, "loss_first": hist[0], "loss_last10": float(np.mean(hist[-10:]))},
+            "per_seed": rows,
+            "speedup_median": statistics.median(r["speedup_vs_jacobi_cg"] for r in rows)}
+
+
+def _lib_mod():
+    from jaxiga_amd import _lib
+
+    return _lib.load()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +181,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (HBM counter calibration)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only run the learned-preconditioner time-to-solve comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -119,6 +207,9 @@ def main():
         print("[bench] copied 5 x 256 MiB", flush=True)
         return
     lib = _lib.load()
+    if args.learned_only:
+        print(json.dumps({"learned_pc": learned_pc_bench()}), flush=True)
+        return
     precision = _lib.IGA_FP32 if args.precision == "fp32" else _lib.IGA_FP64
     dtype = "f32" if args.precision == "fp32" else "f64"
     peak_tf = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_FP64_TFLOPS
@@ -192,6 +283,7 @@ def main():
                  "relative_residual_fp64": rn / bn}
         print(f"[bench] jacobi-cg {t_solve:.3f} s, {res.iterations} it, rel.res {rn / bn:.2e}", flush=True)
 
+    learned = None if (args.no_learned or args.no_solve) else learned_pc_bench()
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "assembly Melem/s; time-to-solve @1e-10 rel res, learned-PC vs plain CG",
@@ -213,6 +305,7 @@ def main():
         "roofline_secondary": secondary,
         "cpu_baseline": cpu,
         "time_to_solve": solve,
+        "learned_pc": learned,
     }
     print(json.dumps(line), flush=True)
 

@@ -307,6 +307,35 @@ typedef struct iga_mg {
   float* work;
 } iga_mg_t;
 
+/* Numeric set-up of a hierarchy for a new level-0 operator: Galerkin levels
+ * K_{l+1} = P_l^T K_l P_l (FP64, one wavefront per coarse row; the fine
+ * stencil of every coarse row must fit a 16 x 16 box of fine grid points),
+ * FP32 copies, inverse diagonals (1 where zero) and D^-1/2, and the FP32
+ * dense inverse of the coarsest level (n <= 4096). pt: the CSR of P_l^T
+ * (coarse x fine) in FP64; radius: the operators' stencil radius. */
+typedef struct iga_mg_setup_level {
+  int32_t n, h, w, nnz;
+  const int32_t* indptr;
+  const int32_t* cols;
+  double* vals64;
+  float* vals32;
+  double* dinv;
+  float* dsq;
+  const int32_t* pt_ptr;
+  const int32_t* pt_cols;
+  const double* pt_vals;
+} iga_mg_setup_level_t;
+
+typedef struct iga_mg_setup {
+  int32_t n_levels;
+  int32_t radius;
+  iga_mg_setup_level_t level[16];
+  float* coarse_inv;
+  double* inv_work; /* 2 n_c^2 doubles */
+} iga_mg_setup_t;
+
+int iga_mg_load(const iga_mg_setup_t* setup, const double* vals0, void* stream);
+
 int64_t iga_mg_work_floats(const iga_mg_t* mg);
 /* x = V(b) on level `level` (FP32 vectors of that level's size) */
 int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, float* x, void* stream);
@@ -341,6 +370,20 @@ int iga_energy_loss(int32_t batch, int64_t n, const float* z, const float* kz, f
  * graph_iters > 0 captures that many iterations (rounded up to even) into a
  * hipGraph that is replayed. work: iga_fcg_workspace(n) doubles; work32: 2n floats. */
 int64_t iga_fcg_workspace(int32_t n);
+/* A reusable plan: the iterations are captured once over fixed buffers (b, x,
+ * vals, work, and everything the iga_mg_t points to); each run re-initialises
+ * the solver state on the device and replays the graph, so one plan serves
+ * every right-hand side written into b and every operator whose values are
+ * rewritten in place. The plan owns only the graph; destroy it before the
+ * buffers are freed. */
+typedef struct iga_fcg_plan iga_fcg_plan_t;
+int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* indptr,
+                    const int32_t* cols, const double* vals, const double* b, double* x,
+                    double* work, float* work32, int32_t graph_iters, void* stream,
+                    iga_fcg_plan_t** out);
+int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter, int32_t* out_iters,
+                   double* out_rr, double* out_bb, void* stream);
+void iga_fcg_mg_destroy(iga_fcg_plan_t* plan);
 int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32_t* cols,
                const double* vals, const double* b, double* x, double tol, int32_t maxiter,
                int32_t graph_iters, double* work, float* work32, int32_t* out_iters,

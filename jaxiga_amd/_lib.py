@@ -72,6 +72,19 @@ class IgaMgLevel(ctypes.Structure):
     ]
 
 
+class IgaMgSetupLevel(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("h", c_int), ("w", c_int), ("nnz", c_int), ("indptr", c_void_p),
+        ("cols", c_void_p), ("vals64", c_void_p), ("vals32", c_void_p), ("dinv", c_void_p),
+        ("dsq", c_void_p), ("pt_ptr", c_void_p), ("pt_cols", c_void_p), ("pt_vals", c_void_p),
+    ]
+
+
+class IgaMgSetup(ctypes.Structure):
+    _fields_ = [("n_levels", c_int), ("radius", c_int), ("level", IgaMgSetupLevel * 16),
+                ("coarse_inv", c_void_p), ("inv_work", c_void_p)]
+
+
 class IgaMg(ctypes.Structure):
     _fields_ = [("n_levels", c_int), ("ks", c_int), ("level", IgaMgLevel * 16),
                 ("coarse_inv", c_void_p), ("work", c_void_p)]
@@ -122,6 +135,7 @@ _SIGNATURES = {
     "iga_nmse": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_adam": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
                          ctypes.c_float, ctypes.c_float, c_int, c_void_p]),
+    "iga_mg_load": (c_int, [ctypes.POINTER(IgaMgSetup), c_void_p, c_void_p]),
     "iga_mg_work_floats": (c_i64, [ctypes.POINTER(IgaMg)]),
     "iga_mg_vcycle": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
     "iga_masked_spgemm": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -138,6 +152,11 @@ _SIGNATURES = {
     "iga_energy_loss": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p]),
     "iga_fcg_workspace": (c_i64, [c_int]),
+    "iga_fcg_mg_plan": (c_int, [ctypes.POINTER(IgaMg), c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                ctypes.POINTER(c_void_p)]),
+    "iga_fcg_mg_run": (c_int, [c_void_p, c_double, c_int, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_fcg_mg_destroy": (None, [c_void_p]),
     "iga_fcg_mg": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_double, c_int, c_int, c_void_p, c_void_p, P_int, P_dbl, P_dbl,
                            c_void_p]),

@@ -54,6 +54,91 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Galerkin product K_c = P^T K P, one wavefront per coarse row I. The row of
+// T = P^T K is accumulated over the fine rows a of P^T row I (each a's K row
+// is spread over the lanes; one a at a time, so lanes never collide) into a
+// dense box of fine grid points held in LDS; then each coarse entry (I, J) of
+// the row's CSR pattern is sum_b P^T[J, b] T[I, b] over P^T row J. Fixed
+// order throughout: bitwise reproducible. The box (kGalBox^2 fine points)
+// must cover the stencil of every fine row of P^T row I (checked on the host).
+constexpr int kGalBox = 16;
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+    k_galerkin_rows(int nc, int wf, int R, const int32_t* __restrict__ k_ptr,
+                    const int32_t* __restrict__ k_cols, const double* __restrict__ k_vals,
+                    const int32_t* __restrict__ pt_ptr, const int32_t* __restrict__ pt_cols,
+                    const double* __restrict__ pt_vals, const int32_t* __restrict__ c_ptr,
+                    const int32_t* __restrict__ c_cols, double* __restrict__ c_vals) {
+  __shared__ double boxes[4][kGalBox * kGalBox];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int I = blockIdx.x * 4 + wave;
+  if (I >= nc) return;
+  double* box = boxes[wave];
+  for (int t = lane; t < kGalBox * kGalBox; t += kWave) box[t] = 0.0;
+  const int a0 = pt_ptr[I], a1 = pt_ptr[I + 1];
+  int rmin = 1 << 30, cmin = 1 << 30;
+  for (int k = a0 + lane; k < a1; k += kWave) {
+    const int a = pt_cols[k];
+    rmin = min(rmin, a / wf);
+    cmin = min(cmin, a % wf);
+  }
+  const int r0 = wave_min(rmin) - R, c0 = wave_min(cmin) - R;
+  __builtin_amdgcn_wave_barrier();
+  for (int k = a0; k < a1; ++k) {
+    const int a = pt_cols[k];
+    const double pa = pt_vals[k];
+    for (int q = k_ptr[a] + lane; q < k_ptr[a + 1]; q += kWave) {
+      const int b = k_cols[q];
+      const int br = b / wf - r0, bc = b % wf - c0;
+      box[br * kGalBox + bc] += pa * k_vals[q];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int k = c_ptr[I] + lane; k < c_ptr[I + 1]; k += kWave) {
+    const int J = c_cols[k];
+    double acc = 0.0;
+    for (int q = pt_ptr[J]; q < pt_ptr[J + 1]; ++q) {
+      const int b = pt_cols[q];
+      const int br = b / wf - r0, bc = b % wf - c0;
+      if (br >= 0 && br < kGalBox && bc >= 0 && bc < kGalBox) acc += pt_vals[q] * box[br * kGalBox + bc];
+    }
+    c_vals[k] = acc;
+  }
+}
+
+// vals32 = (float) vals over all entries
+__global__ void k_to_f32(int64_t nnz, const double* __restrict__ vals, float* __restrict__ vals32) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x)
+    vals32[k] = (float)vals[k];
+}
+
+// per row (sorted columns): dinv = 1 / K_ii (1 where K_ii = 0), dsq = sqrt(dinv)
+__global__ void k_level_scalings(int n, const int32_t* __restrict__ indptr,
+                                 const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                                 double* __restrict__ dinv, float* __restrict__ dsq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int lo = indptr[i], hi = indptr[i + 1];
+  const int k = lower_bound(cols, lo, hi, i);
+  const double d = (k < hi && cols[k] == i) ? vals[k] : 0.0;
+  const double di = d != 0.0 ? 1.0 / d : 1.0;
+  dinv[i] = di;
+  dsq[i] = (float)sqrt(fabs(di));
+}
+
+__global__ void k_copy_d(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 // feats[o][i] = K[i, i + off(o)] / K[i, i] on the h x w grid, o over the
 // (2R+1)^2 offsets (feats cleared beforehand).
 __global__ void k_stencil_features(int n, int w, int R, const int32_t* __restrict__ indptr,
@@ -194,16 +279,70 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// x = M b for a dense n x n matrix (coarsest level)
-__global__ void k_dense_gemv(int n, const float* __restrict__ M, const float* __restrict__ b,
-                             float* __restrict__ x, const int* __restrict__ skip) {
+// Jacobi pre-smoothing from x = 0 fused with the residual:
+// x = w D^-1 b, r = b - A x (A x evaluated with the x it writes: row products
+// recompute w D^-1 b at the neighbours instead of reading x)
+template <int LPR>
+__global__ void __launch_bounds__(256)
+    k_jacobi_pre(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                 const float* __restrict__ vals, float omega, const float* __restrict__ dsq,
+                 const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
+                 const int* __restrict__ skip) {
   if (skipped(skip)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
+    const int lo = indptr[row], hi = indptr[row + 1];
+    float s = 0.f;
+    for (int k = lo + l; k < hi; k += LPR) {
+      const int j = cols[k];
+      s += vals[k] * (omega * dsq[j] * dsq[j] * b[j]);
+    }
+#pragma unroll
+    for (int off = LPR / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LPR);
+    if (l == 0) {
+      x[row] = omega * dsq[row] * dsq[row] * b[row];
+      r[row] = b[row] - s;
+    }
+  }
+}
+
+// Jacobi post-smoothing fused with its residual: x_out = x + w D^-1 (b - A x)
+template <int LPR>
+__global__ void __launch_bounds__(256)
+    k_jacobi_post(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                  const float* __restrict__ vals, float omega, const float* __restrict__ dsq,
+                  const float* __restrict__ b, const float* __restrict__ x,
+                  float* __restrict__ x_out, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
+    const float s = row_dot_f<LPR>(row, indptr, cols, vals, x, l);
+    if (l == 0) x_out[row] = x[row] + omega * dsq[row] * dsq[row] * (b[row] - s);
+  }
+}
+
+// x = M b for a dense n x n matrix (coarsest level), one wavefront per row
+__global__ void __launch_bounds__(256)
+    k_dense_gemv(int n, const float* __restrict__ M, const float* __restrict__ b,
+                 float* __restrict__ x, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int i = blockIdx.x * 4 + threadIdx.x / kWave, l = threadIdx.x % kWave;
   if (i >= n) return;
   const float* row = M + (size_t)i * n;
   float s = 0.f;
-  for (int j = 0; j < n; ++j) s += row[j] * b[j];
-  x[i] = s;
+  for (int j = l; j < n; j += kWave) s += row[j] * b[j];
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (l == 0) x[i] = s;
+}
+
+__global__ void k_copy_fs(int n, const float* __restrict__ a, float* __restrict__ b,
+                          const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 __global__ void k_d2f(int n, const double* __restrict__ a, float* __restrict__ b,
@@ -339,6 +478,21 @@ static int64_t level_offset(const iga_mg_t& mg, int l) {
 }
 
 // x = V(b) starting at level l0 (b, x of level l0's size)
+template <typename F>
+static int launch_rows(int n, int nnz, F f) {
+  const int lpr = pick_lpr(n, nnz);
+  const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)n * lpr + 255) / 256, 4096));
+  return by_lpr(lpr, [&](auto L) {
+    f(L, rb);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+}
+
+// x = V(b) starting at level l0 (b, x of level l0's size). Levels with a
+// stencil smoother run pre-smooth / residual / restrict and prolong /
+// residual / post-smooth as separate kernels; damped-Jacobi levels fuse the
+// smoother into the residual kernels.
 static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const int* skip,
                   hipStream_t st) {
   const int Lc = mg.n_levels - 1;
@@ -356,9 +510,18 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
   int rc;
   for (int l = l0; l < Lc; ++l) {
     const iga_mg_level_t& L = mg.level[l];
-    if ((rc = smooth(mg, l, B[l], X[l], false, 0, skip, st))) return rc;
-    if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
-      return rc;
+    if (L.stencil) {
+      if ((rc = smooth(mg, l, B[l], X[l], false, 0, skip, st))) return rc;
+      if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
+        return rc;
+    } else {
+      rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
+        constexpr int LPR = decltype(K)::value;
+        hipLaunchKernelGGL(k_jacobi_pre<LPR>, dim3(rb), dim3(256), 0, st, L.n, L.indptr, L.cols,
+                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip);
+      });
+      if (rc) return rc;
+    }
     const iga_mg_level_t& C = mg.level[l + 1];
     if ((rc = csr_f(C.n, L.pt_nnz, L.pt_ptr, L.pt_cols, L.pt_vals, Rr[l], nullptr,
                     const_cast<float*>(B[l + 1]), 0, 1, 0, 0, skip, st)))
@@ -366,7 +529,7 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
   }
   {
     const iga_mg_level_t& C = mg.level[Lc];
-    hipLaunchKernelGGL(k_dense_gemv, dim3(blocks_for(C.n)), dim3(256), 0, st, C.n, mg.coarse_inv,
+    hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, mg.coarse_inv,
                        B[Lc], X[Lc], skip);
     IGA_CHECK_LAUNCH();
   }
@@ -375,9 +538,25 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
     if ((rc = csr_f(L.n, L.p_nnz, L.p_ptr, L.p_cols, L.p_vals, X[l + 1], nullptr, X[l], 2, 1, 0, 0,
                     skip, st)))
       return rc;
-    if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
-      return rc;
-    if ((rc = smooth(mg, l, Rr[l], X[l], true, 1, skip, st))) return rc;
+    if (L.stencil) {
+      if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
+        return rc;
+      if ((rc = smooth(mg, l, Rr[l], X[l], true, 1, skip, st))) return rc;
+    } else {
+      // post-smooth into the residual buffer, then copy back only at the
+      // caller's level (the parent prolongs from whichever buffer holds x)
+      rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
+        constexpr int LPR = decltype(K)::value;
+        hipLaunchKernelGGL(k_jacobi_post<LPR>, dim3(rb), dim3(256), 0, st, L.n, L.indptr, L.cols,
+                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip);
+      });
+      if (rc) return rc;
+      std::swap(X[l], Rr[l]);
+      if (l == l0) {
+        hipLaunchKernelGGL(k_copy_fs, dim3(blocks_for(L.n)), dim3(256), 0, st, L.n, X[l], x0, skip);
+        IGA_CHECK_LAUNCH();
+      }
+    }
   }
   return IGA_OK;
 }
@@ -419,12 +598,13 @@ struct FBufs {
 };
 
 __global__ void __launch_bounds__(kVecThreads)
-    k_fcg_init(int n, const double* __restrict__ b, double* x, double* r, double* p1, double* q1,
-               FBufs F) {
+    k_fcg_init(int n, const double* __restrict__ b, double* x, double* r, float* r32, double* p1,
+               double* q1, FBufs F) {
   double bb = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] = 0.0;
     r[i] = b[i];
+    r32[i] = (float)b[i];
     p1[i] = 0.0;  // the "previous" direction of the first iteration
     q1[i] = 0.0;
     bb += b[i] * b[i];
@@ -451,12 +631,17 @@ __global__ void k_fcg_check(int nblk, FBufs F, int init, double tol2, int maxite
   }
 }
 
+// z = (double) z32 (the V-cycle output) and the partials of z . q_prev
 __global__ void __launch_bounds__(kVecThreads)
-    k_fcg_zq(int n, const double* __restrict__ z, const double* __restrict__ qp, FBufs F) {
+    k_fcg_zq(int n, const float* __restrict__ z32, double* __restrict__ z,
+             const double* __restrict__ qp, FBufs F) {
   if (F.st->done) return;
   double s = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    s += z[i] * qp[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double zi = (double)z32[i];
+    z[i] = zi;
+    s += zi * qp[i];
+  }
   store_partial(F.P_a, s);
 }
 
@@ -496,7 +681,7 @@ __global__ void __launch_bounds__(kVecThreads)
 
 __global__ void __launch_bounds__(kVecThreads)
     k_fcg_update(int n, int nblk_c, int nblk, const double* __restrict__ p,
-                 const double* __restrict__ q, double* x, double* r, FBufs F) {
+                 const double* __restrict__ q, double* x, double* r, float* r32, FBufs F) {
   FState* st = F.st;
   if (st->done) return;
   const double pq = reduce_partials(F.P_c, nblk_c);
@@ -507,6 +692,7 @@ __global__ void __launch_bounds__(kVecThreads)
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
+    r32[i] = (float)ri;
     rr += ri * ri;
   }
   store_partial(F.P_d, rr);
@@ -530,6 +716,42 @@ extern "C" int iga_masked_spgemm(int64_t nnz_c, const int32_t* c_rows, const int
                      c_rows, c_cols, a_ptr, a_cols, a_vals, bt_ptr, bt_cols, bt_vals, c_vals);
   IGA_CHECK_LAUNCH();
   return IGA_OK;
+}
+
+extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* stream) {
+  if (!s || s->n_levels < 1 || s->n_levels > 16 || !vals0 || !s->coarse_inv || !s->inv_work)
+    return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int L = s->n_levels;
+  for (int l = 0; l < L; ++l) {
+    const iga_mg_setup_level_t& V = s->level[l];
+    if (V.n <= 0 || !V.indptr || !V.cols || !V.vals64 || !V.vals32 || !V.dinv || !V.dsq)
+      return IGA_ERR_ARG;
+    if (l + 1 < L && (!V.pt_ptr || !V.pt_cols || !V.pt_vals || V.w <= 0)) return IGA_ERR_ARG;
+  }
+  if (s->level[L - 1].n > 4096) return IGA_ERR_SHAPE;
+  {
+    const iga_mg_setup_level_t& V = s->level[0];
+    if (V.vals64 != vals0)
+      hipLaunchKernelGGL(k_copy_d, dim3((int)std::min<int64_t>(blocks_for(V.nnz), 4096)), dim3(256), 0,
+                         st, (int64_t)V.nnz, vals0, V.vals64);
+  }
+  for (int l = 0; l < L; ++l) {
+    const iga_mg_setup_level_t& V = s->level[l];
+    if (l + 1 < L) {
+      const iga_mg_setup_level_t& C = s->level[l + 1];
+      hipLaunchKernelGGL(k_galerkin_rows, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, V.w,
+                         s->radius, V.indptr, V.cols, V.vals64, V.pt_ptr, V.pt_cols, V.pt_vals,
+                         C.indptr, C.cols, C.vals64);
+    }
+    hipLaunchKernelGGL(k_to_f32, dim3((int)std::min<int64_t>(blocks_for(V.nnz), 4096)), dim3(256), 0,
+                       st, (int64_t)V.nnz, V.vals64, V.vals32);
+    hipLaunchKernelGGL(k_level_scalings, dim3(blocks_for(V.n)), dim3(256), 0, st, V.n, V.indptr,
+                       V.cols, V.vals64, V.dinv, V.dsq);
+  }
+  IGA_CHECK_LAUNCH();
+  const iga_mg_setup_level_t& C = s->level[L - 1];
+  return iga_dense_inverse(C.n, C.indptr, C.cols, C.vals64, s->inv_work, s->coarse_inv, stream);
 }
 
 extern "C" int iga_stencil_features(int32_t n, int32_t w, int32_t radius, const int32_t* indptr,
@@ -631,80 +853,52 @@ extern "C" int iga_energy_loss(int32_t batch, int64_t n, const float* z, const f
 
 extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 6 * (int64_t)n + 4 * kRedBlocks; }
 
-extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
-                          const int32_t* cols, const double* vals, const double* b, double* x,
-                          double tol, int32_t maxiter, int32_t graph_iters, double* work,
-                          float* work32, int32_t* out_iters, double* out_rr, double* out_bb,
-                          void* stream) {
-  int rc0 = check_mg(mg);
-  if (rc0) return rc0;
-  if (!indptr || !cols || !vals || !b || !x || !work || !work32) return IGA_ERR_ARG;
-  if (mg->level[0].n != n) return IGA_ERR_SHAPE;
-  hipStream_t st = (hipStream_t)stream;
-  if (n == 0) {
-    if (out_iters) *out_iters = 0;
-    return IGA_OK;
-  }
+// A captured flexible-CG solve over fixed buffers: every run re-initialises
+// the state on the device and replays the graph until the device flag says
+// done, so one plan serves every right-hand side written into `b` and every
+// operator whose values are rewritten in place.
+struct iga_fcg_plan {
+  iga_mg_t mg;
+  int n, nnz, lpr, vb, rb, eb, G;
+  const int32_t* indptr;
+  const int32_t* cols;
+  const double* vals;
+  const double* b;
+  double* x;
   FBufs F;
-  F.st = reinterpret_cast<FState*>(work);
-  double* r = work + 8;
-  double* z = r + n;
-  double* P[2] = {z + n, z + 2 * n};
-  double* Q[2] = {z + 3 * n, z + 4 * n};
-  double* parts = z + 5 * n;
-  F.P_a = parts;
-  F.P_b = parts + kRedBlocks;
-  F.P_c = parts + 2 * kRedBlocks;
-  F.P_d = parts + 3 * kRedBlocks;
-  float* r32 = work32;
-  float* z32 = work32 + n;
-  int nnz = 0;
-  {
-    int32_t v = 0;
-    if (hipMemcpyAsync(&v, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
-      return IGA_ERR_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
-    nnz = v;
-  }
-  const int lpr = pick_lpr(n, nnz);
-  const int vb = vec_blocks(n);
-  const int rb = row_blocks(n, lpr);
-  const int eb = blocks_for(n);
-  const int* skip = &F.st->done;
-  hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, r, P[1], Q[1], F);
-  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol, maxiter);
-  IGA_CHECK_LAUNCH();
+  double *r, *z, *P[2], *Q[2];
+  float *r32, *z32;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
 
-  auto iteration = [&](int par) -> int {
+  int iteration(int par, hipStream_t st) {
     double* p = P[par];
     double* pp = P[1 - par];
     double* q = Q[par];
     double* qp = Q[1 - par];
-    hipLaunchKernelGGL(k_d2f, dim3(eb), dim3(256), 0, st, n, r, r32, skip);
-    int rc = vcycle(*mg, 0, r32, z32, skip, st);
+    const int* skip = &F.st->done;
+    int rc = vcycle(mg, 0, r32, z32, skip, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_f2d, dim3(eb), dim3(256), 0, st, n, z32, z, skip);
-    hipLaunchKernelGGL(k_fcg_zq, dim3(vb), dim3(kVecThreads), 0, st, n, z, qp, F);
+    hipLaunchKernelGGL(k_fcg_zq, dim3(vb), dim3(kVecThreads), 0, st, n, z32, z, qp, F);
     hipLaunchKernelGGL(k_fcg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, z, pp, p, r, F);
     by_lpr(lpr, [&](auto L) {
       constexpr int LPR = decltype(L)::value;
-      hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols, vals,
-                         p, q, F);
+      hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
+                         vals, p, q, F);
       return 0;
     });
-    hipLaunchKernelGGL(k_fcg_update, dim3(vb), dim3(kVecThreads), 0, st, n, rb, vb, p, q, x, r, F);
+    hipLaunchKernelGGL(k_fcg_update, dim3(vb), dim3(kVecThreads), 0, st, n, rb, vb, p, q, x, r,
+                       r32, F);
     hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 0, 0.0, 0);
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
-  };
+  }
 
-  const int G = graph_iters > 0 ? (graph_iters + 1) & ~1 : 0;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (G > 0) {
+  int capture(hipStream_t st) {
+    if (G <= 0) return IGA_OK;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
     int rc = 0;
-    for (int k = 0; k < G && !rc; ++k) rc = iteration(k & 1);
+    for (int k = 0; k < G && !rc; ++k) rc = iteration(k & 1, st);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc || e != hipSuccess) {
@@ -714,43 +908,129 @@ extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
     graph = g;
     if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
       (void)hipGraphDestroy(graph);
+      graph = nullptr;
+      exec = nullptr;
       return IGA_ERR_HIP;
     }
+    return IGA_OK;
   }
-  FState host{};
-  int launched = 0;
-  int status = IGA_OK;
-  for (;;) {
-    if (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) != hipSuccess) {
-      status = IGA_ERR_HIP;
-      break;
-    }
-    if (hipStreamSynchronize(st) != hipSuccess) {
-      status = IGA_ERR_HIP;
-      break;
-    }
-    if (host.done) break;
-    if (exec) {
-      if (hipGraphLaunch(exec, st) != hipSuccess) {
+
+  int run(double tol, int maxiter, int32_t* out_iters, double* out_rr, double* out_bb,
+          hipStream_t st) {
+    hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, r, r32, P[1], Q[1],
+                       F);
+    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol,
+                       maxiter);
+    IGA_CHECK_LAUNCH();
+    FState host{};
+    int launched = 0, status = IGA_OK;
+    for (;;) {
+      if (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
         status = IGA_ERR_HIP;
         break;
       }
-      launched += G;
-    } else {
-      for (int k = 0; k < 2; ++k, ++launched) {
-        const int rc = iteration(launched & 1);
-        if (rc) {
-          status = rc;
+      if (host.done) break;
+      if (exec) {
+        if (hipGraphLaunch(exec, st) != hipSuccess) {
+          status = IGA_ERR_HIP;
           break;
         }
+        launched += G;
+      } else {
+        for (int k = 0; k < 2 && !status; ++k, ++launched) status = iteration(launched & 1, st);
+        if (status) break;
       }
-      if (status) break;
     }
+    if (out_iters) *out_iters = host.iter;
+    if (out_rr) *out_rr = host.rr;
+    if (out_bb) *out_bb = host.bb;
+    return status;
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  if (out_iters) *out_iters = host.iter;
-  if (out_rr) *out_rr = host.rr;
-  if (out_bb) *out_bb = host.bb;
-  return status;
+
+  ~iga_fcg_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* indptr,
+                               const int32_t* cols, const double* vals, const double* b, double* x,
+                               double* work, float* work32, int32_t graph_iters, void* stream,
+                               iga_fcg_plan_t** out) {
+  if (!out) return IGA_ERR_ARG;
+  *out = nullptr;
+  int rc = check_mg(mg);
+  if (rc) return rc;
+  if (!indptr || !cols || !vals || !b || !x || !work || !work32 || n <= 0) return IGA_ERR_ARG;
+  if (mg->level[0].n != n) return IGA_ERR_SHAPE;
+  iga_fcg_plan* P = new iga_fcg_plan();
+  P->mg = *mg;
+  P->n = n;
+  P->nnz = nnz;
+  P->indptr = indptr;
+  P->cols = cols;
+  P->vals = vals;
+  P->b = b;
+  P->x = x;
+  P->F.st = reinterpret_cast<FState*>(work);
+  P->r = work + 8;
+  P->z = P->r + n;
+  P->P[0] = P->z + n;
+  P->P[1] = P->z + 2 * n;
+  P->Q[0] = P->z + 3 * n;
+  P->Q[1] = P->z + 4 * n;
+  double* parts = P->z + 5 * n;
+  P->F.P_a = parts;
+  P->F.P_b = parts + kRedBlocks;
+  P->F.P_c = parts + 2 * kRedBlocks;
+  P->F.P_d = parts + 3 * kRedBlocks;
+  P->r32 = work32;
+  P->z32 = work32 + n;
+  P->lpr = pick_lpr(n, nnz);
+  P->vb = vec_blocks(n);
+  P->rb = row_blocks(n, P->lpr);
+  P->eb = blocks_for(n);
+  P->G = graph_iters > 0 ? (graph_iters + 1) & ~1 : 0;
+  P->graph = nullptr;
+  P->exec = nullptr;
+  rc = P->capture((hipStream_t)stream);
+  if (rc) {
+    delete P;
+    return rc;
+  }
+  *out = P;
+  return IGA_OK;
+}
+
+extern "C" int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter,
+                              int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
+  if (!plan) return IGA_ERR_ARG;
+  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, (hipStream_t)stream);
+}
+
+extern "C" void iga_fcg_mg_destroy(iga_fcg_plan_t* plan) { delete plan; }
+
+extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
+                          const int32_t* cols, const double* vals, const double* b, double* x,
+                          double tol, int32_t maxiter, int32_t graph_iters, double* work,
+                          float* work32, int32_t* out_iters, double* out_rr, double* out_bb,
+                          void* stream) {
+  if (n == 0) {
+    if (out_iters) *out_iters = 0;
+    return IGA_OK;
+  }
+  if (!indptr) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  int32_t nnz = 0;
+  if (hipMemcpyAsync(&nnz, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return IGA_ERR_HIP;
+  iga_fcg_plan_t* plan = nullptr;
+  int rc = iga_fcg_mg_plan(mg, n, nnz, indptr, cols, vals, b, x, work, work32, graph_iters, stream,
+                           &plan);
+  if (rc) return rc;
+  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, st);
+  delete plan;
+  return rc;
 }

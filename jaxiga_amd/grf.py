@@ -49,19 +49,21 @@ def gauss_point_field(ctx, seed, ell, sigma, M=4096):
     return rff_eval(ctx.gauss_points(), omega, phase, sigma)
 
 
-def poisson_instance(refine, seed, ell=0.1, sigma=1.5, degree=3, source_ell=0.25, source_sigma=1.0):
+def poisson_instance(refine, seed, ell=0.1, sigma=1.5, degree=3, source_ell=0.25, source_sigma=1.0,
+                     space=None):
     """A seeded variable-coefficient Poisson problem on the unit square:
     ``-div(a grad u) = f``, ``u = 0`` on the boundary, ``a = exp(g)`` with
     ``g`` a GRF (``ell``, ``sigma``) sampled at the Gauss points and ``f`` a
     smooth random source (a second GRF). The patch is the bilinear unit square
-    elevated to ``degree`` and refined to ``2**refine`` elements per side.
-    Returns ``(ctx, vals, rhs)``: the method context, FP64 CSR values and the
-    free load vector, assembled on the GPU."""
+    elevated to ``degree`` and refined to ``2**refine`` elements per side
+    (or ``space``, reused with its cached patterns). Returns ``(ctx, vals,
+    rhs)``: the method context, FP64 CSR values and the free load vector,
+    assembled on the GPU."""
     import jaxiga_amd as jx
     from jaxiga_amd import methods as M
 
-    V = jx.FunctionSpace(jx.primitives.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]])
-                         .elevate(degree).refine(refine))
+    V = space if space is not None else jx.FunctionSpace(
+        jx.primitives.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]]).elevate(degree).refine(refine))
     om_s, ph_s = rff_params(10_000 + seed, 2, source_ell)
 
     def source(x, p=None):

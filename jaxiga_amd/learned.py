@@ -263,7 +263,7 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     """Multigrid operator with the generator's smoothers on the ``levels``
     finest levels and damped Jacobi elsewhere (everywhere when ``gen`` is None)."""
     hier = MG.hierarchy(ctx)
-    op = MG.MGOperator(hier, vals64, omega=OMEGA0, ks=KS)
+    op = MG.operator(hier, omega=OMEGA0, ks=KS).load(vals64)
     if gen is not None:
         op.set_stencils(smoother_stencils(gen, op, levels))
     return op
@@ -373,34 +373,53 @@ def level_pairs(op: MG.MGOperator, errors64, n_random=4, seed=0, levels=LEARNED_
     return pairs
 
 
+@dataclasses.dataclass
+class LevelData:
+    """What the training loss needs of one level of one operator."""
+
+    lv: object  # multigrid.LevelStructure
+    vals32: object
+    dsq: object
+    ks: int
+    feats: object
+
+
 class TrainingSet:
-    """Operators, their per-level features and residual/error pairs."""
+    """Per operator and trained level: features, FP32 operator, scalings and
+    residual/error pairs (copied, so operators of one space may share a
+    multigrid instance)."""
 
     def __init__(self):
-        self.items = []  # (op, [features], [(E_c, R_c)])
+        self.items = []  # [[(LevelData, E_c, R_c) per level]]
 
     def add(self, ctx, vals64, b, n_random=4, levels=LEARNED_LEVELS):
         op = setup(ctx, vals64, None)
         errs = solver_errors(op, ctx.assembly, vals64, b)
         pairs = level_pairs(op, errs, n_random, seed=len(self.items), levels=levels)
-        feats = [features(op, l) for l in range(len(pairs))]
-        self.items.append((op, feats, pairs))
+        item = []
+        for l, (Ec, Rc) in enumerate(pairs):
+            d = LevelData(op.hier.levels[l], op.vals32[l].clone(), op.dsq[l].clone(), op.ks,
+                          features(op, l))
+            item.append((d, Ec, Rc))
+        self.items.append(item)
 
     @property
     def n_pairs(self):
-        return sum(E.shape[0] for _, _, pairs in self.items for E, _ in pairs)
+        return sum(E.shape[0] for item in self.items for _, E, _ in item)
 
 
-def energy_loss(op, l, W, Ec, Rc):
+def energy_loss(d: LevelData, W, Ec, Rc):
     """Loss ``mean ||E_c - S^T R_c||_K^2`` of one level and its gradient with
     respect to the stencil weights ``W (ks*ks, n)``."""
     import torch
 
-    lv = op.hier.levels[l]
+    lv = d.lv
     L = lib()
-    y = stencil_apply(op, l, W, Rc, transpose=True)
+    y = torch.empty_like(Rc)
+    _lib.check(L.iga_stencil_apply(lv.h, lv.w, d.ks, _lib.ptr(W), _lib.ptr(d.dsq), _lib.ptr(Rc), _lib.ptr(y),
+                                   1, 0, Rc.shape[0], stream()), "iga_stencil_apply")
     Z = Ec - y
-    KZ = _op_apply(op, l, Z)
+    KZ = csr_f32(lv.n, lv.indptr, lv.cols, lv.nnz, d.vals32, Z)
     B = Z.shape[0]
     gy = torch.empty_like(Z)
     per = torch.empty(B, dtype=torch.float64, device=Z.device)
@@ -409,9 +428,14 @@ def energy_loss(op, l, W, Ec, Rc):
                                  _lib.ptr(loss), stream()), "iga_energy_loss")
     # gy = d loss / d y (y = S^T R_c enters the loss through Z = E_c - y)
     dW = torch.empty_like(W)
-    _lib.check(L.iga_stencil_wgrad_t(lv.h, lv.w, op.ks, B, _lib.ptr(op.dsq[l]), _lib.ptr(Rc), _lib.ptr(gy),
+    _lib.check(L.iga_stencil_wgrad_t(lv.h, lv.w, d.ks, B, _lib.ptr(d.dsq), _lib.ptr(Rc), _lib.ptr(gy),
                                      _lib.ptr(dW), stream()), "iga_stencil_wgrad_t")
     return loss, dW
+
+
+def level_data(op: MG.MGOperator, l):
+    """:class:`LevelData` view of one level of a loaded multigrid operator."""
+    return LevelData(op.hier.levels[l], op.vals32[l], op.dsq[l], op.ks, None)
 
 
 def train(gen: ConvNet, data: TrainingSet, steps=200, lr=1e-3, batch=None, seed=0, log_every=0):
@@ -430,15 +454,13 @@ def train(gen: ConvNet, data: TrainingSet, steps=200, lr=1e-3, batch=None, seed=
         count = 0
         chosen = range(n_items) if not batch or batch >= n_items else rng.choice(n_items, batch, replace=False)
         for i in chosen:
-            op, feats, pairs = data.items[i]
-            for l, (Ec, Rc) in enumerate(pairs):
+            for d, Ec, Rc in data.items[i]:
                 if Ec.shape[0] == 0:
                     continue
-                W4, act = gen.forward(feats[l])
-                W = W4[0].reshape(op.ks * op.ks, -1)
-                loss, dW = energy_loss(op, l, W, Ec, Rc)
-                lv = op.hier.levels[l]
-                gen.backward(act, dW, 1, lv.h, lv.w)
+                W4, act = gen.forward(d.feats)
+                W = W4[0].reshape(d.ks * d.ks, -1)
+                loss, dW = energy_loss(d, W, Ec, Rc)
+                gen.backward(act, dW, 1, d.lv.h, d.lv.w)
                 total += gen.grads
                 losses.append(loss)
                 count += 1

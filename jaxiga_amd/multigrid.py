@@ -136,6 +136,18 @@ class Hierarchy:
             Kc = sp.csr_matrix(Tpat @ P)
             Kc.data[:] = 1.0
             Kc.sort_indices()
+            # the Galerkin kernel's box: every coarse row's fine stencil must fit
+            # 16 x 16 fine grid points
+            ptr = PT.indptr
+            rows_f = PT.indices // fine.w
+            cols_f = PT.indices % fine.w
+            ext_r = np.maximum.reduceat(rows_f, ptr[:-1]) - np.minimum.reduceat(rows_f, ptr[:-1])
+            ext_c = np.maximum.reduceat(cols_f, ptr[:-1]) - np.minimum.reduceat(cols_f, ptr[:-1])
+            Kcoo = K.tocoo()
+            radius = max(np.abs(Kcoo.col // fine.w - Kcoo.row // fine.w).max(),
+                         np.abs(Kcoo.col % fine.w - Kcoo.row % fine.w).max())
+            if max(ext_r.max(), ext_c.max()) + 1 + 2 * radius > 16 or radius > p:
+                raise NotImplementedError("coarse-row stencil exceeds the Galerkin kernel's box")
             fine.P = P
             fine.p_dev = self._transfer_dev(P, dev)
             fine.pt_dev = self._transfer_dev(PT, dev)
@@ -168,39 +180,45 @@ def hierarchy(ctx, max_coarse=64):
     return cache[key]
 
 
-def galerkin(hier: Hierarchy, vals64):
-    """FP64 operator values of every level: ``K_{l+1} = P_l^T K_l P_l``."""
+def galerkin(hier: Hierarchy, vals64, out=None, tbufs=None):
+    """FP64 operator values of every level, ``K_{l+1} = P_l^T K_l P_l``, into
+    ``out`` (per-level buffers, ``out[0]`` may be ``vals64`` itself) or new
+    tensors; ``tbufs`` are optional buffers for the intermediate ``P^T K``."""
     import torch
 
     L = lib()
     st = stream()
-    out = [vals64]
+    dev = vals64.device
+    if out is None:
+        out = [vals64] + [torch.empty(lv.nnz, dtype=torch.float64, device=dev) for lv in hier.levels[1:]]
+    elif out[0] is not vals64:
+        out[0].copy_(vals64)
     for l in range(hier.n_levels - 1):
         fine, coarse = hier.levels[l], hier.levels[l + 1]
         tp_ptr, tp_cols, tp_rows = fine.t_pat
         pt_ptr, pt_cols, pt_v64, _ = fine.pt_dev
-        tvals = torch.empty(tp_cols.shape[0], dtype=torch.float64, device=vals64.device)
+        tvals = tbufs[l] if tbufs is not None else torch.empty(tp_cols.shape[0], dtype=torch.float64, device=dev)
         # T = P^T K (K symmetric: its CSR is the CSR of K^T)
         _lib.check(L.iga_masked_spgemm(tvals.shape[0], _lib.ptr(tp_rows), _lib.ptr(tp_cols),
                                        _lib.ptr(pt_ptr), _lib.ptr(pt_cols), _lib.ptr(pt_v64),
-                                       _lib.ptr(fine.indptr), _lib.ptr(fine.cols), _lib.ptr(out[-1]),
+                                       _lib.ptr(fine.indptr), _lib.ptr(fine.cols), _lib.ptr(out[l]),
                                        _lib.ptr(tvals), st), "iga_masked_spgemm")
-        kc = torch.empty(coarse.nnz, dtype=torch.float64, device=vals64.device)
-        # K_c = T P (the CSR of P^T is the CSR of (P)^T)
-        _lib.check(L.iga_masked_spgemm(kc.shape[0], _lib.ptr(coarse.rows), _lib.ptr(coarse.cols),
+        # K_c = T P (B = P is given by the CSR of P^T)
+        _lib.check(L.iga_masked_spgemm(coarse.nnz, _lib.ptr(coarse.rows), _lib.ptr(coarse.cols),
                                        _lib.ptr(tp_ptr), _lib.ptr(tp_cols), _lib.ptr(tvals),
                                        _lib.ptr(pt_ptr), _lib.ptr(pt_cols), _lib.ptr(pt_v64),
-                                       _lib.ptr(kc), st), "iga_masked_spgemm")
-        out.append(kc)
+                                       _lib.ptr(out[l + 1]), st), "iga_masked_spgemm")
     return out
 
 
 class MGOperator:
-    """Numeric multigrid for one operator: Galerkin levels in FP64 and FP32,
-    diagonal scalings, the coarsest inverse and the smoothers (stencils per
-    level, or damped Jacobi with ``omega`` where a level has none)."""
+    """Numeric multigrid over persistent device buffers: Galerkin levels in FP64
+    and FP32, diagonal scalings, the coarsest inverse and the smoothers
+    (stencils per level, or damped Jacobi with ``omega`` where a level has
+    none). :meth:`load` rewrites the buffers in place for a new operator, so
+    the captured flexible-CG plans stay valid; one instance per hierarchy."""
 
-    def __init__(self, hier: Hierarchy, vals64, stencils=None, omega=0.5, ks=7):
+    def __init__(self, hier: Hierarchy, omega=0.5, ks=7):
         import torch
 
         from jaxiga_amd import linear as LIN
@@ -208,32 +226,75 @@ class MGOperator:
         self.hier = hier
         self.ks = ks
         self.omega = float(omega)
-        dev = vals64.device
-        self.vals64 = galerkin(hier, vals64)
-        self.vals32 = [v.float() for v in self.vals64]
-        self.dinv = []
-        self.dsq = []
-        for lv, v in zip(hier.levels, self.vals64):
-            pat = LIN.CSRPattern(lv.indptr, lv.cols, lv.rows, lv.n)
-            d = LIN.inverse_diagonal(pat, v)
-            self.dinv.append(d)
-            self.dsq.append(d.sqrt().float())
+        dev = hier.levels[0].indptr.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.patterns = [LIN.CSRPattern(lv.indptr, lv.cols, lv.rows, lv.n) for lv in hier.levels]
+        self.vals64 = [torch.empty(lv.nnz, **f64) for lv in hier.levels]
+        self.vals32 = [torch.empty(lv.nnz, **f32) for lv in hier.levels]
+        self.dinv = [torch.empty(lv.n, **f64) for lv in hier.levels]
+        self.dsq = [torch.empty(lv.n, **f32) for lv in hier.levels]
         c = hier.levels[-1]
-        work = torch.empty(2 * c.n * c.n, dtype=torch.float64, device=dev)
-        self.coarse_inv = torch.empty(c.n * c.n, dtype=torch.float32, device=dev)
-        _lib.check(lib().iga_dense_inverse(c.n, _lib.ptr(c.indptr), _lib.ptr(c.cols),
-                                           _lib.ptr(self.vals64[-1]), _lib.ptr(work),
-                                           _lib.ptr(self.coarse_inv), stream()), "iga_dense_inverse")
+        self._inv_work = torch.empty(2 * c.n * c.n, **f64)
+        self.coarse_inv = torch.empty(c.n * c.n, **f32)
+        self._stencil_buf = {}
         self.stencils = [None] * hier.n_levels
-        if stencils is not None:
-            self.set_stencils(stencils)
+        n0 = hier.levels[0].n
+        self.b = torch.empty(n0, **f64)
+        self.x = torch.empty(n0, **f64)
+        self._fwork = torch.empty(int(lib().iga_fcg_workspace(n0)), **f64)
+        self._fwork32 = torch.empty(2 * n0, **f32)
         self._work = None
         self._struct = None
+        self._setup = None
+        self._plans = {}
+
+    def _setup_struct(self):
+        if getattr(self, "_setup", None) is None:
+            h = self.hier
+            m = _lib.IgaMgSetup()
+            m.n_levels = h.n_levels
+            m.radius = h.p
+            for l, lv in enumerate(h.levels):
+                L = m.level[l]
+                L.n, L.h, L.w, L.nnz = lv.n, lv.h, lv.w, lv.nnz
+                L.indptr, L.cols = lv.indptr.data_ptr(), lv.cols.data_ptr()
+                L.vals64, L.vals32 = self.vals64[l].data_ptr(), self.vals32[l].data_ptr()
+                L.dinv, L.dsq = self.dinv[l].data_ptr(), self.dsq[l].data_ptr()
+                if lv.pt_dev is not None:
+                    ip, cc, v64, _ = lv.pt_dev
+                    L.pt_ptr, L.pt_cols, L.pt_vals = ip.data_ptr(), cc.data_ptr(), v64.data_ptr()
+            m.coarse_inv = self.coarse_inv.data_ptr()
+            m.inv_work = self._inv_work.data_ptr()
+            self._setup = m
+        return self._setup
+
+    def load(self, vals64, stencils=None):
+        """Write a new level-0 operator (FP64 CSR values) and its smoothers:
+        one library call forms every Galerkin level, the FP32 copies, the
+        diagonal scalings and the coarsest inverse."""
+        _lib.check(lib().iga_mg_load(self._setup_struct(), _lib.ptr(vals64), stream()), "iga_mg_load")
+        self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
+        return self
 
     def set_stencils(self, stencils):
-        for l, s in enumerate(stencils):
-            self.stencils[l] = s
-        self._struct = None
+        import torch
+
+        new = []
+        for l, s in enumerate(list(stencils) + [None] * (self.hier.n_levels - len(stencils))):
+            if s is None:
+                new.append(None)
+                continue
+            buf = self._stencil_buf.get(l)
+            if buf is None:
+                buf = torch.empty(self.ks * self.ks * self.hier.levels[l].n, dtype=torch.float32,
+                                  device=self.b.device)
+                self._stencil_buf[l] = buf
+            buf.copy_(s.reshape(-1))
+            new.append(buf)
+        if [x is None for x in new] != [x is None for x in self.stencils]:
+            self._struct = None
+        self.stencils = new
 
     def struct(self):
         import torch
@@ -259,7 +320,7 @@ class MGOperator:
         m.coarse_inv = self.coarse_inv.data_ptr()
         if self._work is None:
             nw = int(lib().iga_mg_work_floats(m))
-            self._work = torch.empty(max(nw, 1), dtype=torch.float32, device=self.coarse_inv.device)
+            self._work = torch.empty(max(nw, 1), dtype=torch.float32, device=self.b.device)
         m.work = self._work.data_ptr()
         self._struct = m
         return m
@@ -272,25 +333,55 @@ class MGOperator:
                    "iga_mg_vcycle")
         return x
 
+    def _plan(self, graph_iters):
+        key = (tuple(s is None for s in self.stencils), graph_iters)
+        plan = self._plans.get(key)
+        if plan is None:
+            lv = self.hier.levels[0]
+            handle = _lib.c_void_p()
+            _lib.check(lib().iga_fcg_mg_plan(self.struct(), lv.n, lv.nnz, _lib.ptr(lv.indptr),
+                                             _lib.ptr(lv.cols), _lib.ptr(self.vals64[0]), _lib.ptr(self.b),
+                                             _lib.ptr(self.x), _lib.ptr(self._fwork), _lib.ptr(self._fwork32),
+                                             int(graph_iters), stream(), _lib.ctypes.byref(handle)),
+                       "iga_fcg_mg_plan")
+            plan = _Plan(handle)
+            self._plans[key] = plan
+        return plan
+
     def fcg(self, b, tol=1e-10, maxiter=1000, graph_iters=8):
         """Flexible CG on the level-0 FP64 operator with one V-cycle per
-        iteration; returns ``(x, iterations, rr, bb)``."""
+        iteration; returns ``(x, iterations, rr, bb)`` (x a new tensor)."""
         import torch
 
-        L = lib()
-        lv = self.hier.levels[0]
-        n = lv.n
-        work = torch.empty(int(L.iga_fcg_workspace(n)), dtype=torch.float64, device=b.device)
-        work32 = torch.empty(2 * n, dtype=torch.float32, device=b.device)
-        x = torch.empty(n, dtype=torch.float64, device=b.device)
         it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
-        m = self.struct()
         side = torch.cuda.Stream() if graph_iters > 0 else torch.cuda.current_stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):  # graph capture needs a non-default stream
-            _lib.check(L.iga_fcg_mg(m, n, _lib.ptr(lv.indptr), _lib.ptr(lv.cols), _lib.ptr(self.vals64[0]),
-                                    _lib.ptr(b), _lib.ptr(x), float(tol), int(maxiter), int(graph_iters),
-                                    _lib.ptr(work), _lib.ptr(work32), _lib.ctypes.byref(it),
-                                    _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()), "iga_fcg_mg")
+            self.b.copy_(b)
+            plan = self._plan(graph_iters)
+            _lib.check(lib().iga_fcg_mg_run(plan.handle, float(tol), int(maxiter), _lib.ctypes.byref(it),
+                                            _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()),
+                       "iga_fcg_mg_run")
+            x = self.x.clone()
         torch.cuda.current_stream().wait_stream(side)
         return x, it.value, rr.value, bb.value
+
+
+class _Plan:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        try:
+            lib().iga_fcg_mg_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def operator(hier: Hierarchy, omega=0.5, ks=7):
+    """The persistent :class:`MGOperator` of a hierarchy."""
+    key = (omega, ks)
+    ops = hier.__dict__.setdefault("_operators", {})
+    if key not in ops:
+        ops[key] = MGOperator(hier, omega, ks)
+    return ops[key]

@@ -128,14 +128,16 @@ def test_stencil_kernels_and_loss_match_oracle(gpu):
     for tr in (False, True):
         y = LN.stencil_apply(op, l, Wd, Ed, transpose=tr).double().cpu().numpy()
         assert _rel(y, OM.stencil_apply(W, dsq, E, lv.h, lv.w, 7, transpose=tr)) < 1e-5
-    loss, dW = LN.energy_loss(op, l, Wd, Ed, Rd)
+    loss, dW = LN.energy_loss(LN.level_data(op, l), Wd, Ed, Rd)
     loss_ref, dW_ref = OM.energy_loss(Kr, W, dsq, E, R, lv.h, lv.w, 7)
     assert abs(float(loss) - loss_ref) <= 1e-4 * abs(loss_ref)
     assert _rel(dW.double().cpu().numpy(), dW_ref) < 1e-4
 
 
-def test_vcycle_matches_oracle(gpu):
-    ctx, vals, rhs = _problem(5, 6)
+@pytest.mark.parametrize("refine", [5, 7])
+def test_vcycle_matches_oracle(gpu, refine):
+    # refine 5: four levels; refine 7: six levels
+    ctx, vals, rhs = _problem(refine, 6)
     ref = _oracle_levels(ctx, vals)
     rng = np.random.default_rng(9)
     stencils = [rng.normal(size=(49, lv["K"].shape[0])) * 0.05 for lv in ref[:-1]]

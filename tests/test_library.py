@@ -16,7 +16,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def declared_symbols():
     text = (ROOT / "include" / "jaxiga_hip.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t)\s+(iga_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|void)\s+(iga_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_the_bound_symbols():
