@@ -240,7 +240,9 @@ int iga_inv_guard(int32_t n, const double* d, double* inv, void* stream);
  * woff[l]   : offset (floats) of the layer's weights in the parameter buffer
  * boff[l]   : offset of the layer's bias (cout floats), or -1 for none
  * relu[l]   : 1 applies max(0, .) after the bias (and after the skip add)
- * H/W[0]    : input size; H/W[l+1]: output size of layer l                 */
+ * H/W[0]    : input size; H/W[l+1]: output size of layer l
+ * precision : 0 FP32 throughout; 1 FP16 operands (FP32 accumulation) for the
+ *             stride-1 convolutions of the forward pass (inference)        */
 typedef struct iga_net {
   int32_t n_layers;
   int32_t kind[16], cin[16], cout[16], stride[16], src[16], skip[16];
@@ -249,6 +251,7 @@ typedef struct iga_net {
   int32_t c_in;
   int32_t relu[16];
   int64_t boff[16];
+  int32_t precision;
 } iga_net_t;
 
 /* floats of activation buffer for `batch` samples (input at offset 0, the

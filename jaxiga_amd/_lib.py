@@ -58,7 +58,7 @@ class IgaNet(ctypes.Structure):
         ("n_layers", c_int), ("kind", c_int * 16), ("cin", c_int * 16), ("cout", c_int * 16),
         ("stride", c_int * 16), ("src", c_int * 16), ("skip", c_int * 16), ("woff", c_i64 * 16),
         ("H", c_int * 17), ("W", c_int * 17), ("c_in", c_int), ("relu", c_int * 16),
-        ("boff", c_i64 * 16),
+        ("boff", c_i64 * 16), ("precision", c_int),
     ]
 
 

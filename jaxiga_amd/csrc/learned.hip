@@ -57,6 +57,144 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Stride-1 3x3 convolution as an implicit GEMM on the FP32 matrix cores:
+//   out[n][co][p] = act(sum_kk W[co][kk] X[kk][p] + bias[co]),
+//   kk = ci*9 + k, X[kk][p] = in[n][ci][p + off(k)]  (zero outside the image).
+// A block computes a row segment of 64 pixels for every output channel
+// (Cout <= 64, Cin <= 64): the input rows y-1..y+1 of the segment are staged
+// in LDS; each wave owns 16 pixels and four 16x16 output-channel tiles, and
+// each step of 4 along kk issues four v_mfma_f32_16x16x4f32 sharing the
+// pixel operand. Output mapping: lane l holds pixel l%16 and channels
+// 4*(l/16)+r of each tile.
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int kConvSeg = 64;
+constexpr int kConvMaxC = 64;
+
+constexpr int kConvCh = 8;                 // input channels per weight chunk
+constexpr int kConvKc = kConvCh * 9;       // kk per chunk (a multiple of 4)
+constexpr int kConvWs = kConvKc + 1;       // padded LDS row of the weight chunk
+
+__global__ void __launch_bounds__(256)
+    k_conv_mfma(int Cin, int H, int W, int Cout, const float* __restrict__ in,
+                const float* __restrict__ w, const float* __restrict__ bias, int relu,
+                float* __restrict__ out) {
+  __shared__ float tile[kConvMaxC][3][kConvSeg + 2];
+  __shared__ float wsm[kConvMaxC][kConvWs];
+  const int x0 = blockIdx.x * kConvSeg, y = blockIdx.y, n = blockIdx.z;
+  const float* inb = in + (size_t)n * Cin * H * W;
+  for (int t = threadIdx.x; t < Cin * 3 * (kConvSeg + 2); t += blockDim.x) {
+    const int xx = t % (kConvSeg + 2), dy = (t / (kConvSeg + 2)) % 3, ci = t / (3 * (kConvSeg + 2));
+    const int gy = y + dy - 1, gx = x0 + xx - 1;
+    tile[ci][dy][xx] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? inb[((size_t)ci * H + gy) * W + gx] : 0.f;
+  }
+  const int wave = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int px = wave * 16 + l % 16;  // pixel of the B operand and of the output
+  const int kq = l / 16;              // row of the 4-deep k slice this lane feeds
+  const int K = Cin * 9;
+  f4v acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < Cin; ci0 += kConvCh) {
+    __syncthreads();  // the previous chunk's weights are consumed (and the tile is staged)
+    for (int t = threadIdx.x; t < kConvMaxC * kConvKc; t += blockDim.x) {
+      const int co = t / kConvKc, kl = t - co * kConvKc;
+      const int kk = ci0 * 9 + kl;
+      wsm[co][kl] = (co < Cout && kk < K) ? w[(size_t)co * K + kk] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kl0 = 0; kl0 < kConvKc; kl0 += 4) {
+      const int kl = kl0 + kq;
+      const int ci = ci0 + kl / 9, k = kl % 9;
+      const float bv = ci < Cin ? tile[ci][k / 3][px + k % 3] : 0.f;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wsm[t * 16 + l % 16][kl], bv, acc[t], 0, 0, 0);
+    }
+  }
+  const int x = x0 + px;
+  if (x >= W) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = t * 16 + 4 * kq + r;
+      if (co >= Cout) continue;
+      float v = acc[t][r];
+      if (bias) v += bias[co];
+      if (relu) v = fmaxf(v, 0.f);
+      out[(((size_t)n * Cout + co) * H + y) * W + x] = v;
+    }
+}
+
+// The same convolution with FP16 operands on v_mfma_f32_16x16x16f16 (FP32
+// accumulation, bias and ReLU): lane l feeds A[l%16][4(l/16)..+3] and
+// B[4(l/16)..+3][l%16] per 16-deep step, four output-channel tiles share the
+// pixel operand. Inputs and weights are rounded to FP16 when staged in LDS.
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+constexpr int kConvChH = 16;              // input channels per weight chunk
+constexpr int kConvKcH = kConvChH * 9;    // 144 kk per chunk, a multiple of 16
+constexpr int kConvWsH = kConvKcH + 4;    // padded LDS row (8-byte aligned)
+
+__global__ void __launch_bounds__(256)
+    k_conv_mfma_f16(int Cin, int H, int W, int Cout, const float* __restrict__ in,
+                    const float* __restrict__ w, const float* __restrict__ bias, int relu,
+                    float* __restrict__ out) {
+  __shared__ _Float16 tile[kConvMaxC][3][kConvSeg + 2];
+  __shared__ __attribute__((aligned(8))) _Float16 wsm[kConvMaxC][kConvWsH];
+  const int x0 = blockIdx.x * kConvSeg, y = blockIdx.y, n = blockIdx.z;
+  const float* inb = in + (size_t)n * Cin * H * W;
+  for (int t = threadIdx.x; t < Cin * 3 * (kConvSeg + 2); t += blockDim.x) {
+    const int xx = t % (kConvSeg + 2), dy = (t / (kConvSeg + 2)) % 3, ci = t / (3 * (kConvSeg + 2));
+    const int gy = y + dy - 1, gx = x0 + xx - 1;
+    tile[ci][dy][xx] = (_Float16)((gy >= 0 && gy < H && gx >= 0 && gx < W)
+                                      ? inb[((size_t)ci * H + gy) * W + gx] : 0.f);
+  }
+  const int wave = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int px = wave * 16 + l % 16;
+  const int kq = l / 16;
+  const int K = Cin * 9;
+  f4v acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < Cin; ci0 += kConvChH) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kConvMaxC * kConvKcH; t += blockDim.x) {
+      const int co = t / kConvKcH, kl = t - co * kConvKcH;
+      const int kk = ci0 * 9 + kl;
+      wsm[co][kl] = (_Float16)((co < Cout && kk < K) ? w[(size_t)co * K + kk] : 0.f);
+    }
+    __syncthreads();
+    for (int kl0 = 0; kl0 < kConvKcH; kl0 += 16) {
+      h4v bv;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int kl = kl0 + 4 * kq + j;
+        const int ci = ci0 + kl / 9, k = kl % 9;
+        bv[j] = ci < Cin ? tile[ci][k / 3][px + k % 3] : (_Float16)0.f;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const h4v av = *reinterpret_cast<const h4v*>(&wsm[t * 16 + l % 16][kl0 + 4 * kq]);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const int x = x0 + px;
+  if (x >= W) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = t * 16 + 4 * kq + r;
+      if (co >= Cout) continue;
+      float v = acc[t][r];
+      if (bias) v += bias[co];
+      if (relu) v = fmaxf(v, 0.f);
+      out[(((size_t)n * Cout + co) * H + y) * W + x] = v;
+    }
+}
+
 // Transposed convolution, weights (Cin, Cout, 3, 3) (Cin = channels of `in`):
 // out[n][co][Y][X] (+)= sum_ci sum_k w[ci][co][k] in[n][ci][y][x], Y = s*y+ky-1.
 template <int COT>
@@ -312,6 +450,20 @@ static int net_forward(const iga_net_t& net, int batch, const float* params, flo
     const float* w = params + net.woff[l];
     if (Ci != net.cin[l]) return IGA_ERR_SHAPE;
     const int64_t n = act_size(net, l, batch);
+    const float* bias = net.boff[l] >= 0 ? params + net.boff[l] : nullptr;
+    if (net.kind[l] == 0 && s == 1 && Ci <= kConvMaxC && Co <= kConvMaxC) {
+      // stride 1 on the matrix cores (FP32, or FP16 operands when the
+      // descriptor asks for it), bias and ReLU in the epilogue
+      dim3 grid((Wo + kConvSeg - 1) / kConvSeg, Ho, batch);
+      if (net.precision == 1)
+        hipLaunchKernelGGL(k_conv_mfma_f16, grid, dim3(256), 0, st, Ci, Hi, Wi, Co, A(src), w, bias,
+                           net.relu[l], A(l));
+      else
+        hipLaunchKernelGGL(k_conv_mfma, grid, dim3(256), 0, st, Ci, Hi, Wi, Co, A(src), w, bias,
+                           net.relu[l], A(l));
+      IGA_CHECK_LAUNCH();
+      continue;
+    }
     if (net.kind[l] == 0) {
       launch_conv(batch, Ci, Hi, Wi, Co, Ho, Wo, s, A(src), w, A(l), 0, st);
     } else {

@@ -89,11 +89,30 @@ __global__ void __launch_bounds__(256)
     cmin = min(cmin, a % wf);
   }
   const int r0 = wave_min(rmin) - R, c0 = wave_min(cmin) - R;
+  // the fine rows of P^T row I and their K-row bounds, fetched once, one per lane
+  int my_q0 = 0, my_q1 = 0;
+  double my_pa = 0.0;
+  if (lane < a1 - a0 && lane < kWave) {
+    const int a = pt_cols[a0 + lane];
+    my_pa = pt_vals[a0 + lane];
+    my_q0 = k_ptr[a];
+    my_q1 = k_ptr[a + 1];
+  }
   __builtin_amdgcn_wave_barrier();
-  for (int k = a0; k < a1; ++k) {
-    const int a = pt_cols[k];
-    const double pa = pt_vals[k];
-    for (int q = k_ptr[a] + lane; q < k_ptr[a + 1]; q += kWave) {
+  for (int k = 0; k < a1 - a0; ++k) {
+    int q0, q1;
+    double pa;
+    if (k < kWave) {
+      q0 = __shfl(my_q0, k, kWave);
+      q1 = __shfl(my_q1, k, kWave);
+      pa = __shfl(my_pa, k, kWave);
+    } else {
+      const int a = pt_cols[a0 + k];
+      pa = pt_vals[a0 + k];
+      q0 = k_ptr[a];
+      q1 = k_ptr[a + 1];
+    }
+    for (int q = q0 + lane; q < q1; q += kWave) {
       const int b = k_cols[q];
       const int br = b / wf - r0, bc = b % wf - c0;
       box[br * kGalBox + bc] += pa * k_vals[q];
@@ -140,19 +159,20 @@ __global__ void k_copy_d(int64_t n, const double* __restrict__ a, double* __rest
 }
 
 // feats[o][i] = K[i, i + off(o)] / K[i, i] on the h x w grid, o over the
-// (2R+1)^2 offsets (feats cleared beforehand).
-__global__ void k_stencil_features(int n, int w, int R, const int32_t* __restrict__ indptr,
-                                   const int32_t* __restrict__ cols,
-                                   const double* __restrict__ vals, float* __restrict__ feats) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// (2R+1)^2 offsets (feats cleared beforehand). 16 lanes per row (sorted
+// columns: the diagonal by binary search), one entry per lane.
+__global__ void __launch_bounds__(256)
+    k_stencil_features(int n, int w, int R, const int32_t* __restrict__ indptr,
+                       const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                       float* __restrict__ feats) {
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / 16, l = threadIdx.x % 16;
   if (i >= n) return;
   const int lo = indptr[i], hi = indptr[i + 1];
-  double d = 1.0;
-  for (int k = lo; k < hi; ++k)
-    if (cols[k] == i) d = vals[k];
+  const int kd = lower_bound(cols, lo, hi, i);
+  double d = (kd < hi && cols[kd] == i) ? vals[kd] : 0.0;
   if (d == 0.0) d = 1.0;
   const int ri = i / w, ci = i - ri * w, B = 2 * R + 1;
-  for (int k = lo; k < hi; ++k) {
+  for (int k = lo + l; k < hi; k += 16) {
     const int j = cols[k];
     const int dr = j / w - ri, dc = j % w - ci;
     if (dr < -R || dr > R || dc < -R || dc > R) continue;
@@ -209,7 +229,13 @@ __global__ void k_inverse_f32(int n, const double* __restrict__ A, float* __rest
 
 __device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip; }
 
-// out (=|+=) D^-1/2 T (D^-1/2 v), T = T_W or its transpose; batch over blockIdx.y
+// out (=|+=) D^-1/2 T (D^-1/2 v), T = T_W or its transpose; batch over
+// blockIdx.z. A block computes a 16 x 16 tile of the grid: u = D^-1/2 v over
+// the tile and its halo is staged in LDS (zero outside the grid), so the tap
+// loop has no bounds tests on u; the weights stream from HBM, coalesced
+// along the tile rows.
+constexpr int kStTile = 16;
+
 template <int KS, bool TR>
 __global__ void __launch_bounds__(256)
     k_stencil(int h, int w, const float* __restrict__ W, const float* __restrict__ dsq,
@@ -217,27 +243,37 @@ __global__ void __launch_bounds__(256)
               const int* __restrict__ skip) {
   if (skipped(skip)) return;
   constexpr int R = KS / 2;
+  constexpr int T = kStTile + 2 * R;
+  __shared__ float u[T][T];
   const int n = h * w;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* vb = v + (size_t)blockIdx.y * n;
-  const int r = i / w, c = i - r * w;
+  const float* vb = v + (size_t)blockIdx.z * n;
+  const int r0 = blockIdx.y * kStTile, c0 = blockIdx.x * kStTile;
+  for (int t = threadIdx.x; t < T * T; t += blockDim.x) {
+    const int rr = r0 + t / T - R, cc = c0 + t % T - R;
+    u[t / T][t % T] = (rr >= 0 && rr < h && cc >= 0 && cc < w) ? dsq[rr * w + cc] * vb[rr * w + cc] : 0.f;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / kStTile, tx = threadIdx.x % kStTile;
+  const int r = r0 + ty, c = c0 + tx;
+  if (r >= h || c >= w) return;
+  const int i = r * w + c;
   float acc = 0.f;
 #pragma unroll
   for (int dy = -R; dy <= R; ++dy) {
-    const int rr = TR ? r - dy : r + dy;
-    if (rr < 0 || rr >= h) continue;
 #pragma unroll
     for (int dx = -R; dx <= R; ++dx) {
-      const int cc = TR ? c - dx : c + dx;
-      if (cc < 0 || cc >= w) continue;
       const int o = (dy + R) * KS + (dx + R);
-      const int j = rr * w + cc;
-      const float wt = W[(size_t)o * n + (TR ? j : i)];
-      acc += wt * dsq[j] * vb[j];
+      if (!TR) {
+        acc += W[(size_t)o * n + i] * u[ty + dy + R][tx + dx + R];
+      } else {
+        const int rr = r - dy, cc = c - dx;
+        const bool in = rr >= 0 && rr < h && cc >= 0 && cc < w;
+        const float wt = in ? W[(size_t)o * n + rr * w + cc] : 0.f;
+        acc += wt * u[ty - dy + R][tx - dx + R];
+      }
     }
   }
-  float* ob = out + (size_t)blockIdx.y * n;
+  float* ob = out + (size_t)blockIdx.z * n;
   const float val = dsq[i] * acc;
   ob[i] = accumulate ? ob[i] + val : val;
 }
@@ -458,14 +494,15 @@ static int smooth(const iga_mg_t& mg, int l, const float* v, float* out, bool tr
     IGA_CHECK_LAUNCH();
     return IGA_OK;
   }
+  const dim3 grid((L.w + kStTile - 1) / kStTile, (L.h + kStTile - 1) / kStTile, 1);
   return by_ks(mg.ks, [&](auto K) {
     constexpr int KS = decltype(K)::value;
     if (transpose)
-      hipLaunchKernelGGL((k_stencil<KS, true>), dim3(blocks_for(L.n)), dim3(256), 0, st, L.h, L.w,
-                         L.stencil, L.dsq, v, out, accumulate, skip);
+      hipLaunchKernelGGL((k_stencil<KS, true>), grid, dim3(256), 0, st, L.h, L.w, L.stencil, L.dsq,
+                         v, out, accumulate, skip);
     else
-      hipLaunchKernelGGL((k_stencil<KS, false>), dim3(blocks_for(L.n)), dim3(256), 0, st, L.h,
-                         L.w, L.stencil, L.dsq, v, out, accumulate, skip);
+      hipLaunchKernelGGL((k_stencil<KS, false>), grid, dim3(256), 0, st, L.h, L.w, L.stencil,
+                         L.dsq, v, out, accumulate, skip);
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
   });
@@ -763,8 +800,8 @@ extern "C" int iga_stencil_features(int32_t n, int32_t w, int32_t radius, const 
   const int64_t total = (int64_t)(2 * radius + 1) * (2 * radius + 1) * n;
   hipLaunchKernelGGL(k_zero_range, dim3((int)std::min<int64_t>(blocks_for(total), 4096)), dim3(256),
                      0, st, total, feats);
-  hipLaunchKernelGGL(k_stencil_features, dim3(blocks_for(n)), dim3(256), 0, st, n, w, radius,
-                     indptr, cols, vals, feats);
+  hipLaunchKernelGGL(k_stencil_features, dim3(blocks_for((int64_t)n * 16)), dim3(256), 0, st, n, w,
+                     radius, indptr, cols, vals, feats);
   IGA_CHECK_LAUNCH();
   return IGA_OK;
 }
@@ -811,15 +848,15 @@ extern "C" int iga_stencil_apply(int32_t h, int32_t w, int32_t ks, const float* 
   if ((int64_t)h * w == 0 || batch == 0) return IGA_OK;
   if (!stencil || !dsq || !v || !out || h < 0 || w < 0) return IGA_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  const int n = h * w;
+  const dim3 grid((w + kStTile - 1) / kStTile, (h + kStTile - 1) / kStTile, batch);
   return by_ks(ks, [&](auto K) {
     constexpr int KS = decltype(K)::value;
     if (transpose)
-      hipLaunchKernelGGL((k_stencil<KS, true>), dim3(blocks_for(n), batch), dim3(256), 0, st, h, w,
-                         stencil, dsq, v, out, accumulate, nullptr);
+      hipLaunchKernelGGL((k_stencil<KS, true>), grid, dim3(256), 0, st, h, w, stencil, dsq, v, out,
+                         accumulate, nullptr);
     else
-      hipLaunchKernelGGL((k_stencil<KS, false>), dim3(blocks_for(n), batch), dim3(256), 0, st, h,
-                         w, stencil, dsq, v, out, accumulate, nullptr);
+      hipLaunchKernelGGL((k_stencil<KS, false>), grid, dim3(256), 0, st, h, w, stencil, dsq, v,
+                         out, accumulate, nullptr);
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
   });

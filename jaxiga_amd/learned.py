@@ -115,13 +115,14 @@ class ConvNet:
             else:
                 self.weights(name).copy_(torch.as_tensor(np.asarray(arr, np.float32)))
 
-    def struct(self, H, W):
-        key = (H, W)
+    def struct(self, H, W, precision=0):
+        key = (H, W, precision)
         if key in self._structs:
             return self._structs[key]
         s = _lib.IgaNet()
         s.n_layers = len(self.arch)
         s.c_in = self.c_in
+        s.precision = precision
         names = [l[0] for l in self.arch]
         sizes = {"x": (H, W)}
         s.H[0], s.W[0] = H, W
@@ -142,14 +143,16 @@ class ConvNet:
         self._structs[key] = s
         return s
 
-    def forward(self, x):
-        """``x (B, c_in, H, W)`` float32 on the GPU -> ``(output view, activations)``."""
+    def forward(self, x, precision=0):
+        """``x (B, c_in, H, W)`` float32 on the GPU -> ``(output view, activations)``.
+        ``precision=1`` runs the stride-1 convolutions with FP16 operands on
+        the matrix cores (inference)."""
         import torch
 
         B, C, H, W = x.shape
         if C != self.c_in:
             raise ValueError(f"network expects {self.c_in} input channels, got {C}")
-        st = self.struct(H, W)
+        st = self.struct(H, W, precision)
         L = lib()
         act = torch.empty(int(L.iga_net_act_floats(st, B)), dtype=torch.float32, device=x.device)
         act[: x.numel()].copy_(x.reshape(-1))
@@ -252,7 +255,7 @@ def smoother_stencils(gen: ConvNet, op: MG.MGOperator, levels=LEARNED_LEVELS):
     out = []
     for l in range(op.hier.n_levels - 1):
         if l < levels:
-            y, _ = gen.forward(features(op, l))
+            y, _ = gen.forward(features(op, l), precision=1)
             out.append(y[0].reshape(op.ks * op.ks, -1).contiguous())
         else:
             out.append(None)

@@ -65,6 +65,9 @@ def test_network_forward_backward_match_oracle(gpu, kind):
             got = net.grads[o: o + ref.size].view(ref.shape)
         assert _rel(got.double().cpu().numpy(), ref) < 1e-4, name
     assert _rel(gx.double().cpu().numpy(), gx_ref) < 1e-4
+    # inference with FP16 operands on the matrix cores (FP32 accumulation)
+    y16, _ = net.forward(torch.as_tensor(x, dtype=torch.float32, device="cuda"), precision=1)
+    assert _rel(y16.double().cpu().numpy(), y_ref) < 1e-2
 
 
 def _problem(refine, seed, sigma=1.5, ell=0.1):
