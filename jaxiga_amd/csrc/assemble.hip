@@ -103,33 +103,48 @@ struct Shape {
   static constexpr int TI = (NLV + 15) / 16;
 };
 
+// Element pass. Per element (one workgroup): extracted 1-D tables, the
+// rational basis and physical gradients at every Gauss point (FP64), then the
+// element matrix in gradient-product form on the matrix cores:
+//   M_cd[i][j] = sum_q G_c[q][i] (w_q G_d[q][j])      (G_c = d R / d x_c)
+//   Poisson:    K_e[i][j] = sum_c M_cc[i][j]  with a(x_q) folded into w_q
+//   elasticity: K_e[(i,a),(j,b)] = sum_cd E^cd_ab M_cd[i][j],
+//               E^cd_ab = sum_st C_st Bs(s; a, c) Bs(t; b, d)
+// where Bs(s; a, c) = 1 when the engineering-Voigt strain s contains
+// d u_a / d x_c. Each wave owns 16x16 tiles of (i, j) and keeps every M_cd of
+// its tile in accumulators, so the constitutive combination, the lifting
+// -K_fp g and the diagonal are done in registers; the element matrix is
+// streamed to the element-matrix buffer.
 template <int DIM, int P, int PROB, typename T>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(256)
     k_element(iga_patch_t pt, int nq1, const double* __restrict__ qp,
               const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all) {
   using S = Shape<DIM, P, PROB>;
-  constexpr int N1 = S::N1, NL = S::NL, NLP = S::NLP, NS = S::NS, NLV = S::NLV, TI = S::TI;
+  constexpr int N1 = S::N1, NL = S::NL, NLP = S::NLP, NS = S::NS, NLV = S::NLV, VEC = S::VEC;
+  constexpr int NLT = (NL + 15) / 16 * 16;  // tile-padded basis count
+  constexpr int TI = NLT / 16;
+  constexpr int NCD = PROB == IGA_ELASTICITY ? DIM * DIM : 1;
   const int nq = DIM == 2 ? nq1 * nq1 : nq1 * nq1 * nq1;
-  const int KK = nq * NS;
-  const int KKp = (KK + 3) & ~3;
+  const int nqp = (nq + 3) & ~3;
 
   extern __shared__ double smem[];
   double* N1t = smem;                   // [DIM][nq1][N1]
   double* dN1t = N1t + DIM * nq1 * N1;  // [DIM][nq1][N1]
-  // G holds the physical gradients [(q*DIM + d)][NL], zero-padded to KKp rows
-  // for the Poisson contraction, which uses it directly as its strain table.
-  const int Grows = PROB == IGA_POISSON ? KKp : nq * DIM;
-  double* G = dN1t + DIM * nq1 * N1;    // [Grows][NL]
-  double* Rs = G + Grows * NL;          // [nq][NL]
-  double* Wk = Rs + nq * NL;            // [nq] weight * coefficient
-  double* Wf = Wk + nq;                 // [nq] weight
-  T* Hm = reinterpret_cast<T*>(Wf + nq);             // [KKp][NLV] = (W D) Bm
-  int* fdof = reinterpret_cast<int*>(Hm + KKp * NLV + 2);  // [NL*3] free index or -1
-  int* gdof = fdof + NL * 3;                         // [NL] scalar dofs
-  // constitutive matrix and constant source staged in LDS: indexing the
+  double* Rs = dN1t + DIM * nq1 * N1;   // [nq][NL]
+  double* Wf = Rs + nq * NL;            // [nq] weight
+  T* GA = reinterpret_cast<T*>(Wf + nq);  // [DIM][nqp][NLT] gradients
+  T* GW = GA + DIM * nqp * NLT;           // [DIM][nqp][NLT] weighted gradients
+  int* fdof = reinterpret_cast<int*>(GW + DIM * nqp * NLT);  // [NL*3] free index or -1
+  int* gdof = fdof + NL * 3;                                 // [NL] scalar dofs
+  // small elasticity elements: each wave forms one M_cd tile into LDS and all
+  // threads combine the entries; larger ones combine in registers
+  constexpr bool kLdsCombine = PROB == IGA_ELASTICITY && NLT == 16 && DIM == 2;
+  double* Mb = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(gdof + NL) + 7) &
+                                         ~(uintptr_t)7);  // [DIM*DIM][16][16]
+  // constitutive data and constant source staged in LDS: indexing the
   // kernel-argument copies with runtime indices would place the whole
   // argument block in per-lane scratch memory
-  __shared__ double cm[36], srcc[3];
+  __shared__ double cm[36], srcc[3], Ecd[81];
   if (threadIdx.x < 36) cm[threadIdx.x] = a.cmat[threadIdx.x];
   if (threadIdx.x < 3) srcc[threadIdx.x] = a.src_const[threadIdx.x];
 
@@ -150,7 +165,7 @@ __global__ void __launch_bounds__(1024)
     }
   }
 
-  // ---- 1-D extracted tables -------------------------------------------
+  // ---- 1-D extracted tables, zeroed operand tiles --------------------------
   for (int t = threadIdx.x; t < DIM * nq1 * N1; t += blockDim.x) {
     const int ai = t % N1;
     const int q = (t / N1) % nq1;
@@ -167,6 +182,28 @@ __global__ void __launch_bounds__(1024)
     const double h = pt.hi[d][ei[d]] - pt.lo[d][ei[d]];
     N1t[t] = s;
     dN1t[t] = ds * (2.0 / h);
+  }
+  for (int t = threadIdx.x; t < 2 * DIM * nqp * NLT; t += blockDim.x) GA[t] = (T)0;
+  if constexpr (PROB == IGA_ELASTICITY) {
+    // E^cd_ab from the Voigt matrix
+    if (threadIdx.x < DIM * DIM * DIM * DIM) {
+      const int t = threadIdx.x;
+      const int bb = t % DIM, aa = (t / DIM) % DIM, dd = (t / (DIM * DIM)) % DIM, cc = t / (DIM * DIM * DIM);
+      double v = 0.0;
+      for (int s1 = 0; s1 < NS; ++s1) {
+        int p1, q1;
+        voigt<DIM>(s1, p1, q1);
+        const bool in1 = (aa == p1 && cc == q1) || (p1 != q1 && aa == q1 && cc == p1);
+        if (!in1) continue;
+        for (int s2 = 0; s2 < NS; ++s2) {
+          int p2, q2;
+          voigt<DIM>(s2, p2, q2);
+          const bool in2 = (bb == p2 && dd == q2) || (p2 != q2 && bb == q2 && dd == p2);
+          if (in2) v += cm[s1 * NS + s2];
+        }
+      }
+      Ecd[t] = v;
+    }
   }
 
   // ---- lane -> local basis function ------------------------------------
@@ -226,49 +263,55 @@ __global__ void __launch_bounds__(1024)
     double wq = jac_ref * fabs(g.detJ);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) wq *= qw[qi[d]];
+    double wk = wq;
+    if constexpr (PROB == IGA_POISSON)
+      wk *= a.coef_kind == IGA_COEF_POINTS ? coefp[(size_t)e * nq + q] : a.coef_const;
     if (active) {
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) G[(q * DIM + k) * NL + lane] = g.dR[k];
+      for (int c = 0; c < DIM; ++c) {
+        GA[(c * nqp + q) * NLT + lane] = (T)g.dR[c];
+        GW[(c * nqp + q) * NLT + lane] = (T)(wk * g.dR[c]);
+      }
       Rs[q * NL + lane] = g.R;
     }
-    if (lane == 0) {
-      const double coef =
-          a.coef_kind == IGA_COEF_POINTS ? coefp[(size_t)e * nq + q] : a.coef_const;
-      Wk[q] = wq * coef;
-      Wf[q] = wq;
-    }
+    if (lane == 0) Wf[q] = wq;
   }
   __syncthreads();
 
-  // ---- H = (W D) Bm in LDS; Bm itself is formed on the fly from G ---------
-  if constexpr (PROB == IGA_POISSON)
-    for (int idx = KK * NL + threadIdx.x; idx < KKp * NL; idx += blockDim.x) G[idx] = 0.0;
-  for (int idx = threadIdx.x; idx < KKp * NLV; idx += blockDim.x) {
-    const int k = idx / NLV, col = idx % NLV;
-    double hv = 0.0;
-    if (k < KK) {
-      const int q = k / NS, s = k % NS;
-      if constexpr (PROB == IGA_POISSON) {
-        hv = Wk[q] * G[(q * DIM + s) * NL + col];
-      } else {
-        const int i = col / DIM, c = col % DIM;
-        double gi[DIM];
+  // ---- M_cd on the matrix cores -------------------------------------------
+  if constexpr (kLdsCombine) {
+    const int wave = threadIdx.x / kWave, wl = threadIdx.x % kWave;
+    const int r16 = wl % 16, k4 = wl / 16;
+    for (int cd = wave; cd < DIM * DIM; cd += blockDim.x / kWave) {
+      const int c = cd / DIM, d = cd % DIM;
+      auto acc = Mfma<T>::zero();
+      for (int q0 = 0; q0 < nqp; q0 += 4)
+        acc = Mfma<T>::step(GA[(c * nqp + q0 + k4) * NLT + r16], GW[(d * nqp + q0 + k4) * NLT + r16], acc);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) gi[d] = G[(q * DIM + d) * NL + i];
-        double acc = 0.0;
+      for (int r = 0; r < 4; ++r) Mb[(cd * 16 + Mfma<T>::row(k4, r)) * 16 + r16] = (double)acc[r];
+    }
+    __syncthreads();
+    const bool lift = rhsp != nullptr && a.gfull != nullptr;
+    T* kout = ke ? ke + (size_t)e * NLV * NLV : nullptr;
+    for (int t = threadIdx.x; t < NLV * NLV; t += blockDim.x) {
+      const int ri = t / NLV, cj = t - ri * NLV;
+      const int i = ri / VEC, aa = ri - i * VEC, j = cj / VEC, bb = cj - j * VEC;
+      double v = 0.0;
 #pragma unroll
-        for (int t = 0; t < NS; ++t) acc += cm[s * NS + t] * strain<DIM>(gi, c, t);
-        hv = Wf[q] * acc;
+      for (int cd = 0; cd < DIM * DIM; ++cd)
+        v += Ecd[(cd * DIM + aa) * DIM + bb] * Mb[(cd * 16 + i) * 16 + j];
+      if (kout) kout[t] = (T)v;
+      const int rr = fdof[ri];
+      if (rr < 0) continue;
+      const int cc = fdof[cj];
+      if (cc >= 0) {
+        if (diagp && ri == cj) unsafeAtomicAdd(diagp + rr, v);
+      } else if (lift) {
+        const double gv = a.gfull[(size_t)DIM * gdof[j] + bb];
+        if (gv != 0.0) unsafeAtomicAdd(rhsp + rr, -v * gv);
       }
     }
-    Hm[idx] = (T)hv;
-  }
-  __syncthreads();
-
-  // ---- K_e = Bm^T H on the matrix cores, one 16x16 tile per wave; each
-  // ---- lane streams its entries to the element-matrix buffer and applies the
-  // ---- lifting and the diagonal from its accumulators
-  {
+  } else {
     const bool lift = rhsp != nullptr && a.gfull != nullptr;
     T* kout = ke ? ke + (size_t)e * NLV * NLV : nullptr;
     const int wave = threadIdx.x / kWave, wl = threadIdx.x % kWave;
@@ -276,32 +319,38 @@ __global__ void __launch_bounds__(1024)
     const int r16 = wl % 16, k4 = wl / 16;
     for (int tile = wave; tile < TI * TI; tile += nwaves) {
       const int i0 = (tile / TI) * 16, j0 = (tile % TI) * 16;
-      const int ii = i0 + r16, jj = j0 + r16;
-      auto acc = Mfma<T>::zero();
-      for (int k0 = 0; k0 < KKp; k0 += 4) {
-        const int k = k0 + k4;
-        T av = (T)0;
-        if (ii < NLV) {
-          if constexpr (PROB == IGA_POISSON) {
-            av = (T)G[k * NL + ii];
-          } else if (k < KK) {
-            const int q = k / NS, s = k % NS, il = ii / DIM, c = ii % DIM;
-            double gi[DIM];
+      typename Mfma<T>::acc_t acc[NCD];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) gi[d] = G[(q * DIM + d) * NL + il];
-            av = (T)strain<DIM>(gi, c, s);
+      for (int t = 0; t < NCD; ++t) acc[t] = Mfma<T>::zero();
+      if constexpr (PROB == IGA_POISSON) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          for (int q0 = 0; q0 < nqp; q0 += 4) {
+            const int o = (c * nqp + q0 + k4) * NLT;
+            acc[0] = Mfma<T>::step(GA[o + i0 + r16], GW[o + j0 + r16], acc[0]);
           }
+      } else {
+        for (int q0 = 0; q0 < nqp; q0 += 4) {
+          T av[DIM], bv[DIM];
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) {
+            const int o = (c * nqp + q0 + k4) * NLT;
+            av[c] = GA[o + i0 + r16];
+            bv[c] = GW[o + j0 + r16];
+          }
+#pragma unroll
+          for (int c = 0; c < DIM; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) acc[c * DIM + d] = Mfma<T>::step(av[c], bv[d], acc[c * DIM + d]);
         }
-        const T bv = jj < NLV ? Hm[k * NLV + jj] : (T)0;
-        acc = Mfma<T>::step(av, bv, acc);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + Mfma<T>::row(k4, r), j = j0 + r16;
-        if (i >= NLV || j >= NLV) continue;
-        const double v = (double)acc[r];
-        if (kout) kout[i * NLV + j] = acc[r];
+        if (i >= NL || j >= NL) continue;
         if constexpr (PROB == IGA_POISSON) {
+          const double v = (double)acc[0][r];
+          if (kout) kout[i * NLV + j] = acc[0][r];
           for (int c = 0; c < a.vec; ++c) {
             const int rr = fdof[i * a.vec + c];
             if (rr < 0) continue;
@@ -314,15 +363,28 @@ __global__ void __launch_bounds__(1024)
             }
           }
         } else {
-          const int rr = fdof[i];
-          if (rr < 0) continue;
-          const int cc = fdof[j];
-          if (cc >= 0) {
-            if (diagp && i == j) unsafeAtomicAdd(diagp + rr, v);
-          } else if (lift) {
-            const double gv = a.gfull[(size_t)DIM * gdof[j / DIM] + (j % DIM)];
-            if (gv != 0.0) unsafeAtomicAdd(rhsp + rr, -v * gv);
-          }
+#pragma unroll
+          for (int aa = 0; aa < VEC; ++aa)
+#pragma unroll
+            for (int bb = 0; bb < VEC; ++bb) {
+              double v = 0.0;
+#pragma unroll
+              for (int c = 0; c < DIM; ++c)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  v += Ecd[((c * DIM + d) * DIM + aa) * DIM + bb] * (double)acc[c * DIM + d][r];
+              const int ri = i * VEC + aa, cj = j * VEC + bb;
+              if (kout) kout[ri * NLV + cj] = (T)v;
+              const int rr = fdof[ri];
+              if (rr < 0) continue;
+              const int cc = fdof[cj];
+              if (cc >= 0) {
+                if (diagp && ri == cj) unsafeAtomicAdd(diagp + rr, v);
+              } else if (lift) {
+                const double gv = a.gfull[(size_t)DIM * gdof[j] + bb];
+                if (gv != 0.0) unsafeAtomicAdd(rhsp + rr, -v * gv);
+              }
+            }
         }
       }
     }
@@ -532,14 +594,15 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
   constexpr int NL = S::NL, NLP = S::NLP, NLV = S::NLV, N1 = S::N1, NS = S::NS;
   const int nq1 = rule.n_q1;
   const int nq = DIM == 2 ? nq1 * nq1 : nq1 * nq1 * nq1;
-  const int KKp = (nq * NS + 3) & ~3;
-  int threads = nq * NLP;
-  threads = ((threads + kWave - 1) / kWave) * kWave;
-  if (threads > 1024) threads = 1024;
-  if (threads < 256) threads = 256;
-  const int Grows = PROB == IGA_POISSON ? KKp : nq * DIM;
-  const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + Grows * NL + nq * NL + 2 * nq) +
-                     sizeof(T) * (KKp * NLV + 2) + sizeof(int) * (NL * 3 + NL);
+  const int nqp = (nq + 3) & ~3;
+  constexpr int NLT = (NL + 15) / 16 * 16;
+  // 256 threads: lanes map to basis functions, the slots loop over Gauss points
+  const int threads = 256;
+  static_assert(NLP <= 256, "one slot of basis lanes must fit the workgroup");
+  constexpr bool lds_combine = PROB == IGA_ELASTICITY && NLT == 16 && DIM == 2;
+  const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + nq * NL + nq) +
+                     sizeof(T) * (2 * DIM * nqp * NLT) + sizeof(int) * (NL * 3 + NL + 2) +
+                     (lds_combine ? sizeof(double) * DIM * DIM * 256 : 0) + 16;
   if (lds > 64 * 1024) return IGA_ERR_UNSUPPORTED;
   int n_e = 1;
   for (int d = 0; d < DIM; ++d) n_e *= pt.n_elem[d];
