@@ -516,15 +516,20 @@ __global__ void __launch_bounds__(256)
   }
   const int n_sup = ne[0] * ne[1] * ne[2];
   const int grp = lane / NLVP, gl = lane % NLVP;
-  for (int s0 = 0; s0 < n_sup; s0 += GROUPS) {
-    const int sidx = s0 + grp;
-    if (sidx >= n_sup) {
-      __builtin_amdgcn_wave_barrier();
-      continue;
-    }
+  // all element-matrix reads of the row are issued before any accumulation
+  // (independent loads in flight together), then added in a fixed order
+  constexpr int NSUP = S::NL;  // supporting elements of a basis function: (P+1)^DIM
+  constexpr int STEPS = (NSUP + GROUPS - 1) / GROUPS;
+  constexpr int CPL = (NLV + NLVP - 1) / NLVP;  // columns per lane
+  double vbuf[STEPS][CPL];
+  int obuf[STEPS][CPL];
+#pragma unroll
+  for (int st = 0; st < STEPS; ++st) {
+    const int sidx = st * GROUPS + grp;
+    const bool ok = sidx < n_sup;
     int es[3] = {0, 0, 0};
     {
-      int rem = sidx;
+      int rem = ok ? sidx : 0;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
         es[d] = lo[d] + rem % ne[d];
@@ -543,7 +548,12 @@ __global__ void __launch_bounds__(256)
     }
     const int lrow = PROB == IGA_POISSON ? li : li * VEC + ci;
     const T* krow = ke + (size_t)e * NLV * NLV + (size_t)lrow * NLV;
-    for (int c = gl; c < NLV; c += NLVP) {
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+      const int c = gl + u * NLVP;
+      obuf[st][u] = -1;
+      vbuf[st][u] = 0.0;
+      if (!ok || c >= NLV) continue;
       const int f = PROB == IGA_POISSON ? c : c / VEC;
       const int cj = PROB == IGA_POISSON ? 0 : c - f * VEC;
       int rem = f, o = 0, ostride = 1;
@@ -553,8 +563,15 @@ __global__ void __launch_bounds__(256)
         rem /= N1;
         ostride *= BW;
       }
-      mybox[grp * BOXV + o * VEC + cj] += (double)krow[c];
+      obuf[st][u] = grp * BOXV + o * VEC + cj;
+      vbuf[st][u] = (double)krow[c];
     }
+  }
+#pragma unroll
+  for (int st = 0; st < STEPS; ++st) {
+#pragma unroll
+    for (int u = 0; u < CPL; ++u)
+      if (obuf[st][u] >= 0) mybox[obuf[st][u]] += vbuf[st][u];
     __builtin_amdgcn_wave_barrier();
   }
   // fixed-order sum of the group boxes

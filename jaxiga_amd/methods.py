@@ -44,6 +44,7 @@ class MethodContext:
     _pattern: object = None
     _xq: object = None
     _ke: object = None
+    _neumann: object = None
 
     @property
     def space(self):
@@ -269,17 +270,30 @@ def element_workspace(ctx, precision=_lib.IGA_FP64):
     return ws
 
 
-def neumann_load(ctx, params, rhs):
+def _neumann_sets(ctx, dev):
+    """Static part of the Neumann work (boundary point sets and element dof
+    tables), resolved once per context."""
     import torch
 
+    if ctx._neumann is None:
+        out = []
+        for bc, bset in ctx.problem.resolve_neumann():
+            ps = boundary_gauss(ctx.space, bset, ctx.n_q1)
+            if ps.n_elems == 0:
+                continue
+            dofs = ctx.space.elem_dofs[np.asarray(ps.elems)]
+            out.append((bc, ps, torch.as_tensor(dofs.astype(np.int32), device=dev)))
+        ctx._neumann = out
+    return ctx._neumann
+
+
+def neumann_load(ctx, params, rhs):
+    """Neumann work: the boundary basis is evaluated on the GPU at every call,
+    the traction at the boundary points, then scattered into the free load."""
     space = ctx.space
-    for bc, bset in ctx.problem.resolve_neumann():
-        ps = boundary_gauss(space, bset, ctx.n_q1)
-        if ps.n_elems == 0:
-            continue
+    for bc, ps, edofs in _neumann_sets(ctx, rhs.device):
         basis = evaluate(space, ps, outputs=("R", "x", "w", "normal"))
         t = call_points(bc.value, basis.x, basis.normal, params=params, vec=space.vec)
-        edofs = torch.as_tensor(basis.dofs.astype(np.int32), device=rhs.device)
         _lib.check(lib().iga_load_points(ps.n_elems, ps.n_q, space.n_local, space.vec,
                                          _lib.ptr(basis.R), _lib.ptr(basis.w), _lib.ptr(t),
                                          _lib.ptr(edofs), _lib.ptr(ctx.f2f_d), _lib.ptr(rhs), stream()),
