@@ -48,6 +48,23 @@ def build_problem(refine=REFINE):
     return V, prob, {"E": 1.0, "nu": 0.3}
 
 
+def measured_traffic(path=os.path.join(ROOT, "profiles", "assembly_elasticity_256.md")):
+    """HBM bytes per launch of the assembly kernels from the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a device copy), keyed
+    by kernel name; empty if the summary is absent."""
+    out = {}
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch"):
+                    for item in line.split()[1:]:
+                        k, v = item.split("=")
+                        out[k] = int(v)
+    except OSError:
+        pass
+    return out
+
+
 def elasticity_flops_per_element(n_q=16, n_s=3, n_lv=32):
     # element contraction K_e = sum_q w B^T C B (basis evaluation excluded)
     return 2 * n_q * (n_s * n_s * n_lv + n_s * n_lv * n_lv)
@@ -109,7 +126,8 @@ def learned_pc_bench(refine=8, n_train=48, steps=300, batch=4, test_seeds=(1000,
         data.add(ctx, vals, rhs)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    log(f"[bench] learned PC: {data.n_pairs} residual/error pairs from {n_train} fields in {t_data:.1f} s")
+    n_pairs = data.n_pairs
+    log(f"[bench] learned PC: {n_pairs} residual/error pairs from {n_train} fields in {t_data:.1f} s")
     gen = LN.generator(3, seed=1)
     t0 = time.perf_counter()
     hist = LN.train(gen, data, steps=steps, lr=1e-3, batch=batch)
@@ -159,10 +177,119 @@ def learned_pc_bench(refine=8, n_train=48, steps=300, batch=4, test_seeds=(1000,
         rows.append(row)
     return {"workload": f"unit-square Poisson, p=3, {2**refine}x{2**refine} elements, a=exp(GRF l=0.1 sigma=1.5), "
                         "smooth random source, rel. residual 1e-10, unseen coefficient seeds",
-            "training": {"fields": n_train, "pairs": None, "steps": steps, "seconds": t_train,
+            "training": {"fields": n_train, "pairs": n_pairs, "steps": steps, "seconds": t_train,
                          "data_seconds": t_data, "loss_first": hist[0], "loss_last10": float(np.mean(hist[-10:]))},
             "per_seed": rows,
             "speedup_median": statistics.median(r["speedup_vs_jacobi_cg"] for r in rows)}
+
+
+def matfree_bench(refine=6, reps=5, log=print):
+    """cfg3: 3-D Poisson on the unit cube, p=2, 64^3 elements, a = exp(GRF l=0.1,
+    sigma=1) at the Gauss points, source 1, zero Dirichlet. Matrix-free
+    sum-factorised apply (FP64 and FP32) against the assembled FP64 CSR SpMV,
+    and the solves to 1e-10: FP64 Jacobi-CG on CSR, FP64 matrix-free
+    Jacobi-CG, FP32 matrix-free inner solve with FP64 refinement."""
+    import torch
+
+    import jaxiga_amd as jx
+    from jaxiga_amd import _lib
+    from jaxiga_amd import grf
+    from jaxiga_amd import linear as LIN
+    from jaxiga_amd import matfree as MF
+    from jaxiga_amd import methods as M
+    from jaxiga_amd import precision as PR
+
+    V = jx.FunctionSpace(jx.primitives.cuboid([0, 0, 0], [1, 1, 1]).elevate(2).refine(refine))
+    prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=lambda x: np.ones(len(x), bool))], source=1.0)
+    ctx = M.build_context(prob)
+    a = grf.gauss_point_field(ctx, seed=3, ell=0.1, sigma=1.0)
+    a.exp_()
+    params = {"a0": jx.PointField(a)}
+    vals, b, diag = M.assemble(ctx, params, diagonal=True)
+    pattern = ctx.assembly
+    dinv = MF.inverse_guard(diag)
+    op64 = MF.MatrixFreeOperator(ctx, params, _lib.IGA_FP64)
+    op32 = MF.MatrixFreeOperator(ctx, params, _lib.IGA_FP32)
+    x = torch.randn(ctx.n_free, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps * 1e-3
+
+    t_csr = timed(lambda: LIN.spmv(pattern, vals, x))
+    t_mf64 = timed(lambda: op64.apply(x))
+    t_mf32 = timed(lambda: op32.apply(x))
+    y_csr = LIN.spmv(pattern, vals, x)
+    err64 = float((op64.apply(x) - y_csr).abs().max() / y_csr.abs().max())
+    err32 = float((op32.apply(x) - y_csr).abs().max() / y_csr.abs().max())
+    n_e, n_q, n = V.n_elems, 27, ctx.n_free
+    mf_bytes = {8: n_e * 6 * n_q * 8 + n_e * 27 * 8 * 2 + n * 8 * 2, 4: n_e * 6 * n_q * 4 + n_e * 27 * 4 * 2 + n * 8 * 2}
+    csr_bytes = pattern.nnz * 12 + n * 8 * 2 + (n + 1) * 4
+
+    def solve_time(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    jac, t_j = solve_time(lambda: LIN.cg_csr(pattern, vals, b, 1e-10, dinv=dinv))
+    mfr, t_m = solve_time(lambda: op64.cg(b, 1e-10, dinv=dinv))
+    ref, t_r = solve_time(lambda: PR.mf_mixed_solve(op64, op32, b, dinv, 1e-10))
+    rn, bn = LIN.residual_norms(pattern, vals, ref.x, b)
+    rn_m, _ = LIN.residual_norms(pattern, vals, mfr.x, b)
+    out = {
+        "workload": f"unit-cube Poisson, p=2, {2**refine}^3 elements, a=exp(GRF l=0.1 sigma=1), {n} free dofs, "
+                    f"{pattern.nnz} nonzeros",
+        "apply_seconds": {"csr_fp64": t_csr, "matfree_fp64": t_mf64, "matfree_fp32": t_mf32},
+        "apply_hbm_GBs": {"csr_fp64": csr_bytes / t_csr / 1e9, "matfree_fp64": mf_bytes[8] / t_mf64 / 1e9,
+                          "matfree_fp32": mf_bytes[4] / t_mf32 / 1e9},
+        "apply_rel_error_vs_csr_fp64": {"matfree_fp64": err64, "matfree_fp32": err32},
+        "solve": {"csr_fp64_jacobi_cg": {"seconds": t_j, "iterations": jac.iterations},
+                  "matfree_fp64_jacobi_cg": {"seconds": t_m, "iterations": mfr.iterations,
+                                             "relative_residual": rn_m / bn},
+                  "matfree_fp32_refined": {"seconds": t_r, "outer": ref.outer, "inner": ref.total_inner,
+                                           "relative_residual": rn / bn}},
+    }
+    log(f"[bench] cfg3 apply: csr64 {t_csr * 1e3:.3f} ms, mf64 {t_mf64 * 1e3:.3f} ms, mf32 {t_mf32 * 1e3:.3f} ms; "
+        f"solve csr {t_j * 1e3:.1f} ms ({jac.iterations} it), mf64 {t_m * 1e3:.1f} ms, "
+        f"mf32+IR {t_r * 1e3:.1f} ms (rel.res {rn / bn:.1e})")
+    return out
+
+
+def batched_bench(n_instances=512, log=print):
+    """cfg4: 512 uneven unit-square Poisson instances (p in {2,3}, 16..128
+    elements per side, a = exp(GRF), source 1): batched assembly plus one
+    persistent batched Jacobi-CG launch, tiles from the autotuner, replayed
+    from a captured graph; instances per second and the worst relative
+    residual."""
+    import torch
+
+    from jaxiga_amd import batched as BT
+
+    bp = BT.BatchedPoisson(BT.cfg4_specs(n_instances))
+    best, timings = bp.autotune()
+    graph = bp.capture(tol=1e-10)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    iters, rel = bp.results()
+    log(f"[bench] cfg4: {n_instances} instances in {dt * 1e3:.1f} ms ({n_instances / dt:.0f} instances/s), "
+        f"tiles {best}, max rel.res {float(rel.max()):.2e}, iterations {int(iters.min())}..{int(iters.max())}")
+    return {"workload": f"{n_instances} unit-square Poisson instances, p in {{2,3}}, 16..128 elements per side",
+            "seconds": dt, "instances_per_second": n_instances / dt, "lanes_per_row_threads": list(best),
+            "max_relative_residual": float(rel.max()), "iterations_min_max": [int(iters.min()), int(iters.max())],
+            "elements": bp.n_elements}
 
 
 def _lib_mod():
@@ -184,6 +311,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the learned-preconditioner time-to-solve comparison")
+    ap.add_argument("--learned-fields", type=int, default=48, help="training fields of the learned PC")
+    ap.add_argument("--learned-steps", type=int, default=300, help="training steps of the learned PC")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the matrix-free (3-D) and batched sections")
+    ap.add_argument("--extras-only", action="store_true",
+                    help="only run the matrix-free (3-D) and batched sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -208,7 +341,11 @@ def main():
         return
     lib = _lib.load()
     if args.learned_only:
-        print(json.dumps({"learned_pc": learned_pc_bench()}), flush=True)
+        print(json.dumps({"learned_pc": learned_pc_bench(n_train=args.learned_fields,
+                                                          steps=args.learned_steps)}), flush=True)
+        return
+    if args.extras_only:
+        print(json.dumps({"matfree": matfree_bench(), "batched": batched_bench()}), flush=True)
         return
     precision = _lib.IGA_FP32 if args.precision == "fp32" else _lib.IGA_FP64
     dtype = "f32" if args.precision == "fp32" else "f64"
@@ -262,12 +399,14 @@ def main():
     el["frac"] = el["achieved"] / peak_tf
     gbytes = n_e * 32 * 32 * vbytes + pattern.nnz * (8 + vbytes)
     ga = {"bound": "hbm", "achieved": gbytes / (t_ga * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-          "kernel": f"k_gather<2,3,elasticity,{dtype}>", "kernel_ms": t_ga,
+          "kernel": f"k_gather_rows<2,3,elasticity,{dtype}>", "kernel_ms": t_ga,
           "algorithmic": f"element matrices {n_e}x32x32x{vbytes} B read + {pattern.nnz} nonzeros x "
                          f"(8 B rows/cols + {vbytes} B value)"}
     ga["frac"] = ga["achieved"] / PEAK_HBM_GBS
+    traffic = measured_traffic()
+    el["traffic"] = traffic.get(el["kernel"])
+    ga["traffic"] = traffic.get(ga["kernel"])
     dominant, secondary = (el, ga) if t_el >= t_ga else (ga, el)
-    dominant["traffic"] = None
     print(f"[bench] assembly {ms_per_step:.3f} ms/step (element {t_el:.3f} ms, gather {t_ga:.3f} ms), "
           f"{value:.1f} Melem/s", flush=True)
 
@@ -287,7 +426,9 @@ def main():
                  "relative_residual_fp64": rn / bn}
         print(f"[bench] jacobi-cg {t_solve:.3f} s, {res.iterations} it, rel.res {rn / bn:.2e}", flush=True)
 
-    learned = None if (args.no_learned or args.no_solve) else learned_pc_bench()
+    learned = None if (args.no_learned or args.no_solve) else learned_pc_bench(
+        n_train=args.learned_fields, steps=args.learned_steps)
+    extras = {} if (args.no_extras or args.no_solve) else {"matfree": matfree_bench(), "batched": batched_bench()}
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "assembly Melem/s; time-to-solve @1e-10 rel res, learned-PC vs plain CG",
@@ -310,6 +451,7 @@ def main():
         "cpu_baseline": cpu,
         "time_to_solve": solve,
         "learned_pc": learned,
+        **extras,
     }
     print(json.dumps(line), flush=True)
 
