@@ -292,6 +292,73 @@ def batched_bench(n_instances=512, log=print):
             "elements": bp.n_elements}
 
 
+def assembly_run(ctx, params, prec_name, steps, warmup):
+    """Time ``steps`` full assemblies of the operator (element pass, row gather,
+    load with its boundary basis) with inputs resident; HIP events on the
+    stream the kernels run on give the per-kernel durations."""
+    import torch
+
+    from jaxiga_amd import _lib
+    from jaxiga_amd import methods as M
+
+    precision = _lib.IGA_FP32 if prec_name == "fp32" else _lib.IGA_FP64
+    dtype = "f32" if prec_name == "fp32" else "f64"
+    peak_tf = PEAK_FP32_TFLOPS if prec_name == "fp32" else PEAK_FP64_TFLOPS
+    pattern = ctx.assembly
+    n_e = ctx.space.n_elems
+    ev = {"element": [], "gather": []}
+    # arguments and output buffers are prepared once; every step recomputes
+    # everything (basis, element matrices, gather, load with its boundary basis)
+    job = M.prepare_assembly(ctx, params, precision=precision)
+
+    def step(timed):
+        job.rhs.zero_()
+        if timed:
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record()
+        job.launch(ctx, passes=1)
+        if timed:
+            e[1].record()
+        job.launch(ctx, passes=2)
+        if timed:
+            e[2].record()
+            ev["element"].append((e[0], e[1]))
+            ev["gather"].append((e[1], e[2]))
+        M.neumann_load(ctx, params, job.rhs)
+
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    t_el = statistics.mean(a.elapsed_time(b) for a, b in ev["element"])
+    t_ga = statistics.mean(a.elapsed_time(b) for a, b in ev["gather"])
+    vbytes = 4 if precision == _lib.IGA_FP32 else 8
+    flops = elasticity_flops_per_element() * n_e
+    el = {"bound": "mfma", "achieved": flops / (t_el * 1e-3) / 1e12, "peak": peak_tf, "unit": "TFLOP/s",
+          "kernel": f"k_element<2,3,elasticity,{dtype}>", "kernel_ms": t_el,
+          "algorithmic": f"{elasticity_flops_per_element()} flop/element x {n_e} elements"}
+    el["frac"] = el["achieved"] / peak_tf
+    gbytes = n_e * 32 * 32 * vbytes + pattern.nnz * (4 + vbytes)
+    ga = {"bound": "hbm", "achieved": gbytes / (t_ga * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+          "kernel": f"k_gather_rows<2,3,elasticity,{dtype}>", "kernel_ms": t_ga,
+          "algorithmic": f"element matrices {n_e}x32x32x{vbytes} B read + {pattern.nnz} nonzeros x "
+                         f"(4 B column read + {vbytes} B value written)"}
+    ga["frac"] = ga["achieved"] / PEAK_HBM_GBS
+    traffic = measured_traffic()
+    el["traffic"] = traffic.get(el["kernel"])
+    ga["traffic"] = traffic.get(ga["kernel"])
+    dominant, secondary = (el, ga) if t_el >= t_ga else (ga, el)
+    value = n_e * steps / elapsed / 1e6
+    print(f"[bench] {dtype} assembly {elapsed / steps * 1e3:.3f} ms/step (element {t_el:.3f} ms, "
+          f"gather {t_ga:.3f} ms), {value:.1f} Melem/s", flush=True)
+    return {"value": value, "ms_per_step": elapsed / steps * 1e3, "dtype": dtype, "roofline": dominant,
+            "roofline_secondary": secondary, "vals": job.vals, "rhs": job.rhs}
+
+
 def _lib_mod():
     from jaxiga_amd import _lib
 
@@ -305,7 +372,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solve measurement")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32",
+                    help="assembly precision of the headline value (the other is reported beside it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (HBM counter calibration)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
@@ -347,84 +415,50 @@ def main():
     if args.extras_only:
         print(json.dumps({"matfree": matfree_bench(), "batched": batched_bench()}), flush=True)
         return
-    precision = _lib.IGA_FP32 if args.precision == "fp32" else _lib.IGA_FP64
-    dtype = "f32" if args.precision == "fp32" else "f64"
-    peak_tf = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_FP64_TFLOPS
     V, prob, params = build_problem()
     ctx = M.build_context(prob, params)
     pattern = ctx.assembly
     n_e = V.n_elems
     print(f"[bench] elements={n_e} dofs={V.n_dofs} free={ctx.n_free} nnz={pattern.nnz}", flush=True)
+    other = "fp64" if args.precision == "fp32" else "fp32"
+    main_run = assembly_run(ctx, params, args.precision, args.steps, args.warmup)
+    side_run = assembly_run(ctx, params, other, args.steps, args.warmup)
+    runs = {args.precision: main_run, other: side_run}
+    value, ms_per_step, dtype = main_run["value"], main_run["ms_per_step"], main_run["dtype"]
+    dominant, secondary = main_run["roofline"], main_run["roofline_secondary"]
+    v64, v32 = runs["fp64"]["vals"], runs["fp32"]["vals"]
+    stiff_err = float((v32.double() - v64).abs().max() / v64.abs().max())
+    print(f"[bench] FP32 vs FP64 stiffness entries: max |dK| / max |K| = {stiff_err:.2e}", flush=True)
 
-    # -- assembly steps ---------------------------------------------------------
-    ev = {"element": [], "gather": []}
-
-    # arguments and output buffers are prepared once; every step recomputes
-    # everything (basis, element matrices, gather, load with its boundary basis)
-    job = M.prepare_assembly(ctx, params, precision=precision)
-
-    def step(timed):
-        job.rhs.zero_()
-        if timed:
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-            e[0].record()
-        job.launch(ctx, passes=1)
-        if timed:
-            e[1].record()
-        job.launch(ctx, passes=2)
-        if timed:
-            e[2].record()
-            ev["element"].append((e[0], e[1]))
-            ev["gather"].append((e[1], e[2]))
-        M.neumann_load(ctx, params, job.rhs)
-        return job.vals, job.rhs
-
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        vals, rhs = step(True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    t_el = statistics.mean(a.elapsed_time(b) for a, b in ev["element"])
-    t_ga = statistics.mean(a.elapsed_time(b) for a, b in ev["gather"])
-    value = n_e * args.steps / elapsed / 1e6
-    vbytes = 4 if precision == _lib.IGA_FP32 else 8
-    flops = elasticity_flops_per_element() * n_e
-    el = {"bound": "mfma", "achieved": flops / (t_el * 1e-3) / 1e12, "peak": peak_tf, "unit": "TFLOP/s",
-          "kernel": f"k_element<2,3,elasticity,{dtype}>", "kernel_ms": t_el,
-          "algorithmic": f"{elasticity_flops_per_element()} flop/element x {n_e} elements"}
-    el["frac"] = el["achieved"] / peak_tf
-    gbytes = n_e * 32 * 32 * vbytes + pattern.nnz * (8 + vbytes)
-    ga = {"bound": "hbm", "achieved": gbytes / (t_ga * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-          "kernel": f"k_gather_rows<2,3,elasticity,{dtype}>", "kernel_ms": t_ga,
-          "algorithmic": f"element matrices {n_e}x32x32x{vbytes} B read + {pattern.nnz} nonzeros x "
-                         f"(8 B rows/cols + {vbytes} B value)"}
-    ga["frac"] = ga["achieved"] / PEAK_HBM_GBS
-    traffic = measured_traffic()
-    el["traffic"] = traffic.get(el["kernel"])
-    ga["traffic"] = traffic.get(ga["kernel"])
-    dominant, secondary = (el, ga) if t_el >= t_ga else (ga, el)
-    print(f"[bench] assembly {ms_per_step:.3f} ms/step (element {t_el:.3f} ms, gather {t_ga:.3f} ms), "
-          f"{value:.1f} Melem/s", flush=True)
-
-    # -- time to solve (FP64 Jacobi-CG, rel. residual 1e-10) --------------------
+    # -- time to solve (rel. residual 1e-10): FP64 Jacobi-CG, and FP32 operator
+    # -- with FP64 iterative refinement (outer residual on the FP64 operator)
     solve = None
-    if not args.no_solve and precision == _lib.IGA_FP64:
-        dinv = LIN.inverse_diagonal(pattern, vals)
+    if not args.no_solve:
+        from jaxiga_amd import precision as PR
+
+        rhs = runs["fp64"]["rhs"]
+        dinv = LIN.inverse_diagonal(pattern, v64)
         work = torch.empty(int(lib.iga_cg_workspace(ctx.n_free)), dtype=torch.float64, device="cuda")
-        LIN.cg_csr(pattern, vals, rhs, 1e-10, maxiter=50, dinv=dinv, workspace=work)  # warm-up
+        LIN.cg_csr(pattern, v64, rhs, 1e-10, maxiter=50, dinv=dinv, workspace=work)  # warm-up
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        res = LIN.cg_csr(pattern, vals, rhs, 1e-10, maxiter=20 * ctx.n_free, dinv=dinv, workspace=work)
+        res = LIN.cg_csr(pattern, v64, rhs, 1e-10, maxiter=20 * ctx.n_free, dinv=dinv, workspace=work)
         torch.cuda.synchronize()
         t_solve = time.perf_counter() - t1
-        rn, bn = LIN.residual_norms(pattern, vals, res.x, rhs)
-        solve = {"method": "fp64 jacobi-cg", "seconds": t_solve, "iterations": res.iterations,
-                 "relative_residual_fp64": rn / bn}
-        print(f"[bench] jacobi-cg {t_solve:.3f} s, {res.iterations} it, rel.res {rn / bn:.2e}", flush=True)
+        rn, bn = LIN.residual_norms(pattern, v64, res.x, rhs)
+        mixed = PR.csr_mixed_solve(pattern, v64, v32, rhs, dinv, tol=1e-10)
+        rn_m, _ = LIN.residual_norms(pattern, v64, mixed.x, rhs)
+        disp_err = float((mixed.x - res.x).abs().max() / res.x.abs().max())
+        solve = {"fp64_jacobi_cg": {"seconds": t_solve, "iterations": res.iterations,
+                                    "relative_residual_fp64": rn / bn},
+                 "fp32_operator_fp64_refinement": {"seconds": mixed.seconds, "outer": mixed.outer,
+                                                   "inner_iterations": mixed.total_inner,
+                                                   "relative_residual_fp64": rn_m / bn,
+                                                   "max_rel_displacement_diff_vs_fp64": disp_err},
+                 "fp32_vs_fp64_stiffness_max_rel": stiff_err}
+        print(f"[bench] jacobi-cg fp64 {t_solve:.3f} s, {res.iterations} it, rel.res {rn / bn:.2e}; "
+              f"fp32 + refinement {mixed.seconds:.3f} s ({mixed.outer} outer, {mixed.total_inner} inner), "
+              f"rel.res {rn_m / bn:.2e}, displacement diff {disp_err:.2e}", flush=True)
 
     learned = None if (args.no_learned or args.no_solve) else learned_pc_bench(
         n_train=args.learned_fields, steps=args.learned_steps)
@@ -444,12 +478,15 @@ def main():
         "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": "quarter-annulus plane-strain elasticity, p=3, 256x256 elements, "
-                               f"134162 dofs, {dtype} assembly (basis + K_e + CSR scatter + load)",
+                               f"134162 dofs, {dtype} assembly (basis + K_e + CSR scatter + load); "
+                               "FP64 iterative refinement for the solve",
                    "elements": n_e, "nnz": pattern.nnz, "n_free": ctx.n_free},
         "roofline": dominant,
         "roofline_secondary": secondary,
         "cpu_baseline": cpu,
         "time_to_solve": solve,
+        f"assembly_{other}": {k: side_run[k] for k in ("value", "ms_per_step", "dtype", "roofline",
+                                                          "roofline_secondary")},
         "learned_pc": learned,
         **extras,
     }
