@@ -100,7 +100,9 @@ enum iga_prec { IGA_FP64 = 0, IGA_FP32 = 1, IGA_BF16 = 2 };
  * gfull      : (n_dofs) prescribed values (0 on free dofs), or null
  * rows/cols  : (nnz) COO view of the CSR pattern; vals (nnz) is overwritten
  * ke_work    : element-matrix buffer of iga_assemble_workspace() bytes
- * rhs        : (n_free) += source work - K_fp g ; diag (n_free) += diag(K_e)
+ * rhs        : (n_free) += source work - K_fp g ; diag (n_free) += diag(K),
+ *              added by the row-form gather pass when it runs, else by the
+ *              element pass
  * passes     : bit 1 element pass, bit 2 gather pass (0 = both); the
  *              gather reads the element matrices of an earlier element pass
  * batch      : number of coefficient/source instances sharing the patch and

@@ -36,7 +36,17 @@ struct AsmDev {
   double* rhs;
   double* diag;
   int n_free;  // rhs/diag stride between batch instances
+  // E^cd_ab = sum_st C_st Bs(s; a, c) Bs(t; b, d), index ((c*dim + d)*dim + a)*dim + b
+  double ecd[81];
 };
+
+// Orders the LDS accesses of one wavefront: lanes exchange data through LDS
+// with no workgroup barrier (LDS executes a wave's operations in order; the
+// fence keeps the compiler from moving accesses across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -408,6 +418,285 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Element pass, one wavefront per element, for planar patches with at most
+// four functions and four Gauss points per direction. Lane l owns the local
+// function i = l % 16 (i0 = i % 4, i1 = i / 4, padded to 4 x 4) and, in step
+// k = 0..3, the Gauss point q = 4k + l / 16 (q0 = l / 16, q1 = k). That is the
+// operand layout of the 16x16x4 MFMA step k -- A[i][q] = G_c[q][i] and
+// B[q][j] = w_q G_d[q][j] -- so the gradients feed the matrix cores straight
+// from the lane's registers. The geometry (W, dW, x, dx/dxi: nine tensor
+// sums per Gauss point) is sum-factorised once per point, FP64, and shared
+// through LDS; no workgroup barrier is needed.
+template <int P, int PROB, typename T>
+__global__ void __launch_bounds__(256)
+    k_element_w(iga_patch_t pt, int nq1, const double* __restrict__ qp,
+                const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all, int n_e) {
+  constexpr int N1 = P + 1;
+  static_assert(N1 <= 4, "at most four functions per direction");
+  constexpr int NL = N1 * N1;
+  constexpr int VEC = PROB == IGA_ELASTICITY ? 2 : 1;
+  constexpr int NLV = NL * VEC;
+  constexpr int NCD = PROB == IGA_ELASTICITY ? 4 : 1;
+  constexpr int WPB = 4;    // elements (waves) per workgroup
+  constexpr int REC = 12;   // doubles per Gauss-point record
+  __shared__ double s_tabq[WPB][2][2][4][4];  // [dir][value|derivative][q][a]
+  __shared__ double s_taba[WPB][2][2][4][4];  // [dir][value|derivative][a][q]
+  __shared__ double s_coef[WPB][3][16];       // w, w x, w y per local function
+  __shared__ double s_geo[WPB][16][3][4];     // [q][field][value, d/dxi0, d/dxi1, -]
+  __shared__ __align__(16) double s_rec[WPB][16][REC];
+  __shared__ int s_fdof[WPB][16 * VEC];
+  __shared__ int s_gdof[WPB][16];
+
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  const int e = blockIdx.x * WPB + w;
+  if (e >= n_e) return;
+  const int64_t bi = blockIdx.y;
+  const int nq = nq1 * nq1;
+  const double* coefp = a.coef_points ? a.coef_points + bi * n_e * nq : nullptr;
+  const double* srcp = a.src_points ? a.src_points + bi * n_e * nq * VEC : nullptr;
+  double* rhsp = a.rhs ? a.rhs + bi * a.n_free : nullptr;
+  double* diagp = a.diag ? a.diag + bi * a.n_free : nullptr;
+  T* kout = ke_all ? ke_all + (bi * n_e + e) * (int64_t)(NLV * NLV) : nullptr;
+  const int ei0 = e % pt.n_elem[0], ei1 = e / pt.n_elem[0];
+  const double h0 = pt.hi[0][ei0] - pt.lo[0][ei0], h1 = pt.hi[1][ei1] - pt.lo[1][ei1];
+
+  // ---- 1-D extracted tables: lane (dir, a, q) for l < 32 --------------------
+  if (l < 32) {
+    const int d = l >> 4, fa = (l >> 2) & 3, q = l & 3;
+    double v = 0.0, dv = 0.0;
+    if (fa < N1 && q < nq1) {
+      double B[N1], dB[N1];
+      bernstein<P>(qp[q], B, dB);
+      const double* C = (d ? pt.ext[1] + ((size_t)ei1 * N1 + fa) * N1
+                           : pt.ext[0] + ((size_t)ei0 * N1 + fa) * N1);
+#pragma unroll
+      for (int b = 0; b < N1; ++b) {
+        v += C[b] * B[b];
+        dv += C[b] * dB[b];
+      }
+      dv *= 2.0 / (d ? h1 : h0);
+    }
+    s_tabq[w][d][0][q][fa] = v;
+    s_tabq[w][d][1][q][fa] = dv;
+    s_taba[w][d][0][fa][q] = v;
+    s_taba[w][d][1][fa][q] = dv;
+  }
+  // ---- local functions: weights, weighted control points, dof indices -------
+  if (l < 16) {
+    const int i0 = l & 3, i1 = l >> 2;
+    double wt = 0.0, px = 0.0, py = 0.0;
+    int sdof = 0;
+    if (i0 < N1 && i1 < N1) {
+      sdof = (pt.first[0][ei0] + i0) + (pt.first[1][ei1] + i1) * pt.n_cp[0];
+      wt = pt.wgts[sdof];
+      px = pt.cpts[(size_t)sdof * 2];
+      py = pt.cpts[(size_t)sdof * 2 + 1];
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) s_fdof[w][l * VEC + c] = a.f2f[(size_t)VEC * sdof + c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) s_fdof[w][l * VEC + c] = -1;
+    }
+    s_gdof[w][l] = sdof;
+    s_coef[w][0][l] = wt;
+    s_coef[w][1][l] = wt * px;
+    s_coef[w][2][l] = wt * py;
+  }
+  wave_lds_sync();
+
+  // ---- geometry per Gauss point: lane (field f, point q), sum-factorised -----
+  {
+    const int q = l & 15, f = l >> 4, q0 = q & 3, q1 = q >> 2;
+    if (f < 3 && q0 < nq1 && q1 < nq1) {
+      const double* F = s_coef[w][f];
+      double tv[4], td[4];
+#pragma unroll
+      for (int i1 = 0; i1 < N1; ++i1) {
+        double sv = 0.0, sd = 0.0;
+#pragma unroll
+        for (int i0 = 0; i0 < N1; ++i0) {
+          const double c = F[i0 + 4 * i1];
+          sv += s_tabq[w][0][0][q0][i0] * c;
+          sd += s_tabq[w][0][1][q0][i0] * c;
+        }
+        tv[i1] = sv;
+        td[i1] = sd;
+      }
+      double v = 0.0, d0 = 0.0, d1 = 0.0;
+#pragma unroll
+      for (int i1 = 0; i1 < N1; ++i1) {
+        const double b = s_tabq[w][1][0][q1][i1], db = s_tabq[w][1][1][q1][i1];
+        v += b * tv[i1];
+        d0 += b * td[i1];
+        d1 += db * tv[i1];
+      }
+      s_geo[w][q][f][0] = v;
+      s_geo[w][q][f][1] = d0;
+      s_geo[w][q][f][2] = d1;
+    }
+  }
+  wave_lds_sync();
+  // ---- Gauss-point records: 1/W, dW, inverse Jacobian, weights ---------------
+  if (l < 16) {
+    const int q = l, q0 = q & 3, q1 = q >> 2;
+    double rec[REC];
+#pragma unroll
+    for (int t = 0; t < REC; ++t) rec[t] = 0.0;
+    if (q0 < nq1 && q1 < nq1) {
+      const double W = s_geo[w][q][0][0], dW0 = s_geo[w][q][0][1], dW1 = s_geo[w][q][0][2];
+      const double iW = 1.0 / W;
+      const double x0 = s_geo[w][q][1][0] * iW, x1 = s_geo[w][q][2][0] * iW;
+      // Jm[c][d] = d x_c / d xi_d
+      const double j00 = (s_geo[w][q][1][1] - x0 * dW0) * iW;
+      const double j01 = (s_geo[w][q][1][2] - x0 * dW1) * iW;
+      const double j10 = (s_geo[w][q][2][1] - x1 * dW0) * iW;
+      const double j11 = (s_geo[w][q][2][2] - x1 * dW1) * iW;
+      const double det = j00 * j11 - j01 * j10;
+      const double id = 1.0 / det;
+      const double wq = 0.25 * h0 * h1 * fabs(det) * qw[q0] * qw[q1];
+      const int qn = q0 + nq1 * q1;
+      double wk = wq;
+      if constexpr (PROB == IGA_POISSON)
+        wk *= a.coef_kind == IGA_COEF_POINTS ? coefp[(size_t)e * nq + qn] : a.coef_const;
+      rec[0] = iW;
+      rec[1] = dW0;
+      rec[2] = dW1;
+      // inverse Jacobian, d xi_d / d x_c at [2d + c]
+      rec[3] = j11 * id;
+      rec[4] = -j01 * id;
+      rec[5] = -j10 * id;
+      rec[6] = j00 * id;
+      rec[7] = wk;
+#pragma unroll
+      for (int c = 0; c < VEC; ++c)
+        rec[8 + c] = a.src_kind == IGA_SRC_CONST    ? wq * a.src_const[c]
+                     : a.src_kind == IGA_SRC_POINTS ? wq * srcp[((size_t)e * nq + qn) * VEC + c]
+                                                    : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < REC; ++t) s_rec[w][q][t] = rec[t];
+  }
+  wave_lds_sync();
+
+  // ---- gradient products on the matrix cores --------------------------------
+  const int i = l & 15, g = l >> 4;
+  const double wi = s_coef[w][0][i];
+  const double a0 = s_tabq[w][0][0][g][i & 3], da0 = s_tabq[w][0][1][g][i & 3];
+  typename Mfma<T>::acc_t acc[NCD];
+#pragma unroll
+  for (int t = 0; t < NCD; ++t) acc[t] = Mfma<T>::zero();
+  double ld[VEC];
+#pragma unroll
+  for (int c = 0; c < VEC; ++c) ld[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double* rc = s_rec[w][4 * k + g];
+    const double b = s_taba[w][1][0][i >> 2][k], db = s_taba[w][1][1][i >> 2][k];
+    const double iW = rc[0];
+    const double Nw = a0 * b * wi;
+    const double R = Nw * iW;
+    const double r0 = (da0 * b * wi - R * rc[1]) * iW;
+    const double r1 = (a0 * db * wi - R * rc[2]) * iW;
+    const double gx = r0 * rc[3] + r1 * rc[5];
+    const double gy = r0 * rc[4] + r1 * rc[6];
+    const double wk = rc[7];
+    const T ga0 = (T)gx, ga1 = (T)gy, gw0 = (T)(wk * gx), gw1 = (T)(wk * gy);
+    if constexpr (PROB == IGA_ELASTICITY) {
+      acc[0] = Mfma<T>::step(ga0, gw0, acc[0]);
+      acc[1] = Mfma<T>::step(ga0, gw1, acc[1]);
+      acc[2] = Mfma<T>::step(ga1, gw0, acc[2]);
+      acc[3] = Mfma<T>::step(ga1, gw1, acc[3]);
+    } else {
+      acc[0] = Mfma<T>::step(ga0, gw0, acc[0]);
+      acc[0] = Mfma<T>::step(ga1, gw1, acc[0]);
+    }
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) ld[c] += R * rc[8 + c];
+  }
+
+  // ---- load vector: sum the four Gauss-point groups of each function --------
+  if (rhsp && a.src_kind != IGA_SRC_NONE) {
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) {
+      ld[c] += __shfl_xor(ld[c], 16);
+      ld[c] += __shfl_xor(ld[c], 32);
+    }
+    if (l < 16 && (i & 3) < N1 && (i >> 2) < N1) {
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) {
+        const int r = s_fdof[w][i * VEC + c];
+        if (r >= 0) unsafeAtomicAdd(rhsp + r, ld[c]);
+      }
+    }
+  }
+
+  // ---- constitutive combination, element matrix out, diagonal, lifting -------
+  const bool lift = rhsp != nullptr && a.gfull != nullptr;
+  bool pres = false;
+  if (l < 16 && (i & 3) < N1 && (i >> 2) < N1)
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) pres |= s_fdof[w][i * VEC + c] < 0;
+  const bool any_pres = __ballot(pres) != 0;
+  const int j = i;  // accumulator column
+  const bool jok = (j & 3) < N1 && (j >> 2) < N1;
+  const int jc = (j & 3) + N1 * (j >> 2);  // compact local index
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ir = Mfma<T>::row(g, r);
+    const bool ok = jok && (ir & 3) < N1 && (ir >> 2) < N1;
+    if (!ok) continue;
+    const int ic = (ir & 3) + N1 * (ir >> 2);
+    // combined in the accumulator type
+    T v[VEC][VEC];
+    if constexpr (PROB == IGA_ELASTICITY) {
+#pragma unroll
+      for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          T s = (T)0;
+#pragma unroll
+          for (int cd = 0; cd < 4; ++cd) s += (T)a.ecd[(cd * 2 + aa) * 2 + bb] * acc[cd][r];
+          v[aa][bb] = s;
+        }
+    } else {
+      v[0][0] = acc[0][r];
+    }
+    if (kout) {
+#pragma unroll
+      for (int aa = 0; aa < VEC; ++aa) {
+        T* dst = kout + (size_t)(ic * VEC + aa) * NLV + jc * VEC;
+        if constexpr (VEC == 2) {
+          using T2 = T __attribute__((ext_vector_type(2)));
+          *reinterpret_cast<T2*>(dst) = T2{v[aa][0], v[aa][1]};
+        } else {
+          *dst = v[aa][0];
+        }
+      }
+    }
+    if (diagp && ir == j) {
+#pragma unroll
+      for (int aa = 0; aa < VEC; ++aa) {
+        const int rr = s_fdof[w][ir * VEC + aa];
+        if (rr >= 0) unsafeAtomicAdd(diagp + rr, (double)v[aa][aa]);
+      }
+    }
+    if (lift && any_pres) {
+#pragma unroll
+      for (int aa = 0; aa < VEC; ++aa) {
+        const int rr = s_fdof[w][ir * VEC + aa];
+        if (rr < 0) continue;
+#pragma unroll
+        for (int bb = 0; bb < VEC; ++bb) {
+          if (s_fdof[w][j * VEC + bb] >= 0) continue;
+          const double gv = a.gfull[(size_t)VEC * s_gdof[w][j] + bb];
+          if (gv != 0.0) unsafeAtomicAdd(rhsp + rr, -(double)v[aa][bb] * gv);
+        }
+      }
+    }
+  }
+}
+
 // Pass 2: vals[k] = sum over the elements shared by row and column of the
 // matching element-matrix entries (fixed order -> bitwise reproducible).
 template <int DIM, int P, int PROB, typename T>
@@ -470,17 +759,24 @@ __global__ void __launch_bounds__(256)
 // Pass 2 (row form): one wavefront per CSR row. For each element supporting
 // the row's basis function (fixed order) the wave reads that element's
 // matrix row -- NLV contiguous values, a coalesced read -- and accumulates it
-// into a dense stencil box of the row held in LDS; the box is then written
-// out in CSR order. Lane groups take different elements into separate boxes
-// that are summed in a fixed order at the end: bitwise reproducible.
+// into a dense stencil box of the row held in LDS. Lane groups take
+// different elements into separate boxes that are summed in a fixed order:
+// bitwise reproducible. The box is then written out in CSR order: columns
+// are sorted by dof, and dofs grow with the box order (last direction
+// slowest, component fastest), so the slot of a present box entry is the
+// number of present entries before it -- a ballot and a popcount. An entry
+// is present when its control point lies in the patch, its dof is free and
+// the two functions share an element. The row's diagonal (the box centre)
+// is added to diag when it is given.
 template <int DIM, int P, int PROB, typename T>
 __global__ void __launch_bounds__(256)
     k_gather_rows(iga_patch_t pt, int vec, int32_t n_rows, const int32_t* __restrict__ indptr,
                   const int32_t* __restrict__ cols, const int32_t* __restrict__ freed,
-                  const T* __restrict__ ke_all, T* __restrict__ vals_all, int64_t nnz,
-                  int64_t n_e) {
+                  const int32_t* __restrict__ f2f, const T* __restrict__ ke_all,
+                  T* __restrict__ vals_all, double* __restrict__ diag_all, int32_t n_free,
+                  int64_t nnz, int64_t n_e) {
   using S = Shape<DIM, P, PROB>;
-  constexpr int N1 = S::N1, NL = S::NL, NLV = S::NLV;
+  constexpr int N1 = S::N1, NLV = S::NLV;
   constexpr int VEC = S::VEC;
   constexpr int BW = 2 * P + 1;                           // stencil box width per direction
   constexpr int BOX = DIM == 2 ? BW * BW : BW * BW * BW;  // control points in the box
@@ -489,17 +785,15 @@ __global__ void __launch_bounds__(256)
   constexpr int GROUPS = kWave / NLVP;                                       // elements per step
   constexpr int WAVES = 4;
   __shared__ double box[WAVES][GROUPS][BOXV];
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
   const int row = blockIdx.x * WAVES + wave;
   if (row >= n_rows) return;
   const int64_t b = blockIdx.y;
   const T* ke = ke_all + b * n_e * NLV * NLV;
   T* vals = vals_all + b * nnz;
   double* mybox = &box[wave][0][0];
-  // the box is private to the wave; LDS operations of one wave execute in
-  // order, so wave barriers (which stop compiler reordering) separate phases
   for (int t = lane; t < GROUPS * BOXV; t += kWave) mybox[t] = 0.0;
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
 
   const int gi = freed[row];
   const int si = gi / vec, ci = gi - si * vec;
@@ -514,11 +808,10 @@ __global__ void __launch_bounds__(256)
       ne[d] = pt.ehi[d][id[d]] - lo[d] + 1;
     }
   }
-  const int n_sup = ne[0] * ne[1] * ne[2];
   const int grp = lane / NLVP, gl = lane % NLVP;
-  // all element-matrix reads of the row are issued before any accumulation
-  // (independent loads in flight together), then added in a fixed order
-  constexpr int NSUP = S::NL;  // supporting elements of a basis function: (P+1)^DIM
+  // supporting elements as the fixed (P+1)^DIM grid from the first one;
+  // every element-matrix read of the row is issued before any accumulation
+  constexpr int NSUP = S::NL;
   constexpr int STEPS = (NSUP + GROUPS - 1) / GROUPS;
   constexpr int CPL = (NLV + NLVP - 1) / NLVP;  // columns per lane
   double vbuf[STEPS][CPL];
@@ -526,14 +819,16 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int st = 0; st < STEPS; ++st) {
     const int sidx = st * GROUPS + grp;
-    const bool ok = sidx < n_sup;
     int es[3] = {0, 0, 0};
+    bool ok = sidx < NSUP;
     {
-      int rem = ok ? sidx : 0;
+      int rem = sidx;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
-        es[d] = lo[d] + rem % ne[d];
-        rem /= ne[d];
+        const int sd = rem % N1;
+        rem /= N1;
+        ok = ok && sd < ne[d];
+        es[d] = lo[d] + (ok ? sd : 0);
       }
     }
     int e = 0, estride = 1, li = 0, lstride = 1, fo[3] = {0, 0, 0};
@@ -572,35 +867,242 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int u = 0; u < CPL; ++u)
       if (obuf[st][u] >= 0) mybox[obuf[st][u]] += vbuf[st][u];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
-  // fixed-order sum of the group boxes
-  if (GROUPS > 1)
-    for (int t = lane; t < BOXV; t += kWave) {
-      double acc = mybox[t];
-#pragma unroll
-      for (int g = 1; g < GROUPS; ++g) acc += mybox[g * BOXV + t];
-      mybox[t] = acc;
-    }
-  __builtin_amdgcn_wave_barrier();
   const int k0 = indptr[row], k1 = indptr[row + 1];
-  for (int k = k0 + lane; k < k1; k += kWave) {
-    const int gj = freed[cols[k]];
-    const int sj = gj / vec, cjj = gj - sj * vec;
-    int rem = sj, o = 0, ostride = 1;
+  double* diag = diag_all ? diag_all + b * n_free : nullptr;
+  if (vec == VEC) {
+    constexpr int CENTER = DIM == 2 ? P * BW + P : (P * BW + P) * BW + P;
+    int base = 0;
+#pragma unroll
+    for (int t0 = 0; t0 < BOXV; t0 += kWave) {
+      const int t = t0 + lane;
+      bool present = false;
+      if (t < BOXV) {
+        const int node = t / VEC, cj = t - node * VEC;
+        int rem = node, full = 0, stride = 1;
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const int jd = id[d] + rem % BW - P;
+          rem /= BW;
+          ok = ok && jd >= 0 && jd < pt.n_cp[d];
+          if (ok) ok = max(lo[d], pt.elo[d][jd]) <= min(lo[d] + ne[d] - 1, pt.ehi[d][jd]);
+          full += jd * stride;
+          stride *= pt.n_cp[d];
+        }
+        present = ok && f2f[(size_t)full * VEC + cj] >= 0;
+      }
+      const uint64_t m = __ballot(present);
+      const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+      if (present && k0 + slot < k1) {
+        double acc = mybox[t];
+#pragma unroll
+        for (int g = 1; g < GROUPS; ++g) acc += mybox[g * BOXV + t];
+        vals[k0 + slot] = (T)acc;
+        if (diag && t == CENTER * VEC + ci) diag[row] += acc;
+      }
+      base += __popcll(m);
+    }
+  } else {
+    // vector Poisson: components decouple, the box holds one per point
+    if (GROUPS > 1)
+      for (int t = lane; t < BOXV; t += kWave) {
+        double acc = mybox[t];
+#pragma unroll
+        for (int g = 1; g < GROUPS; ++g) acc += mybox[g * BOXV + t];
+        mybox[t] = acc;
+      }
+    wave_lds_sync();
+    for (int k = k0 + lane; k < k1; k += kWave) {
+      const int gj = freed[cols[k]];
+      const int sj = gj / vec, cjj = gj - sj * vec;
+      int rem = sj, o = 0, ostride = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int jd = rem % pt.n_cp[d];
+        rem /= pt.n_cp[d];
+        o += (jd - id[d] + P) * ostride;
+        ostride *= BW;
+      }
+      const double v = cjj == ci ? mybox[o * VEC] : 0.0;
+      vals[k] = (T)v;
+      if (diag && gj == gi) diag[row] += v;
+    }
+  }
+}
+
+// Pass 2 (control-point form): one wavefront per control point, covering the
+// rows of all its components (VEC rows share the supporting elements, the
+// stencil box and the column set). Per supporting element the wave reads the
+// VEC adjacent element-matrix rows -- VEC * NLV contiguous values -- into the
+// component boxes; the column test (in the patch, free, overlapping support)
+// is issued before the accumulation so its loads overlap the element-matrix
+// reads. Write-out as in the row form, once per free component row.
+template <int DIM, int P, int PROB, typename T>
+__global__ void __launch_bounds__(256)
+    k_gather_points(iga_patch_t pt, int32_t n_points, const int32_t* __restrict__ indptr,
+                    const int32_t* __restrict__ f2f, const T* __restrict__ ke_all,
+                    T* __restrict__ vals_all, double* __restrict__ diag_all, int32_t n_free,
+                    int64_t nnz, int64_t n_e) {
+  using S = Shape<DIM, P, PROB>;
+  constexpr int N1 = S::N1, NLV = S::NLV;
+  constexpr int VEC = S::VEC;
+  constexpr int BW = 2 * P + 1;
+  constexpr int BOX = DIM == 2 ? BW * BW : BW * BW * BW;
+  constexpr int BOXV = BOX * VEC;            // entries of one row's box
+  constexpr int VALS = VEC * NLV;            // values per supporting element
+  constexpr int NLVP = VALS <= 8 ? 8 : VALS <= 16 ? 16 : VALS <= 32 ? 32 : 64;
+  constexpr int GROUPS = kWave / NLVP;
+  constexpr int CPL = (VALS + NLVP - 1) / NLVP;
+  constexpr int WAVES = 4;
+  constexpr int CHUNKS = (BOXV + kWave - 1) / kWave;
+  __shared__ double box[WAVES][GROUPS][VEC][BOXV];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+  const int pnt = blockIdx.x * WAVES + wave;
+  if (pnt >= n_points) return;
+  int rowc[VEC];
+  bool any_free = false;
+#pragma unroll
+  for (int c = 0; c < VEC; ++c) {
+    rowc[c] = f2f[(size_t)pnt * VEC + c];
+    any_free |= rowc[c] >= 0;
+  }
+  if (!any_free) return;
+  const int64_t b = blockIdx.y;
+  const T* ke = ke_all + b * n_e * NLV * NLV;
+  T* vals = vals_all + b * nnz;
+  double* mybox = &box[wave][0][0][0];
+  for (int t = lane; t < GROUPS * VEC * BOXV; t += kWave) mybox[t] = 0.0;
+
+  int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, ne[3] = {1, 1, 1};
+  {
+    int rem = pnt;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      const int jd = rem % pt.n_cp[d];
+      id[d] = rem % pt.n_cp[d];
       rem /= pt.n_cp[d];
-      o += (jd - id[d] + P) * ostride;
-      ostride *= BW;
+      lo[d] = pt.elo[d][id[d]];
+      ne[d] = pt.ehi[d][id[d]] - lo[d] + 1;
     }
-    double v;
-    if constexpr (PROB == IGA_POISSON)
-      v = cjj == ci ? mybox[o] : 0.0;
-    else
-      v = mybox[o * VEC + cjj];
-    vals[k] = (T)v;
+  }
+  const int grp = lane / NLVP, gl = lane % NLVP;
+  constexpr int NSUP = S::NL;
+  constexpr int STEPS = (NSUP + GROUPS - 1) / GROUPS;
+  // element-matrix reads in flight per batch: at most 16 values per lane
+  constexpr int SB = CPL >= 16 ? 1 : (16 / CPL < STEPS ? 16 / CPL : STEPS);
+  double vbuf[SB][CPL];
+  int obuf[SB][CPL];
+  auto issue = [&](int st, int sb) {
+    const int sidx = st * GROUPS + grp;
+    int es[3] = {0, 0, 0};
+    bool ok = sidx < NSUP;
+    {
+      int rem = sidx;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int sd = rem % N1;
+        rem /= N1;
+        ok = ok && sd < ne[d];
+        es[d] = lo[d] + (ok ? sd : 0);
+      }
+    }
+    int e = 0, estride = 1, li = 0, lstride = 1, fo[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int f = pt.first[d][es[d]];
+      e += es[d] * estride;
+      estride *= pt.n_elem[d];
+      li += (id[d] - f) * lstride;
+      lstride *= N1;
+      fo[d] = f - id[d] + P;
+    }
+    // rows li*VEC .. li*VEC + VEC-1 of the element matrix are contiguous
+    const T* krow = ke + (size_t)e * NLV * NLV + (size_t)li * VEC * NLV;
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+      const int v = gl + u * NLVP;
+      obuf[sb][u] = -1;
+      vbuf[sb][u] = 0.0;
+      if (!ok || v >= VALS) continue;
+      const int rc = v / NLV, c = v - rc * NLV;
+      const int f = c / VEC, cj = c - f * VEC;
+      int rem = f, o = 0, ostride = 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        o += (fo[d] + rem % N1) * ostride;
+        rem /= N1;
+        ostride *= BW;
+      }
+      obuf[sb][u] = (grp * VEC + rc) * BOXV + o * VEC + cj;
+      vbuf[sb][u] = (double)krow[v];
+    }
+  };
+  auto accumulate = [&](int sb) {
+#pragma unroll
+    for (int u = 0; u < CPL; ++u)
+      if (obuf[sb][u] >= 0) mybox[obuf[sb][u]] += vbuf[sb][u];
+    wave_lds_sync();
+  };
+  // first batch issued before the column test so both sets of loads overlap
+#pragma unroll
+  for (int sb = 0; sb < SB; ++sb) issue(sb, sb);
+  // column test per box entry, independent of the element-matrix reads
+  uint64_t mask[CHUNKS];
+#pragma unroll
+  for (int ch = 0; ch < CHUNKS; ++ch) {
+    const int t = ch * kWave + lane;
+    bool present = false;
+    if (t < BOXV) {
+      const int node = t / VEC, cj = t - node * VEC;
+      int rem = node, full = 0, stride = 1;
+      bool ok = true;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int jd = id[d] + rem % BW - P;
+        rem /= BW;
+        ok = ok && jd >= 0 && jd < pt.n_cp[d];
+        if (ok) ok = max(lo[d], pt.elo[d][jd]) <= min(lo[d] + ne[d] - 1, pt.ehi[d][jd]);
+        full += jd * stride;
+        stride *= pt.n_cp[d];
+      }
+      present = ok && f2f[(size_t)full * VEC + cj] >= 0;
+    }
+    mask[ch] = __ballot(present);
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int sb = 0; sb < SB; ++sb) accumulate(sb);
+  for (int st0 = SB; st0 < STEPS; st0 += SB) {
+#pragma unroll
+    for (int sb = 0; sb < SB; ++sb)
+      if (st0 + sb < STEPS) issue(st0 + sb, sb);
+#pragma unroll
+    for (int sb = 0; sb < SB; ++sb)
+      if (st0 + sb < STEPS) accumulate(sb);
+  }
+  constexpr int CENTER = DIM == 2 ? P * BW + P : (P * BW + P) * BW + P;
+  double* diag = diag_all ? diag_all + b * n_free : nullptr;
+#pragma unroll
+  for (int rc = 0; rc < VEC; ++rc) {
+    const int row = rowc[rc];
+    if (row < 0) continue;
+    const int k0 = indptr[row], k1 = indptr[row + 1];
+    int base = 0;
+#pragma unroll
+    for (int ch = 0; ch < CHUNKS; ++ch) {
+      const int t = ch * kWave + lane;
+      const uint64_t m = mask[ch];
+      const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+      if (((m >> lane) & 1ull) && k0 + slot < k1) {
+        double acc = mybox[rc * BOXV + t];
+#pragma unroll
+        for (int g = 1; g < GROUPS; ++g) acc += mybox[(g * VEC + rc) * BOXV + t];
+        vals[k0 + slot] = (T)acc;
+        if (diag && t == CENTER * VEC + rc) diag[row] += acc;
+      }
+      base += __popcll(m);
+    }
   }
 }
 
@@ -635,17 +1137,39 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
     ke = nullptr;
   }
   const int passes = args.passes ? args.passes : 3;
+  // with the row-form gather, the diagonal comes from the assembled rows
+  const bool row_gather = args.vals && args.nnz > 0 && args.indptr && args.n_free > 0;
+  AsmDev ae = a;
+  if (row_gather) ae.diag = nullptr;
   if (passes & 1) {
-    hipLaunchKernelGGL((k_element<DIM, P, PROB, T>), dim3(n_e, batch), dim3(threads), lds, st, pt,
-                       nq1, rule.pts, rule.wts, a, ke);
+    bool done = false;
+    if constexpr (DIM == 2 && P <= 3) {
+      // planar, at most 4 x 4 functions and Gauss points: one wave per element
+      if (nq1 <= 4 && a.vec == S::VEC) {
+        hipLaunchKernelGGL((k_element_w<P, PROB, T>), dim3((unsigned)((n_e + 3) / 4), batch),
+                           dim3(256), 0, st, pt, nq1, rule.pts, rule.wts, ae, ke, n_e);
+        done = true;
+      }
+    }
+    if (!done)
+      hipLaunchKernelGGL((k_element<DIM, P, PROB, T>), dim3(n_e, batch), dim3(threads), lds, st,
+                         pt, nq1, rule.pts, rule.wts, ae, ke);
     IGA_CHECK_LAUNCH();
   }
   if ((passes & 2) && args.vals && args.nnz > 0) {
-    if (args.indptr && args.n_free > 0) {
+    int n_pts = 1;
+    for (int d = 0; d < DIM; ++d) n_pts *= pt.n_cp[d];
+    if (row_gather && args.vec == S::VEC) {
+      hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T>), dim3((unsigned)((n_pts + 3) / 4), batch),
+                         dim3(256), 0, st, pt, n_pts, args.indptr, args.full_to_free, ke,
+                         reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
+                         (int64_t)n_e);
+    } else if (row_gather) {
       const unsigned blocks = (unsigned)((args.n_free + 3) / 4);
       hipLaunchKernelGGL((k_gather_rows<DIM, P, PROB, T>), dim3(blocks, batch), dim3(256), 0, st,
-                         pt, args.vec, args.n_free, args.indptr, args.cols, args.freed, ke,
-                         reinterpret_cast<T*>(args.vals), args.nnz, (int64_t)n_e);
+                         pt, args.vec, args.n_free, args.indptr, args.cols, args.freed,
+                         args.full_to_free, ke, reinterpret_cast<T*>(args.vals), args.diag,
+                         args.n_free, args.nnz, (int64_t)n_e);
     } else {
       const int64_t blocks = std::min<int64_t>((args.nnz * batch + 255) / 256, 256 * 64);
       hipLaunchKernelGGL((k_gather<DIM, P, PROB, T>), dim3((unsigned)blocks), dim3(256), 0, st,
@@ -710,6 +1234,29 @@ extern "C" int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule,
   a.coef_const = args->coef_const;
   a.coef_points = args->coef_points;
   for (int i = 0; i < 36; ++i) a.cmat[i] = args->cmat[i];
+  // E^cd_ab from the Voigt matrix: the strain s = (p, q) holds d u_p / d x_q
+  // (and d u_q / d x_p when p != q)
+  for (int t = 0; t < 81; ++t) a.ecd[t] = 0.0;
+  if (args->problem == IGA_ELASTICITY) {
+    const int D = patch->dim, NSv = D == 2 ? 3 : 6;
+    const int vp2[3] = {0, 1, 0}, vq2[3] = {0, 1, 1};
+    const int vp3[6] = {0, 1, 2, 1, 0, 0}, vq3[6] = {0, 1, 2, 2, 2, 1};
+    const int* vp = D == 2 ? vp2 : vp3;
+    const int* vq = D == 2 ? vq2 : vq3;
+    auto holds = [&](int s, int u, int x) {
+      return (u == vp[s] && x == vq[s]) || (vp[s] != vq[s] && u == vq[s] && x == vp[s]);
+    };
+    for (int cc = 0; cc < D; ++cc)
+      for (int dd = 0; dd < D; ++dd)
+        for (int aa = 0; aa < D; ++aa)
+          for (int bb = 0; bb < D; ++bb) {
+            double v = 0.0;
+            for (int s1 = 0; s1 < NSv; ++s1)
+              for (int s2 = 0; s2 < NSv; ++s2)
+                if (holds(s1, aa, cc) && holds(s2, bb, dd)) v += args->cmat[s1 * NSv + s2];
+            a.ecd[((cc * D + dd) * D + aa) * D + bb] = v;
+          }
+  }
   a.src_kind = args->src_kind;
   for (int i = 0; i < 3; ++i) a.src_const[i] = args->src_const[i];
   a.src_points = args->src_points;
