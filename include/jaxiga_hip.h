@@ -73,6 +73,10 @@ typedef struct iga_patch {
   const double* wgts;
   const int32_t* elo[3]; /* (n_cp[d]) first span supporting each 1-D basis function */
   const int32_t* ehi[3]; /* (n_cp[d]) last span supporting each 1-D basis function */
+  int32_t regular;       /* 1 when no interior knot repeats in any direction:
+                            first[e] = e, elo[i] = max(0, i - p),
+                            ehi[i] = min(n_elem - 1, i) (kernels may use the
+                            closed forms instead of the tables) */
 } iga_patch_t;
 
 /* Gauss rule: the same n_q1 points per direction, tensor order first fastest. */

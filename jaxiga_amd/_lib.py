@@ -34,6 +34,7 @@ class IgaPatch(ctypes.Structure):
         ("dim", c_int), ("degree", c_int), ("n_elem", c_int * 3), ("n_cp", c_int * 3),
         ("ext", c_void_p * 3), ("first", c_void_p * 3), ("lo", c_void_p * 3), ("hi", c_void_p * 3),
         ("cpts", c_void_p), ("wgts", c_void_p), ("elo", c_void_p * 3), ("ehi", c_void_p * 3),
+        ("regular", c_int),
     ]
 
 

@@ -932,14 +932,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Pass 2 (control-point form): one wavefront per control point, covering the
-// rows of all its components (VEC rows share the supporting elements, the
-// stencil box and the column set). Per supporting element the wave reads the
-// VEC adjacent element-matrix rows -- VEC * NLV contiguous values -- into the
-// component boxes; the column test (in the patch, free, overlapping support)
-// is issued before the accumulation so its loads overlap the element-matrix
-// reads. Write-out as in the row form, once per free component row.
-template <int DIM, int P, int PROB, typename T>
+// Pass 2 (control-point form): one wavefront per control point, covering
+// the rows of all its components (VEC rows share the supporting elements,
+// the stencil box and the column set). Per supporting element the wave reads
+// the VEC adjacent element-matrix rows -- VEC * NLV contiguous values -- and
+// adds them into the component boxes in LDS with atomic adds: within one
+// instruction every lane hits its own box entry and a wave's LDS operations
+// execute in program order, so the summation order is fixed (bitwise
+// reproducible) with no read-modify-write round trip per element. Every
+// element-matrix read is issued unconditionally and at once, then the
+// free-dof reads of the column test (in the patch, free, overlapping
+// support); on a regular patch the support tables are closed forms, so no
+// dependent read stands between the point index and the element matrices.
+// Write-out as in the row form, once per free component row.
+template <int DIM, int P, int PROB, typename T, bool REG>
 __global__ void __launch_bounds__(256)
     k_gather_points(iga_patch_t pt, int32_t n_points, const int32_t* __restrict__ indptr,
                     const int32_t* __restrict__ f2f, const T* __restrict__ ke_all,
@@ -957,42 +963,65 @@ __global__ void __launch_bounds__(256)
   constexpr int CPL = (VALS + NLVP - 1) / NLVP;
   constexpr int WAVES = 4;
   constexpr int CHUNKS = (BOXV + kWave - 1) / kWave;
-  __shared__ double box[WAVES][GROUPS][VEC][BOXV];
+  __shared__ double box[WAVES][GROUPS * VEC * BOXV];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
   const int pnt = blockIdx.x * WAVES + wave;
   if (pnt >= n_points) return;
-  int rowc[VEC];
-  bool any_free = false;
-#pragma unroll
-  for (int c = 0; c < VEC; ++c) {
-    rowc[c] = f2f[(size_t)pnt * VEC + c];
-    any_free |= rowc[c] >= 0;
-  }
-  if (!any_free) return;
   const int64_t b = blockIdx.y;
   const T* ke = ke_all + b * n_e * NLV * NLV;
   T* vals = vals_all + b * nnz;
-  double* mybox = &box[wave][0][0][0];
+  double* mybox = box[wave];
   for (int t = lane; t < GROUPS * VEC * BOXV; t += kWave) mybox[t] = 0.0;
+  // support tables, closed forms on a regular patch
+  auto elo = [&](int d, int i) {
+    if constexpr (REG) return max(0, i - P);
+    else return pt.elo[d][i];
+  };
+  auto ehi = [&](int d, int i) {
+    if constexpr (REG) return min(pt.n_elem[d] - 1, i);
+    else return pt.ehi[d][i];
+  };
+  auto first = [&](int d, int e) {
+    if constexpr (REG) return e;
+    else return pt.first[d][e];
+  };
 
-  int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, ne[3] = {1, 1, 1};
+  int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, ne[3] = {1, 1, 1}, rowc[VEC];
   {
     int rem = pnt;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       id[d] = rem % pt.n_cp[d];
       rem /= pt.n_cp[d];
-      lo[d] = pt.elo[d][id[d]];
-      ne[d] = pt.ehi[d][id[d]] - lo[d] + 1;
+      lo[d] = elo(d, id[d]);
+      ne[d] = ehi(d, id[d]) - lo[d] + 1;
     }
   }
+#pragma unroll
+  for (int c = 0; c < VEC; ++c) rowc[c] = f2f[(size_t)pnt * VEC + c];
+  // lane part of the box offset of each value it reads
   const int grp = lane / NLVP, gl = lane % NLVP;
+  int loff[CPL];
+#pragma unroll
+  for (int u = 0; u < CPL; ++u) {
+    const int v = gl + u * NLVP;
+    const int rc = v / NLV, c = v - rc * NLV;
+    const int f = c / VEC, cj = c - f * VEC;
+    int rem = f, o = 0, ostride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      o += (rem % N1) * ostride;
+      rem /= N1;
+      ostride *= BW;
+    }
+    loff[u] = v < VALS ? (grp * VEC + rc) * BOXV + o * VEC + cj : -1;
+  }
   constexpr int NSUP = S::NL;
   constexpr int STEPS = (NSUP + GROUPS - 1) / GROUPS;
   // element-matrix reads in flight per batch: at most 16 values per lane
   constexpr int SB = CPL >= 16 ? 1 : (16 / CPL < STEPS ? 16 / CPL : STEPS);
-  double vbuf[SB][CPL];
-  int obuf[SB][CPL];
+  T vbuf[SB][CPL];
+  int sofs[SB];  // element part of the box offset, -1 when no such element
   auto issue = [&](int st, int sb) {
     const int sidx = st * GROUPS + grp;
     int es[3] = {0, 0, 0};
@@ -1007,52 +1036,46 @@ __global__ void __launch_bounds__(256)
         es[d] = lo[d] + (ok ? sd : 0);
       }
     }
-    int e = 0, estride = 1, li = 0, lstride = 1, fo[3] = {0, 0, 0};
+    int e = 0, estride = 1, li = 0, lstride = 1, o = 0, ostride = 1;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      const int f = pt.first[d][es[d]];
+      const int f = first(d, es[d]);
       e += es[d] * estride;
       estride *= pt.n_elem[d];
       li += (id[d] - f) * lstride;
       lstride *= N1;
-      fo[d] = f - id[d] + P;
+      o += (f - id[d] + P) * ostride;  // box offset of the element's first function
+      ostride *= BW;
     }
-    // rows li*VEC .. li*VEC + VEC-1 of the element matrix are contiguous
+    sofs[sb] = ok ? o * VEC : -1;
+    // rows li*VEC .. li*VEC + VEC-1 of the element matrix are contiguous.
+    // The reads are unconditional (an absent element reads the first
+    // supporting one, skipped when adding): a conditional read would make
+    // the compiler wait for each one before issuing the next.
     const T* krow = ke + (size_t)e * NLV * NLV + (size_t)li * VEC * NLV;
 #pragma unroll
     for (int u = 0; u < CPL; ++u) {
       const int v = gl + u * NLVP;
-      obuf[sb][u] = -1;
-      vbuf[sb][u] = 0.0;
-      if (!ok || v >= VALS) continue;
-      const int rc = v / NLV, c = v - rc * NLV;
-      const int f = c / VEC, cj = c - f * VEC;
-      int rem = f, o = 0, ostride = 1;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        o += (fo[d] + rem % N1) * ostride;
-        rem /= N1;
-        ostride *= BW;
-      }
-      obuf[sb][u] = (grp * VEC + rc) * BOXV + o * VEC + cj;
-      vbuf[sb][u] = (double)krow[v];
+      vbuf[sb][u] = krow[v < VALS ? v : VALS - 1];
     }
   };
   auto accumulate = [&](int sb) {
+    if (sofs[sb] < 0) return;
 #pragma unroll
     for (int u = 0; u < CPL; ++u)
-      if (obuf[sb][u] >= 0) mybox[obuf[sb][u]] += vbuf[sb][u];
-    wave_lds_sync();
+      if (loff[u] >= 0)
+        __hip_atomic_fetch_add(mybox + sofs[sb] + loff[u], (double)vbuf[sb][u], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
   };
-  // first batch issued before the column test so both sets of loads overlap
 #pragma unroll
   for (int sb = 0; sb < SB; ++sb) issue(sb, sb);
-  // column test per box entry, independent of the element-matrix reads
-  uint64_t mask[CHUNKS];
+  // column test, in flight with the element-matrix reads: free index of each
+  // box entry, or -1 (unconditional reads: the point's own dof out of the box)
+  int colf[CHUNKS];
 #pragma unroll
   for (int ch = 0; ch < CHUNKS; ++ch) {
     const int t = ch * kWave + lane;
-    bool present = false;
+    colf[ch] = -1;
     if (t < BOXV) {
       const int node = t / VEC, cj = t - node * VEC;
       int rem = node, full = 0, stride = 1;
@@ -1062,14 +1085,17 @@ __global__ void __launch_bounds__(256)
         const int jd = id[d] + rem % BW - P;
         rem /= BW;
         ok = ok && jd >= 0 && jd < pt.n_cp[d];
-        if (ok) ok = max(lo[d], pt.elo[d][jd]) <= min(lo[d] + ne[d] - 1, pt.ehi[d][jd]);
+        if (ok) ok = max(lo[d], elo(d, jd)) <= min(lo[d] + ne[d] - 1, ehi(d, jd));
         full += jd * stride;
         stride *= pt.n_cp[d];
       }
-      present = ok && f2f[(size_t)full * VEC + cj] >= 0;
+      const int fv = f2f[ok ? (size_t)full * VEC + cj : (size_t)pnt * VEC];
+      colf[ch] = ok ? fv : -1;
     }
-    mask[ch] = __ballot(present);
   }
+  uint64_t mask[CHUNKS];
+#pragma unroll
+  for (int ch = 0; ch < CHUNKS; ++ch) mask[ch] = __ballot(colf[ch] >= 0);
   wave_lds_sync();
 #pragma unroll
   for (int sb = 0; sb < SB; ++sb) accumulate(sb);
@@ -1081,6 +1107,7 @@ __global__ void __launch_bounds__(256)
     for (int sb = 0; sb < SB; ++sb)
       if (st0 + sb < STEPS) accumulate(sb);
   }
+  wave_lds_sync();
   constexpr int CENTER = DIM == 2 ? P * BW + P : (P * BW + P) * BW + P;
   double* diag = diag_all ? diag_all + b * n_free : nullptr;
 #pragma unroll
@@ -1160,10 +1187,17 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
     int n_pts = 1;
     for (int d = 0; d < DIM; ++d) n_pts *= pt.n_cp[d];
     if (row_gather && args.vec == S::VEC) {
-      hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T>), dim3((unsigned)((n_pts + 3) / 4), batch),
-                         dim3(256), 0, st, pt, n_pts, args.indptr, args.full_to_free, ke,
-                         reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
-                         (int64_t)n_e);
+      dim3 grid((unsigned)((n_pts + 3) / 4), batch);
+      if (pt.regular)
+        hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, true>), grid, dim3(256), 0, st, pt,
+                           n_pts, args.indptr, args.full_to_free, ke,
+                           reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
+                           (int64_t)n_e);
+      else
+        hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, false>), grid, dim3(256), 0, st, pt,
+                           n_pts, args.indptr, args.full_to_free, ke,
+                           reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
+                           (int64_t)n_e);
     } else if (row_gather) {
       const unsigned blocks = (unsigned)((args.n_free + 3) / 4);
       hipLaunchKernelGGL((k_gather_rows<DIM, P, PROB, T>), dim3(blocks, batch), dim3(256), 0, st,

@@ -48,6 +48,25 @@ class BasisData:
         return self.R.reshape(n, -1), self.dR.reshape(n, self.dR.shape[2], -1), self.x.reshape(n, -1)
 
 
+def _point_inputs(ps, dev):
+    """Device copies of a point set's element list, reference coordinates,
+    weights and sides, uploaded once per point set (a point set is immutable)
+    so repeated evaluations issue no host-to-device copies."""
+    import torch
+
+    cached = getattr(ps, "_device_inputs", None)
+    if cached is not None and cached[0] == dev:
+        return cached[1]
+    f64 = dict(dtype=torch.float64, device=dev)
+    elems = torch.as_tensor(np.asarray(ps.elems), dtype=torch.int32, device=dev)
+    ref = torch.as_tensor(np.ascontiguousarray(ps.ref), **f64)
+    wref = None if ps.weights is None else torch.as_tensor(np.ascontiguousarray(ps.weights), **f64)
+    sides = None if ps.sides is None else torch.as_tensor(np.asarray(ps.sides), dtype=torch.int32, device=dev)
+    out = (elems, ref, wref, sides)
+    object.__setattr__(ps, "_device_inputs", (dev, out))
+    return out
+
+
 def evaluate(space, ps, order: int = 1, outputs=("R", "dR", "x", "w", "normal", "detJ")) -> BasisData:
     """Evaluate ``space``'s basis at the points of ``ps`` on the GPU."""
     import torch
@@ -61,10 +80,7 @@ def evaluate(space, ps, order: int = 1, outputs=("R", "dR", "x", "w", "normal", 
     n_q = ps.n_q
     n_l = space.n_local
     f64 = dict(dtype=torch.float64, device=dev)
-    elems = torch.as_tensor(np.asarray(ps.elems), dtype=torch.int32, device=dev)
-    ref = torch.as_tensor(np.ascontiguousarray(ps.ref), **f64)
-    wref = None if ps.weights is None else torch.as_tensor(np.ascontiguousarray(ps.weights), **f64)
-    sides = None if ps.sides is None else torch.as_tensor(np.asarray(ps.sides), dtype=torch.int32, device=dev)
+    elems, ref, wref, sides = _point_inputs(ps, dev)
     want = set(outputs)
     R = torch.empty((n_e, n_q, n_l), **f64) if "R" in want else None
     dR = torch.empty((n_e, n_q, dim, n_l), **f64) if "dR" in want else None

@@ -237,6 +237,23 @@ class FunctionSpace:
         return self._device
 
 
+def _regular_spans(space, sup) -> bool:
+    """True when no interior knot repeats in any direction, so that span e
+    starts at function e and function i is supported on spans
+    max(0, i - p) .. min(n_elem - 1, i)."""
+    p = space.degree[0]
+    for d in range(space.dim):
+        ne = space.n_elem_per_dir[d]
+        n = space.patch.n_cp_per_dir[d]
+        i = np.arange(n)
+        elo, ehi = sup[d]
+        if (n != ne + p or not np.array_equal(space._spans[d][2], np.arange(ne))
+                or not np.array_equal(elo, np.maximum(0, i - p))
+                or not np.array_equal(ehi, np.minimum(ne - 1, i))):
+            return False
+    return True
+
+
 class DeviceSpace:
     """Patch tables in HBM plus the ``iga_patch_t`` that points at them."""
 
@@ -286,6 +303,7 @@ class DeviceSpace:
                 s.n_cp[d] = 1
         s.cpts = self.cpts.data_ptr()
         s.wgts = self.wgts.data_ptr()
+        s.regular = int(_regular_spans(space, sup))
         self.struct = s
 
 
