@@ -28,6 +28,9 @@
  *   iga_inv_guard             solvers/linear.py:426-430    (Jacobi inverse, zero guard)
  *   iga_residual_vec/iga_axpy solvers/diagnostics.py:8-19 (FP64 outer residual of the
  *                             mixed-precision refinement; no reference counterpart)
+ *   iga_cg_solve_mixed        solvers/linear.py:324-351    (the same Jacobi-PCG, on an
+ *                             FP32 matrix with FP64 residual replacement; the
+ *                             reference runs FP64 only)
  *   iga_net_* / iga_nmse / iga_adam / iga_mg_* / iga_masked_spgemm /
  *   iga_stencil_* / iga_dense_inverse / iga_csr_f32 / iga_energy_loss / iga_fcg_mg
  *                             (no reference counterpart: the learned preconditioner,
@@ -198,6 +201,19 @@ int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const vo
                  int32_t precision, const double* dinv, const double* b, double* x, double tol,
                  int32_t maxiter, double* work, int32_t* out_iters, double* out_rr, double* out_bb,
                  void* stream);
+
+/* Jacobi-PCG on the FP32 matrix vals32 with FP64 vectors, dot products and
+ * recurrences, and reliable updates: every replace_every iterations (>= 1),
+ * and whenever the recursion reports convergence, r is recomputed as
+ * b - A x with the FP64 matrix vals64 (same pattern) while the search
+ * direction is kept; the solve stops only on the true FP64 residual
+ * ||b - A64 x||_2 <= tol ||b||_2 (or at maxiter). work: iga_cg_workspace(n).
+ * out: iterations, final true ||r||^2, ||b||^2.                               */
+int iga_cg_solve_mixed(int32_t n, const int32_t* indptr, const int32_t* cols,
+                       const float* vals32, const double* vals64, const double* dinv,
+                       const double* b, double* x, double tol, int32_t maxiter,
+                       int32_t replace_every, double* work, int32_t* out_iters, double* out_rr,
+                       double* out_bb, void* stream);
 
 /* r = b - A u (FP64) and ||r||^2, ||b||^2 into out[0], out[1]; work 2048 doubles */
 int iga_residual_vec(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,

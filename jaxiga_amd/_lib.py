@@ -112,6 +112,9 @@ _SIGNATURES = {
     "iga_cg_workspace": (c_i64, [c_int]),
     "iga_cg_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                              c_void_p, c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_cg_solve_mixed": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_double, c_int, c_int, c_void_p, P_int,
+                                   P_dbl, P_dbl, c_void_p]),
     "iga_residual_vec": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     "iga_axpy": (c_int, [c_int, c_double, c_void_p, c_void_p, c_void_p]),

@@ -38,6 +38,10 @@ struct AsmDev {
   int n_free;  // rhs/diag stride between batch instances
   // E^cd_ab = sum_st C_st Bs(s; a, c) Bs(t; b, d), index ((c*dim + d)*dim + a)*dim + b
   double ecd[81];
+  // planar elasticity with no normal-shear coupling (C_02 = C_12 = 0): only
+  // eight of the sixteen E^cd_ab are nonzero
+  int ortho;
+  float ecd32[16];  // planar E^cd_ab rounded to FP32 (FP32 combination)
 };
 
 // Orders the LDS accesses of one wavefront: lanes exchange data through LDS
@@ -427,7 +431,7 @@ __global__ void __launch_bounds__(256)
 // from the lane's registers. The geometry (W, dW, x, dx/dxi: nine tensor
 // sums per Gauss point) is sum-factorised once per point, FP64, and shared
 // through LDS; no workgroup barrier is needed.
-template <int P, int PROB, typename T>
+template <int P, int PROB, typename T, bool REG>
 __global__ void __launch_bounds__(256)
     k_element_w(iga_patch_t pt, int nq1, const double* __restrict__ qp,
                 const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all, int n_e) {
@@ -459,49 +463,78 @@ __global__ void __launch_bounds__(256)
   double* diagp = a.diag ? a.diag + bi * a.n_free : nullptr;
   T* kout = ke_all ? ke_all + (bi * n_e + e) * (int64_t)(NLV * NLV) : nullptr;
   const int ei0 = e % pt.n_elem[0], ei1 = e / pt.n_elem[0];
+
+  // ---- every global read of the element, issued together ------------------
+  // Unconditional reads from clamped (valid) addresses, selected afterwards:
+  // a read under a condition would be waited for before the next one.
   const double h0 = pt.hi[0][ei0] - pt.lo[0][ei0], h1 = pt.hi[1][ei1] - pt.lo[1][ei1];
+  // table lane: direction td, function ta, Gauss point tq (meaningful for l < 32)
+  const int td = (l >> 4) & 1, ta = (l >> 2) & 3, tq = l & 3;
+  const double xq = qp[tq < nq1 ? tq : nq1 - 1];
+  double crow[N1];
+  {
+    const int fa = ta < N1 ? ta : N1 - 1;
+    const double* C = td ? pt.ext[1] + ((size_t)ei1 * N1 + fa) * N1
+                         : pt.ext[0] + ((size_t)ei0 * N1 + fa) * N1;
+#pragma unroll
+    for (int b = 0; b < N1; ++b) crow[b] = C[b];
+  }
+  // function lane i = l % 16 (i0 = i % 4, i1 = i / 4)
+  const int fi0 = l & 3, fi1 = (l >> 2) & 3;
+  const bool fvalid = fi0 < N1 && fi1 < N1;
+  int sdof;
+  {
+    const int f0 = REG ? ei0 : pt.first[0][ei0], f1 = REG ? ei1 : pt.first[1][ei1];
+    sdof = (f0 + (fi0 < N1 ? fi0 : N1 - 1)) + (f1 + (fi1 < N1 ? fi1 : N1 - 1)) * pt.n_cp[0];
+  }
+  const double wt_r = pt.wgts[sdof];
+  const double px_r = pt.cpts[(size_t)sdof * 2], py_r = pt.cpts[(size_t)sdof * 2 + 1];
+  int fd_r[VEC];
+#pragma unroll
+  for (int c = 0; c < VEC; ++c) fd_r[c] = a.f2f[(size_t)VEC * sdof + c];
+  // record lane q = l % 16 (q0 = q % 4, q1 = q / 4)
+  const int rq0 = l & 3, rq1 = (l >> 2) & 3;
+  const bool qvalid = rq0 < nq1 && rq1 < nq1;
+  const int cq0 = rq0 < nq1 ? rq0 : nq1 - 1, cq1 = rq1 < nq1 ? rq1 : nq1 - 1;
+  const int qn = cq0 + nq1 * cq1;
+  const double qwt = qw[cq0] * qw[cq1];
+  // coefficient and source at the point: raw reads now, selected in the
+  // record phase (selecting here would place the wait here)
+  const double coef_r = *(coefp ? coefp + (size_t)e * nq + qn : qw);
+  double src_r[VEC];
+  {
+    const double* sp = srcp ? srcp + ((size_t)e * nq + qn) * VEC : qw;
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) src_r[c] = sp[srcp ? c : 0];
+  }
 
   // ---- 1-D extracted tables: lane (dir, a, q) for l < 32 --------------------
   if (l < 32) {
-    const int d = l >> 4, fa = (l >> 2) & 3, q = l & 3;
     double v = 0.0, dv = 0.0;
-    if (fa < N1 && q < nq1) {
+    if (ta < N1 && tq < nq1) {
       double B[N1], dB[N1];
-      bernstein<P>(qp[q], B, dB);
-      const double* C = (d ? pt.ext[1] + ((size_t)ei1 * N1 + fa) * N1
-                           : pt.ext[0] + ((size_t)ei0 * N1 + fa) * N1);
+      bernstein<P>(xq, B, dB);
 #pragma unroll
       for (int b = 0; b < N1; ++b) {
-        v += C[b] * B[b];
-        dv += C[b] * dB[b];
+        v += crow[b] * B[b];
+        dv += crow[b] * dB[b];
       }
-      dv *= 2.0 / (d ? h1 : h0);
+      dv *= 2.0 / (td ? h1 : h0);
     }
-    s_tabq[w][d][0][q][fa] = v;
-    s_tabq[w][d][1][q][fa] = dv;
-    s_taba[w][d][0][fa][q] = v;
-    s_taba[w][d][1][fa][q] = dv;
+    s_tabq[w][td][0][tq][ta] = v;
+    s_tabq[w][td][1][tq][ta] = dv;
+    s_taba[w][td][0][ta][tq] = v;
+    s_taba[w][td][1][ta][tq] = dv;
   }
   // ---- local functions: weights, weighted control points, dof indices -------
   if (l < 16) {
-    const int i0 = l & 3, i1 = l >> 2;
-    double wt = 0.0, px = 0.0, py = 0.0;
-    int sdof = 0;
-    if (i0 < N1 && i1 < N1) {
-      sdof = (pt.first[0][ei0] + i0) + (pt.first[1][ei1] + i1) * pt.n_cp[0];
-      wt = pt.wgts[sdof];
-      px = pt.cpts[(size_t)sdof * 2];
-      py = pt.cpts[(size_t)sdof * 2 + 1];
+    const double wt = fvalid ? wt_r : 0.0;
 #pragma unroll
-      for (int c = 0; c < VEC; ++c) s_fdof[w][l * VEC + c] = a.f2f[(size_t)VEC * sdof + c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < VEC; ++c) s_fdof[w][l * VEC + c] = -1;
-    }
-    s_gdof[w][l] = sdof;
+    for (int c = 0; c < VEC; ++c) s_fdof[w][l * VEC + c] = fvalid ? fd_r[c] : -1;
+    s_gdof[w][l] = fvalid ? sdof : 0;
     s_coef[w][0][l] = wt;
-    s_coef[w][1][l] = wt * px;
-    s_coef[w][2][l] = wt * py;
+    s_coef[w][1][l] = wt * px_r;
+    s_coef[w][2][l] = wt * py_r;
   }
   wave_lds_sync();
 
@@ -539,11 +572,11 @@ __global__ void __launch_bounds__(256)
   wave_lds_sync();
   // ---- Gauss-point records: 1/W, dW, inverse Jacobian, weights ---------------
   if (l < 16) {
-    const int q = l, q0 = q & 3, q1 = q >> 2;
+    const int q = l;
     double rec[REC];
 #pragma unroll
     for (int t = 0; t < REC; ++t) rec[t] = 0.0;
-    if (q0 < nq1 && q1 < nq1) {
+    if (qvalid) {
       const double W = s_geo[w][q][0][0], dW0 = s_geo[w][q][0][1], dW1 = s_geo[w][q][0][2];
       const double iW = 1.0 / W;
       const double x0 = s_geo[w][q][1][0] * iW, x1 = s_geo[w][q][2][0] * iW;
@@ -554,11 +587,7 @@ __global__ void __launch_bounds__(256)
       const double j11 = (s_geo[w][q][2][2] - x1 * dW1) * iW;
       const double det = j00 * j11 - j01 * j10;
       const double id = 1.0 / det;
-      const double wq = 0.25 * h0 * h1 * fabs(det) * qw[q0] * qw[q1];
-      const int qn = q0 + nq1 * q1;
-      double wk = wq;
-      if constexpr (PROB == IGA_POISSON)
-        wk *= a.coef_kind == IGA_COEF_POINTS ? coefp[(size_t)e * nq + qn] : a.coef_const;
+      const double wq = 0.25 * h0 * h1 * fabs(det) * qwt;
       rec[0] = iW;
       rec[1] = dW0;
       rec[2] = dW1;
@@ -567,12 +596,15 @@ __global__ void __launch_bounds__(256)
       rec[4] = -j01 * id;
       rec[5] = -j10 * id;
       rec[6] = j00 * id;
-      rec[7] = wk;
+      double coefv = 1.0;
+      if constexpr (PROB == IGA_POISSON)
+        coefv = a.coef_kind == IGA_COEF_POINTS ? coef_r : a.coef_const;
+      rec[7] = wq * coefv;
 #pragma unroll
       for (int c = 0; c < VEC; ++c)
-        rec[8 + c] = a.src_kind == IGA_SRC_CONST    ? wq * a.src_const[c]
-                     : a.src_kind == IGA_SRC_POINTS ? wq * srcp[((size_t)e * nq + qn) * VEC + c]
-                                                    : 0.0;
+        rec[8 + c] = a.src_kind == IGA_SRC_POINTS  ? wq * src_r[c]
+                     : a.src_kind == IGA_SRC_CONST ? wq * a.src_const[c]
+                                                   : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < REC; ++t) s_rec[w][q][t] = rec[t];
@@ -582,7 +614,8 @@ __global__ void __launch_bounds__(256)
   // ---- gradient products on the matrix cores --------------------------------
   const int i = l & 15, g = l >> 4;
   const double wi = s_coef[w][0][i];
-  const double a0 = s_tabq[w][0][0][g][i & 3], da0 = s_tabq[w][0][1][g][i & 3];
+  // weighted first-direction factors of the lane's function at q0 = g
+  const double a0w = s_tabq[w][0][0][g][i & 3] * wi, da0w = s_tabq[w][0][1][g][i & 3] * wi;
   typename Mfma<T>::acc_t acc[NCD];
 #pragma unroll
   for (int t = 0; t < NCD; ++t) acc[t] = Mfma<T>::zero();
@@ -594,10 +627,9 @@ __global__ void __launch_bounds__(256)
     const double* rc = s_rec[w][4 * k + g];
     const double b = s_taba[w][1][0][i >> 2][k], db = s_taba[w][1][1][i >> 2][k];
     const double iW = rc[0];
-    const double Nw = a0 * b * wi;
-    const double R = Nw * iW;
-    const double r0 = (da0 * b * wi - R * rc[1]) * iW;
-    const double r1 = (a0 * db * wi - R * rc[2]) * iW;
+    const double R = a0w * b * iW;
+    const double r0 = (da0w * b - R * rc[1]) * iW;
+    const double r1 = (a0w * db - R * rc[2]) * iW;
     const double gx = r0 * rc[3] + r1 * rc[5];
     const double gy = r0 * rc[4] + r1 * rc[6];
     const double wk = rc[7];
@@ -640,25 +672,38 @@ __global__ void __launch_bounds__(256)
   const bool any_pres = __ballot(pres) != 0;
   const int j = i;  // accumulator column
   const bool jok = (j & 3) < N1 && (j >> 2) < N1;
-  const int jc = (j & 3) + N1 * (j >> 2);  // compact local index
+  const int jc = N1 == 4 ? j : (j & 3) + N1 * (j >> 2);  // compact local index
+  // constitutive table in the accumulator type, converted once
+  T et[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) et[t] = std::is_same_v<T, float> ? (T)a.ecd32[t] : (T)a.ecd[t];
+  const bool ortho = a.ortho != 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int ir = Mfma<T>::row(g, r);
     const bool ok = jok && (ir & 3) < N1 && (ir >> 2) < N1;
     if (!ok) continue;
-    const int ic = (ir & 3) + N1 * (ir >> 2);
+    const int ic = N1 == 4 ? ir : (ir & 3) + N1 * (ir >> 2);
     // combined in the accumulator type
     T v[VEC][VEC];
     if constexpr (PROB == IGA_ELASTICITY) {
+      // E index ((c*2 + d)*2 + a)*2 + b, acc[c*2 + d] = M_cd
+      if (ortho) {
+        v[0][0] = et[0] * acc[0][r] + et[12] * acc[3][r];
+        v[0][1] = et[5] * acc[1][r] + et[9] * acc[2][r];
+        v[1][0] = et[6] * acc[1][r] + et[10] * acc[2][r];
+        v[1][1] = et[3] * acc[0][r] + et[15] * acc[3][r];
+      } else {
 #pragma unroll
-      for (int aa = 0; aa < 2; ++aa)
+        for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-          T s = (T)0;
+          for (int bb = 0; bb < 2; ++bb) {
+            T s = (T)0;
 #pragma unroll
-          for (int cd = 0; cd < 4; ++cd) s += (T)a.ecd[(cd * 2 + aa) * 2 + bb] * acc[cd][r];
-          v[aa][bb] = s;
-        }
+            for (int cd = 0; cd < 4; ++cd) s += et[(cd * 2 + aa) * 2 + bb] * acc[cd][r];
+            v[aa][bb] = s;
+          }
+      }
     } else {
       v[0][0] = acc[0][r];
     }
@@ -1173,8 +1218,13 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
     if constexpr (DIM == 2 && P <= 3) {
       // planar, at most 4 x 4 functions and Gauss points: one wave per element
       if (nq1 <= 4 && a.vec == S::VEC) {
-        hipLaunchKernelGGL((k_element_w<P, PROB, T>), dim3((unsigned)((n_e + 3) / 4), batch),
-                           dim3(256), 0, st, pt, nq1, rule.pts, rule.wts, ae, ke, n_e);
+        const dim3 grid((unsigned)((n_e + 3) / 4), batch);
+        if (pt.regular)
+          hipLaunchKernelGGL((k_element_w<P, PROB, T, true>), grid, dim3(256), 0, st, pt, nq1,
+                             rule.pts, rule.wts, ae, ke, n_e);
+        else
+          hipLaunchKernelGGL((k_element_w<P, PROB, T, false>), grid, dim3(256), 0, st, pt, nq1,
+                             rule.pts, rule.wts, ae, ke, n_e);
         done = true;
       }
     }
@@ -1290,6 +1340,14 @@ extern "C" int iga_assemble(const iga_patch_t* patch, const iga_rule_t* rule,
                 if (holds(s1, aa, cc) && holds(s2, bb, dd)) v += args->cmat[s1 * NSv + s2];
             a.ecd[((cc * D + dd) * D + aa) * D + bb] = v;
           }
+  }
+  for (int t = 0; t < 16; ++t) a.ecd32[t] = (float)a.ecd[t];
+  // the eight E^cd_ab that vanish when C_02 = C_12 = 0 (planar)
+  a.ortho = 0;
+  if (args->problem == IGA_ELASTICITY && patch->dim == 2) {
+    const int zero[8] = {1, 2, 4, 7, 8, 11, 13, 14};
+    a.ortho = 1;
+    for (int z : zero) a.ortho &= a.ecd[z] == 0.0;
   }
   a.src_kind = args->src_kind;
   for (int i = 0; i < 3; ++i) a.src_const[i] = args->src_const[i];

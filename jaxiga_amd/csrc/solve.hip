@@ -425,12 +425,71 @@ __global__ void __launch_bounds__(kVecThreads)
   store_partial(B.P_pq, s);
 }
 
+// Residual replacement for CG on an inexact operator: r = b - A x with the
+// exact (FP64) matrix, z = D^-1 r, and the r.z and r.r partials into the
+// given r.z slot (one partial per block; launched with the vector-kernel
+// block count, rows strided over the grid). `force` runs it after the loop
+// has stopped.
+template <int LPR>
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_replace(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                 const double* __restrict__ vals, const double* __restrict__ b,
+                 const double* __restrict__ dinv, const double* __restrict__ x, double* r,
+                 double* z, CGBufs B, int slot, int force) {
+  if (B.st->done && !force) return;
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  double rz = 0.0, rr = 0.0;
+  for (int i = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; i < n; i += groups) {
+    const double s = row_dot<LPR>(i, indptr, cols, vals, x, l);
+    if (l == 0) {
+      const double ri = b[i] - s;
+      const double zi = dinv[i] * ri;
+      r[i] = ri;
+      z[i] = zi;
+      rz += ri * zi;
+      rr += ri * ri;
+    }
+  }
+  store_partial(B.P_rz + slot * kRedBlocks, rz);
+  store_partial(B.P_rr, rr);
+}
+
+// After a forced replacement: the true residual decides whether to stop.
+__global__ void k_cg_recheck(int nblk, CGBufs B) {
+  const double rr = reduce_partials(B.P_rr, nblk);
+  if (threadIdx.x == 0) {
+    CGState* st = B.st;
+    st->rr = rr;
+    st->done = (!(rr > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
+  }
+}
+
+// When the check fails, the direction restarts from the replaced residual:
+// the recursion had run past the true residual, so the old direction is
+// orders of magnitude smaller than the new r and no longer pairs with it.
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg_restart(int n, const double* __restrict__ z, double* p, CGBufs B) {
+  if (B.st->done) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = z[i];
+}
+
 // Jacobi-PCG loop; `apply(p, q, B)` must produce q = A p and the p.q
 // partials (returns the number of partial blocks, or a negative status).
-template <typename Apply>
+struct NoReplace {
+  int every = 0;
+  int operator()(const double*, double*, double*, CGBufs, int, int) const { return IGA_OK; }
+};
+
+// `replace(x, r, z, B, slot, force)`, when `replace.every` > 0, swaps the
+// recursive residual for the true one every `every` iterations and once more
+// when the recursion reports convergence (reliable updates: the search
+// direction is kept, only r and z are refreshed).
+template <typename Apply, typename Replace = NoReplace>
 static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, double tol,
                    int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
-                   double* out_bb, hipStream_t st, Apply apply) {
+                   double* out_bb, hipStream_t st, Apply apply, Replace replace = Replace{}) {
   if (n == 0) {
     if (out_iters) *out_iters = 0;
     if (out_rr) *out_rr = 0.0;
@@ -449,17 +508,30 @@ static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, do
   CGState host{};
   const int batch = 16;
   int launched = 0;
+  bool verified = false;
   for (;;) {
     if (hipMemcpyAsync(&host, B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess)
       return IGA_ERR_HIP;
     if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
-    if (host.done) break;
+    if (host.done) {
+      if (replace.every <= 0 || verified || host.iter >= maxiter) break;
+      // the recursion says converged: check with the true residual
+      if (int rc = replace(x, r, z, B, launched & 1, 1)) return rc;
+      hipLaunchKernelGGL(k_cg_recheck, dim3(1), dim3(kVecThreads), 0, st, vb, B);
+      hipLaunchKernelGGL(k_cg_restart, dim3(vb), dim3(kVecThreads), 0, st, n, z, p, B);
+      IGA_CHECK_LAUNCH();
+      verified = true;
+      continue;
+    }
+    verified = false;
     for (int k = 0; k < batch; ++k, ++launched) {
       const int par = launched & 1;
       const int pb = apply(p, q, B);
       if (pb < 0) return pb;
       hipLaunchKernelGGL(k_cg_update, dim3(vb), dim3(kVecThreads), 0, st, n, pb, vb, par, dinv, p,
                          q, x, r, z, B);
+      if (replace.every > 0 && (launched + 1) % replace.every == 0)
+        if (int rc = replace(x, r, z, B, 1 - par, 0)) return rc;
       hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, par, z, p, B);
       IGA_CHECK_LAUNCH();
     }
@@ -499,6 +571,59 @@ extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* col
                    });
                    return rc ? rc : rb;
                  });
+}
+
+// CG on an FP32 matrix with FP64 vectors and recurrences, and reliable
+// updates: every `replace_every` iterations, and when the recursion reports
+// convergence, the residual is recomputed with the FP64 matrix. The stopping
+// test therefore holds for the FP64 operator.
+struct CsrReplace {
+  int every;
+  int32_t n, lpr, vb;
+  const int32_t* indptr;
+  const int32_t* cols;
+  const double* vals64;
+  const double* b;
+  const double* dinv;
+  hipStream_t st;
+  int operator()(const double* x, double* r, double* z, CGBufs B, int slot, int force) const {
+    return by_lpr(lpr, [&](auto L) {
+      constexpr int LPR = decltype(L)::value;
+      hipLaunchKernelGGL((k_cg_replace<LPR>), dim3(vb), dim3(kVecThreads), 0, st, n, indptr, cols,
+                         vals64, b, dinv, x, r, z, B, slot, force);
+      IGA_CHECK_LAUNCH();
+      return (int)IGA_OK;
+    });
+  }
+};
+
+extern "C" int iga_cg_solve_mixed(int32_t n, const int32_t* indptr, const int32_t* cols,
+                                  const float* vals32, const double* vals64, const double* dinv,
+                                  const double* b, double* x, double tol, int32_t maxiter,
+                                  int32_t replace_every, double* work, int32_t* out_iters,
+                                  double* out_rr, double* out_bb, void* stream) {
+  if (!indptr || !cols || !vals32 || !vals64 || !dinv || !b || !x || !work) return IGA_ERR_ARG;
+  if (replace_every < 1) return IGA_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  int nnz = 0;
+  if (n > 0)
+    if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  const int lpr = pick_lpr(n, nnz);
+  const int rb = row_blocks(n, lpr);
+  CsrReplace rep{replace_every, n, lpr, vec_blocks(n), indptr, cols, vals64, b, dinv, st};
+  return cg_loop(
+      n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
+      [&](const double* p, double* q, CGBufs B) -> int {
+        int rc = by_lpr(lpr, [&](auto L) {
+          constexpr int LPR = decltype(L)::value;
+          hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kVecThreads), 0, st, n,
+                             indptr, cols, vals32, p, q, B);
+          IGA_CHECK_LAUNCH();
+          return (int)IGA_OK;
+        });
+        return rc ? rc : rb;
+      },
+      rep);
 }
 
 extern "C" int iga_cg_solve_mf(const iga_patch_t* patch, int32_t precision, int32_t rational,

@@ -196,10 +196,120 @@ def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagona
     diagonal. Returns ``(vals, rhs, diag)`` (tensors or None)."""
     job = prepare_assembly(ctx, params, values=values, load=load, diagonal=diagonal,
                            precision=precision)
-    job.launch(ctx)
-    if load:
-        neumann_load(ctx, params, job.rhs)
+    run_assembly(ctx, params, job, load=load)
     return job.vals, job.rhs, job.diag
+
+
+_SIDE_STREAMS = {}
+
+
+def side_stream(dev):
+    """A second stream per device (default priority) for work that overlaps
+    the element pass."""
+    import torch
+
+    key = str(dev)
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return _SIDE_STREAMS[key]
+
+
+def run_assembly(ctx, params, job, *, load=True, passes=(0,), on_pass=None):
+    """Launch a prepared assembly on the current stream. The Neumann work
+    (boundary basis, traction, load scatter) runs on a side stream alongside
+    the element pass -- both add into the load with atomics -- and the
+    current stream waits for it before returning. ``passes`` splits the
+    iga_assemble call (``(1, 2)``: element pass, then gather) and
+    ``on_pass(p)`` runs after each launch (e.g. to record a timing event)."""
+    import torch
+
+    neumann = load and job.rhs is not None and bool(ctx.problem.neumann)
+    done = None
+    for k, p in enumerate(passes):
+        if k == 0 and neumann:
+            # the side stream starts from the same point as the element pass
+            side = side_stream(job.rhs.device)
+            side.wait_stream(torch.cuda.current_stream())
+        job.launch(ctx, passes=p)
+        if on_pass is not None:
+            on_pass(p)
+        if k == 0 and neumann:
+            # issued after the element pass, so the host prepares the
+            # boundary work while the GPU runs the element pass
+            with torch.cuda.stream(side):
+                neumann_load(ctx, params, job.rhs)
+            done = torch.cuda.Event()
+            done.record(side)
+    if done is not None:
+        torch.cuda.current_stream().wait_event(done)
+
+
+class CapturedAssembly:
+    """A full assembly (load zeroed, element pass with the Neumann work
+    forked to a side stream, gather) captured in HIP graphs over persistent
+    buffers. ``replay()`` recomputes everything -- basis, element matrices,
+    CSR values, load -- with one graph launch per part and no host work;
+    ``vals``, ``rhs`` and ``diag`` hold the results. With ``split`` the
+    element pass and the gather are separate graphs (``replay_part(0)``,
+    ``replay_part(1)``), so events can be recorded between them."""
+
+    def __init__(self, ctx, params=None, *, values=True, load=True, diagonal=False,
+                 precision=_lib.IGA_FP64, split=False):
+        import torch
+
+        self.ctx = ctx
+        self.job = prepare_assembly(ctx, params, values=values, load=load, diagonal=diagonal,
+                                    precision=precision)
+        job = self.job
+        parts = [(1,), (2,)] if split else [(0,)]
+
+        def zero():
+            if job.rhs is not None:
+                job.rhs.zero_()
+            if job.diag is not None:
+                job.diag.zero_()
+
+        def part(k):
+            if k == 0:
+                zero()
+            if split and k == 0:
+                run_assembly(ctx, params, job, load=load, passes=(1,))
+            elif split:
+                job.launch(ctx, passes=2)
+            else:
+                run_assembly(ctx, params, job, load=load, passes=(0,))
+
+        # one eager run first: lazy set-up (point sets, uploads) stays out of
+        # the graphs
+        for k in range(len(parts)):
+            part(k)
+        torch.cuda.synchronize()
+        self.graphs = []
+        for k in range(len(parts)):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                part(k)
+            self.graphs.append(g)
+        torch.cuda.synchronize()
+
+    @property
+    def vals(self):
+        return self.job.vals
+
+    @property
+    def rhs(self):
+        return self.job.rhs
+
+    @property
+    def diag(self):
+        return self.job.diag
+
+    def replay_part(self, k):
+        self.graphs[k].replay()
+
+    def replay(self):
+        for g in self.graphs:
+            g.replay()
 
 
 def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True, diagonal=False,

@@ -1,8 +1,16 @@
-"""Mixed precision with FP64 iterative refinement.
+"""Mixed precision with FP64 accuracy recovered on the FP64 operator.
 
 The reference is FP64 only (``src/jaxiga/config.py:1-33``). This module adds
-the low-precision operators of this build with FP64 accuracy recovered by
-iterative refinement:
+the low-precision operators of this build with FP64 accuracy recovered in
+two ways.
+
+Reliable updates (:func:`csr_mixed_cg`, the default for CSR): one Jacobi-PCG
+run on the FP32 matrix with FP64 vectors and recurrences; every few
+iterations, and whenever the recursion reports convergence, the residual is
+replaced by the true ``b - K64 x``, keeping the search direction, so the
+Krylov space is never discarded and the stopping test holds for K64.
+
+Iterative refinement (:func:`csr_mixed_solve`, :func:`mf_mixed_solve`):
 
 * outer loop in FP64: residual ``r = b - K64 u`` from the FP64 operator (CSR
   or matrix-free), convergence judged on ``||r|| <= tol ||b||``;
@@ -85,6 +93,41 @@ def refine(residual_fn, inner_fn, b, tol=1e-10, inner_tol=1e-6, max_outer=12):
         k += 1
     torch.cuda.synchronize()
     return RefinementResult(u, k, inner, hist, time.perf_counter() - t0)
+
+
+@dataclasses.dataclass
+class MixedCGResult:
+    x: object
+    iterations: int
+    relative_residual: float  # true FP64 ||b - K64 x|| / ||b||
+    seconds: float
+
+
+def csr_mixed_cg(pattern, vals64, vals32, b, dinv, tol=1e-10, maxiter=None, replace_every=64,
+                 workspace=None):
+    """Jacobi-PCG on the FP32 CSR values with FP64 vectors and reliable
+    updates against the FP64 values (same pattern); stops on the true FP64
+    residual."""
+    import torch
+
+    L = lib()
+    n = pattern.shape[0]
+    if workspace is None:
+        workspace = torch.empty(int(L.iga_cg_workspace(n)), dtype=torch.float64, device=b.device)
+    x = torch.empty(n, dtype=torch.float64, device=b.device)
+    it = _lib.c_int(0)
+    rr = _lib.c_double(0.0)
+    bb = _lib.c_double(0.0)
+    maxiter = 10 * n if maxiter is None else int(maxiter)
+    t0 = time.perf_counter()
+    _lib.check(L.iga_cg_solve_mixed(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d),
+                                    _lib.ptr(vals32), _lib.ptr(vals64), _lib.ptr(dinv), _lib.ptr(b),
+                                    _lib.ptr(x), float(tol), maxiter, int(replace_every),
+                                    _lib.ptr(workspace), _lib.ctypes.byref(it),
+                                    _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()),
+               "iga_cg_solve_mixed")
+    rel = float(np.sqrt(rr.value / bb.value)) if bb.value > 0 else 0.0
+    return MixedCGResult(x, it.value, rel, time.perf_counter() - t0)
 
 
 def csr_mixed_solve(pattern, vals64, vals32, b, dinv, tol=1e-10, inner_tol=1e-6, maxiter=None):

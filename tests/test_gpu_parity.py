@@ -113,6 +113,28 @@ def test_assembled_system_matches_oracle(gpu, make):
     assert rel(diag.cpu(), K.diagonal()) < TOL_FP64
 
 
+@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("make", [SYSTEM_CASES[5], SYSTEM_CASES[4]])
+def test_captured_assembly_replays_match_eager(gpu, make, split):
+    # graph replays with eager work in between: CSR values and diagonal come
+    # from the deterministic gather (bitwise), the load has atomic sums
+    case = make()
+    print(f"[graph] {case.name} split={split}", flush=True)
+    ctx = M.build_context(case.problem, case.params)
+    vals, rhs, diag = M.assemble(ctx, case.params, diagonal=True)
+    cap = M.CapturedAssembly(ctx, case.params, diagonal=True, split=split)
+    for _ in range(3):
+        cap.vals.fill_(float("nan"))
+        cap.rhs.fill_(float("nan"))
+        cap.diag.fill_(float("nan"))
+        cap.replay()
+        M.assemble(ctx, case.params, diagonal=True)
+        torch.cuda.synchronize()
+        assert torch.equal(cap.vals, vals)
+        assert torch.equal(cap.diag, diag)
+        assert rel(cap.rhs.cpu(), rhs.cpu().numpy()) < 1e-13
+
+
 @pytest.mark.parametrize("make", SYSTEM_CASES[:3] + SYSTEM_CASES[5:6])
 def test_cg_matches_oracle(gpu, make):
     case = make()

@@ -44,6 +44,34 @@ def test_fp32_assembly_entries_and_refined_solution(gpu, make):
     assert np.abs(u - u_ref).max() / np.abs(u_ref).max() < 1e-8
 
 
+@pytest.mark.parametrize("make", [lambda: C.elasticity_case(3, 3, "strain"),
+                                  lambda: C.poisson_case("annulus", 3, 3)])
+def test_fp32_csr_cg_with_reliable_updates(gpu, make):
+    # one CG run on the FP32 values, residual replaced by the FP64 one: the
+    # stopping test holds for the FP64 operator, with about the FP64 CG's
+    # iteration count
+    case = make()
+    print(f"[mixed-cg] {case.name}", flush=True)
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    u_ref = OSV.jacobi_pcg(lambda v: K @ v, b, K.diagonal(), tol=1e-13)[0]
+    ctx = M.build_context(case.problem, case.params)
+    v64, rhs, _ = M.assemble(ctx, case.params)
+    v32, _, _ = M.assemble(ctx, case.params, load=False, precision=_lib.IGA_FP32)
+    dinv = LIN.inverse_diagonal(ctx.assembly, v64)
+    ref = LIN.cg_csr(ctx.assembly, v64, rhs, 1e-10, dinv=dinv)
+    for every in (16, 64):
+        res = PR.csr_mixed_cg(ctx.assembly, v64, v32, rhs, dinv, tol=1e-10, replace_every=every)
+        rn, bn = LIN.residual_norms(ctx.assembly, v64, res.x, rhs)
+        print(f"[mixed-cg] every {every}: {res.iterations} it (fp64 cg {ref.iterations}), "
+              f"true rel. res {rn / bn:.2e}", flush=True)
+        assert rn <= 1.0001e-10 * bn
+        assert abs(res.relative_residual - rn / bn) <= 1e-3 * rn / bn
+        assert res.iterations <= 2 * ref.iterations
+        u = res.x.cpu().numpy()
+        assert np.abs(u - u_ref).max() / np.abs(u_ref).max() < 1e-8
+
+
 def test_fp32_matrix_free_refined_solution(gpu):
     case = C.poisson_case("cube", 2, 3, source=lambda x, p: 1.0 + 0 * x[0])
     dm, pat, data, b = _oracle_system(case)
