@@ -197,8 +197,14 @@ __global__ void k_cg_init_fin(int nblk, CGBufs B, double tol2, int maxiter) {
   }
 }
 
+// q = A p, one row per group of LPR lanes, and the p.q partials (one per
+// block). Blocks of kSpmvThreads (16 waves) keep up to kRedBlocks x 16 x
+// 64 / LPR rows in flight, so the grid-stride loop makes few passes and each
+// pass's dependent loads (columns, then x) overlap across the whole grid.
+constexpr int kSpmvThreads = 1024;
+
 template <int LPR, typename TV>
-__global__ void __launch_bounds__(kVecThreads)
+__global__ void __launch_bounds__(kSpmvThreads)
     k_cg_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
               const TV* __restrict__ vals, const double* __restrict__ p, double* q, CGBufs B) {
   if (B.st->done) return;
@@ -415,6 +421,12 @@ static CGBufs cg_bufs(double* work, int32_t n) {
   return B;
 }
 
+// blocks of the SpMV (kSpmvThreads each, at most kRedBlocks partials)
+static int spmv_blocks(int32_t n, int lpr) {
+  const int64_t b = ((int64_t)n * lpr + kSpmvThreads - 1) / kSpmvThreads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
+}
+
 // Dot-product partials p.q for an operator applied outside the fused SpMV.
 __global__ void __launch_bounds__(kVecThreads)
     k_cg_dot(int n, const double* __restrict__ p, const double* __restrict__ q, CGBufs B) {
@@ -553,17 +565,17 @@ extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* col
   if (n > 0)
     if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
   const int lpr = pick_lpr(n, nnz);
-  const int rb = row_blocks(n, lpr);
+  const int rb = spmv_blocks(n, lpr);
   return cg_loop(n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
                  [&](const double* p, double* q, CGBufs B) -> int {
                    int rc = by_lpr(lpr, [&](auto L) {
                      constexpr int LPR = decltype(L)::value;
                      if (precision == IGA_FP32)
-                       hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kVecThreads), 0,
+                       hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kSpmvThreads), 0,
                                           st, n, indptr, cols, reinterpret_cast<const float*>(vals),
                                           p, q, B);
                      else
-                       hipLaunchKernelGGL((k_cg_spmv<LPR, double>), dim3(rb), dim3(kVecThreads), 0,
+                       hipLaunchKernelGGL((k_cg_spmv<LPR, double>), dim3(rb), dim3(kSpmvThreads), 0,
                                           st, n, indptr, cols,
                                           reinterpret_cast<const double*>(vals), p, q, B);
                      IGA_CHECK_LAUNCH();
@@ -609,14 +621,14 @@ extern "C" int iga_cg_solve_mixed(int32_t n, const int32_t* indptr, const int32_
   if (n > 0)
     if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
   const int lpr = pick_lpr(n, nnz);
-  const int rb = row_blocks(n, lpr);
+  const int rb = spmv_blocks(n, lpr);
   CsrReplace rep{replace_every, n, lpr, vec_blocks(n), indptr, cols, vals64, b, dinv, st};
   return cg_loop(
       n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
       [&](const double* p, double* q, CGBufs B) -> int {
         int rc = by_lpr(lpr, [&](auto L) {
           constexpr int LPR = decltype(L)::value;
-          hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kVecThreads), 0, st, n,
+          hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kSpmvThreads), 0, st, n,
                              indptr, cols, vals32, p, q, B);
           IGA_CHECK_LAUNCH();
           return (int)IGA_OK;
