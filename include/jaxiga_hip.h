@@ -392,14 +392,17 @@ int iga_energy_loss(int32_t batch, int64_t n, const float* z, const float* kz, f
                     double* per_sample, double* loss, void* stream);
 /* Flexible CG (one stored direction) on the FP64 operator with one FP32
  * V-cycle as the preconditioner; stops at ||r|| <= tol ||b|| or maxiter.
- * graph_iters > 0 captures that many iterations (rounded up to even) into a
- * hipGraph that is replayed. work: iga_fcg_workspace(n) doubles; work32: 2n floats. */
+ * graph_iters > 0 captures graphs of 1 and 2 iterations, and of 4 and 8 when
+ * graph_iters reaches them, and replays them. work: iga_fcg_workspace(n)
+ * doubles; work32: 2n floats. */
 int64_t iga_fcg_workspace(int32_t n);
 /* A reusable plan: the iterations are captured once over fixed buffers (b, x,
  * vals, work, and everything the iga_mg_t points to); each run re-initialises
- * the solver state on the device and replays the graph, so one plan serves
+ * the solver state on the device and replays the graphs, so one plan serves
  * every right-hand side written into b and every operator whose values are
- * rewritten in place. The plan owns only the graph; destroy it before the
+ * rewritten in place. A run replays as many iterations as the plan's last
+ * run needed, less one, before it first reads the convergence flag, then
+ * one per read. The plan owns only the graphs; destroy it before the
  * buffers are freed. */
 typedef struct iga_fcg_plan iga_fcg_plan_t;
 int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* indptr,

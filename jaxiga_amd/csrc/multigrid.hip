@@ -891,9 +891,18 @@ extern "C" int iga_energy_loss(int32_t batch, int64_t n, const float* z, const f
 extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 6 * (int64_t)n + 4 * kRedBlocks; }
 
 // A captured flexible-CG solve over fixed buffers: every run re-initialises
-// the state on the device and replays the graph until the device flag says
+// the state on the device and replays graphs until the device flag says
 // done, so one plan serves every right-hand side written into `b` and every
-// operator whose values are rewritten in place.
+// operator whose values are rewritten in place. Graphs of 8, 4, 2 and 1
+// iterations are kept; a run first replays as many iterations as the last
+// run needed, less one, without looking, then one iteration per look, so
+// few iterations run past convergence (their kernels return at once but
+// still cost a launch each).
+struct FcgGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct iga_fcg_plan {
   iga_mg_t mg;
   int n, nnz, lpr, vb, rb, eb, G;
@@ -905,8 +914,8 @@ struct iga_fcg_plan {
   FBufs F;
   double *r, *z, *P[2], *Q[2];
   float *r32, *z32;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
+  FcgGraph g8, g4, g2, g1[2];  // g8/g4/g2 start at parity 0, g1[p] at parity p
+  int last_iters = 0;
 
   int iteration(int par, hipStream_t st) {
     double* p = P[par];
@@ -931,23 +940,61 @@ struct iga_fcg_plan {
     return (int)IGA_OK;
   }
 
-  int capture(hipStream_t st) {
-    if (G <= 0) return IGA_OK;
+  int capture_one(FcgGraph& out, int iters, int par0, hipStream_t st) {
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
     int rc = 0;
-    for (int k = 0; k < G && !rc; ++k) rc = iteration(k & 1, st);
+    for (int k = 0; k < iters && !rc; ++k) rc = iteration((par0 + k) & 1, st);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc || e != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       return rc ? rc : IGA_ERR_HIP;
     }
-    graph = g;
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      graph = nullptr;
-      exec = nullptr;
+    out.graph = g;
+    if (hipGraphInstantiate(&out.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      out.graph = nullptr;
+      out.exec = nullptr;
       return IGA_ERR_HIP;
+    }
+    return IGA_OK;
+  }
+
+  int capture(hipStream_t st) {
+    if (G <= 0) return IGA_OK;
+    int rc = capture_one(g1[0], 1, 0, st);
+    if (!rc) rc = capture_one(g1[1], 1, 1, st);
+    if (!rc) rc = capture_one(g2, 2, 0, st);
+    if (!rc && G >= 4) rc = capture_one(g4, 4, 0, st);
+    if (!rc && G >= 8) rc = capture_one(g8, 8, 0, st);
+    return rc;
+  }
+
+  // k more iterations from the `launched`-th, as few graph launches as the
+  // parity allows
+  int replay(int k, int& launched, hipStream_t st) {
+    while (k > 0) {
+      const FcgGraph* g;
+      int m;
+      if (launched & 1) {
+        g = &g1[1];
+        m = 1;
+      } else if (k >= 8 && g8.exec) {
+        g = &g8;
+        m = 8;
+      } else if (k >= 4 && g4.exec) {
+        g = &g4;
+        m = 4;
+      } else if (k >= 2) {
+        g = &g2;
+        m = 2;
+      } else {
+        g = &g1[0];
+        m = 1;
+      }
+      if (hipGraphLaunch(g->exec, st) != hipSuccess) return IGA_ERR_HIP;
+      k -= m;
+      launched += m;
     }
     return IGA_OK;
   }
@@ -961,6 +1008,10 @@ struct iga_fcg_plan {
     IGA_CHECK_LAUNCH();
     FState host{};
     int launched = 0, status = IGA_OK;
+    const bool graphs = g1[0].exec != nullptr;
+    // iterations before the first look: the last run's count less one (four
+    // on the first run), then one per look
+    int next = last_iters > 1 ? last_iters - 1 : 4;
     for (;;) {
       if (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess) {
@@ -968,17 +1019,16 @@ struct iga_fcg_plan {
         break;
       }
       if (host.done) break;
-      if (exec) {
-        if (hipGraphLaunch(exec, st) != hipSuccess) {
-          status = IGA_ERR_HIP;
-          break;
-        }
-        launched += G;
+      if (graphs) {
+        status = replay(next, launched, st);
+        if (status) break;
+        next = last_iters > 1 ? 1 : 2;
       } else {
         for (int k = 0; k < 2 && !status; ++k, ++launched) status = iteration(launched & 1, st);
         if (status) break;
       }
     }
+    if (status == IGA_OK) last_iters = host.iter;
     if (out_iters) *out_iters = host.iter;
     if (out_rr) *out_rr = host.rr;
     if (out_bb) *out_bb = host.bb;
@@ -986,8 +1036,10 @@ struct iga_fcg_plan {
   }
 
   ~iga_fcg_plan() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (FcgGraph* g : {&g8, &g4, &g2, &g1[0], &g1[1]}) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+    }
   }
 };
 
@@ -1028,9 +1080,7 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   P->vb = vec_blocks(n);
   P->rb = row_blocks(n, P->lpr);
   P->eb = blocks_for(n);
-  P->G = graph_iters > 0 ? (graph_iters + 1) & ~1 : 0;
-  P->graph = nullptr;
-  P->exec = nullptr;
+  P->G = graph_iters > 0 ? graph_iters : 0;
   rc = P->capture((hipStream_t)stream);
   if (rc) {
     delete P;

@@ -25,7 +25,7 @@ from jaxiga_amd import linear as LIN
 from jaxiga_amd.evaluation import evaluate
 from jaxiga_amd.forms import PointField, Problem
 from jaxiga_amd.pointset import boundary_gauss, gauss, n_points, tensor_rule
-from jaxiga_amd.runtime import device, lib, stream
+from jaxiga_amd.runtime import device, lib, side_stream, stream
 from jaxiga_amd.space import make_dofmap
 
 CG_CROSSOVER = 3000
@@ -200,18 +200,6 @@ def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagona
     return job.vals, job.rhs, job.diag
 
 
-_SIDE_STREAMS = {}
-
-
-def side_stream(dev):
-    """A second stream per device (default priority) for work that overlaps
-    the element pass."""
-    import torch
-
-    key = str(dev)
-    if key not in _SIDE_STREAMS:
-        _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
-    return _SIDE_STREAMS[key]
 
 
 def run_assembly(ctx, params, job, *, load=True, passes=(0,), on_pass=None):
@@ -228,7 +216,7 @@ def run_assembly(ctx, params, job, *, load=True, passes=(0,), on_pass=None):
     for k, p in enumerate(passes):
         if k == 0 and neumann:
             # the side stream starts from the same point as the element pass
-            side = side_stream(job.rhs.device)
+            side = side_stream(job.rhs.device, "neumann")
             side.wait_stream(torch.cuda.current_stream())
         job.launch(ctx, passes=p)
         if on_pass is not None:

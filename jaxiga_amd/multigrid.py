@@ -20,13 +20,15 @@ dense inverse.
 from __future__ import annotations
 
 import dataclasses
+import hashlib
+import weakref
 
 import numpy as np
 import scipy.sparse as sp
 
 from jaxiga_amd import _lib
 from jaxiga_amd.geometry import change_of_basis
-from jaxiga_amd.runtime import lib, stream
+from jaxiga_amd.runtime import lib, side_stream, stream
 
 
 def coarsen_knots(knots, p):
@@ -171,9 +173,22 @@ class Hierarchy:
         return len(self.levels)
 
 
+_FREE_KEYS = weakref.WeakKeyDictionary()
+
+
+def _free_key(dofmap):
+    """Digest of a dof map's free set, computed once per dof map: contexts of
+    one space with the same free set share one hierarchy."""
+    k = _FREE_KEYS.get(dofmap)
+    if k is None:
+        k = hashlib.sha1(np.ascontiguousarray(dofmap.free).tobytes()).digest()
+        _FREE_KEYS[dofmap] = k
+    return k
+
+
 def hierarchy(ctx, max_coarse=64):
     """Cached :class:`Hierarchy` of a method context's space and free set."""
-    key = ("mg", ctx.dofmap.free.tobytes(), max_coarse)
+    key = ("mg", _free_key(ctx.dofmap), max_coarse)
     cache = ctx.space._patterns
     if key not in cache:
         cache[key] = Hierarchy(ctx, max_coarse)
@@ -354,7 +369,7 @@ class MGOperator:
         import torch
 
         it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
-        side = torch.cuda.Stream() if graph_iters > 0 else torch.cuda.current_stream()
+        side = side_stream(self.b.device, "fcg") if graph_iters > 0 else torch.cuda.current_stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):  # graph capture needs a non-default stream
             self.b.copy_(b)

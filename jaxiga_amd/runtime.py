@@ -20,3 +20,19 @@ def lib():
 
 def stream():
     return _lib.stream_handle()
+
+
+_SIDE_STREAMS = {}
+
+
+def side_stream(dev=None, name="side"):
+    """A persistent second stream per device and purpose (default priority):
+    for work that overlaps the current stream, and for graph capture, which
+    needs a stream other than the legacy default one."""
+    import torch
+
+    dev = torch.device(dev) if dev is not None else device()
+    key = (str(dev), name)
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return _SIDE_STREAMS[key]
