@@ -127,55 +127,103 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// The same convolution with FP16 operands on v_mfma_f32_16x16x16f16 (FP32
-// accumulation, bias and ReLU): lane l feeds A[l%16][4(l/16)..+3] and
-// B[4(l/16)..+3][l%16] per 16-deep step, four output-channel tiles share the
-// pixel operand. Inputs and weights are rounded to FP16 when staged in LDS.
+// FP16 operands of v_mfma_f32_16x16x16f16 (FP32 accumulation): lane l feeds
+// A[l%16][4(l/16)..+3] and B[4(l/16)..+3][l%16] per 16-deep step.
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
-constexpr int kConvChH = 16;              // input channels per weight chunk
-constexpr int kConvKcH = kConvChH * 9;    // 144 kk per chunk, a multiple of 16
-constexpr int kConvWsH = kConvKcH + 4;    // padded LDS row (8-byte aligned)
+// FP16-operand 3x3 stride-1 convolution, channel-last in LDS. K runs tap by
+// tap over 16-channel groups (k = tap * Cp + ci, Cp = Cin rounded up to 16),
+// so the four k values of a lane's B operand are four consecutive channels
+// of one input point: one 8-byte LDS read. The weights are packed once per
+// call as FP16 w16[co][tap][ci] (zero padded) and read as A operands
+// straight from L2; only the 16-channel output tiles the layer has are
+// computed. Block: one image row, kConvSeg pixels (16 per wave).
+__global__ void k_w16_pack(int Cin, int Cp, int Cout, int Cop, const float* __restrict__ w,
+                           _Float16* __restrict__ w16) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Cop * 9 * Cp) return;
+  const int ci = t % Cp, tap = (t / Cp) % 9, co = t / (9 * Cp);
+  w16[t] = (_Float16)((co < Cout && ci < Cin) ? w[((size_t)co * Cin + ci) * 9 + tap] : 0.f);
+}
 
+// LDS of one k_conv3_f16 block: the channel-last input tile and the packed
+// weights, rows padded by 4 halves (8-byte operand reads conflict-free).
+static size_t conv3_lds_bytes(int Cp, int Cop) {
+  return sizeof(_Float16) * ((size_t)3 * (kConvSeg + 2) * (Cp + 4) + (size_t)Cop * (9 * Cp + 4));
+}
+
+template <int COT>
 __global__ void __launch_bounds__(256)
-    k_conv_mfma_f16(int Cin, int H, int W, int Cout, const float* __restrict__ in,
-                    const float* __restrict__ w, const float* __restrict__ bias, int relu,
-                    float* __restrict__ out) {
-  __shared__ _Float16 tile[kConvMaxC][3][kConvSeg + 2];
-  __shared__ __attribute__((aligned(8))) _Float16 wsm[kConvMaxC][kConvWsH];
+    k_conv3_f16(int Cin, int Cp, int H, int W, int Cout, const float* __restrict__ in,
+                const _Float16* __restrict__ w16, const float* __restrict__ bias, int relu,
+                float* __restrict__ out) {
+  extern __shared__ __attribute__((aligned(16))) _Float16 csm[];
+  const int cs = Cp + 4;          // channel stride of the tile
+  const int ws = 9 * Cp + 4;      // row stride of the weights
+  _Float16* tile = csm;
+  _Float16* wsm = csm + 3 * (kConvSeg + 2) * cs;
   const int x0 = blockIdx.x * kConvSeg, y = blockIdx.y, n = blockIdx.z;
-  const float* inb = in + (size_t)n * Cin * H * W;
-  for (int t = threadIdx.x; t < Cin * 3 * (kConvSeg + 2); t += blockDim.x) {
-    const int xx = t % (kConvSeg + 2), dy = (t / (kConvSeg + 2)) % 3, ci = t / (3 * (kConvSeg + 2));
-    const int gy = y + dy - 1, gx = x0 + xx - 1;
-    tile[ci][dy][xx] = (_Float16)((gy >= 0 && gy < H && gx >= 0 && gx < W)
-                                      ? inb[((size_t)ci * H + gy) * W + gx] : 0.f);
+  // weights: rows of 9*Cp halves, copied 4 halves (8 bytes) at a time,
+  // eight reads in flight per thread
+  {
+    const int q = 9 * Cp / 4;  // 8-byte pieces per row
+    const int tw = COT * 16 * q;
+    for (int t0 = 0; t0 < tw; t0 += 8 * 256) {
+      h4v v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = min(t0 + u * 256 + (int)threadIdx.x, tw - 1);
+        const int co = t / q, k4 = t - co * q;
+        v[u] = *reinterpret_cast<const h4v*>(w16 + (size_t)co * 9 * Cp + 4 * k4);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = t0 + u * 256 + threadIdx.x;
+        if (t >= tw) continue;
+        const int co = t / q, k4 = t - co * q;
+        *reinterpret_cast<h4v*>(wsm + (size_t)co * ws + 4 * k4) = v[u];
+      }
+    }
   }
+  // input tile: reads from clamped (valid) addresses, unconditionally and
+  // eight at a time, zeroed afterwards outside the image (a read under a
+  // condition would be waited for before the next one)
+  const float* inb = in + (size_t)n * Cin * H * W;
+  const int tn = Cp * 3 * (kConvSeg + 2);
+  for (int t0 = 0; t0 < tn; t0 += 8 * 256) {
+    float v[8];
+    int dst[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * 256 + threadIdx.x;
+      const int xx = t % (kConvSeg + 2), dy = (t / (kConvSeg + 2)) % 3, ci = t / (3 * (kConvSeg + 2));
+      const int gy = y + dy - 1, gx = x0 + xx - 1;
+      const bool ok = t < tn && ci < Cin && gy >= 0 && gy < H && gx >= 0 && gx < W;
+      const int cy = min(max(gy, 0), H - 1), cx = min(max(gx, 0), W - 1), cc = min(ci, Cin - 1);
+      v[u] = inb[((size_t)cc * H + cy) * W + cx];
+      v[u] = ok ? v[u] : 0.f;
+      dst[u] = t < tn ? (dy * (kConvSeg + 2) + xx) * cs + ci : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (dst[u] >= 0) tile[dst[u]] = (_Float16)v[u];
+  }
+  __syncthreads();
   const int wave = threadIdx.x / kWave, l = threadIdx.x % kWave;
   const int px = wave * 16 + l % 16;
   const int kq = l / 16;
-  const int K = Cin * 9;
-  f4v acc[4];
+  f4v acc[COT];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
-  for (int ci0 = 0; ci0 < Cin; ci0 += kConvChH) {
-    __syncthreads();
-    for (int t = threadIdx.x; t < kConvMaxC * kConvKcH; t += blockDim.x) {
-      const int co = t / kConvKcH, kl = t - co * kConvKcH;
-      const int kk = ci0 * 9 + kl;
-      wsm[co][kl] = (_Float16)((co < Cout && kk < K) ? w[(size_t)co * K + kk] : 0.f);
-    }
-    __syncthreads();
-    for (int kl0 = 0; kl0 < kConvKcH; kl0 += 16) {
-      h4v bv;
+  for (int t = 0; t < COT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+  const _Float16* wl = wsm + (size_t)(l % 16) * ws + 4 * kq;
+  for (int tap = 0; tap < 9; ++tap) {
+    const int ky = tap / 3, kx = tap - 3 * ky;
+    const _Float16* bp = tile + (ky * (kConvSeg + 2) + px + kx) * cs + 4 * kq;
+    const _Float16* ap = wl + tap * Cp;
+    for (int ci0 = 0; ci0 < Cp; ci0 += 16) {
+      const h4v bv = *reinterpret_cast<const h4v*>(bp + ci0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int kl = kl0 + 4 * kq + j;
-        const int ci = ci0 + kl / 9, k = kl % 9;
-        bv[j] = ci < Cin ? tile[ci][k / 3][px + k % 3] : (_Float16)0.f;
-      }
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const h4v av = *reinterpret_cast<const h4v*>(&wsm[t * 16 + l % 16][kl0 + 4 * kq]);
+      for (int t = 0; t < COT; ++t) {
+        const h4v av = *reinterpret_cast<const h4v*>(ap + (size_t)t * 16 * ws + ci0);
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc[t], 0, 0, 0);
       }
     }
@@ -183,7 +231,7 @@ __global__ void __launch_bounds__(256)
   const int x = x0 + px;
   if (x >= W) return;
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+  for (int t = 0; t < COT; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int co = t * 16 + 4 * kq + r;
@@ -394,8 +442,17 @@ __global__ void k_add_f(int64_t n, const float* __restrict__ a, float* __restric
 struct NetPlan {
   int64_t in_off;        // input activation offset (floats)
   int64_t off[16];       // layer output offsets
+  int64_t w16_off[16];   // packed FP16 weights of FP16 matrix-core layers, or -1
   int64_t total;
 };
+
+// FP16 matrix-core convolution: stride-1 forward layers of at most
+// kConvMaxC channels, when the descriptor asks for FP16 operands
+static bool f16_layer(const iga_net_t& net, int l, int Ci) {
+  return net.precision == 1 && net.kind[l] == 0 && net.stride[l] == 1 && Ci <= kConvMaxC &&
+         net.cout[l] <= kConvMaxC &&
+         conv3_lds_bytes((Ci + 15) / 16 * 16, (net.cout[l] + 15) / 16 * 16) <= 64 * 1024;
+}
 
 static int in_C(const iga_net_t& net, int src) {
   return src < 0 ? (net.c_in > 0 ? net.c_in : 1) : net.cout[src];
@@ -407,6 +464,9 @@ static int64_t act_size(const iga_net_t& net, int l, int batch) {
   return (int64_t)batch * net.cout[l] * net.H[l + 1] * net.W[l + 1];
 }
 
+static int in_H(const iga_net_t& net, int src) { return net.H[src + 1]; }
+static int in_W(const iga_net_t& net, int src) { return net.W[src + 1]; }
+
 static NetPlan plan(const iga_net_t& net, int batch) {
   NetPlan P{};
   int64_t o = 0;
@@ -416,12 +476,17 @@ static NetPlan plan(const iga_net_t& net, int batch) {
     P.off[l] = o;
     o += (act_size(net, l, batch) + 63) & ~(int64_t)63;
   }
+  for (int l = 0; l < net.n_layers; ++l) {
+    P.w16_off[l] = -1;
+    const int Ci = in_C(net, net.src[l]);
+    if (!f16_layer(net, l, Ci)) continue;
+    const int64_t halves = (int64_t)((net.cout[l] + 15) / 16 * 16) * 9 * ((Ci + 15) / 16 * 16);
+    P.w16_off[l] = o;
+    o += ((halves + 1) / 2 + 63) & ~(int64_t)63;
+  }
   P.total = o;
   return P;
 }
-
-static int in_H(const iga_net_t& net, int src) { return net.H[src + 1]; }
-static int in_W(const iga_net_t& net, int src) { return net.W[src + 1]; }
 
 static void launch_conv(int B, int Cin, int H, int W, int Cout, int Ho, int Wo, int s,
                         const float* in, const float* w, float* out, int acc, hipStream_t st) {
@@ -455,12 +520,34 @@ static int net_forward(const iga_net_t& net, int batch, const float* params, flo
       // stride 1 on the matrix cores (FP32, or FP16 operands when the
       // descriptor asks for it), bias and ReLU in the epilogue
       dim3 grid((Wo + kConvSeg - 1) / kConvSeg, Ho, batch);
-      if (net.precision == 1)
-        hipLaunchKernelGGL(k_conv_mfma_f16, grid, dim3(256), 0, st, Ci, Hi, Wi, Co, A(src), w, bias,
-                           net.relu[l], A(l));
-      else
+      if (P.w16_off[l] >= 0) {
+        const int Cp = (Ci + 15) / 16 * 16, Cop = (Co + 15) / 16 * 16;
+        _Float16* w16 = reinterpret_cast<_Float16*>(act + P.w16_off[l]);
+        const int nw = Cop * 9 * Cp;
+        hipLaunchKernelGGL(k_w16_pack, dim3((nw + 255) / 256), dim3(256), 0, st, Ci, Cp, Co, Cop, w,
+                           w16);
+        const size_t lds = conv3_lds_bytes(Cp, Cop);
+        switch (Cop / 16) {
+          case 1:
+            hipLaunchKernelGGL((k_conv3_f16<1>), grid, dim3(256), lds, st, Ci, Cp, Hi, Wi, Co,
+                               A(src), w16, bias, net.relu[l], A(l));
+            break;
+          case 2:
+            hipLaunchKernelGGL((k_conv3_f16<2>), grid, dim3(256), lds, st, Ci, Cp, Hi, Wi, Co,
+                               A(src), w16, bias, net.relu[l], A(l));
+            break;
+          case 3:
+            hipLaunchKernelGGL((k_conv3_f16<3>), grid, dim3(256), lds, st, Ci, Cp, Hi, Wi, Co,
+                               A(src), w16, bias, net.relu[l], A(l));
+            break;
+          default:
+            hipLaunchKernelGGL((k_conv3_f16<4>), grid, dim3(256), lds, st, Ci, Cp, Hi, Wi, Co,
+                               A(src), w16, bias, net.relu[l], A(l));
+        }
+      } else {
         hipLaunchKernelGGL(k_conv_mfma, grid, dim3(256), 0, st, Ci, Hi, Wi, Co, A(src), w, bias,
                            net.relu[l], A(l));
+      }
       IGA_CHECK_LAUNCH();
       continue;
     }
