@@ -312,26 +312,37 @@ def assembly_run(ctx, params, prec_name, steps, warmup):
     # Neumann work overlaps them on a side stream; the timing events are
     # created before the timed region.
     job = M.prepare_assembly(ctx, params, precision=precision)
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
 
-    def step(e):
+    def step():
         job.rhs.zero_()
-        if e is not None:
-            e[0].record()
-        M.run_assembly(ctx, params, job, passes=(1, 2),
-                       on_pass=(lambda p: e[p].record()) if e is not None else None)
+        M.run_assembly(ctx, params, job, passes=(1, 2))
 
     for _ in range(warmup):
-        step(None)
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(steps):
-        step(events[k])
+    for _ in range(steps):
+        step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    # the element interval also holds the Neumann work that overlaps it
-    t_el = statistics.mean(e[0].elapsed_time(e[1]) for e in events)
-    t_ga = statistics.mean(e[1].elapsed_time(e[2]) for e in events)
+
+    # per-kernel durations: each pass launched `steps` times back to back on
+    # this stream (the GPU never waits for the host) between two HIP events
+    def kernel_ms(passes):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        job.launch(ctx, passes=passes)
+        e0.record()
+        for _ in range(steps):
+            job.launch(ctx, passes=passes)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    t_el = kernel_ms(1)
+    t_ga = kernel_ms(2)
+    # the operator and load of one full step, for the checks below
+    step()
+    torch.cuda.synchronize()
     vbytes = 4 if precision == _lib.IGA_FP32 else 8
     flops = elasticity_flops_per_element() * n_e
     el = {"bound": "mfma", "achieved": flops / (t_el * 1e-3) / 1e12, "peak": peak_tf, "unit": "TFLOP/s",

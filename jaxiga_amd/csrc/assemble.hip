@@ -1,23 +1,24 @@
 // Galerkin assembly for a single tensor-product NURBS patch on gfx950, in two
 // streaming passes with no atomics on the matrix values.
 //
-// Pass 1 (k_element) -- one workgroup per element. The element's 1-D
-// Bezier-extracted tables (N1 = C_d B, dN1 = C_d dB * 2/h) are staged in LDS;
-// the rational basis, the FP64 Jacobian/determinant and the physical
-// gradients come from one lane per (Gauss point, local function) with
-// group-of-lanes reductions. The element matrix K_e = Bm^T (W D) Bm is then
-// contracted on the matrix cores (v_mfma_f64_16x16x4 in FP64,
-// v_mfma_f32_16x16x4 in FP32) -- (W D) Bm staged in LDS, the strain operand Bm
-// formed on the fly from the gradients -- and each lane streams its
-// accumulators to an element-matrix buffer. The load (source work) and the Dirichlet lifting
-// -K_fp g, and optionally the diagonal, are accumulated in the same pass.
+// Pass 1, element matrices. Planar patches with p <= 3 and at most four
+// Gauss points per direction (k_element_w): one wavefront per element, lane
+// l holding local function l % 16 at Gauss point 4k + l / 16 in step k --
+// the operand layout of the 16x16x4 MFMA -- so the physical gradients feed
+// v_mfma_f64_16x16x4 / v_mfma_f32_16x16x4 from registers; the FP64 rational
+// geometry is sum-factorised once per Gauss point. Other shapes (k_element):
+// one workgroup per element with the basis in LDS. Both write K_e in the
+// gradient-product form K_e = sum_cd E^cd (x) G_c^T W G_d, and accumulate the
+// load (source work) and the Dirichlet lifting -K_fp g.
 //
-// Pass 2 (k_gather) -- one lane per CSR nonzero of the free-free block. The
-// elements that share the row's and the column's basis functions form a
-// small tensor box; the lane sums their element-matrix entries in a fixed
-// order and stores the value once. This is the reference's scatter-add into
-// unique CSR slots, realised as a deterministic gather: the assembled values
-// are bitwise reproducible run to run.
+// Pass 2, CSR values (k_gather_points): one wavefront per control point sums
+// the element-matrix rows of its supporting elements into a stencil box in
+// LDS in a fixed order and writes the box out in CSR order, with the slots
+// from a ballot over the present columns. This is the reference's scatter-add
+// into unique CSR slots, realised as a deterministic gather: the assembled
+// values are bitwise reproducible run to run. k_gather_rows (one wavefront
+// per row) and k_gather (one lane per nonzero) serve vector Poisson and
+// callers without row pointers.
 #include "common.hpp"
 
 namespace iga {
