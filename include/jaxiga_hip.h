@@ -330,6 +330,11 @@ typedef struct iga_mg {
   iga_mg_level_t level[16];
   const float* coarse_inv;
   float* work;
+  /* optional: the V-cycle below tail_level (1 <= tail_level < n_levels - 1,
+   * damped-Jacobi levels only) as one dense n x n FP32 matrix (row-major,
+   * from iga_mg_tail); null = none */
+  int32_t tail_level;
+  const float* tail;
 } iga_mg_t;
 
 /* Numeric set-up of a hierarchy for a new level-0 operator: Galerkin levels
@@ -363,6 +368,14 @@ int iga_mg_load(const iga_mg_setup_t* setup, const double* vals0, void* stream);
 
 int64_t iga_mg_work_floats(const iga_mg_t* mg);
 /* x = V(b) on level `level` (FP32 vectors of that level's size) */
+/* The V-cycle started at `level` (damped-Jacobi levels down to the dense
+ * coarsest solve; mg->tail ignored) is linear: out (n x n, row-major,
+ * n = mg->level[level].n) = its matrix, formed by running the same FP32
+ * kernels on the n unit vectors at once. Setting mg->tail = out and
+ * mg->tail_level = level then replaces the levels below `level` by one dense
+ * product per cycle. work: iga_mg_tail_floats(mg, level) floats. */
+int64_t iga_mg_tail_floats(const iga_mg_t* mg, int32_t level);
+int iga_mg_tail(const iga_mg_t* mg, int32_t level, float* work, float* out, void* stream);
 int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, float* x, void* stream);
 /* C = A B on C's pattern (COO c_rows/c_cols, nnz_c entries): A in CSR, B as the
  * CSR of B^T, sorted columns; FP64. Two calls form a Galerkin product P^T K P. */

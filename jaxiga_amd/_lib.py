@@ -88,7 +88,8 @@ class IgaMgSetup(ctypes.Structure):
 
 class IgaMg(ctypes.Structure):
     _fields_ = [("n_levels", c_int), ("ks", c_int), ("level", IgaMgLevel * 16),
-                ("coarse_inv", c_void_p), ("work", c_void_p)]
+                ("coarse_inv", c_void_p), ("work", c_void_p), ("tail_level", c_int),
+                ("tail", c_void_p)]
 
 
 _SIGNATURES = {
@@ -142,6 +143,8 @@ _SIGNATURES = {
     "iga_mg_load": (c_int, [ctypes.POINTER(IgaMgSetup), c_void_p, c_void_p]),
     "iga_mg_work_floats": (c_i64, [ctypes.POINTER(IgaMg)]),
     "iga_mg_vcycle": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_mg_tail_floats": (c_i64, [ctypes.POINTER(IgaMg), c_int]),
+    "iga_mg_tail": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
     "iga_masked_spgemm": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_stencil_features": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

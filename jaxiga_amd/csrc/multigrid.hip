@@ -74,7 +74,9 @@ __global__ void __launch_bounds__(256)
                     const int32_t* __restrict__ k_cols, const double* __restrict__ k_vals,
                     const int32_t* __restrict__ pt_ptr, const int32_t* __restrict__ pt_cols,
                     const double* __restrict__ pt_vals, const int32_t* __restrict__ c_ptr,
-                    const int32_t* __restrict__ c_cols, double* __restrict__ c_vals) {
+                    const int32_t* __restrict__ c_cols, double* __restrict__ c_vals,
+                    float* __restrict__ c_vals32, double* __restrict__ c_dinv,
+                    float* __restrict__ c_dsq) {
   __shared__ double boxes[4][kGalBox * kGalBox];
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int I = blockIdx.x * 4 + wave;
@@ -119,6 +121,9 @@ __global__ void __launch_bounds__(256)
     }
     __builtin_amdgcn_wave_barrier();
   }
+  // the coarse row in FP64 and FP32, and its diagonal scalings (1 where the
+  // diagonal is zero or absent)
+  bool diag = false;
   for (int k = c_ptr[I] + lane; k < c_ptr[I + 1]; k += kWave) {
     const int J = c_cols[k];
     double acc = 0.0;
@@ -128,6 +133,17 @@ __global__ void __launch_bounds__(256)
       if (br >= 0 && br < kGalBox && bc >= 0 && bc < kGalBox) acc += pt_vals[q] * box[br * kGalBox + bc];
     }
     c_vals[k] = acc;
+    c_vals32[k] = (float)acc;
+    if (J == I) {
+      const double di = acc != 0.0 ? 1.0 / acc : 1.0;
+      c_dinv[I] = di;
+      c_dsq[I] = (float)sqrt(fabs(di));
+      diag = true;
+    }
+  }
+  if (__ballot(diag) == 0 && lane == 0) {
+    c_dinv[I] = 1.0;
+    c_dsq[I] = 1.f;
   }
 }
 
@@ -323,8 +339,11 @@ __global__ void __launch_bounds__(256)
     k_jacobi_pre(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
                  const float* __restrict__ vals, float omega, const float* __restrict__ dsq,
                  const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
-                 const int* __restrict__ skip) {
+                 const int* __restrict__ skip, int64_t vs) {
   if (skipped(skip)) return;
+  b += blockIdx.y * vs;  // vector blockIdx.y of a batch
+  x += blockIdx.y * vs;
+  r += blockIdx.y * vs;
   const int l = threadIdx.x % LPR;
   const int groups = gridDim.x * (blockDim.x / LPR);
   for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
@@ -349,8 +368,11 @@ __global__ void __launch_bounds__(256)
     k_jacobi_post(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
                   const float* __restrict__ vals, float omega, const float* __restrict__ dsq,
                   const float* __restrict__ b, const float* __restrict__ x,
-                  float* __restrict__ x_out, const int* __restrict__ skip) {
+                  float* __restrict__ x_out, const int* __restrict__ skip, int64_t vs) {
   if (skipped(skip)) return;
+  b += blockIdx.y * vs;
+  x += blockIdx.y * vs;
+  x_out += blockIdx.y * vs;
   const int l = threadIdx.x % LPR;
   const int groups = gridDim.x * (blockDim.x / LPR);
   for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
@@ -362,8 +384,10 @@ __global__ void __launch_bounds__(256)
 // x = M b for a dense n x n matrix (coarsest level), one wavefront per row
 __global__ void __launch_bounds__(256)
     k_dense_gemv(int n, const float* __restrict__ M, const float* __restrict__ b,
-                 float* __restrict__ x, const int* __restrict__ skip) {
+                 float* __restrict__ x, const int* __restrict__ skip, int64_t vs) {
   if (skipped(skip)) return;
+  b += blockIdx.y * vs;
+  x += blockIdx.y * vs;
   const int i = blockIdx.x * 4 + threadIdx.x / kWave, l = threadIdx.x % kWave;
   if (i >= n) return;
   const float* row = M + (size_t)i * n;
@@ -545,7 +569,9 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
   X[l0] = x0;
   B[l0] = b0;
   int rc;
-  for (int l = l0; l < Lc; ++l) {
+  // below the tail level the cycle is one dense product
+  const int t = (mg.tail && mg.tail_level >= l0 && mg.tail_level < Lc) ? mg.tail_level : Lc;
+  for (int l = l0; l < t; ++l) {
     const iga_mg_level_t& L = mg.level[l];
     if (L.stencil) {
       if ((rc = smooth(mg, l, B[l], X[l], false, 0, skip, st))) return rc;
@@ -555,7 +581,7 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
       rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
         constexpr int LPR = decltype(K)::value;
         hipLaunchKernelGGL(k_jacobi_pre<LPR>, dim3(rb), dim3(256), 0, st, L.n, L.indptr, L.cols,
-                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip);
+                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip, (int64_t)0);
       });
       if (rc) return rc;
     }
@@ -565,12 +591,12 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
       return rc;
   }
   {
-    const iga_mg_level_t& C = mg.level[Lc];
-    hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, mg.coarse_inv,
-                       B[Lc], X[Lc], skip);
+    const iga_mg_level_t& C = mg.level[t];
+    hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n,
+                       t < Lc ? mg.tail : mg.coarse_inv, B[t], X[t], skip, (int64_t)0);
     IGA_CHECK_LAUNCH();
   }
-  for (int l = Lc - 1; l >= l0; --l) {
+  for (int l = t - 1; l >= l0; --l) {
     const iga_mg_level_t& L = mg.level[l];
     if ((rc = csr_f(L.n, L.p_nnz, L.p_ptr, L.p_cols, L.p_vals, X[l + 1], nullptr, X[l], 2, 1, 0, 0,
                     skip, st)))
@@ -585,7 +611,7 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
       rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
         constexpr int LPR = decltype(K)::value;
         hipLaunchKernelGGL(k_jacobi_post<LPR>, dim3(rb), dim3(256), 0, st, L.n, L.indptr, L.cols,
-                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip);
+                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip, (int64_t)0);
       });
       if (rc) return rc;
       std::swap(X[l], Rr[l]);
@@ -601,6 +627,11 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
 static int check_mg(const iga_mg_t* mg) {
   if (!mg || mg->n_levels < 1 || mg->n_levels > 16 || !mg->work || !mg->coarse_inv)
     return IGA_ERR_ARG;
+  if (mg->tail) {
+    if (mg->tail_level < 0 || mg->tail_level >= mg->n_levels - 1) return IGA_ERR_ARG;
+    for (int l = mg->tail_level; l < mg->n_levels - 1; ++l)
+      if (mg->level[l].stencil) return IGA_ERR_UNSUPPORTED;
+  }
   for (int l = 0; l < mg->n_levels; ++l) {
     const iga_mg_level_t& L = mg->level[l];
     if (L.n <= 0 || (int64_t)L.h * L.w != L.n) return IGA_ERR_SHAPE;
@@ -773,18 +804,21 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
       hipLaunchKernelGGL(k_copy_d, dim3((int)std::min<int64_t>(blocks_for(V.nnz), 4096)), dim3(256), 0,
                          st, (int64_t)V.nnz, vals0, V.vals64);
   }
-  for (int l = 0; l < L; ++l) {
-    const iga_mg_setup_level_t& V = s->level[l];
-    if (l + 1 < L) {
-      const iga_mg_setup_level_t& C = s->level[l + 1];
-      hipLaunchKernelGGL(k_galerkin_rows, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, V.w,
-                         s->radius, V.indptr, V.cols, V.vals64, V.pt_ptr, V.pt_cols, V.pt_vals,
-                         C.indptr, C.cols, C.vals64);
-    }
+  {
+    // the finest level's FP32 copy and scalings; the Galerkin kernel writes
+    // those of every coarser level with its values
+    const iga_mg_setup_level_t& V = s->level[0];
     hipLaunchKernelGGL(k_to_f32, dim3((int)std::min<int64_t>(blocks_for(V.nnz), 4096)), dim3(256), 0,
                        st, (int64_t)V.nnz, V.vals64, V.vals32);
     hipLaunchKernelGGL(k_level_scalings, dim3(blocks_for(V.n)), dim3(256), 0, st, V.n, V.indptr,
                        V.cols, V.vals64, V.dinv, V.dsq);
+  }
+  for (int l = 0; l + 1 < L; ++l) {
+    const iga_mg_setup_level_t& V = s->level[l];
+    const iga_mg_setup_level_t& C = s->level[l + 1];
+    hipLaunchKernelGGL(k_galerkin_rows, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, V.w,
+                       s->radius, V.indptr, V.cols, V.vals64, V.pt_ptr, V.pt_cols, V.pt_vals,
+                       C.indptr, C.cols, C.vals64, C.vals32, C.dinv, C.dsq);
   }
   IGA_CHECK_LAUNCH();
   const iga_mg_setup_level_t& C = s->level[L - 1];
@@ -821,6 +855,106 @@ extern "C" int iga_dense_inverse(int32_t n, const int32_t* indptr, const int32_t
 extern "C" int64_t iga_mg_work_floats(const iga_mg_t* mg) {
   if (!mg || mg->n_levels < 1 || mg->n_levels > 16) return -1;
   return level_offset(*mg, mg->n_levels);
+}
+
+__global__ void k_identity_f(int n, float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (int64_t)n * n) out[t] = (t / n == t % n) ? 1.f : 0.f;
+}
+
+__global__ void k_transpose_f(int n, const float* __restrict__ a, float* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (int64_t)n * n) out[t] = a[(t % n) * n + t / n];
+}
+
+// Floats of the batched cycle from level t: b, x, r for every level, each
+// n_t vectors long, plus the identity input.
+static int64_t tail_floats(const iga_mg_t& mg, int t) {
+  const int64_t B = mg.level[t].n;
+  int64_t f = B * B;
+  for (int l = t; l < mg.n_levels; ++l) f += 3 * B * mg.level[l].n;
+  return f;
+}
+
+// The V-cycle from level t applied to the n_t unit vectors at once, with the
+// kernels of vcycle() over a batch (vector j in blockIdx.y): row j of the
+// result is V e_j; `out` receives the matrix of V (its transpose).
+static int tail_matrix(const iga_mg_t& mg, int t, float* work, float* out, hipStream_t st) {
+  const int Lc = mg.n_levels - 1;
+  const int Bn = mg.level[t].n;
+  float* I = work;
+  float* X[16];
+  float* Bv[16];
+  float* Rr[16];
+  float* p = work + (int64_t)Bn * Bn;
+  for (int l = t; l <= Lc; ++l) {
+    const int64_t sz = (int64_t)Bn * mg.level[l].n;
+    X[l] = p;
+    Bv[l] = p + sz;
+    Rr[l] = p + 2 * sz;
+    p += 3 * sz;
+  }
+  const int64_t nn = (int64_t)Bn * Bn;
+  hipLaunchKernelGGL(k_identity_f, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Bn, I);
+  IGA_CHECK_LAUNCH();
+  Bv[t] = I;
+  int rc;
+  for (int l = t; l < Lc; ++l) {
+    const iga_mg_level_t& L = mg.level[l];
+    rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
+      constexpr int LPR = decltype(K)::value;
+      hipLaunchKernelGGL(k_jacobi_pre<LPR>, dim3(rb, Bn), dim3(256), 0, st, L.n, L.indptr, L.cols,
+                         L.vals, L.omega, L.dsq, Bv[l], X[l], Rr[l], nullptr, (int64_t)L.n);
+    });
+    if (rc) return rc;
+    const iga_mg_level_t& C = mg.level[l + 1];
+    if ((rc = csr_f(C.n, L.pt_nnz, L.pt_ptr, L.pt_cols, L.pt_vals, Rr[l], nullptr, Bv[l + 1], 0, Bn,
+                    L.n, C.n, nullptr, st)))
+      return rc;
+  }
+  {
+    const iga_mg_level_t& C = mg.level[Lc];
+    hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4, Bn), dim3(256), 0, st, C.n,
+                       mg.coarse_inv, Bv[Lc], X[Lc], nullptr, (int64_t)C.n);
+    IGA_CHECK_LAUNCH();
+  }
+  for (int l = Lc - 1; l >= t; --l) {
+    const iga_mg_level_t& L = mg.level[l];
+    const iga_mg_level_t& C = mg.level[l + 1];
+    if ((rc = csr_f(L.n, L.p_nnz, L.p_ptr, L.p_cols, L.p_vals, X[l + 1], nullptr, X[l], 2, Bn, C.n,
+                    L.n, nullptr, st)))
+      return rc;
+    rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
+      constexpr int LPR = decltype(K)::value;
+      hipLaunchKernelGGL(k_jacobi_post<LPR>, dim3(rb, Bn), dim3(256), 0, st, L.n, L.indptr, L.cols,
+                         L.vals, L.omega, L.dsq, Bv[l], X[l], Rr[l], nullptr, (int64_t)L.n);
+    });
+    if (rc) return rc;
+    std::swap(X[l], Rr[l]);
+  }
+  hipLaunchKernelGGL(k_transpose_f, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Bn, X[t],
+                     out);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int64_t iga_mg_tail_floats(const iga_mg_t* mg, int32_t level) {
+  if (!mg || mg->n_levels < 2 || mg->n_levels > 16 || level < 0 || level >= mg->n_levels - 1)
+    return -1;
+  return tail_floats(*mg, level);
+}
+
+extern "C" int iga_mg_tail(const iga_mg_t* mg, int32_t level, float* work, float* out,
+                           void* stream) {
+  if (!mg || !work || !out || !mg->coarse_inv) return IGA_ERR_ARG;
+  if (mg->n_levels < 2 || mg->n_levels > 16 || level < 0 || level >= mg->n_levels - 1)
+    return IGA_ERR_ARG;
+  for (int l = level; l < mg->n_levels - 1; ++l) {
+    const iga_mg_level_t& L = mg->level[l];
+    if (L.stencil) return IGA_ERR_UNSUPPORTED;
+    if (!L.indptr || !L.cols || !L.vals || !L.dsq || !L.p_ptr || !L.pt_ptr) return IGA_ERR_ARG;
+  }
+  return tail_matrix(*mg, level, work, out, (hipStream_t)stream);
 }
 
 extern "C" int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, float* x,

@@ -175,6 +175,10 @@ class Hierarchy:
 
 _FREE_KEYS = weakref.WeakKeyDictionary()
 
+# levels at or below this size take the dense tail (a 1089-dof level: 4.7 MB
+# FP32, one product per cycle instead of about a dozen launches)
+TAIL_ROWS = 300
+
 
 def _free_key(dofmap):
     """Digest of a dof map's free set, computed once per dof map: contexts of
@@ -263,6 +267,16 @@ class MGOperator:
         self._struct = None
         self._setup = None
         self._plans = {}
+        # the cycle below the first level with at most TAIL_ROWS dofs (level
+        # >= 1, damped Jacobi) is formed as one dense matrix per operator
+        self.tail_level = next((l for l in range(1, hier.n_levels - 1)
+                                if hier.levels[l].n <= TAIL_ROWS), -1)
+        if self.tail_level > 0:
+            nt = hier.levels[self.tail_level].n
+            self.tail = torch.empty(nt * nt, **f32)
+            self._tail_work = None
+        else:
+            self.tail = None
 
     def _setup_struct(self):
         if getattr(self, "_setup", None) is None:
@@ -287,10 +301,28 @@ class MGOperator:
     def load(self, vals64, stencils=None):
         """Write a new level-0 operator (FP64 CSR values) and its smoothers:
         one library call forms every Galerkin level, the FP32 copies, the
-        diagonal scalings and the coarsest inverse."""
+        diagonal scalings and the coarsest inverse, a second the dense matrix
+        of the cycle below the tail level."""
         _lib.check(lib().iga_mg_load(self._setup_struct(), _lib.ptr(vals64), stream()), "iga_mg_load")
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
+        self._form_tail()
         return self
+
+    def _form_tail(self):
+        import torch
+
+        if self.tail is None or not self.tail_active():
+            return
+        m = self.struct()
+        if self._tail_work is None:
+            nw = int(lib().iga_mg_tail_floats(m, self.tail_level))
+            self._tail_work = torch.empty(nw, dtype=torch.float32, device=self.b.device)
+        _lib.check(lib().iga_mg_tail(m, self.tail_level, _lib.ptr(self._tail_work), _lib.ptr(self.tail),
+                                     stream()), "iga_mg_tail")
+
+    def tail_active(self):
+        """The dense tail applies while no level at or below it has a stencil."""
+        return self.tail is not None and all(s is None for s in self.stencils[self.tail_level:])
 
     def set_stencils(self, stencils):
         import torch
@@ -309,7 +341,10 @@ class MGOperator:
             new.append(buf)
         if [x is None for x in new] != [x is None for x in self.stencils]:
             self._struct = None
+        was = self.tail_active()
         self.stencils = new
+        if self.tail_active() and not was:
+            self._form_tail()
 
     def struct(self):
         import torch
@@ -333,6 +368,9 @@ class MGOperator:
                 ip, cc, _, v32 = lv.pt_dev
                 L.pt_nnz, L.pt_ptr, L.pt_cols, L.pt_vals = cc.shape[0], ip.data_ptr(), cc.data_ptr(), v32.data_ptr()
         m.coarse_inv = self.coarse_inv.data_ptr()
+        if self.tail_active():
+            m.tail_level = self.tail_level
+            m.tail = self.tail.data_ptr()
         if self._work is None:
             nw = int(lib().iga_mg_work_floats(m))
             self._work = torch.empty(max(nw, 1), dtype=torch.float32, device=self.b.device)
@@ -349,7 +387,7 @@ class MGOperator:
         return x
 
     def _plan(self, graph_iters):
-        key = (tuple(s is None for s in self.stencils), graph_iters)
+        key = (tuple(s is None for s in self.stencils), self.tail_active(), graph_iters)
         plan = self._plans.get(key)
         if plan is None:
             lv = self.hier.levels[0]
