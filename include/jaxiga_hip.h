@@ -324,6 +324,33 @@ typedef struct iga_mg_level {
   const float* pt_vals;
 } iga_mg_level_t;
 
+/* Grid form of a hierarchy (optional). The free dofs of every level form an
+ * h x w grid and every level operator couples grid points at most `radius`
+ * apart, so the operators are kept in stencil layout, op[o][i] = K[i, i +
+ * off(o)] (o over the (2 radius + 1)^2 offsets, row-major, zero off the
+ * grid), and each transfer P_l = P_l^(1) (x) P_l^(0) as two 1-D bands:
+ * c_lo[f], pw[f][k] = P[f][c_lo[f] + k] (k < 4) for the fine rows and
+ * f_lo[c], rw[c][k] = P[f_lo[c] + k][c] (k < 6) for the coarse columns.
+ * Direction 0 runs along a grid row (fastest index), direction 1 across.
+ * With it, iga_mg_vcycle from level 0 and the flexible-CG V-cycle run as
+ * tile kernels that fuse every transfer with the smoother beside it (radius
+ * 3 and ks = 7 or no level-0 stencil; damped Jacobi on levels 1 .. tail-1;
+ * a dense tail at tail_level >= 2 whose finer level has <= 8192 dofs). */
+typedef struct iga_mg_band {
+  const int32_t* c_lo;
+  const float* pw;
+  const int32_t* f_lo;
+  const float* rw;
+} iga_mg_band_t;
+
+typedef struct iga_mg_grid {
+  int32_t n_levels; /* levels 0 .. n_levels-1 carry op (filled by iga_mg_grid_ops) */
+  int32_t radius;
+  float* op[16];
+  iga_mg_band_t band[16][2]; /* [l][d]: level l -> l + 1, direction d */
+  double* op64; /* optional FP64 stencil layout of level 0 (the flexible-CG operator) */
+} iga_mg_grid_t;
+
 typedef struct iga_mg {
   int32_t n_levels; /* 1..16 */
   int32_t ks;       /* stencil width: 3, 5, 7 or 9 */
@@ -331,10 +358,12 @@ typedef struct iga_mg {
   const float* coarse_inv;
   float* work;
   /* optional: the V-cycle below tail_level (1 <= tail_level < n_levels - 1,
-   * damped-Jacobi levels only) as one dense n x n FP32 matrix (row-major,
-   * from iga_mg_tail); null = none */
+   * damped-Jacobi levels only, n <= 2048) as one dense n x n FP32 matrix
+   * (row-major, rows padded with zeros to iga_mg_dense_ld(n) floats, from
+   * iga_mg_tail); null = none */
   int32_t tail_level;
   const float* tail;
+  const iga_mg_grid_t* grid; /* optional grid form (tile kernels); null = CSR kernels */
 } iga_mg_t;
 
 /* Numeric set-up of a hierarchy for a new level-0 operator: Galerkin levels
@@ -367,13 +396,19 @@ typedef struct iga_mg_setup {
 int iga_mg_load(const iga_mg_setup_t* setup, const double* vals0, void* stream);
 
 int64_t iga_mg_work_floats(const iga_mg_t* mg);
+/* Stencil layout of the FP32 level operators 0 .. mg->grid->n_levels-1 into
+ * mg->grid->op (from each level's CSR: indptr, cols, vals), and of the FP64
+ * level-0 values vals64 (CSR of level 0) into mg->grid->op64 when both are set. */
+int iga_mg_grid_ops(const iga_mg_t* mg, const double* vals64, void* stream);
 /* x = V(b) on level `level` (FP32 vectors of that level's size) */
 /* The V-cycle started at `level` (damped-Jacobi levels down to the dense
- * coarsest solve; mg->tail ignored) is linear: out (n x n, row-major,
- * n = mg->level[level].n) = its matrix, formed by running the same FP32
- * kernels on the n unit vectors at once. Setting mg->tail = out and
- * mg->tail_level = level then replaces the levels below `level` by one dense
- * product per cycle. work: iga_mg_tail_floats(mg, level) floats. */
+ * coarsest solve; mg->tail ignored) is linear: out (n x ld, row-major,
+ * n = mg->level[level].n <= 2048, ld = iga_mg_dense_ld(n), zero beyond column
+ * n) = its matrix, formed by running the same FP32 kernels on the n unit
+ * vectors at once. Setting mg->tail = out and mg->tail_level = level then
+ * replaces the levels below `level` by one dense product per cycle.
+ * work: iga_mg_tail_floats(mg, level) floats. */
+int iga_mg_dense_ld(int32_t n);
 int64_t iga_mg_tail_floats(const iga_mg_t* mg, int32_t level);
 int iga_mg_tail(const iga_mg_t* mg, int32_t level, float* work, float* out, void* stream);
 int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, float* x, void* stream);
@@ -414,16 +449,18 @@ int64_t iga_fcg_workspace(int32_t n);
  * the solver state on the device and replays the graphs, so one plan serves
  * every right-hand side written into b and every operator whose values are
  * rewritten in place. A run replays as many iterations as the plan's last
- * run needed, less one, before it first reads the convergence flag, then
- * one per read. The plan owns only the graphs; destroy it before the
- * buffers are freed. */
+ * run needed before it first reads the convergence flag, then one per read,
+ * and never more than maxiter in all. The plan owns only the
+ * graphs; destroy it before the buffers are freed. */
 typedef struct iga_fcg_plan iga_fcg_plan_t;
 int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* indptr,
                     const int32_t* cols, const double* vals, const double* b, double* x,
                     double* work, float* work32, int32_t graph_iters, void* stream,
                     iga_fcg_plan_t** out);
+/* out_res (optional): ||b - A x|| recomputed in FP64 from the returned x
+ * (independent of the recursion), read back with the solver state */
 int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter, int32_t* out_iters,
-                   double* out_rr, double* out_bb, void* stream);
+                   double* out_rr, double* out_bb, double* out_res, void* stream);
 void iga_fcg_mg_destroy(iga_fcg_plan_t* plan);
 int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32_t* cols,
                const double* vals, const double* b, double* x, double tol, int32_t maxiter,

@@ -86,10 +86,19 @@ class IgaMgSetup(ctypes.Structure):
                 ("coarse_inv", c_void_p), ("inv_work", c_void_p)]
 
 
+class IgaMgBand(ctypes.Structure):
+    _fields_ = [("c_lo", c_void_p), ("pw", c_void_p), ("f_lo", c_void_p), ("rw", c_void_p)]
+
+
+class IgaMgGrid(ctypes.Structure):
+    _fields_ = [("n_levels", c_int), ("radius", c_int), ("op", c_void_p * 16),
+                ("band", (IgaMgBand * 2) * 16), ("op64", c_void_p)]
+
+
 class IgaMg(ctypes.Structure):
     _fields_ = [("n_levels", c_int), ("ks", c_int), ("level", IgaMgLevel * 16),
                 ("coarse_inv", c_void_p), ("work", c_void_p), ("tail_level", c_int),
-                ("tail", c_void_p)]
+                ("tail", c_void_p), ("grid", ctypes.POINTER(IgaMgGrid))]
 
 
 _SIGNATURES = {
@@ -143,6 +152,8 @@ _SIGNATURES = {
     "iga_mg_load": (c_int, [ctypes.POINTER(IgaMgSetup), c_void_p, c_void_p]),
     "iga_mg_work_floats": (c_i64, [ctypes.POINTER(IgaMg)]),
     "iga_mg_vcycle": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_mg_dense_ld": (c_int, [c_int]),
+    "iga_mg_grid_ops": (c_int, [ctypes.POINTER(IgaMg), c_void_p, c_void_p]),
     "iga_mg_tail_floats": (c_i64, [ctypes.POINTER(IgaMg), c_int]),
     "iga_mg_tail": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
     "iga_masked_spgemm": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -162,7 +173,7 @@ _SIGNATURES = {
     "iga_fcg_mg_plan": (c_int, [ctypes.POINTER(IgaMg), c_int, c_int, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                 ctypes.POINTER(c_void_p)]),
-    "iga_fcg_mg_run": (c_int, [c_void_p, c_double, c_int, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_fcg_mg_run": (c_int, [c_void_p, c_double, c_int, P_int, P_dbl, P_dbl, P_dbl, c_void_p]),
     "iga_fcg_mg_destroy": (None, [c_void_p]),
     "iga_fcg_mg": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_double, c_int, c_int, c_void_p, c_void_p, P_int, P_dbl, P_dbl,

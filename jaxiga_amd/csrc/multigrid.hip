@@ -245,6 +245,52 @@ __global__ void k_inverse_f32(int n, const double* __restrict__ A, float* __rest
 
 __device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip; }
 
+// Device state of a flexible-CG solve (scalars never leave the device).
+struct FState {
+  double rr, bb, tol2, pq_prev;
+  int iter, maxiter, done, pad;
+  double res2;  // ||b - A x||^2 recomputed from x after the solve
+};
+
+// Gate of the first kernel of a V-cycle. Inside flexible CG it is also the
+// convergence test of the previous iteration: every block reduces the r.r
+// partials of the last update and returns when ||r||^2 <= tol^2 ||b||^2;
+// block 0 records rr and the flag, and counts the iteration it lets through.
+// Every later kernel of the iteration reads that flag (`flag`).
+struct Gate {
+  const int* flag;   // return when *flag != 0 (null: never)
+  const double* rr;  // r.r partials (null: only `flag`)
+  int n_rr;
+  FState* st;
+};
+
+__device__ __forceinline__ bool gate_closed(const Gate& g) {
+  if (g.rr) {
+    const double rr = reduce_partials(g.rr, g.n_rr);
+    const bool done = !(rr > g.st->tol2 * g.st->bb);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+      g.st->rr = rr;
+      g.st->done = done ? 1 : 0;
+      if (!done) g.st->iter += 1;
+    }
+    return done;
+  }
+  return g.flag && *g.flag;
+}
+
+// Optional dot product fused into the kernel that writes a vector z:
+// parts[block] = sum over the block's entries of z_i y_i (FP64).
+struct DotOut {
+  const double* y;
+  double* parts;
+};
+
+__device__ __forceinline__ void store_partial_at(double* parts, int idx, double v) {
+  __shared__ double scratch[16];
+  v = block_sum(v, scratch);
+  if (threadIdx.x == 0) parts[idx] = v;
+}
+
 // out (=|+=) D^-1/2 T (D^-1/2 v), T = T_W or its transpose; batch over
 // blockIdx.z. A block computes a 16 x 16 tile of the grid: u = D^-1/2 v over
 // the tile and its halo is staged in LDS (zero outside the grid), so the tap
@@ -255,9 +301,9 @@ constexpr int kStTile = 16;
 template <int KS, bool TR>
 __global__ void __launch_bounds__(256)
     k_stencil(int h, int w, const float* __restrict__ W, const float* __restrict__ dsq,
-              const float* __restrict__ v, float* __restrict__ out, int accumulate,
-              const int* __restrict__ skip) {
-  if (skipped(skip)) return;
+              const float* __restrict__ v, float* __restrict__ out, int accumulate, Gate gate,
+              DotOut dot) {
+  if (gate_closed(gate)) return;
   constexpr int R = KS / 2;
   constexpr int T = kStTile + 2 * R;
   __shared__ float u[T][T];
@@ -271,8 +317,8 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   const int ty = threadIdx.x / kStTile, tx = threadIdx.x % kStTile;
   const int r = r0 + ty, c = c0 + tx;
-  if (r >= h || c >= w) return;
-  const int i = r * w + c;
+  const bool inside = r < h && c < w;
+  const int i = inside ? r * w + c : 0;
   float acc = 0.f;
 #pragma unroll
   for (int dy = -R; dy <= R; ++dy) {
@@ -290,8 +336,15 @@ __global__ void __launch_bounds__(256)
     }
   }
   float* ob = out + (size_t)blockIdx.z * n;
-  const float val = dsq[i] * acc;
-  ob[i] = accumulate ? ob[i] + val : val;
+  float z = 0.f;
+  if (inside) {
+    const float val = dsq[i] * acc;
+    z = accumulate ? ob[i] + val : val;
+    ob[i] = z;
+  }
+  if (dot.parts)  // one grid of tiles (blockIdx.z == 0): partial per tile
+    store_partial_at(dot.parts, blockIdx.y * gridDim.x + blockIdx.x,
+                     inside ? (double)z * dot.y[i] : 0.0);
 }
 
 // damped Jacobi (used where a level has no stencil): out (=|+=) omega D^-1 v
@@ -339,8 +392,8 @@ __global__ void __launch_bounds__(256)
     k_jacobi_pre(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
                  const float* __restrict__ vals, float omega, const float* __restrict__ dsq,
                  const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
-                 const int* __restrict__ skip, int64_t vs) {
-  if (skipped(skip)) return;
+                 Gate gate, int64_t vs) {
+  if (gate_closed(gate)) return;
   b += blockIdx.y * vs;  // vector blockIdx.y of a batch
   x += blockIdx.y * vs;
   r += blockIdx.y * vs;
@@ -384,8 +437,8 @@ __global__ void __launch_bounds__(256)
 // x = M b for a dense n x n matrix (coarsest level), one wavefront per row
 __global__ void __launch_bounds__(256)
     k_dense_gemv(int n, const float* __restrict__ M, const float* __restrict__ b,
-                 float* __restrict__ x, const int* __restrict__ skip, int64_t vs) {
-  if (skipped(skip)) return;
+                 float* __restrict__ x, Gate gate, int64_t vs) {
+  if (gate_closed(gate)) return;
   b += blockIdx.y * vs;
   x += blockIdx.y * vs;
   const int i = blockIdx.x * 4 + threadIdx.x / kWave, l = threadIdx.x % kWave;
@@ -398,11 +451,19 @@ __global__ void __launch_bounds__(256)
   if (l == 0) x[i] = s;
 }
 
-__global__ void k_copy_fs(int n, const float* __restrict__ a, float* __restrict__ b,
-                          const int* __restrict__ skip) {
+// b = a, with the partials of b . dot.y when requested (grid-stride, at most
+// kRedBlocks blocks)
+__global__ void __launch_bounds__(256)
+    k_copy_fs(int n, const float* __restrict__ a, float* __restrict__ b,
+              const int* __restrict__ skip, DotOut dot) {
   if (skipped(skip)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[i];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float v = a[i];
+    b[i] = v;
+    if (dot.parts) s += (double)v * dot.y[i];
+  }
+  if (dot.parts) store_partial_at(dot.parts, blockIdx.x, s);
 }
 
 __global__ void k_d2f(int n, const double* __restrict__ a, float* __restrict__ b,
@@ -510,11 +571,12 @@ static int csr_f(int n, int nnz, const int32_t* ptr, const int32_t* cols, const 
 }
 
 static int smooth(const iga_mg_t& mg, int l, const float* v, float* out, bool transpose,
-                  int accumulate, const int* skip, hipStream_t st) {
+                  int accumulate, const Gate& gate, const DotOut& dot, hipStream_t st) {
   const iga_mg_level_t& L = mg.level[l];
   if (!L.stencil) {
+    if (gate.rr || dot.parts) return IGA_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(k_jacobi_f, dim3(blocks_for(L.n)), dim3(256), 0, st, L.n, L.omega, L.dsq, v,
-                       out, accumulate, skip);
+                       out, accumulate, gate.flag);
     IGA_CHECK_LAUNCH();
     return IGA_OK;
   }
@@ -523,18 +585,81 @@ static int smooth(const iga_mg_t& mg, int l, const float* v, float* out, bool tr
     constexpr int KS = decltype(K)::value;
     if (transpose)
       hipLaunchKernelGGL((k_stencil<KS, true>), grid, dim3(256), 0, st, L.h, L.w, L.stencil, L.dsq,
-                         v, out, accumulate, skip);
+                         v, out, accumulate, gate, dot);
     else
       hipLaunchKernelGGL((k_stencil<KS, false>), grid, dim3(256), 0, st, L.h, L.w, L.stencil,
-                         L.dsq, v, out, accumulate, skip);
+                         L.dsq, v, out, accumulate, gate, dot);
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
   });
 }
 
+// tiles of the stencil kernel on level l (its partial count when it carries a dot)
+static int stencil_tiles(const iga_mg_level_t& L) {
+  return ((L.w + kStTile - 1) / kStTile) * ((L.h + kStTile - 1) / kStTile);
+}
+
+// Leading dimension of the dense tail matrix: rows padded with zeros to a
+// multiple of 256 floats, so a wavefront reads a row as float4s unconditionally.
+static inline int dense_ld(int n) { return (n + 255) / 256 * 256; }
+constexpr int kDenseMaxRows = 2048;
+
+// x = M b for the dense tail (n x n, leading dimension ld = dense_ld(n)): one
+// wavefront per row, every lane issues its NV float4 loads at once.
+template <int NV>
+__global__ void __launch_bounds__(256)
+    k_dense_gemv_ld(int n, int ld, const float* __restrict__ M, const float* __restrict__ b,
+                    float* __restrict__ x, Gate gate, int64_t vs) {
+  if (gate_closed(gate)) return;
+  b += blockIdx.y * vs;
+  x += blockIdx.y * vs;
+  const int i = blockIdx.x * 4 + threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (i >= n) return;
+  const float4* row = reinterpret_cast<const float4*>(M + (size_t)i * ld);
+  float4 m[NV];
+  float bv[NV][4];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) m[v] = row[l + kWave * v];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) bv[v][k] = b[min(4 * (l + kWave * v) + k, n - 1)];
+  float s = 0.f;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s += m[v].x * bv[v][0] + m[v].y * bv[v][1] + m[v].z * bv[v][2] + m[v].w * bv[v][3];
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (l == 0) x[i] = s;
+}
+
+static int dense_tail_gemv(int n, const float* M, const float* b, float* x, const Gate& gate,
+                           int batch, hipStream_t st) {
+  const int ld = dense_ld(n);
+  const int nv = ld / 256;
+  auto go = [&](auto V) {
+    constexpr int NV = decltype(V)::value;
+    hipLaunchKernelGGL(k_dense_gemv_ld<NV>, dim3((n + 3) / 4, batch), dim3(256), 0, st, n, ld, M, b,
+                       x, gate, (int64_t)n);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  };
+  switch (nv) {
+    case 1: return go(std::integral_constant<int, 1>{});
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 5: return go(std::integral_constant<int, 5>{});
+    case 6: return go(std::integral_constant<int, 6>{});
+    case 7: return go(std::integral_constant<int, 7>{});
+    case 8: return go(std::integral_constant<int, 8>{});
+    default: return IGA_ERR_UNSUPPORTED;
+  }
+}
+
+// four vectors per level (x, b, r and the post-smoothed x of the grid cycle)
 static int64_t level_offset(const iga_mg_t& mg, int l) {
   int64_t o = 0;
-  for (int k = 0; k < l; ++k) o += 3 * (int64_t)mg.level[k].n;
+  for (int k = 0; k < l; ++k) o += 4 * (int64_t)mg.level[k].n;
   return o;
 }
 
@@ -550,12 +675,760 @@ static int launch_rows(int n, int nnz, F f) {
   });
 }
 
+// partials of x . y (x FP32, y FP64), grid-stride over at most kRedBlocks blocks
+__global__ void __launch_bounds__(256)
+    k_dot_fd(int n, const float* __restrict__ x, const int* __restrict__ skip, DotOut dot) {
+  if (skipped(skip)) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += (double)x[i] * dot.y[i];
+  store_partial_at(dot.parts, blockIdx.x, s);
+}
+
+// ---------------------------------------------------------------------------
+// Grid V-cycle: tile kernels over the level grids (operators in stencil
+// layout, transfers as 1-D bands). Each kernel computes a 16 x 16 tile of its
+// output grid and recomputes what it needs of the previous step on the tile's
+// halo in LDS, so every transfer runs fused with the smoother beside it:
+//   k_g0_pre        level 0: x0 = S b0 (learned stencil or damped Jacobi), r0 = b0 - A0 x0
+//   k_g_restrict    level l+1: b = P^T r_l, x = w D^-1 b, r = b - A x
+//   k_g_tail        level t: x_t = M P^T r_{t-1} (dense tail)
+//   k_g_prolong     level l: x += P x_{l+1}, x += w D^-1 (b - A x)
+//   k_g0_post       level 0: x0 += P x1, r = b0 - A0 x0, z = x0 + S^T r (+ z . q)
+// Values off the grid are zero. A level's radius-3 operator reaches a 22 x 22
+// halo of the tile; the level-0 kernels reach 28 x 28.
+// ---------------------------------------------------------------------------
+
+constexpr int kGR = 3;                     // operator and smoother radius
+constexpr int kGS = 2 * kGR + 1;           // 7
+constexpr int kGN = kGS * kGS;             // 49 stencil offsets
+constexpr int kGT = 16;                    // tile
+constexpr int kGH1 = kGT + 2 * kGR;        // 22
+constexpr int kGH2 = kGT + 4 * kGR;        // 28
+constexpr int kGTh = 256;                  // threads per tile block
+constexpr int kKPF = 4, kKFC = 6;          // band widths (coarse per fine, fine per coarse)
+constexpr int kFineSpan = 56;              // fine rows/cols staged by k_g_restrict
+constexpr int kCoarseSpan = 24;            // coarse rows/cols staged by the prolongations
+constexpr int kTailFine = 8192, kTailCoarse = 2048;
+
+__host__ __device__ constexpr int passes(int n) { return (n + kGTh - 1) / kGTh; }
+
+struct GLevel {
+  int h, w;
+  const float* A;    // (49, n) stencil layout
+  const float* dsq;  // D^-1/2
+  float omega;
+};
+
+struct GBand {
+  const int32_t* c_lo;
+  const float* pw;  // [nf][4]
+  const int32_t* f_lo;
+  const float* rw;  // [nc][6]
+};
+
+__device__ __forceinline__ bool on_grid(int r, int c, int h, int w) {
+  return r >= 0 && r < h && c >= 0 && c < w;
+}
+
+// the 49 stencil values of grid point i, all loads issued before any use
+template <typename T>
+__device__ __forceinline__ void load_taps(const T* __restrict__ A, int n, int i, T (&a)[kGN]) {
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = A[(size_t)o * n + i];
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// sum_o a[o] v[ly + dy][lx + dx], v an LDS array of row length LD holding the
+// neighbourhood of the point at (ly, lx)
+template <int LD, typename T, typename V>
+__device__ __forceinline__ T tap_sum(const T (&a)[kGN], const V (*v)[LD], int ly, int lx) {
+  T acc = 0;
+#pragma unroll
+  for (int dy = -kGR; dy <= kGR; ++dy)
+#pragma unroll
+    for (int dx = -kGR; dx <= kGR; ++dx)
+      acc += a[(dy + kGR) * kGS + dx + kGR] * (T)v[ly + dy][lx + dx];
+  return acc;
+}
+
+// (P x_c) at fine point (r, c) from the coarse values staged in cs (origin
+// (cr0, cc0)); weights off the band are zero and indices are clamped to the
+// staging, so no access leaves it
+__device__ __forceinline__ float prolong_at(const GBand& br, const GBand& bc, int r, int c,
+                                            const float (*cs)[kCoarseSpan], int cr0, int cc0) {
+  const int ka = br.c_lo[r] - cr0, kb = bc.c_lo[c] - cc0;
+  float wr[kKPF], wc[kKPF];
+#pragma unroll
+  for (int a = 0; a < kKPF; ++a) {
+    wr[a] = br.pw[r * kKPF + a];
+    wc[a] = bc.pw[c * kKPF + a];
+  }
+  float v = 0.f;
+#pragma unroll
+  for (int a = 0; a < kKPF; ++a) {
+    const int ia = min(max(ka + a, 0), kCoarseSpan - 1);
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < kKPF; ++b) s += wc[b] * cs[ia][min(max(kb + b, 0), kCoarseSpan - 1)];
+    v += wr[a] * s;
+  }
+  return v;
+}
+
+// stage x_c over coarse rows [cr0, cr0 + 24) x cols [cc0, cc0 + 24), all
+// loads in flight together
+__device__ __forceinline__ void stage_coarse(float (*cs)[kCoarseSpan], const float* __restrict__ xc,
+                                             int hc, int wc, int cr0, int cc0) {
+  constexpr int N = kCoarseSpan * kCoarseSpan, NP = passes(N);
+  float v[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = cr0 + t / kCoarseSpan, cc = cc0 + t % kCoarseSpan;
+    const bool in = t < N && on_grid(rr, cc, hc, wc);
+    v[k] = xc[in ? rr * wc + cc : 0];
+    v[k] = in ? v[k] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    if (t < N) cs[t / kCoarseSpan][t % kCoarseSpan] = v[k];
+  }
+}
+
+// Every global load of a block (staging values, the stencil values of its
+// halo and tile points) is issued at kernel entry, before the gate's own
+// reduction, so the block waits on memory once.
+template <bool STW>
+__global__ void __launch_bounds__(kGTh)
+    k_g0_pre(GLevel g, const float* __restrict__ W, const float* __restrict__ b,
+             float* __restrict__ x, float* __restrict__ r, Gate gate) {
+  __shared__ float ub[kGH2][kGH2];  // D^-1/2 b on the tile +- 6
+  __shared__ float xs[kGH1][kGH1];  // x on the tile +- 3
+  constexpr int NS = kGH2 * kGH2, NSP = passes(NS);
+  constexpr int NH = kGH1 * kGH1, NHP = passes(NH);
+  const int n = g.h * g.w;
+  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
+  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+  const int rt = r0 + ty, ct = c0 + tx;
+  const bool in_t = on_grid(rt, ct, g.h, g.w);
+  const int it = in_t ? rt * g.w + ct : 0;
+  float sv[NSP], sd[NSP];
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = r0 - 2 * kGR + t / kGH2, cc = c0 - 2 * kGR + t % kGH2;
+    const int i = t < NS && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0;
+    sv[k] = b[i];
+    sd[k] = g.dsq[i];
+  }
+  float wh[NHP][kGN], dh[NHP];
+#pragma unroll
+  for (int k = 0; k < NHP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = r0 - kGR + min(t / kGH1, kGH1 - 1), cc = c0 - kGR + t % kGH1;
+    const int i = t < NH && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0;
+    dh[k] = g.dsq[i];
+    if (STW)
+#pragma unroll
+      for (int o = 0; o < kGN; ++o) wh[k][o] = W[(size_t)o * n + i];
+  }
+  float a[kGN];
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + it];
+  const float bt = b[it];
+  __builtin_amdgcn_sched_barrier(0);
+  if (gate_closed(gate)) return;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = r0 - 2 * kGR + t / kGH2, cc = c0 - 2 * kGR + t % kGH2;
+    if (t < NS) ub[t / kGH2][t % kGH2] = on_grid(rr, cc, g.h, g.w) ? sd[k] * sv[k] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NHP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const bool in = t < NH && on_grid(r0 - kGR + ly, c0 - kGR + lx, g.h, g.w);
+    float v;
+    if (STW)
+      v = dh[k] * tap_sum<kGH2>(wh[k], ub, ly + kGR, lx + kGR);
+    else
+      v = g.omega * dh[k] * ub[ly + kGR][lx + kGR];
+    if (t < NH) xs[ly][lx] = in ? v : 0.f;
+  }
+  __syncthreads();
+  const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
+  if (!in_t) return;
+  x[it] = xs[ty + kGR][tx + kGR];
+  r[it] = bt - ax;
+}
+
+// coarse tile of level l+1 from the fine residual r_f of level l
+__global__ void __launch_bounds__(kGTh)
+    k_g_restrict(int hf, int wf, GLevel gc, GBand br, GBand bc, const float* __restrict__ rf,
+                 float* __restrict__ bo, float* __restrict__ xo, float* __restrict__ ro,
+                 const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  __shared__ float fs[kFineSpan][kFineSpan];
+  __shared__ float bs[kGH1][kGH1];
+  __shared__ float xs[kGH1][kGH1];
+  const int nc = gc.h * gc.w;
+  const int R0 = blockIdx.y * kGT, C0 = blockIdx.x * kGT;
+  const int fr0 = br.f_lo[max(R0 - kGR, 0)], fc0 = bc.f_lo[max(C0 - kGR, 0)];
+  {
+    constexpr int N = kFineSpan * kFineSpan, NP = passes(N);
+    float v[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      const int rr = fr0 + t / kFineSpan, cc = fc0 + t % kFineSpan;
+      const bool in = t < N && on_grid(rr, cc, hf, wf);
+      v[k] = rf[in ? rr * wf + cc : 0];
+      v[k] = in ? v[k] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      if (t < N) fs[t / kFineSpan][t % kFineSpan] = v[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < passes(kGH1 * kGH1); ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const int R = R0 - kGR + ly, C = C0 - kGR + lx;
+    const bool in = t < kGH1 * kGH1 && on_grid(R, C, gc.h, gc.w);
+    const int Rc = in ? R : 0, Cc = in ? C : 0;
+    const int fa = br.f_lo[Rc] - fr0, fb = bc.f_lo[Cc] - fc0;
+    float wr[kKFC], wc[kKFC];
+#pragma unroll
+    for (int a = 0; a < kKFC; ++a) {
+      wr[a] = br.rw[Rc * kKFC + a];
+      wc[a] = bc.rw[Cc * kKFC + a];
+    }
+    const float d = gc.dsq[Rc * gc.w + Cc];
+    float bv = 0.f;
+#pragma unroll
+    for (int a = 0; a < kKFC; ++a) {
+      const int ia = min(max(fa + a, 0), kFineSpan - 1);
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < kKFC; ++q) s += wc[q] * fs[ia][min(max(fb + q, 0), kFineSpan - 1)];
+      bv += wr[a] * s;
+    }
+    if (t < kGH1 * kGH1) {
+      bs[ly][lx] = in ? bv : 0.f;
+      xs[ly][lx] = in ? gc.omega * d * d * bv : 0.f;
+    }
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+  const int R = R0 + ty, C = C0 + tx;
+  const bool in = on_grid(R, C, gc.h, gc.w);
+  const int i = in ? R * gc.w + C : 0;
+  float a[kGN];
+  load_taps(gc.A, nc, i, a);
+  const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
+  if (!in) return;
+  const float bv = bs[ty + kGR][tx + kGR];
+  bo[i] = bv;
+  xo[i] = xs[ty + kGR][tx + kGR];
+  ro[i] = bv - ax;
+}
+
+// x_t = M (P^T r_f) on the tail level: every block restricts the whole
+// (small) fine residual into LDS, then one wavefront per row of M
+template <int NV>
+__global__ void __launch_bounds__(kGTh)
+    k_g_tail(int hf, int wf, int hc, int wc, GBand br, GBand bc, const float* __restrict__ rf,
+             const float* __restrict__ M, int ld, float* __restrict__ xo,
+             const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  __shared__ float fs[kTailFine];
+  __shared__ float bs[kTailCoarse];
+  const int nf = hf * wf, nc = hc * wc;
+  const int i = blockIdx.x * 4 + threadIdx.x / kWave, l = threadIdx.x % kWave;
+  // this wavefront's row of M, in flight while the block restricts
+  const float4* row = reinterpret_cast<const float4*>(M + (size_t)min(i, nc - 1) * ld);
+  float4 m[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) m[v] = row[l + kWave * v];
+  for (int t = threadIdx.x; t < nf; t += kGTh) fs[t] = rf[t];
+  __syncthreads();
+  for (int J = threadIdx.x; J < nc; J += kGTh) {
+    const int R = J / wc, C = J - R * wc;
+    const int fa = br.f_lo[R], fb = bc.f_lo[C];
+    float bv = 0.f;
+#pragma unroll
+    for (int a = 0; a < kKFC; ++a) {
+      const float wa = br.rw[R * kKFC + a];
+      const int ra = min(fa + a, hf - 1);
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < kKFC; ++q) s += bc.rw[C * kKFC + q] * fs[ra * wf + min(fb + q, wf - 1)];
+      bv += wa * s;
+    }
+    bs[J] = bv;
+  }
+  __syncthreads();
+  if (i >= nc) return;
+  float s = 0.f;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int j = 4 * (l + kWave * v);
+    s += m[v].x * bs[min(j, nc - 1)] + m[v].y * bs[min(j + 1, nc - 1)] +
+         m[v].z * bs[min(j + 2, nc - 1)] + m[v].w * bs[min(j + 3, nc - 1)];
+  }
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (l == 0) xo[i] = s;
+}
+
+// fine tile of level l >= 1: x = x_pre + P x_c on the halo, then one damped
+// Jacobi step x + w D^-1 (b - A x) on the tile
+__global__ void __launch_bounds__(kGTh)
+    k_g_prolong(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ xc,
+                const float* __restrict__ xpre, const float* __restrict__ bf,
+                float* __restrict__ xo, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  __shared__ float cs[kCoarseSpan][kCoarseSpan];
+  __shared__ float xs[kGH1][kGH1];
+  const int n = g.h * g.w;
+  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
+  const int cr0 = br.c_lo[max(r0 - kGR, 0)], cc0 = bc.c_lo[max(c0 - kGR, 0)];
+  stage_coarse(cs, xc, hc, wc, cr0, cc0);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < passes(kGH1 * kGH1); ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const int rr = r0 - kGR + ly, cc = c0 - kGR + lx;
+    const bool in = t < kGH1 * kGH1 && on_grid(rr, cc, g.h, g.w);
+    const int rc = in ? rr : 0, ccc = in ? cc : 0;
+    const float v = xpre[rc * g.w + ccc] + prolong_at(br, bc, rc, ccc, cs, cr0, cc0);
+    if (t < kGH1 * kGH1) xs[ly][lx] = in ? v : 0.f;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+  const int rr = r0 + ty, cc = c0 + tx;
+  const bool in = on_grid(rr, cc, g.h, g.w);
+  const int i = in ? rr * g.w + cc : 0;
+  float a[kGN];
+  load_taps(g.A, n, i, a);
+  const float d = g.dsq[i], bi = bf[i];
+  const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
+  if (!in) return;
+  xo[i] = xs[ty + kGR][tx + kGR] + g.omega * d * d * (bi - ax);
+}
+
+// level-0 tile: x = x0 + P x1 on +-6, r = b0 - A0 x on +-3, z = x + S^T r on
+// the tile (S^T = D^-1/2 T_W^T D^-1/2, or w D^-1 without a stencil), and the
+// partial of z . dot.y per tile. The loads that need no other load (x0 on
+// the halo, the stencil values and right-hand side of every point, the band
+// rows of the tile's window) go out at entry; the coarse staging follows.
+template <bool STW>
+__global__ void __launch_bounds__(kGTh)
+    k_g0_post(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ W,
+              const float* __restrict__ xc, const float* __restrict__ x0,
+              const float* __restrict__ b0, float* __restrict__ z, const int* __restrict__ skip,
+              DotOut dot) {
+  if (skipped(skip)) return;
+  __shared__ float cs[kCoarseSpan][kCoarseSpan];
+  __shared__ float xs[kGH2][kGH2];
+  __shared__ float us[kGH1][kGH1];  // D^-1/2 r
+  __shared__ int clo[2][kGH2];      // band rows of the window: c_lo, weights
+  __shared__ float pwt[2][kGH2][kKPF];
+  constexpr int NX = kGH2 * kGH2, NXP = passes(NX);
+  constexpr int NH = kGH1 * kGH1, NHP = passes(NH);
+  const int n = g.h * g.w;
+  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
+  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+  const int rt = r0 + ty, ct = c0 + tx;
+  const bool in_t = on_grid(rt, ct, g.h, g.w);
+  const int it = in_t ? rt * g.w + ct : 0;
+  // band rows of the 28-point window in each direction (threads 0..55)
+  int bl = 0;
+  float bw[kKPF] = {0.f, 0.f, 0.f, 0.f};
+  {
+    const int d = threadIdx.x / kGH2, m = threadIdx.x % kGH2;
+    const int pos = (d == 0 ? r0 : c0) - 2 * kGR + m;
+    const int lim = d == 0 ? g.h : g.w;
+    const GBand& B = d == 0 ? br : bc;
+    const int q = min(max(pos, 0), lim - 1);
+    if (threadIdx.x < 2 * kGH2) {
+      bl = B.c_lo[q];
+#pragma unroll
+      for (int a2 = 0; a2 < kKPF; ++a2) bw[a2] = B.pw[q * kKPF + a2];
+    }
+  }
+  float xv[NXP];
+#pragma unroll
+  for (int k = 0; k < NXP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = r0 - 2 * kGR + t / kGH2, cc = c0 - 2 * kGR + t % kGH2;
+    xv[k] = x0[t < NX && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0];
+  }
+  float ah[NHP][kGN], bh[NHP], dh[NHP];
+#pragma unroll
+  for (int k = 0; k < NHP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = r0 - kGR + min(t / kGH1, kGH1 - 1), cc = c0 - kGR + t % kGH1;
+    const int i = t < NH && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0;
+    bh[k] = b0[i];
+    dh[k] = g.dsq[i];
+#pragma unroll
+    for (int o = 0; o < kGN; ++o) ah[k][o] = g.A[(size_t)o * n + i];
+  }
+  float wt[kGN];
+  if (STW) {
+#pragma unroll
+    for (int dy = -kGR; dy <= kGR; ++dy)
+#pragma unroll
+      for (int dx = -kGR; dx <= kGR; ++dx) {
+        const int o = (dy + kGR) * kGS + dx + kGR;
+        const bool in = on_grid(rt - dy, ct - dx, g.h, g.w);
+        wt[o] = W[(size_t)o * n + (in ? (rt - dy) * g.w + ct - dx : it)];
+      }
+  }
+  const float dt = g.dsq[it];
+  const double yt = dot.parts ? dot.y[it] : 0.0;
+  const int cr0 = br.c_lo[max(r0 - 2 * kGR, 0)], cc0 = bc.c_lo[max(c0 - 2 * kGR, 0)];
+  __builtin_amdgcn_sched_barrier(0);
+  stage_coarse(cs, xc, hc, wc, cr0, cc0);
+  if (threadIdx.x < 2 * kGH2) {
+    const int d = threadIdx.x / kGH2, m = threadIdx.x % kGH2;
+    clo[d][m] = bl;
+#pragma unroll
+    for (int a2 = 0; a2 < kKPF; ++a2) pwt[d][m][a2] = bw[a2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NXP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH2, kGH2 - 1), lx = t % kGH2;
+    const bool in = t < NX && on_grid(r0 - 2 * kGR + ly, c0 - 2 * kGR + lx, g.h, g.w);
+    const int ka = clo[0][ly] - cr0, kb = clo[1][lx] - cc0;
+    float v = 0.f;
+#pragma unroll
+    for (int a2 = 0; a2 < kKPF; ++a2) {
+      const int ia = min(max(ka + a2, 0), kCoarseSpan - 1);
+      float s2 = 0.f;
+#pragma unroll
+      for (int b2 = 0; b2 < kKPF; ++b2)
+        s2 += pwt[1][lx][b2] * cs[ia][min(max(kb + b2, 0), kCoarseSpan - 1)];
+      v += pwt[0][ly][a2] * s2;
+    }
+    if (t < NX) xs[ly][lx] = in ? xv[k] + v : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NHP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const bool in = t < NH && on_grid(r0 - kGR + ly, c0 - kGR + lx, g.h, g.w);
+    const float v = dh[k] * (bh[k] - tap_sum<kGH2>(ah[k], xs, ly + kGR, lx + kGR));
+    if (t < NH) us[ly][lx] = in ? v : 0.f;
+  }
+  __syncthreads();
+  float acc;
+  if (STW) {
+    acc = 0.f;
+#pragma unroll
+    for (int dy = -kGR; dy <= kGR; ++dy)
+#pragma unroll
+      for (int dx = -kGR; dx <= kGR; ++dx) {
+        const int o = (dy + kGR) * kGS + dx + kGR;
+        const bool in = on_grid(rt - dy, ct - dx, g.h, g.w);
+        acc += (in ? wt[o] : 0.f) * us[ty + kGR - dy][tx + kGR - dx];
+      }
+  } else {
+    acc = g.omega * us[ty + kGR][tx + kGR];
+  }
+  const float zi = xs[ty + 2 * kGR][tx + 2 * kGR] + dt * acc;
+  if (in_t) z[it] = zi;
+  if (dot.parts)
+    store_partial_at(dot.parts, blockIdx.y * gridDim.x + blockIdx.x, in_t ? (double)zi * yt : 0.0);
+}
+
+// FP64 flexible-CG product on level 0 in stencil layout: beta from the
+// z . q_prev partials, p = z + beta p_prev on the tile +- 3 (LDS), q = A p on
+// the tile; partials of p . q and p . r per tile
+__global__ void __launch_bounds__(kGTh)
+    k_fcg_spmv_grid(int h, int w, int nblk_a, const double* __restrict__ A,
+                    const float* __restrict__ z32, const double* __restrict__ pp,
+                    const double* __restrict__ r, double* __restrict__ p, double* __restrict__ q,
+                    FState* st, const double* __restrict__ P_a, double* __restrict__ P_b,
+                    double* __restrict__ P_c) {
+  __shared__ double ps[kGH1][kGH1];
+  constexpr int NH = kGH1 * kGH1, NHP = passes(NH);
+  const int n = h * w;
+  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
+  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+  const int rt = r0 + ty, ct = c0 + tx;
+  const bool in_t = on_grid(rt, ct, h, w);
+  const int it = in_t ? rt * w + ct : 0;
+  float zh[NHP];
+  double ph[NHP];
+#pragma unroll
+  for (int k = 0; k < NHP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int rr = r0 - kGR + min(t / kGH1, kGH1 - 1), cc = c0 - kGR + t % kGH1;
+    const int i = t < NH && on_grid(rr, cc, h, w) ? rr * w + cc : 0;
+    zh[k] = z32[i];
+    ph[k] = pp[i];
+  }
+  double a[kGN];
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = A[(size_t)o * n + it];
+  const double ri = r[it];
+  __builtin_amdgcn_sched_barrier(0);
+  if (st->done) return;
+  const double beta = -reduce_partials(P_a, nblk_a) / st->pq_prev;
+#pragma unroll
+  for (int k = 0; k < NHP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const bool in = t < NH && on_grid(r0 - kGR + ly, c0 - kGR + lx, h, w);
+    if (t < NH) ps[ly][lx] = in ? (double)zh[k] + beta * ph[k] : 0.0;
+  }
+  __syncthreads();
+  const double qi = tap_sum<kGH1>(a, ps, ty + kGR, tx + kGR);
+  const double pi = ps[ty + kGR][tx + kGR];
+  if (in_t) {
+    p[it] = pi;
+    q[it] = qi;
+  }
+  const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+  store_partial_at(P_c, blk, in_t ? pi * qi : 0.0);
+  store_partial_at(P_b, blk, in_t ? pi * ri : 0.0);
+}
+
+// stencil layout of the level operators (blockIdx.y = level; the FP64 copy of
+// level 0 too when op64 is set). A block converts 64 consecutive rows: their
+// CSR segment is staged in LDS with coalesced reads, then every (offset, row)
+// pair is written coalesced along the rows. A row's CSR entries are its
+// in-grid neighbours in row-major order, so entry (dy, dx) sits at an offset
+// computed from the clipped window; an entry found elsewhere (a pattern with
+// holes, or a segment larger than the staging) is searched in the CSR row.
+struct GridOpsArgs {
+  int n_levels;
+  int n[16], w[16];
+  const int32_t* indptr[16];
+  const int32_t* cols[16];
+  const float* vals[16];
+  float* op[16];
+  const double* vals64;
+  double* op64;
+};
+
+constexpr int kOpsRows = 64;
+constexpr int kOpsMax = kOpsRows * kGN;
+
+__global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
+  __shared__ int sc[kOpsMax];
+  __shared__ float sv[kOpsMax];
+  __shared__ double sv64[kOpsMax];
+  __shared__ int sp[kOpsRows + 1];
+  const int l = blockIdx.y;
+  if (l >= a.n_levels) return;
+  const int n = a.n[l], w = a.w[l], h = n / w;
+  const int i0 = blockIdx.x * kOpsRows;
+  if (i0 >= n) return;
+  const int i1 = min(i0 + kOpsRows, n);
+  const int32_t* cols = a.cols[l];
+  const float* vals = a.vals[l];
+  const int lo = a.indptr[l][i0], cnt = a.indptr[l][i1] - lo;
+  const bool f64 = l == 0 && a.op64;
+  const bool fits = cnt <= kOpsMax;
+  if (threadIdx.x <= i1 - i0) sp[threadIdx.x] = a.indptr[l][i0 + threadIdx.x] - lo;
+  if (fits && cnt > 0) {
+    constexpr int NP = passes(kOpsMax);
+    int c[NP];
+    float v[NP];
+    double v64[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int t = min((int)threadIdx.x + k * kGTh, max(cnt - 1, 0));
+      c[k] = cols[lo + t];
+      v[k] = vals[lo + t];
+      v64[k] = f64 ? a.vals64[lo + t] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      if (t < cnt) {
+        sc[t] = c[k];
+        sv[t] = v[k];
+        sv64[t] = v64[k];
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kGN * kOpsRows; t += kGTh) {
+    const int il = t % kOpsRows, o = t / kOpsRows;
+    const int i = i0 + il;
+    if (i >= i1) continue;
+    const int r = i / w, c = i - r * w;
+    const int dy = o / kGS - kGR, dx = o % kGS - kGR;
+    float v = 0.f;
+    double v64 = 0.0;
+    if (on_grid(r + dy, c + dx, h, w)) {
+      const int j = i + dy * w + dx;
+      const int s0 = sp[il], e0 = sp[il + 1];
+      const int dy0 = max(-kGR, -r), dx0 = max(-kGR, -c);
+      const int ncol = min(kGR, w - 1 - c) - dx0 + 1;
+      const int k = s0 + (dy - dy0) * ncol + (dx - dx0);
+      if (fits && k < e0 && sc[k] == j) {
+        v = sv[k];
+        v64 = sv64[k];
+      } else {
+        for (int q = lo + s0; q < lo + e0; ++q)
+          if (cols[q] == j) {
+            v = vals[q];
+            v64 = f64 ? a.vals64[q] : 0.0;
+          }
+      }
+    }
+    a.op[l][(size_t)o * n + i] = v;
+    if (f64) a.op64[(size_t)o * n + i] = v64;
+  }
+}
+
+static inline GLevel glevel(const iga_mg_t& mg, int l) {
+  const iga_mg_level_t& L = mg.level[l];
+  return GLevel{L.h, L.w, mg.grid->op[l], L.dsq, L.omega};
+}
+
+static inline GBand gband(const iga_mg_t& mg, int l, int d) {
+  const iga_mg_band_t& B = mg.grid->band[l][d];
+  return GBand{B.c_lo, B.pw, B.f_lo, B.rw};
+}
+
+// whether the grid V-cycle applies (see iga_mg_grid_t)
+static bool grid_cycle_ok(const iga_mg_t& mg) {
+  const iga_mg_grid_t* G = mg.grid;
+  if (!G || !mg.tail || G->radius != kGR) return false;
+  const int t = mg.tail_level;
+  if (t < 2 || t >= mg.n_levels - 1 || G->n_levels < t) return false;
+  if (mg.level[t - 1].n > kTailFine || mg.level[t].n > kTailCoarse) return false;
+  if (mg.level[0].stencil && mg.ks != kGS) return false;
+  for (int l = 1; l < t; ++l)
+    if (mg.level[l].stencil) return false;
+  for (int l = 0; l < t; ++l) {
+    if (!G->op[l]) return false;
+    for (int d = 0; d < 2; ++d) {
+      const iga_mg_band_t& B = G->band[l][d];
+      if (!B.c_lo || !B.pw || !B.f_lo || !B.rw) return false;
+    }
+  }
+  return true;
+}
+
+static inline dim3 tiles_of(const iga_mg_level_t& L) {
+  return dim3((L.w + kGT - 1) / kGT, (L.h + kGT - 1) / kGT);
+}
+
+// Per-level buffers of the grid cycle: x (pre-smoothed), b, r, x (post)
+static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate& gate,
+                       const DotOut& dot, int* dot_blocks, hipStream_t st) {
+  const int t = mg.tail_level;
+  float* X[16];
+  float* B[16];
+  float* Rr[16];
+  float* XO[16];
+  for (int l = 0; l <= t; ++l) {
+    float* base = mg.work + level_offset(mg, l);
+    const int n = mg.level[l].n;
+    X[l] = base;
+    B[l] = base + n;
+    Rr[l] = base + 2 * n;
+    XO[l] = base + 3 * n;
+  }
+  const int* skip = gate.flag;
+  const iga_mg_level_t& L0 = mg.level[0];
+  const GLevel g0 = glevel(mg, 0);
+  if (L0.stencil)
+    hipLaunchKernelGGL(k_g0_pre<true>, tiles_of(L0), dim3(kGTh), 0, st, g0, L0.stencil, b0, X[0],
+                       Rr[0], gate);
+  else
+    hipLaunchKernelGGL(k_g0_pre<false>, tiles_of(L0), dim3(kGTh), 0, st, g0, nullptr, b0, X[0],
+                       Rr[0], gate);
+  IGA_CHECK_LAUNCH();
+  for (int l = 0; l + 1 < t; ++l) {
+    const iga_mg_level_t& F = mg.level[l];
+    hipLaunchKernelGGL(k_g_restrict, tiles_of(mg.level[l + 1]), dim3(kGTh), 0, st, F.h, F.w,
+                       glevel(mg, l + 1), gband(mg, l, 1), gband(mg, l, 0), Rr[l], B[l + 1],
+                       X[l + 1], Rr[l + 1], skip);
+    IGA_CHECK_LAUNCH();
+  }
+  {
+    const iga_mg_level_t& F = mg.level[t - 1];
+    const iga_mg_level_t& C = mg.level[t];
+    const int ld = dense_ld(C.n);
+    auto go = [&](auto V) {
+      constexpr int NV = decltype(V)::value;
+      hipLaunchKernelGGL(k_g_tail<NV>, dim3((C.n + 3) / 4), dim3(kGTh), 0, st, F.h, F.w, C.h, C.w,
+                         gband(mg, t - 1, 1), gband(mg, t - 1, 0), Rr[t - 1], mg.tail, ld, XO[t],
+                         skip);
+      IGA_CHECK_LAUNCH();
+      return (int)IGA_OK;
+    };
+    int rc;
+    switch (ld / 256) {
+      case 1: rc = go(std::integral_constant<int, 1>{}); break;
+      case 2: rc = go(std::integral_constant<int, 2>{}); break;
+      case 3: rc = go(std::integral_constant<int, 3>{}); break;
+      case 4: rc = go(std::integral_constant<int, 4>{}); break;
+      case 5: rc = go(std::integral_constant<int, 5>{}); break;
+      case 6: rc = go(std::integral_constant<int, 6>{}); break;
+      case 7: rc = go(std::integral_constant<int, 7>{}); break;
+      case 8: rc = go(std::integral_constant<int, 8>{}); break;
+      default: rc = IGA_ERR_UNSUPPORTED;
+    }
+    if (rc) return rc;
+  }
+  for (int l = t - 1; l >= 1; --l) {
+    const iga_mg_level_t& C = mg.level[l + 1];
+    hipLaunchKernelGGL(k_g_prolong, tiles_of(mg.level[l]), dim3(kGTh), 0, st, glevel(mg, l), C.h,
+                       C.w, gband(mg, l, 1), gband(mg, l, 0), XO[l + 1], X[l], B[l], XO[l], skip);
+    IGA_CHECK_LAUNCH();
+  }
+  const iga_mg_level_t& L1 = mg.level[1];
+  const dim3 tg = tiles_of(L0);
+  const bool fuse = dot.parts && (int)(tg.x * tg.y) <= kRedBlocks;
+  const DotOut d = fuse ? dot : DotOut{nullptr, nullptr};
+  if (L0.stencil)
+    hipLaunchKernelGGL(k_g0_post<true>, tg, dim3(kGTh), 0, st, g0, L1.h, L1.w, gband(mg, 0, 1),
+                       gband(mg, 0, 0), L0.stencil, XO[1], X[0], b0, z, skip, d);
+  else
+    hipLaunchKernelGGL(k_g0_post<false>, tg, dim3(kGTh), 0, st, g0, L1.h, L1.w, gband(mg, 0, 1),
+                       gband(mg, 0, 0), nullptr, XO[1], X[0], b0, z, skip, d);
+  IGA_CHECK_LAUNCH();
+  if (fuse) {
+    *dot_blocks = (int)(tg.x * tg.y);
+  } else if (dot.parts) {
+    const int nb = std::min(blocks_for(L0.n), kRedBlocks);
+    hipLaunchKernelGGL(k_dot_fd, dim3(nb), dim3(256), 0, st, L0.n, z, skip, dot);
+    IGA_CHECK_LAUNCH();
+    *dot_blocks = nb;
+  }
+  return IGA_OK;
+}
+
 // x = V(b) starting at level l0 (b, x of level l0's size). Levels with a
 // stencil smoother run pre-smooth / residual / restrict and prolong /
 // residual / post-smooth as separate kernels; damped-Jacobi levels fuse the
-// smoother into the residual kernels.
-static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const int* skip,
-                  hipStream_t st) {
+// smoother into the residual kernels; below the tail level the cycle is one
+// dense product. `gate` gates the first kernel, its flag every later one.
+// With `dot.parts` set, the partials of x . dot.y come with the last kernel
+// (or one more kernel) and their count is returned in *dot_blocks.
+static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const Gate& gate,
+                  const DotOut& dot, int* dot_blocks, hipStream_t st) {
+  if (l0 == 0 && grid_cycle_ok(mg)) return vcycle_grid(mg, b0, x0, gate, dot, dot_blocks, st);
   const int Lc = mg.n_levels - 1;
   const float* B[16];
   float* X[16];
@@ -568,20 +1441,30 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
   }
   X[l0] = x0;
   B[l0] = b0;
+  const int* skip = gate.flag;
+  const Gate later{gate.flag, nullptr, 0, nullptr};
+  const DotOut none{nullptr, nullptr};
+  bool first = true;
+  auto next_gate = [&]() -> Gate {
+    if (!first) return later;
+    first = false;
+    return gate;
+  };
+  bool dot_done = dot.parts == nullptr;
   int rc;
-  // below the tail level the cycle is one dense product
   const int t = (mg.tail && mg.tail_level >= l0 && mg.tail_level < Lc) ? mg.tail_level : Lc;
   for (int l = l0; l < t; ++l) {
     const iga_mg_level_t& L = mg.level[l];
     if (L.stencil) {
-      if ((rc = smooth(mg, l, B[l], X[l], false, 0, skip, st))) return rc;
+      if ((rc = smooth(mg, l, B[l], X[l], false, 0, next_gate(), none, st))) return rc;
       if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
         return rc;
     } else {
+      const Gate g = next_gate();
       rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
         constexpr int LPR = decltype(K)::value;
         hipLaunchKernelGGL(k_jacobi_pre<LPR>, dim3(rb), dim3(256), 0, st, L.n, L.indptr, L.cols,
-                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], skip, (int64_t)0);
+                           L.vals, L.omega, L.dsq, B[l], X[l], Rr[l], g, (int64_t)0);
       });
       if (rc) return rc;
     }
@@ -592,9 +1475,14 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
   }
   {
     const iga_mg_level_t& C = mg.level[t];
-    hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n,
-                       t < Lc ? mg.tail : mg.coarse_inv, B[t], X[t], skip, (int64_t)0);
-    IGA_CHECK_LAUNCH();
+    const Gate g = next_gate();
+    if (t < Lc) {
+      if ((rc = dense_tail_gemv(C.n, mg.tail, B[t], X[t], g, 1, st))) return rc;
+    } else {
+      hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, mg.coarse_inv,
+                         B[t], X[t], g, (int64_t)0);
+      IGA_CHECK_LAUNCH();
+    }
   }
   for (int l = t - 1; l >= l0; --l) {
     const iga_mg_level_t& L = mg.level[l];
@@ -604,7 +1492,12 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
     if (L.stencil) {
       if ((rc = csr_f(L.n, L.nnz, L.indptr, L.cols, L.vals, X[l], B[l], Rr[l], 1, 1, 0, 0, skip, st)))
         return rc;
-      if ((rc = smooth(mg, l, Rr[l], X[l], true, 1, skip, st))) return rc;
+      const bool fuse = l == l0 && !dot_done && stencil_tiles(L) <= kRedBlocks;
+      if ((rc = smooth(mg, l, Rr[l], X[l], true, 1, later, fuse ? dot : none, st))) return rc;
+      if (fuse) {
+        *dot_blocks = stencil_tiles(L);
+        dot_done = true;
+      }
     } else {
       // post-smooth into the residual buffer, then copy back only at the
       // caller's level (the parent prolongs from whichever buffer holds x)
@@ -616,10 +1509,23 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
       if (rc) return rc;
       std::swap(X[l], Rr[l]);
       if (l == l0) {
-        hipLaunchKernelGGL(k_copy_fs, dim3(blocks_for(L.n)), dim3(256), 0, st, L.n, X[l], x0, skip);
+        const int nb = std::min(blocks_for(L.n), kRedBlocks);
+        hipLaunchKernelGGL(k_copy_fs, dim3(nb), dim3(256), 0, st, L.n, X[l], x0, skip,
+                           dot_done ? none : dot);
         IGA_CHECK_LAUNCH();
+        if (!dot_done) {
+          *dot_blocks = nb;
+          dot_done = true;
+        }
       }
     }
+  }
+  if (!dot_done) {
+    const int n = mg.level[l0].n;
+    const int nb = std::min(blocks_for(n), kRedBlocks);
+    hipLaunchKernelGGL(k_dot_fd, dim3(nb), dim3(256), 0, st, n, x0, skip, dot);
+    IGA_CHECK_LAUNCH();
+    *dot_blocks = nb;
   }
   return IGA_OK;
 }
@@ -629,6 +1535,7 @@ static int check_mg(const iga_mg_t* mg) {
     return IGA_ERR_ARG;
   if (mg->tail) {
     if (mg->tail_level < 0 || mg->tail_level >= mg->n_levels - 1) return IGA_ERR_ARG;
+    if (mg->level[mg->tail_level].n > kDenseMaxRows) return IGA_ERR_UNSUPPORTED;
     for (int l = mg->tail_level; l < mg->n_levels - 1; ++l)
       if (mg->level[l].stencil) return IGA_ERR_UNSUPPORTED;
   }
@@ -649,13 +1556,14 @@ static int check_mg(const iga_mg_t* mg) {
 // ---------------------------------------------------------------------------
 // flexible CG (Notay's FCG with one stored direction): device-resident
 // scalars; G iterations are captured into a hipGraph and replayed, and the
-// host reads the convergence flag once per replay
+// host reads the convergence flag once per replay. One iteration is
+//   z = V(r)                       (V-cycle; its last kernel adds z . q_prev)
+//   beta = -(z . q_prev) / (p_prev . q_prev),  p = z + beta p_prev,  q = A p
+//                                  (one kernel: p formed row by row)
+//   alpha = (p . r) / (p . q),  x += alpha p,  r -= alpha q
+// and the test ||r|| <= tol ||b|| is made by the first kernel of the next
+// V-cycle (the gate).
 // ---------------------------------------------------------------------------
-
-struct FState {
-  double rr, bb, tol2, pq_prev;
-  int iter, maxiter, done, pad;
-};
 
 struct FBufs {
   FState* st;
@@ -673,16 +1581,18 @@ __global__ void __launch_bounds__(kVecThreads)
     x[i] = 0.0;
     r[i] = b[i];
     r32[i] = (float)b[i];
-    p1[i] = 0.0;  // the "previous" direction of the first iteration
+    p1[i] = 0.0;  // the "previous" direction of the first iteration (beta = 0)
     q1[i] = 0.0;
     bb += b[i] * b[i];
   }
   store_partial(F.P_d, bb);
 }
 
+// init: record ||b||^2 and the tolerance; otherwise record ||r||^2 and the
+// flag after the last iteration of a graph (the iteration count is kept by
+// the gates)
 __global__ void k_fcg_check(int nblk, FBufs F, int init, double tol2, int maxiter) {
   FState* st = F.st;
-  if (!init && st->done) return;
   const double rr = reduce_partials(F.P_d, nblk);
   if (threadIdx.x == 0) {
     if (init) {
@@ -691,68 +1601,83 @@ __global__ void k_fcg_check(int nblk, FBufs F, int init, double tol2, int maxite
       st->maxiter = maxiter;
       st->iter = 0;
       st->pq_prev = 1.0;
-    } else {
-      st->iter += 1;
     }
     st->rr = rr;
-    st->done = (!(rr > st->tol2 * st->bb) || st->iter >= st->maxiter) ? 1 : 0;
+    st->done = !(rr > st->tol2 * st->bb) ? 1 : 0;
   }
 }
 
-// z = (double) z32 (the V-cycle output) and the partials of z . q_prev
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_zq(int n, const float* __restrict__ z32, double* __restrict__ z,
-             const double* __restrict__ qp, FBufs F) {
-  if (F.st->done) return;
-  double s = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double zi = (double)z32[i];
-    z[i] = zi;
-    s += zi * qp[i];
-  }
-  store_partial(F.P_a, s);
-}
-
-__global__ void __launch_bounds__(kVecThreads)
-    k_fcg_dir(int n, int nblk, const double* __restrict__ z, const double* __restrict__ pp,
-              double* p, const double* __restrict__ r, FBufs F) {
-  const FState* st = F.st;
-  if (st->done) return;
-  double beta = 0.0;
-  if (st->iter > 0) beta = -reduce_partials(F.P_a, nblk) / st->pq_prev;
-  double pr = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double pi = z[i] + beta * pp[i];
-    p[i] = pi;
-    pr += pi * r[i];
-  }
-  store_partial(F.P_b, pr);
-}
-
+// beta from the z . q_prev partials; p = z + beta p_prev (z = the FP32
+// V-cycle output, widened), q = A p with p formed at every column read; the
+// partials of p . q and p . r
 template <int LPR>
 __global__ void __launch_bounds__(kVecThreads)
-    k_fcg_spmv(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
-               const double* __restrict__ vals, const double* __restrict__ p, double* q, FBufs F) {
-  if (F.st->done) return;
+    k_fcg_spmv(int n, int nblk_a, const int32_t* __restrict__ indptr,
+               const int32_t* __restrict__ cols, const double* __restrict__ vals,
+               const float* __restrict__ z32, const double* __restrict__ pp,
+               const double* __restrict__ r, double* __restrict__ p, double* __restrict__ q,
+               FBufs F) {
+  const FState* st = F.st;
+  if (st->done) return;
+  const double beta = -reduce_partials(F.P_a, nblk_a) / st->pq_prev;
   const int l = threadIdx.x % LPR;
   const int groups = gridDim.x * (blockDim.x / LPR);
-  double pq = 0.0;
-  for (int r = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < n; r += groups) {
-    const double s = row_dot<LPR>(r, indptr, cols, vals, p, l);
+  double pq = 0.0, pr = 0.0;
+  for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
+    const int lo = indptr[row], hi = indptr[row + 1];
+    double s = 0.0;
+    for (int k = lo + l; k < hi; k += LPR) {
+      const int j = cols[k];
+      s += vals[k] * ((double)z32[j] + beta * pp[j]);
+    }
+#pragma unroll
+    for (int off = LPR / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LPR);
     if (l == 0) {
-      q[r] = s;
-      pq += s * p[r];
+      const double pi = (double)z32[row] + beta * pp[row];
+      p[row] = pi;
+      q[row] = s;
+      pq += pi * s;
+      pr += pi * r[row];
     }
   }
-  store_partial(F.P_c, pq);
+  __shared__ double scratch[16];
+  pq = block_sum(pq, scratch);
+  if (threadIdx.x == 0) F.P_c[blockIdx.x] = pq;
+  pr = block_sum(pr, scratch);
+  if (threadIdx.x == 0) F.P_b[blockIdx.x] = pr;
+}
+
+// independent FP64 residual of the result: partials of ||b - A x||^2 (P_b),
+// then their sum into the state
+template <int LPR>
+__global__ void __launch_bounds__(kVecThreads)
+    k_fcg_resid(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                const double* __restrict__ vals, const double* __restrict__ x,
+                const double* __restrict__ b, FBufs F) {
+  const int l = threadIdx.x % LPR;
+  const int groups = gridDim.x * (blockDim.x / LPR);
+  double ss = 0.0;
+  for (int row = blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; row < n; row += groups) {
+    const double s = row_dot<LPR>(row, indptr, cols, vals, x, l);
+    if (l == 0) {
+      const double ri = b[row] - s;
+      ss += ri * ri;
+    }
+  }
+  store_partial(F.P_b, ss);
+}
+
+__global__ void k_fcg_resid_fin(int nblk, FBufs F) {
+  const double s = reduce_partials(F.P_b, nblk);
+  if (threadIdx.x == 0) F.st->res2 = s;
 }
 
 __global__ void __launch_bounds__(kVecThreads)
-    k_fcg_update(int n, int nblk_c, int nblk, const double* __restrict__ p,
-                 const double* __restrict__ q, double* x, double* r, float* r32, FBufs F) {
+    k_fcg_update(int n, int nblk, const double* __restrict__ p, const double* __restrict__ q,
+                 double* x, double* r, float* r32, FBufs F) {
   FState* st = F.st;
   if (st->done) return;
-  const double pq = reduce_partials(F.P_c, nblk_c);
+  const double pq = reduce_partials(F.P_c, nblk);
   const double pr = reduce_partials(F.P_b, nblk);
   const double alpha = pr / pq;
   double rr = 0.0;
@@ -862,9 +1787,12 @@ __global__ void k_identity_f(int n, float* __restrict__ out) {
   if (t < (int64_t)n * n) out[t] = (t / n == t % n) ? 1.f : 0.f;
 }
 
-__global__ void k_transpose_f(int n, const float* __restrict__ a, float* __restrict__ out) {
+// out (n x ld, zero beyond column n) = a^T (a n x n)
+__global__ void k_transpose_f(int n, int ld, const float* __restrict__ a, float* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (int64_t)n * n) out[t] = a[(t % n) * n + t / n];
+  if (t >= (int64_t)n * ld) return;
+  const int i = (int)(t / ld), j = (int)(t % ld);
+  out[t] = j < n ? a[(int64_t)j * n + i] : 0.f;
 }
 
 // Floats of the batched cycle from level t: b, x, r for every level, each
@@ -904,7 +1832,7 @@ static int tail_matrix(const iga_mg_t& mg, int t, float* work, float* out, hipSt
     rc = launch_rows(L.n, L.nnz, [&](auto K, int rb) {
       constexpr int LPR = decltype(K)::value;
       hipLaunchKernelGGL(k_jacobi_pre<LPR>, dim3(rb, Bn), dim3(256), 0, st, L.n, L.indptr, L.cols,
-                         L.vals, L.omega, L.dsq, Bv[l], X[l], Rr[l], nullptr, (int64_t)L.n);
+                         L.vals, L.omega, L.dsq, Bv[l], X[l], Rr[l], Gate{}, (int64_t)L.n);
     });
     if (rc) return rc;
     const iga_mg_level_t& C = mg.level[l + 1];
@@ -915,7 +1843,7 @@ static int tail_matrix(const iga_mg_t& mg, int t, float* work, float* out, hipSt
   {
     const iga_mg_level_t& C = mg.level[Lc];
     hipLaunchKernelGGL(k_dense_gemv, dim3((C.n + 3) / 4, Bn), dim3(256), 0, st, C.n,
-                       mg.coarse_inv, Bv[Lc], X[Lc], nullptr, (int64_t)C.n);
+                       mg.coarse_inv, Bv[Lc], X[Lc], Gate{}, (int64_t)C.n);
     IGA_CHECK_LAUNCH();
   }
   for (int l = Lc - 1; l >= t; --l) {
@@ -932,8 +1860,9 @@ static int tail_matrix(const iga_mg_t& mg, int t, float* work, float* out, hipSt
     if (rc) return rc;
     std::swap(X[l], Rr[l]);
   }
-  hipLaunchKernelGGL(k_transpose_f, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Bn, X[t],
-                     out);
+  const int64_t nl = (int64_t)Bn * dense_ld(Bn);
+  hipLaunchKernelGGL(k_transpose_f, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, Bn,
+                     dense_ld(Bn), X[t], out);
   IGA_CHECK_LAUNCH();
   return IGA_OK;
 }
@@ -944,11 +1873,42 @@ extern "C" int64_t iga_mg_tail_floats(const iga_mg_t* mg, int32_t level) {
   return tail_floats(*mg, level);
 }
 
+extern "C" int iga_mg_dense_ld(int32_t n) { return n > 0 ? dense_ld(n) : 0; }
+
+extern "C" int iga_mg_grid_ops(const iga_mg_t* mg, const double* vals64, void* stream) {
+  if (!mg || !mg->grid || mg->n_levels < 1 || mg->n_levels > 16) return IGA_ERR_ARG;
+  const iga_mg_grid_t& G = *mg->grid;
+  if (G.radius != kGR) return IGA_ERR_UNSUPPORTED;
+  if (G.n_levels < 1 || G.n_levels > mg->n_levels) return IGA_ERR_ARG;
+  GridOpsArgs a{};
+  a.n_levels = G.n_levels;
+  int nmax = 0;
+  for (int l = 0; l < G.n_levels; ++l) {
+    const iga_mg_level_t& L = mg->level[l];
+    if (!L.indptr || !L.cols || !L.vals || !G.op[l]) return IGA_ERR_ARG;
+    if (L.n <= 0 || L.w <= 0 || (int64_t)L.h * L.w != L.n) return IGA_ERR_SHAPE;
+    a.n[l] = L.n;
+    a.w[l] = L.w;
+    a.indptr[l] = L.indptr;
+    a.cols[l] = L.cols;
+    a.vals[l] = L.vals;
+    a.op[l] = G.op[l];
+    nmax = std::max(nmax, L.n);
+  }
+  a.vals64 = vals64;
+  a.op64 = vals64 ? G.op64 : nullptr;
+  hipLaunchKernelGGL(k_grid_ops, dim3((nmax + kOpsRows - 1) / kOpsRows, G.n_levels), dim3(kGTh),
+                     0, (hipStream_t)stream, a);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
 extern "C" int iga_mg_tail(const iga_mg_t* mg, int32_t level, float* work, float* out,
                            void* stream) {
   if (!mg || !work || !out || !mg->coarse_inv) return IGA_ERR_ARG;
   if (mg->n_levels < 2 || mg->n_levels > 16 || level < 0 || level >= mg->n_levels - 1)
     return IGA_ERR_ARG;
+  if (mg->level[level].n > kDenseMaxRows) return IGA_ERR_UNSUPPORTED;
   for (int l = level; l < mg->n_levels - 1; ++l) {
     const iga_mg_level_t& L = mg->level[l];
     if (L.stencil) return IGA_ERR_UNSUPPORTED;
@@ -962,7 +1922,8 @@ extern "C" int iga_mg_vcycle(const iga_mg_t* mg, int32_t level, const float* b, 
   int rc = check_mg(mg);
   if (rc) return rc;
   if (level < 0 || level >= mg->n_levels || !b || !x) return IGA_ERR_ARG;
-  return vcycle(*mg, level, b, x, nullptr, (hipStream_t)stream);
+  int dot_blocks = 0;
+  return vcycle(*mg, level, b, x, Gate{}, DotOut{}, &dot_blocks, (hipStream_t)stream);
 }
 
 extern "C" int iga_csr_f32(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* cols,
@@ -987,10 +1948,10 @@ extern "C" int iga_stencil_apply(int32_t h, int32_t w, int32_t ks, const float* 
     constexpr int KS = decltype(K)::value;
     if (transpose)
       hipLaunchKernelGGL((k_stencil<KS, true>), grid, dim3(256), 0, st, h, w, stencil, dsq, v, out,
-                         accumulate, nullptr);
+                         accumulate, Gate{}, DotOut{});
     else
       hipLaunchKernelGGL((k_stencil<KS, false>), grid, dim3(256), 0, st, h, w, stencil, dsq, v,
-                         out, accumulate, nullptr);
+                         out, accumulate, Gate{}, DotOut{});
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
   });
@@ -1022,16 +1983,16 @@ extern "C" int iga_energy_loss(int32_t batch, int64_t n, const float* z, const f
   return IGA_OK;
 }
 
-extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 6 * (int64_t)n + 4 * kRedBlocks; }
+extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 5 * (int64_t)n + 4 * kRedBlocks; }
 
 // A captured flexible-CG solve over fixed buffers: every run re-initialises
 // the state on the device and replays graphs until the device flag says
 // done, so one plan serves every right-hand side written into `b` and every
 // operator whose values are rewritten in place. Graphs of 8, 4, 2 and 1
 // iterations are kept; a run first replays as many iterations as the last
-// run needed, less one, without looking, then one iteration per look, so
-// few iterations run past convergence (their kernels return at once but
-// still cost a launch each).
+// run needed without looking, then one iteration per look, so few
+// iterations run past convergence (their kernels return at once but still
+// cost a launch each) and a run like the last one reads the flag once.
 struct FcgGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1040,13 +2001,15 @@ struct FcgGraph {
 struct iga_fcg_plan {
   iga_mg_t mg;
   int n, nnz, lpr, vb, rb, eb, G;
+  bool grid64;  // level-0 product in FP64 stencil layout (grid cycle)
+  int nb_spmv;  // blocks (partials) of the product kernel
   const int32_t* indptr;
   const int32_t* cols;
   const double* vals;
   const double* b;
   double* x;
   FBufs F;
-  double *r, *z, *P[2], *Q[2];
+  double *r, *P[2], *Q[2];
   float *r32, *z32;
   FcgGraph g8, g4, g2, g1[2];  // g8/g4/g2 start at parity 0, g1[p] at parity p
   int last_iters = 0;
@@ -1056,28 +2019,40 @@ struct iga_fcg_plan {
     double* pp = P[1 - par];
     double* q = Q[par];
     double* qp = Q[1 - par];
-    const int* skip = &F.st->done;
-    int rc = vcycle(mg, 0, r32, z32, skip, st);
+    const Gate gate{&F.st->done, F.P_d, vb, F.st};
+    const DotOut dot{qp, F.P_a};
+    int na = 0;
+    int rc = vcycle(mg, 0, r32, z32, gate, dot, &na, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fcg_zq, dim3(vb), dim3(kVecThreads), 0, st, n, z32, z, qp, F);
-    hipLaunchKernelGGL(k_fcg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, z, pp, p, r, F);
-    by_lpr(lpr, [&](auto L) {
-      constexpr int LPR = decltype(L)::value;
-      hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
-                         vals, p, q, F);
-      return 0;
-    });
-    hipLaunchKernelGGL(k_fcg_update, dim3(vb), dim3(kVecThreads), 0, st, n, rb, vb, p, q, x, r,
+    if (grid64) {
+      const iga_mg_level_t& L0 = mg.level[0];
+      hipLaunchKernelGGL(k_fcg_spmv_grid, tiles_of(L0), dim3(kGTh), 0, st, L0.h, L0.w, na,
+                         mg.grid->op64, z32, pp, r, p, q, F.st, F.P_a, F.P_b, F.P_c);
+    } else {
+      by_lpr(lpr, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, na, indptr, cols,
+                           vals, z32, pp, r, p, q, F);
+        return 0;
+      });
+    }
+    hipLaunchKernelGGL(k_fcg_update, dim3(vb), dim3(kVecThreads), 0, st, n, nb_spmv, p, q, x, r,
                        r32, F);
-    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 0, 0.0, 0);
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
+  }
+
+  int check(hipStream_t st) {
+    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 0, 0.0, 0);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
   }
 
   int capture_one(FcgGraph& out, int iters, int par0, hipStream_t st) {
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
     int rc = 0;
     for (int k = 0; k < iters && !rc; ++k) rc = iteration((par0 + k) & 1, st);
+    if (!rc) rc = check(st);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc || e != hipSuccess) {
@@ -1133,8 +2108,20 @@ struct iga_fcg_plan {
     return IGA_OK;
   }
 
+  int true_residual(hipStream_t st) {
+    by_lpr(lpr, [&](auto L) {
+      constexpr int LPR = decltype(L)::value;
+      hipLaunchKernelGGL(k_fcg_resid<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
+                         vals, x, b, F);
+      return 0;
+    });
+    hipLaunchKernelGGL(k_fcg_resid_fin, dim3(1), dim3(kVecThreads), 0, st, rb, F);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
+  }
+
   int run(double tol, int maxiter, int32_t* out_iters, double* out_rr, double* out_bb,
-          hipStream_t st) {
+          double* out_res, hipStream_t st) {
     hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, r, r32, P[1], Q[1],
                        F);
     hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol,
@@ -1142,30 +2129,49 @@ struct iga_fcg_plan {
     IGA_CHECK_LAUNCH();
     FState host{};
     int launched = 0, status = IGA_OK;
+    // the FP64 residual is recomputed after every batch when asked for, so
+    // the look that finds convergence also reads it
+    bool res_fresh = false;
     const bool graphs = g1[0].exec != nullptr;
-    // iterations before the first look: the last run's count less one (four
-    // on the first run), then one per look
-    int next = last_iters > 1 ? last_iters - 1 : 4;
-    for (;;) {
-      if (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess) {
+    // iterations before the first look: the last run's count (four on the
+    // first run), then one per look; never more than maxiter in all
+    int next = last_iters > 0 ? last_iters : 4;
+    for (bool look = false;; look = true) {
+      if (look && (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) !=
+                       hipSuccess ||
+                   hipStreamSynchronize(st) != hipSuccess)) {
         status = IGA_ERR_HIP;
         break;
       }
-      if (host.done) break;
+      if ((look && host.done) || launched >= maxiter) {
+        if (!look) continue;
+        if (!out_res || res_fresh) break;
+        if ((status = true_residual(st))) break;
+        res_fresh = true;
+        continue;
+      }
+      const int k = std::min(next, maxiter - launched);
       if (graphs) {
-        status = replay(next, launched, st);
+        status = replay(k, launched, st);
         if (status) break;
-        next = last_iters > 1 ? 1 : 2;
+        next = 1;
       } else {
-        for (int k = 0; k < 2 && !status; ++k, ++launched) status = iteration(launched & 1, st);
+        for (int j = 0; j < std::min(k, 2) && !status; ++j, ++launched)
+          status = iteration(launched & 1, st);
+        if (!status) status = check(st);
         if (status) break;
+      }
+      res_fresh = false;
+      if (out_res) {
+        if ((status = true_residual(st))) break;
+        res_fresh = true;
       }
     }
     if (status == IGA_OK) last_iters = host.iter;
     if (out_iters) *out_iters = host.iter;
     if (out_rr) *out_rr = host.rr;
     if (out_bb) *out_bb = host.bb;
+    if (out_res) *out_res = res_fresh && status == IGA_OK ? sqrt(host.res2) : -1.0;
     return status;
   }
 
@@ -1198,12 +2204,11 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   P->x = x;
   P->F.st = reinterpret_cast<FState*>(work);
   P->r = work + 8;
-  P->z = P->r + n;
-  P->P[0] = P->z + n;
-  P->P[1] = P->z + 2 * n;
-  P->Q[0] = P->z + 3 * n;
-  P->Q[1] = P->z + 4 * n;
-  double* parts = P->z + 5 * n;
+  P->P[0] = P->r + n;
+  P->P[1] = P->r + 2 * n;
+  P->Q[0] = P->r + 3 * n;
+  P->Q[1] = P->r + 4 * n;
+  double* parts = P->r + 5 * n;
   P->F.P_a = parts;
   P->F.P_b = parts + kRedBlocks;
   P->F.P_c = parts + 2 * kRedBlocks;
@@ -1215,6 +2220,11 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   P->rb = row_blocks(n, P->lpr);
   P->eb = blocks_for(n);
   P->G = graph_iters > 0 ? graph_iters : 0;
+  {
+    const dim3 tg = tiles_of(mg->level[0]);
+    P->grid64 = grid_cycle_ok(*mg) && mg->grid->op64 && (int)(tg.x * tg.y) <= kRedBlocks;
+    P->nb_spmv = P->grid64 ? (int)(tg.x * tg.y) : P->rb;
+  }
   rc = P->capture((hipStream_t)stream);
   if (rc) {
     delete P;
@@ -1225,9 +2235,10 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
 }
 
 extern "C" int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter,
-                              int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
+                              int32_t* out_iters, double* out_rr, double* out_bb,
+                              double* out_res, void* stream) {
   if (!plan) return IGA_ERR_ARG;
-  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, (hipStream_t)stream);
+  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, out_res, (hipStream_t)stream);
 }
 
 extern "C" void iga_fcg_mg_destroy(iga_fcg_plan_t* plan) { delete plan; }
@@ -1251,7 +2262,7 @@ extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
   int rc = iga_fcg_mg_plan(mg, n, nnz, indptr, cols, vals, b, x, work, work32, graph_iters, stream,
                            &plan);
   if (rc) return rc;
-  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, st);
+  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, nullptr, st);
   delete plan;
   return rc;
 }

@@ -86,6 +86,8 @@ class LevelStructure:
     cols: object
     rows: object
     P: object = None  # host CSR (n x n_coarse)
+    P1: tuple = None  # the 1-D factors of P (direction 0, direction 1), P = P1[1] (x) P1[0]
+    radius: int = 0  # largest grid distance the level operator couples
     p_dev: tuple = None  # (indptr, cols, vals64, vals32)
     pt_dev: tuple = None
     t_pat: tuple = None  # pattern of P^T K (n_coarse x n): (indptr, cols, rows)
@@ -151,6 +153,8 @@ class Hierarchy:
             if max(ext_r.max(), ext_c.max()) + 1 + 2 * radius > 16 or radius > p:
                 raise NotImplementedError("coarse-row stencil exceeds the Galerkin kernel's box")
             fine.P = P
+            fine.P1 = (P1[0], P1[1])
+            fine.radius = int(radius)
             fine.p_dev = self._transfer_dev(P, dev)
             fine.pt_dev = self._transfer_dev(PT, dev)
             fine.t_pat = _csr_dev(Tpat, dev)
@@ -158,7 +162,12 @@ class Hierarchy:
             ip, cc, rr = _csr_dev(Kc, dev)
             levels.append(LevelStructure(Kc.shape[0], hc, wc, ck, ip, cc, rr))
             K = Kc
+        Kcoo = K.tocoo()
+        w_last = levels[-1].w
+        levels[-1].radius = int(max(np.abs(Kcoo.col // w_last - Kcoo.row // w_last).max(initial=0),
+                                    np.abs(Kcoo.col % w_last - Kcoo.row % w_last).max(initial=0)))
         self.levels = levels
+        self._bands = {}
 
     @staticmethod
     def _transfer_dev(M, dev):
@@ -172,11 +181,98 @@ class Hierarchy:
     def n_levels(self):
         return len(self.levels)
 
+    def grid_bands(self, tail_level):
+        """Device band tables of the transfers of levels ``0 .. tail_level-1``
+        for the grid V-cycle, or None when a level's operator reaches beyond
+        three grid points or a tile's transfer footprint exceeds the tile
+        kernels' staging (see :func:`band_tables`)."""
+        import torch
+
+        if tail_level in self._bands:
+            return self._bands[tail_level]
+        out = None
+        if 2 <= tail_level < self.n_levels and all(lv.radius <= GRID_RADIUS for lv in self.levels[:tail_level]):
+            dev = self.levels[0].indptr.device
+            bands = []
+            for l in range(tail_level):
+                per_dir = []
+                for d in range(2):
+                    tb = band_tables(self.levels[l].P1[d])
+                    if tb is None or not band_spans_fit(*tb):
+                        per_dir = None
+                        break
+                    per_dir.append(tuple(torch.as_tensor(a, device=dev) for a in tb))
+                if per_dir is None:
+                    bands = None
+                    break
+                bands.append(per_dir)
+            out = bands
+        self._bands[tail_level] = out
+        return out
+
+
+# the grid V-cycle's tile kernels: operator radius, band widths (coarse values
+# per fine row and fine values per coarse column of a 1-D transfer), and the
+# fine / coarse extents a tile stages (16-point tiles with a 3-point halo;
+# 6 points for the level-0 post-smoother)
+GRID_RADIUS = 3
+KPF, KFC = 4, 6
+FINE_SPAN, COARSE_SPAN = 56, 24
+
+
+def band_tables(T):
+    """Band form of a 1-D transfer ``T`` (fine x coarse): ``c_lo[f]`` and
+    ``pw[f, k] = T[f, c_lo[f] + k]`` (k < KPF) per fine row, ``f_lo[c]`` and
+    ``rw[c, k] = T[f_lo[c] + k, c]`` (k < KFC) per coarse column; first
+    indices non-decreasing. None if a row or column is wider than its band."""
+    T = np.asarray(T, float)
+    nf, nc = T.shape
+    c_lo = np.zeros(nf, np.int32)
+    pw = np.zeros((nf, KPF), np.float32)
+    for f in range(nf):
+        nz = np.flatnonzero(T[f])
+        if len(nz):
+            if nz[-1] - nz[0] >= KPF:
+                return None
+            c_lo[f] = nz[0]
+            pw[f, : nz[-1] - nz[0] + 1] = T[f, nz[0]: nz[-1] + 1]
+    f_lo = np.zeros(nc, np.int32)
+    rw = np.zeros((nc, KFC), np.float32)
+    for c in range(nc):
+        nz = np.flatnonzero(T[:, c])
+        if len(nz):
+            if nz[-1] - nz[0] >= KFC:
+                return None
+            f_lo[c] = nz[0]
+            rw[c, : nz[-1] - nz[0] + 1] = T[nz[0]: nz[-1] + 1, c]
+    if np.any(np.diff(c_lo) < 0) or np.any(np.diff(f_lo) < 0):
+        return None
+    return c_lo, pw, f_lo, rw
+
+
+def band_spans_fit(c_lo, pw, f_lo, rw, tile=16, halo=3):
+    """Whether every tile's transfer footprint fits the tile kernels' staging:
+    the fine points a coarse tile (with halo) restricts from, and the coarse
+    points a fine tile (halo 3, and 6 on level 0) prolongs from."""
+    nf, nc = len(c_lo), len(f_lo)
+    for c0 in range(0, nc, tile):
+        a, z = max(c0 - halo, 0), min(c0 + tile + halo - 1, nc - 1)
+        if f_lo[z] + KFC - f_lo[a] > FINE_SPAN:
+            return False
+    for h in (halo, 2 * halo):
+        for f0 in range(0, nf, tile):
+            a, z = max(f0 - h, 0), min(f0 + tile + h - 1, nf - 1)
+            if c_lo[z] + KPF - c_lo[a] > COARSE_SPAN:
+                return False
+    return True
+
 
 _FREE_KEYS = weakref.WeakKeyDictionary()
 
-# levels at or below this size take the dense tail (a 1089-dof level: 4.7 MB
-# FP32, one product per cycle instead of about a dozen launches)
+# levels at or below this size take the dense tail (a 289-dof level: 0.3 MB
+# FP32 with padded rows, one product per cycle instead of about a dozen
+# launches; formed by batched cycles, whose cost grows with the square of the
+# size); the dense product takes at most 2048 rows
 TAIL_ROWS = 300
 
 
@@ -273,10 +369,30 @@ class MGOperator:
                                 if hier.levels[l].n <= TAIL_ROWS), -1)
         if self.tail_level > 0:
             nt = hier.levels[self.tail_level].n
-            self.tail = torch.empty(nt * nt, **f32)
+            self.tail = torch.empty(nt * int(lib().iga_mg_dense_ld(nt)), **f32)
             self._tail_work = None
         else:
             self.tail = None
+        # grid form for the tile-kernel V-cycle: level operators above the
+        # tail in stencil layout, transfers as 1-D bands
+        self._bands = hier.grid_bands(self.tail_level) if self.tail_level > 0 else None
+        self.grid_ops = None
+        self._grid = None
+        if self._bands is not None:
+            nst = (2 * GRID_RADIUS + 1) ** 2
+            self.grid_ops = [torch.empty(nst * hier.levels[l].n, **f32) for l in range(self.tail_level)]
+            self.grid_op64 = torch.empty(nst * n0, **f64)  # the flexible-CG operator
+            g = _lib.IgaMgGrid()
+            g.n_levels = self.tail_level
+            g.radius = GRID_RADIUS
+            g.op64 = self.grid_op64.data_ptr()
+            for l, (op, bands) in enumerate(zip(self.grid_ops, self._bands)):
+                g.op[l] = op.data_ptr()
+                for d in range(2):
+                    c_lo, pw, f_lo, rw = bands[d]
+                    B = g.band[l][d]
+                    B.c_lo, B.pw, B.f_lo, B.rw = c_lo.data_ptr(), pw.data_ptr(), f_lo.data_ptr(), rw.data_ptr()
+            self._grid = g
 
     def _setup_struct(self):
         if getattr(self, "_setup", None) is None:
@@ -306,7 +422,17 @@ class MGOperator:
         _lib.check(lib().iga_mg_load(self._setup_struct(), _lib.ptr(vals64), stream()), "iga_mg_load")
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
         self._form_tail()
+        if self._grid is not None:
+            _lib.check(lib().iga_mg_grid_ops(self.struct(), _lib.ptr(self.vals64[0]), stream()),
+                       "iga_mg_grid_ops")
         return self
+
+    def grid_active(self):
+        """Whether the V-cycle from level 0 runs as the grid tile kernels: a
+        grid form exists, the dense tail applies and only level 0 may carry a
+        (7 x 7) stencil."""
+        return (self._grid is not None and self.tail_active() and self.ks == 2 * GRID_RADIUS + 1
+                and all(s is None for s in self.stencils[1:]))
 
     def _form_tail(self):
         import torch
@@ -371,6 +497,8 @@ class MGOperator:
         if self.tail_active():
             m.tail_level = self.tail_level
             m.tail = self.tail.data_ptr()
+        if self._grid is not None:
+            m.grid = _lib.ctypes.pointer(self._grid)
         if self._work is None:
             nw = int(lib().iga_mg_work_floats(m))
             self._work = torch.empty(max(nw, 1), dtype=torch.float32, device=self.b.device)
@@ -401,22 +529,27 @@ class MGOperator:
             self._plans[key] = plan
         return plan
 
-    def fcg(self, b, tol=1e-10, maxiter=1000, graph_iters=8):
+    def fcg(self, b, tol=1e-10, maxiter=1000, graph_iters=8, true_residual=False):
         """Flexible CG on the level-0 FP64 operator with one V-cycle per
-        iteration; returns ``(x, iterations, rr, bb)`` (x a new tensor)."""
+        iteration; returns ``(x, iterations, rr, bb)`` (x a new tensor), and
+        ``||b - K x||`` recomputed in FP64 from x as a fifth item when
+        ``true_residual``."""
         import torch
 
-        it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
+        it, rr, bb, res = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0), _lib.c_double(-1.0)
         side = side_stream(self.b.device, "fcg") if graph_iters > 0 else torch.cuda.current_stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):  # graph capture needs a non-default stream
             self.b.copy_(b)
             plan = self._plan(graph_iters)
             _lib.check(lib().iga_fcg_mg_run(plan.handle, float(tol), int(maxiter), _lib.ctypes.byref(it),
-                                            _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()),
+                                            _lib.ctypes.byref(rr), _lib.ctypes.byref(bb),
+                                            _lib.ctypes.byref(res) if true_residual else None, stream()),
                        "iga_fcg_mg_run")
             x = self.x.clone()
         torch.cuda.current_stream().wait_stream(side)
+        if true_residual:
+            return x, it.value, rr.value, bb.value, res.value
         return x, it.value, rr.value, bb.value
 
 

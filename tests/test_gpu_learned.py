@@ -156,6 +156,27 @@ def test_vcycle_matches_oracle(gpu, refine):
         assert _rel(x, xr) < 1e-4
 
 
+@pytest.mark.parametrize("refine", [7, 8])
+def test_grid_vcycle_matches_oracle(gpu, refine):
+    # the tile-kernel cycle (operators in stencil layout, fused transfers):
+    # a stencil on level 0 only, damped Jacobi below, dense tail at 289 dofs
+    ctx, vals, rhs = _problem(refine, 12)
+    ref = _oracle_levels(ctx, vals)
+    rng = np.random.default_rng(13)
+    s0 = rng.normal(size=(49, ref[0]["K"].shape[0])) * 0.05
+    s0[24] += 0.5
+    b = rng.normal(size=ref[0]["K"].shape[0])
+    for st in (None, [s0] + [None] * (len(ref) - 2)):
+        op = LN.setup(ctx, vals)
+        if st is not None:
+            op.set_stencils([torch.as_tensor(st[0], dtype=torch.float32, device="cuda")])
+        assert op.grid_active()
+        x = op.vcycle(torch.as_tensor(b, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+        xr = OM.vcycle(ref, st, b)
+        print(f"[grid] refine {refine} stencil {st is not None}: rel {_rel(x, xr):.2e}", flush=True)
+        assert _rel(x, xr) < 1e-4
+
+
 def test_multigrid_fcg_converges_and_matches_jacobi_cg(gpu):
     ctx, vals, rhs = _problem(6, 11)
     op = LN.setup(ctx, vals)
