@@ -687,16 +687,20 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------------------
 // Grid V-cycle: tile kernels over the level grids (operators in stencil
-// layout, transfers as 1-D bands). Each kernel computes a 16 x 16 tile of its
-// output grid and recomputes what it needs of the previous step on the tile's
-// halo in LDS, so every transfer runs fused with the smoother beside it:
-//   k_g0_pre        level 0: x0 = S b0 (learned stencil or damped Jacobi), r0 = b0 - A0 x0
+// layout, transfers as 1-D bands). On the coarse levels each kernel computes
+// a 16 x 16 tile of its output grid and recomputes what it needs of the
+// previous step on the tile's halo in LDS, so every transfer runs fused with
+// the smoother beside it; on level 0 (row tiles, below) each big array is
+// read once per point:
+//   k_g0_smooth     level 0: x0 = S b0 (learned stencil or damped Jacobi)
+//   k_g0_resid      level 0: r0 = b0 - A0 x0
 //   k_g_restrict    level l+1: b = P^T r_l, x = w D^-1 b, r = b - A x
 //   k_g_tail        level t: x_t = M P^T r_{t-1} (dense tail)
 //   k_g_prolong     level l: x += P x_{l+1}, x += w D^-1 (b - A x)
-//   k_g0_post       level 0: x0 += P x1, r = b0 - A0 x0, z = x0 + S^T r (+ z . q)
+//   k_g0_prolong    level 0: x = x0 + P x1, u = D^-1/2 (b0 - A0 x)
+//   k_g0_smooth_t   level 0: z = x + D^-1/2 T_W^T u (+ z . q)
 // Values off the grid are zero. A level's radius-3 operator reaches a 22 x 22
-// halo of the tile; the level-0 kernels reach 28 x 28.
+// halo of a 16 x 16 tile and a 70 x 10 box around a 64 x 4 row tile.
 // ---------------------------------------------------------------------------
 
 constexpr int kGR = 3;                     // operator and smoother radius
@@ -795,75 +799,6 @@ __device__ __forceinline__ void stage_coarse(float (*cs)[kCoarseSpan], const flo
     const int t = threadIdx.x + k * kGTh;
     if (t < N) cs[t / kCoarseSpan][t % kCoarseSpan] = v[k];
   }
-}
-
-// Every global load of a block (staging values, the stencil values of its
-// halo and tile points) is issued at kernel entry, before the gate's own
-// reduction, so the block waits on memory once.
-template <bool STW>
-__global__ void __launch_bounds__(kGTh)
-    k_g0_pre(GLevel g, const float* __restrict__ W, const float* __restrict__ b,
-             float* __restrict__ x, float* __restrict__ r, Gate gate) {
-  __shared__ float ub[kGH2][kGH2];  // D^-1/2 b on the tile +- 6
-  __shared__ float xs[kGH1][kGH1];  // x on the tile +- 3
-  constexpr int NS = kGH2 * kGH2, NSP = passes(NS);
-  constexpr int NH = kGH1 * kGH1, NHP = passes(NH);
-  const int n = g.h * g.w;
-  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
-  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
-  const int rt = r0 + ty, ct = c0 + tx;
-  const bool in_t = on_grid(rt, ct, g.h, g.w);
-  const int it = in_t ? rt * g.w + ct : 0;
-  float sv[NSP], sd[NSP];
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int rr = r0 - 2 * kGR + t / kGH2, cc = c0 - 2 * kGR + t % kGH2;
-    const int i = t < NS && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0;
-    sv[k] = b[i];
-    sd[k] = g.dsq[i];
-  }
-  float wh[NHP][kGN], dh[NHP];
-#pragma unroll
-  for (int k = 0; k < NHP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int rr = r0 - kGR + min(t / kGH1, kGH1 - 1), cc = c0 - kGR + t % kGH1;
-    const int i = t < NH && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0;
-    dh[k] = g.dsq[i];
-    if (STW)
-#pragma unroll
-      for (int o = 0; o < kGN; ++o) wh[k][o] = W[(size_t)o * n + i];
-  }
-  float a[kGN];
-#pragma unroll
-  for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + it];
-  const float bt = b[it];
-  __builtin_amdgcn_sched_barrier(0);
-  if (gate_closed(gate)) return;
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int rr = r0 - 2 * kGR + t / kGH2, cc = c0 - 2 * kGR + t % kGH2;
-    if (t < NS) ub[t / kGH2][t % kGH2] = on_grid(rr, cc, g.h, g.w) ? sd[k] * sv[k] : 0.f;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NHP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
-    const bool in = t < NH && on_grid(r0 - kGR + ly, c0 - kGR + lx, g.h, g.w);
-    float v;
-    if (STW)
-      v = dh[k] * tap_sum<kGH2>(wh[k], ub, ly + kGR, lx + kGR);
-    else
-      v = g.omega * dh[k] * ub[ly + kGR][lx + kGR];
-    if (t < NH) xs[ly][lx] = in ? v : 0.f;
-  }
-  __syncthreads();
-  const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
-  if (!in_t) return;
-  x[it] = xs[ty + kGR][tx + kGR];
-  r[it] = bt - ax;
 }
 
 // coarse tile of level l+1 from the fine residual r_f of level l
@@ -1025,137 +960,248 @@ __global__ void __launch_bounds__(kGTh)
   xo[i] = xs[ty + kGR][tx + kGR] + g.omega * d * d * (bi - ax);
 }
 
-// level-0 tile: x = x0 + P x1 on +-6, r = b0 - A0 x on +-3, z = x + S^T r on
-// the tile (S^T = D^-1/2 T_W^T D^-1/2, or w D^-1 without a stencil), and the
-// partial of z . dot.y per tile. The loads that need no other load (x0 on
-// the halo, the stencil values and right-hand side of every point, the band
-// rows of the tile's window) go out at entry; the coarse staging follows.
-template <bool STW>
-__global__ void __launch_bounds__(kGTh)
-    k_g0_post(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ W,
-              const float* __restrict__ xc, const float* __restrict__ x0,
-              const float* __restrict__ b0, float* __restrict__ z, const int* __restrict__ skip,
-              DotOut dot) {
-  if (skipped(skip)) return;
-  __shared__ float cs[kCoarseSpan][kCoarseSpan];
-  __shared__ float xs[kGH2][kGH2];
-  __shared__ float us[kGH1][kGH1];  // D^-1/2 r
-  __shared__ int clo[2][kGH2];      // band rows of the window: c_lo, weights
-  __shared__ float pwt[2][kGH2][kKPF];
-  constexpr int NX = kGH2 * kGH2, NXP = passes(NX);
-  constexpr int NH = kGH1 * kGH1, NHP = passes(NH);
-  const int n = g.h * g.w;
-  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
-  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
-  const int rt = r0 + ty, ct = c0 + tx;
-  const bool in_t = on_grid(rt, ct, g.h, g.w);
-  const int it = in_t ? rt * g.w + ct : 0;
-  // band rows of the 28-point window in each direction (threads 0..55)
-  int bl = 0;
-  float bw[kKPF] = {0.f, 0.f, 0.f, 0.f};
-  {
-    const int d = threadIdx.x / kGH2, m = threadIdx.x % kGH2;
-    const int pos = (d == 0 ? r0 : c0) - 2 * kGR + m;
-    const int lim = d == 0 ? g.h : g.w;
-    const GBand& B = d == 0 ? br : bc;
-    const int q = min(max(pos, 0), lim - 1);
-    if (threadIdx.x < 2 * kGH2) {
-      bl = B.c_lo[q];
+// ---------------------------------------------------------------------------
+// Level-0 kernels of the grid cycle: row tiles of 64 x 4 points (one
+// wavefront per 64-point row segment, so every load of a stencil array is
+// 256 contiguous bytes), each big array (learned weights W, operator A) read
+// once per point; only vectors are staged on the 70 x 10 halo box.
+// ---------------------------------------------------------------------------
+
+constexpr int kL0W = 64, kL0H = 4;
+constexpr int kL0HW = kL0W + 2 * kGR, kL0HH = kL0H + 2 * kGR;  // 70 x 10
+constexpr int kL0Box = kL0HW * kL0HH;
+constexpr int kL0CW = 44, kL0CH = 12;  // coarse points staged by the level-0 prolongation
+
+struct L0Tile {
+  int r0, c0, ty, tx, rt, ct, it;
+  bool in;
+};
+
+__device__ __forceinline__ L0Tile l0_tile(int h, int w) {
+  L0Tile t;
+  t.r0 = blockIdx.y * kL0H;
+  t.c0 = blockIdx.x * kL0W;
+  t.ty = threadIdx.x / kL0W;
+  t.tx = threadIdx.x % kL0W;
+  t.rt = t.r0 + t.ty;
+  t.ct = t.c0 + t.tx;
+  t.in = on_grid(t.rt, t.ct, h, w);
+  t.it = t.in ? t.rt * w + t.ct : 0;
+  return t;
+}
+
+// halo-box loads of v (times dsq when SCALE) into registers; stored by box_store
+template <bool SCALE, typename T>
+struct BoxLoad {
+  static constexpr int NP = passes(kL0Box);
+  T v[NP];
+  float d[NP];
+  bool in[NP];
+  __device__ __forceinline__ void load(const T* __restrict__ src, const float* __restrict__ dsq,
+                                       int h, int w, int r0, int c0) {
 #pragma unroll
-      for (int a2 = 0; a2 < kKPF; ++a2) bw[a2] = B.pw[q * kKPF + a2];
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      const int rr = r0 - kGR + t / kL0HW, cc = c0 - kGR + t % kL0HW;
+      in[k] = t < kL0Box && on_grid(rr, cc, h, w);
+      const int i = in[k] ? rr * w + cc : 0;
+      v[k] = src[i];
+      if (SCALE) d[k] = dsq[i];
     }
   }
-  float xv[NXP];
+  __device__ __forceinline__ void store(T (*box)[kL0HW]) const {
 #pragma unroll
-  for (int k = 0; k < NXP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int rr = r0 - 2 * kGR + t / kGH2, cc = c0 - 2 * kGR + t % kGH2;
-    xv[k] = x0[t < NX && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0];
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      if (t < kL0Box) box[t / kL0HW][t % kL0HW] = in[k] ? (SCALE ? (T)d[k] * v[k] : v[k]) : (T)0;
+    }
   }
-  float ah[NHP][kGN], bh[NHP], dh[NHP];
+};
+
+// x0 = S b0: D^-1/2 T_W (D^-1/2 b0), or w D^-1 b0 without a stencil; the
+// first kernel of a cycle (gated)
+template <bool STW>
+__global__ void __launch_bounds__(kGTh)
+    k_g0_smooth(GLevel g, const float* __restrict__ W, const float* __restrict__ b,
+                float* __restrict__ x, Gate gate) {
+  __shared__ float ub[kL0HH][kL0HW];
+  const int n = g.h * g.w;
+  const L0Tile T = l0_tile(g.h, g.w);
+  float a[kGN];
+  if (STW)
 #pragma unroll
-  for (int k = 0; k < NHP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int rr = r0 - kGR + min(t / kGH1, kGH1 - 1), cc = c0 - kGR + t % kGH1;
-    const int i = t < NH && on_grid(rr, cc, g.h, g.w) ? rr * g.w + cc : 0;
-    bh[k] = b0[i];
-    dh[k] = g.dsq[i];
-#pragma unroll
-    for (int o = 0; o < kGN; ++o) ah[k][o] = g.A[(size_t)o * n + i];
+    for (int o = 0; o < kGN; ++o) a[o] = W[(size_t)o * n + T.it];
+  const float dt = g.dsq[T.it], bt = b[T.it];
+  BoxLoad<true, float> bl;
+  if (STW) bl.load(b, g.dsq, g.h, g.w, T.r0, T.c0);
+  __builtin_amdgcn_sched_barrier(0);
+  if (gate_closed(gate)) return;
+  float v;
+  if (STW) {
+    bl.store(ub);
+    __syncthreads();
+    v = dt * tap_sum<kL0HW>(a, ub, T.ty + kGR, T.tx + kGR);
+  } else {
+    v = g.omega * dt * dt * bt;
   }
-  float wt[kGN];
+  if (T.in) x[T.it] = v;
+}
+
+// r0 = b0 - A0 x0
+__global__ void __launch_bounds__(kGTh)
+    k_g0_resid(GLevel g, const float* __restrict__ x, const float* __restrict__ b,
+               float* __restrict__ r, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  __shared__ float xb[kL0HH][kL0HW];
+  const int n = g.h * g.w;
+  const L0Tile T = l0_tile(g.h, g.w);
+  float a[kGN];
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + T.it];
+  const float bt = b[T.it];
+  BoxLoad<false, float> bl;
+  bl.load(x, nullptr, g.h, g.w, T.r0, T.c0);
+  __builtin_amdgcn_sched_barrier(0);
+  bl.store(xb);
+  __syncthreads();
+  const float ax = tap_sum<kL0HW>(a, xb, T.ty + kGR, T.tx + kGR);
+  if (T.in) r[T.it] = bt - ax;
+}
+
+// x = x0 + P x1 on the halo box (prolongation from the staged coarse values
+// and the band rows of the box), then u = D^-1/2 (b0 - A0 x) and x on the tile
+__global__ void __launch_bounds__(kGTh)
+    k_g0_prolong(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ xc,
+                 const float* __restrict__ x0, const float* __restrict__ b0,
+                 float* __restrict__ xo, float* __restrict__ uo, const int* __restrict__ skip) {
+  if (skipped(skip)) return;
+  __shared__ float cs[kL0CH][kL0CW];
+  __shared__ float xb[kL0HH][kL0HW];
+  __shared__ int rlo[kL0HH], clo[kL0HW];
+  __shared__ float rpw[kL0HH][kKPF], cpw[kL0HW][kKPF];
+  const int n = g.h * g.w;
+  const L0Tile T = l0_tile(g.h, g.w);
+  // band rows of the box's rows (threads 0..9) and columns (threads 64..133)
+  int bl = 0;
+  float bw[kKPF] = {0.f, 0.f, 0.f, 0.f};
+  const bool is_row = threadIdx.x < kL0HH;
+  const bool is_col = threadIdx.x >= 64 && threadIdx.x < 64 + kL0HW;
+  if (is_row || is_col) {
+    const int q = is_row ? min(max(T.r0 - kGR + (int)threadIdx.x, 0), g.h - 1)
+                         : min(max(T.c0 - kGR + (int)threadIdx.x - 64, 0), g.w - 1);
+    const GBand& B = is_row ? br : bc;
+    bl = B.c_lo[q];
+#pragma unroll
+    for (int k = 0; k < kKPF; ++k) bw[k] = B.pw[q * kKPF + k];
+  }
+  float a[kGN];
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + T.it];
+  const float bt = b0[T.it], dt = g.dsq[T.it];
+  BoxLoad<false, float> xl;
+  xl.load(x0, nullptr, g.h, g.w, T.r0, T.c0);
+  const int cr0 = br.c_lo[max(T.r0 - kGR, 0)], cc0 = bc.c_lo[max(T.c0 - kGR, 0)];
+  __builtin_amdgcn_sched_barrier(0);
+  {
+    constexpr int N = kL0CH * kL0CW, NP = passes(N);
+    float v[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      const int rr = cr0 + t / kL0CW, cc = cc0 + t % kL0CW;
+      const bool in = t < N && on_grid(rr, cc, hc, wc);
+      v[k] = xc[in ? rr * wc + cc : 0];
+      v[k] = in ? v[k] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      if (t < N) cs[t / kL0CW][t % kL0CW] = v[k];
+    }
+  }
+  if (is_row) {
+    rlo[threadIdx.x] = bl;
+#pragma unroll
+    for (int k = 0; k < kKPF; ++k) rpw[threadIdx.x][k] = bw[k];
+  }
+  if (is_col) {
+    clo[threadIdx.x - 64] = bl;
+#pragma unroll
+    for (int k = 0; k < kKPF; ++k) cpw[threadIdx.x - 64][k] = bw[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < BoxLoad<false, float>::NP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kL0HW, kL0HH - 1), lx = t % kL0HW;
+    const int ka = rlo[ly] - cr0, kb = clo[lx] - cc0;
+    float v = 0.f;
+#pragma unroll
+    for (int p = 0; p < kKPF; ++p) {
+      const int ia = min(max(ka + p, 0), kL0CH - 1);
+      float s2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < kKPF; ++q) s2 += cpw[lx][q] * cs[ia][min(max(kb + q, 0), kL0CW - 1)];
+      v += rpw[ly][p] * s2;
+    }
+    if (t < kL0Box) xb[ly][lx] = xl.in[k] ? xl.v[k] + v : 0.f;
+  }
+  __syncthreads();
+  const float ax = tap_sum<kL0HW>(a, xb, T.ty + kGR, T.tx + kGR);
+  if (!T.in) return;
+  xo[T.it] = xb[T.ty + kGR][T.tx + kGR];
+  uo[T.it] = dt * (bt - ax);
+}
+
+// z = x + D^-1/2 T_W^T u (or x + w D^-1/2 u), u = D^-1/2 r, with the partial
+// of z . dot.y per tile; the transposed stencil reads the neighbours' weights
+template <bool STW>
+__global__ void __launch_bounds__(kGTh)
+    k_g0_smooth_t(GLevel g, const float* __restrict__ W, const float* __restrict__ x,
+                  const float* __restrict__ u, float* __restrict__ z,
+                  const int* __restrict__ skip, DotOut dot) {
+  if (skipped(skip)) return;
+  __shared__ float ubx[kL0HH][kL0HW];
+  const int n = g.h * g.w;
+  const L0Tile T = l0_tile(g.h, g.w);
+  float a[kGN];
   if (STW) {
 #pragma unroll
     for (int dy = -kGR; dy <= kGR; ++dy)
 #pragma unroll
       for (int dx = -kGR; dx <= kGR; ++dx) {
         const int o = (dy + kGR) * kGS + dx + kGR;
-        const bool in = on_grid(rt - dy, ct - dx, g.h, g.w);
-        wt[o] = W[(size_t)o * n + (in ? (rt - dy) * g.w + ct - dx : it)];
+        const bool in = on_grid(T.rt - dy, T.ct - dx, g.h, g.w);
+        a[o] = W[(size_t)o * n + (in ? (T.rt - dy) * g.w + T.ct - dx : T.it)];
+        a[o] = in ? a[o] : 0.f;
       }
   }
-  const float dt = g.dsq[it];
-  const double yt = dot.parts ? dot.y[it] : 0.0;
-  const int cr0 = br.c_lo[max(r0 - 2 * kGR, 0)], cc0 = bc.c_lo[max(c0 - 2 * kGR, 0)];
+  const float xt = x[T.it], dt = g.dsq[T.it], ut = u[T.it];
+  const double yt = dot.parts ? dot.y[T.it] : 0.0;
+  BoxLoad<false, float> bl;
+  if (STW) bl.load(u, nullptr, g.h, g.w, T.r0, T.c0);
   __builtin_amdgcn_sched_barrier(0);
-  stage_coarse(cs, xc, hc, wc, cr0, cc0);
-  if (threadIdx.x < 2 * kGH2) {
-    const int d = threadIdx.x / kGH2, m = threadIdx.x % kGH2;
-    clo[d][m] = bl;
-#pragma unroll
-    for (int a2 = 0; a2 < kKPF; ++a2) pwt[d][m][a2] = bw[a2];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NXP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int ly = min(t / kGH2, kGH2 - 1), lx = t % kGH2;
-    const bool in = t < NX && on_grid(r0 - 2 * kGR + ly, c0 - 2 * kGR + lx, g.h, g.w);
-    const int ka = clo[0][ly] - cr0, kb = clo[1][lx] - cc0;
-    float v = 0.f;
-#pragma unroll
-    for (int a2 = 0; a2 < kKPF; ++a2) {
-      const int ia = min(max(ka + a2, 0), kCoarseSpan - 1);
-      float s2 = 0.f;
-#pragma unroll
-      for (int b2 = 0; b2 < kKPF; ++b2)
-        s2 += pwt[1][lx][b2] * cs[ia][min(max(kb + b2, 0), kCoarseSpan - 1)];
-      v += pwt[0][ly][a2] * s2;
-    }
-    if (t < NX) xs[ly][lx] = in ? xv[k] + v : 0.f;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NHP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
-    const bool in = t < NH && on_grid(r0 - kGR + ly, c0 - kGR + lx, g.h, g.w);
-    const float v = dh[k] * (bh[k] - tap_sum<kGH2>(ah[k], xs, ly + kGR, lx + kGR));
-    if (t < NH) us[ly][lx] = in ? v : 0.f;
-  }
-  __syncthreads();
   float acc;
   if (STW) {
+    bl.store(ubx);
+    __syncthreads();
     acc = 0.f;
 #pragma unroll
     for (int dy = -kGR; dy <= kGR; ++dy)
 #pragma unroll
-      for (int dx = -kGR; dx <= kGR; ++dx) {
-        const int o = (dy + kGR) * kGS + dx + kGR;
-        const bool in = on_grid(rt - dy, ct - dx, g.h, g.w);
-        acc += (in ? wt[o] : 0.f) * us[ty + kGR - dy][tx + kGR - dx];
-      }
+      for (int dx = -kGR; dx <= kGR; ++dx)
+        acc += a[(dy + kGR) * kGS + dx + kGR] * ubx[T.ty + kGR - dy][T.tx + kGR - dx];
   } else {
-    acc = g.omega * us[ty + kGR][tx + kGR];
+    acc = g.omega * ut;
   }
-  const float zi = xs[ty + 2 * kGR][tx + 2 * kGR] + dt * acc;
-  if (in_t) z[it] = zi;
+  const float zi = xt + dt * acc;
+  if (T.in) z[T.it] = zi;
   if (dot.parts)
-    store_partial_at(dot.parts, blockIdx.y * gridDim.x + blockIdx.x, in_t ? (double)zi * yt : 0.0);
+    store_partial_at(dot.parts, blockIdx.y * gridDim.x + blockIdx.x, T.in ? (double)zi * yt : 0.0);
 }
 
 // FP64 flexible-CG product on level 0 in stencil layout: beta from the
-// z . q_prev partials, p = z + beta p_prev on the tile +- 3 (LDS), q = A p on
+// z . q_prev partials, p = z + beta p_prev on the halo box (LDS), q = A p on
 // the tile; partials of p . q and p . r per tile
 __global__ void __launch_bounds__(kGTh)
     k_fcg_spmv_grid(int h, int w, int nblk_a, const double* __restrict__ A,
@@ -1163,48 +1209,35 @@ __global__ void __launch_bounds__(kGTh)
                     const double* __restrict__ r, double* __restrict__ p, double* __restrict__ q,
                     FState* st, const double* __restrict__ P_a, double* __restrict__ P_b,
                     double* __restrict__ P_c) {
-  __shared__ double ps[kGH1][kGH1];
-  constexpr int NH = kGH1 * kGH1, NHP = passes(NH);
+  __shared__ double pb[kL0HH][kL0HW];
   const int n = h * w;
-  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
-  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
-  const int rt = r0 + ty, ct = c0 + tx;
-  const bool in_t = on_grid(rt, ct, h, w);
-  const int it = in_t ? rt * w + ct : 0;
-  float zh[NHP];
-  double ph[NHP];
-#pragma unroll
-  for (int k = 0; k < NHP; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int rr = r0 - kGR + min(t / kGH1, kGH1 - 1), cc = c0 - kGR + t % kGH1;
-    const int i = t < NH && on_grid(rr, cc, h, w) ? rr * w + cc : 0;
-    zh[k] = z32[i];
-    ph[k] = pp[i];
-  }
+  const L0Tile T = l0_tile(h, w);
   double a[kGN];
 #pragma unroll
-  for (int o = 0; o < kGN; ++o) a[o] = A[(size_t)o * n + it];
-  const double ri = r[it];
+  for (int o = 0; o < kGN; ++o) a[o] = A[(size_t)o * n + T.it];
+  const double ri = r[T.it];
+  BoxLoad<false, float> zl;
+  BoxLoad<false, double> pl;
+  zl.load(z32, nullptr, h, w, T.r0, T.c0);
+  pl.load(pp, nullptr, h, w, T.r0, T.c0);
   __builtin_amdgcn_sched_barrier(0);
   if (st->done) return;
   const double beta = -reduce_partials(P_a, nblk_a) / st->pq_prev;
 #pragma unroll
-  for (int k = 0; k < NHP; ++k) {
+  for (int k = 0; k < BoxLoad<false, float>::NP; ++k) {
     const int t = threadIdx.x + k * kGTh;
-    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
-    const bool in = t < NH && on_grid(r0 - kGR + ly, c0 - kGR + lx, h, w);
-    if (t < NH) ps[ly][lx] = in ? (double)zh[k] + beta * ph[k] : 0.0;
+    if (t < kL0Box) pb[t / kL0HW][t % kL0HW] = zl.in[k] ? (double)zl.v[k] + beta * pl.v[k] : 0.0;
   }
   __syncthreads();
-  const double qi = tap_sum<kGH1>(a, ps, ty + kGR, tx + kGR);
-  const double pi = ps[ty + kGR][tx + kGR];
-  if (in_t) {
-    p[it] = pi;
-    q[it] = qi;
+  const double qi = tap_sum<kL0HW>(a, pb, T.ty + kGR, T.tx + kGR);
+  const double pi = pb[T.ty + kGR][T.tx + kGR];
+  if (T.in) {
+    p[T.it] = pi;
+    q[T.it] = qi;
   }
   const int blk = blockIdx.y * gridDim.x + blockIdx.x;
-  store_partial_at(P_c, blk, in_t ? pi * qi : 0.0);
-  store_partial_at(P_b, blk, in_t ? pi * ri : 0.0);
+  store_partial_at(P_c, blk, T.in ? pi * qi : 0.0);
+  store_partial_at(P_b, blk, T.in ? pi * ri : 0.0);
 }
 
 // stencil layout of the level operators (blockIdx.y = level; the FP64 copy of
@@ -1332,6 +1365,10 @@ static inline dim3 tiles_of(const iga_mg_level_t& L) {
   return dim3((L.w + kGT - 1) / kGT, (L.h + kGT - 1) / kGT);
 }
 
+static inline dim3 row_tiles(const iga_mg_level_t& L) {
+  return dim3((L.w + kL0W - 1) / kL0W, (L.h + kL0H - 1) / kL0H);
+}
+
 // Per-level buffers of the grid cycle: x (pre-smoothed), b, r, x (post)
 static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate& gate,
                        const DotOut& dot, int* dot_blocks, hipStream_t st) {
@@ -1351,12 +1388,12 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
   const int* skip = gate.flag;
   const iga_mg_level_t& L0 = mg.level[0];
   const GLevel g0 = glevel(mg, 0);
+  const dim3 t0 = row_tiles(L0);
   if (L0.stencil)
-    hipLaunchKernelGGL(k_g0_pre<true>, tiles_of(L0), dim3(kGTh), 0, st, g0, L0.stencil, b0, X[0],
-                       Rr[0], gate);
+    hipLaunchKernelGGL(k_g0_smooth<true>, t0, dim3(kGTh), 0, st, g0, L0.stencil, b0, X[0], gate);
   else
-    hipLaunchKernelGGL(k_g0_pre<false>, tiles_of(L0), dim3(kGTh), 0, st, g0, nullptr, b0, X[0],
-                       Rr[0], gate);
+    hipLaunchKernelGGL(k_g0_smooth<false>, t0, dim3(kGTh), 0, st, g0, nullptr, b0, X[0], gate);
+  hipLaunchKernelGGL(k_g0_resid, t0, dim3(kGTh), 0, st, g0, X[0], b0, Rr[0], skip);
   IGA_CHECK_LAUNCH();
   for (int l = 0; l + 1 < t; ++l) {
     const iga_mg_level_t& F = mg.level[l];
@@ -1398,15 +1435,17 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
     IGA_CHECK_LAUNCH();
   }
   const iga_mg_level_t& L1 = mg.level[1];
-  const dim3 tg = tiles_of(L0);
+  hipLaunchKernelGGL(k_g0_prolong, t0, dim3(kGTh), 0, st, g0, L1.h, L1.w, gband(mg, 0, 1),
+                     gband(mg, 0, 0), XO[1], X[0], b0, XO[0], B[0], skip);
+  const dim3 tg = t0;
   const bool fuse = dot.parts && (int)(tg.x * tg.y) <= kRedBlocks;
   const DotOut d = fuse ? dot : DotOut{nullptr, nullptr};
   if (L0.stencil)
-    hipLaunchKernelGGL(k_g0_post<true>, tg, dim3(kGTh), 0, st, g0, L1.h, L1.w, gband(mg, 0, 1),
-                       gband(mg, 0, 0), L0.stencil, XO[1], X[0], b0, z, skip, d);
+    hipLaunchKernelGGL(k_g0_smooth_t<true>, tg, dim3(kGTh), 0, st, g0, L0.stencil, XO[0], B[0], z,
+                       skip, d);
   else
-    hipLaunchKernelGGL(k_g0_post<false>, tg, dim3(kGTh), 0, st, g0, L1.h, L1.w, gband(mg, 0, 1),
-                       gband(mg, 0, 0), nullptr, XO[1], X[0], b0, z, skip, d);
+    hipLaunchKernelGGL(k_g0_smooth_t<false>, tg, dim3(kGTh), 0, st, g0, nullptr, XO[0], B[0], z,
+                       skip, d);
   IGA_CHECK_LAUNCH();
   if (fuse) {
     *dot_blocks = (int)(tg.x * tg.y);
@@ -2026,7 +2065,7 @@ struct iga_fcg_plan {
     if (rc) return rc;
     if (grid64) {
       const iga_mg_level_t& L0 = mg.level[0];
-      hipLaunchKernelGGL(k_fcg_spmv_grid, tiles_of(L0), dim3(kGTh), 0, st, L0.h, L0.w, na,
+      hipLaunchKernelGGL(k_fcg_spmv_grid, row_tiles(L0), dim3(kGTh), 0, st, L0.h, L0.w, na,
                          mg.grid->op64, z32, pp, r, p, q, F.st, F.P_a, F.P_b, F.P_c);
     } else {
       by_lpr(lpr, [&](auto L) {
@@ -2221,7 +2260,7 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   P->eb = blocks_for(n);
   P->G = graph_iters > 0 ? graph_iters : 0;
   {
-    const dim3 tg = tiles_of(mg->level[0]);
+    const dim3 tg = row_tiles(mg->level[0]);
     P->grid64 = grid_cycle_ok(*mg) && mg->grid->op64 && (int)(tg.x * tg.y) <= kRedBlocks;
     P->nb_spmv = P->grid64 ? (int)(tg.x * tg.y) : P->rb;
   }

@@ -198,7 +198,7 @@ class Hierarchy:
                 per_dir = []
                 for d in range(2):
                     tb = band_tables(self.levels[l].P1[d])
-                    if tb is None or not band_spans_fit(*tb):
+                    if tb is None or not band_spans_fit(*tb, level0_dir=d if l == 0 else None):
                         per_dir = None
                         break
                     per_dir.append(tuple(torch.as_tensor(a, device=dev) for a in tb))
@@ -213,11 +213,14 @@ class Hierarchy:
 
 # the grid V-cycle's tile kernels: operator radius, band widths (coarse values
 # per fine row and fine values per coarse column of a 1-D transfer), and the
-# fine / coarse extents a tile stages (16-point tiles with a 3-point halo;
-# 6 points for the level-0 post-smoother)
+# fine / coarse extents a tile stages: 16 x 16 tiles with a 3-point halo on
+# the coarse levels; on level 0 row tiles of 64 (direction 0) x 4
+# (direction 1) points whose prolongation stages 44 x 12 coarse points
 GRID_RADIUS = 3
 KPF, KFC = 4, 6
 FINE_SPAN, COARSE_SPAN = 56, 24
+L0_TILE = (64, 4)
+L0_COARSE_SPAN = (44, 12)
 
 
 def band_tables(T):
@@ -250,20 +253,21 @@ def band_tables(T):
     return c_lo, pw, f_lo, rw
 
 
-def band_spans_fit(c_lo, pw, f_lo, rw, tile=16, halo=3):
+def band_spans_fit(c_lo, pw, f_lo, rw, level0_dir=None, tile=16, halo=3):
     """Whether every tile's transfer footprint fits the tile kernels' staging:
     the fine points a coarse tile (with halo) restricts from, and the coarse
-    points a fine tile (halo 3, and 6 on level 0) prolongs from."""
+    points a fine tile (with halo) prolongs from; ``level0_dir`` (0 or 1) for
+    the transfer out of level 0 along that direction (row tiles)."""
     nf, nc = len(c_lo), len(f_lo)
     for c0 in range(0, nc, tile):
         a, z = max(c0 - halo, 0), min(c0 + tile + halo - 1, nc - 1)
         if f_lo[z] + KFC - f_lo[a] > FINE_SPAN:
             return False
-    for h in (halo, 2 * halo):
-        for f0 in range(0, nf, tile):
-            a, z = max(f0 - h, 0), min(f0 + tile + h - 1, nf - 1)
-            if c_lo[z] + KPF - c_lo[a] > COARSE_SPAN:
-                return False
+    ft, span = (tile, COARSE_SPAN) if level0_dir is None else (L0_TILE[level0_dir], L0_COARSE_SPAN[level0_dir])
+    for f0 in range(0, nf, ft):
+        a, z = max(f0 - halo, 0), min(f0 + ft + halo - 1, nf - 1)
+        if c_lo[z] + KPF - c_lo[a] > span:
+            return False
     return True
 
 
