@@ -284,6 +284,10 @@ int64_t iga_net_scratch_floats(const iga_net_t* net, int32_t batch);
 /* forward: reads the input slot of `act`, writes every layer's slot */
 int iga_net_forward(const iga_net_t* net, int32_t batch, const float* params, float* act,
                     void* stream);
+/* inference: as iga_net_forward, reading the input from `input` and writing
+ * the last layer to `output` (the other layers use their slots of act) */
+int iga_net_infer(const iga_net_t* net, int32_t batch, const float* params, float* act,
+                  const float* input, float* output, void* stream);
 /* backward: `gact` (same layout as act) holds d loss/d output in the output
  * slot on entry; on exit `grads` (parameter layout) and the input slot hold
  * d loss / d parameters (weights and biases) and d loss / d input */
@@ -349,6 +353,7 @@ typedef struct iga_mg_grid {
   float* op[16];
   iga_mg_band_t band[16][2]; /* [l][d]: level l -> l + 1, direction d */
   double* op64; /* optional FP64 stencil layout of level 0 (the flexible-CG operator) */
+  float* feats; /* optional, with op64: level 0's op64[o][i] / op64[centre][i] in FP32 */
 } iga_mg_grid_t;
 
 typedef struct iga_mg {
@@ -367,8 +372,9 @@ typedef struct iga_mg {
 } iga_mg_t;
 
 /* Numeric set-up of a hierarchy for a new level-0 operator: Galerkin levels
- * K_{l+1} = P_l^T K_l P_l (FP64, one wavefront per coarse row; the fine
- * stencil of every coarse row must fit a 16 x 16 box of fine grid points),
+ * K_{l+1} = P_l^T K_l P_l (FP64: tensor passes with setup->grid, else one
+ * wavefront per coarse row, where the fine stencil of every coarse row must
+ * fit a 16 x 16 box of fine grid points),
  * FP32 copies, inverse diagonals (1 where zero) and D^-1/2, and the FP32
  * dense inverse of the coarsest level (n <= 4096). pt: the CSR of P_l^T
  * (coarse x fine) in FP64; radius: the operators' stencil radius. */
@@ -385,12 +391,30 @@ typedef struct iga_mg_setup_level {
   const double* pt_vals;
 } iga_mg_setup_level_t;
 
+/* Optional tensor form of the set-up (every level's free dofs an h x w grid,
+ * operator radius 3, each transfer P_l = P_l^(1) (x) P_l^(0)): the coarse
+ * column bands of the 1-D transfers in FP64 (f_lo[c], rw[c][k] = P[f_lo[c] +
+ * k][c], k < 6; d = 0 along a grid row) and the operators in stencil layout
+ * (as iga_mg_grid_t). Each Galerkin level is then two 1-D passes over FP64
+ * stencils; the second writes the coarse stencil (FP64, and FP32 where op32
+ * is set), its CSR values and scalings. Level 0's stencils (and feats) are
+ * formed from its CSR first. work: 49 max_l(h_l w_{l+1}) doubles. */
+typedef struct iga_mg_setup_grid {
+  const int32_t* f_lo[16][2];
+  const double* rw[16][2];
+  double* op64[16];
+  float* op32[16];
+  float* feats;
+  double* work;
+} iga_mg_setup_grid_t;
+
 typedef struct iga_mg_setup {
   int32_t n_levels;
   int32_t radius;
   iga_mg_setup_level_t level[16];
   float* coarse_inv;
   double* inv_work; /* 2 n_c^2 doubles */
+  const iga_mg_setup_grid_t* grid; /* optional tensor form; null = row-wise Galerkin kernel */
 } iga_mg_setup_t;
 
 int iga_mg_load(const iga_mg_setup_t* setup, const double* vals0, void* stream);

@@ -81,9 +81,15 @@ class IgaMgSetupLevel(ctypes.Structure):
     ]
 
 
+class IgaMgSetupGrid(ctypes.Structure):
+    _fields_ = [("f_lo", (c_void_p * 2) * 16), ("rw", (c_void_p * 2) * 16), ("op64", c_void_p * 16),
+                ("op32", c_void_p * 16), ("feats", c_void_p), ("work", c_void_p)]
+
+
 class IgaMgSetup(ctypes.Structure):
     _fields_ = [("n_levels", c_int), ("radius", c_int), ("level", IgaMgSetupLevel * 16),
-                ("coarse_inv", c_void_p), ("inv_work", c_void_p)]
+                ("coarse_inv", c_void_p), ("inv_work", c_void_p),
+                ("grid", ctypes.POINTER(IgaMgSetupGrid))]
 
 
 class IgaMgBand(ctypes.Structure):
@@ -92,7 +98,7 @@ class IgaMgBand(ctypes.Structure):
 
 class IgaMgGrid(ctypes.Structure):
     _fields_ = [("n_levels", c_int), ("radius", c_int), ("op", c_void_p * 16),
-                ("band", (IgaMgBand * 2) * 16), ("op64", c_void_p)]
+                ("band", (IgaMgBand * 2) * 16), ("op64", c_void_p), ("feats", c_void_p)]
 
 
 class IgaMg(ctypes.Structure):
@@ -144,6 +150,8 @@ _SIGNATURES = {
     "iga_net_act_offset": (c_i64, [ctypes.POINTER(IgaNet), c_int, c_int]),
     "iga_net_scratch_floats": (c_i64, [ctypes.POINTER(IgaNet), c_int]),
     "iga_net_forward": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_net_infer": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p]),
     "iga_net_backward": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),
     "iga_nmse": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

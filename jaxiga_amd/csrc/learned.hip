@@ -130,13 +130,15 @@ __global__ void __launch_bounds__(256)
 // FP16 operands of v_mfma_f32_16x16x16f16 (FP32 accumulation): lane l feeds
 // A[l%16][4(l/16)..+3] and B[4(l/16)..+3][l%16] per 16-deep step.
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
-// FP16-operand 3x3 stride-1 convolution, channel-last in LDS. K runs tap by
-// tap over 16-channel groups (k = tap * Cp + ci, Cp = Cin rounded up to 16),
-// so the four k values of a lane's B operand are four consecutive channels
-// of one input point: one 8-byte LDS read. The weights are packed once per
-// call as FP16 w16[co][tap][ci] (zero padded) and read as A operands
-// straight from L2; only the 16-channel output tiles the layer has are
-// computed. Block: one image row, kConvSeg pixels (16 per wave).
+// FP16-operand 3x3 stride-1 convolution on v_mfma_f32_16x16x16f16, FP32
+// accumulation. A block computes kCvRows image rows x kCvPx pixels (one
+// wavefront per row, four 16-pixel MFMA tiles each) for all output channels,
+// looping over 16-channel input chunks: per chunk the input rows (with a
+// one-pixel halo) are staged channel-last and the packed FP16 weights
+// w16[co][tap][ci] of the chunk as [co][tap][16], so a lane's 4-deep A and B
+// operands are one 8-byte LDS read each (rows padded to 20 halves). Global
+// reads are issued from clamped addresses, eight at a time, and zeroed off
+// the image afterwards.
 __global__ void k_w16_pack(int Cin, int Cp, int Cout, int Cop, const float* __restrict__ w,
                            _Float16* __restrict__ w16) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,10 +147,14 @@ __global__ void k_w16_pack(int Cin, int Cp, int Cout, int Cop, const float* __re
   w16[t] = (_Float16)((co < Cout && ci < Cin) ? w[((size_t)co * Cin + ci) * 9 + tap] : 0.f);
 }
 
-// LDS of one k_conv3_f16 block: the channel-last input tile and the packed
-// weights, rows padded by 4 halves (8-byte operand reads conflict-free).
+constexpr int kCvRows = 4;            // image rows per block (one wavefront each)
+constexpr int kCvPx = 64;             // pixels per row
+constexpr int kCvHx = kCvPx + 2;      // with the halo
+constexpr int kCvCs = 20;             // halves per staged pixel / weight row (16 + pad)
+
 static size_t conv3_lds_bytes(int Cp, int Cop) {
-  return sizeof(_Float16) * ((size_t)3 * (kConvSeg + 2) * (Cp + 4) + (size_t)Cop * (9 * Cp + 4));
+  (void)Cp;
+  return sizeof(_Float16) * ((size_t)(kCvRows + 2) * kCvHx * kCvCs + (size_t)Cop * 9 * kCvCs);
 }
 
 template <int COT>
@@ -156,91 +162,89 @@ __global__ void __launch_bounds__(256)
     k_conv3_f16(int Cin, int Cp, int H, int W, int Cout, const float* __restrict__ in,
                 const _Float16* __restrict__ w16, const float* __restrict__ bias, int relu,
                 float* __restrict__ out) {
-  extern __shared__ __attribute__((aligned(16))) _Float16 csm[];
-  const int cs = Cp + 4;          // channel stride of the tile
-  const int ws = 9 * Cp + 4;      // row stride of the weights
-  _Float16* tile = csm;
-  _Float16* wsm = csm + 3 * (kConvSeg + 2) * cs;
-  const int x0 = blockIdx.x * kConvSeg, y = blockIdx.y, n = blockIdx.z;
-  // weights: rows of 9*Cp halves, copied 4 halves (8 bytes) at a time,
-  // eight reads in flight per thread
-  {
-    const int q = 9 * Cp / 4;  // 8-byte pieces per row
-    const int tw = COT * 16 * q;
-    for (int t0 = 0; t0 < tw; t0 += 8 * 256) {
-      h4v v[8];
+  __shared__ __attribute__((aligned(16))) _Float16 tin[(kCvRows + 2) * kCvHx * kCvCs];
+  __shared__ __attribute__((aligned(16))) _Float16 twt[COT * 16 * 9 * kCvCs];
+  const int x0 = blockIdx.x * kCvPx, y0 = blockIdx.y * kCvRows, n = blockIdx.z;
+  const int wave = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  const int lr = l % 16, kq = l / 16;
+  const float* inb = in + (size_t)n * Cin * H * W;
+  f4v acc[4][COT];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int t = min(t0 + u * 256 + (int)threadIdx.x, tw - 1);
-        const int co = t / q, k4 = t - co * q;
-        v[u] = *reinterpret_cast<const h4v*>(w16 + (size_t)co * 9 * Cp + 4 * k4);
-      }
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int t = 0; t < COT; ++t) acc[p][t] = f4v{0.f, 0.f, 0.f, 0.f};
+  constexpr int NIN = 16 * (kCvRows + 2) * kCvHx;  // input values per chunk
+  constexpr int NW = COT * 16 * 9 * 4;             // 4-half weight pieces per chunk
+  for (int ci0 = 0; ci0 < Cp; ci0 += 16) {
+    for (int t0 = 0; t0 < NIN; t0 += 8 * 256) {
+      float v[8];
+      int dst[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int t = t0 + u * 256 + threadIdx.x;
-        if (t >= tw) continue;
-        const int co = t / q, k4 = t - co * q;
-        *reinterpret_cast<h4v*>(wsm + (size_t)co * ws + 4 * k4) = v[u];
+        const int xx = t % kCvHx, rr = (t / kCvHx) % (kCvRows + 2), c = t / (kCvHx * (kCvRows + 2));
+        const int gy = y0 + rr - 1, gx = x0 + xx - 1, ci = ci0 + c;
+        const bool ok = t < NIN && ci < Cin && gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const int cy = min(max(gy, 0), H - 1), cx = min(max(gx, 0), W - 1), cc = min(ci, Cin - 1);
+        v[u] = inb[((size_t)cc * H + cy) * W + cx];
+        v[u] = ok ? v[u] : 0.f;
+        dst[u] = t < NIN ? (rr * kCvHx + xx) * kCvCs + c : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (dst[u] >= 0) tin[dst[u]] = (_Float16)v[u];
+    }
+    for (int t0 = 0; t0 < NW; t0 += 4 * 256) {
+      h4v v[4];
+      int dst[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = min(t0 + u * 256 + (int)threadIdx.x, NW - 1);
+        const int q = t % 4, tap = (t / 4) % 9, co = t / 36;
+        v[u] = *reinterpret_cast<const h4v*>(w16 + ((size_t)co * 9 + tap) * Cp + ci0 + 4 * q);
+        dst[u] = t0 + u * 256 + (int)threadIdx.x < NW ? (co * 9 + tap) * kCvCs + 4 * q : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (dst[u] >= 0) *reinterpret_cast<h4v*>(twt + dst[u]) = v[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      h4v av[COT];
+#pragma unroll
+      for (int t = 0; t < COT; ++t)
+        av[t] = *reinterpret_cast<const h4v*>(twt + ((t * 16 + lr) * 9 + tap) * kCvCs + 4 * kq);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const h4v bv = *reinterpret_cast<const h4v*>(
+            tin + ((wave + ky) * kCvHx + p * 16 + lr + kx) * kCvCs + 4 * kq);
+#pragma unroll
+        for (int t = 0; t < COT; ++t)
+          acc[p][t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av[t], bv, acc[p][t], 0, 0, 0);
       }
     }
+    __syncthreads();
   }
-  // input tile: reads from clamped (valid) addresses, unconditionally and
-  // eight at a time, zeroed afterwards outside the image (a read under a
-  // condition would be waited for before the next one)
-  const float* inb = in + (size_t)n * Cin * H * W;
-  const int tn = Cp * 3 * (kConvSeg + 2);
-  for (int t0 = 0; t0 < tn; t0 += 8 * 256) {
-    float v[8];
-    int dst[8];
+  const int y = y0 + wave;
+  if (y >= H) return;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int t = t0 + u * 256 + threadIdx.x;
-      const int xx = t % (kConvSeg + 2), dy = (t / (kConvSeg + 2)) % 3, ci = t / (3 * (kConvSeg + 2));
-      const int gy = y + dy - 1, gx = x0 + xx - 1;
-      const bool ok = t < tn && ci < Cin && gy >= 0 && gy < H && gx >= 0 && gx < W;
-      const int cy = min(max(gy, 0), H - 1), cx = min(max(gx, 0), W - 1), cc = min(ci, Cin - 1);
-      v[u] = inb[((size_t)cc * H + cy) * W + cx];
-      v[u] = ok ? v[u] : 0.f;
-      dst[u] = t < tn ? (dy * (kConvSeg + 2) + xx) * cs + ci : -1;
-    }
+  for (int p = 0; p < 4; ++p) {
+    const int x = x0 + p * 16 + lr;
+    if (x >= W) continue;
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (dst[u] >= 0) tile[dst[u]] = (_Float16)v[u];
-  }
-  __syncthreads();
-  const int wave = threadIdx.x / kWave, l = threadIdx.x % kWave;
-  const int px = wave * 16 + l % 16;
-  const int kq = l / 16;
-  f4v acc[COT];
+    for (int t = 0; t < COT; ++t)
 #pragma unroll
-  for (int t = 0; t < COT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
-  const _Float16* wl = wsm + (size_t)(l % 16) * ws + 4 * kq;
-  for (int tap = 0; tap < 9; ++tap) {
-    const int ky = tap / 3, kx = tap - 3 * ky;
-    const _Float16* bp = tile + (ky * (kConvSeg + 2) + px + kx) * cs + 4 * kq;
-    const _Float16* ap = wl + tap * Cp;
-    for (int ci0 = 0; ci0 < Cp; ci0 += 16) {
-      const h4v bv = *reinterpret_cast<const h4v*>(bp + ci0);
-#pragma unroll
-      for (int t = 0; t < COT; ++t) {
-        const h4v av = *reinterpret_cast<const h4v*>(ap + (size_t)t * 16 * ws + ci0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc[t], 0, 0, 0);
+      for (int r = 0; r < 4; ++r) {
+        const int co = t * 16 + 4 * kq + r;
+        if (co >= Cout) continue;
+        float v = acc[p][t][r];
+        if (bias) v += bias[co];
+        if (relu) v = fmaxf(v, 0.f);
+        out[(((size_t)n * Cout + co) * H + y) * W + x] = v;
       }
-    }
   }
-  const int x = x0 + px;
-  if (x >= W) return;
-#pragma unroll
-  for (int t = 0; t < COT; ++t)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int co = t * 16 + 4 * kq + r;
-      if (co >= Cout) continue;
-      float v = acc[t][r];
-      if (bias) v += bias[co];
-      if (relu) v = fmaxf(v, 0.f);
-      out[(((size_t)n * Cout + co) * H + y) * W + x] = v;
-    }
 }
 
 // Transposed convolution, weights (Cin, Cout, 3, 3) (Cin = channels of `in`):
@@ -504,10 +508,16 @@ static void launch_conv_t(int B, int Cin, int h, int wd, int Cout, int H, int W,
                      w, out, acc);
 }
 
+// input / output (optional): the input slot and the last layer's slot read
+// and written there instead of in act
 static int net_forward(const iga_net_t& net, int batch, const float* params, float* act,
-                       hipStream_t st) {
+                       hipStream_t st, const float* input = nullptr, float* output = nullptr) {
   const NetPlan P = plan(net, batch);
-  auto A = [&](int l) { return act + (l < 0 ? P.in_off : P.off[l]); };
+  auto A = [&](int l) -> float* {
+    if (l < 0 && input) return const_cast<float*>(input);
+    if (l == net.n_layers - 1 && output) return output;
+    return act + (l < 0 ? P.in_off : P.off[l]);
+  };
   for (int l = 0; l < net.n_layers; ++l) {
     const int src = net.src[l];
     const int Ci = in_C(net, src), Hi = in_H(net, src), Wi = in_W(net, src);
@@ -521,12 +531,13 @@ static int net_forward(const iga_net_t& net, int batch, const float* params, flo
       // descriptor asks for it), bias and ReLU in the epilogue
       dim3 grid((Wo + kConvSeg - 1) / kConvSeg, Ho, batch);
       if (P.w16_off[l] >= 0) {
+        grid = dim3((Wo + kCvPx - 1) / kCvPx, (Ho + kCvRows - 1) / kCvRows, batch);
         const int Cp = (Ci + 15) / 16 * 16, Cop = (Co + 15) / 16 * 16;
         _Float16* w16 = reinterpret_cast<_Float16*>(act + P.w16_off[l]);
         const int nw = Cop * 9 * Cp;
         hipLaunchKernelGGL(k_w16_pack, dim3((nw + 255) / 256), dim3(256), 0, st, Ci, Cp, Co, Cop, w,
                            w16);
-        const size_t lds = conv3_lds_bytes(Cp, Cop);
+        const size_t lds = 0;  // static LDS (conv3_lds_bytes)
         switch (Cop / 16) {
           case 1:
             hipLaunchKernelGGL((k_conv3_f16<1>), grid, dim3(256), lds, st, Ci, Cp, Hi, Wi, Co,
@@ -659,6 +670,12 @@ extern "C" int iga_net_forward(const iga_net_t* net, int32_t batch, const float*
                                float* act, void* stream) {
   if (!net || !params || !act) return IGA_ERR_ARG;
   return net_forward(*net, batch, params, act, (hipStream_t)stream);
+}
+
+extern "C" int iga_net_infer(const iga_net_t* net, int32_t batch, const float* params, float* act,
+                             const float* input, float* output, void* stream) {
+  if (!net || !params || !act || !input || !output || net->n_layers < 1) return IGA_ERR_ARG;
+  return net_forward(*net, batch, params, act, (hipStream_t)stream, input, output);
 }
 
 extern "C" int iga_net_backward(const iga_net_t* net, int32_t batch, const float* params,

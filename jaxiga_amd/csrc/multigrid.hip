@@ -69,6 +69,8 @@ __device__ __forceinline__ int wave_min(int v) {
   return v;
 }
 
+constexpr int kGalRows = 36;  // fine rows of a P^T row (and P^T entries of a coarse column) held in registers
+
 __global__ void __launch_bounds__(256)
     k_galerkin_rows(int nc, int wf, int R, const int32_t* __restrict__ k_ptr,
                     const int32_t* __restrict__ k_cols, const double* __restrict__ k_vals,
@@ -84,32 +86,53 @@ __global__ void __launch_bounds__(256)
   double* box = boxes[wave];
   for (int t = lane; t < kGalBox * kGalBox; t += kWave) box[t] = 0.0;
   const int a0 = pt_ptr[I], a1 = pt_ptr[I + 1];
-  int rmin = 1 << 30, cmin = 1 << 30;
-  for (int k = a0 + lane; k < a1; k += kWave) {
-    const int a = pt_cols[k];
-    rmin = min(rmin, a / wf);
-    cmin = min(cmin, a % wf);
-  }
-  const int r0 = wave_min(rmin) - R, c0 = wave_min(cmin) - R;
-  // the fine rows of P^T row I and their K-row bounds, fetched once, one per lane
-  int my_q0 = 0, my_q1 = 0;
+  const int na = a1 - a0;
+  // the fine rows of P^T row I and their K-row bounds, one per lane
+  int my_a = 0, my_q0 = 0, my_q1 = 0;
   double my_pa = 0.0;
-  if (lane < a1 - a0 && lane < kWave) {
-    const int a = pt_cols[a0 + lane];
+  if (lane < na) {
+    my_a = pt_cols[a0 + lane];
     my_pa = pt_vals[a0 + lane];
-    my_q0 = k_ptr[a];
-    my_q1 = k_ptr[a + 1];
+    my_q0 = k_ptr[my_a];
+    my_q1 = k_ptr[my_a + 1];
   }
+  const int r0 = wave_min(lane < na ? my_a / wf : 1 << 30) - R;
+  const int c0 = wave_min(lane < na ? my_a % wf : 1 << 30) - R;
+  // every K entry of the first kGalRows fine rows, one per lane, loaded at once
+  int cb[kGalRows];
+  double cv[kGalRows];
+#pragma unroll
+  for (int k = 0; k < kGalRows; ++k) {
+    const int q0 = __shfl(my_q0, k, kWave), q1 = __shfl(my_q1, k, kWave);
+    const int q = (k < na && lane < q1 - q0) ? q0 + lane : -1;
+    cb[k] = k_cols[q >= 0 ? q : 0];
+    cv[k] = q >= 0 ? k_vals[q] : 0.0;
+    cb[k] = q >= 0 ? cb[k] : -1;
+  }
+  __builtin_amdgcn_sched_barrier(0);
   __builtin_amdgcn_wave_barrier();
-  for (int k = 0; k < a1 - a0; ++k) {
-    int q0, q1;
+  // T = P^T K row I in the box, fine rows in order (lanes never collide
+  // within one row)
+#pragma unroll
+  for (int k = 0; k < kGalRows; ++k) {
+    if (k < na) {
+      const double pa = __shfl(my_pa, k, kWave);
+      if (cb[k] >= 0) {
+        const int br = cb[k] / wf - r0, bc = cb[k] % wf - c0;
+        box[br * kGalBox + bc] += pa * cv[k];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int k = kGalRows; k < na; ++k) {
+    int q0, q1, a;
     double pa;
     if (k < kWave) {
       q0 = __shfl(my_q0, k, kWave);
       q1 = __shfl(my_q1, k, kWave);
       pa = __shfl(my_pa, k, kWave);
     } else {
-      const int a = pt_cols[a0 + k];
+      a = pt_cols[a0 + k];
       pa = pt_vals[a0 + k];
       q0 = k_ptr[a];
       q1 = k_ptr[a + 1];
@@ -122,12 +145,30 @@ __global__ void __launch_bounds__(256)
     __builtin_amdgcn_wave_barrier();
   }
   // the coarse row in FP64 and FP32, and its diagonal scalings (1 where the
-  // diagonal is zero or absent)
+  // diagonal is zero or absent); the P^T entries of each coarse column J are
+  // loaded at once
   bool diag = false;
   for (int k = c_ptr[I] + lane; k < c_ptr[I + 1]; k += kWave) {
     const int J = c_cols[k];
+    const int s0 = pt_ptr[J], s1 = pt_ptr[J + 1];
+    int fb[kGalRows];
+    double fw[kGalRows];
+#pragma unroll
+    for (int m = 0; m < kGalRows; ++m) {
+      const int q = s0 + m < s1 ? s0 + m : -1;
+      fb[m] = pt_cols[q >= 0 ? q : s0];
+      fw[m] = q >= 0 ? pt_vals[q] : 0.0;
+      fb[m] = q >= 0 ? fb[m] : -1;
+    }
+    __builtin_amdgcn_sched_barrier(0);
     double acc = 0.0;
-    for (int q = pt_ptr[J]; q < pt_ptr[J + 1]; ++q) {
+#pragma unroll
+    for (int m = 0; m < kGalRows; ++m) {
+      if (fb[m] < 0) continue;
+      const int br = fb[m] / wf - r0, bc = fb[m] % wf - c0;
+      if (br >= 0 && br < kGalBox && bc >= 0 && bc < kGalBox) acc += fw[m] * box[br * kGalBox + bc];
+    }
+    for (int q = s0 + kGalRows; q < s1; ++q) {
       const int b = pt_cols[q];
       const int br = b / wf - r0, bc = b % wf - c0;
       if (br >= 0 && br < kGalBox && bc >= 0 && bc < kGalBox) acc += pt_vals[q] * box[br * kGalBox + bc];
@@ -166,6 +207,30 @@ __global__ void k_level_scalings(int n, const int32_t* __restrict__ indptr,
   const double di = d != 0.0 ? 1.0 / d : 1.0;
   dinv[i] = di;
   dsq[i] = (float)sqrt(fabs(di));
+}
+
+// level 0 of a new operator: vals64 = vals (when a copy is wanted), vals32 =
+// (float) vals, and the row scalings (read from vals, so no pass waits on
+// another)
+__global__ void k_level0_prep(int n, int64_t nnz, const int32_t* __restrict__ indptr,
+                              const int32_t* __restrict__ cols, const double* __restrict__ vals,
+                              double* __restrict__ vals64, float* __restrict__ vals32,
+                              double* __restrict__ dinv, float* __restrict__ dsq) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double v = vals[k];
+    if (vals64) vals64[k] = v;
+    vals32[k] = (float)v;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int lo = indptr[i], hi = indptr[i + 1];
+    const int k = lower_bound(cols, lo, hi, (int)i);
+    const double d = (k < hi && cols[k] == i) ? vals[k] : 0.0;
+    const double di = d != 0.0 ? 1.0 / d : 1.0;
+    dinv[i] = di;
+    dsq[i] = (float)sqrt(fabs(di));
+  }
 }
 
 __global__ void k_copy_d(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -211,6 +276,45 @@ __global__ void k_densify(int n, const int32_t* __restrict__ indptr,
   for (int j = threadIdx.x; j < 2 * n; j += blockDim.x) row[j] = j == n + i ? 1.0 : 0.0;
   __syncthreads();
   for (int k = indptr[i] + threadIdx.x; k < indptr[i + 1]; k += blockDim.x) row[cols[k]] = vals[k];
+}
+
+// Gauss-Jordan elimination of [A | I] without pivoting (A is SPD), one block,
+// with the augmented matrix held in LDS (n <= kGjLds); written back at the end.
+constexpr int kGjLds = 48;
+
+// The FP32 inverse of a small SPD CSR matrix in one block: [A | I] is formed,
+// eliminated and read out in LDS.
+__global__ void __launch_bounds__(1024)
+    k_dense_inverse_lds(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                        const double* __restrict__ vals, float* __restrict__ inv) {
+  __shared__ double M[kGjLds * 2 * kGjLds];
+  __shared__ double fac[kGjLds];
+  const int n2 = 2 * n;
+  for (int t = threadIdx.x; t < n * n2; t += blockDim.x) {
+    const int i = t / n2, j = t - i * n2;
+    M[t] = j == n + i ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    for (int k = indptr[i]; k < indptr[i + 1]; ++k) M[i * n2 + cols[k]] = vals[k];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double piv = M[k * n2 + k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) M[k * n2 + j] /= piv;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = M[i * n2 + k];
+    __syncthreads();
+    for (int t = threadIdx.x; t < n * n2; t += blockDim.x) {
+      const int i = t / n2, j = t - i * n2;
+      if (i != k) M[t] -= fac[i] * M[k * n2 + j];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t - i * n;
+    inv[t] = (float)M[i * n2 + n + j];
+  }
 }
 
 // Gauss-Jordan elimination of [A | I] without pivoting (A is SPD), one block.
@@ -1256,6 +1360,7 @@ struct GridOpsArgs {
   float* op[16];
   const double* vals64;
   double* op64;
+  float* feats;  // level 0: K[i, i + off] / K[i, i] (FP32), with op64
 };
 
 constexpr int kOpsRows = 64;
@@ -1307,27 +1412,165 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     if (i >= i1) continue;
     const int r = i / w, c = i - r * w;
     const int dy = o / kGS - kGR, dx = o % kGS - kGR;
-    float v = 0.f;
-    double v64 = 0.0;
-    if (on_grid(r + dy, c + dx, h, w)) {
-      const int j = i + dy * w + dx;
-      const int s0 = sp[il], e0 = sp[il + 1];
-      const int dy0 = max(-kGR, -r), dx0 = max(-kGR, -c);
-      const int ncol = min(kGR, w - 1 - c) - dx0 + 1;
-      const int k = s0 + (dy - dy0) * ncol + (dx - dx0);
+    const int s0 = sp[il], e0 = sp[il + 1];
+    const int dy0 = max(-kGR, -r), dx0 = max(-kGR, -c);
+    const int ncol = min(kGR, w - 1 - c) - dx0 + 1;
+    // the FP64 entry at grid offset (ey, ex) of row i (0 off the grid or absent)
+    auto entry = [&](int ey, int ex, float& v32) -> double {
+      v32 = 0.f;
+      if (!on_grid(r + ey, c + ex, h, w)) return 0.0;
+      const int j = i + ey * w + ex;
+      const int k = s0 + (ey - dy0) * ncol + (ex - dx0);
       if (fits && k < e0 && sc[k] == j) {
-        v = sv[k];
-        v64 = sv64[k];
-      } else {
-        for (int q = lo + s0; q < lo + e0; ++q)
-          if (cols[q] == j) {
-            v = vals[q];
-            v64 = f64 ? a.vals64[q] : 0.0;
-          }
+        v32 = sv[k];
+        return sv64[k];
+      }
+      double v64 = 0.0;
+      for (int q = lo + s0; q < lo + e0; ++q)
+        if (cols[q] == j) {
+          v32 = vals[q];
+          v64 = f64 ? a.vals64[q] : 0.0;
+        }
+      return v64;
+    };
+    float v;
+    const double v64 = entry(dy, dx, v);
+    a.op[l][(size_t)o * n + i] = v;
+    if (f64) {
+      a.op64[(size_t)o * n + i] = v64;
+      if (a.feats) {
+        float dummy;
+        double d = entry(0, 0, dummy);
+        if (d == 0.0) d = 1.0;
+        a.feats[(size_t)o * n + i] = (float)(v64 / d);
       }
     }
-    a.op[l][(size_t)o * n + i] = v;
-    if (f64) a.op64[(size_t)o * n + i] = v64;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tensor Galerkin passes: K_c = P^T K P with P = P1 (x) P0 in two 1-D passes
+// over FP64 stencil layouts (radius 3 in and out):
+//   pass 0 (along grid rows): K'[(r, C), (dr, dC)] = sum_{c, c'} P0[c, C]
+//          K[(r, c), (r + dr, c')] P0[c', C + dC]
+//   pass 1 (across rows):     K_c[(R, C), (dR, dC)] = sum_{r, r'} P1[r, R]
+//          K'[(r, C), (r', C + dC)] P1[r', R + dR]
+// Each thread keeps u[j] = sum_c P[c, C] K[c, c'] for the 12 points c' of
+// its window (c' = f_lo[C] - 3 + j), then forms the 7 outputs of its offset
+// line from u and the coarse-column bands of its neighbours.
+// ---------------------------------------------------------------------------
+
+constexpr int kGalWin = kKFC + 2 * kGR;  // 12
+
+// out[d] = sum_j u[j] P[f0 - 3 + j, C + d - 3] for d < 7 (zero for a coarse
+// index off [0, nc)); P[f, C2] = rw[C2][f - f_lo[C2]] inside the band
+__device__ __forceinline__ void gal_close(const double (&u)[kGalWin], int f0, int C, int nc,
+                                          const int32_t* __restrict__ f_lo,
+                                          const double* __restrict__ rw, double (&out)[kGS]) {
+#pragma unroll
+  for (int d = 0; d < kGS; ++d) {
+    const int C2 = C + d - kGR;
+    const bool in = C2 >= 0 && C2 < nc;
+    const int Cc = in ? C2 : 0;
+    const int off = f0 - kGR - f_lo[Cc];
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGalWin; ++j) {
+      const int k = j + off;
+      const double wv = rw[Cc * kKFC + min(max(k, 0), kKFC - 1)];
+      s += (k >= 0 && k < kKFC) ? u[j] * wv : 0.0;
+    }
+    out[d] = in ? s : 0.0;
+  }
+}
+
+// thread (dr, r, C), C fastest
+__global__ void __launch_bounds__(256)
+    k_gal_pass0(int hf, int wf, int wc, const int32_t* __restrict__ f_lo,
+                const double* __restrict__ rw, const double* __restrict__ K,
+                double* __restrict__ Kp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t np = (int64_t)hf * wc;
+  if (t >= kGS * np) return;
+  const int C = (int)(t % wc), r = (int)((t / wc) % hf), dr = (int)(t / np) - kGR;
+  const int64_t nf = (int64_t)hf * wf;
+  const int f0 = f_lo[C];
+  double u[kGalWin];
+#pragma unroll
+  for (int j = 0; j < kGalWin; ++j) u[j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < kKFC; ++a) {
+    const double wa = rw[C * kKFC + a];
+    const int64_t idx = (int64_t)r * wf + min(f0 + a, wf - 1);
+#pragma unroll
+    for (int dc = -kGR; dc <= kGR; ++dc)
+      u[a + dc + kGR] += wa * K[((dr + kGR) * kGS + dc + kGR) * nf + idx];
+  }
+  double out[kGS];
+  gal_close(u, f0, C, wc, f_lo, rw, out);
+#pragma unroll
+  for (int d = 0; d < kGS; ++d) Kp[((dr + kGR) * kGS + d) * np + (int64_t)r * wc + C] = out[d];
+}
+
+// thread (dC, R, C), C fastest: the coarse stencil line (., dC) of point
+// (R, C) in FP64 (and FP32) stencil layout, its CSR values and, for dC = 0,
+// the diagonal scalings (1 where the diagonal is zero)
+__global__ void __launch_bounds__(256)
+    k_gal_pass1(int hf, int hc, int wc, const int32_t* __restrict__ f_lo,
+                const double* __restrict__ rw, const double* __restrict__ Kp,
+                double* __restrict__ S64, float* __restrict__ S32,
+                const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
+                double* __restrict__ vals64, float* __restrict__ vals32,
+                double* __restrict__ dinv, float* __restrict__ dsq) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)hc * wc, np = (int64_t)hf * wc;
+  if (t >= kGS * nc) return;
+  const int C = (int)(t % wc), R = (int)((t / wc) % hc), dC = (int)(t / nc) - kGR;
+  const int f0 = f_lo[R];
+  double u[kGalWin];
+#pragma unroll
+  for (int j = 0; j < kGalWin; ++j) u[j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < kKFC; ++a) {
+    const double wa = rw[R * kKFC + a];
+    const int64_t idx = (int64_t)min(f0 + a, hf - 1) * wc + C;
+#pragma unroll
+    for (int dr = -kGR; dr <= kGR; ++dr)
+      u[a + dr + kGR] += wa * Kp[((dr + kGR) * kGS + dC + kGR) * np + idx];
+  }
+  double out[kGS];
+  gal_close(u, f0, R, hc, f_lo, rw, out);
+  const int I = R * wc + C;
+  const bool cin = C + dC >= 0 && C + dC < wc;
+  const int lo = indptr[I], hi = indptr[I + 1];
+  const int dR0 = max(-kGR, -R), dC0 = max(-kGR, -C);
+  const int ncol = min(kGR, wc - 1 - C) - dC0 + 1;
+#pragma unroll
+  for (int d = 0; d < kGS; ++d) {
+    const int dR = d - kGR;
+    const double v = cin ? out[d] : 0.0;
+    const int o = d * kGS + dC + kGR;
+    S64[(int64_t)o * nc + I] = v;
+    if (S32) S32[(int64_t)o * nc + I] = (float)v;
+    if (cin && R + dR >= 0 && R + dR < hc) {
+      const int J = I + dR * wc + dC;
+      int k = lo + (dR - dR0) * ncol + (dC - dC0);
+      if (k >= hi || cols[k] != J) {
+        k = -1;
+        for (int q = lo; q < hi; ++q)
+          if (cols[q] == J) k = q;
+      }
+      if (k >= 0) {
+        vals64[k] = v;
+        vals32[k] = (float)v;
+      }
+    }
+  }
+  if (dC == 0) {
+    const double dv = out[kGR];
+    const double di = dv != 0.0 ? 1.0 / dv : 1.0;
+    dinv[I] = di;
+    dsq[I] = (float)sqrt(fabs(di));
   }
 }
 
@@ -1763,26 +2006,57 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
   }
   if (s->level[L - 1].n > 4096) return IGA_ERR_SHAPE;
   {
+    // the finest level's copy, FP32 copy and scalings in one pass; the
+    // Galerkin kernel writes those of every coarser level with its values
     const iga_mg_setup_level_t& V = s->level[0];
-    if (V.vals64 != vals0)
-      hipLaunchKernelGGL(k_copy_d, dim3((int)std::min<int64_t>(blocks_for(V.nnz), 4096)), dim3(256), 0,
-                         st, (int64_t)V.nnz, vals0, V.vals64);
+    const int64_t m = std::max<int64_t>(V.nnz, V.n);
+    hipLaunchKernelGGL(k_level0_prep, dim3((int)std::min<int64_t>(blocks_for(m), 8192)), dim3(256), 0,
+                       st, V.n, (int64_t)V.nnz, V.indptr, V.cols, vals0,
+                       V.vals64 != vals0 ? V.vals64 : nullptr, V.vals32, V.dinv, V.dsq);
   }
-  {
-    // the finest level's FP32 copy and scalings; the Galerkin kernel writes
-    // those of every coarser level with its values
+  const iga_mg_setup_grid_t* G = s->grid;
+  if (G) {
+    for (int l = 0; l < L; ++l) {
+      const iga_mg_setup_level_t& V = s->level[l];
+      if (!G->op64[l] || (int64_t)V.h * V.w != V.n) return IGA_ERR_ARG;
+      if (l + 1 < L && (!G->f_lo[l][0] || !G->f_lo[l][1] || !G->rw[l][0] || !G->rw[l][1] || !G->work))
+        return IGA_ERR_ARG;
+    }
+    if (s->radius != kGR) return IGA_ERR_UNSUPPORTED;
+    // level 0 in stencil layout (FP32, FP64, features) from its CSR
     const iga_mg_setup_level_t& V = s->level[0];
-    hipLaunchKernelGGL(k_to_f32, dim3((int)std::min<int64_t>(blocks_for(V.nnz), 4096)), dim3(256), 0,
-                       st, (int64_t)V.nnz, V.vals64, V.vals32);
-    hipLaunchKernelGGL(k_level_scalings, dim3(blocks_for(V.n)), dim3(256), 0, st, V.n, V.indptr,
-                       V.cols, V.vals64, V.dinv, V.dsq);
-  }
-  for (int l = 0; l + 1 < L; ++l) {
-    const iga_mg_setup_level_t& V = s->level[l];
-    const iga_mg_setup_level_t& C = s->level[l + 1];
-    hipLaunchKernelGGL(k_galerkin_rows, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, V.w,
-                       s->radius, V.indptr, V.cols, V.vals64, V.pt_ptr, V.pt_cols, V.pt_vals,
-                       C.indptr, C.cols, C.vals64, C.vals32, C.dinv, C.dsq);
+    GridOpsArgs a{};
+    a.n_levels = 1;
+    a.n[0] = V.n;
+    a.w[0] = V.w;
+    a.indptr[0] = V.indptr;
+    a.cols[0] = V.cols;
+    a.vals[0] = V.vals32;
+    float* op32_0 = G->op32[0];
+    if (!op32_0) return IGA_ERR_ARG;
+    a.op[0] = op32_0;
+    a.vals64 = vals0;
+    a.op64 = G->op64[0];
+    a.feats = G->feats;
+    hipLaunchKernelGGL(k_grid_ops, dim3((V.n + kOpsRows - 1) / kOpsRows, 1), dim3(kGTh), 0, st, a);
+    for (int l = 0; l + 1 < L; ++l) {
+      const iga_mg_setup_level_t& F = s->level[l];
+      const iga_mg_setup_level_t& C = s->level[l + 1];
+      const int64_t n0 = (int64_t)kGS * F.h * C.w, n1 = (int64_t)kGS * C.n;
+      hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, F.h, F.w,
+                         C.w, G->f_lo[l][0], G->rw[l][0], G->op64[l], G->work);
+      hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, F.h, C.h,
+                         C.w, G->f_lo[l][1], G->rw[l][1], G->work, G->op64[l + 1], G->op32[l + 1],
+                         C.indptr, C.cols, C.vals64, C.vals32, C.dinv, C.dsq);
+    }
+  } else {
+    for (int l = 0; l + 1 < L; ++l) {
+      const iga_mg_setup_level_t& V = s->level[l];
+      const iga_mg_setup_level_t& C = s->level[l + 1];
+      hipLaunchKernelGGL(k_galerkin_rows, dim3((C.n + 3) / 4), dim3(256), 0, st, C.n, V.w,
+                         s->radius, V.indptr, V.cols, V.vals64, V.pt_ptr, V.pt_cols, V.pt_vals,
+                         C.indptr, C.cols, C.vals64, C.vals32, C.dinv, C.dsq);
+    }
   }
   IGA_CHECK_LAUNCH();
   const iga_mg_setup_level_t& C = s->level[L - 1];
@@ -1808,6 +2082,11 @@ extern "C" int iga_dense_inverse(int32_t n, const int32_t* indptr, const int32_t
                                  const double* vals, double* work, float* inv, void* stream) {
   if (n <= 0 || n > 4096 || !indptr || !cols || !vals || !work || !inv) return IGA_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
+  if (n <= kGjLds) {
+    hipLaunchKernelGGL(k_dense_inverse_lds, dim3(1), dim3(1024), 0, st, n, indptr, cols, vals, inv);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
+  }
   hipLaunchKernelGGL(k_densify, dim3(n), dim3(256), 0, st, n, indptr, cols, vals, work);
   hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), n * sizeof(double), st, n, work);
   hipLaunchKernelGGL(k_inverse_f32, dim3(blocks_for((int64_t)n * n)), dim3(256), 0, st, n, work,
@@ -1936,6 +2215,7 @@ extern "C" int iga_mg_grid_ops(const iga_mg_t* mg, const double* vals64, void* s
   }
   a.vals64 = vals64;
   a.op64 = vals64 ? G.op64 : nullptr;
+  a.feats = a.op64 ? G.feats : nullptr;
   hipLaunchKernelGGL(k_grid_ops, dim3((nmax + kOpsRows - 1) / kOpsRows, G.n_levels), dim3(kGTh),
                      0, (hipStream_t)stream, a);
   IGA_CHECK_LAUNCH();

@@ -163,6 +163,27 @@ class ConvNet:
         Ho, Wo = st.H[len(self.arch)], st.W[len(self.arch)]
         return act[out_off: out_off + B * cout * Ho * Wo].view(B, cout, Ho, Wo), act
 
+    def infer(self, x, out, precision=1):
+        """Inference of one sample without copies: input ``x (c_in, H, W)`` and
+        output ``out`` (``(c_out, H, W)`` floats) are read and written in place;
+        the hidden activations use a buffer kept per input size."""
+        import torch
+
+        C, H, W = x.shape[-3:]
+        if C != self.c_in:
+            raise ValueError(f"network expects {self.c_in} input channels, got {C}")
+        st = self.struct(H, W, precision)
+        L = lib()
+        act = self._acts.get((H, W, precision)) if hasattr(self, "_acts") else None
+        if act is None:
+            act = torch.empty(int(L.iga_net_act_floats(st, 1)), dtype=torch.float32, device=x.device)
+            self.__dict__.setdefault("_acts", {})[(H, W, precision)] = act
+        if out.numel() != self.arch[-1][3] * st.H[len(self.arch)] * st.W[len(self.arch)]:
+            raise ValueError("output buffer size does not match the network")
+        _lib.check(L.iga_net_infer(st, 1, _lib.ptr(self.params), _lib.ptr(act), _lib.ptr(x),
+                                   _lib.ptr(out), stream()), "iga_net_infer")
+        return out
+
     def backward(self, act, gy, B, H, W):
         """Parameter gradients into ``self.grads``; returns d/d input."""
         import torch
@@ -202,6 +223,8 @@ def features(op: MG.MGOperator, level):
 
     lv = op.hier.levels[level]
     R = op.hier.p
+    if level == 0 and op.grid_ops is not None and R == MG.GRID_RADIUS:
+        return op.grid_feats.view(1, -1, lv.h, lv.w)  # written by MGOperator.load
     f = torch.empty(((2 * R + 1) ** 2, lv.n), dtype=torch.float32, device=op.vals64[0].device)
     _lib.check(lib().iga_stencil_features(lv.n, lv.w, R, _lib.ptr(lv.indptr), _lib.ptr(lv.cols),
                                           _lib.ptr(op.vals64[level]), _lib.ptr(f), stream()),
@@ -255,8 +278,8 @@ def smoother_stencils(gen: ConvNet, op: MG.MGOperator, levels=LEARNED_LEVELS):
     out = []
     for l in range(op.hier.n_levels - 1):
         if l < levels:
-            y, _ = gen.forward(features(op, l), precision=1)
-            out.append(y[0].reshape(op.ks * op.ks, -1).contiguous())
+            # written straight into the operator's stencil buffer of the level
+            out.append(gen.infer(features(op, l), op.stencil_buffer(l)))
         else:
             out.append(None)
     return out

@@ -181,6 +181,37 @@ class Hierarchy:
     def n_levels(self):
         return len(self.levels)
 
+    def tensor_bands(self):
+        """Device coarse-column bands ``(f_lo, rw64)`` of every 1-D transfer
+        (per level, per direction) for the tensor Galerkin passes, or None
+        when an operator reaches beyond three grid points or a transfer's
+        coupling does (a fine point of coarse column C, moved by up to three
+        points, must stay in the support of columns within three of C)."""
+        import torch
+
+        if "tensor" in self._bands:
+            return self._bands["tensor"]
+        out = None
+        if all(lv.radius <= GRID_RADIUS for lv in self.levels):
+            dev = self.levels[0].indptr.device
+            out = []
+            for lv in self.levels[:-1]:
+                per_dir = []
+                for d in range(2):
+                    tb = band_tables(lv.P1[d])
+                    if tb is None or not tensor_coupling_ok(*tb):
+                        return self._bands.setdefault("tensor", None)
+                    c_lo, pw, f_lo, rw = tb
+                    rw64 = np.zeros_like(rw, dtype=np.float64)
+                    T = np.asarray(lv.P1[d], float)
+                    for c in range(T.shape[1]):
+                        k = min(KFC, T.shape[0] - f_lo[c])
+                        rw64[c, :k] = T[f_lo[c]: f_lo[c] + k, c]
+                    per_dir.append((torch.as_tensor(f_lo, device=dev), torch.as_tensor(rw64, device=dev)))
+                out.append(per_dir)
+        self._bands["tensor"] = out
+        return out
+
     def grid_bands(self, tail_level):
         """Device band tables of the transfers of levels ``0 .. tail_level-1``
         for the grid V-cycle, or None when a level's operator reaches beyond
@@ -268,6 +299,23 @@ def band_spans_fit(c_lo, pw, f_lo, rw, level0_dir=None, tile=16, halo=3):
         a, z = max(f0 - halo, 0), min(f0 + ft + halo - 1, nf - 1)
         if c_lo[z] + KPF - c_lo[a] > span:
             return False
+    return True
+
+
+def tensor_coupling_ok(c_lo, pw, f_lo, rw):
+    """Whether the 1-D Galerkin pass of a transfer stays within the radius:
+    every fine point f of coarse column C (P[f, C] != 0),
+    moved by up to GRID_RADIUS points, lies only in columns C' with
+    |C' - C| <= GRID_RADIUS."""
+    nf, nc = len(c_lo), len(f_lo)
+    for C in range(nc):
+        for f in range(f_lo[C], min(f_lo[C] + KFC, nf)):
+            if rw[C, f - f_lo[C]] == 0.0:
+                continue
+            for g in range(max(f - GRID_RADIUS, 0), min(f + GRID_RADIUS, nf - 1) + 1):
+                cols = np.flatnonzero(pw[g]) + c_lo[g]
+                if len(cols) and np.abs(cols - C).max() > GRID_RADIUS:
+                    return False
     return True
 
 
@@ -386,10 +434,13 @@ class MGOperator:
             nst = (2 * GRID_RADIUS + 1) ** 2
             self.grid_ops = [torch.empty(nst * hier.levels[l].n, **f32) for l in range(self.tail_level)]
             self.grid_op64 = torch.empty(nst * n0, **f64)  # the flexible-CG operator
+            # level 0's stencil divided by its diagonal, written with the layouts
+            self.grid_feats = torch.empty(nst * n0, **f32)
             g = _lib.IgaMgGrid()
             g.n_levels = self.tail_level
             g.radius = GRID_RADIUS
             g.op64 = self.grid_op64.data_ptr()
+            g.feats = self.grid_feats.data_ptr()
             for l, (op, bands) in enumerate(zip(self.grid_ops, self._bands)):
                 g.op[l] = op.data_ptr()
                 for d in range(2):
@@ -397,6 +448,26 @@ class MGOperator:
                     B = g.band[l][d]
                     B.c_lo, B.pw, B.f_lo, B.rw = c_lo.data_ptr(), pw.data_ptr(), f_lo.data_ptr(), rw.data_ptr()
             self._grid = g
+        # tensor Galerkin set-up (with the grid cycle): FP64 stencils of every
+        # level, the pass intermediate, and the FP32 stencils the cycle reads
+        self._tensor = None
+        tb = hier.tensor_bands() if self._grid is not None else None
+        if tb is not None:
+            nst = (2 * GRID_RADIUS + 1) ** 2
+            lv = hier.levels
+            self.grid_s64 = [self.grid_op64] + [torch.empty(nst * l.n, **f64) for l in lv[1:]]
+            self._gal_work = torch.empty(nst * max(lv[l].h * lv[l + 1].w for l in range(len(lv) - 1)), **f64)
+            sg = _lib.IgaMgSetupGrid()
+            for l in range(len(lv)):
+                sg.op64[l] = self.grid_s64[l].data_ptr()
+                sg.op32[l] = self.grid_ops[l].data_ptr() if l < len(self.grid_ops) else None
+            for l, per_dir in enumerate(tb):
+                for d in range(2):
+                    sg.f_lo[l][d] = per_dir[d][0].data_ptr()
+                    sg.rw[l][d] = per_dir[d][1].data_ptr()
+            sg.feats = self.grid_feats.data_ptr()
+            sg.work = self._gal_work.data_ptr()
+            self._tensor = sg
 
     def _setup_struct(self):
         if getattr(self, "_setup", None) is None:
@@ -415,6 +486,8 @@ class MGOperator:
                     L.pt_ptr, L.pt_cols, L.pt_vals = ip.data_ptr(), cc.data_ptr(), v64.data_ptr()
             m.coarse_inv = self.coarse_inv.data_ptr()
             m.inv_work = self._inv_work.data_ptr()
+            if self._tensor is not None:
+                m.grid = _lib.ctypes.pointer(self._tensor)
             self._setup = m
         return self._setup
 
@@ -426,7 +499,8 @@ class MGOperator:
         _lib.check(lib().iga_mg_load(self._setup_struct(), _lib.ptr(vals64), stream()), "iga_mg_load")
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
         self._form_tail()
-        if self._grid is not None:
+        if self._grid is not None and self._tensor is None:
+            # the tensor set-up writes the stencil layouts itself
             _lib.check(lib().iga_mg_grid_ops(self.struct(), _lib.ptr(self.vals64[0]), stream()),
                        "iga_mg_grid_ops")
         return self
@@ -454,20 +528,27 @@ class MGOperator:
         """The dense tail applies while no level at or below it has a stencil."""
         return self.tail is not None and all(s is None for s in self.stencils[self.tail_level:])
 
-    def set_stencils(self, stencils):
+    def stencil_buffer(self, l):
+        """The persistent stencil buffer of level l (``ks*ks*n_l`` floats); a
+        stencil written there is taken by :meth:`set_stencils` without a copy."""
         import torch
 
+        buf = self._stencil_buf.get(l)
+        if buf is None:
+            buf = torch.empty(self.ks * self.ks * self.hier.levels[l].n, dtype=torch.float32,
+                              device=self.b.device)
+            self._stencil_buf[l] = buf
+        return buf
+
+    def set_stencils(self, stencils):
         new = []
         for l, s in enumerate(list(stencils) + [None] * (self.hier.n_levels - len(stencils))):
             if s is None:
                 new.append(None)
                 continue
-            buf = self._stencil_buf.get(l)
-            if buf is None:
-                buf = torch.empty(self.ks * self.ks * self.hier.levels[l].n, dtype=torch.float32,
-                                  device=self.b.device)
-                self._stencil_buf[l] = buf
-            buf.copy_(s.reshape(-1))
+            buf = self.stencil_buffer(l)
+            if s.data_ptr() != buf.data_ptr():
+                buf.copy_(s.reshape(-1))
             new.append(buf)
         if [x is None for x in new] != [x is None for x in self.stencils]:
             self._struct = None
@@ -481,6 +562,13 @@ class MGOperator:
 
         if self._struct is not None:
             return self._struct
+        # one descriptor per pattern of levels with stencils (their buffers
+        # are persistent), built once
+        key = (tuple(s is None for s in self.stencils), self.tail_active())
+        cached = self.__dict__.setdefault("_structs", {}).get(key)
+        if cached is not None:
+            self._struct = cached
+            return cached
         h = self.hier
         m = _lib.IgaMg()
         m.n_levels = h.n_levels
@@ -508,6 +596,7 @@ class MGOperator:
             self._work = torch.empty(max(nw, 1), dtype=torch.float32, device=self.b.device)
         m.work = self._work.data_ptr()
         self._struct = m
+        self._structs[key] = m
         return m
 
     def vcycle(self, b32, level=0, out=None):

@@ -166,6 +166,17 @@ def test_grid_vcycle_matches_oracle(gpu, refine):
     s0 = rng.normal(size=(49, ref[0]["K"].shape[0])) * 0.05
     s0[24] += 0.5
     b = rng.normal(size=ref[0]["K"].shape[0])
+    op = LN.setup(ctx, vals)
+    # Galerkin levels (batched-load row kernel), level-0 features (written
+    # with the stencil layouts), coarsest inverse (one-block LDS elimination)
+    for l, (lv, rl) in enumerate(zip(op.hier.levels, ref)):
+        K = _scipy(lv.indptr, lv.cols, op.vals64[l], lv.n)
+        assert abs(K - rl["K"]).max() <= 1e-12 * abs(rl["K"]).max()
+    lv0 = op.hier.levels[0]
+    f = LN.features(op, 0)[0].reshape((7 * 7, -1)).double().cpu().numpy()
+    assert _rel(f, OM.stencil_features(ref[0]["K"], lv0.h, lv0.w, 3)) < 1e-6
+    inv = op.coarse_inv.view(ref[-1]["K"].shape).double().cpu().numpy()
+    assert _rel(inv, np.linalg.inv(ref[-1]["K"].toarray())) < 1e-5
     for st in (None, [s0] + [None] * (len(ref) - 2)):
         op = LN.setup(ctx, vals)
         if st is not None:
