@@ -406,6 +406,16 @@ typedef struct iga_mg_setup_grid {
   float* op32[16];
   float* feats;
   double* work;
+  /* optional (tail_level >= 1): the dense tail of the V-cycle from tail_level
+   * (damped Jacobi with weight omega below it, exact coarsest solve) formed
+   * in FP64 from the stencils, V_l = 2M - MAM + G V_{l+1} G^T, G = (I - MA)P,
+   * into tail (n x iga_mg_dense_ld(n) FP32), with the coarse inverse; the
+   * coarsest level has <= 48 dofs and level tail_level+1 <= 128.
+   * tail_work: iga_mg_tail_dense_doubles(setup, tail_level) doubles. */
+  int32_t tail_level;
+  double omega;
+  float* tail;
+  double* tail_work;
 } iga_mg_setup_grid_t;
 
 typedef struct iga_mg_setup {
@@ -418,6 +428,7 @@ typedef struct iga_mg_setup {
 } iga_mg_setup_t;
 
 int iga_mg_load(const iga_mg_setup_t* setup, const double* vals0, void* stream);
+int64_t iga_mg_tail_dense_doubles(const iga_mg_setup_t* setup, int32_t level);
 
 int64_t iga_mg_work_floats(const iga_mg_t* mg);
 /* Stencil layout of the FP32 level operators 0 .. mg->grid->n_levels-1 into

@@ -83,7 +83,8 @@ class IgaMgSetupLevel(ctypes.Structure):
 
 class IgaMgSetupGrid(ctypes.Structure):
     _fields_ = [("f_lo", (c_void_p * 2) * 16), ("rw", (c_void_p * 2) * 16), ("op64", c_void_p * 16),
-                ("op32", c_void_p * 16), ("feats", c_void_p), ("work", c_void_p)]
+                ("op32", c_void_p * 16), ("feats", c_void_p), ("work", c_void_p),
+                ("tail_level", c_int), ("omega", c_double), ("tail", c_void_p), ("tail_work", c_void_p)]
 
 
 class IgaMgSetup(ctypes.Structure):
@@ -161,6 +162,7 @@ _SIGNATURES = {
     "iga_mg_work_floats": (c_i64, [ctypes.POINTER(IgaMg)]),
     "iga_mg_vcycle": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),
     "iga_mg_dense_ld": (c_int, [c_int]),
+    "iga_mg_tail_dense_doubles": (c_i64, [ctypes.POINTER(IgaMgSetup), c_int]),
     "iga_mg_grid_ops": (c_int, [ctypes.POINTER(IgaMg), c_void_p, c_void_p]),
     "iga_mg_tail_floats": (c_i64, [ctypes.POINTER(IgaMg), c_int]),
     "iga_mg_tail": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p]),

@@ -1574,6 +1574,230 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dense tail from the tensor set-up: the V-cycle started at level t (damped
+// Jacobi pre- and post-smoothing with M = w D^-1, exact coarsest solve) is
+//   V_Lc = A_Lc^-1,   V_l = 2 M_l - M_l A_l M_l + G_l V_{l+1} G_l^T,
+//   G_l = (I - M_l A_l) P_l,
+// formed in FP64 from the levels' stencil layouts and transfer bands:
+// k_tail_small (one block) the coarse inverse and V_{t+1}, G_t into scratch;
+// k_tail_rows one wavefront per row of V_t (FP32, padded rows).
+// ---------------------------------------------------------------------------
+
+struct TailLevel {
+  int n, h, w;
+  const double* op;    // (49, n) FP64 stencil
+  const double* dinv;  // D^-1
+  const int32_t* f_lo[2];
+  const double* rw[2];  // coarse-column bands of P_l (l -> l+1), d = 0 along rows
+};
+
+struct TailArgs {
+  int t, lc;  // tail level, coarsest level
+  double omega;
+  TailLevel lv[16];
+  double* work;        // scratch (see tail_work_doubles)
+  float* coarse_inv;   // FP32 inverse of the coarsest level
+};
+
+// P_l[f][c] for fine grid point f and coarse point c of the next level
+__device__ __forceinline__ double p_entry(const TailLevel& L, int wc, int f, int c) {
+  const int fr = f / L.w, fc = f - fr * L.w, cr = c / wc, cc = c - cr * wc;
+  const int a = fr - L.f_lo[1][cr], b = fc - L.f_lo[0][cc];
+  if (a < 0 || a >= kKFC || b < 0 || b >= kKFC) return 0.0;
+  return L.rw[1][cr * kKFC + a] * L.rw[0][cc * kKFC + b];
+}
+
+// A_l[i][j] from the stencil (0 off the 7 x 7 neighbourhood)
+__device__ __forceinline__ double a_entry(const TailLevel& L, int i, int j) {
+  const int ir = i / L.w, ic = i - ir * L.w, jr = j / L.w, jc = j - jr * L.w;
+  const int dr = jr - ir, dc = jc - ic;
+  if (dr < -kGR || dr > kGR || dc < -kGR || dc > kGR) return 0.0;
+  return L.op[(size_t)((dr + kGR) * kGS + dc + kGR) * L.n + i];
+}
+
+// Scratch of the dense tail (doubles): V ping-pong (vmax each), T (vmax), W
+// (n_{t+1} x n_t), and G_l of every level l = t .. lc-1 (n_l x n_{l+1}) at
+// goff[l].
+struct TailWork {
+  int64_t vmax, w_off, goff[16];
+};
+
+// G_l[i][c] = P[i][c] - m_i sum_o A[i][i+o] P[i+o][c] for every level, one
+// thread per entry (blockIdx.y = level - t). The bands of coarse point c are
+// read once; P[f][c] = w1[fr - f_lo1] w0[fc - f_lo0] inside them.
+__device__ __forceinline__ double band_w(const double (&w)[kKFC], int k) {
+  double v = 0.0;
+#pragma unroll
+  for (int q = 0; q < kKFC; ++q) v = k == q ? w[q] : v;
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw) {
+  const int l = a.t + blockIdx.y;
+  if (l >= a.lc) return;
+  const TailLevel& L = a.lv[l];
+  const TailLevel& C = a.lv[l + 1];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L.n * C.n) return;
+  const int i = e / C.n, c = e - i * C.n;
+  const int ir = i / L.w, ic = i - ir * L.w, cr = c / C.w, cc = c - cr * C.w;
+  const int flo1 = L.f_lo[1][cr], flo0 = L.f_lo[0][cc];
+  double w1[kKFC], w0[kKFC], av[kGN];
+#pragma unroll
+  for (int q = 0; q < kKFC; ++q) {
+    w1[q] = L.rw[1][cr * kKFC + q];
+    w0[q] = L.rw[0][cc * kKFC + q];
+  }
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) av[o] = L.op[(size_t)o * L.n + i];
+  const double mi = a.omega * L.dinv[i];
+  __builtin_amdgcn_sched_barrier(0);
+  double s = 0.0;
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) {
+    const int rr = ir + o / kGS - kGR, cc2 = ic + o % kGS - kGR;
+    s += av[o] * band_w(w1, rr - flo1) * band_w(w0, cc2 - flo0);
+  }
+  a.work[tw.goff[l] + e] = band_w(w1, ir - flo1) * band_w(w0, ic - flo0) - mi * s;
+}
+
+// one block: the coarse inverse (Gauss-Jordan in LDS, SPD without pivoting)
+// and V_l = G_l V_{l+1} G_l^T + 2M - M A M for l = lc-1 .. t+1; V_{t+1} ends
+// at the front of the work
+constexpr int kTailLds = 4800;  // doubles of LDS: the elimination, then V_{l+1}, G_l, T
+
+__global__ void __launch_bounds__(1024) k_tail_small(TailArgs a, TailWork tw) {
+  __shared__ double M[kTailLds];
+  __shared__ double fac[kGjLds];
+  const TailLevel& C = a.lv[a.lc];
+  const int n = C.n, n2 = 2 * n;
+  double* Va = a.work;
+  double* Vb = a.work + tw.vmax;
+  double* T = a.work + 2 * tw.vmax;
+  for (int e = threadIdx.x; e < n * n2; e += blockDim.x) {
+    const int i = e / n2, j = e - i * n2;
+    M[e] = j < n ? a_entry(C, i, j) : (j == n + i ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double piv = M[k * n2 + k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) M[k * n2 + j] /= piv;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = M[i * n2 + k];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n2; e += blockDim.x) {
+      const int i = e / n2, j = e - i * n2;
+      if (i != k) M[e] -= fac[i] * M[k * n2 + j];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    Va[e] = M[i * n2 + n + j];
+    a.coarse_inv[e] = (float)M[i * n2 + n + j];
+  }
+  __syncthreads();
+  double* Vc = Va;
+  double* Vn = Vb;
+  for (int l = a.lc - 1; l > a.t; --l) {
+    const TailLevel& L = a.lv[l];
+    const int nl = L.n, nc = a.lv[l + 1].n;
+    const double* g = a.work + tw.goff[l];
+    if (nc * nc + 2 * nl * nc <= kTailLds) {
+      // V_{l+1}, G_l and T in LDS
+      double* sv = M;
+      double* sg = M + nc * nc;
+      double* st2 = sg + nl * nc;
+      for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) sv[e] = Vc[e];
+      for (int e = threadIdx.x; e < nl * nc; e += blockDim.x) sg[e] = g[e];
+      __syncthreads();
+      for (int e = threadIdx.x; e < nl * nc; e += blockDim.x) {
+        const int i = e / nc, k = e - i * nc;
+        double s = 0.0;
+        for (int m = 0; m < nc; ++m) s += sg[i * nc + m] * sv[m * nc + k];
+        st2[e] = s;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < nl * nl; e += blockDim.x) {
+        const int i = e / nl, j = e - i * nl;
+        double s = 0.0;
+        for (int k = 0; k < nc; ++k) s += st2[i * nc + k] * sg[j * nc + k];
+        const double mi = a.omega * L.dinv[i], mj = a.omega * L.dinv[j];
+        s += (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * mj;
+        Vn[e] = s;
+      }
+      __syncthreads();
+      double* tmp = Vc;
+      Vc = Vn;
+      Vn = tmp;
+      continue;
+    }
+    for (int e = threadIdx.x; e < nl * nc; e += blockDim.x) {
+      const int i = e / nc, k = e - i * nc;
+      double s = 0.0;
+#pragma unroll 5
+      for (int m = 0; m < nc; ++m) s += g[i * nc + m] * Vc[m * nc + k];
+      T[e] = s;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nl * nl; e += blockDim.x) {
+      const int i = e / nl, j = e - i * nl;
+      double s = 0.0;
+#pragma unroll 5
+      for (int k = 0; k < nc; ++k) s += T[i * nc + k] * g[j * nc + k];
+      const double mi = a.omega * L.dinv[i], mj = a.omega * L.dinv[j];
+      s += (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * mj;
+      Vn[e] = s;
+    }
+    __syncthreads();
+    double* tmp = Vc;
+    Vc = Vn;
+    Vn = tmp;
+  }
+  if (Vc != Va) {
+    const int nc = a.lv[a.t + 1].n;
+    for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) Va[e] = Vc[e];
+  }
+}
+
+// W = V_{t+1} G_t^T (n_{t+1} x n_t), one thread per entry
+__global__ void __launch_bounds__(256) k_tail_w(TailArgs a, TailWork tw) {
+  const int nt = a.lv[a.t].n, nc = a.lv[a.t + 1].n;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nc * nt) return;
+  const int k = e / nt, j = e - k * nt;
+  const double* V = a.work;
+  const double* g = a.work + tw.goff[a.t];
+  double s = 0.0;
+#pragma unroll 9
+  for (int m = 0; m < nc; ++m) s += V[k * nc + m] * g[j * nc + m];
+  a.work[tw.w_off + e] = s;
+}
+
+// V_t[i][j] = sum_k G_t[i][k] W[k][j] + 2M - M A M into the tail (FP32, ld
+// floats per row, zero beyond n_t), one thread per entry
+__global__ void __launch_bounds__(256) k_tail_v(TailArgs a, TailWork tw, float* __restrict__ tail,
+                                                int ld) {
+  const TailLevel& L = a.lv[a.t];
+  const int nt = L.n, nc = a.lv[a.t + 1].n;
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ld) return;
+  float out = 0.f;
+  if (j < nt) {
+    const double* g = a.work + tw.goff[a.t] + (size_t)i * nc;
+    const double* W = a.work + tw.w_off;
+    double s = 0.0;
+#pragma unroll 9
+    for (int k = 0; k < nc; ++k) s += g[k] * W[(size_t)k * nt + j];
+    const double mi = a.omega * L.dinv[i];
+    s += (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * a.omega * L.dinv[j];
+    out = (float)s;
+  }
+  tail[(size_t)i * ld + j] = out;
+}
+
 static inline GLevel glevel(const iga_mg_t& mg, int l) {
   const iga_mg_level_t& L = mg.level[l];
   return GLevel{L.h, L.w, mg.grid->op[l], L.dsq, L.omega};
@@ -1993,6 +2217,71 @@ extern "C" int iga_masked_spgemm(int64_t nnz_c, const int32_t* c_rows, const int
   return IGA_OK;
 }
 
+// scratch layout of the dense tail (TailWork)
+static TailWork tail_layout(const iga_mg_setup_t& s, int t, int64_t* total) {
+  TailWork w{};
+  w.vmax = 1;
+  for (int l = t + 1; l < s.n_levels; ++l)
+    w.vmax = std::max<int64_t>(w.vmax, (int64_t)s.level[l].n * s.level[l].n);
+  int64_t o = 3 * w.vmax;
+  w.w_off = o;
+  o += (int64_t)s.level[t + 1].n * s.level[t].n;
+  for (int l = t; l + 1 < s.n_levels; ++l) {
+    w.goff[l] = o;
+    o += (int64_t)s.level[l].n * s.level[l + 1].n;
+  }
+  if (total) *total = o;
+  return w;
+}
+
+static int64_t tail_dense_doubles(const iga_mg_setup_t& s, int t) {
+  int64_t total = 0;
+  tail_layout(s, t, &total);
+  return total;
+}
+
+extern "C" int64_t iga_mg_tail_dense_doubles(const iga_mg_setup_t* s, int32_t level) {
+  if (!s || level < 1 || level >= s->n_levels - 1) return -1;
+  return tail_dense_doubles(*s, level);
+}
+
+// the dense tail of level t from the tensor set-up (after the Galerkin passes)
+static int tail_dense(const iga_mg_setup_t& s, hipStream_t st) {
+  const iga_mg_setup_grid_t& G = *s.grid;
+  const int t = G.tail_level, lc = s.n_levels - 1;
+  if (t < 1 || t >= lc || !G.tail || !G.tail_work || !s.coarse_inv) return IGA_ERR_ARG;
+  if (s.level[lc].n > kGjLds || s.level[t + 1].n > 128 || s.level[t].n > kDenseMaxRows)
+    return IGA_ERR_UNSUPPORTED;
+  TailArgs a{};
+  a.t = t;
+  a.lc = lc;
+  a.omega = G.omega;
+  for (int l = t; l <= lc; ++l) {
+    TailLevel& L = a.lv[l];
+    L.n = s.level[l].n;
+    L.h = s.level[l].h;
+    L.w = s.level[l].w;
+    L.op = G.op64[l];
+    L.dinv = s.level[l].dinv;
+    for (int d = 0; d < 2; ++d) {
+      L.f_lo[d] = l < lc ? G.f_lo[l][d] : nullptr;
+      L.rw[d] = l < lc ? G.rw[l][d] : nullptr;
+    }
+  }
+  a.work = G.tail_work;
+  a.coarse_inv = s.coarse_inv;
+  const TailWork tw = tail_layout(s, t, nullptr);
+  int gmax = 1;
+  for (int l = t; l < lc; ++l) gmax = std::max(gmax, s.level[l].n * s.level[l + 1].n);
+  const int nt = s.level[t].n, nc = s.level[t + 1].n, ld = dense_ld(nt);
+  hipLaunchKernelGGL(k_tail_g, dim3((gmax + 255) / 256, lc - t), dim3(256), 0, st, a, tw);
+  hipLaunchKernelGGL(k_tail_small, dim3(1), dim3(1024), 0, st, a, tw);
+  hipLaunchKernelGGL(k_tail_w, dim3((nc * nt + 255) / 256), dim3(256), 0, st, a, tw);
+  hipLaunchKernelGGL(k_tail_v, dim3((ld + 255) / 256, nt), dim3(256), 0, st, a, tw, G.tail, ld);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
 extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* stream) {
   if (!s || s->n_levels < 1 || s->n_levels > 16 || !vals0 || !s->coarse_inv || !s->inv_work)
     return IGA_ERR_ARG;
@@ -2059,6 +2348,9 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     }
   }
   IGA_CHECK_LAUNCH();
+  // with the tensor set-up and a tail level, the dense tail (and with it the
+  // coarse inverse) comes straight from the stencils
+  if (G && G->tail_level > 0) return tail_dense(*s, st);
   const iga_mg_setup_level_t& C = s->level[L - 1];
   return iga_dense_inverse(C.n, C.indptr, C.cols, C.vals64, s->inv_work, s->coarse_inv, stream);
 }

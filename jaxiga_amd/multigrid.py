@@ -326,6 +326,9 @@ _FREE_KEYS = weakref.WeakKeyDictionary()
 # launches; formed by batched cycles, whose cost grows with the square of the
 # size); the dense product takes at most 2048 rows
 TAIL_ROWS = 300
+# the dense tail formed from the stencils: coarsest level and the level below
+# the tail at most this large
+DENSE_TAIL_COARSE, DENSE_TAIL_NEXT = 48, 128
 
 
 def _free_key(dofmap):
@@ -467,9 +470,17 @@ class MGOperator:
                     sg.rw[l][d] = per_dir[d][1].data_ptr()
             sg.feats = self.grid_feats.data_ptr()
             sg.work = self._gal_work.data_ptr()
+            # the dense tail straight from the stencils (small levels below it)
+            t = self.tail_level
+            if (t > 0 and lv[-1].n <= DENSE_TAIL_COARSE and lv[t + 1].n <= DENSE_TAIL_NEXT):
+                sg.tail_level = t
+                sg.omega = self.omega
+                sg.tail = self.tail.data_ptr()
             self._tensor = sg
 
     def _setup_struct(self):
+        import torch
+
         if getattr(self, "_setup", None) is None:
             h = self.hier
             m = _lib.IgaMgSetup()
@@ -488,6 +499,10 @@ class MGOperator:
             m.inv_work = self._inv_work.data_ptr()
             if self._tensor is not None:
                 m.grid = _lib.ctypes.pointer(self._tensor)
+                if self._tensor.tail_level > 0:
+                    nw = int(lib().iga_mg_tail_dense_doubles(m, self._tensor.tail_level))
+                    self._tail_dense_work = torch.empty(nw, dtype=torch.float64, device=self.b.device)
+                    self._tensor.tail_work = self._tail_dense_work.data_ptr()
             self._setup = m
         return self._setup
 
@@ -517,6 +532,8 @@ class MGOperator:
 
         if self.tail is None or not self.tail_active():
             return
+        if self._tensor is not None and self._tensor.tail_level > 0:
+            return  # formed by iga_mg_load from the stencils
         m = self.struct()
         if self._tail_work is None:
             nw = int(lib().iga_mg_tail_floats(m, self.tail_level))
