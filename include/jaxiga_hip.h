@@ -416,6 +416,10 @@ typedef struct iga_mg_setup_grid {
   double omega;
   float* tail;
   double* tail_work;
+  /* regular[l] != 0: every CSR row of level l holds exactly the in-grid
+   * points of its 7 x 7 window (checked by the caller; values are then
+   * written at computed slots) */
+  int32_t regular[16];
 } iga_mg_setup_grid_t;
 
 typedef struct iga_mg_setup {

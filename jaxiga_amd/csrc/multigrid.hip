@@ -1361,6 +1361,11 @@ struct GridOpsArgs {
   const double* vals64;
   double* op64;
   float* feats;  // level 0: K[i, i + off] / K[i, i] (FP32), with op64
+  // level 0 from vals64 alone (vals[0] null): its CSR copies and scalings
+  double* csr64;  // optional copy of vals64
+  float* csr32;
+  double* dinv;
+  float* dsq;
 };
 
 constexpr int kOpsRows = 64;
@@ -1378,9 +1383,10 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
   if (i0 >= n) return;
   const int i1 = min(i0 + kOpsRows, n);
   const int32_t* cols = a.cols[l];
-  const float* vals = a.vals[l];
+  const float* vals = a.vals[l];  // null: the FP32 values from vals64 (level 0)
   const int lo = a.indptr[l][i0], cnt = a.indptr[l][i1] - lo;
   const bool f64 = l == 0 && a.op64;
+  const bool prep = l == 0 && !vals;
   const bool fits = cnt <= kOpsMax;
   if (threadIdx.x <= i1 - i0) sp[threadIdx.x] = a.indptr[l][i0 + threadIdx.x] - lo;
   if (fits && cnt > 0) {
@@ -1392,8 +1398,8 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     for (int k = 0; k < NP; ++k) {
       const int t = min((int)threadIdx.x + k * kGTh, max(cnt - 1, 0));
       c[k] = cols[lo + t];
-      v[k] = vals[lo + t];
       v64[k] = f64 ? a.vals64[lo + t] : 0.0;
+      v[k] = prep ? (float)v64[k] : vals[lo + t];
     }
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -1402,10 +1408,30 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
         sc[t] = c[k];
         sv[t] = v[k];
         sv64[t] = v64[k];
+        if (prep) {
+          if (a.csr64) a.csr64[lo + t] = v64[k];
+          a.csr32[lo + t] = v[k];
+        }
       }
+    }
+  } else if (prep) {
+    for (int t = threadIdx.x; t < cnt; t += kGTh) {
+      const double v64 = a.vals64[lo + t];
+      if (a.csr64) a.csr64[lo + t] = v64;
+      a.csr32[lo + t] = (float)v64;
     }
   }
   __syncthreads();
+  if (prep && (int)threadIdx.x < i1 - i0) {
+    // the row's scalings: 1 / K_ii (1 where zero or absent), D^-1/2
+    const int i = i0 + threadIdx.x;
+    const int q0 = lo + sp[threadIdx.x], q1 = lo + sp[threadIdx.x + 1];
+    const int k = lower_bound(cols, q0, q1, i);
+    const double d = (k < q1 && cols[k] == i) ? a.vals64[k] : 0.0;
+    const double di = d != 0.0 ? 1.0 / d : 1.0;
+    a.dinv[i] = di;
+    a.dsq[i] = (float)sqrt(fabs(di));
+  }
   for (int t = threadIdx.x; t < kGN * kOpsRows; t += kGTh) {
     const int il = t % kOpsRows, o = t / kOpsRows;
     const int i = i0 + il;
@@ -1428,8 +1454,8 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
       double v64 = 0.0;
       for (int q = lo + s0; q < lo + e0; ++q)
         if (cols[q] == j) {
-          v32 = vals[q];
           v64 = f64 ? a.vals64[q] : 0.0;
+          v32 = prep ? (float)v64 : vals[q];
         }
       return v64;
     };
@@ -1484,94 +1510,109 @@ __device__ __forceinline__ void gal_close(const double (&u)[kGalWin], int f0, in
   }
 }
 
-// thread (dr, r, C), C fastest
-__global__ void __launch_bounds__(256)
-    k_gal_pass0(int hf, int wf, int wc, const int32_t* __restrict__ f_lo,
-                const double* __restrict__ rw, const double* __restrict__ K,
-                double* __restrict__ Kp) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t np = (int64_t)hf * wc;
-  if (t >= kGS * np) return;
-  const int C = (int)(t % wc), r = (int)((t / wc) % hf), dr = (int)(t / np) - kGR;
-  const int64_t nf = (int64_t)hf * wf;
-  const int f0 = f_lo[C];
+// One 1-D Galerkin step of a level (see above): the arguments of both passes.
+struct GalLevel {
+  int hf, wf, hc, wc;
+  const int32_t* f_lo[2];
+  const double* rw[2];
+  const double* K;  // fine FP64 stencil
+  double* Kp;       // intermediate
+  double* S64;      // coarse FP64 stencil
+  float* S32;       // coarse FP32 stencil (optional)
+  const int32_t* indptr;
+  const int32_t* cols;
+  double* vals64;
+  float* vals32;
+  double* dinv;
+  float* dsq;
+  int regular;  // every coarse CSR row is exactly its clipped 7 x 7 window
+};
+
+// pass 0 entry t = (dr, r, C), C fastest
+__device__ __forceinline__ void gal_pass0_entry(const GalLevel& g, int64_t t) {
+  const int64_t np = (int64_t)g.hf * g.wc;
+  const int C = (int)(t % g.wc), r = (int)((t / g.wc) % g.hf), dr = (int)(t / np) - kGR;
+  const int64_t nf = (int64_t)g.hf * g.wf;
+  const int f0 = g.f_lo[0][C];
   double u[kGalWin];
 #pragma unroll
   for (int j = 0; j < kGalWin; ++j) u[j] = 0.0;
 #pragma unroll
   for (int a = 0; a < kKFC; ++a) {
-    const double wa = rw[C * kKFC + a];
-    const int64_t idx = (int64_t)r * wf + min(f0 + a, wf - 1);
+    const double wa = g.rw[0][C * kKFC + a];
+    const int64_t idx = (int64_t)r * g.wf + min(f0 + a, g.wf - 1);
 #pragma unroll
     for (int dc = -kGR; dc <= kGR; ++dc)
-      u[a + dc + kGR] += wa * K[((dr + kGR) * kGS + dc + kGR) * nf + idx];
+      u[a + dc + kGR] += wa * g.K[((dr + kGR) * kGS + dc + kGR) * nf + idx];
   }
   double out[kGS];
-  gal_close(u, f0, C, wc, f_lo, rw, out);
+  gal_close(u, f0, C, g.wc, g.f_lo[0], g.rw[0], out);
 #pragma unroll
-  for (int d = 0; d < kGS; ++d) Kp[((dr + kGR) * kGS + d) * np + (int64_t)r * wc + C] = out[d];
+  for (int d = 0; d < kGS; ++d) g.Kp[((dr + kGR) * kGS + d) * np + (int64_t)r * g.wc + C] = out[d];
 }
 
-// thread (dC, R, C), C fastest: the coarse stencil line (., dC) of point
-// (R, C) in FP64 (and FP32) stencil layout, its CSR values and, for dC = 0,
-// the diagonal scalings (1 where the diagonal is zero)
-__global__ void __launch_bounds__(256)
-    k_gal_pass1(int hf, int hc, int wc, const int32_t* __restrict__ f_lo,
-                const double* __restrict__ rw, const double* __restrict__ Kp,
-                double* __restrict__ S64, float* __restrict__ S32,
-                const int32_t* __restrict__ indptr, const int32_t* __restrict__ cols,
-                double* __restrict__ vals64, float* __restrict__ vals32,
-                double* __restrict__ dinv, float* __restrict__ dsq) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nc = (int64_t)hc * wc, np = (int64_t)hf * wc;
-  if (t >= kGS * nc) return;
-  const int C = (int)(t % wc), R = (int)((t / wc) % hc), dC = (int)(t / nc) - kGR;
-  const int f0 = f_lo[R];
+// pass 1 entry t = (dC, R, C), C fastest: the coarse stencil line (., dC) of
+// point (R, C) in FP64 (and FP32) stencil layout, its CSR values and, for
+// dC = 0, the diagonal scalings (1 where the diagonal is zero)
+__device__ __forceinline__ void gal_pass1_entry(const GalLevel& g, int64_t t) {
+  const int64_t nc = (int64_t)g.hc * g.wc, np = (int64_t)g.hf * g.wc;
+  const int C = (int)(t % g.wc), R = (int)((t / g.wc) % g.hc), dC = (int)(t / nc) - kGR;
+  const int f0 = g.f_lo[1][R];
   double u[kGalWin];
 #pragma unroll
   for (int j = 0; j < kGalWin; ++j) u[j] = 0.0;
 #pragma unroll
   for (int a = 0; a < kKFC; ++a) {
-    const double wa = rw[R * kKFC + a];
-    const int64_t idx = (int64_t)min(f0 + a, hf - 1) * wc + C;
+    const double wa = g.rw[1][R * kKFC + a];
+    const int64_t idx = (int64_t)min(f0 + a, g.hf - 1) * g.wc + C;
 #pragma unroll
     for (int dr = -kGR; dr <= kGR; ++dr)
-      u[a + dr + kGR] += wa * Kp[((dr + kGR) * kGS + dC + kGR) * np + idx];
+      u[a + dr + kGR] += wa * g.Kp[((dr + kGR) * kGS + dC + kGR) * np + idx];
   }
   double out[kGS];
-  gal_close(u, f0, R, hc, f_lo, rw, out);
-  const int I = R * wc + C;
-  const bool cin = C + dC >= 0 && C + dC < wc;
-  const int lo = indptr[I], hi = indptr[I + 1];
+  gal_close(u, f0, R, g.hc, g.f_lo[1], g.rw[1], out);
+  const int I = R * g.wc + C;
+  const bool cin = C + dC >= 0 && C + dC < g.wc;
+  const int lo = g.indptr[I], hi = g.indptr[I + 1];
   const int dR0 = max(-kGR, -R), dC0 = max(-kGR, -C);
-  const int ncol = min(kGR, wc - 1 - C) - dC0 + 1;
+  const int ncol = min(kGR, g.wc - 1 - C) - dC0 + 1;
 #pragma unroll
   for (int d = 0; d < kGS; ++d) {
     const int dR = d - kGR;
     const double v = cin ? out[d] : 0.0;
     const int o = d * kGS + dC + kGR;
-    S64[(int64_t)o * nc + I] = v;
-    if (S32) S32[(int64_t)o * nc + I] = (float)v;
-    if (cin && R + dR >= 0 && R + dR < hc) {
-      const int J = I + dR * wc + dC;
+    g.S64[(int64_t)o * nc + I] = v;
+    if (g.S32) g.S32[(int64_t)o * nc + I] = (float)v;
+    if (cin && R + dR >= 0 && R + dR < g.hc) {
+      const int J = I + dR * g.wc + dC;
       int k = lo + (dR - dR0) * ncol + (dC - dC0);
-      if (k >= hi || cols[k] != J) {
+      if (!g.regular && (k >= hi || g.cols[k] != J)) {
         k = -1;
         for (int q = lo; q < hi; ++q)
-          if (cols[q] == J) k = q;
+          if (g.cols[q] == J) k = q;
       }
       if (k >= 0) {
-        vals64[k] = v;
-        vals32[k] = (float)v;
+        g.vals64[k] = v;
+        g.vals32[k] = (float)v;
       }
     }
   }
   if (dC == 0) {
     const double dv = out[kGR];
     const double di = dv != 0.0 ? 1.0 / dv : 1.0;
-    dinv[I] = di;
-    dsq[I] = (float)sqrt(fabs(di));
+    g.dinv[I] = di;
+    g.dsq[I] = (float)sqrt(fabs(di));
   }
+}
+
+__global__ void __launch_bounds__(256) k_gal_pass0(GalLevel g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (int64_t)kGS * g.hf * g.wc) gal_pass0_entry(g, t);
+}
+
+__global__ void __launch_bounds__(256) k_gal_pass1(GalLevel g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (int64_t)kGS * g.hc * g.wc) gal_pass1_entry(g, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -2294,16 +2335,17 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     if (l + 1 < L && (!V.pt_ptr || !V.pt_cols || !V.pt_vals || V.w <= 0)) return IGA_ERR_ARG;
   }
   if (s->level[L - 1].n > 4096) return IGA_ERR_SHAPE;
-  {
+  const iga_mg_setup_grid_t* G = s->grid;
+  if (!G) {
     // the finest level's copy, FP32 copy and scalings in one pass; the
     // Galerkin kernel writes those of every coarser level with its values
+    // (with the tensor set-up the stencil conversion writes them)
     const iga_mg_setup_level_t& V = s->level[0];
     const int64_t m = std::max<int64_t>(V.nnz, V.n);
     hipLaunchKernelGGL(k_level0_prep, dim3((int)std::min<int64_t>(blocks_for(m), 8192)), dim3(256), 0,
                        st, V.n, (int64_t)V.nnz, V.indptr, V.cols, vals0,
                        V.vals64 != vals0 ? V.vals64 : nullptr, V.vals32, V.dinv, V.dsq);
   }
-  const iga_mg_setup_grid_t* G = s->grid;
   if (G) {
     for (int l = 0; l < L; ++l) {
       const iga_mg_setup_level_t& V = s->level[l];
@@ -2320,7 +2362,11 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     a.w[0] = V.w;
     a.indptr[0] = V.indptr;
     a.cols[0] = V.cols;
-    a.vals[0] = V.vals32;
+    a.vals[0] = nullptr;  // from vals0: the CSR copies and scalings too
+    a.csr64 = V.vals64 != vals0 ? V.vals64 : nullptr;
+    a.csr32 = V.vals32;
+    a.dinv = V.dinv;
+    a.dsq = V.dsq;
     float* op32_0 = G->op32[0];
     if (!op32_0) return IGA_ERR_ARG;
     a.op[0] = op32_0;
@@ -2328,15 +2374,38 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     a.op64 = G->op64[0];
     a.feats = G->feats;
     hipLaunchKernelGGL(k_grid_ops, dim3((V.n + kOpsRows - 1) / kOpsRows, 1), dim3(kGTh), 0, st, a);
+    auto gal = [&](int l) {
+      const iga_mg_setup_level_t& F = s->level[l];
+      const iga_mg_setup_level_t& C = s->level[l + 1];
+      GalLevel g{};
+      g.hf = F.h;
+      g.wf = F.w;
+      g.hc = C.h;
+      g.wc = C.w;
+      for (int d = 0; d < 2; ++d) {
+        g.f_lo[d] = G->f_lo[l][d];
+        g.rw[d] = G->rw[l][d];
+      }
+      g.K = G->op64[l];
+      g.Kp = G->work;
+      g.S64 = G->op64[l + 1];
+      g.S32 = G->op32[l + 1];
+      g.indptr = C.indptr;
+      g.cols = C.cols;
+      g.vals64 = C.vals64;
+      g.vals32 = C.vals32;
+      g.dinv = C.dinv;
+      g.dsq = C.dsq;
+      g.regular = G->regular[l + 1];
+      return g;
+    };
     for (int l = 0; l + 1 < L; ++l) {
       const iga_mg_setup_level_t& F = s->level[l];
       const iga_mg_setup_level_t& C = s->level[l + 1];
       const int64_t n0 = (int64_t)kGS * F.h * C.w, n1 = (int64_t)kGS * C.n;
-      hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, F.h, F.w,
-                         C.w, G->f_lo[l][0], G->rw[l][0], G->op64[l], G->work);
-      hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, F.h, C.h,
-                         C.w, G->f_lo[l][1], G->rw[l][1], G->work, G->op64[l + 1], G->op32[l + 1],
-                         C.indptr, C.cols, C.vals64, C.vals32, C.dinv, C.dsq);
+      const GalLevel g = gal(l);
+      hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, g);
+      hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, g);
     }
   } else {
     for (int l = 0; l + 1 < L; ++l) {

@@ -85,6 +85,7 @@ class LevelStructure:
     indptr: object
     cols: object
     rows: object
+    regular: bool = False  # CSR rows are exactly the clipped 7 x 7 windows
     P: object = None  # host CSR (n x n_coarse)
     P1: tuple = None  # the 1-D factors of P (direction 0, direction 1), P = P1[1] (x) P1[0]
     radius: int = 0  # largest grid distance the level operator couples
@@ -160,7 +161,8 @@ class Hierarchy:
             fine.t_pat = _csr_dev(Tpat, dev)
             hc, wc = P1[1].shape[1], P1[0].shape[1]
             ip, cc, rr = _csr_dev(Kc, dev)
-            levels.append(LevelStructure(Kc.shape[0], hc, wc, ck, ip, cc, rr))
+            levels.append(LevelStructure(Kc.shape[0], hc, wc, ck, ip, cc, rr,
+                                         regular=regular_pattern(Kc.indptr, Kc.indices, hc, wc)))
             K = Kc
         Kcoo = K.tocoo()
         w_last = levels[-1].w
@@ -300,6 +302,28 @@ def band_spans_fit(c_lo, pw, f_lo, rw, level0_dir=None, tile=16, halo=3):
         if c_lo[z] + KPF - c_lo[a] > span:
             return False
     return True
+
+
+def regular_pattern(indptr, cols, h, w, radius=GRID_RADIUS):
+    """Whether every CSR row (sorted columns) holds exactly the in-grid points
+    of its (2 radius + 1)^2 window."""
+    indptr = np.asarray(indptr)
+    cols = np.asarray(cols)
+    n = h * w
+    if len(indptr) != n + 1:
+        return False
+    i = np.arange(n)
+    r, c = i // w, i % w
+    count = ((np.minimum(radius, h - 1 - r) - np.maximum(-radius, -r) + 1)
+             * (np.minimum(radius, w - 1 - c) - np.maximum(-radius, -c) + 1))
+    if not np.array_equal(np.diff(indptr), count):
+        return False
+    rows = np.repeat(i, count)
+    dr, dc = cols // w - rows // w, cols % w - rows % w
+    if np.abs(dr).max(initial=0) > radius or np.abs(dc).max(initial=0) > radius:
+        return False
+    key = rows.astype(np.int64) * n + cols
+    return bool(np.all(np.diff(key) > 0))
 
 
 def tensor_coupling_ok(c_lo, pw, f_lo, rw):
@@ -470,6 +494,8 @@ class MGOperator:
                     sg.rw[l][d] = per_dir[d][1].data_ptr()
             sg.feats = self.grid_feats.data_ptr()
             sg.work = self._gal_work.data_ptr()
+            for l in range(len(lv)):
+                sg.regular[l] = 1 if lv[l].regular else 0
             # the dense tail straight from the stencils (small levels below it)
             t = self.tail_level
             if (t > 0 and lv[-1].n <= DENSE_TAIL_COARSE and lv[t + 1].n <= DENSE_TAIL_NEXT):
