@@ -497,9 +497,11 @@ int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* i
                     double* work, float* work32, int32_t graph_iters, void* stream,
                     iga_fcg_plan_t** out);
 /* out_res (optional): ||b - A x|| recomputed in FP64 from the returned x
- * (independent of the recursion), read back with the solver state */
+ * (independent of the recursion), read back with the solver state;
+ * x_out (optional): n doubles receiving x, copied on the stream behind the
+ * iterations (the plan's x is overwritten by its next run) */
 int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter, int32_t* out_iters,
-                   double* out_rr, double* out_bb, double* out_res, void* stream);
+                   double* out_rr, double* out_bb, double* out_res, double* x_out, void* stream);
 void iga_fcg_mg_destroy(iga_fcg_plan_t* plan);
 int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32_t* cols,
                const double* vals, const double* b, double* x, double tol, int32_t maxiter,

@@ -352,7 +352,8 @@ __device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip;
 // Device state of a flexible-CG solve (scalars never leave the device).
 struct FState {
   double rr, bb, tol2, pq_prev;
-  int iter, maxiter, done, pad;
+  int iter, maxiter, done;
+  int pending;  // an update x += alpha p, r -= alpha q waits (grid cycle)
   double res2;  // ||b - A x||^2 recomputed from x after the solve
 };
 
@@ -366,6 +367,7 @@ struct Gate {
   const double* rr;  // r.r partials (null: only `flag`)
   int n_rr;
   FState* st;
+  int clear_pending;  // block 0 also clears st->pending (the update was applied)
 };
 
 __device__ __forceinline__ bool gate_closed(const Gate& g) {
@@ -376,6 +378,7 @@ __device__ __forceinline__ bool gate_closed(const Gate& g) {
       g.st->rr = rr;
       g.st->done = done ? 1 : 0;
       if (!done) g.st->iter += 1;
+      if (g.clear_pending) g.st->pending = 0;
     }
     return done;
   }
@@ -1151,11 +1154,111 @@ __global__ void __launch_bounds__(kGTh)
   if (T.in) x[T.it] = v;
 }
 
+// Flexible-CG state for the level-0 kernel that applies the waiting update
+// of the previous iteration: alpha = (p.r)/(p.q) from the product's
+// partials, r_new = r_old - alpha q (ping-pong buffers: the halo reads r_old),
+// x += alpha p, r32 = (float) r_new and the partials of r_new . r_new.
+struct FcgFuse {
+  FState* st;
+  const double* r_old;
+  double* r_new;
+  const double* p;
+  const double* q;
+  double* x;
+  float* r32;
+  const double* P_b;  // p.r partials of the product
+  const double* P_c;  // p.q partials
+  double* P_d;        // r.r partials out (one per tile)
+  int nb;             // partials of the product
+};
+
+// the update (when one waits), then x0 = S b0 with b0 = r32 on the box
+template <bool STW>
+__global__ void __launch_bounds__(kGTh)
+    k_g0_smooth_upd(GLevel g, const float* __restrict__ W, FcgFuse f, float* __restrict__ x0) {
+  __shared__ float ub[kL0HH][kL0HW];
+  const int n = g.h * g.w;
+  const L0Tile T = l0_tile(g.h, g.w);
+  float a[kGN];
+  if (STW)
+#pragma unroll
+    for (int o = 0; o < kGN; ++o) a[o] = W[(size_t)o * n + T.it];
+  const float dt = g.dsq[T.it];
+  const double pt = f.p[T.it], xt = f.x[T.it];
+  BoxLoad<false, double> rl, ql;
+  BoxLoad<false, float> dl;
+  rl.load(f.r_old, nullptr, g.h, g.w, T.r0, T.c0);
+  ql.load(f.q, nullptr, g.h, g.w, T.r0, T.c0);
+  dl.load(g.dsq, nullptr, g.h, g.w, T.r0, T.c0);
+  __builtin_amdgcn_sched_barrier(0);
+  if (f.st->done) return;
+  const bool pend = f.st->pending != 0;
+  double alpha = 0.0, pq = 0.0;
+  if (pend) {
+    pq = reduce_partials(f.P_c, f.nb);
+    alpha = reduce_partials(f.P_b, f.nb) / pq;
+  }
+#pragma unroll
+  for (int k = 0; k < BoxLoad<false, float>::NP; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const double rn = rl.v[k] - alpha * ql.v[k];
+    if (t < kL0Box) ub[t / kL0HW][t % kL0HW] = rl.in[k] ? dl.v[k] * (float)rn : 0.f;
+  }
+  __syncthreads();
+  float v;
+  if (STW)
+    v = dt * tap_sum<kL0HW>(a, ub, T.ty + kGR, T.tx + kGR);
+  else
+    v = g.omega * dt * ub[T.ty + kGR][T.tx + kGR];
+  double rr = 0.0;
+  if (T.in) {
+    x0[T.it] = v;
+    if (pend) f.x[T.it] = xt + alpha * pt;
+  }
+  // the FP64 new residual of the tile point
+  {
+    const double ro = f.r_old[T.it], qo = f.q[T.it];
+    const double rn = ro - alpha * qo;
+    if (T.in) {
+      f.r_new[T.it] = rn;
+      f.r32[T.it] = (float)rn;
+      rr = rn * rn;
+    }
+  }
+  store_partial_at(f.P_d, blockIdx.y * gridDim.x + blockIdx.x, rr);
+  if (pend && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) f.st->pq_prev = pq;
+}
+
+// the waiting update in place at the end of a replay (each point its own),
+// with the r.r partials for the check that follows
+__global__ void __launch_bounds__(kGTh) k_fcg_flush(int h, int w, FcgFuse f) {
+  const L0Tile T = l0_tile(h, w);
+  if (f.st->done) return;
+  const bool pend = f.st->pending != 0;
+  double alpha = 0.0, pq = 0.0;
+  if (pend) {
+    pq = reduce_partials(f.P_c, f.nb);
+    alpha = reduce_partials(f.P_b, f.nb) / pq;
+  }
+  double rr = 0.0;
+  if (T.in) {
+    const double rn = f.r_old[T.it] - alpha * f.q[T.it];
+    if (pend) {
+      f.r_new[T.it] = rn;
+      f.r32[T.it] = (float)rn;
+      f.x[T.it] += alpha * f.p[T.it];
+    }
+    rr = rn * rn;
+  }
+  store_partial_at(f.P_d, blockIdx.y * gridDim.x + blockIdx.x, rr);
+  if (pend && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) f.st->pq_prev = pq;
+}
+
 // r0 = b0 - A0 x0
 __global__ void __launch_bounds__(kGTh)
     k_g0_resid(GLevel g, const float* __restrict__ x, const float* __restrict__ b,
-               float* __restrict__ r, const int* __restrict__ skip) {
-  if (skipped(skip)) return;
+               float* __restrict__ r, Gate gate) {
+  if (gate_closed(gate)) return;
   __shared__ float xb[kL0HH][kL0HW];
   const int n = g.h * g.w;
   const L0Tile T = l0_tile(g.h, g.w);
@@ -1342,6 +1445,8 @@ __global__ void __launch_bounds__(kGTh)
   const int blk = blockIdx.y * gridDim.x + blockIdx.x;
   store_partial_at(P_c, blk, T.in ? pi * qi : 0.0);
   store_partial_at(P_b, blk, T.in ? pi * ri : 0.0);
+  // the update x += alpha p, r -= alpha q waits for the next cycle (or the flush)
+  if (blk == 0 && threadIdx.x == 0) st->pending = 1;
 }
 
 // stencil layout of the level operators (blockIdx.y = level; the FP64 copy of
@@ -1879,7 +1984,8 @@ static inline dim3 row_tiles(const iga_mg_level_t& L) {
 
 // Per-level buffers of the grid cycle: x (pre-smoothed), b, r, x (post)
 static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate& gate,
-                       const DotOut& dot, int* dot_blocks, hipStream_t st) {
+                       const DotOut& dot, int* dot_blocks, hipStream_t st,
+                       const FcgFuse* fz = nullptr) {
   const int t = mg.tail_level;
   float* X[16];
   float* B[16];
@@ -1897,11 +2003,22 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
   const iga_mg_level_t& L0 = mg.level[0];
   const GLevel g0 = glevel(mg, 0);
   const dim3 t0 = row_tiles(L0);
-  if (L0.stencil)
-    hipLaunchKernelGGL(k_g0_smooth<true>, t0, dim3(kGTh), 0, st, g0, L0.stencil, b0, X[0], gate);
-  else
-    hipLaunchKernelGGL(k_g0_smooth<false>, t0, dim3(kGTh), 0, st, g0, nullptr, b0, X[0], gate);
-  hipLaunchKernelGGL(k_g0_resid, t0, dim3(kGTh), 0, st, g0, X[0], b0, Rr[0], skip);
+  if (fz) {
+    // flexible CG: the waiting update first, its convergence test (the
+    // gate) in the residual kernel
+    if (L0.stencil)
+      hipLaunchKernelGGL(k_g0_smooth_upd<true>, t0, dim3(kGTh), 0, st, g0, L0.stencil, *fz, X[0]);
+    else
+      hipLaunchKernelGGL(k_g0_smooth_upd<false>, t0, dim3(kGTh), 0, st, g0, nullptr, *fz, X[0]);
+    hipLaunchKernelGGL(k_g0_resid, t0, dim3(kGTh), 0, st, g0, X[0], b0, Rr[0], gate);
+  } else {
+    if (L0.stencil)
+      hipLaunchKernelGGL(k_g0_smooth<true>, t0, dim3(kGTh), 0, st, g0, L0.stencil, b0, X[0], gate);
+    else
+      hipLaunchKernelGGL(k_g0_smooth<false>, t0, dim3(kGTh), 0, st, g0, nullptr, b0, X[0], gate);
+    hipLaunchKernelGGL(k_g0_resid, t0, dim3(kGTh), 0, st, g0, X[0], b0, Rr[0],
+                       Gate{skip, nullptr, 0, nullptr, 0});
+  }
   IGA_CHECK_LAUNCH();
   for (int l = 0; l + 1 < t; ++l) {
     const iga_mg_level_t& F = mg.level[l];
@@ -1974,8 +2091,10 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
 // With `dot.parts` set, the partials of x . dot.y come with the last kernel
 // (or one more kernel) and their count is returned in *dot_blocks.
 static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const Gate& gate,
-                  const DotOut& dot, int* dot_blocks, hipStream_t st) {
-  if (l0 == 0 && grid_cycle_ok(mg)) return vcycle_grid(mg, b0, x0, gate, dot, dot_blocks, st);
+                  const DotOut& dot, int* dot_blocks, hipStream_t st,
+                  const FcgFuse* fz = nullptr) {
+  if (l0 == 0 && grid_cycle_ok(mg)) return vcycle_grid(mg, b0, x0, gate, dot, dot_blocks, st, fz);
+  if (fz) return IGA_ERR_UNSUPPORTED;
   const int Lc = mg.n_levels - 1;
   const float* B[16];
   float* X[16];
@@ -1989,7 +2108,7 @@ static int vcycle(const iga_mg_t& mg, int l0, const float* b0, float* x0, const 
   X[l0] = x0;
   B[l0] = b0;
   const int* skip = gate.flag;
-  const Gate later{gate.flag, nullptr, 0, nullptr};
+  const Gate later{gate.flag, nullptr, 0, nullptr, 0};
   const DotOut none{nullptr, nullptr};
   bool first = true;
   auto next_gate = [&]() -> Gate {
@@ -2149,6 +2268,7 @@ __global__ void k_fcg_check(int nblk, FBufs F, int init, double tol2, int maxite
       st->iter = 0;
       st->pq_prev = 1.0;
     }
+    st->pending = 0;
     st->rr = rr;
     st->done = !(rr > st->tol2 * st->bb) ? 1 : 0;
   }
@@ -2663,7 +2783,7 @@ extern "C" int iga_energy_loss(int32_t batch, int64_t n, const float* z, const f
   return IGA_OK;
 }
 
-extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 5 * (int64_t)n + 4 * kRedBlocks; }
+extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 6 * (int64_t)n + 4 * kRedBlocks; }
 
 // A captured flexible-CG solve over fixed buffers: every run re-initialises
 // the state on the device and replays graphs until the device flag says
@@ -2690,25 +2810,43 @@ struct iga_fcg_plan {
   double* x;
   FBufs F;
   double *r, *P[2], *Q[2];
+  double* R[2];  // grid cycle: residual ping-pong (an iteration of parity k writes R[k])
+  int nt0;       // row tiles of level 0 (grid cycle: partials of r.r, p.r, p.q)
   float *r32, *z32;
   FcgGraph g8, g4, g2, g1[2];  // g8/g4/g2 start at parity 0, g1[p] at parity p
   int last_iters = 0;
+
+  // the waiting update of the iteration before one of parity par, applied
+  // from R[1 - par] into r_new (R[par] inside a cycle, in place at a flush)
+  FcgFuse fuse(int par, double* r_new) const {
+    return FcgFuse{F.st, R[1 - par], r_new, P[1 - par], Q[1 - par], x, r32, F.P_b, F.P_c, F.P_d,
+                   nt0};
+  }
 
   int iteration(int par, hipStream_t st) {
     double* p = P[par];
     double* pp = P[1 - par];
     double* q = Q[par];
     double* qp = Q[1 - par];
-    const Gate gate{&F.st->done, F.P_d, vb, F.st};
     const DotOut dot{qp, F.P_a};
     int na = 0;
-    int rc = vcycle(mg, 0, r32, z32, gate, dot, &na, st);
-    if (rc) return rc;
     if (grid64) {
+      // the previous update runs in the cycle's first kernel, the gate in
+      // its second; the product leaves its own update waiting
+      const Gate gate{&F.st->done, F.P_d, nt0, F.st, 1};
+      const FcgFuse fz = fuse(par, R[par]);
+      int rc = vcycle(mg, 0, r32, z32, gate, dot, &na, st, &fz);
+      if (rc) return rc;
       const iga_mg_level_t& L0 = mg.level[0];
       hipLaunchKernelGGL(k_fcg_spmv_grid, row_tiles(L0), dim3(kGTh), 0, st, L0.h, L0.w, na,
-                         mg.grid->op64, z32, pp, r, p, q, F.st, F.P_a, F.P_b, F.P_c);
-    } else {
+                         mg.grid->op64, z32, pp, R[par], p, q, F.st, F.P_a, F.P_b, F.P_c);
+      IGA_CHECK_LAUNCH();
+      return (int)IGA_OK;
+    }
+    const Gate gate{&F.st->done, F.P_d, vb, F.st, 0};
+    int rc = vcycle(mg, 0, r32, z32, gate, dot, &na, st);
+    if (rc) return rc;
+    {
       by_lpr(lpr, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         hipLaunchKernelGGL(k_fcg_spmv<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, na, indptr, cols,
@@ -2722,8 +2860,16 @@ struct iga_fcg_plan {
     return (int)IGA_OK;
   }
 
-  int check(hipStream_t st) {
-    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 0, 0.0, 0);
+  // end of a batch whose last iteration had parity par: the waiting update
+  // (grid cycle) and the convergence test
+  int check(int par, hipStream_t st) {
+    if (grid64) {
+      const FcgFuse fz = fuse(1 - par, R[par]);  // R[par] in place
+      hipLaunchKernelGGL(k_fcg_flush, row_tiles(mg.level[0]), dim3(kGTh), 0, st, mg.level[0].h,
+                         mg.level[0].w, fz);
+    }
+    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, grid64 ? nt0 : vb, F, 0,
+                       0.0, 0);
     IGA_CHECK_LAUNCH();
     return IGA_OK;
   }
@@ -2732,7 +2878,9 @@ struct iga_fcg_plan {
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
     int rc = 0;
     for (int k = 0; k < iters && !rc; ++k) rc = iteration((par0 + k) & 1, st);
-    if (!rc) rc = check(st);
+    if (!rc) rc = check((par0 + iters - 1) & 1, st);
+    // grid cycle: the FP64 residual of x is part of every replay
+    if (!rc && grid64) rc = true_residual(st);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc || e != hipSuccess) {
@@ -2801,9 +2949,10 @@ struct iga_fcg_plan {
   }
 
   int run(double tol, int maxiter, int32_t* out_iters, double* out_rr, double* out_bb,
-          double* out_res, hipStream_t st) {
-    hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, r, r32, P[1], Q[1],
-                       F);
+          double* out_res, double* x_out, hipStream_t st) {
+    // the first iteration (parity 0) reads R[1] on the grid cycle
+    hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, grid64 ? R[1] : r,
+                       r32, P[1], Q[1], F);
     hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol,
                        maxiter);
     IGA_CHECK_LAUNCH();
@@ -2838,13 +2987,19 @@ struct iga_fcg_plan {
       } else {
         for (int j = 0; j < std::min(k, 2) && !status; ++j, ++launched)
           status = iteration(launched & 1, st);
-        if (!status) status = check(st);
+        if (!status) status = check((launched - 1) & 1, st);
         if (status) break;
       }
-      res_fresh = false;
-      if (out_res) {
+      res_fresh = graphs && grid64;  // the replayed graphs end with the residual
+      if (out_res && !res_fresh) {
         if ((status = true_residual(st))) break;
         res_fresh = true;
+      }
+      // the caller's copy of x rides behind the batch
+      if (x_out && hipMemcpyAsync(x_out, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
+                                  st) != hipSuccess) {
+        status = IGA_ERR_HIP;
+        break;
       }
     }
     if (status == IGA_OK) last_iters = host.iter;
@@ -2888,7 +3043,9 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   P->P[1] = P->r + 2 * n;
   P->Q[0] = P->r + 3 * n;
   P->Q[1] = P->r + 4 * n;
-  double* parts = P->r + 5 * n;
+  P->R[0] = P->r;
+  P->R[1] = P->r + 5 * n;
+  double* parts = P->r + 6 * n;
   P->F.P_a = parts;
   P->F.P_b = parts + kRedBlocks;
   P->F.P_c = parts + 2 * kRedBlocks;
@@ -2904,6 +3061,7 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
     const dim3 tg = row_tiles(mg->level[0]);
     P->grid64 = grid_cycle_ok(*mg) && mg->grid->op64 && (int)(tg.x * tg.y) <= kRedBlocks;
     P->nb_spmv = P->grid64 ? (int)(tg.x * tg.y) : P->rb;
+    P->nt0 = (int)(tg.x * tg.y);
   }
   rc = P->capture((hipStream_t)stream);
   if (rc) {
@@ -2916,9 +3074,9 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
 
 extern "C" int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter,
                               int32_t* out_iters, double* out_rr, double* out_bb,
-                              double* out_res, void* stream) {
+                              double* out_res, double* x_out, void* stream) {
   if (!plan) return IGA_ERR_ARG;
-  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, out_res, (hipStream_t)stream);
+  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, out_res, x_out, (hipStream_t)stream);
 }
 
 extern "C" void iga_fcg_mg_destroy(iga_fcg_plan_t* plan) { delete plan; }
@@ -2942,7 +3100,7 @@ extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
   int rc = iga_fcg_mg_plan(mg, n, nnz, indptr, cols, vals, b, x, work, work32, graph_iters, stream,
                            &plan);
   if (rc) return rc;
-  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, nullptr, st);
+  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, nullptr, nullptr, st);
   delete plan;
   return rc;
 }

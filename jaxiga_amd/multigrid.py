@@ -678,11 +678,12 @@ class MGOperator:
         with torch.cuda.stream(side):  # graph capture needs a non-default stream
             self.b.copy_(b)
             plan = self._plan(graph_iters)
+            x = torch.empty_like(self.x)  # filled by the run, behind the iterations
             _lib.check(lib().iga_fcg_mg_run(plan.handle, float(tol), int(maxiter), _lib.ctypes.byref(it),
                                             _lib.ctypes.byref(rr), _lib.ctypes.byref(bb),
-                                            _lib.ctypes.byref(res) if true_residual else None, stream()),
+                                            _lib.ctypes.byref(res) if true_residual else None,
+                                            _lib.ptr(x), stream()),
                        "iga_fcg_mg_run")
-            x = self.x.clone()
         torch.cuda.current_stream().wait_stream(side)
         if true_residual:
             return x, it.value, rr.value, bb.value, res.value
