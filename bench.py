@@ -103,8 +103,8 @@ def cpu_baseline(sample_refine=7):
                       f"median of 3 runs of {t:.2f} s"}
 
 
-def learned_pc_bench(refine=8, n_train=48, steps=300, batch=4, test_seeds=(1000, 1001, 1002), reps=3,
-                     log=print):
+def learned_pc_bench(refine=8, n_train=88, steps=300, batch=4, test_seeds=tuple(range(10_000, 10_032)),
+                     reps=3, log=print):
     """Time to a 1e-10 relative residual on the 256x256 cubic variable-coefficient
     Poisson problem (a = exp(GRF), l = 0.1, sigma = 1.5, smooth random source):
     FP64 Jacobi-CG versus flexible CG with the learned multigrid preconditioner
@@ -161,26 +161,37 @@ def learned_pc_bench(refine=8, n_train=48, steps=300, batch=4, test_seeds=(1000,
             tl.append(le.seconds)
         rn, bn = LIN.residual_norms(pattern, vals, jac.x, rhs)
         diff = float((le.x - jac.x).abs().max() / jac.x.abs().max())
-        row = {"seed": seed, "jacobi_cg_fp64": {"seconds": statistics.median(tj), "iterations": jac.iterations,
-                                                "relative_residual": rn / bn},
-               "mg_fcg_untrained": {"seconds": statistics.median(tu), "iterations": un.iterations,
-                                    "relative_residual": un.relative_residual},
-               "learned_mg_fcg": {"seconds": statistics.median(tl), "setup_seconds": le.setup_seconds,
-                                  "iterations": le.iterations, "fallback_iterations": le.fallback_iterations,
-                                  "relative_residual": le.relative_residual, "first_call_seconds": t_first,
-                                  "max_rel_diff_vs_jacobi_cg": diff}}
-        row["speedup_vs_jacobi_cg"] = row["jacobi_cg_fp64"]["seconds"] / row["learned_mg_fcg"]["seconds"]
-        log(f"[bench] seed {seed}: jacobi-cg {jac.iterations} it {row['jacobi_cg_fp64']['seconds'] * 1e3:.2f} ms | "
-            f"mg-fcg untrained {un.iterations} it {row['mg_fcg_untrained']['seconds'] * 1e3:.2f} ms | "
-            f"learned {le.iterations} it {row['learned_mg_fcg']['seconds'] * 1e3:.2f} ms "
-            f"(setup {le.setup_seconds * 1e3:.2f} ms) -> {row['speedup_vs_jacobi_cg']:.2f}x")
+        row = {"seed": seed,
+               "jacobi_cg_fp64": [jac.iterations, statistics.median(tj) * 1e3],
+               "mg_fcg_untrained": [un.iterations, statistics.median(tu) * 1e3],
+               "learned_mg_fcg": [le.iterations, statistics.median(tl) * 1e3, le.setup_seconds * 1e3,
+                                  le.fallback_iterations],
+               "relative_residual": [rn / bn, un.relative_residual, le.relative_residual],
+               "first_call_ms": t_first * 1e3, "max_rel_diff_vs_jacobi_cg": diff}
+        row["speedup"] = statistics.median(tj) / statistics.median(tl)
+        log(f"[bench] seed {seed}: jacobi-cg {jac.iterations} it {statistics.median(tj) * 1e3:.2f} ms | "
+            f"mg-fcg untrained {un.iterations} it {statistics.median(tu) * 1e3:.2f} ms | "
+            f"learned {le.iterations} it {statistics.median(tl) * 1e3:.2f} ms "
+            f"(setup {le.setup_seconds * 1e3:.2f} ms) -> {row['speedup']:.2f}x")
         rows.append(row)
     return {"workload": f"unit-square Poisson, p=3, {2**refine}x{2**refine} elements, a=exp(GRF l=0.1 sigma=1.5), "
-                        "smooth random source, rel. residual 1e-10, unseen coefficient seeds",
+                        f"smooth random source, rel. residual 1e-10, {len(rows)} unseen coefficient seeds "
+                        f"{test_seeds[0]}..{test_seeds[-1]}",
+            "per_seed_columns": {"jacobi_cg_fp64": ["iterations", "ms"], "mg_fcg_untrained": ["iterations", "ms"],
+                                 "learned_mg_fcg": ["iterations", "ms", "setup_ms", "fallback_iterations"],
+                                 "relative_residual": ["jacobi_cg", "untrained", "learned"]},
             "training": {"fields": n_train, "pairs": n_pairs, "steps": steps, "seconds": t_train,
                          "data_seconds": t_data, "loss_first": hist[0], "loss_last10": float(np.mean(hist[-10:]))},
             "per_seed": rows,
-            "speedup_median": statistics.median(r["speedup_vs_jacobi_cg"] for r in rows)}
+            "speedup_median": statistics.median(r["speedup"] for r in rows),
+            "speedup_min": min(r["speedup"] for r in rows),
+            "speedup_max": max(r["speedup"] for r in rows),
+            "iterations_learned": sorted({r["learned_mg_fcg"][0] for r in rows}),
+            "iterations_jacobi_cg_median": statistics.median(r["jacobi_cg_fp64"][0] for r in rows),
+            "ms_learned_median": statistics.median(r["learned_mg_fcg"][1] for r in rows),
+            "ms_jacobi_cg_median": statistics.median(r["jacobi_cg_fp64"][1] for r in rows),
+            "max_relative_residual_learned": max(r["relative_residual"][2] for r in rows),
+            "fallback_iterations_total": sum(r["learned_mg_fcg"][3] for r in rows)}
 
 
 def matfree_bench(refine=6, reps=5, log=print):
@@ -396,8 +407,11 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the learned-preconditioner time-to-solve comparison")
-    ap.add_argument("--learned-fields", type=int, default=48, help="training fields of the learned PC")
+    ap.add_argument("--learned-fields", type=int, default=88,
+                    help="training fields of the learned PC (about 23 residual/error pairs each)")
     ap.add_argument("--learned-steps", type=int, default=300, help="training steps of the learned PC")
+    ap.add_argument("--learned-seeds", type=int, default=32,
+                    help="unseen coefficient seeds (10000, 10001, ...) of the learned-PC comparison")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the matrix-free (3-D) and batched sections")
     ap.add_argument("--extras-only", action="store_true",
@@ -426,8 +440,9 @@ def main():
         return
     lib = _lib.load()
     if args.learned_only:
-        print(json.dumps({"learned_pc": learned_pc_bench(n_train=args.learned_fields,
-                                                          steps=args.learned_steps)}), flush=True)
+        print(json.dumps({"learned_pc": learned_pc_bench(
+            n_train=args.learned_fields, steps=args.learned_steps,
+            test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)))}), flush=True)
         return
     if args.extras_only:
         print(json.dumps({"matfree": matfree_bench(), "batched": batched_bench()}), flush=True)
@@ -489,7 +504,8 @@ def main():
               f"{refined.seconds:.3f} s ({refined.outer} outer, {refined.total_inner} inner)", flush=True)
 
     learned = None if (args.no_learned or args.no_solve) else learned_pc_bench(
-        n_train=args.learned_fields, steps=args.learned_steps)
+        n_train=args.learned_fields, steps=args.learned_steps,
+        test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)))
     extras = {} if (args.no_extras or args.no_solve) else {"matfree": matfree_bench(), "batched": batched_bench()}
     cpu = None if args.no_cpu else cpu_baseline()
     line = {

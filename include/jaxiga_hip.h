@@ -479,8 +479,8 @@ int iga_energy_loss(int32_t batch, int64_t n, const float* z, const float* kz, f
                     double* per_sample, double* loss, void* stream);
 /* Flexible CG (one stored direction) on the FP64 operator with one FP32
  * V-cycle as the preconditioner; stops at ||r|| <= tol ||b|| or maxiter.
- * graph_iters > 0 captures graphs of 1 and 2 iterations, and of 4 and 8 when
- * graph_iters reaches them, and replays them. work: iga_fcg_workspace(n)
+ * graph_iters > 0 captures one graph for every iteration count up to
+ * graph_iters (at most 16) and replays them. work: iga_fcg_workspace(n)
  * doubles; work32: 2n floats. */
 int64_t iga_fcg_workspace(int32_t n);
 /* A reusable plan: the iterations are captured once over fixed buffers (b, x,

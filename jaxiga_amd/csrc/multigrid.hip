@@ -2788,9 +2788,9 @@ extern "C" int64_t iga_fcg_workspace(int32_t n) { return 8 + 6 * (int64_t)n + 4 
 // A captured flexible-CG solve over fixed buffers: every run re-initialises
 // the state on the device and replays graphs until the device flag says
 // done, so one plan serves every right-hand side written into `b` and every
-// operator whose values are rewritten in place. Graphs of 8, 4, 2 and 1
-// iterations are kept; a run first replays as many iterations as the last
-// run needed without looking, then one iteration per look, so few
+// operator whose values are rewritten in place. One graph per iteration
+// count is kept; a run first replays as many iterations as the last run
+// needed without looking (one launch), then one iteration per look, so few
 // iterations run past convergence (their kernels return at once but still
 // cost a launch each) and a run like the last one reads the flag once.
 struct FcgGraph {
@@ -2813,7 +2813,9 @@ struct iga_fcg_plan {
   double* R[2];  // grid cycle: residual ping-pong (an iteration of parity k writes R[k])
   int nt0;       // row tiles of level 0 (grid cycle: partials of r.r, p.r, p.q)
   float *r32, *z32;
-  FcgGraph g8, g4, g2, g1[2];  // g8/g4/g2 start at parity 0, g1[p] at parity p
+  static constexpr int kMaxGraph = 16;
+  FcgGraph gk[kMaxGraph + 1];  // gk[k]: k iterations from parity 0
+  FcgGraph g1odd;              // one iteration from parity 1
   int last_iters = 0;
 
   // the waiting update of the iteration before one of parity par, applied
@@ -2897,37 +2899,28 @@ struct iga_fcg_plan {
     return IGA_OK;
   }
 
+  // one graph per iteration count up to G (a batch is one launch: its end,
+  // the waiting update, the test and the residual, runs once) and one
+  // single iteration from the odd parity
   int capture(hipStream_t st) {
     if (G <= 0) return IGA_OK;
-    int rc = capture_one(g1[0], 1, 0, st);
-    if (!rc) rc = capture_one(g1[1], 1, 1, st);
-    if (!rc) rc = capture_one(g2, 2, 0, st);
-    if (!rc && G >= 4) rc = capture_one(g4, 4, 0, st);
-    if (!rc && G >= 8) rc = capture_one(g8, 8, 0, st);
+    int rc = capture_one(g1odd, 1, 1, st);
+    for (int k = 1; k <= std::min(G, kMaxGraph) && !rc; ++k) rc = capture_one(gk[k], k, 0, st);
     return rc;
   }
 
-  // k more iterations from the `launched`-th, as few graph launches as the
-  // parity allows
+  // k more iterations from the `launched`-th
   int replay(int k, int& launched, hipStream_t st) {
+    const int gmax = std::min(G, kMaxGraph);
     while (k > 0) {
       const FcgGraph* g;
       int m;
       if (launched & 1) {
-        g = &g1[1];
+        g = &g1odd;
         m = 1;
-      } else if (k >= 8 && g8.exec) {
-        g = &g8;
-        m = 8;
-      } else if (k >= 4 && g4.exec) {
-        g = &g4;
-        m = 4;
-      } else if (k >= 2) {
-        g = &g2;
-        m = 2;
       } else {
-        g = &g1[0];
-        m = 1;
+        m = std::min(k, gmax);
+        g = &gk[m];
       }
       if (hipGraphLaunch(g->exec, st) != hipSuccess) return IGA_ERR_HIP;
       k -= m;
@@ -2961,7 +2954,7 @@ struct iga_fcg_plan {
     // the FP64 residual is recomputed after every batch when asked for, so
     // the look that finds convergence also reads it
     bool res_fresh = false;
-    const bool graphs = g1[0].exec != nullptr;
+    const bool graphs = gk[1].exec != nullptr;
     // iterations before the first look: the last run's count (four on the
     // first run), then one per look; never more than maxiter in all
     int next = last_iters > 0 ? last_iters : 4;
@@ -3011,7 +3004,8 @@ struct iga_fcg_plan {
   }
 
   ~iga_fcg_plan() {
-    for (FcgGraph* g : {&g8, &g4, &g2, &g1[0], &g1[1]}) {
+    for (int k = 0; k <= kMaxGraph + 1; ++k) {
+      FcgGraph* g = k <= kMaxGraph ? &gk[k] : &g1odd;
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
       if (g->graph) (void)hipGraphDestroy(g->graph);
     }
