@@ -420,6 +420,12 @@ typedef struct iga_mg_setup_grid {
    * points of its 7 x 7 window (checked by the caller; values are then
    * written at computed slots) */
   int32_t regular[16];
+  /* which part of the set-up a call runs: bit 0 the level-0 stencil layouts,
+   * features, copies and scalings; bit 1 the Galerkin levels, the dense tail
+   * and the coarse inverse (which read what bit 0 wrote); 0 = both. A caller
+   * may run bit 0, start work that reads the level-0 features on another
+   * stream, and run bit 1 beside it. */
+  int32_t stages;
 } iga_mg_setup_grid_t;
 
 typedef struct iga_mg_setup {
@@ -498,8 +504,10 @@ int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* i
                     iga_fcg_plan_t** out);
 /* out_res (optional): ||b - A x|| recomputed in FP64 from the returned x
  * (independent of the recursion), read back with the solver state;
- * x_out (optional): n doubles receiving x, copied on the stream behind the
- * iterations (the plan's x is overwritten by its next run) */
+ * x_out (optional): n doubles receiving x (the plan's x is overwritten by
+ * its next run): written by the last kernel of every replay on the grid
+ * cycle, else copied on the stream behind the iterations. A run may use any
+ * stream, also one other than the plan's capture stream. */
 int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter, int32_t* out_iters,
                    double* out_rr, double* out_bb, double* out_res, double* x_out, void* stream);
 void iga_fcg_mg_destroy(iga_fcg_plan_t* plan);

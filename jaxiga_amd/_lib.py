@@ -85,7 +85,7 @@ class IgaMgSetupGrid(ctypes.Structure):
     _fields_ = [("f_lo", (c_void_p * 2) * 16), ("rw", (c_void_p * 2) * 16), ("op64", c_void_p * 16),
                 ("op32", c_void_p * 16), ("feats", c_void_p), ("work", c_void_p),
                 ("tail_level", c_int), ("omega", c_double), ("tail", c_void_p), ("tail_work", c_void_p),
-                ("regular", c_int * 16)]
+                ("regular", c_int * 16), ("stages", c_int)]
 
 
 class IgaMgSetup(ctypes.Structure):

@@ -355,6 +355,7 @@ struct FState {
   int iter, maxiter, done;
   int pending;  // an update x += alpha p, r -= alpha q waits (grid cycle)
   double res2;  // ||b - A x||^2 recomputed from x after the solve
+  double* x_out;  // the caller's copy of x, written by the flush (grid cycle)
 };
 
 // Gate of the first kernel of a V-cycle. Inside flexible CG it is also the
@@ -1233,7 +1234,13 @@ __global__ void __launch_bounds__(kGTh)
 // with the r.r partials for the check that follows
 __global__ void __launch_bounds__(kGTh) k_fcg_flush(int h, int w, FcgFuse f) {
   const L0Tile T = l0_tile(h, w);
-  if (f.st->done) return;
+  // the caller's copy of x: every replay ends here, also one whose gate
+  // closed inside it (x is then final already)
+  double* xo = f.st->x_out;
+  if (f.st->done) {
+    if (xo && T.in) xo[T.it] = f.x[T.it];
+    return;
+  }
   const bool pend = f.st->pending != 0;
   double alpha = 0.0, pq = 0.0;
   if (pend) {
@@ -1243,11 +1250,14 @@ __global__ void __launch_bounds__(kGTh) k_fcg_flush(int h, int w, FcgFuse f) {
   double rr = 0.0;
   if (T.in) {
     const double rn = f.r_old[T.it] - alpha * f.q[T.it];
+    double xn = f.x[T.it];
     if (pend) {
       f.r_new[T.it] = rn;
       f.r32[T.it] = (float)rn;
-      f.x[T.it] += alpha * f.p[T.it];
+      xn += alpha * f.p[T.it];
+      f.x[T.it] = xn;
     }
+    if (xo) xo[T.it] = xn;
     rr = rn * rn;
   }
   store_partial_at(f.P_d, blockIdx.y * gridDim.x + blockIdx.x, rr);
@@ -2241,7 +2251,8 @@ struct FBufs {
 
 __global__ void __launch_bounds__(kVecThreads)
     k_fcg_init(int n, const double* __restrict__ b, double* x, double* r, float* r32, double* p1,
-               double* q1, FBufs F) {
+               double* q1, FBufs F, double* x_out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) F.st->x_out = x_out;
   double bb = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     x[i] = 0.0;
@@ -2474,6 +2485,10 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
         return IGA_ERR_ARG;
     }
     if (s->radius != kGR) return IGA_ERR_UNSUPPORTED;
+    if (G->stages < 0 || G->stages > 3) return IGA_ERR_ARG;
+  }
+  const int stages = G && G->stages ? G->stages : 3;
+  if (G && (stages & 1)) {
     // level 0 in stencil layout (FP32, FP64, features) from its CSR
     const iga_mg_setup_level_t& V = s->level[0];
     GridOpsArgs a{};
@@ -2494,6 +2509,8 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     a.op64 = G->op64[0];
     a.feats = G->feats;
     hipLaunchKernelGGL(k_grid_ops, dim3((V.n + kOpsRows - 1) / kOpsRows, 1), dim3(kGTh), 0, st, a);
+  }
+  if (G && (stages & 2)) {
     auto gal = [&](int l) {
       const iga_mg_setup_level_t& F = s->level[l];
       const iga_mg_setup_level_t& C = s->level[l + 1];
@@ -2527,7 +2544,7 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
       hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, g);
       hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, g);
     }
-  } else {
+  } else if (!G) {
     for (int l = 0; l + 1 < L; ++l) {
       const iga_mg_setup_level_t& V = s->level[l];
       const iga_mg_setup_level_t& C = s->level[l + 1];
@@ -2537,6 +2554,7 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     }
   }
   IGA_CHECK_LAUNCH();
+  if (!(stages & 2)) return IGA_OK;
   // with the tensor set-up and a tail level, the dense tail (and with it the
   // coarse inverse) comes straight from the stencils
   if (G && G->tail_level > 0) return tail_dense(*s, st);
@@ -2944,8 +2962,9 @@ struct iga_fcg_plan {
   int run(double tol, int maxiter, int32_t* out_iters, double* out_rr, double* out_bb,
           double* out_res, double* x_out, hipStream_t st) {
     // the first iteration (parity 0) reads R[1] on the grid cycle
+    // on the grid cycle the flush at the end of every replay writes x_out
     hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, grid64 ? R[1] : r,
-                       r32, P[1], Q[1], F);
+                       r32, P[1], Q[1], F, grid64 ? x_out : nullptr);
     hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol,
                        maxiter);
     IGA_CHECK_LAUNCH();
@@ -2989,7 +3008,7 @@ struct iga_fcg_plan {
         res_fresh = true;
       }
       // the caller's copy of x rides behind the batch
-      if (x_out && hipMemcpyAsync(x_out, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
+      if (x_out && !grid64 && hipMemcpyAsync(x_out, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice,
                                   st) != hipSuccess) {
         status = IGA_ERR_HIP;
         break;

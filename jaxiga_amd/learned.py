@@ -41,7 +41,7 @@ import numpy as np
 from jaxiga_amd import _lib
 from jaxiga_amd import linear as LIN
 from jaxiga_amd import multigrid as MG
-from jaxiga_amd.runtime import lib, stream
+from jaxiga_amd.runtime import lib, side_stream, stream
 
 KS = 7  # smoother stencil width
 OMEGA0 = 0.5  # damped-Jacobi weight of the untrained generator
@@ -288,10 +288,29 @@ def smoother_stencils(gen: ConvNet, op: MG.MGOperator, levels=LEARNED_LEVELS):
 def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     """Multigrid operator with the generator's smoothers on the ``levels``
     finest levels and damped Jacobi elsewhere (everywhere when ``gen`` is None)."""
+    import torch
+
     hier = MG.hierarchy(ctx)
-    op = MG.operator(hier, omega=OMEGA0, ks=KS).load(vals64)
-    if gen is not None:
-        op.set_stencils(smoother_stencils(gen, op, levels))
+    op = MG.operator(hier, omega=OMEGA0, ks=KS)
+    if gen is None or not op.split_load():
+        op.load(vals64)
+        if gen is not None:
+            op.set_stencils(smoother_stencils(gen, op, levels))
+        return op
+    # the generator reads only level 0's features: it runs on a second
+    # stream beside the Galerkin levels and the dense tail, which are
+    # enqueued first so that the host's work on the generator overlaps them
+    op.load(vals64, stage=1)
+    main = torch.cuda.current_stream()
+    level0 = torch.cuda.Event()
+    level0.record(main)
+    op.load(vals64, stage=2)
+    side = side_stream(op.b.device, "generator")
+    side.wait_event(level0)
+    with torch.cuda.stream(side):
+        stencils = smoother_stencils(gen, op, levels)
+    main.wait_stream(side)
+    op.set_stencils(stencils)
     return op
 
 

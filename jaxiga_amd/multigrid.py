@@ -532,12 +532,28 @@ class MGOperator:
             self._setup = m
         return self._setup
 
-    def load(self, vals64, stencils=None):
+    def split_load(self):
+        """Whether :meth:`load` can run in two stages (the tensor set-up)."""
+        return self._tensor is not None
+
+    def load(self, vals64, stencils=None, stage=0):
         """Write a new level-0 operator (FP64 CSR values) and its smoothers:
         one library call forms every Galerkin level, the FP32 copies, the
         diagonal scalings and the coarsest inverse, a second the dense matrix
-        of the cycle below the tail level."""
-        _lib.check(lib().iga_mg_load(self._setup_struct(), _lib.ptr(vals64), stream()), "iga_mg_load")
+        of the cycle below the tail level.
+
+        With the tensor set-up, ``stage=1`` writes level 0 only (its stencil
+        layouts, the generator features, copies and scalings) and ``stage=2``
+        the rest, so work that reads the features can run on another stream
+        beside stage 2; ``stage=0`` does both."""
+        if stage and not self.split_load():
+            raise ValueError("a staged load needs the tensor set-up")
+        s = self._setup_struct()
+        if self._tensor is not None:
+            self._tensor.stages = stage
+        _lib.check(lib().iga_mg_load(s, _lib.ptr(vals64), stream()), "iga_mg_load")
+        if stage == 1:
+            return self
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
         self._form_tail()
         if self._grid is not None and self._tensor is None:
@@ -654,13 +670,21 @@ class MGOperator:
         key = (tuple(s is None for s in self.stencils), self.tail_active(), graph_iters)
         plan = self._plans.get(key)
         if plan is None:
+            import torch
+
             lv = self.hier.levels[0]
             handle = _lib.c_void_p()
-            _lib.check(lib().iga_fcg_mg_plan(self.struct(), lv.n, lv.nnz, _lib.ptr(lv.indptr),
-                                             _lib.ptr(lv.cols), _lib.ptr(self.vals64[0]), _lib.ptr(self.b),
-                                             _lib.ptr(self.x), _lib.ptr(self._fwork), _lib.ptr(self._fwork32),
-                                             int(graph_iters), stream(), _lib.ctypes.byref(handle)),
-                       "iga_fcg_mg_plan")
+            cur = torch.cuda.current_stream()
+            side = side_stream(self.b.device, "fcg") if graph_iters > 0 else cur
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):  # graph capture needs a non-default stream
+                _lib.check(lib().iga_fcg_mg_plan(self.struct(), lv.n, lv.nnz, _lib.ptr(lv.indptr),
+                                                 _lib.ptr(lv.cols), _lib.ptr(self.vals64[0]),
+                                                 _lib.ptr(self.b), _lib.ptr(self.x), _lib.ptr(self._fwork),
+                                                 _lib.ptr(self._fwork32), int(graph_iters), stream(),
+                                                 _lib.ctypes.byref(handle)),
+                           "iga_fcg_mg_plan")
+            cur.wait_stream(side)
             plan = _Plan(handle)
             self._plans[key] = plan
         return plan
@@ -673,18 +697,14 @@ class MGOperator:
         import torch
 
         it, rr, bb, res = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0), _lib.c_double(-1.0)
-        side = side_stream(self.b.device, "fcg") if graph_iters > 0 else torch.cuda.current_stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):  # graph capture needs a non-default stream
-            self.b.copy_(b)
-            plan = self._plan(graph_iters)
-            x = torch.empty_like(self.x)  # filled by the run, behind the iterations
-            _lib.check(lib().iga_fcg_mg_run(plan.handle, float(tol), int(maxiter), _lib.ctypes.byref(it),
-                                            _lib.ctypes.byref(rr), _lib.ctypes.byref(bb),
-                                            _lib.ctypes.byref(res) if true_residual else None,
-                                            _lib.ptr(x), stream()),
-                       "iga_fcg_mg_run")
-        torch.cuda.current_stream().wait_stream(side)
+        plan = self._plan(graph_iters)  # captured once; replayed on the current stream
+        self.b.copy_(b)
+        x = torch.empty_like(self.x)  # filled by the run, behind the iterations
+        _lib.check(lib().iga_fcg_mg_run(plan.handle, float(tol), int(maxiter), _lib.ctypes.byref(it),
+                                        _lib.ctypes.byref(rr), _lib.ctypes.byref(bb),
+                                        _lib.ctypes.byref(res) if true_residual else None,
+                                        _lib.ptr(x), stream()),
+                   "iga_fcg_mg_run")
         if true_residual:
             return x, it.value, rr.value, bb.value, res.value
         return x, it.value, rr.value, bb.value

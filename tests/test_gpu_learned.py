@@ -199,6 +199,16 @@ def test_multigrid_fcg_converges_and_matches_jacobi_cg(gpu):
     assert it < jac.iterations / 2
     ref = jac.x.cpu().numpy()
     assert np.abs(x.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+    # a looser tolerance on the same plan converges inside the replay of the
+    # previous run's iteration count: the returned x must still be the
+    # iterate at convergence; then the tight tolerance again
+    for tol in (1e-4, 1e-10):
+        x2, it2, _, _ = op.fcg(rhs, tol, 200)
+        rn2, _ = LIN.residual_norms(ctx.assembly, vals, x2, rhs)
+        print(f"[mg] tol {tol:g}: {it2} it, relative residual {rn2 / bn:.2e}", flush=True)
+        assert rn2 <= 1.01 * tol * bn
+        assert (it2 < it) if tol > 1e-10 else (it2 == it)
+    assert np.abs(x2.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
 
 
 def test_training_gains_over_untrained_smoothers(gpu):
@@ -220,6 +230,8 @@ def test_training_gains_over_untrained_smoothers(gpu):
     for res in (before, after):
         assert res.relative_residual <= 1e-10
         assert res.fallback_iterations == 0
+        rn, bn = LIN.residual_norms(ctx.assembly, vals, res.x, rhs)  # of the returned x
+        assert rn <= 1.01e-10 * bn
     assert after.iterations < before.iterations
     assert after.iterations < jac.iterations
     ref = jac.x.cpu().numpy()
