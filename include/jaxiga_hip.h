@@ -254,6 +254,29 @@ int iga_diff_norms(int32_t n, const double* y, const double* b, double* out, dou
 /* inv[i] = 1/d[i], or 1 where d[i] == 0 (Jacobi zero guard) */
 int iga_inv_guard(int32_t n, const double* d, double* inv, void* stream);
 
+/* Implicit differentiation of the CSR solve (replaces the reverse rule of
+ * _csr_sparse_solve, src/jaxiga/solvers/linear.py:376-386): after one
+ * adjoint solve K lam = u_bar on the same symmetric CSR, the cotangent of
+ * the CSR values is data_bar[k] = alpha * a[rows[k]] * b[cols[k]] with
+ * alpha = -1, a = lam, b = u.                                              */
+int iga_csr_outer(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* a,
+                  const double* b, double alpha, double* out, void* stream);
+
+/* Fields at points from basis data (replaces Solution.at / Solution.grad,
+ * src/jaxiga/post/solution.py:34-50, and gives the point sensitivities of
+ * the element kernels in the reverse pass of a Galerkin solve,
+ * src/jaxiga/methods/galerkin.py:37-148 under jax.grad): for the one or two
+ * coefficient vectors u0, u1 (n_cp * vec doubles, component fastest; u1 may
+ * be null),
+ *   val  (n_f, n_e, n_q, vec)      = sum_a R[e,q,a] u[dofs[e,a]]
+ *   grad (n_f, n_e, n_q, vec, dim) = sum_a dR[e,q,d,a] u[dofs[e,a]]
+ * R (n_e,n_q,n_l) and dR (n_e,n_q,dim,n_l) as iga_basis_eval writes them,
+ * dofs (n_e,n_l) control-point indices. val or grad may be null (not both).
+ * dim 2 with vec 1 or 2, dim 3 with vec 1 or 3.                            */
+int iga_field_points(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, int32_t vec,
+                     const double* R, const double* dR, const int32_t* dofs, const double* u0,
+                     const double* u1, double* val, double* grad, void* stream);
+
 /* Learned networks on the control-point grid: 3x3 convolutions, float32 NCHW.
  * kind[l]   : 0 forward convolution (weights (cout, cin, 3, 3)),
  *             1 transposed convolution + skip add (weights (cin, cout, 3, 3))

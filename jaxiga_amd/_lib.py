@@ -148,6 +148,10 @@ _SIGNATURES = {
                                 c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
     "iga_diff_norms": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_inv_guard": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_csr_outer": (c_int, [ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                              c_void_p]),
+    "iga_field_points": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_net_act_floats": (c_i64, [ctypes.POINTER(IgaNet), c_int]),
     "iga_net_act_offset": (c_i64, [ctypes.POINTER(IgaNet), c_int, c_int]),
     "iga_net_scratch_floats": (c_i64, [ctypes.POINTER(IgaNet), c_int]),

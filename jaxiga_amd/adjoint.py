@@ -1,0 +1,203 @@
+"""Implicit differentiation of the Galerkin solve (``jax.grad`` through
+``jx.solve`` in the reference).
+
+The reference differentiates ``solve`` by JAX autodiff of the assembly plus
+the custom reverse rule of the sparse solve (``_csr_sparse_solve``,
+``src/jaxiga/solvers/linear.py:362-389``): one adjoint solve ``K lam = u_bar``
+on the same symmetric CSR, ``data_bar = -lam[rows] u[cols]`` and
+``b_bar = lam``, which then flow back through the scatter-add and the element
+kernels (``src/jaxiga/methods/galerkin.py:37-148``) to the parameters.
+
+Here the parameters reach the element kernels only as values at points: the
+coefficient ``a0`` (a number or its values at the interior Gauss points), the
+constitutive matrix ``C``, the source at the interior Gauss points and the
+tractions at the boundary Gauss points (:func:`jaxiga_amd.methods.point_inputs`,
+evaluated with torch autograd on). Because ``data_bar`` is the rank-one
+``-lam u^T`` on the pattern and the load enters linearly, its pull-back through
+the scatter-add and the element kernels is a sum over points of the two
+fields, with ``u`` the full solution (prescribed values included, which
+carries the lifting ``-K_fp g``) and ``lam`` zero on prescribed dofs:
+
+* coefficient at a point: ``-w grad(lam) . grad(u)``;
+* ``C``: ``-sum_points w eps(lam) eps(u)^T`` (Voigt, engineering shear);
+* source at a point: ``w lam``; traction at a boundary point: ``w lam``.
+
+``lam`` and ``u`` at the points come from one HIP kernel
+(``iga_field_points``) over the basis data of the same points. The reverse
+pass therefore costs exactly one adjoint solve (``SOLVE_COUNTER['adjoint']``),
+whatever the forward iteration did, and torch carries the point cotangents
+back to the parameters through the user's callables. Dirichlet values are
+resolved at set-up and are not differentiated (as in the reference); nor are
+control points or weights (geometry enters the kernels as fixed data).
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from jaxiga_amd import _lib
+from jaxiga_amd import linear as LIN
+from jaxiga_amd.evaluation import evaluate
+from jaxiga_amd.pointset import gauss
+from jaxiga_amd.runtime import lib, stream
+
+
+def field_points(basis, u0, u1=None, vec=1, values=True, gradients=True):
+    """Values ``(n_f, n_e, n_q, vec)`` and gradients ``(n_f, n_e, n_q, vec,
+    dim)`` at the points of ``basis`` of one or two full coefficient vectors
+    (``n_cp * vec`` doubles, component fastest); either output may be
+    skipped (None)."""
+    import torch
+
+    if values and basis.R is None:
+        raise ValueError("values need the basis functions R")
+    if gradients and basis.dR is None:
+        raise ValueError("gradients need the basis gradients dR")
+    ref = basis.R if basis.R is not None else basis.dR
+    n_e, n_q, n_l = ref.shape[0], ref.shape[1], ref.shape[-1]
+    dim = basis.dR.shape[2] if basis.dR is not None else _dim(basis)
+    dev = u0.device
+    nf = 1 if u1 is None else 2
+    dofs = torch.as_tensor(np.ascontiguousarray(basis.dofs, dtype=np.int32), device=dev)
+    u0 = u0.detach().to(torch.float64).contiguous()
+    u1 = None if u1 is None else u1.detach().to(torch.float64).contiguous()
+    val = torch.empty((nf, n_e, n_q, vec), dtype=torch.float64, device=dev) if values else None
+    grad = torch.empty((nf, n_e, n_q, vec, dim), dtype=torch.float64, device=dev) if gradients else None
+    _lib.check(lib().iga_field_points(n_e, n_q, n_l, dim, vec, _lib.ptr(basis.R) if values else None,
+                                      _lib.ptr(basis.dR) if gradients else None, _lib.ptr(dofs),
+                                      _lib.ptr(u0), _lib.ptr(u1), _lib.ptr(val), _lib.ptr(grad),
+                                      stream()), "iga_field_points")
+    return val, grad
+
+
+def _dim(basis):
+    for a in (basis.x, basis.normal):
+        if a is not None:
+            return int(a.shape[-1])
+    raise ValueError("the basis data carries no spatial dimension (evaluate 'dR' or 'x')")
+
+
+def voigt(g):
+    """Engineering Voigt strain ``(..., n_s)`` from displacement gradients
+    ``g[..., c, d] = du_c/dx_d`` (2-D: xx, yy, xy; 3-D: xx, yy, zz, yz, xz, xy)."""
+    import torch
+
+    if g.shape[-1] == 2:
+        return torch.stack([g[..., 0, 0], g[..., 1, 1], g[..., 0, 1] + g[..., 1, 0]], -1)
+    return torch.stack([g[..., 0, 0], g[..., 1, 1], g[..., 2, 2], g[..., 1, 2] + g[..., 2, 1],
+                        g[..., 0, 2] + g[..., 2, 0], g[..., 0, 1] + g[..., 1, 0]], -1)
+
+
+class _Run:
+    """One differentiable Galerkin solve: the forward assembly and solve, and
+    the reverse pass (one adjoint solve + the point cotangents)."""
+
+    def __init__(self, ctx, params, linear, inputs):
+        self.ctx, self.params, self.linear, self.inputs = ctx, params, linear, inputs
+        self.stats = None
+
+    def forward(self):
+        import torch
+
+        from jaxiga_amd.methods import assemble
+
+        ctx, linear = self.ctx, self.linear
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            vals, b, _ = assemble(ctx, self.params, inputs=self.inputs)
+            u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
+            rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
+            u = ctx.dofmap.lift(u_free)
+        torch.cuda.synchronize()
+        stats = {"method": "galerkin", "linear": linear.method, "matrix_free": False,
+                 "n_free": ctx.n_free, "nnz": ctx.assembly.nnz, **info,
+                 "wall_time": time.perf_counter() - t0}
+        stats.update(LIN.linear_diagnostics(rn, bn, u_free, linear.tol))
+        LIN.warn_if_failed("linear Galerkin solve", stats["converged"], stats["residual"])
+        self.stats, self.vals, self.u = stats, vals, u
+        return u
+
+    def backward(self, u_bar, needs):
+        """Cotangents of ``(coef, src, C, *tractions)`` (None where not needed)."""
+        import torch
+
+        ctx = self.ctx
+        space = ctx.space
+        dev = u_bar.device
+        free = getattr(ctx, "_free_long", None)
+        if free is None:
+            free = ctx.free_d.long()
+            ctx._free_long = free
+        lam_free = LIN.adjoint_solve(self.linear, ctx.assembly, self.vals, u_bar[free])
+        lam = torch.zeros(space.n_dofs, dtype=torch.float64, device=dev)
+        lam[free] = lam_free
+        coef, src, C = self.inputs["coef"], self.inputs["src"], self.inputs["C"]
+        tractions = self.inputs["tractions"] or []
+        out = [None] * (3 + len(tractions))
+        need_grad = needs[0] or needs[2]
+        if needs[0] or needs[1] or needs[2]:
+            outputs = ("R",) * bool(needs[1]) + ("dR",) * bool(need_grad) + ("x", "w")
+            basis = evaluate(space, gauss(space, ctx.n_q1), outputs=outputs)
+            w = basis.w
+            if need_grad:
+                _, g = field_points(basis, self.u, lam, space.vec, values=False)
+                if needs[0]:
+                    gq = -w * (g[1] * g[0]).sum((-1, -2))
+                    out[0] = gq if coef.dim() == 2 else gq.sum().reshape(coef.shape)
+                if needs[2]:
+                    eu, el = voigt(g[0]), voigt(g[1])
+                    gc = -torch.einsum("eqi,eqj->ij", el * w[..., None], eu)
+                    out[2] = gc.to(C.device).reshape(C.shape)
+            if needs[1]:
+                v, _ = field_points(basis, lam, vec=space.vec, gradients=False)
+                out[1] = w[..., None] * v[0]
+        for k, t in enumerate(tractions):
+            if not needs[3 + k]:
+                continue
+            from jaxiga_amd.methods import _neumann_sets
+
+            _, ps, _ = _neumann_sets(ctx, dev)[k]
+            basis = evaluate(space, ps, outputs=("R", "x", "w"))
+            v, _ = field_points(basis, lam, vec=space.vec, gradients=False)
+            out[3 + k] = basis.w[..., None] * v[0]
+        return out
+
+
+def _galerkin_fn():
+    import torch
+
+    class GalerkinSolve(torch.autograd.Function):
+        @staticmethod
+        def forward(fc, run, coef, src, C, *tractions):
+            fc.run = run
+            return run.forward()
+
+        @staticmethod
+        def backward(fc, u_bar):
+            grads = fc.run.backward(u_bar.detach().to(torch.float64).contiguous(), fc.needs_input_grad[1:])
+            return (None, *grads)
+
+    return GalerkinSolve
+
+
+_FN = []
+
+
+def galerkin_solve(ctx, params, linear, inputs):
+    """``(u, stats)``: the full solution as a tensor that carries the
+    gradient of every tensor in ``inputs`` (see the module docstring)."""
+    import torch
+
+    if not _FN:
+        _FN.append(_galerkin_fn())
+    run = _Run(ctx, params, linear, inputs)
+
+    def as_arg(v):
+        return v if isinstance(v, torch.Tensor) and v.requires_grad else None
+
+    tractions = inputs["tractions"] or []
+    u = _FN[0].apply(run, as_arg(inputs["coef"]), as_arg(inputs["src"]), as_arg(inputs["C"]),
+                     *[as_arg(t) for t in tractions])
+    return u, run.stats

@@ -1,0 +1,140 @@
+// Implicit differentiation of the Galerkin solve, and fields at points.
+//
+// The reverse pass of K(theta) u = f(theta) is one adjoint solve K lam = u_bar
+// on the same symmetric CSR (a Galerkin operator is symmetric, so no
+// transposed matrix is formed), followed by
+//   * the cotangent of the CSR values, data_bar[k] = -lam[row k] u[col k]
+//     (iga_csr_outer), and
+//   * the point sensitivities of the element kernels, formed from lam and
+//     the full solution u (prescribed values included) at the quadrature
+//     points: values and gradients of both fields from the basis data
+//     (iga_field_points). The same kernel evaluates a solution and its
+//     gradient at any point set for post-processing.
+#include "common.hpp"
+
+namespace iga {
+
+// out[k] = alpha * a[rows[k]] * b[cols[k]]
+__global__ void __launch_bounds__(256)
+    k_csr_outer(int64_t nnz, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+                const double* __restrict__ a, const double* __restrict__ b, double alpha,
+                double* __restrict__ out) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = alpha * a[rows[k]] * b[cols[k]];
+}
+
+// One thread per point p = (element e, point q): for each field f,
+//   val[f][p][c]     = sum_a R[p][a] u_f[dofs[e][a] * VEC + c]
+//   grad[f][p][c][d] = sum_a dR[p][d][a] u_f[dofs[e][a] * VEC + c]
+// R is (n_p, n_l), dR (n_p, DIM, n_l): each thread reads its own contiguous
+// rows, so consecutive threads read consecutive memory.
+template <int VEC, int DIM, int NF>
+__global__ void __launch_bounds__(256)
+    k_field_points(int64_t n_p, int n_q, int n_l, const double* __restrict__ R,
+                   const double* __restrict__ dR, const int32_t* __restrict__ dofs,
+                   const double* __restrict__ u0, const double* __restrict__ u1,
+                   double* __restrict__ val, double* __restrict__ grad) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_p) return;
+  const int64_t e = p / n_q;
+  const double* Rp = R + p * n_l;
+  const double* dRp = dR ? dR + p * DIM * n_l : nullptr;
+  const int32_t* dp = dofs + e * n_l;
+  double v[NF][VEC], g[NF][VEC][DIM];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) {
+      v[f][c] = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) g[f][c][d] = 0.0;
+    }
+#pragma unroll 4
+  for (int a = 0; a < n_l; ++a) {
+    const int64_t base = (int64_t)dp[a] * VEC;
+    double w[NF][VEC];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) {
+      w[0][c] = u0[base + c];
+      if (NF > 1) w[NF - 1][c] = u1[base + c];
+    }
+    if (val) {
+      const double r = Rp[a];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) v[f][c] += r * w[f][c];
+    }
+    if (dRp) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const double r = dRp[d * n_l + a];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int c = 0; c < VEC; ++c) g[f][c][d] += r * w[f][c];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    if (val)
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) val[((int64_t)f * n_p + p) * VEC + c] = v[f][c];
+    if (grad && dRp)
+#pragma unroll
+      for (int c = 0; c < VEC; ++c)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) grad[(((int64_t)f * n_p + p) * VEC + c) * DIM + d] = g[f][c][d];
+  }
+}
+
+template <int VEC, int DIM>
+static int field_points_nf(int nf, int64_t n_p, int n_q, int n_l, const double* R, const double* dR,
+                           const int32_t* dofs, const double* u0, const double* u1, double* val,
+                           double* grad, hipStream_t st) {
+  const dim3 grid((unsigned)((n_p + 255) / 256)), block(256);
+  if (nf == 1)
+    hipLaunchKernelGGL((k_field_points<VEC, DIM, 1>), grid, block, 0, st, n_p, n_q, n_l, R, dR, dofs,
+                       u0, u1, val, grad);
+  else
+    hipLaunchKernelGGL((k_field_points<VEC, DIM, 2>), grid, block, 0, st, n_p, n_q, n_l, R, dR, dofs,
+                       u0, u1, val, grad);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+}  // namespace iga
+
+using namespace iga;
+
+extern "C" int iga_csr_outer(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* a,
+                             const double* b, double alpha, double* out, void* stream) {
+  if (nnz == 0) return IGA_OK;
+  if (nnz < 0 || !rows || !cols || !a || !b || !out) return IGA_ERR_ARG;
+  const int64_t blocks = std::min<int64_t>((nnz + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_csr_outer, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, nnz,
+                     rows, cols, a, b, alpha, out);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}
+
+extern "C" int iga_field_points(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, int32_t vec,
+                                const double* R, const double* dR, const int32_t* dofs,
+                                const double* u0, const double* u1, double* val, double* grad,
+                                void* stream) {
+  if (n_e == 0 || n_q == 0) return IGA_OK;
+  if (n_e < 0 || n_q < 0 || n_l <= 0 || !dofs || !u0) return IGA_ERR_ARG;
+  if ((val && !R) || (grad && !dR) || (!val && !grad)) return IGA_ERR_ARG;
+  const int nf = u1 ? 2 : 1;
+  const int64_t n_p = (int64_t)n_e * n_q;
+  hipStream_t st = (hipStream_t)stream;
+  const double* Ru = val ? R : nullptr;
+  const double* dRu = grad ? dR : nullptr;
+  if (dim == 2 && vec == 1) return field_points_nf<1, 2>(nf, n_p, n_q, n_l, Ru, dRu, dofs, u0, u1, val, grad, st);
+  if (dim == 2 && vec == 2) return field_points_nf<2, 2>(nf, n_p, n_q, n_l, Ru, dRu, dofs, u0, u1, val, grad, st);
+  if (dim == 3 && vec == 1) return field_points_nf<1, 3>(nf, n_p, n_q, n_l, Ru, dRu, dofs, u0, u1, val, grad, st);
+  if (dim == 3 && vec == 3) return field_points_nf<3, 3>(nf, n_p, n_q, n_l, Ru, dRu, dofs, u0, u1, val, grad, st);
+  return IGA_ERR_UNSUPPORTED;
+}

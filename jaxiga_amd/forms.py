@@ -55,24 +55,39 @@ class PointField:
 # ---------------------------------------------------------------------------
 
 
+def _is_tensor(*xs):
+    return any(type(x).__module__.split(".")[0] == "torch" for x in xs)
+
+
+def _matrix(scale, rows):
+    """``scale * rows`` as a float64 torch tensor when an entry is a torch
+    tensor (so gradients reach ``E`` and ``nu``), else as a NumPy array."""
+    flat = [x for r in rows for x in r]
+    if not _is_tensor(scale, *flat):
+        return float(scale) * np.array(rows, float)
+    import torch
+
+    ref = next(x for x in (scale, *flat) if _is_tensor(x))
+    t = lambda x: torch.as_tensor(x, dtype=torch.float64, device=ref.device)  # noqa: E731
+    return t(scale) * torch.stack([torch.stack([t(x) for x in r]) for r in rows])
+
+
 class materials:
     @staticmethod
     def plane_stress(E, nu):
-        return float(E) / (1 - nu**2) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]], float)
+        return _matrix(E / (1 - nu**2), [[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
 
     @staticmethod
     def plane_strain(E, nu):
-        f = float(E) / ((1 + nu) * (1 - 2 * nu))
-        return f * np.array([[1 - nu, nu, 0], [nu, 1 - nu, 0], [0, 0, (1 - 2 * nu) / 2]], float)
+        return _matrix(E / ((1 + nu) * (1 - 2 * nu)),
+                       [[1 - nu, nu, 0], [nu, 1 - nu, 0], [0, 0, (1 - 2 * nu) / 2]])
 
     @staticmethod
     def isotropic_3d(E, nu):
-        f = float(E) / ((1 + nu) * (1 - 2 * nu))
-        C = np.zeros((6, 6))
-        C[:3, :3] = nu
-        C[0, 0] = C[1, 1] = C[2, 2] = 1 - nu
-        C[3, 3] = C[4, 4] = C[5, 5] = (1 - 2 * nu) / 2
-        return f * C
+        d, s = 1 - nu, (1 - 2 * nu) / 2
+        rows = [[d, nu, nu, 0, 0, 0], [nu, d, nu, 0, 0, 0], [nu, nu, d, 0, 0, 0],
+                [0, 0, 0, s, 0, 0], [0, 0, 0, 0, s, 0], [0, 0, 0, 0, 0, s]]
+        return _matrix(E / ((1 + nu) * (1 - 2 * nu)), rows)
 
     @staticmethod
     def lame(E, nu):

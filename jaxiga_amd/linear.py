@@ -215,11 +215,10 @@ def dense_from_csr(pattern: CSRPattern, data):
     return A
 
 
-def sparse_solve(opts: LinearOptions, pattern: CSRPattern, data, b):
-    """Solve ``A u = b`` on the GPU; returns ``(u, info)``."""
+def _solve_impl(opts: LinearOptions, pattern: CSRPattern, data, b):
+    """``A u = b`` on the GPU with the method of ``opts``; ``(u, info)``."""
     import torch
 
-    SOLVE_COUNTER["forward"] += 1
     if opts.method == "cg":
         if opts.preconditioner == "jacobi":
             res = cg_csr(pattern, data, b, opts.tol, opts.maxiter)
@@ -233,6 +232,74 @@ def sparse_solve(opts: LinearOptions, pattern: CSRPattern, data, b):
                              "backend; use LinearOptions(method='cg')")
         return torch.linalg.solve(dense_from_csr(pattern, data), b), {"iterations": None}
     raise NotImplementedError(f"linear method {opts.method!r} is not provided on this backend; use 'cg'")
+
+
+def csr_outer(pattern: CSRPattern, a, b, alpha=1.0):
+    """``alpha * a[rows] * b[cols]`` over the pattern's nonzeros (FP64)."""
+    import torch
+
+    out = torch.empty(pattern.nnz, dtype=torch.float64, device=a.device)
+    _lib.check(lib().iga_csr_outer(pattern.nnz, _lib.ptr(pattern.rows_d), _lib.ptr(pattern.cols_d),
+                                   _lib.ptr(a), _lib.ptr(b), float(alpha), _lib.ptr(out), stream()),
+               "iga_csr_outer")
+    return out
+
+
+def adjoint_solve(opts: LinearOptions, pattern: CSRPattern, data, u_bar):
+    """The one adjoint solve of a reverse pass, ``A lam = u_bar`` on the same
+    (symmetric) CSR; counted in ``SOLVE_COUNTER['adjoint']``."""
+    if not pattern.symmetric:
+        raise NotImplementedError("the CSR adjoint requires a symmetric pattern")
+    SOLVE_COUNTER["adjoint"] += 1
+    return _solve_impl(opts, pattern, data, u_bar.contiguous())[0]
+
+
+def _sparse_solve_fn():
+    import torch
+
+    class SparseSolve(torch.autograd.Function):
+        """``u = A(data)^-1 b`` with the implicit reverse rule of
+        ``_csr_sparse_solve`` (``src/jaxiga/solvers/linear.py:362-389``):
+        one adjoint solve ``A lam = u_bar`` on the same symmetric CSR, then
+        ``data_bar = -lam[rows] u[cols]`` and ``b_bar = lam``."""
+
+        @staticmethod
+        def forward(fc, opts, pattern, info, data, b):
+            u, inf = _solve_impl(opts, pattern, data.detach(), b.detach())
+            info.update(inf)
+            fc.opts, fc.pattern = opts, pattern
+            fc.save_for_backward(data.detach(), u)
+            return u
+
+        @staticmethod
+        def backward(fc, u_bar):
+            data, u = fc.saved_tensors
+            lam = adjoint_solve(fc.opts, fc.pattern, data, u_bar)
+            data_bar = None
+            if fc.needs_input_grad[3]:
+                data_bar = csr_outer(fc.pattern, lam, u, -1.0).to(data.dtype)
+            return None, None, None, data_bar, lam if fc.needs_input_grad[4] else None
+
+    return SparseSolve
+
+
+_SPARSE_SOLVE = []
+
+
+def sparse_solve(opts: LinearOptions, pattern: CSRPattern, data, b):
+    """Solve ``A u = b`` on the GPU; returns ``(u, info)``. Differentiable in
+    ``data`` and ``b`` (one adjoint solve in the reverse pass, never through
+    the iterations), as ``sparse_solve`` of ``src/jaxiga/solvers/linear.py:392-396``."""
+    import torch
+
+    SOLVE_COUNTER["forward"] += 1
+    if torch.is_grad_enabled() and (data.requires_grad or b.requires_grad):
+        if not _SPARSE_SOLVE:
+            _SPARSE_SOLVE.append(_sparse_solve_fn())
+        info = {}
+        u = _SPARSE_SOLVE[0].apply(opts, pattern, info, data, b)
+        return u, info
+    return _solve_impl(opts, pattern, data, b)
 
 
 def linear_diagnostics(residual_norm, b_norm, u, tol):

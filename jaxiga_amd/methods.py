@@ -134,44 +134,104 @@ def call_points(fn, x, *extra, params=None, vec=1, with_params=True):
     return out.contiguous()
 
 
-def _coefficient_args(ctx, params, args):
+def coefficient_values(ctx, params):
+    """The Poisson coefficient ``params['a0']`` as the element pass takes it:
+    None (a0 = 1), a number or 0-d tensor, or its values at the interior
+    Gauss points ``(n_e, n_q)``. Tensors keep their autograd history."""
     import torch
 
     value = None if params is None else params.get("a0") if isinstance(params, dict) else None
     if value is None:
-        args.coef_kind, args.coef_const = _lib.IGA_COEF_CONST, 1.0
         return None
     if isinstance(value, PointField):
-        vals = torch.as_tensor(value.values, dtype=torch.float64, device=ctx.f2f_d.device).contiguous()
+        vals = torch.as_tensor(value.values, dtype=torch.float64, device=ctx.f2f_d.device)
         space = ctx.space
         if tuple(vals.shape) != (space.n_elems, ctx.n_q1 ** space.dim):
             raise ValueError(f"PointField a0 has shape {tuple(vals.shape)}, expected "
                              f"{(space.n_elems, ctx.n_q1 ** space.dim)}")
-        args.coef_kind, args.coef_points = _lib.IGA_COEF_POINTS, vals.data_ptr()
         return vals
     if callable(value):
-        vals = call_points(value, ctx.gauss_points(), params=params)[..., 0].contiguous()
-        args.coef_kind, args.coef_points = _lib.IGA_COEF_POINTS, vals.data_ptr()
-        return vals
-    args.coef_kind, args.coef_const = _lib.IGA_COEF_CONST, float(value)
-    return None
+        return call_points(value, ctx.gauss_points(), params=params)[..., 0]
+    if isinstance(value, torch.Tensor):
+        if value.numel() != 1:
+            raise ValueError("a tensor a0 must hold one value; use a PointField for values at points")
+        return value.reshape(()).to(torch.float64)
+    return float(value)
 
 
-def _source_args(ctx, params, args):
+def _bind_coefficient(args, value):
+    import torch
+
+    if value is None:
+        args.coef_kind, args.coef_const = _lib.IGA_COEF_CONST, 1.0
+        return None
+    if not isinstance(value, torch.Tensor) or value.dim() == 0:
+        args.coef_kind, args.coef_const = _lib.IGA_COEF_CONST, float(value)
+        return None
+    vals = value.detach().to(torch.float64).contiguous()
+    args.coef_kind, args.coef_points = _lib.IGA_COEF_POINTS, vals.data_ptr()
+    return vals
+
+
+def _coefficient_args(ctx, params, args):
+    return _bind_coefficient(args, coefficient_values(ctx, params))
+
+
+def source_values(ctx, params):
+    """The source as the element pass takes it: None, a constant ``(vec,)``
+    array, or its values at the interior Gauss points ``(n_e, n_q, vec)``
+    (a callable, or a constant tensor that requires a gradient)."""
+    import torch
+
     src = ctx.problem.source
     vec = ctx.space.vec
     if src is None:
+        return None
+    if callable(src):
+        return call_points(src, ctx.gauss_points(), params=params, vec=vec)
+    if isinstance(src, torch.Tensor) and src.requires_grad:
+        xq = ctx.gauss_points()
+        return src.to(torch.float64).reshape(-1).expand(vec).expand(*xq.shape[:-1], vec)
+    return np.broadcast_to(np.asarray(src, float), (vec,))
+
+
+def _bind_source(args, value, vec):
+    import torch
+
+    if value is None:
         args.src_kind = _lib.IGA_SRC_NONE
         return None
-    if not callable(src):
-        v = np.broadcast_to(np.asarray(src, float), (vec,))
+    if not isinstance(value, torch.Tensor):
         args.src_kind = _lib.IGA_SRC_CONST
         for c in range(vec):
-            args.src_const[c] = float(v[c])
+            args.src_const[c] = float(value[c])
         return None
-    vals = call_points(src, ctx.gauss_points(), params=params, vec=vec).contiguous()
+    vals = value.detach().to(torch.float64).contiguous()
     args.src_kind, args.src_points = _lib.IGA_SRC_POINTS, vals.data_ptr()
     return vals
+
+
+def _source_args(ctx, params, args):
+    return _bind_source(args, source_values(ctx, params), ctx.space.vec)
+
+
+def traction_values(ctx, params):
+    """The Neumann tractions at the boundary Gauss points of every Neumann
+    set, ``(n_e_b, n_q_b, vec)`` each, in the order of the load's sets."""
+    space = ctx.space
+    out = []
+    for bc, ps, _ in _neumann_sets(ctx, ctx.f2f_d.device):
+        basis = evaluate(space, ps, outputs=("x", "normal"))
+        out.append(call_points(bc.value, basis.x, basis.normal, params=params, vec=space.vec))
+    return out
+
+
+def inputs_require_grad(inputs):
+    """Whether a value of :func:`point_inputs` carries an autograd history."""
+    import torch
+
+    vals = [inputs.get("coef"), inputs.get("src"), inputs.get("C"), *(inputs.get("tractions") or [])]
+    return any(isinstance(v, torch.Tensor) and v.requires_grad for v in vals)
 
 
 @dataclasses.dataclass
@@ -183,6 +243,7 @@ class AssemblyLaunch:
     rhs: object
     diag: object
     keep: list
+    tractions: list | None = None  # Neumann tractions given by the caller
 
     def launch(self, ctx, passes=0):
         self.args.passes = passes
@@ -191,11 +252,13 @@ class AssemblyLaunch:
 
 
 def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagonal=False,
-             precision=_lib.IGA_FP64):
+             precision=_lib.IGA_FP64, inputs=None):
     """CSR values, free load (source work + lifting + Neumann work) and/or the
-    diagonal. Returns ``(vals, rhs, diag)`` (tensors or None)."""
+    diagonal. Returns ``(vals, rhs, diag)`` (tensors or None). ``inputs``
+    (optional) holds the parameter-dependent values already evaluated
+    (:func:`point_inputs`); their detached copies are assembled."""
     job = prepare_assembly(ctx, params, values=values, load=load, diagonal=diagonal,
-                           precision=precision)
+                           precision=precision, inputs=inputs)
     run_assembly(ctx, params, job, load=load)
     return job.vals, job.rhs, job.diag
 
@@ -225,7 +288,7 @@ def run_assembly(ctx, params, job, *, load=True, passes=(0,), on_pass=None):
             # issued after the element pass, so the host prepares the
             # boundary work while the GPU runs the element pass
             with torch.cuda.stream(side):
-                neumann_load(ctx, params, job.rhs)
+                neumann_load(ctx, params, job.rhs, job.tractions)
             done = torch.cuda.Event()
             done.record(side)
     if done is not None:
@@ -300,8 +363,33 @@ class CapturedAssembly:
             g.replay()
 
 
+def point_inputs(ctx: MethodContext, params=None, load=True):
+    """The parameter-dependent inputs of the assembly, evaluated with autograd
+    on: ``coef`` (Poisson), ``C`` (elasticity), ``src`` and ``tractions``
+    (with ``load``). See :func:`coefficient_values`, :func:`source_values`,
+    :func:`traction_values`."""
+    problem = ctx.problem
+    out = {"coef": None, "C": None, "src": None, "tractions": None}
+    if problem.kernel == "poisson":
+        out["coef"] = coefficient_values(ctx, params)
+    elif problem.kernel == "elasticity":
+        out["C"] = problem.C(params)
+    if load:
+        out["src"] = source_values(ctx, params)
+        out["tractions"] = traction_values(ctx, params) if problem.neumann else []
+    return out
+
+
+def _bind_cmat(args, C):
+    import torch
+
+    C = C.detach().cpu().numpy() if isinstance(C, torch.Tensor) else np.asarray(C, float)
+    for i, c in enumerate(np.asarray(C, float).ravel()):
+        args.cmat[i] = float(c)
+
+
 def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True, diagonal=False,
-                     precision=_lib.IGA_FP64) -> AssemblyLaunch:
+                     precision=_lib.IGA_FP64, inputs=None) -> AssemblyLaunch:
     import torch
 
     space = ctx.space
@@ -314,12 +402,11 @@ def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True,
     keep = []
     if problem.kernel == "poisson":
         args.problem = _lib.IGA_POISSON
-        keep.append(_coefficient_args(ctx, params, args))
+        coef = inputs["coef"] if inputs is not None else coefficient_values(ctx, params)
+        keep.append(_bind_coefficient(args, coef))
     elif problem.kernel == "elasticity":
         args.problem = _lib.IGA_ELASTICITY
-        C = np.asarray(problem.C(params), float).ravel()
-        for i, c in enumerate(C):
-            args.cmat[i] = c
+        _bind_cmat(args, inputs["C"] if inputs is not None else problem.C(params))
     else:
         raise NotImplementedError(
             f"{type(problem).__name__} has no GPU element kernel; the Galerkin hot path covers "
@@ -330,7 +417,8 @@ def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True,
     rhs = torch.zeros(ctx.n_free, **f64) if load else None
     diag = torch.zeros(ctx.n_free, **f64) if diagonal else None
     if load:
-        keep.append(_source_args(ctx, params, args))
+        src = inputs["src"] if inputs is not None else source_values(ctx, params)
+        keep.append(_bind_source(args, src, space.vec))
     args.full_to_free = ctx.f2f_d.data_ptr()
     args.freed = ctx.free_d.data_ptr() if ctx.n_free else None
     args.n_free = ctx.n_free
@@ -347,7 +435,10 @@ def prepare_assembly(ctx: MethodContext, params=None, *, values=True, load=True,
         args.ke_work_bytes = ws.numel()
     args.rhs = rhs.data_ptr() if load and ctx.n_free else None
     args.diag = diag.data_ptr() if diagonal and ctx.n_free else None
-    return AssemblyLaunch(args, vals, rhs, diag, keep)
+    tractions = None
+    if inputs is not None and inputs.get("tractions") is not None:
+        tractions = [t.detach().contiguous() for t in inputs["tractions"]]
+    return AssemblyLaunch(args, vals, rhs, diag, keep, tractions)
 
 
 def element_workspace(ctx, precision=_lib.IGA_FP64):
@@ -385,13 +476,18 @@ def _neumann_sets(ctx, dev):
     return ctx._neumann
 
 
-def neumann_load(ctx, params, rhs):
+def neumann_load(ctx, params, rhs, tractions=None):
     """Neumann work: the boundary basis is evaluated on the GPU at every call,
-    the traction at the boundary points, then scattered into the free load."""
+    the traction at the boundary points (or taken from ``tractions``, one
+    array per set), then scattered into the free load."""
     space = ctx.space
-    for bc, ps, edofs in _neumann_sets(ctx, rhs.device):
-        basis = evaluate(space, ps, outputs=("R", "x", "w", "normal"))
-        t = call_points(bc.value, basis.x, basis.normal, params=params, vec=space.vec)
+    for k, (bc, ps, edofs) in enumerate(_neumann_sets(ctx, rhs.device)):
+        if tractions is not None:
+            basis = evaluate(space, ps, outputs=("R", "w"))
+            t = tractions[k]
+        else:
+            basis = evaluate(space, ps, outputs=("R", "x", "w", "normal"))
+            t = call_points(bc.value, basis.x, basis.normal, params=params, vec=space.vec)
         _lib.check(lib().iga_load_points(ps.n_elems, ps.n_q, space.n_local, space.vec,
                                          _lib.ptr(basis.R), _lib.ptr(basis.w), _lib.ptr(t),
                                          _lib.ptr(edofs), _lib.ptr(ctx.f2f_d), _lib.ptr(rhs), stream()),
@@ -436,12 +532,26 @@ def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadra
     ctx = build_context(problem, params, quadrature, chunk)
     if linear is None:
         linear = default_linear(problem, ctx.n_free)
+    # with autograd on, the parameter-dependent values at points are
+    # evaluated once here; if any carries a gradient (a tensor in params, or
+    # one a callable closes over), the solve is differentiated implicitly:
+    # the reverse pass is one adjoint solve
+    inputs = point_inputs(ctx, params) if torch.is_grad_enabled() else None
+    traced = inputs is not None and inputs_require_grad(inputs)
     if linear.matrix_free:
         from jaxiga_amd.matfree import matrix_free_solve_ctx
 
+        if traced:
+            raise NotImplementedError("gradients through the matrix-free solve are not provided; "
+                                      "use the assembled CSR solve (matrix_free=False)")
         return matrix_free_solve_ctx(ctx, params, linear)
+    if traced:
+        from jaxiga_amd import adjoint as AD
+
+        u, stats = AD.galerkin_solve(ctx, params, linear, inputs)
+        return u, ctx, stats
     t0 = time.perf_counter()
-    vals, b, _ = assemble(ctx, params)
+    vals, b, _ = assemble(ctx, params, inputs=inputs)
     u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
     rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
     torch.cuda.synchronize()

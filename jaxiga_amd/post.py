@@ -15,6 +15,7 @@ import dataclasses
 
 import numpy as np
 
+from jaxiga_amd.adjoint import field_points
 from jaxiga_amd.evaluation import evaluate
 from jaxiga_amd.pointset import gauss
 from jaxiga_amd.space import expand_dofs
@@ -35,19 +36,31 @@ class Solution:
         idx = torch.as_tensor(expand_dofs(np.asarray(basis.dofs), vec), device=self.u.device)
         return self.u[idx].reshape(basis.n_elems, basis.n_local, vec)
 
+    def _traced(self):
+        import torch
+
+        return torch.is_grad_enabled() and self.u.requires_grad
+
     def at(self, ps=None, basis=None):
+        """Values ``(n_e, n_q, vec)`` at a point set (the field kernel; the
+        differentiable contraction when ``u`` carries a gradient)."""
         import torch
 
         basis = basis if basis is not None else evaluate(self.space, ps)
-        return torch.einsum("eqn,enc->eqc", basis.R, self._local(basis))
+        if self._traced():
+            return torch.einsum("eqn,enc->eqc", basis.R, self._local(basis))
+        return field_points(basis, self.u, vec=self.space.vec, gradients=False)[0][0]
 
     __call__ = at
 
     def grad(self, ps=None, basis=None):
+        """Gradients ``(n_e, n_q, vec, dim)`` at a point set."""
         import torch
 
         basis = basis if basis is not None else evaluate(self.space, ps)
-        return torch.einsum("eqdn,enc->eqcd", basis.dR, self._local(basis))
+        if self._traced():
+            return torch.einsum("eqdn,enc->eqcd", basis.dR, self._local(basis))
+        return field_points(basis, self.u, vec=self.space.vec, values=False)[1][0]
 
 
 def _user(fn, x, shape):

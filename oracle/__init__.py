@@ -21,6 +21,7 @@ pipeline   end-to-end ``solve`` for the configurations used by the tests
 grf        seeded random-Fourier-feature Gaussian random fields (inputs)
 learned    networks of the learned preconditioner (forward, exact gradients)
 multigrid  knot coarsening, Galerkin levels, stencil smoothers, V-cycle, FCG
+adjoint    reverse pass of the solve: CSR reverse rule, element and point cotangents
 
 Pinning status of every function is listed in ``PINNING`` below; each entry
 names the test module that pins it against closed forms, SciPy, or brute
@@ -50,4 +51,6 @@ PINNING = {
     "multigrid.stencil_apply/wgrad_t": "tests/test_oracle_multigrid.py (adjointness, finite differences, dense stencil)",
     "multigrid.stencil_features": "tests/test_oracle_multigrid.py (dense-matrix restatement)",
     "multigrid.vcycle/fcg": "tests/test_oracle_multigrid.py (symmetry, exact two-level solve, dense solve)",
+    "adjoint.csr_solve_vjp": "tests/test_oracle_adjoint.py (central finite differences of the direct solve in data and b)",
+    "adjoint.element_cotangent/point_cotangents/boundary_cotangent": "tests/test_oracle_adjoint.py (finite differences in a0 at points, source amplitude, E, nu, traction; non-zero Dirichlet data)",
 }
