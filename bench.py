@@ -303,6 +303,148 @@ def batched_bench(n_instances=512, log=print):
             "elements": bp.n_elements}
 
 
+def adjoint_bench(refine=8, reps=3, log=print):
+    """Reverse pass of the Galerkin solve (implicit differentiation) on the
+    cfg5 Poisson problem: unit square, p=3, 256^2 elements, a = exp(GRF l=0.1
+    sigma=1.5) as values at the 16 Gauss points of every element that carry a
+    gradient, loss c . u. Forward (assembly + Jacobi-CG to 1e-10) and reverse
+    pass (one adjoint Jacobi-CG, basis gradients at the Gauss points, the
+    fields-at-points kernel, the point cotangents), medians of ``reps`` after a
+    warm-up; the full-size gradient checked by a central difference in a
+    random direction; ``k_field_points`` timed alone with HIP events."""
+    import torch
+
+    import jaxiga_amd as jx
+    from jaxiga_amd import adjoint as AD
+    from jaxiga_amd import grf
+    from jaxiga_amd import linear as LIN
+    from jaxiga_amd.evaluation import evaluate
+    from jaxiga_amd.pointset import gauss
+
+    ctx, _, _ = grf.poisson_instance(refine, 0)
+    prob, V = ctx.problem, ctx.space
+    a = grf.gauss_point_field(ctx, 0, 0.1, 1.5).exp_()
+    lin = jx.LinearOptions(method="cg", preconditioner="jacobi", tol=1e-10)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    c = torch.randn(V.n_dofs, dtype=torch.float64, device="cuda", generator=gen)
+
+    def run():
+        a_t = a.clone().requires_grad_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = jx.solve(prob, params={"a0": jx.PointField(a_t)}, linear=lin)
+        loss = (c * sol.u).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        LIN.reset_solve_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return a_t.grad, t1 - t0, t2 - t1, sol, dict(LIN.SOLVE_COUNTER)
+
+    run()
+    runs = [run() for _ in range(reps)]
+    g, _, _, sol, counter = runs[-1]
+    t_fwd = statistics.median(r[1] for r in runs)
+    t_bwd = statistics.median(r[2] for r in runs)
+    # the parts of the reverse pass, timed alone
+    free = ctx.free_d.long()
+    vals, b, _ = jx.methods.assemble(ctx, {"a0": jx.PointField(a)})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lam_free = LIN.adjoint_solve(lin, ctx.assembly, vals, c[free])
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    lam = torch.zeros_like(sol.u)
+    lam[free] = lam_free
+    t0 = time.perf_counter()
+    basis = evaluate(V, gauss(V, ctx.n_q1), outputs=("dR", "x", "w"))
+    torch.cuda.synchronize()
+    t_basis = time.perf_counter() - t0
+    AD.field_points(basis, sol.u, lam, 1, values=False)
+    k_reps = 20
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(k_reps):
+        AD.field_points(basis, sol.u, lam, 1, values=False)
+    ev[1].record()
+    torch.cuda.synchronize()
+    t_k = ev[0].elapsed_time(ev[1]) / k_reps * 1e-3
+    n_e, n_q, dim, n_l = basis.dR.shape
+    n_p = n_e * n_q
+    k_bytes = n_p * dim * n_l * 8 + n_e * n_l * 4 + 2 * V.n_dofs * 8 + 2 * n_p * dim * 8
+    # directional check of the full-size gradient
+    d = torch.randn(a.shape, dtype=torch.float64, device="cuda", generator=gen) * a
+    h = 1e-4
+
+    def loss_at(aa):
+        with torch.no_grad():
+            return float((c * jx.solve(prob, params={"a0": jx.PointField(aa)}, linear=lin).u).sum())
+
+    fd = (loss_at(a + h * d) - loss_at(a - h * d)) / (2 * h)
+    ad = float((g * d).sum())
+    out = {
+        "workload": f"unit-square Poisson, p=3, {2**refine}x{2**refine} elements, a=exp(GRF l=0.1 sigma=1.5) "
+                    f"at {n_p} Gauss points (the differentiated input), loss c.u, Jacobi-CG to 1e-10",
+        "elements": n_e, "forward_seconds": t_fwd, "reverse_seconds": t_bwd,
+        "reverse_parts_seconds": {"adjoint_cg": t_adj, "basis_gradients": t_basis, "field_points": t_k},
+        "solves_per_gradient": counter,
+        "directional_check": {"adjoint": ad, "central_difference": fd, "rel_diff": abs(ad - fd) / abs(fd)},
+        "roofline": {"bound": "hbm", "kernel": "k_field_points<1,2,2>", "kernel_ms": t_k * 1e3,
+                     "achieved": k_bytes / t_k / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": k_bytes / t_k / 1e9 / 8000.0,
+                     "algorithmic": f"{n_p} points x ({dim}x{n_l} basis gradients x 8 B + 2 fields x {dim} "
+                                    f"gradients x 8 B) + {n_e}x{n_l} dof indices x 4 B + 2 x {V.n_dofs} "
+                                    f"coefficients x 8 B", "traffic": None},
+    }
+    log(f"[bench] adjoint: forward {t_fwd * 1e3:.1f} ms, reverse {t_bwd * 1e3:.1f} ms (adjoint CG "
+        f"{t_adj * 1e3:.1f} ms, basis {t_basis * 1e3:.2f} ms, k_field_points {t_k * 1e6:.1f} us = "
+        f"{k_bytes / t_k / 1e9:.0f} GB/s); solves {counter}; d/dir {ad:.8e} vs central difference {fd:.8e}")
+    return out
+
+
+def adjoint_cpu_baseline(sample_refine=6):
+    """The oracle's reverse pass (adjoint direct solve, element and point
+    cotangents) for the same Poisson problem at 2**sample_refine elements per
+    direction, one core; median of 3 after a warm-up."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import adjoint as OJ
+    from oracle import forms as OF
+    from oracle import geometry as OG
+    from oracle import grf as OGR
+    from oracle import pipeline as OP
+    from oracle import space as OS
+
+    VO = OS.function_space(OG.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]]).elevate(3).refine(sample_refine))
+    x = OS.gauss_basis(VO).x
+    om, ph = OGR.rff_params(0, 2, 0.1)
+    a0 = np.exp(OGR.rff_field(x.reshape(-1, 2), om, ph, 1.5).reshape(x.shape[:2]))
+    out = OP.solve("poisson", VO, [OF.DirichletBC(0.0, lambda xx: np.ones(len(xx), bool))],
+                   source=lambda xx, p: 1.0 + 0.0 * xx[0], params={"a0": OF.PointField(a0)}, method="direct")
+    c = np.random.default_rng(7).standard_normal(VO.n_dofs)
+    gd = OF.gdofs(VO)
+    dm = out["dofmap"]
+
+    def once():
+        db, lam = OJ.csr_solve_vjp(out["pattern"], out["data"], out["u_free"], c[dm.free])
+        Kbar, lb = OJ.element_cotangent(out["pattern"], db, lam, gd, dm)
+        return OJ.point_cotangents("poisson", out["basis"], Kbar, lb, gd, 1)
+
+    with threadpool_limits(1):
+        once()
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            once()
+            times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    return {"value": VO.n_elems / t / 1e6, "unit": "Melem/s (reverse pass)", "cores": 1, "kind": "port",
+            "sample": f"oracle reverse pass (adjoint direct solve + element and point cotangents) of the "
+                      f"Poisson problem at {2**sample_refine}x{2**sample_refine} elements, median of 3 runs "
+                      f"of {t:.3f} s"}
+
+
 def assembly_run(ctx, params, prec_name, steps, warmup):
     """Time ``steps`` full assemblies of the operator (element pass, row gather,
     load with its boundary basis) with inputs resident; HIP events on the
@@ -414,6 +556,8 @@ def main():
                     help="unseen coefficient seeds (10000, 10001, ...) of the learned-PC comparison")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the matrix-free (3-D) and batched sections")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the reverse-pass (implicit differentiation) measurement")
     ap.add_argument("--extras-only", action="store_true",
                     help="only run the matrix-free (3-D) and batched sections")
     args = ap.parse_args()
@@ -443,6 +587,13 @@ def main():
         print(json.dumps({"learned_pc": learned_pc_bench(
             n_train=args.learned_fields, steps=args.learned_steps,
             test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)))}), flush=True)
+        return
+    if args.adjoint_only:
+        adj = adjoint_bench()
+        if not args.no_cpu:
+            adj["cpu_baseline"] = adjoint_cpu_baseline()
+            adj["gpu_reverse_Melem_per_s"] = adj["elements"] / adj["reverse_seconds"] / 1e6
+        print(json.dumps({"adjoint": adj}), flush=True)
         return
     if args.extras_only:
         print(json.dumps({"matfree": matfree_bench(), "batched": batched_bench()}), flush=True)
@@ -507,6 +658,12 @@ def main():
         n_train=args.learned_fields, steps=args.learned_steps,
         test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)))
     extras = {} if (args.no_extras or args.no_solve) else {"matfree": matfree_bench(), "batched": batched_bench()}
+    if not (args.no_extras or args.no_solve):
+        adj = adjoint_bench()
+        if not args.no_cpu:
+            adj["cpu_baseline"] = adjoint_cpu_baseline()
+            adj["gpu_reverse_Melem_per_s"] = adj["elements"] / adj["reverse_seconds"] / 1e6
+        extras["adjoint"] = adj
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "assembly Melem/s; time-to-solve @1e-10 rel res, learned-PC vs plain CG",

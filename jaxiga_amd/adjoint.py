@@ -60,7 +60,11 @@ def field_points(basis, u0, u1=None, vec=1, values=True, gradients=True):
     dim = basis.dR.shape[2] if basis.dR is not None else _dim(basis)
     dev = u0.device
     nf = 1 if u1 is None else 2
-    dofs = torch.as_tensor(np.ascontiguousarray(basis.dofs, dtype=np.int32), device=dev)
+    dofs = basis.__dict__.get("_dofs_dev")
+    if dofs is None or dofs.device != dev:
+        # the element dof table on the device, kept with the basis data
+        dofs = torch.as_tensor(np.ascontiguousarray(basis.dofs, dtype=np.int32), device=dev)
+        basis.__dict__["_dofs_dev"] = dofs
     u0 = u0.detach().to(torch.float64).contiguous()
     u1 = None if u1 is None else u1.detach().to(torch.float64).contiguous()
     val = torch.empty((nf, n_e, n_q, vec), dtype=torch.float64, device=dev) if values else None

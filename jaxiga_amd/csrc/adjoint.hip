@@ -90,10 +90,133 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same fields through LDS, for elements whose basis data fits: a block
+// takes `eb` consecutive elements, copies their R / dR rows (contiguous in
+// HBM) into padded LDS rows with batched loads, gathers the coefficients of
+// their functions once, and each thread forms rows (e, q) of values and
+// (e, q, d) of gradients from LDS. Rows are padded to n_l + 1 doubles so
+// that threads reading different rows hit different banks.
+constexpr int kFpThreads = 256;
+constexpr int kFpBatch = 8;
+
+__device__ __forceinline__ void fp_stage_rows(const double* __restrict__ src, int tot, int n_l,
+                                              double* __restrict__ dst) {
+  const int ld = n_l + 1;
+  for (int base = 0; base < tot; base += kFpThreads * kFpBatch) {
+    double v[kFpBatch];
+#pragma unroll
+    for (int k = 0; k < kFpBatch; ++k) {
+      const int i = base + k * kFpThreads + (int)threadIdx.x;
+      v[k] = src[min(i, tot - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < kFpBatch; ++k) {
+      const int i = base + k * kFpThreads + (int)threadIdx.x;
+      if (i < tot) dst[(i / n_l) * ld + i % n_l] = v[k];
+    }
+  }
+}
+
+template <int VEC, int DIM, int NF>
+__global__ void __launch_bounds__(kFpThreads)
+    k_field_points_lds(int n_e, int n_q, int n_l, int eb, const double* __restrict__ R,
+                       const double* __restrict__ dR, const int32_t* __restrict__ dofs,
+                       const double* __restrict__ u0, const double* __restrict__ u1,
+                       double* __restrict__ val, double* __restrict__ grad) {
+  extern __shared__ double fp_smem[];
+  const int e0 = blockIdx.x * eb;
+  const int ne = min(eb, n_e - e0);
+  const int ld = n_l + 1;
+  const int64_t n_p = (int64_t)n_e * n_q;
+  double* cs = fp_smem;                          // [NF][eb * n_l][VEC]
+  double* rs = cs + NF * eb * n_l * VEC;         // [eb * n_q][ld]        (values)
+  double* gs = rs + (val ? eb * n_q * ld : 0);   // [eb * n_q * DIM][ld]  (gradients)
+  for (int i = threadIdx.x; i < ne * n_l; i += kFpThreads) {
+    const int64_t cp = dofs[(int64_t)e0 * n_l + i];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) {
+      cs[i * VEC + c] = u0[cp * VEC + c];
+      if (NF > 1) cs[(eb * n_l + i) * VEC + c] = u1[cp * VEC + c];
+    }
+  }
+  if (val) fp_stage_rows(R + (int64_t)e0 * n_q * n_l, ne * n_q * n_l, n_l, rs);
+  if (grad) fp_stage_rows(dR + (int64_t)e0 * n_q * DIM * n_l, ne * n_q * DIM * n_l, n_l, gs);
+  __syncthreads();
+  if (val)
+    for (int r = threadIdx.x; r < ne * n_q; r += kFpThreads) {
+      const int e = r / n_q;
+      double acc[NF][VEC];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc[f][c] = 0.0;
+      for (int a = 0; a < n_l; ++a) {
+        const double w = rs[r * ld + a];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int c = 0; c < VEC; ++c) acc[f][c] += w * cs[((f * eb + e) * n_l + a) * VEC + c];
+      }
+      const int64_t p = (int64_t)e0 * n_q + r;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) val[((int64_t)f * n_p + p) * VEC + c] = acc[f][c];
+    }
+  if (grad)
+    for (int r = threadIdx.x; r < ne * n_q * DIM; r += kFpThreads) {
+      const int e = r / (n_q * DIM), d = r % DIM;
+      double acc[NF][VEC];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc[f][c] = 0.0;
+      for (int a = 0; a < n_l; ++a) {
+        const double w = gs[r * ld + a];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int c = 0; c < VEC; ++c) acc[f][c] += w * cs[((f * eb + e) * n_l + a) * VEC + c];
+      }
+      const int64_t p = (int64_t)e0 * n_q + r / DIM;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) grad[(((int64_t)f * n_p + p) * VEC + c) * DIM + d] = acc[f][c];
+    }
+}
+
+// elements per block of the LDS kernel (0: the basis data of one element
+// does not fit in 64 KB of LDS)
+static int field_points_eb(int n_q, int n_l, int dim, int vec, int nf, bool val, bool grad) {
+  const int64_t per = (int64_t)nf * n_l * vec + (val ? (int64_t)n_q * (n_l + 1) : 0) +
+                      (grad ? (int64_t)n_q * dim * (n_l + 1) : 0);
+  const int64_t fit = (64 * 1024) / (8 * per);
+  const int rows = n_q * (grad ? dim : 1);
+  const int want = std::max(1, kFpThreads / std::max(rows, 1));
+  return (int)std::min<int64_t>(fit, want);
+}
+
 template <int VEC, int DIM>
 static int field_points_nf(int nf, int64_t n_p, int n_q, int n_l, const double* R, const double* dR,
                            const int32_t* dofs, const double* u0, const double* u1, double* val,
                            double* grad, hipStream_t st) {
+  const int n_e = (int)(n_p / n_q);
+  const int eb = field_points_eb(n_q, n_l, DIM, VEC, nf, val != nullptr, grad != nullptr);
+  if (eb >= 1) {
+    const int64_t per = (int64_t)nf * n_l * VEC + (val ? (int64_t)n_q * (n_l + 1) : 0) +
+                        (grad ? (int64_t)n_q * DIM * (n_l + 1) : 0);
+    const size_t lds = (size_t)(8 * per * eb);
+    const dim3 g((unsigned)((n_e + eb - 1) / eb)), b(kFpThreads);
+    if (nf == 1)
+      hipLaunchKernelGGL((k_field_points_lds<VEC, DIM, 1>), g, b, lds, st, n_e, n_q, n_l, eb, R, dR,
+                         dofs, u0, u1, val, grad);
+    else
+      hipLaunchKernelGGL((k_field_points_lds<VEC, DIM, 2>), g, b, lds, st, n_e, n_q, n_l, eb, R, dR,
+                         dofs, u0, u1, val, grad);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
+  }
   const dim3 grid((unsigned)((n_p + 255) / 256)), block(256);
   if (nf == 1)
     hipLaunchKernelGGL((k_field_points<VEC, DIM, 1>), grid, block, 0, st, n_p, n_q, n_l, R, dR, dofs,
