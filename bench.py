@@ -390,12 +390,14 @@ def adjoint_bench(refine=8, reps=3, log=print):
         "reverse_parts_seconds": {"adjoint_cg": t_adj, "basis_gradients": t_basis, "field_points": t_k},
         "solves_per_gradient": counter,
         "directional_check": {"adjoint": ad, "central_difference": fd, "rel_diff": abs(ad - fd) / abs(fd)},
-        "roofline": {"bound": "hbm", "kernel": "k_field_points<1,2,2>", "kernel_ms": t_k * 1e3,
+        "roofline": {"bound": "hbm", "kernel": "k_field_points_lds<1,2,2>", "kernel_ms": t_k * 1e3,
                      "achieved": k_bytes / t_k / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": k_bytes / t_k / 1e9 / 8000.0,
                      "algorithmic": f"{n_p} points x ({dim}x{n_l} basis gradients x 8 B + 2 fields x {dim} "
                                     f"gradients x 8 B) + {n_e}x{n_l} dof indices x 4 B + 2 x {V.n_dofs} "
-                                    f"coefficients x 8 B", "traffic": None},
+                                    f"coefficients x 8 B",
+                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "adjoint_poisson_256.md")).get(
+                         "k_field_points_lds<1,2,2>")},
     }
     log(f"[bench] adjoint: forward {t_fwd * 1e3:.1f} ms, reverse {t_bwd * 1e3:.1f} ms (adjoint CG "
         f"{t_adj * 1e3:.1f} ms, basis {t_basis * 1e3:.2f} ms, k_field_points {t_k * 1e6:.1f} us = "

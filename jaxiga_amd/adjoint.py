@@ -28,7 +28,9 @@ pass therefore costs exactly one adjoint solve (``SOLVE_COUNTER['adjoint']``),
 whatever the forward iteration did, and torch carries the point cotangents
 back to the parameters through the user's callables. Dirichlet values are
 resolved at set-up and are not differentiated (as in the reference); nor are
-control points or weights (geometry enters the kernels as fixed data).
+control points or weights (geometry enters the kernels as fixed data). With
+``matrix_free=True`` both solves are the matrix-free CG (the symmetric
+operator is its own transpose, ``src/jaxiga/solvers/linear.py:436-442``).
 """
 
 from __future__ import annotations
@@ -110,18 +112,47 @@ class _Run:
         ctx, linear = self.ctx, self.linear
         t0 = time.perf_counter()
         with torch.no_grad():
-            vals, b, _ = assemble(ctx, self.params, inputs=self.inputs)
-            u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
-            rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
+            if linear.matrix_free:
+                u_free, info, rn, bn = self._forward_matrix_free()
+            else:
+                vals, b, _ = assemble(ctx, self.params, inputs=self.inputs)
+                u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
+                rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
+                # the reverse pass: one solve on the same symmetric CSR
+                self._adjoint = lambda ub: LIN.adjoint_solve(linear, ctx.assembly, vals, ub)
             u = ctx.dofmap.lift(u_free)
         torch.cuda.synchronize()
-        stats = {"method": "galerkin", "linear": linear.method, "matrix_free": False,
-                 "n_free": ctx.n_free, "nnz": ctx.assembly.nnz, **info,
+        stats = {"method": "galerkin", "linear": linear.method, "matrix_free": bool(linear.matrix_free),
+                 "n_free": ctx.n_free, "nnz": None if linear.matrix_free else ctx.assembly.nnz, **info,
                  "wall_time": time.perf_counter() - t0}
         stats.update(LIN.linear_diagnostics(rn, bn, u_free, linear.tol))
         LIN.warn_if_failed("linear Galerkin solve", stats["converged"], stats["residual"])
-        self.stats, self.vals, self.u = stats, vals, u
+        self.stats, self.u = stats, u
         return u
+
+    def _forward_matrix_free(self):
+        """The matrix-free branch (``src/jaxiga/solvers/linear.py:414-442``):
+        the adjoint of the symmetric operator is the same matrix-free CG."""
+        from jaxiga_amd.matfree import MatrixFreeOperator, diff_norms, inverse_guard
+        from jaxiga_amd.methods import assemble
+
+        ctx, linear = self.ctx, self.linear
+        if linear.method != "cg":
+            raise NotImplementedError(f"matrix-free method {linear.method!r} is not provided; use 'cg'")
+        _, b, diag = assemble(ctx, self.params, values=False, load=True,
+                              diagonal=linear.preconditioner == "jacobi", inputs=self.inputs)
+        op = MatrixFreeOperator(ctx, self.params, inputs=self.inputs)
+        dinv = inverse_guard(diag) if diag is not None else None
+        LIN.SOLVE_COUNTER["forward"] += 1
+        res = op.cg(b, linear.tol, linear.maxiter, dinv=dinv)
+        rn, bn = diff_norms(op.apply(res.x), b)
+
+        def adjoint(ub):
+            LIN.SOLVE_COUNTER["adjoint"] += 1
+            return op.cg(ub.contiguous(), linear.tol, linear.maxiter, dinv=dinv).x
+
+        self._adjoint = adjoint
+        return res.x, {"iterations": res.iterations}, rn, bn
 
     def backward(self, u_bar, needs):
         """Cotangents of ``(coef, src, C, *tractions)`` (None where not needed)."""
@@ -134,7 +165,7 @@ class _Run:
         if free is None:
             free = ctx.free_d.long()
             ctx._free_long = free
-        lam_free = LIN.adjoint_solve(self.linear, ctx.assembly, self.vals, u_bar[free])
+        lam_free = self._adjoint(u_bar[free])
         lam = torch.zeros(space.n_dofs, dtype=torch.float64, device=dev)
         lam[free] = lam_free
         coef, src, C = self.inputs["coef"], self.inputs["src"], self.inputs["C"]

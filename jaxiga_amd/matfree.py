@@ -25,10 +25,12 @@ from jaxiga_amd.runtime import lib, stream
 class MatrixFreeOperator:
     """``y = K x`` on the free dofs of a scalar Poisson problem."""
 
-    def __init__(self, ctx, params=None, precision=_lib.IGA_FP64):
+    def __init__(self, ctx, params=None, precision=_lib.IGA_FP64, inputs=None):
+        """``inputs`` (optional): the values of :func:`jaxiga_amd.methods.point_inputs`;
+        its coefficient is used (detached) instead of evaluating ``params``."""
         import torch
 
-        from jaxiga_amd.methods import _coefficient_args
+        from jaxiga_amd.methods import _bind_coefficient, _coefficient_args
 
         space = ctx.space
         if ctx.problem.kernel != "poisson" or space.vec != 1:
@@ -46,7 +48,8 @@ class MatrixFreeOperator:
         self.work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=ctx.f2f_d.device)
         self.nbytes = nbytes
         args = _lib.IgaAsmArgs()
-        keep = _coefficient_args(ctx, params, args)
+        keep = (_bind_coefficient(args, inputs["coef"]) if inputs is not None
+                else _coefficient_args(ctx, params, args))
         _lib.check(L.iga_mf_setup(st, ctx.rule(), precision, self.rational, args.coef_kind,
                                   args.coef_const, args.coef_points, _lib.ptr(self.work), nbytes,
                                   stream()), "iga_mf_setup")

@@ -538,12 +538,9 @@ def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadra
     # the reverse pass is one adjoint solve
     inputs = point_inputs(ctx, params) if torch.is_grad_enabled() else None
     traced = inputs is not None and inputs_require_grad(inputs)
-    if linear.matrix_free:
+    if linear.matrix_free and not traced:
         from jaxiga_amd.matfree import matrix_free_solve_ctx
 
-        if traced:
-            raise NotImplementedError("gradients through the matrix-free solve are not provided; "
-                                      "use the assembled CSR solve (matrix_free=False)")
         return matrix_free_solve_ctx(ctx, params, linear)
     if traced:
         from jaxiga_amd import adjoint as AD

@@ -163,6 +163,38 @@ def test_vjp_uses_exactly_one_adjoint_solve(gpu):
     np.testing.assert_allclose(k.grad.item(), fd, rtol=1e-6)
 
 
+def test_matrix_free_vjp_costs_one_adjoint_solve(gpu):
+    """The reference's tests/test_advanced.py:272-283: the matrix-free solve
+    (CG at 1e-13) differentiated with one adjoint solve, against a central
+    difference; and the gradient in a0 at every Gauss point equals the
+    assembled path's."""
+    opts = jx.LinearOptions(method="cg", matrix_free=True, tol=1e-13)
+    V = jx.FunctionSpace(jx.primitives.quadrilateral(QUAD).elevate(2).refine(2))
+    prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(lin_g, where=everywhere)], source=amp_source)
+
+    def loss(a0, linear, grad=True):
+        with torch.set_grad_enabled(grad):
+            return (jx.solve(prob, params={"a0": a0, "amp": 1.5}, linear=linear).u ** 2).sum()
+
+    a0 = torch.tensor(1.0, dtype=torch.float64, device=gpu, requires_grad=True)
+    LIN.reset_solve_counter()
+    loss(a0, opts).backward()
+    assert LIN.SOLVE_COUNTER == {"forward": 1, "adjoint": 1}
+    h = 1e-6
+    fd = (loss(1.0 + h, opts, False) - loss(1.0 - h, opts, False)).item() / (2 * h)
+    print(f"[matfree] d/da0 {a0.grad.item():.12e} fd {fd:.12e}", flush=True)
+    assert abs(a0.grad.item() - fd) / abs(fd) < 1e-7
+    a = 1.0 + torch.rand((V.n_elems, 9), dtype=torch.float64, device=gpu,
+                         generator=torch.Generator(device=gpu).manual_seed(8))
+    grads = []
+    for linear in (opts, TIGHT):
+        at = a.clone().requires_grad_()
+        loss(jx.PointField(at), linear).backward()
+        grads.append(at.grad.cpu().numpy())
+    print(f"[matfree] point gradients vs assembled: {rel(grads[0], grads[1]):.1e}", flush=True)
+    assert rel(grads[0], grads[1]) < 1e-9
+
+
 def _elastic_traction(x, n, p):
     m = xp(x)
     return m.stack([p["t"] * (1.0 + 0.0 * x[0]), 0.3 * p["t"] * x[1]])
