@@ -104,7 +104,7 @@ def cpu_baseline(sample_refine=7):
 
 
 def learned_pc_bench(refine=8, n_train=88, steps=300, batch=4, test_seeds=tuple(range(10_000, 10_032)),
-                     reps=3, log=print):
+                     reps=3, log=print, cpu=True):
     """Time to a 1e-10 relative residual on the 256x256 cubic variable-coefficient
     Poisson problem (a = exp(GRF), l = 0.1, sigma = 1.5, smooth random source):
     FP64 Jacobi-CG versus flexible CG with the learned multigrid preconditioner
@@ -136,6 +136,7 @@ def learned_pc_bench(refine=8, n_train=88, steps=300, batch=4, test_seeds=tuple(
     log(f"[bench] trained {steps} steps in {t_train:.1f} s, loss {hist[0]:.3f} -> {np.mean(hist[-10:]):.3f}")
     del data
     rows = []
+    cpu_out = None
     for seed in test_seeds:
         ctx, vals, rhs = grf.poisson_instance(refine, seed, space=V)
         pattern = ctx.assembly
@@ -174,6 +175,11 @@ def learned_pc_bench(refine=8, n_train=88, steps=300, batch=4, test_seeds=tuple(
             f"learned {le.iterations} it {statistics.median(tl) * 1e3:.2f} ms "
             f"(setup {le.setup_seconds * 1e3:.2f} ms) -> {row['speedup']:.2f}x")
         rows.append(row)
+        if cpu and len(rows) == 1:
+            cpu_out = cpu_time_to_solve(pattern, vals, rhs)
+            log(f"[bench] CPU time to solve (seed {seed}): SciPy Jacobi-CG "
+                f"{cpu_out['scipy_jacobi_cg']['seconds']:.2f} s ({cpu_out['scipy_jacobi_cg']['iterations']} it), "
+                f"SuperLU {cpu_out['superlu']['seconds']:.2f} s")
     return {"workload": f"unit-square Poisson, p=3, {2**refine}x{2**refine} elements, a=exp(GRF l=0.1 sigma=1.5), "
                         f"smooth random source, rel. residual 1e-10, {len(rows)} unseen coefficient seeds "
                         f"{test_seeds[0]}..{test_seeds[-1]}",
@@ -191,7 +197,44 @@ def learned_pc_bench(refine=8, n_train=88, steps=300, batch=4, test_seeds=tuple(
             "ms_learned_median": statistics.median(r["learned_mg_fcg"][1] for r in rows),
             "ms_jacobi_cg_median": statistics.median(r["jacobi_cg_fp64"][1] for r in rows),
             "max_relative_residual_learned": max(r["relative_residual"][2] for r in rows),
-            "fallback_iterations_total": sum(r["learned_mg_fcg"][3] for r in rows)}
+            "fallback_iterations_total": sum(r["learned_mg_fcg"][3] for r in rows),
+            "cpu_baseline": cpu_out}
+
+
+def cpu_time_to_solve(pattern, vals, rhs, tol=1e-10, reps=3):
+    """The reference's CPU solve paths on the same operator and load (copied
+    to the host): SciPy CSR Jacobi-CG as ``_scipy_csr_cg`` does it
+    (``src/jaxiga/solvers/linear.py:261-321``, restated by the oracle) and
+    SuperLU (``spsolve``, the reference's CPU direct default); warm-up, then
+    median of ``reps``, one core (SciPy's CSR product and SuperLU are
+    single-threaded)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from threadpoolctl import threadpool_limits
+
+    from oracle import solver as OSV
+
+    n = pattern.shape[0]
+    A = sp.csr_matrix((vals.detach().cpu().numpy(), pattern.cols_d.cpu().numpy(), pattern.indptr_d.cpu().numpy()),
+                      shape=(n, n))
+    b = rhs.detach().cpu().numpy()
+    bn = np.linalg.norm(b)
+    out = {}
+    with threadpool_limits(1):
+        for name, fn in (("scipy_jacobi_cg", lambda: OSV.scipy_csr_cg(A, b, tol)),
+                         ("superlu", lambda: (spla.spsolve(A.tocsc(), b, permc_spec="MMD_AT_PLUS_A"), None))):
+            fn()
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                u, it = fn()
+                times.append(time.perf_counter() - t0)
+            out[name] = {"seconds": statistics.median(times), "iterations": it,
+                         "relative_residual": float(np.linalg.norm(b - A @ u) / bn)}
+    out.update({"cores": 1, "kind": "port", "host_cpus": os.cpu_count(),
+                "sample": f"the full operator of the first unseen seed ({n} free dofs, {pattern.nnz} "
+                          f"nonzeros), copied to the host; warm-up, then median of {reps}"})
+    return out
 
 
 def matfree_bench(refine=6, reps=5, log=print):
@@ -588,7 +631,7 @@ def main():
     if args.learned_only:
         print(json.dumps({"learned_pc": learned_pc_bench(
             n_train=args.learned_fields, steps=args.learned_steps,
-            test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)))}), flush=True)
+            test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)), cpu=not args.no_cpu)}), flush=True)
         return
     if args.adjoint_only:
         adj = adjoint_bench()
@@ -658,7 +701,7 @@ def main():
 
     learned = None if (args.no_learned or args.no_solve) else learned_pc_bench(
         n_train=args.learned_fields, steps=args.learned_steps,
-        test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)))
+        test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)), cpu=not args.no_cpu)
     extras = {} if (args.no_extras or args.no_solve) else {"matfree": matfree_bench(), "batched": batched_bench()}
     if not (args.no_extras or args.no_solve):
         adj = adjoint_bench()
