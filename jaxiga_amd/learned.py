@@ -44,7 +44,8 @@ from jaxiga_amd import multigrid as MG
 from jaxiga_amd.runtime import lib, side_stream, stream
 
 KS = 7  # smoother stencil width
-OMEGA0 = 0.5  # damped-Jacobi weight of the untrained generator
+OMEGA0 = 0.5  # damped-Jacobi weight of the untrained generator (and of the training sweeps)
+OMEGA_COARSE = 0.7  # damped-Jacobi weight of the levels below the generator's, and of the tail
 LEARNED_LEVELS = 1  # levels (from the finest) whose smoother the generator provides
 
 
@@ -291,7 +292,7 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     import torch
 
     hier = MG.hierarchy(ctx)
-    op = MG.operator(hier, omega=OMEGA0, ks=KS)
+    op = MG.operator(hier, omega=OMEGA_COARSE, ks=KS)
     if gen is None or not op.split_load():
         op.load(vals64)
         if gen is not None:

@@ -152,7 +152,7 @@ def test_vcycle_matches_oracle(gpu, refine):
         if st is not None:
             op.set_stencils([torch.as_tensor(s, dtype=torch.float32, device="cuda") for s in st])
         x = op.vcycle(torch.as_tensor(b, dtype=torch.float32, device="cuda")).double().cpu().numpy()
-        xr = OM.vcycle(ref, st, b)
+        xr = OM.vcycle(ref, st, b, omega=op.omega)
         assert _rel(x, xr) < 1e-4
 
 
@@ -183,7 +183,7 @@ def test_grid_vcycle_matches_oracle(gpu, refine):
             op.set_stencils([torch.as_tensor(st[0], dtype=torch.float32, device="cuda")])
         assert op.grid_active()
         x = op.vcycle(torch.as_tensor(b, dtype=torch.float32, device="cuda")).double().cpu().numpy()
-        xr = OM.vcycle(ref, st, b)
+        xr = OM.vcycle(ref, st, b, omega=op.omega)
         print(f"[grid] refine {refine} stencil {st is not None}: rel {_rel(x, xr):.2e}", flush=True)
         assert _rel(x, xr) < 1e-4
 
