@@ -432,8 +432,11 @@ __global__ void __launch_bounds__(256)
 // from the lane's registers. The geometry (W, dW, x, dx/dxi: nine tensor
 // sums per Gauss point) is sum-factorised once per point, FP64, and shared
 // through LDS; no workgroup barrier is needed.
+// Eight waves per SIMD: the kernel fits 64 registers (the accumulators in
+// architectural VGPRs) without spilling; left to itself the compiler takes
+// 88 (64 + 24 accumulation registers), which caps residency at five waves.
 template <int P, int PROB, typename T, bool REG>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_element_w(iga_patch_t pt, int nq1, const double* __restrict__ qp,
                 const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all, int n_e) {
   constexpr int N1 = P + 1;
