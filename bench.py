@@ -70,6 +70,27 @@ def elasticity_flops_per_element(n_q=16, n_s=3, n_lv=32):
     return 2 * n_q * (n_s * n_s * n_lv + n_s * n_lv * n_lv)
 
 
+def event_ms(launch, batches=5, per=10):
+    """Average duration (ms) of one launch: batches of ``per`` back-to-back
+    launches on the current stream between two HIP events, each batch
+    started behind an untimed launch (so the GPU is already busy when the
+    first event is recorded); the median over batches, robust to a host
+    pause inside one batch."""
+    import torch
+
+    times = []
+    for _ in range(batches):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        launch()
+        e0.record()
+        for _ in range(per):
+            launch()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / per)
+    return statistics.median(times)
+
+
 def cpu_baseline(sample_refine=7):
     """Oracle FP64 assembly (basis, K_e, CSR scatter-add) on the same geometry
     at 2**sample_refine elements per direction; median of 3 after a warm-up."""
@@ -404,15 +425,7 @@ def adjoint_bench(refine=8, reps=3, log=print):
     basis = evaluate(V, gauss(V, ctx.n_q1), outputs=("dR", "x", "w"))
     torch.cuda.synchronize()
     t_basis = time.perf_counter() - t0
-    AD.field_points(basis, sol.u, lam, 1, values=False)
-    k_reps = 20
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev[0].record()
-    for _ in range(k_reps):
-        AD.field_points(basis, sol.u, lam, 1, values=False)
-    ev[1].record()
-    torch.cuda.synchronize()
-    t_k = ev[0].elapsed_time(ev[1]) / k_reps * 1e-3
+    t_k = event_ms(lambda: AD.field_points(basis, sol.u, lam, 1, values=False)) * 1e-3
     n_e, n_q, dim, n_l = basis.dR.shape
     n_p = n_e * n_q
     k_bytes = n_p * dim * n_l * 8 + n_e * n_l * 4 + 2 * V.n_dofs * 8 + 2 * n_p * dim * 8
@@ -524,20 +537,10 @@ def assembly_run(ctx, params, prec_name, steps, warmup):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
-    # per-kernel durations: each pass launched `steps` times back to back on
-    # this stream (the GPU never waits for the host) between two HIP events
-    def kernel_ms(passes):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        job.launch(ctx, passes=passes)
-        e0.record()
-        for _ in range(steps):
-            job.launch(ctx, passes=passes)
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps
-
-    t_el = kernel_ms(1)
-    t_ga = kernel_ms(2)
+    # per-kernel durations: each pass launched back to back on this stream
+    # between HIP events (median over batches)
+    t_el = event_ms(lambda: job.launch(ctx, passes=1))
+    t_ga = event_ms(lambda: job.launch(ctx, passes=2))
     # the operator and load of one full step, for the checks below
     step()
     torch.cuda.synchronize()
