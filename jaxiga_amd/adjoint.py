@@ -116,10 +116,16 @@ class _Run:
                 u_free, info, rn, bn = self._forward_matrix_free()
             else:
                 vals, b, _ = assemble(ctx, self.params, inputs=self.inputs)
-                u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
+                if linear.preconditioner == "multigrid":
+                    from jaxiga_amd import learned as LN
+
+                    u_free, info = LN.linear_solve(ctx, vals, b, linear)
+                    self._adjoint = lambda ub: LN.linear_solve(ctx, vals, ub, linear, adjoint=True)[0]
+                else:
+                    u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
+                    # the reverse pass: one solve on the same symmetric CSR
+                    self._adjoint = lambda ub: LIN.adjoint_solve(linear, ctx.assembly, vals, ub)
                 rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
-                # the reverse pass: one solve on the same symmetric CSR
-                self._adjoint = lambda ub: LIN.adjoint_solve(linear, ctx.assembly, vals, ub)
             u = ctx.dofmap.lift(u_free)
         torch.cuda.synchronize()
         stats = {"method": "galerkin", "linear": linear.method, "matrix_free": bool(linear.matrix_free),

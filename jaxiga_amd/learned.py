@@ -315,6 +315,29 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     return op
 
 
+_DEFAULT = {"generator": None}
+
+
+def use_generator(gen):
+    """Register the trained generator whose smoothers
+    ``LinearOptions(preconditioner="multigrid")`` uses (None: damped Jacobi)."""
+    _DEFAULT["generator"] = gen
+
+
+def default_generator():
+    return _DEFAULT["generator"]
+
+
+def linear_solve(ctx, vals64, b, opts, adjoint=False):
+    """The ``preconditioner="multigrid"`` solve of the Galerkin driver:
+    flexible CG with the registered generator's smoothers (or damped Jacobi)
+    to ``opts.tol``; ``(u, info)``, counted as a forward or an adjoint solve."""
+    LIN.SOLVE_COUNTER["adjoint" if adjoint else "forward"] += 1
+    res = solve(ctx, vals64, b.contiguous(), default_generator(), tol=opts.tol,
+                maxiter=opts.maxiter if opts.maxiter is not None else 1000)
+    return res.x, {"iterations": res.iterations, "fallback_iterations": res.fallback_iterations}
+
+
 @dataclasses.dataclass
 class SolveResult:
     x: object

@@ -36,7 +36,12 @@ class LinearOptions:
 
     On this backend ``"cg"`` runs the device-resident Jacobi-PCG;
     ``"dense"`` and ``"scipy"`` (a direct solve) run a dense LU on the GPU for
-    systems up to ``DIRECT_LIMIT`` free dofs.
+    systems up to ``DIRECT_LIMIT`` free dofs. ``preconditioner="multigrid"``
+    (an addition; the reference allows ``"none"`` and ``"jacobi"``) runs
+    flexible CG with one multigrid V-cycle per iteration through
+    :func:`jaxiga_amd.solve` -- damped-Jacobi smoothers, or the smoothers of
+    the generator registered with :func:`jaxiga_amd.learned.use_generator` --
+    on scalar 2-D problems whose prescribed dofs are whole sides.
     """
 
     method: str = "scipy"
@@ -57,8 +62,12 @@ class LinearOptions:
         allowed = {"cg", "bicgstab", "gmres", "dense", "scipy"}
         if self.method not in allowed:
             raise ValueError(f"unknown linear method {self.method!r}; expected one of {allowed}")
-        if self.preconditioner not in {"none", "jacobi"}:
-            raise ValueError(f"unknown preconditioner {self.preconditioner!r}; expected 'none' or 'jacobi'")
+        if self.preconditioner not in {"none", "jacobi", "multigrid"}:
+            raise ValueError(f"unknown preconditioner {self.preconditioner!r}; expected 'none', 'jacobi' "
+                             "or 'multigrid'")
+        if self.preconditioner == "multigrid" and (self.method != "cg" or self.matrix_free):
+            raise ValueError("preconditioner='multigrid' runs flexible CG on the assembled operator: "
+                             "use method='cg' with matrix_free=False")
         orderings = {"auto", "NATURAL", "MMD_ATA", "MMD_AT_PLUS_A", "COLAMD"}
         if self.superlu_ordering not in orderings:
             raise ValueError(f"unknown SuperLU ordering {self.superlu_ordering!r}; expected one of {orderings}")
@@ -219,6 +228,9 @@ def _solve_impl(opts: LinearOptions, pattern: CSRPattern, data, b):
     """``A u = b`` on the GPU with the method of ``opts``; ``(u, info)``."""
     import torch
 
+    if opts.preconditioner == "multigrid":
+        raise ValueError("preconditioner='multigrid' needs the problem's multigrid hierarchy: "
+                         "solve through jaxiga_amd.solve")
     if opts.method == "cg":
         if opts.preconditioner == "jacobi":
             res = cg_csr(pattern, data, b, opts.tol, opts.maxiter)

@@ -549,7 +549,12 @@ def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadra
         return u, ctx, stats
     t0 = time.perf_counter()
     vals, b, _ = assemble(ctx, params, inputs=inputs)
-    u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
+    if linear.preconditioner == "multigrid":
+        from jaxiga_amd import learned as LN
+
+        u_free, info = LN.linear_solve(ctx, vals, b, linear)
+    else:
+        u_free, info = LIN.sparse_solve(linear, ctx.assembly, vals, b)
     rn, bn = LIN.residual_norms(ctx.assembly, vals, u_free, b)
     torch.cuda.synchronize()
     stats = {"method": "galerkin", "linear": linear.method, "matrix_free": False,

@@ -236,3 +236,32 @@ def test_training_gains_over_untrained_smoothers(gpu):
     assert after.iterations < jac.iterations
     ref = jac.x.cpu().numpy()
     assert np.abs(after.x.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_drop_in_solve_with_multigrid_preconditioner_and_its_gradient(gpu):
+    # jx.solve(..., LinearOptions(preconditioner="multigrid")): the same
+    # solution as Jacobi-CG in far fewer iterations, and a gradient in a0 at
+    # the Gauss points equal to the Jacobi path's, one adjoint solve each
+    ctx, _, _ = _problem(6, 21)
+    prob = ctx.problem
+    a = torch.exp(grf.gauss_point_field(ctx, seed=21, ell=0.1, sigma=1.5))
+    mg = jx.LinearOptions(method="cg", preconditioner="multigrid", tol=1e-11)
+    jac = jx.LinearOptions(method="cg", tol=1e-11)
+    c = torch.randn(ctx.space.n_dofs, dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(3))
+    sols, grads = [], []
+    for opts in (mg, jac):
+        at = a.clone().requires_grad_()
+        LIN.reset_solve_counter()
+        sol = jx.solve(prob, params={"a0": jx.PointField(at)}, linear=opts)
+        (c * sol.u).sum().backward()
+        assert LIN.SOLVE_COUNTER == {"forward": 1, "adjoint": 1}
+        assert sol.stats["converged"]
+        sols.append(sol)
+        grads.append(at.grad.cpu().numpy())
+    print(f"[mg-drop-in] iterations multigrid {sols[0].stats['iterations']} jacobi {sols[1].stats['iterations']}; "
+          f"u {_rel(sols[0].u.detach().cpu().numpy(), sols[1].u.detach().cpu().numpy()):.1e} "
+          f"grad {_rel(grads[0], grads[1]):.1e}", flush=True)
+    assert sols[0].stats["iterations"] * 4 < sols[1].stats["iterations"]
+    assert _rel(sols[0].u.detach().cpu().numpy(), sols[1].u.detach().cpu().numpy()) < 1e-8
+    assert _rel(grads[0], grads[1]) < 1e-7
