@@ -411,6 +411,17 @@ def adjoint_bench(refine=8, reps=3, log=print):
     g, _, _, sol, counter = runs[-1]
     t_fwd = statistics.median(r[1] for r in runs)
     t_bwd = statistics.median(r[2] for r in runs)
+    # the same gradient through LinearOptions(preconditioner="multigrid"):
+    # multigrid flexible CG (damped-Jacobi smoothers) forward and adjoint
+    lin_jacobi, lin = lin, jx.LinearOptions(method="cg", preconditioner="multigrid", tol=1e-10)
+    run()
+    runs_mg = [run() for _ in range(reps)]
+    g_mg = runs_mg[-1][0]
+    mg = {"forward_seconds": statistics.median(r[1] for r in runs_mg),
+          "reverse_seconds": statistics.median(r[2] for r in runs_mg),
+          "solves_per_gradient": runs_mg[-1][4],
+          "max_rel_diff_vs_jacobi_gradient": float((g_mg - g).abs().max() / g.abs().max())}
+    lin = lin_jacobi
     # the parts of the reverse pass, timed alone
     free = ctx.free_d.long()
     vals, b, _ = jx.methods.assemble(ctx, {"a0": jx.PointField(a)})
@@ -445,6 +456,7 @@ def adjoint_bench(refine=8, reps=3, log=print):
         "elements": n_e, "forward_seconds": t_fwd, "reverse_seconds": t_bwd,
         "reverse_parts_seconds": {"adjoint_cg": t_adj, "basis_gradients": t_basis, "field_points": t_k},
         "solves_per_gradient": counter,
+        "multigrid_preconditioner": mg,
         "directional_check": {"adjoint": ad, "central_difference": fd, "rel_diff": abs(ad - fd) / abs(fd)},
         "roofline": {"bound": "hbm", "kernel": "k_field_points_lds<1,2,2>", "kernel_ms": t_k * 1e3,
                      "achieved": k_bytes / t_k / 1e9, "peak": 8000.0, "unit": "GB/s",
@@ -457,7 +469,9 @@ def adjoint_bench(refine=8, reps=3, log=print):
     }
     log(f"[bench] adjoint: forward {t_fwd * 1e3:.1f} ms, reverse {t_bwd * 1e3:.1f} ms (adjoint CG "
         f"{t_adj * 1e3:.1f} ms, basis {t_basis * 1e3:.2f} ms, k_field_points {t_k * 1e6:.1f} us = "
-        f"{k_bytes / t_k / 1e9:.0f} GB/s); solves {counter}; d/dir {ad:.8e} vs central difference {fd:.8e}")
+        f"{k_bytes / t_k / 1e9:.0f} GB/s); solves {counter}; d/dir {ad:.8e} vs central difference {fd:.8e}; "
+        f"with preconditioner='multigrid': forward {mg['forward_seconds'] * 1e3:.1f} ms, reverse "
+        f"{mg['reverse_seconds'] * 1e3:.1f} ms, gradient {mg['max_rel_diff_vs_jacobi_gradient']:.1e} from Jacobi's")
     return out
 
 
