@@ -600,8 +600,10 @@ def _lib_mod():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    # the GPU needs a few milliseconds of work to reach its steady clock: the
+    # first steps of a cold run are up to 12 % slower
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solve measurement")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp32",

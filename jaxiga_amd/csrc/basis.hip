@@ -23,9 +23,13 @@ __global__ void __launch_bounds__(1024)
   const int be = blockIdx.x;
   if (be >= n_e) return;
   const int e = elems[be];
+  // every loop over directions is unrolled: indexing the by-value patch
+  // struct or these small arrays with a run-time index would place them in
+  // memory (promoted to LDS, sized for 1024 threads, or scratch)
   int ei[3] = {0, 0, 0};
   {
     int rem = e;
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       ei[d] = rem % pt.n_elem[d];
       rem /= pt.n_elem[d];
@@ -38,6 +42,7 @@ __global__ void __launch_bounds__(1024)
   int ai[3] = {0, 0, 0};
   {
     int rem = lane;
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       ai[d] = rem % N1;
       rem /= N1;
@@ -46,6 +51,7 @@ __global__ void __launch_bounds__(1024)
   double Pc[DIM], wgt = 1.0;
   if (active) {
     int stride = 1, sdof = 0;
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       sdof += (pt.first[d][ei[d]] + ai[d]) * stride;
       stride *= pt.n_cp[d];
@@ -118,12 +124,12 @@ __global__ void __launch_bounds__(1024)
           n[0] = (dd == 0 ? J[1][1] : -J[0][1]) * id;
           n[1] = (dd == 0 ? -J[1][0] : J[0][0]) * id;
         } else {
-          const auto& J = g.J;
           // column dd of the inverse = cross product of the two tangent rows / det
           const int t1 = (dd + 1) % 3, t2 = (dd + 2) % 3;
-          n[0] = (J[t1][1] * J[t2][2] - J[t1][2] * J[t2][1]) / g.detJ;
-          n[1] = (J[t1][2] * J[t2][0] - J[t1][0] * J[t2][2]) / g.detJ;
-          n[2] = (J[t1][0] * J[t2][1] - J[t1][1] * J[t2][0]) / g.detJ;
+          auto jr3 = [&](int k, int c) { return k == 0 ? g.J[0][c] : (k == 1 ? g.J[1][c] : g.J[2][c]); };
+          n[0] = (jr3(t1, 1) * jr3(t2, 2) - jr3(t1, 2) * jr3(t2, 1)) / g.detJ;
+          n[1] = (jr3(t1, 2) * jr3(t2, 0) - jr3(t1, 0) * jr3(t2, 2)) / g.detJ;
+          n[2] = (jr3(t1, 0) * jr3(t2, 1) - jr3(t1, 1) * jr3(t2, 0)) / g.detJ;
         }
         double nn = 0.0;
 #pragma unroll
@@ -132,19 +138,32 @@ __global__ void __launch_bounds__(1024)
         if (normal)
 #pragma unroll
           for (int c = 0; c < DIM; ++c) normal[pq * DIM + c] = n[c] / nn * sgn;
+        // rows of J and box sizes picked by value: a run-time index into g.J or
+        // h would keep them in memory
+        auto jr = [&](int k, int c) {
+          double v = g.J[0][c];
+#pragma unroll
+          for (int r = 1; r < DIM; ++r) v = k == r ? g.J[r][c] : v;
+          return v;
+        };
+        auto hs = [&](int k) {
+          double v = h[0];
+#pragma unroll
+          for (int r = 1; r < DIM; ++r) v = k == r ? h[r] : v;
+          return v;
+        };
         double surf, ts = 1.0;
         if constexpr (DIM == 2) {
           const int t = 1 - dd;
-          surf = sqrt(g.J[t][0] * g.J[t][0] + g.J[t][1] * g.J[t][1]);
-          ts = 0.5 * h[t];
+          surf = sqrt(jr(t, 0) * jr(t, 0) + jr(t, 1) * jr(t, 1));
+          ts = 0.5 * hs(t);
         } else {
           const int t1 = (dd + 1) % 3, t2 = (dd + 2) % 3;
-          const auto& J = g.J;
-          const double c0 = J[t1][1] * J[t2][2] - J[t1][2] * J[t2][1];
-          const double c1 = J[t1][2] * J[t2][0] - J[t1][0] * J[t2][2];
-          const double c2 = J[t1][0] * J[t2][1] - J[t1][1] * J[t2][0];
+          const double c0 = jr(t1, 1) * jr(t2, 2) - jr(t1, 2) * jr(t2, 1);
+          const double c1 = jr(t1, 2) * jr(t2, 0) - jr(t1, 0) * jr(t2, 2);
+          const double c2 = jr(t1, 0) * jr(t2, 1) - jr(t1, 1) * jr(t2, 0);
           surf = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-          ts = 0.25 * h[t1] * h[t2];
+          ts = 0.25 * hs(t1) * hs(t2);
         }
         if (w) w[pq] = wr * surf * ts;
       }

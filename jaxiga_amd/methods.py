@@ -244,6 +244,7 @@ class AssemblyLaunch:
     diag: object
     keep: list
     tractions: list | None = None  # Neumann tractions given by the caller
+    events: list | None = None  # (fork, join) events of the Neumann side stream, reused
 
     def launch(self, ctx, passes=0):
         self.args.passes = passes
@@ -275,24 +276,27 @@ def run_assembly(ctx, params, job, *, load=True, passes=(0,), on_pass=None):
     import torch
 
     neumann = load and job.rhs is not None and bool(ctx.problem.neumann)
-    done = None
-    for k, p in enumerate(passes):
-        if k == 0 and neumann:
-            # the side stream starts from the same point as the element pass
-            side = side_stream(job.rhs.device, "neumann")
-            side.wait_stream(torch.cuda.current_stream())
+    cur = torch.cuda.current_stream()
+    if neumann:
+        # the side stream starts from the same point as the element pass; the
+        # fork and join events are created once per prepared job
+        if job.events is None:
+            job.events = [torch.cuda.Event(), torch.cuda.Event()]
+        fork, join = job.events
+        side = side_stream(job.rhs.device, "neumann")
+        fork.record(cur)
+        side.wait_event(fork)
+    for p in passes:
         job.launch(ctx, passes=p)
         if on_pass is not None:
             on_pass(p)
-        if k == 0 and neumann:
-            # issued after the element pass, so the host prepares the
-            # boundary work while the GPU runs the element pass
-            with torch.cuda.stream(side):
-                neumann_load(ctx, params, job.rhs, job.tractions)
-            done = torch.cuda.Event()
-            done.record(side)
-    if done is not None:
-        torch.cuda.current_stream().wait_event(done)
+    if neumann:
+        # issued once every pass is queued: the GPU runs the element pass and
+        # the gather back to back while the host prepares the boundary work
+        with torch.cuda.stream(side):
+            neumann_load(ctx, params, job.rhs, job.tractions)
+        join.record(side)
+        cur.wait_event(join)
 
 
 class CapturedAssembly:
