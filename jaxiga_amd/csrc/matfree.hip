@@ -38,6 +38,10 @@ struct MfShape {
   static constexpr int NG = DIM == 2 ? 3 : 6;  // stored entries of symmetric G
   static constexpr int NW = 1 + DIM;           // W and grad W (rational only)
   static constexpr int LANES = pow2ceil(NQ > NL ? NQ : NL);  // lanes per element
+  // one wavefront per block where an element fits in one: the apply has a
+  // barrier between every pair of contractions, and a one-wave barrier costs
+  // next to nothing, while four waves would wait on the slowest of them
+  static constexpr int BLOCK = LANES > 64 ? LANES : 64;
 };
 
 // index of G[a][b] (a <= b) in the packed symmetric layout
@@ -241,7 +245,7 @@ __global__ void __launch_bounds__(256)
   using S = MfShape<DIM, P>;
   constexpr int N1 = S::N1, Q1 = S::Q1, NL = S::NL, NQ = S::NQ, NG = S::NG, NW = S::NW;
   constexpr int LANES = S::LANES;
-  constexpr int EPB = 256 / LANES;  // elements per block
+  constexpr int EPB = S::BLOCK / LANES;  // elements per block
   constexpr int BUF = NQ > NL ? (NQ > N1 * N1 * Q1 ? NQ : N1 * N1 * Q1) : NL;
   constexpr int NBUF = DIM == 3 ? 8 : 6;
   __shared__ T tabs[EPB][DIM][2][Q1 * N1];
@@ -632,15 +636,15 @@ int mf_apply_launch(const iga_patch_t* patch, int precision, int rational, const
         A.dtab[d] = d < DIM ? base + L.dtabs[d] : nullptr;
       }
       T* rloc = reinterpret_cast<T*>(base + L.rloc);
-      constexpr int EPB = 256 / S::LANES;
+      constexpr int EPB = S::BLOCK / S::LANES;
       const int64_t blocks = (L.n_e + EPB - 1) / EPB;
       if (blocks > 0) {
         if (rational)
-          hipLaunchKernelGGL((k_mf_apply<DIM, P, T, true>), dim3((unsigned)blocks), dim3(256), 0, st,
+          hipLaunchKernelGGL((k_mf_apply<DIM, P, T, true>), dim3((unsigned)blocks), dim3(S::BLOCK), 0, st,
                              *patch, A, reinterpret_cast<const T*>(base + L.geo),
                              reinterpret_cast<const T*>(base + L.wgeo), x, rloc);
         else
-          hipLaunchKernelGGL((k_mf_apply<DIM, P, T, false>), dim3((unsigned)blocks), dim3(256), 0, st,
+          hipLaunchKernelGGL((k_mf_apply<DIM, P, T, false>), dim3((unsigned)blocks), dim3(S::BLOCK), 0, st,
                              *patch, A, reinterpret_cast<const T*>(base + L.geo), nullptr, x, rloc);
         IGA_CHECK_LAUNCH();
       }

@@ -52,6 +52,38 @@ def test_matrix_free_apply_equals_assembled_product(gpu, make):
     assert np.abs(y32 - ref).max() / np.abs(ref).max() < 5e-6
 
 
+@pytest.mark.parametrize("spread", [(1.0, 4e-16), (2.5, 0.0)])
+def test_constant_weights_take_the_polynomial_apply(gpu, spread):
+    """Weights equal up to rounding (knot insertion leaves weights of 1 a few
+    ulps off) or all scaled by one constant give R_i = N_i, so the operator
+    takes the polynomial apply; it must still equal the oracle's rational
+    assembly on the same weights."""
+    import dataclasses
+
+    from oracle import space as OS
+
+    scale, ulps = spread
+    P, O = C.patches("square", 2, 3)
+    w = O.weights * scale * (1.0 + ulps * np.where(np.arange(O.weights.size) % 2 == 0, 1.0, -1.0))
+    assert np.ptp(w) > 0 or ulps == 0
+    P = dataclasses.replace(P, weights=w)
+    O = dataclasses.replace(O, weights=w.copy())
+    a0 = lambda x, p: 1.0 + x[0] * x[1]  # noqa: E731
+    V = jx.FunctionSpace(P)
+    prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=C.everywhere)], source=C.sin_source)
+    case = C.Case("poisson-square-p2-r3-w", "poisson", V, OS.function_space(O), prob,
+                  [C.OF.DirichletBC(0.0, C.everywhere)], [], C.sin_source, {"a0": a0})
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    v = np.random.default_rng(1).normal(size=K.shape[0])
+    ctx = M.build_context(case.problem, case.params)
+    op = MF.MatrixFreeOperator(ctx, case.params)
+    assert op.rational == 0
+    y = op.apply(torch.as_tensor(v, device="cuda")).cpu().numpy()
+    ref = K @ v
+    assert np.abs(y - ref).max() / np.abs(ref).max() < 1e-12
+
+
 def test_matrix_free_diagonal_equals_assembled_diagonal(gpu):
     case = C.poisson_case("cube", 2, 2, source=lambda x, p: 1.0 + 0 * x[0])
     dm, pat, data, b = _oracle_system(case)
