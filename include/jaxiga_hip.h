@@ -232,8 +232,11 @@ int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols, co
  * iga_mf_setup     : fill the workspace once (FP64 geometry, stored in
  *                    `precision`); coefficient constant or per Gauss point
  * iga_mf_apply     : y = K x over the free dofs, x and y FP64 (n_free),
- *                    internal arithmetic in `precision`; rational != 0 when
- *                    the patch weights are not all one                      */
+ *                    internal arithmetic in `precision`; rational = 0 takes
+ *                    the polynomial basis, exact when the patch weights are
+ *                    all equal (any value; the Python operator allows a
+ *                    1e-14 relative spread from rounding), rational != 0 the
+ *                    NURBS quotient                                         */
 int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t rational);
 int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
                  int32_t rational, int32_t coef_kind, double coef_const,
