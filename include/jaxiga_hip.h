@@ -547,8 +547,11 @@ int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32
  *   { int32 n, pad; const int32* indptr, *cols; const double* vals, *dinv,
  *     *b; double* x; double* work (4n) }
  * order: device permutation (largest first); lanes_per_row in {8,16,32,64};
- * threads a multiple of 64 up to 1024. Outputs per instance: iterations and
- * the recursive relative residual. No host synchronisation (graph-capturable). */
+ * threads a multiple of 64 up to 1024. When the recursion meets tol, the true
+ * residual b - A x is recomputed in FP64 and the instance restarts from it
+ * while it is above tol (at most three restarts, within maxiter). Outputs per
+ * instance: iterations and the true relative residual ||b - A x|| / ||b||.
+ * No host synchronisation (graph-capturable). */
 int iga_batched_cg(int32_t n_inst, const void* insts, const int32_t* order,
                    int32_t lanes_per_row, int32_t threads, double tol, int32_t maxiter,
                    int32_t* out_iters, double* out_rel, void* stream);

@@ -4,6 +4,9 @@
 // HBM/L2, every dot product is a workgroup reduction, and the iteration loop
 // runs inside the kernel until the instance converges (||r|| <= tol ||b||,
 // FP64 recurrences, the accelerator semantics of jax cg) or reaches maxiter.
+// When the recursion stops, the true residual b - A x is recomputed in FP64;
+// above tol the instance restarts from it (at most three times, sharing
+// maxiter), so the reported relative residual is the true one.
 // No grid barrier and no host round trip are needed, so a whole batch of
 // uneven instances is one launch that a hipGraph can replay. Instances are
 // issued largest first (the order array) so the longest solves start early.
@@ -63,38 +66,62 @@ __global__ void __launch_bounds__(1024)
   int it = 0;
   const int lane = threadIdx.x % LPR;
   const int groups = blockDim.x / LPR;
-  while (rr > tol2bb && it < maxiter) {
-    double pq = 0.0;
+  double rr_true = bb;
+  for (int restarts = 0;; ++restarts) {
+    while (rr > tol2bb && it < maxiter) {
+      double pq = 0.0;
+      for (int row = threadIdx.x / LPR; row < n; row += groups) {
+        const double s = row_dot<LPR>(row, I.indptr, I.cols, I.vals, p, lane);
+        if (lane == 0) {
+          q[row] = s;
+          pq += s * p[row];
+        }
+      }
+      pq = block_allsum(pq, scratch, &bcast);  // also orders the q writes before their reads
+      const double alpha = rz / pq;
+      double rzn = 0.0, rrn = 0.0;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        I.x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        const double zi = I.dinv[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        rzn += ri * zi;
+        rrn += ri * ri;
+      }
+      rzn = block_allsum(rzn, scratch, &bcast);
+      rr = block_allsum(rrn, scratch, &bcast);
+      const double beta = rzn / rz;
+      rz = rzn;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = z[i] + beta * p[i];
+      __syncthreads();
+      ++it;
+    }
+    // the recursion has stopped: recompute r = b - A x in FP64 and restart
+    // from it (r, z = D^-1 r, p = z) while the true residual is above tol,
+    // at most three times within maxiter
+    double rrt = 0.0, rzt = 0.0;
     for (int row = threadIdx.x / LPR; row < n; row += groups) {
-      const double s = row_dot<LPR>(row, I.indptr, I.cols, I.vals, p, lane);
+      const double s = row_dot<LPR>(row, I.indptr, I.cols, I.vals, I.x, lane);
       if (lane == 0) {
-        q[row] = s;
-        pq += s * p[row];
+        const double ri = I.b[row] - s;
+        const double zi = I.dinv[row] * ri;
+        r[row] = ri;
+        z[row] = zi;
+        p[row] = zi;
+        rrt += ri * ri;
+        rzt += ri * zi;
       }
     }
-    pq = block_allsum(pq, scratch, &bcast);  // also orders the q writes before their reads
-    const double alpha = rz / pq;
-    double rzn = 0.0, rrn = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      I.x[i] += alpha * p[i];
-      const double ri = r[i] - alpha * q[i];
-      const double zi = I.dinv[i] * ri;
-      r[i] = ri;
-      z[i] = zi;
-      rzn += ri * zi;
-      rrn += ri * ri;
-    }
-    rzn = block_allsum(rzn, scratch, &bcast);
-    rr = block_allsum(rrn, scratch, &bcast);
-    const double beta = rzn / rz;
-    rz = rzn;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = z[i] + beta * p[i];
-    __syncthreads();
-    ++it;
+    rr_true = block_allsum(rrt, scratch, &bcast);
+    rzt = block_allsum(rzt, scratch, &bcast);
+    if (rr_true <= tol2bb || it >= maxiter || restarts >= 3) break;
+    rr = rr_true;
+    rz = rzt;
   }
   if (threadIdx.x == 0) {
     out_iters[id] = it;
-    out_rel[id] = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+    out_rel[id] = bb > 0.0 ? sqrt(rr_true / bb) : 0.0;
   }
 }
 

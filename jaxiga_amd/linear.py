@@ -35,8 +35,8 @@ class LinearOptions:
     """Linear-solver configuration (reference field names and checks).
 
     On this backend ``"cg"`` runs the device-resident Jacobi-PCG;
-    ``"dense"`` and ``"scipy"`` (a direct solve) run a dense LU on the GPU for
-    systems up to ``DIRECT_LIMIT`` free dofs. ``preconditioner="multigrid"``
+    ``"dense"`` runs a dense LU on the GPU for systems up to ``DIRECT_LIMIT``
+    free dofs; ``"scipy"`` is the reference's host SuperLU (any size). ``preconditioner="multigrid"``
     (an addition; the reference allows ``"none"`` and ``"jacobi"``) runs
     flexible CG with one multigrid V-cycle per iteration through
     :func:`jaxiga_amd.solve` -- damped-Jacobi smoothers, or the smoothers of
@@ -238,12 +238,45 @@ def _solve_impl(opts: LinearOptions, pattern: CSRPattern, data, b):
             ones = torch.ones(pattern.shape[0], dtype=torch.float64, device=b.device)
             res = cg_csr(pattern, data, b, opts.tol, opts.maxiter, dinv=ones)
         return res.x, {"iterations": res.iterations}
-    if opts.method in ("dense", "scipy"):
+    if opts.method == "dense":
         if pattern.shape[0] > DIRECT_LIMIT:
-            raise ValueError(f"direct solves are limited to {DIRECT_LIMIT} free dofs on this "
-                             "backend; use LinearOptions(method='cg')")
+            raise ValueError(f"dense solves are limited to {DIRECT_LIMIT} free dofs on this "
+                             "backend; use LinearOptions(method='cg') or method='scipy'")
         return torch.linalg.solve(dense_from_csr(pattern, data), b), {"iterations": None}
+    if opts.method == "scipy":
+        return _superlu_solve(opts, pattern, data, b), {"iterations": None}
     raise NotImplementedError(f"linear method {opts.method!r} is not provided on this backend; use 'cg'")
+
+
+def _superlu_solve(opts: LinearOptions, pattern: CSRPattern, data, b):
+    """``method="scipy"``: SuperLU on the host, as the reference runs it
+    (``_scipy_csr_solve``, ``src/jaxiga/solvers/linear.py:221-250``, a
+    ``pure_callback`` to SciPy): the device CSR values and load are copied to
+    the host, factored with the symmetric ordering ``MMD_AT_PLUS_A`` in
+    SuperLU's symmetric mode for ``superlu_ordering="auto"`` on the symmetric
+    pattern (``spsolve`` with the named ordering otherwise), and the solution
+    is copied back. An explicitly requested host direct solver, not a
+    fallback: the device solvers are ``"cg"`` and ``"dense"``."""
+    import scipy.sparse
+    import scipy.sparse.linalg
+    import torch
+
+    n = pattern.shape[0]
+    A = scipy.sparse.csr_matrix((data.detach().to(torch.float64).cpu().numpy(),
+                                 pattern.column_indices, pattern.indptr), shape=pattern.shape)
+    rhs = b.detach().cpu().numpy()
+    ordering = opts.superlu_ordering
+    if ordering == "auto":
+        ordering = "MMD_AT_PLUS_A" if pattern.symmetric else "COLAMD"
+    if n == 0:
+        u = np.zeros(0)
+    elif pattern.symmetric and ordering == "MMD_AT_PLUS_A":
+        factor = scipy.sparse.linalg.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A",
+                                          options={"SymmetricMode": True})
+        u = factor.solve(rhs)
+    else:
+        u = scipy.sparse.linalg.spsolve(A, rhs, permc_spec=ordering)
+    return torch.as_tensor(np.asarray(u, np.float64), device=b.device)
 
 
 def csr_outer(pattern: CSRPattern, a, b, alpha=1.0):

@@ -113,17 +113,47 @@ def build_context(problem, params=None, quadrature=None, chunk=None) -> MethodCo
 
 
 def call_points(fn, x, *extra, params=None, vec=1, with_params=True):
-    """Evaluate a vectorised callable at points ``x (..., dim)`` (GPU tensor):
-    ``fn(x_T, *extra_T, params)`` with ``(dim, n)`` arguments. Returns a
-    ``(..., vec)`` float64 tensor on the same device."""
+    """Evaluate a point callable at points ``x (..., dim)`` (GPU tensor).
+    Returns a ``(..., vec)`` float64 tensor on the same device.
+
+    The callable is first called once on all points, ``fn(x_T, *extra_T,
+    params)`` with ``(dim, n)`` arguments. Like the reference's evaluation of
+    user functions (``post/norms.py:18-32``; coefficients and sources are
+    called per point under ``vmap`` there), a callable written for one point
+    is recognised by its result: when the vectorised call raises, or returns
+    a single value for many points (a reduction such as ``1 + x.sum()``), it
+    is evaluated again per point under ``torch.func.vmap``. A callable whose
+    per-point result is a plain number is a constant."""
     import torch
 
     lead = x.shape[:-1]
     flat = x.reshape(-1, x.shape[-1]).T
     args = [e.reshape(-1, e.shape[-1]).T for e in extra]
-    out = fn(flat, *args, params) if with_params else fn(flat, *args)
-    out = torch.as_tensor(out, dtype=torch.float64, device=x.device)
     n = flat.shape[1]
+
+    def call(*a):
+        return fn(*a, params) if with_params else fn(*a)
+
+    def per_point():
+        cols = [flat.T, *[a.T for a in args]]
+        try:
+            res = torch.func.vmap(call)(*cols)
+        except Exception:
+            return None
+        res = torch.as_tensor(res, dtype=torch.float64, device=x.device)
+        if res.dim() == 0 or res.shape[0] != n:
+            return None
+        return res.reshape(n, -1).T  # (k, n)
+
+    try:
+        out = torch.as_tensor(call(flat, *args), dtype=torch.float64, device=x.device)
+    except Exception:
+        out = per_point()
+        if out is None:
+            raise
+    if out.dim() == 0 and n > 1:
+        pp = per_point()
+        out = out if pp is None else pp
     if out.dim() == 0:
         out = out.expand(n)[None, :]
     elif out.dim() == 1:
@@ -517,11 +547,27 @@ def tangent_diagonal(ctx, params=None):
 
 
 def default_linear(problem, n_free=None) -> LIN.LinearOptions:
-    """CG with Jacobi at ``tol=1e-12`` for declared SPD linear problems (the
-    reference's accelerator choice); a direct solve otherwise."""
-    if problem.is_linear and problem.is_positive_definite:
+    """The reference's choice (``methods/galerkin.py:255-272``) on an
+    accelerator: CG with Jacobi at ``tol=1e-12`` for a linear problem whose
+    class declares it SPD and whose free-dof count is at least
+    ``CG_CROSSOVER`` (or unknown); the default ``LinearOptions()`` (host
+    SuperLU) otherwise. A declaration inherited by a subclass that overrides
+    ``energy`` does not count unless the subclass repeats it."""
+    big_enough = n_free is None or n_free >= CG_CROSSOVER
+    cls = type(problem)
+
+    def defining(name):
+        for k in cls.__mro__:
+            if name in k.__dict__:
+                return k
+        return None
+
+    spd = bool(problem.is_positive_definite) and (
+        "is_positive_definite" in cls.__dict__ or defining("is_positive_definite") is defining("energy")
+        or defining("energy") is Problem)
+    if problem.is_linear and spd and big_enough:
         return LIN.LinearOptions(method="cg", preconditioner="jacobi", tol=1e-12)
-    return LIN.LinearOptions(method="dense")
+    return LIN.LinearOptions()
 
 
 def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadrature=None,

@@ -115,3 +115,81 @@ def test_matrix_free_takes_the_same_iteration_count(gpu, make):
     assert res.iterations == assembled.iterations
     assert abs(res.iterations - it_ref) <= 1
     np.testing.assert_allclose(res.x.cpu().numpy(), assembled.x.cpu().numpy(), rtol=0, atol=1e-9 * float(assembled.x.abs().max()))
+
+
+@pytest.mark.parametrize("ref", [3, 4])
+def test_matrix_free_3d_grf_coefficient(gpu, ref):
+    """The unit-cube workload's input path: a = exp(GRF l=0.1, sigma=1) at the
+    27 Gauss points of every p=2 element, given as a PointField, folded into
+    the 3-D geometric factors; FP64 and FP32 apply against the oracle's
+    assembled FP64 operator on the same coefficient values, and the
+    matrix-free solve against the oracle's direct solution."""
+    import scipy.sparse.linalg as spla
+
+    from jaxiga_amd import grf
+    from oracle import forms as OF
+    from oracle import grf as OGRF
+    from oracle import space as OS
+
+    case = C.poisson_case("cube", 2, ref, source=lambda x, p: 1.0 + 0 * x[0])
+    print(f"[mf-grf] {case.name}", flush=True)
+    ctx = M.build_context(case.problem)
+    a = grf.gauss_point_field(ctx, 20260901, 0.1, 1.0).exp_()
+    a_host = a.cpu().numpy()
+    # the coefficient against the oracle's own GRF at the oracle's Gauss points
+    xo = OS.gauss_basis(case.ospace).x.reshape(-1, 3)
+    om, ph = OGRF.rff_params(20260901, 3, 0.1)
+    pick = np.random.default_rng(ref).choice(len(xo), 3000, replace=False)
+    a_ref = np.exp(OGRF.rff_field(xo[pick], om, ph, 1.0))
+    assert np.abs(a_host.reshape(-1)[pick] - a_ref).max() <= 1e-12 * np.abs(a_ref).max()
+    case.params = {"a0": OF.PointField(a_host)}
+    params = {"a0": jx.PointField(a)}
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    v = np.random.default_rng(2).normal(size=K.shape[0])
+    ref_y = K @ v
+    for prec, tol in ((_lib.IGA_FP64, 1e-12), (_lib.IGA_FP32, 5e-6)):
+        op = MF.MatrixFreeOperator(ctx, params, precision=prec)
+        y = op.apply(torch.as_tensor(v, device="cuda")).cpu().numpy()
+        err = np.abs(y - ref_y).max() / np.abs(ref_y).max()
+        print(f"[mf-grf] precision {prec}: K v rel err {err:.2e}", flush=True)
+        assert err < tol
+    sol = jx.solve(case.problem, params=params,
+                   linear=jx.LinearOptions(method="cg", matrix_free=True, tol=1e-12))
+    assert sol.stats["converged"]
+    x_ref = spla.spsolve(K.tocsc(), b)
+    u_free = sol.u.cpu().numpy()[dm.free]
+    assert np.abs(u_free - x_ref).max() <= 1e-9 * np.abs(x_ref).max()
+
+
+def test_matrix_free_3d_rational_weights_and_coefficient(gpu):
+    """The 3-D rational apply (quotient-rule terms) with a non-symmetric
+    coefficient: cuboid p=2 whose weights are perturbed by up to +-30 %, a0 =
+    1 + x0 + 2 x1 x2; K v against the oracle's assembled operator."""
+    import dataclasses
+
+    from oracle import space as OS
+
+    P, O = C.patches("cube", 2, 2)
+    rng = np.random.default_rng(4)
+    w = O.weights * (1.0 + 0.3 * rng.uniform(-1.0, 1.0, size=O.weights.shape))
+    P = dataclasses.replace(P, weights=w)
+    O = dataclasses.replace(O, weights=w.copy())
+    a0 = lambda x, p: 1.0 + x[0] + 2.0 * x[1] * x[2]  # noqa: E731
+    src = lambda x, p: 1.0 + 0 * x[0]  # noqa: E731
+    V = jx.FunctionSpace(P)
+    prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=C.everywhere)], source=src)
+    case = C.Case("poisson-cube-p2-r2-rational", "poisson", V, OS.function_space(O), prob,
+                  [C.OF.DirichletBC(0.0, C.everywhere)], [], src, {"a0": a0})
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    v = np.random.default_rng(5).normal(size=K.shape[0])
+    ref_y = K @ v
+    ctx = M.build_context(case.problem, case.params)
+    for prec, tol in ((_lib.IGA_FP64, 1e-12), (_lib.IGA_FP32, 5e-6)):
+        op = MF.MatrixFreeOperator(ctx, case.params, precision=prec)
+        assert op.rational == 1
+        y = op.apply(torch.as_tensor(v, device="cuda")).cpu().numpy()
+        err = np.abs(y - ref_y).max() / np.abs(ref_y).max()
+        print(f"[mf-rational] precision {prec}: K v rel err {err:.2e}", flush=True)
+        assert err < tol
