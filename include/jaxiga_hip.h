@@ -20,6 +20,8 @@
  *   iga_csr_diag_inv          solvers/linear.py:334-343    (Jacobi inverse diagonal)
  *   iga_spmv                  solvers/linear.py:331-332    (segment-sum matvec)
  *   iga_cg_solve              solvers/linear.py:324-351    (jax cg, Jacobi M)
+ *   iga_cg_solve_noalloc      (no reference counterpart: iga_cg_solve without a
+ *                             workspace, for callers that cannot allocate)
  *   iga_residual_norms        solvers/diagnostics.py:8-19  (independent ||Au-b||)
  *   iga_mf_setup/iga_mf_apply methods/galerkin.py:207-218 (tangent_matvec,
  *                             sum-factorised instead of a JVP)
@@ -201,6 +203,20 @@ int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const vo
                  int32_t precision, const double* dinv, const double* b, double* x, double tol,
                  int32_t maxiter, double* work, int32_t* out_iters, double* out_rr, double* out_bb,
                  void* stream);
+
+/* Jacobi-PCG exactly as iga_cg_solve (same kernels, same results), with no
+ * workspace argument and no allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its
+ * workspace is a file-scope device array of 16 MiB per GPU, held until the
+ * process ends; n above IGA_CG_NOALLOC_MAX_N returns IGA_ERR_SHAPE. Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * host threads, other streams) race and must be serialised by the caller.  */
+#define IGA_CG_NOALLOC_MAX_N 500000
+int iga_cg_solve_noalloc(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
+                         int32_t precision, const double* dinv, const double* b, double* x,
+                         double tol, int32_t maxiter, int32_t* out_iters, double* out_rr,
+                         double* out_bb, void* stream);
 
 /* Jacobi-PCG on the FP32 matrix vals32 with FP64 vectors, dot products and
  * recurrences, and reliable updates: every replace_every iterations (>= 1),

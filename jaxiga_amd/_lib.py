@@ -24,6 +24,7 @@ IGA_POISSON, IGA_ELASTICITY = 0, 1
 IGA_COEF_CONST, IGA_COEF_POINTS = 0, 1
 IGA_SRC_NONE, IGA_SRC_CONST, IGA_SRC_POINTS = 0, 1, 2
 IGA_FP64, IGA_FP32, IGA_BF16 = 0, 1, 2
+IGA_OK, IGA_ERR_SHAPE, IGA_ERR_UNSUPPORTED, IGA_ERR_HIP, IGA_ERR_ARG = 0, -1, -2, -3, -4
 
 STATUS = {-1: "inconsistent shapes", -2: "unsupported degree/dimension/quadrature",
           -3: "HIP runtime error", -4: "null pointer or invalid argument"}
@@ -130,6 +131,8 @@ _SIGNATURES = {
     "iga_cg_workspace": (c_i64, [c_int]),
     "iga_cg_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                              c_void_p, c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_cg_solve_noalloc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_double, c_int, P_int, P_dbl, P_dbl, c_void_p]),
     "iga_cg_solve_mixed": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_double, c_int, c_int, c_void_p, P_int,
                                    P_dbl, P_dbl, c_void_p]),

@@ -8,6 +8,7 @@
 // Kernels read a device `done` flag, so the host enqueues iterations in
 // batches and synchronises only once per batch.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -554,13 +555,10 @@ static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, do
   return IGA_OK;
 }
 
-extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
-                            const void* vals, int32_t precision, const double* dinv,
-                            const double* b, double* x, double tol, int32_t maxiter, double* work,
-                            int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
-  if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
-  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
-  hipStream_t st = (hipStream_t)stream;
+static int cg_solve_csr(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
+                        int32_t precision, const double* dinv, const double* b, double* x,
+                        double tol, int32_t maxiter, double* work, int32_t* out_iters,
+                        double* out_rr, double* out_bb, hipStream_t st) {
   int nnz = 0;
   if (n > 0)
     if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
@@ -583,6 +581,65 @@ extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* col
                    });
                    return rc ? rc : rb;
                  });
+}
+
+extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
+                            const void* vals, int32_t precision, const double* dinv,
+                            const double* b, double* x, double tol, int32_t maxiter, double* work,
+                            int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
+  if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
+  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
+  return cg_solve_csr(n, indptr, cols, vals, precision, dinv, b, x, tol, maxiter, work, out_iters,
+                      out_rr, out_bb, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// The same Jacobi-PCG for callers that cannot allocate: its workspace (the
+// solver state, r, z, p, q and the reduction partials) is the file-scope
+// device array `scratch`, one copy per GPU, held until the process ends.
+// Host code never passes the array's name (a host-side stand-in, not a
+// device address) to a kernel: the first call on each device looks up the
+// device address with hipGetSymbolAddress and keeps it per device.
+// ---------------------------------------------------------------------------
+
+constexpr int64_t kNoAllocDoubles = int64_t(1) << 21;  // 16 MiB per GPU
+static_assert(kNoAllocDoubles >= 8 + 4 * int64_t(IGA_CG_NOALLOC_MAX_N) + 5 * kRedBlocks,
+              "scratch too small for IGA_CG_NOALLOC_MAX_N");
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[256] = {};
+
+static int scratch_for_current_device(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return IGA_ERR_HIP;
+  if (dev < 0 || dev >= 256) return IGA_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return IGA_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return IGA_OK;
+}
+
+extern "C" int iga_cg_solve_noalloc(int32_t n, const int32_t* indptr, const int32_t* cols,
+                                    const void* vals, int32_t precision, const double* dinv,
+                                    const double* b, double* x, double tol, int32_t maxiter,
+                                    int32_t* out_iters, double* out_rr, double* out_bb,
+                                    void* stream) {
+  if (!indptr || !cols || !vals || !dinv || !b || !x) return IGA_ERR_ARG;
+  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
+  if (n < 0 || n > IGA_CG_NOALLOC_MAX_N) return IGA_ERR_SHAPE;
+  double* work = nullptr;
+  if (int rc = scratch_for_current_device(&work)) return rc;
+  // every field the loop reads (state, r, z, p, x, partials) is written by
+  // its initialisation kernels, so the previous call's contents need no
+  // clearing
+  return cg_solve_csr(n, indptr, cols, vals, precision, dinv, b, x, tol, maxiter, work, out_iters,
+                      out_rr, out_bb, (hipStream_t)stream);
 }
 
 // CG on an FP32 matrix with FP64 vectors and recurrences, and reliable

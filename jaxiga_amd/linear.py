@@ -200,6 +200,31 @@ def cg_csr(pattern: CSRPattern, data, b, tol=1e-10, maxiter=None, dinv=None, wor
     return CGResult(x, it.value, rr.value, bb.value)
 
 
+NOALLOC_MAX_N = 500000  # IGA_CG_NOALLOC_MAX_N of the C ABI
+
+
+def cg_csr_noalloc(pattern: CSRPattern, data, b, x, dinv, tol=1e-10, maxiter=None):
+    """Jacobi-PCG as :func:`cg_csr`, through ``iga_cg_solve_noalloc``: the
+    library allocates nothing (its workspace is a fixed per-GPU device
+    array), the caller passes ``x`` (written) and ``dinv``. Systems above
+    ``NOALLOC_MAX_N`` rows are rejected. Calls that overlap on one GPU share
+    the array and race."""
+    L = lib()
+    n = pattern.shape[0]
+    if n > NOALLOC_MAX_N:
+        raise ValueError(f"iga_cg_solve_noalloc takes at most {NOALLOC_MAX_N} rows, got {n}")
+    it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
+    maxiter = 10 * n if maxiter is None else int(maxiter)
+    import torch
+
+    prec = _lib.IGA_FP32 if data.dtype == torch.float32 else _lib.IGA_FP64
+    _lib.check(L.iga_cg_solve_noalloc(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d),
+                                      _lib.ptr(data), prec, _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x),
+                                      float(tol), maxiter, _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
+                                      _lib.ctypes.byref(bb), stream()), "iga_cg_solve_noalloc")
+    return CGResult(x, it.value, rr.value, bb.value)
+
+
 def residual_norms(pattern: CSRPattern, data, u, b):
     """``(||A u - b||, ||b||)`` in FP64 on the device."""
     import torch
