@@ -261,9 +261,11 @@ def cpu_time_to_solve(pattern, vals, rhs, tol=1e-10, reps=3):
 def matfree_bench(refine=6, reps=5, log=print):
     """cfg3: 3-D Poisson on the unit cube, p=2, 64^3 elements, a = exp(GRF l=0.1,
     sigma=1) at the Gauss points, source 1, zero Dirichlet. Matrix-free
-    sum-factorised apply (FP64 and FP32) against the assembled FP64 CSR SpMV,
-    and the solves to 1e-10: FP64 Jacobi-CG on CSR, FP64 matrix-free
-    Jacobi-CG, FP32 matrix-free inner solve with FP64 refinement."""
+    sum-factorised apply (FP64, FP32, and BF16 coefficient with FP32
+    arithmetic) against the assembled FP64 CSR SpMV, and the solves to 1e-10:
+    FP64 Jacobi-CG on CSR, FP64 matrix-free Jacobi-CG, FP32 and BF16
+    matrix-free inner solves with FP64 refinement (Jacobi from the
+    sum-factorised diagonal)."""
     import torch
 
     import jaxiga_amd as jx
@@ -283,8 +285,9 @@ def matfree_bench(refine=6, reps=5, log=print):
     vals, b, diag = M.assemble(ctx, params, diagonal=True)
     pattern = ctx.assembly
     dinv = MF.inverse_guard(diag)
-    op64 = MF.MatrixFreeOperator(ctx, params, _lib.IGA_FP64)
-    op32 = MF.MatrixFreeOperator(ctx, params, _lib.IGA_FP32)
+    ops = {"fp64": MF.MatrixFreeOperator(ctx, params, _lib.IGA_FP64),
+           "fp32": MF.MatrixFreeOperator(ctx, params, _lib.IGA_FP32),
+           "bf16": MF.MatrixFreeOperator(ctx, params, _lib.IGA_BF16)}
     x = torch.randn(ctx.n_free, dtype=torch.float64, device="cuda")
 
     def timed(fn):
@@ -299,13 +302,17 @@ def matfree_bench(refine=6, reps=5, log=print):
         return ev[0].elapsed_time(ev[1]) / reps * 1e-3
 
     t_csr = timed(lambda: LIN.spmv(pattern, vals, x))
-    t_mf64 = timed(lambda: op64.apply(x))
-    t_mf32 = timed(lambda: op32.apply(x))
     y_csr = LIN.spmv(pattern, vals, x)
-    err64 = float((op64.apply(x) - y_csr).abs().max() / y_csr.abs().max())
-    err32 = float((op32.apply(x) - y_csr).abs().max() / y_csr.abs().max())
+    t_apply = {k: timed(lambda op=op: op.apply(x)) for k, op in ops.items()}
+    err = {k: float((op.apply(x) - y_csr).abs().max() / y_csr.abs().max()) for k, op in ops.items()}
+    d_sf = ops["fp64"].diagonal()
+    diag_err = float((d_sf - diag).abs().max() / diag.abs().max())
+    t_diag = timed(lambda: ops["fp64"].diagonal())
     n_e, n_q, n = V.n_elems, 27, ctx.n_free
-    mf_bytes = {8: n_e * 6 * n_q * 8 + n_e * 27 * 8 * 2 + n * 8 * 2, 4: n_e * 6 * n_q * 4 + n_e * 27 * 4 * 2 + n * 8 * 2}
+    # algorithmic bytes of one apply: the coefficient at every Gauss point in
+    # its storage type, x read and y written once (FP64)
+    coef_bytes = {"fp64": 8, "fp32": 4, "bf16": 2}
+    mf_bytes = {k: n_e * n_q * c + 2 * n * 8 for k, c in coef_bytes.items()}
     csr_bytes = pattern.nnz * 12 + n * 8 * 2 + (n + 1) * 4
 
     def solve_time(fn):
@@ -315,27 +322,37 @@ def matfree_bench(refine=6, reps=5, log=print):
         torch.cuda.synchronize()
         return out, time.perf_counter() - t0
 
+    dinv_sf = MF.inverse_guard(d_sf)
     jac, t_j = solve_time(lambda: LIN.cg_csr(pattern, vals, b, 1e-10, dinv=dinv))
-    mfr, t_m = solve_time(lambda: op64.cg(b, 1e-10, dinv=dinv))
-    ref, t_r = solve_time(lambda: PR.mf_mixed_solve(op64, op32, b, dinv, 1e-10))
-    rn, bn = LIN.residual_norms(pattern, vals, ref.x, b)
-    rn_m, _ = LIN.residual_norms(pattern, vals, mfr.x, b)
+    mfr, t_m = solve_time(lambda: ops["fp64"].cg(b, 1e-10, dinv=dinv_sf))
+    ref32, t_r32 = solve_time(lambda: PR.mf_mixed_solve(ops["fp64"], ops["fp32"], b, dinv_sf, 1e-10))
+    ref16, t_r16 = solve_time(lambda: PR.mf_mixed_solve(ops["fp64"], ops["bf16"], b, dinv_sf, 1e-10,
+                                                          inner_tol=1e-3))
+    bn = LIN.residual_norms(pattern, vals, b * 0, b)[1]
+    rel = {k: LIN.residual_norms(pattern, vals, r.x, b)[0] / bn for k, r in
+           (("fp64", mfr), ("fp32", ref32), ("bf16", ref16))}
     out = {
         "workload": f"unit-cube Poisson, p=2, {2**refine}^3 elements, a=exp(GRF l=0.1 sigma=1), {n} free dofs, "
                     f"{pattern.nnz} nonzeros",
-        "apply_seconds": {"csr_fp64": t_csr, "matfree_fp64": t_mf64, "matfree_fp32": t_mf32},
-        "apply_hbm_GBs": {"csr_fp64": csr_bytes / t_csr / 1e9, "matfree_fp64": mf_bytes[8] / t_mf64 / 1e9,
-                          "matfree_fp32": mf_bytes[4] / t_mf32 / 1e9},
-        "apply_rel_error_vs_csr_fp64": {"matfree_fp64": err64, "matfree_fp32": err32},
+        "apply_seconds": {"csr_fp64": t_csr, **{f"matfree_{k}": t for k, t in t_apply.items()}},
+        "apply_algorithmic_GBs": {"csr_fp64": csr_bytes / t_csr / 1e9,
+                                  **{f"matfree_{k}": mf_bytes[k] / t / 1e9 for k, t in t_apply.items()}},
+        "apply_algorithmic_bytes": {"csr_fp64": csr_bytes, **{f"matfree_{k}": b_ for k, b_ in mf_bytes.items()}},
+        "apply_rel_error_vs_csr_fp64": {f"matfree_{k}": e for k, e in err.items()},
+        "sum_factorised_diagonal": {"seconds": t_diag, "rel_error_vs_assembled": diag_err},
         "solve": {"csr_fp64_jacobi_cg": {"seconds": t_j, "iterations": jac.iterations},
                   "matfree_fp64_jacobi_cg": {"seconds": t_m, "iterations": mfr.iterations,
-                                             "relative_residual": rn_m / bn},
-                  "matfree_fp32_refined": {"seconds": t_r, "outer": ref.outer, "inner": ref.total_inner,
-                                           "relative_residual": rn / bn}},
+                                             "relative_residual": rel["fp64"]},
+                  "matfree_fp32_refined": {"seconds": t_r32, "outer": ref32.outer, "inner": ref32.total_inner,
+                                           "relative_residual": rel["fp32"]},
+                  "matfree_bf16_refined": {"seconds": t_r16, "outer": ref16.outer, "inner": ref16.total_inner,
+                                           "relative_residual": rel["bf16"]}},
     }
-    log(f"[bench] cfg3 apply: csr64 {t_csr * 1e3:.3f} ms, mf64 {t_mf64 * 1e3:.3f} ms, mf32 {t_mf32 * 1e3:.3f} ms; "
-        f"solve csr {t_j * 1e3:.1f} ms ({jac.iterations} it), mf64 {t_m * 1e3:.1f} ms, "
-        f"mf32+IR {t_r * 1e3:.1f} ms (rel.res {rn / bn:.1e})")
+    log(f"[bench] cfg3 apply: csr64 {t_csr * 1e3:.3f} ms, " +
+        ", ".join(f"mf {k} {t * 1e3:.3f} ms" for k, t in t_apply.items()) +
+        f"; solve csr {t_j * 1e3:.1f} ms ({jac.iterations} it), mf64 {t_m * 1e3:.1f} ms ({mfr.iterations} it), "
+        f"mf32+IR {t_r32 * 1e3:.1f} ms, bf16+IR {t_r16 * 1e3:.1f} ms "
+        f"(rel.res {rel['fp32']:.1e}, {rel['bf16']:.1e})")
     return out
 
 

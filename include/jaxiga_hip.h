@@ -20,11 +20,14 @@
  *   iga_csr_diag_inv          solvers/linear.py:334-343    (Jacobi inverse diagonal)
  *   iga_spmv                  solvers/linear.py:331-332    (segment-sum matvec)
  *   iga_cg_solve              solvers/linear.py:324-351    (jax cg, Jacobi M)
+ *   iga_cg1_solve             solvers/linear.py:324-351    (the same Jacobi-PCG in
+ *                             single-reduction form)
  *   iga_cg_solve_noalloc      (no reference counterpart: iga_cg_solve without a
  *                             workspace, for callers that cannot allocate)
  *   iga_residual_norms        solvers/diagnostics.py:8-19  (independent ||Au-b||)
  *   iga_mf_setup/iga_mf_apply methods/galerkin.py:207-218 (tangent_matvec,
  *                             sum-factorised instead of a JVP)
+ *   iga_mf_diag               methods/galerkin.py:221-237 (tangent_diagonal)
  *   iga_cg_solve_mf           solvers/linear.py:414-442    (matrix_free_solve, cg)
  *   iga_diff_norms            solvers/diagnostics.py:8-19  (matrix-free diagnostics)
  *   iga_inv_guard             solvers/linear.py:426-430    (Jacobi inverse, zero guard)
@@ -197,12 +200,20 @@ int iga_spmv(int32_t n, const int32_t* indptr, const int32_t* cols, const double
  * stop when ||r||_2 <= tol ||b||_2 or after maxiter iterations. The CSR
  * values are FP64 or FP32 (`precision`); vectors, dot products and all
  * recurrences are FP64. work: iga_cg_workspace(n) doubles.
- * out: iterations, final recursive ||r||^2, ||b||^2.                          */
+ * out: iterations, final recursive ||r||^2, ||b||^2.
+ * iga_cg1_solve: the same Jacobi-PCG in its single-reduction form
+ * (Chronopoulos-Gear; no reference counterpart): the same iterates in exact
+ * arithmetic, two kernels per iteration (the SpMV with the w.u partials
+ * fused in, one vector kernel) instead of three.                             */
 int64_t iga_cg_workspace(int32_t n);
 int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
                  int32_t precision, const double* dinv, const double* b, double* x, double tol,
                  int32_t maxiter, double* work, int32_t* out_iters, double* out_rr, double* out_bb,
                  void* stream);
+int iga_cg1_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
+                  int32_t precision, const double* dinv, const double* b, double* x, double tol,
+                  int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
+                  double* out_bb, void* stream);
 
 /* Jacobi-PCG exactly as iga_cg_solve (same kernels, same results), with no
  * workspace argument and no allocation. No counterpart in the reference: it
@@ -212,7 +223,7 @@ int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols, const vo
  * process ends; n above IGA_CG_NOALLOC_MAX_N returns IGA_ERR_SHAPE. Every
  * call on one GPU shares that array: calls that overlap on one GPU (other
  * host threads, other streams) race and must be serialised by the caller.  */
-#define IGA_CG_NOALLOC_MAX_N 500000
+#define IGA_CG_NOALLOC_MAX_N 400000
 int iga_cg_solve_noalloc(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
                          int32_t precision, const double* dinv, const double* b, double* x,
                          double tol, int32_t maxiter, int32_t* out_iters, double* out_rr,
@@ -243,29 +254,45 @@ int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols, co
                        const double* u, const double* b, double* out, double* work, void* stream);
 
 /* Matrix-free sum-factorised Poisson operator (scalar, vec = 1).
- * iga_mf_workspace : bytes of the operator workspace (geometric factors, 1-D
- *                    tables, element-local results) at the given precision
- * iga_mf_setup     : fill the workspace once (FP64 geometry, stored in
- *                    `precision`); coefficient constant or per Gauss point
- * iga_mf_apply     : y = K x over the free dofs, x and y FP64 (n_free),
- *                    internal arithmetic in `precision`; rational = 0 takes
- *                    the polynomial basis, exact when the patch weights are
- *                    all equal (any value; the Python operator allows a
- *                    1e-14 relative spread from rounding), rational != 0 the
- *                    NURBS quotient                                         */
-int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t rational);
+ * flags: IGA_MF_RATIONAL  -- NURBS quotient (weights not all equal; with
+ *                            equal weights, any value, R_i = N_i and the
+ *                            polynomial apply is exact);
+ *        IGA_MF_SEPARABLE -- polynomial patch whose map is a tensor product
+ *                            of 1-D maps (control-point coordinate c depends
+ *                            on the c-th index only, e.g. a refined box): the
+ *                            geometry is recomputed from 1-D tables and only
+ *                            the coefficient is stored per Gauss point.
+ *        (not both)
+ * precision: IGA_FP64 or IGA_FP32 arithmetic (geometry always built in FP64);
+ *            IGA_BF16 (separable only) stores the coefficient in BF16 and
+ *            runs the apply in FP32.
+ * iga_mf_workspace : bytes of the operator workspace, or -1
+ * iga_mf_setup     : fill the workspace once; coefficient constant or per
+ *                    Gauss point (n_e, n_q). The workspace starts with a
+ *                    header; the library also records the layout under the
+ *                    workspace address, so an apply on a workspace that was
+ *                    not set up, or for another patch, returns an error.
+ * iga_mf_apply     : y = K x over the free dofs, x and y FP64 (n_free).
+ * iga_mf_diag      : diag(K) over the free dofs (FP64), sum-factorised from
+ *                    squared 1-D tables (separable layout only; replaces the
+ *                    scattered element diagonals of tangent_diagonal,
+ *                    methods/galerkin.py:221-237).                          */
+#define IGA_MF_RATIONAL 1
+#define IGA_MF_SEPARABLE 2
+int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t flags);
 int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
-                 int32_t rational, int32_t coef_kind, double coef_const,
-                 const double* coef_points, void* work, int64_t work_bytes, void* stream);
-int iga_mf_apply(const iga_patch_t* patch, int32_t precision, int32_t rational, const void* work,
-                 int64_t work_bytes, int32_t n_free, const int32_t* freed,
-                 const int32_t* full_to_free, const void* x, void* y, void* stream);
+                 int32_t flags, int32_t coef_kind, double coef_const, const double* coef_points,
+                 void* work, int64_t work_bytes, void* stream);
+int iga_mf_apply(const iga_patch_t* patch, const void* work, int64_t work_bytes, int32_t n_free,
+                 const int32_t* freed, const int32_t* full_to_free, const void* x, void* y,
+                 void* stream);
+int iga_mf_diag(const iga_patch_t* patch, const void* work, int64_t work_bytes, int32_t n_free,
+                const int32_t* freed, const int32_t* full_to_free, double* diag, void* stream);
 /* Jacobi-PCG (semantics of iga_cg_solve) on the matrix-free operator. */
-int iga_cg_solve_mf(const iga_patch_t* patch, int32_t precision, int32_t rational,
-                    const void* mf_work, int64_t mf_work_bytes, int32_t n_free,
-                    const int32_t* freed, const int32_t* full_to_free, const double* dinv,
-                    const double* b, double* x, double tol, int32_t maxiter, double* work,
-                    int32_t* out_iters, double* out_rr, double* out_bb, void* stream);
+int iga_cg_solve_mf(const iga_patch_t* patch, const void* mf_work, int64_t mf_work_bytes,
+                    int32_t n_free, const int32_t* freed, const int32_t* full_to_free,
+                    const double* dinv, const double* b, double* x, double tol, int32_t maxiter,
+                    double* work, int32_t* out_iters, double* out_rr, double* out_bb, void* stream);
 
 /* ||y - b||^2 and ||b||^2 in FP64 into out[0], out[1] (device); work: 2048 doubles */
 int iga_diff_norms(int32_t n, const double* y, const double* b, double* out, double* work,

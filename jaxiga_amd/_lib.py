@@ -25,6 +25,7 @@ IGA_COEF_CONST, IGA_COEF_POINTS = 0, 1
 IGA_SRC_NONE, IGA_SRC_CONST, IGA_SRC_POINTS = 0, 1, 2
 IGA_FP64, IGA_FP32, IGA_BF16 = 0, 1, 2
 IGA_OK, IGA_ERR_SHAPE, IGA_ERR_UNSUPPORTED, IGA_ERR_HIP, IGA_ERR_ARG = 0, -1, -2, -3, -4
+IGA_MF_RATIONAL, IGA_MF_SEPARABLE = 1, 2
 
 STATUS = {-1: "inconsistent shapes", -2: "unsupported degree/dimension/quadrature",
           -3: "HIP runtime error", -4: "null pointer or invalid argument"}
@@ -131,6 +132,8 @@ _SIGNATURES = {
     "iga_cg_workspace": (c_i64, [c_int]),
     "iga_cg_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                              c_void_p, c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_cg1_solve": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                              c_void_p, c_double, c_int, c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
     "iga_cg_solve_noalloc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                      c_void_p, c_double, c_int, P_int, P_dbl, P_dbl, c_void_p]),
     "iga_cg_solve_mixed": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -144,11 +147,13 @@ _SIGNATURES = {
     "iga_mf_workspace": (c_i64, [ctypes.POINTER(IgaPatch), c_int, c_int]),
     "iga_mf_setup": (c_int, [ctypes.POINTER(IgaPatch), ctypes.POINTER(IgaRule), c_int, c_int, c_int,
                              c_double, c_void_p, c_void_p, c_i64, c_void_p]),
-    "iga_mf_apply": (c_int, [ctypes.POINTER(IgaPatch), c_int, c_int, c_void_p, c_i64, c_int, c_void_p,
-                             c_void_p, c_void_p, c_void_p, c_void_p]),
-    "iga_cg_solve_mf": (c_int, [ctypes.POINTER(IgaPatch), c_int, c_int, c_void_p, c_i64, c_int,
-                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
-                                c_void_p, P_int, P_dbl, P_dbl, c_void_p]),
+    "iga_mf_apply": (c_int, [ctypes.POINTER(IgaPatch), c_void_p, c_i64, c_int, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p]),
+    "iga_mf_diag": (c_int, [ctypes.POINTER(IgaPatch), c_void_p, c_i64, c_int, c_void_p, c_void_p,
+                            c_void_p, c_void_p]),
+    "iga_cg_solve_mf": (c_int, [ctypes.POINTER(IgaPatch), c_void_p, c_i64, c_int, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_double, c_int, c_void_p, P_int, P_dbl,
+                                P_dbl, c_void_p]),
     "iga_diff_norms": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "iga_inv_guard": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "iga_csr_outer": (c_int, [ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,

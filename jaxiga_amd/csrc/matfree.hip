@@ -23,6 +23,8 @@
 // the test side carries the transposed correction r_i = w_i [ grad N_i . f/W
 // - N_i (grad W . f)/W^2 ] with f = G grad u_h.
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 
 #include "common.hpp"
 #include "matfree.hpp"
@@ -476,38 +478,476 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// y[r] = sum of the local results of the elements supporting free dof r
+// y[r] = sum of the local results of the elements supporting free dof r;
+// grid-stride over at most kRedBlocks blocks, with the x.y partials (one per
+// block) when `parts` is set
 template <int DIM, int P, typename T, typename TO>
 __global__ void __launch_bounds__(256)
     k_mf_gather(iga_patch_t pt, int n_free, const int32_t* __restrict__ freed,
-                const T* __restrict__ rloc, TO* __restrict__ y, const int* skip) {
+                const T* __restrict__ rloc, TO* __restrict__ y, const int* skip,
+                const double* __restrict__ x, double* parts) {
   if (skip && *skip) return;
   constexpr int N1 = P + 1, NL = ipow(N1, DIM);
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_free) return;
-  int s = freed[r];
-  int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  double dot = 0.0;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n_free; r += gridDim.x * blockDim.x) {
+    int s = freed[r];
+    int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < DIM; ++d) {
+      id[d] = s % pt.n_cp[d];
+      s /= pt.n_cp[d];
+      lo[d] = pt.elo[d][id[d]];
+      hi[d] = pt.ehi[d][id[d]];
+    }
+    double sum = 0.0;
+    for (int e2 = lo[2]; e2 <= hi[2]; ++e2)
+      for (int e1 = lo[1]; e1 <= hi[1]; ++e1)
+        for (int e0 = lo[0]; e0 <= hi[0]; ++e0) {
+          const int ee[3] = {e0, e1, e2};
+          int li = 0, stride = 1, e = 0, estride = 1;
+          for (int d = 0; d < DIM; ++d) {
+            li += (id[d] - pt.first[d][ee[d]]) * stride;
+            stride *= N1;
+            e += ee[d] * estride;
+            estride *= pt.n_elem[d];
+          }
+          sum += (double)rloc[(size_t)e * NL + li];
+        }
+    y[r] = (TO)sum;
+    if (parts) dot += sum * x[r];
+  }
+  if (parts) store_partial(parts, dot);
+}
+
+// ---------------------------------------------------------------------------
+// separable patches: one lane per element, geometry from 1-D tables
+// ---------------------------------------------------------------------------
+//
+// On a polynomial patch whose map is a tensor product of 1-D maps (control
+// point coordinate c depends on the c-th index only, e.g. every refined box),
+// J = diag(x_d'(xi_d)), so at Gauss point q = (q_0, .., q_{D-1})
+//   G_q = a_q prod_k jw_k(q_k) diag(s_d(q_d)),
+//   jw_d = wref(q_d) (h_d / 2) |x_d'|,  s_d = 1 / x_d'^2,
+// and the square roots of these 1-D factors are folded into the 1-D tables
+// of each span. The only stream per point is the coefficient a_q (none when
+// it is constant), stored point-major, coef[q][e], in FP64, FP32 or BF16. A lane owns one element: it gathers the
+// element's coefficients, runs the sum-factorised gradient, flux and
+// transposed contractions in registers (no LDS, no barrier) and stores its
+// local results local-function-major, rloc[i][e], so both the stores and the
+// gather's reads of one local index are contiguous across lanes.
+
+// BF16 storage of the coefficient: FP32 rounded to nearest even
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <typename TC>
+__device__ __forceinline__ TC store_coef(double a) {
+  if constexpr (std::is_same<TC, uint16_t>::value) return f32_to_bf16((float)a);
+  else return (TC)a;
+}
+
+// Consecutive blocks go to the eight XCDs in turn; renumber them so that
+// each XCD runs one contiguous range of elements (and keeps the coefficient
+// vector entries of that range in its own L2).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t per = nb / 8;
+  if (per == 0 || b >= per * 8) return b;
+  return (b % 8) * per + b / 8;
+}
+
+struct MfsArgs {
+  int64_t n_e;
+  const int* skip;
+  const int32_t* f2f;
+  const void* tab[3];   // (n_elem_d, Q1, N1) scaled values
+  const void* dtab[3];  // (n_elem_d, Q1, N1) scaled derivatives
+  const void* coef;     // (NQ, n_e) in TC (BF16: point pairs), or null (constant)
+  double coef_const;
+};
+
+// 1-D tables of direction d with the separable geometry folded in (FP64,
+// stored in T): bt[q][a] = sqrt(jw(q)) N_a(q) and dt[q][a] = sqrt(jw(q) s(q))
+// N_a'(q), jw = wref(q) (h/2) |x_d'|, s = 1/x_d'^2. The direction-d term of
+// the element operator at point q is then a_q (dt_d (x) bt_others)^T (dt_d (x)
+// bt_others): every geometric factor is in the tables, none per point.
+template <int P, typename T>
+__global__ void k_mfs_tables(int n_elem, int d, int dim, int64_t stride, const double* __restrict__ ext,
+                             const int32_t* __restrict__ first, const double* __restrict__ lo,
+                             const double* __restrict__ hi, const double* __restrict__ qp,
+                             const double* __restrict__ qw, const double* __restrict__ cpts,
+                             T* __restrict__ tab, T* __restrict__ dtab) {
+  constexpr int N1 = P + 1, Q1 = P + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_elem * Q1) return;
+  const int q = t % Q1, e = t / Q1;
+  double B[N1], dB[N1], v[N1], dv[N1];
+  bernstein<P>(qp[q], B, dB);
+  const double h = hi[e] - lo[e];
+  double xp = 0.0;
+#pragma unroll
+  for (int a = 0; a < N1; ++a) {
+    const double* C = ext + ((size_t)e * N1 + a) * N1;
+    double s = 0.0, ds = 0.0;
+#pragma unroll
+    for (int b = 0; b < N1; ++b) {
+      s += C[b] * B[b];
+      ds += C[b] * dB[b];
+    }
+    v[a] = s;
+    dv[a] = ds * (2.0 / h);
+    xp += dv[a] * cpts[(size_t)(first[e] + a) * stride * dim + d];
+  }
+  const double rj = sqrt(qw[q] * 0.5 * h * fabs(xp));
+  const double rd = rj / fabs(xp);
+#pragma unroll
+  for (int a = 0; a < N1; ++a) {
+    tab[((size_t)e * Q1 + q) * N1 + a] = (T)(rj * v[a]);
+    dtab[((size_t)e * Q1 + q) * N1 + a] = (T)(rd * dv[a]);
+  }
+}
+
+// coefficient at the Gauss points, (n_e, NQ) FP64 -> (NQ, n_e) TC; BF16 in
+// pairs of points, word [(q/2) n_e + e] = (point q even: low half, odd: high)
+template <typename TC>
+__global__ void k_mfs_coef(int64_t n_e, int nq, const double* __restrict__ in, TC* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_e * nq) return;
+  const int64_t e = t / nq;
+  const int q = (int)(t % nq);
+  if constexpr (std::is_same<TC, uint16_t>::value)
+    out[((int64_t)(q >> 1) * n_e + e) * 2 + (q & 1)] = store_coef<TC>(in[t]);
+  else
+    out[(int64_t)q * n_e + e] = store_coef<TC>(in[t]);
+}
+
+// scalar dof of local function (a_0, .., a_{D-1}) of the element at ei
+template <int DIM>
+__device__ __forceinline__ int mfs_dof(const iga_patch_t& pt, const int* ei, const int* ai) {
+  int s = 0, stride = 1;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int f = pt.regular ? ei[d] : pt.first[d][ei[d]];
+    s += (f + ai[d]) * stride;
+    stride *= pt.n_cp[d];
+  }
+  return s;
+}
+
+// MODE 0: y_loc = K_e u_loc; MODE 1: diag(K_e) (squared tables, no input)
+template <int DIM, int P, typename T, typename TC, int MODE>
+__global__ void __launch_bounds__(256)
+    k_mfs_apply(iga_patch_t pt, MfsArgs A, const double* __restrict__ v, T* __restrict__ rloc) {
+  if (A.skip && *A.skip) return;
+  constexpr int N1 = P + 1, Q1 = P + 1, NL = ipow(N1, DIM);
+  // a block covers a run of elements along direction 0 at fixed (e_1, e_2):
+  // the direction-1 and -2 spans, and so their tables, are uniform across
+  // the block (scalar loads); only direction 0 varies per lane
+  const int n0 = pt.n_elem[0];
+  const int nb0 = (n0 + blockDim.x - 1) / blockDim.x;
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t row = b / nb0;
+  const int e0 = (int)(b % nb0) * blockDim.x + threadIdx.x;
+  if (e0 >= n0) return;
+  const int64_t e = row * n0 + e0;
+  int ei[3] = {e0, 0, 0};
+  if constexpr (DIM == 2) {
+    ei[1] = (int)row;
+  } else {
+    ei[1] = (int)(row % pt.n_elem[1]);
+    ei[2] = (int)(row / pt.n_elem[1]);
+  }
+  // scaled 1-D tables of the element's spans (squared for the diagonal)
+  T B[DIM][Q1][N1], D[DIM][Q1][N1];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const T* tb = reinterpret_cast<const T*>(A.tab[d]) + (size_t)ei[d] * Q1 * N1;
+    const T* td = reinterpret_cast<const T*>(A.dtab[d]) + (size_t)ei[d] * Q1 * N1;
+#pragma unroll
+    for (int q = 0; q < Q1; ++q)
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        T bv = tb[q * N1 + a], dv = td[q * N1 + a];
+        if (MODE == 1) {
+          bv *= bv;
+          dv *= dv;
+        }
+        B[d][q][a] = bv;
+        D[d][q][a] = dv;
+      }
+  }
+  auto coef_at = [&](int q) -> T {
+    if (!A.coef) return (T)A.coef_const;
+    if constexpr (std::is_same<TC, uint16_t>::value) {
+      const uint32_t w = reinterpret_cast<const uint32_t*>(A.coef)[(int64_t)(q >> 1) * A.n_e + e];
+      return (T)bf16_to_f32((uint16_t)((q & 1) ? (w >> 16) : (w & 0xFFFFu)));
+    } else {
+      return (T) reinterpret_cast<const TC*>(A.coef)[(int64_t)q * A.n_e + e];
+    }
+  };
+  T r[NL];
+  if constexpr (DIM == 2) {
+    T A0[N1][Q1], A0d[N1][Q1];  // [i1][q0]
+    if constexpr (MODE == 0) {
+      T u[N1][N1];
+#pragma unroll
+      for (int i1 = 0; i1 < N1; ++i1)
+#pragma unroll
+        for (int i0 = 0; i0 < N1; ++i0) {
+          const int ai[2] = {i0, i1};
+          const int f = A.f2f[mfs_dof<2>(pt, ei, ai)];
+          // unconditional load from a clamped index: all loads issue at once
+          const double val = v[f >= 0 ? f : 0];
+          u[i1][i0] = f >= 0 ? (T)val : (T)0;
+        }
+#pragma unroll
+      for (int i1 = 0; i1 < N1; ++i1)
+#pragma unroll
+        for (int q0 = 0; q0 < Q1; ++q0) {
+          T s = 0, sd = 0;
+#pragma unroll
+          for (int i0 = 0; i0 < N1; ++i0) {
+            s += B[0][q0][i0] * u[i1][i0];
+            sd += D[0][q0][i0] * u[i1][i0];
+          }
+          A0[i1][q0] = s;
+          A0d[i1][q0] = sd;
+        }
+    }
+    T C0[N1][Q1], C1[N1][Q1];
+#pragma unroll
+    for (int i1 = 0; i1 < N1; ++i1)
+#pragma unroll
+      for (int q0 = 0; q0 < Q1; ++q0) C0[i1][q0] = C1[i1][q0] = 0;
+#pragma unroll
+    for (int q1 = 0; q1 < Q1; ++q1)
+#pragma unroll
+      for (int q0 = 0; q0 < Q1; ++q0) {
+        const T c = coef_at(q1 * Q1 + q0);
+        T f0, f1;
+        if constexpr (MODE == 0) {
+          T g0 = 0, g1 = 0;
+#pragma unroll
+          for (int i1 = 0; i1 < N1; ++i1) {
+            g0 += B[1][q1][i1] * A0d[i1][q0];
+            g1 += D[1][q1][i1] * A0[i1][q0];
+          }
+          f0 = c * g0;
+          f1 = c * g1;
+        } else {
+          f0 = f1 = c;
+        }
+#pragma unroll
+        for (int i1 = 0; i1 < N1; ++i1) {
+          C0[i1][q0] += B[1][q1][i1] * f0;
+          C1[i1][q0] += D[1][q1][i1] * f1;
+        }
+      }
+#pragma unroll
+    for (int i1 = 0; i1 < N1; ++i1)
+#pragma unroll
+      for (int i0 = 0; i0 < N1; ++i0) {
+        T s = 0;
+#pragma unroll
+        for (int q0 = 0; q0 < Q1; ++q0) s += D[0][q0][i0] * C0[i1][q0] + B[0][q0][i0] * C1[i1][q0];
+        r[i1 * N1 + i0] = s;
+      }
+  } else {
+    // one 1-D Gauss point q0 at a time: direction 0 at q0, then directions 1
+    // and 2, the flux and the transposed directions 2 and 1 on that slice,
+    // accumulated into r through the transposed direction 0 (fewer values
+    // live at once than whole-element stages)
+    T u[N1][N1][N1];
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int i2 = 0; i2 < N1; ++i2)
+#pragma unroll
+        for (int i1 = 0; i1 < N1; ++i1)
+#pragma unroll
+          for (int i0 = 0; i0 < N1; ++i0) {
+            const int ai[3] = {i0, i1, i2};
+            const int f = A.f2f[mfs_dof<3>(pt, ei, ai)];
+            // unconditional load from a clamped index: all loads issue at once
+            const double val = v[f >= 0 ? f : 0];
+            u[i2][i1][i0] = f >= 0 ? (T)val : (T)0;
+          }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) r[i] = 0;
+#pragma unroll
+    for (int q0 = 0; q0 < Q1; ++q0) {
+      T A1[N1][Q1], A1x[N1][Q1], A1y[N1][Q1];  // [i2][q1] at this q0
+      if constexpr (MODE == 0) {
+#pragma unroll
+        for (int i2 = 0; i2 < N1; ++i2) {
+          T a0[N1], ad[N1];
+#pragma unroll
+          for (int i1 = 0; i1 < N1; ++i1) {
+            T s = 0, sd = 0;
+#pragma unroll
+            for (int i0 = 0; i0 < N1; ++i0) {
+              s += B[0][q0][i0] * u[i2][i1][i0];
+              sd += D[0][q0][i0] * u[i2][i1][i0];
+            }
+            a0[i1] = s;
+            ad[i1] = sd;
+          }
+#pragma unroll
+          for (int q1 = 0; q1 < Q1; ++q1) {
+            T a = 0, ax = 0, ay = 0;
+#pragma unroll
+            for (int i1 = 0; i1 < N1; ++i1) {
+              a += B[1][q1][i1] * a0[i1];
+              ax += B[1][q1][i1] * ad[i1];
+              ay += D[1][q1][i1] * a0[i1];
+            }
+            A1[i2][q1] = a;
+            A1x[i2][q1] = ax;
+            A1y[i2][q1] = ay;
+          }
+        }
+      }
+      T C0[N1][Q1], C1[N1][Q1], C2[N1][Q1];  // [i2][q1]
+#pragma unroll
+      for (int q1 = 0; q1 < Q1; ++q1) {
+        T f0[Q1], f1[Q1], f2[Q1];
+#pragma unroll
+        for (int q2 = 0; q2 < Q1; ++q2) {
+          const T c = coef_at((q2 * Q1 + q1) * Q1 + q0);
+          if constexpr (MODE == 0) {
+            T g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+            for (int i2 = 0; i2 < N1; ++i2) {
+              g0 += B[2][q2][i2] * A1x[i2][q1];
+              g1 += B[2][q2][i2] * A1y[i2][q1];
+              g2 += D[2][q2][i2] * A1[i2][q1];
+            }
+            f0[q2] = c * g0;
+            f1[q2] = c * g1;
+            f2[q2] = c * g2;
+          } else {
+            f0[q2] = f1[q2] = f2[q2] = c;
+          }
+        }
+#pragma unroll
+        for (int i2 = 0; i2 < N1; ++i2) {
+          T c0 = 0, c1 = 0, c2 = 0;
+#pragma unroll
+          for (int q2 = 0; q2 < Q1; ++q2) {
+            c0 += B[2][q2][i2] * f0[q2];
+            c1 += B[2][q2][i2] * f1[q2];
+            c2 += D[2][q2][i2] * f2[q2];
+          }
+          C0[i2][q1] = c0;
+          C1[i2][q1] = c1;
+          C2[i2][q1] = c2;
+        }
+      }
+#pragma unroll
+      for (int i2 = 0; i2 < N1; ++i2)
+#pragma unroll
+        for (int i1 = 0; i1 < N1; ++i1) {
+          T e0 = 0, e1 = 0;
+#pragma unroll
+          for (int q1 = 0; q1 < Q1; ++q1) {
+            e0 += B[1][q1][i1] * C0[i2][q1];
+            e1 += D[1][q1][i1] * C1[i2][q1] + B[1][q1][i1] * C2[i2][q1];
+          }
+#pragma unroll
+          for (int i0 = 0; i0 < N1; ++i0)
+            r[(i2 * N1 + i1) * N1 + i0] += D[0][q0][i0] * e0 + B[0][q0][i0] * e1;
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NL; ++i) rloc[(int64_t)i * A.n_e + e] = r[i];
+}
+
+// y[r] = sum over the elements supporting free dof r of their local result
+// (local-function-major rloc), in a fixed order
+template <int DIM, int P, typename T>
+__device__ __forceinline__ double mfs_gather_one(const iga_patch_t& pt, int64_t n_e, int s,
+                                                 const T* __restrict__ rloc) {
+  constexpr int N1 = P + 1;
+  int id[3] = {0, 0, 0};
+#pragma unroll
   for (int d = 0; d < DIM; ++d) {
     id[d] = s % pt.n_cp[d];
     s /= pt.n_cp[d];
+  }
+  double sum = 0.0;
+  if (pt.regular) {
+    // function i is supported on spans i-P .. i with local index P-k on span
+    // i-P+k: every (k_0, .., k_{D-1}) is read unconditionally from a clamped
+    // address and the absent spans are multiplied out, so all loads are in
+    // flight together; the order of the sum is fixed
+    constexpr int NS = ipow(N1, DIM);
+    T vals[NS];
+    bool ok[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      const int k[3] = {t % N1, (t / N1) % N1, t / (N1 * N1)};
+      int64_t e = 0, estride = 1;
+      int li = 0, stride = 1;
+      bool in = true;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int ed = id[d] - P + k[d];
+        in = in && ed >= 0 && ed < pt.n_elem[d];
+        e += (int64_t)(ed < 0 ? 0 : ed >= pt.n_elem[d] ? pt.n_elem[d] - 1 : ed) * estride;
+        estride *= pt.n_elem[d];
+        li += (P - k[d]) * stride;
+        stride *= N1;
+      }
+      vals[t] = rloc[(int64_t)li * n_e + e];
+      ok[t] = in;
+    }
+#pragma unroll
+    for (int t = 0; t < NS; ++t) sum += ok[t] ? (double)vals[t] : 0.0;
+    return sum;
+  }
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
     lo[d] = pt.elo[d][id[d]];
     hi[d] = pt.ehi[d][id[d]];
   }
-  double sum = 0.0;
   for (int e2 = lo[2]; e2 <= hi[2]; ++e2)
     for (int e1 = lo[1]; e1 <= hi[1]; ++e1)
       for (int e0 = lo[0]; e0 <= hi[0]; ++e0) {
         const int ee[3] = {e0, e1, e2};
-        int li = 0, stride = 1, e = 0, estride = 1;
+        int li = 0, stride = 1;
+        int64_t e = 0, estride = 1;
+#pragma unroll
         for (int d = 0; d < DIM; ++d) {
           li += (id[d] - pt.first[d][ee[d]]) * stride;
           stride *= N1;
           e += ee[d] * estride;
           estride *= pt.n_elem[d];
         }
-        sum += (double)rloc[(size_t)e * NL + li];
+        sum += (double)rloc[(int64_t)li * n_e + e];
       }
-  y[r] = (TO)sum;
+  return sum;
+}
+
+// y[r] = sum over the elements supporting free dof r of their local result
+// (local-function-major rloc), in a fixed order; grid-stride over at most
+// kRedBlocks blocks, with the x.y partials (one per block) when `parts` is set
+template <int DIM, int P, typename T>
+__global__ void __launch_bounds__(256)
+    k_mfs_gather(iga_patch_t pt, int n_free, int64_t n_e, const int32_t* __restrict__ freed,
+                 const T* __restrict__ rloc, double* __restrict__ y, const int* skip,
+                 const double* __restrict__ x, double* parts) {
+  if (skip && *skip) return;
+  double dot = 0.0;
+  const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+  for (int64_t r = b * blockDim.x + threadIdx.x; r < n_free; r += (int64_t)gridDim.x * blockDim.x) {
+    const double sum = mfs_gather_one<DIM, P, T>(pt, n_e, freed[r], rloc);
+    y[r] = sum;
+    if (parts) dot += sum * x[r];
+  }
+  if (parts) store_partial(parts, dot);
 }
 
 }  // namespace iga
@@ -520,29 +960,58 @@ using namespace iga;
 
 namespace {
 
+// Workspace header: what iga_mf_setup built. The library also keeps a copy
+// keyed by the workspace address, so that an apply can check (without a
+// device read) that the workspace was set up, and for which layout.
+struct MfInfo {
+  int32_t magic, dim, degree, precision, flags, coef_kind;
+  int64_t n_e, bytes;
+  double coef_const;
+};
+constexpr int32_t kMfMagic = 0x4d465731;  // "MFW1"
+constexpr int64_t kMfHeader = 256;
+
+std::mutex g_mf_mu;
+std::unordered_map<const void*, MfInfo> g_mf;
+
 struct MfLayout {
-  int64_t n_e, geo, wgeo, tabs[3], dtabs[3], rloc, total;  // byte offsets
+  int64_t n_e, geo, wgeo, tabs[3], dtabs[3], coef, rloc, total;  // byte offsets
 };
 
+// general layout: G (NG per point) and, when rational, W and grad W per point;
+// separable layout: 1-D tables (values, derivatives, jw, s) and the
+// coefficient per point in the storage type of `precision`
 template <int DIM, int P>
-MfLayout mf_layout(const iga_patch_t& pt, int tsize, int rational) {
+MfLayout mf_layout(const iga_patch_t& pt, int precision, int flags) {
   using S = MfShape<DIM, P>;
+  const int ts = precision == IGA_FP64 ? 8 : 4;  // arithmetic type
+  const int cs = precision == IGA_FP64 ? 8 : precision == IGA_FP32 ? 4 : 2;  // coefficient storage
   MfLayout L{};
+  for (int d = 0; d < 3; ++d) L.tabs[d] = L.dtabs[d] = -1;
+  L.geo = L.wgeo = L.coef = -1;
   L.n_e = 1;
   for (int d = 0; d < DIM; ++d) L.n_e *= pt.n_elem[d];
-  int64_t off = 0;
+  int64_t off = kMfHeader;
   auto take = [&](int64_t bytes) {
     const int64_t o = off;
     off += (bytes + 255) & ~(int64_t)255;
     return o;
   };
-  L.geo = take(L.n_e * S::NG * S::NQ * tsize);
-  L.wgeo = rational ? take(L.n_e * S::NW * S::NQ * tsize) : -1;
-  for (int d = 0; d < DIM; ++d) {
-    L.tabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * tsize);
-    L.dtabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * tsize);
+  if (flags & IGA_MF_SEPARABLE) {
+    for (int d = 0; d < DIM; ++d) {
+      L.tabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * ts);
+      L.dtabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * ts);
+    }
+    L.coef = take(L.n_e * (cs == 2 ? (S::NQ + 1) / 2 * 4 : S::NQ * cs));
+  } else {
+    L.geo = take(L.n_e * S::NG * S::NQ * ts);
+    L.wgeo = (flags & IGA_MF_RATIONAL) ? take(L.n_e * S::NW * S::NQ * ts) : -1;
+    for (int d = 0; d < DIM; ++d) {
+      L.tabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * ts);
+      L.dtabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * ts);
+    }
   }
-  L.rloc = take(L.n_e * S::NL * tsize);
+  L.rloc = take(L.n_e * S::NL * ts);
   L.total = off;
   return L;
 }
@@ -560,36 +1029,92 @@ int by_shape(int dim, int degree, F f) {
   }
 }
 
+// valid combinations: FP64 / FP32 with either layout (rational only in the
+// general one), BF16 (BF16 coefficient, FP32 arithmetic) separable only
+int check_flags(int precision, int flags) {
+  if (flags & ~(IGA_MF_RATIONAL | IGA_MF_SEPARABLE)) return IGA_ERR_ARG;
+  if ((flags & IGA_MF_RATIONAL) && (flags & IGA_MF_SEPARABLE)) return IGA_ERR_ARG;
+  if (precision == IGA_FP64 || precision == IGA_FP32) return IGA_OK;
+  if (precision == IGA_BF16 && (flags & IGA_MF_SEPARABLE)) return IGA_OK;
+  return IGA_ERR_UNSUPPORTED;
+}
+
+int lookup(const iga_patch_t* patch, const void* work, int64_t work_bytes, MfInfo* out) {
+  std::lock_guard<std::mutex> lock(g_mf_mu);
+  auto it = g_mf.find(work);
+  if (it == g_mf.end()) return IGA_ERR_ARG;  // not set up by iga_mf_setup
+  const MfInfo& I = it->second;
+  int64_t n_e = 1;
+  for (int d = 0; d < patch->dim; ++d) n_e *= patch->n_elem[d];
+  if (I.dim != patch->dim || I.degree != patch->degree || I.n_e != n_e || work_bytes < I.bytes)
+    return IGA_ERR_SHAPE;
+  *out = I;
+  return IGA_OK;
+}
+
 }  // namespace
 
-extern "C" int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t rational) {
-  if (!patch) return -1;
-  const int ts = precision == IGA_FP32 ? 4 : 8;
+extern "C" int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t flags) {
+  if (!patch || check_flags(precision, flags)) return -1;
   int64_t out = -1;
   by_shape(patch->dim, patch->degree, [&](auto D, auto P) {
-    out = mf_layout<decltype(D)::value, decltype(P)::value>(*patch, ts, rational).total;
+    out = mf_layout<decltype(D)::value, decltype(P)::value>(*patch, precision, flags).total;
     return 0;
   });
   return out;
 }
 
 extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
-                            int32_t rational, int32_t coef_kind, double coef_const,
+                            int32_t flags, int32_t coef_kind, double coef_const,
                             const double* coef_points, void* work, int64_t work_bytes,
                             void* stream) {
   if (!patch || !rule || !work) return IGA_ERR_ARG;
   if (rule->n_q1 != patch->degree + 1) return IGA_ERR_UNSUPPORTED;
-  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
+  if (int rc = check_flags(precision, flags)) return rc;
+  if (coef_kind == IGA_COEF_POINTS && !coef_points) return IGA_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  return by_shape(patch->dim, patch->degree, [&](auto D, auto Pc) -> int {
+  const int rational = flags & IGA_MF_RATIONAL;
+  int rc = by_shape(patch->dim, patch->degree, [&](auto D, auto Pc) -> int {
     constexpr int DIM = decltype(D)::value, P = decltype(Pc)::value;
+    using S = MfShape<DIM, P>;
+    const MfLayout L = mf_layout<DIM, P>(*patch, precision, flags);
+    if (work_bytes < L.total) return IGA_ERR_SHAPE;
+    char* base = reinterpret_cast<char*>(work);
+    const int64_t pts = L.n_e * S::NQ;
+    if (flags & IGA_MF_SEPARABLE) {
+      auto tables = [&](auto tag) -> int {
+        using T = decltype(tag);
+        int64_t stride = 1;
+        for (int d = 0; d < DIM; ++d) {
+          const int n = patch->n_elem[d] * S::Q1;
+          hipLaunchKernelGGL((k_mfs_tables<P, T>), dim3((n + 255) / 256), dim3(256), 0, st,
+                             patch->n_elem[d], d, DIM, stride, patch->ext[d], patch->first[d],
+                             patch->lo[d], patch->hi[d], rule->pts, rule->wts, patch->cpts,
+                             reinterpret_cast<T*>(base + L.tabs[d]),
+                             reinterpret_cast<T*>(base + L.dtabs[d]));
+          IGA_CHECK_LAUNCH();
+          stride *= patch->n_cp[d];
+        }
+        return IGA_OK;
+      };
+      if (int rc2 = precision == IGA_FP64 ? tables(double{}) : tables(float{})) return rc2;
+      if (coef_kind == IGA_COEF_POINTS && pts > 0) {
+        const unsigned g = (unsigned)((pts + 255) / 256);
+        if (precision == IGA_FP64)
+          hipLaunchKernelGGL(k_mfs_coef<double>, dim3(g), dim3(256), 0, st, L.n_e, S::NQ, coef_points,
+                             reinterpret_cast<double*>(base + L.coef));
+        else if (precision == IGA_FP32)
+          hipLaunchKernelGGL(k_mfs_coef<float>, dim3(g), dim3(256), 0, st, L.n_e, S::NQ, coef_points,
+                             reinterpret_cast<float*>(base + L.coef));
+        else
+          hipLaunchKernelGGL(k_mfs_coef<uint16_t>, dim3(g), dim3(256), 0, st, L.n_e, S::NQ,
+                             coef_points, reinterpret_cast<uint16_t*>(base + L.coef));
+        IGA_CHECK_LAUNCH();
+      }
+      return IGA_OK;
+    }
     auto run = [&](auto tag) -> int {
       using T = decltype(tag);
-      const MfLayout L = mf_layout<DIM, P>(*patch, sizeof(T), rational);
-      if (work_bytes < L.total) return IGA_ERR_SHAPE;
-      char* base = reinterpret_cast<char*>(work);
-      using S = MfShape<DIM, P>;
-      const int64_t pts = L.n_e * S::NQ;
       if (pts > 0) {
         hipLaunchKernelGGL((k_mf_geometry<DIM, P, T>), dim3((unsigned)((pts + 255) / 256)), dim3(256),
                            0, st, *patch, rule->pts, rule->wts, coef_kind, coef_const, coef_points,
@@ -609,24 +1134,86 @@ extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, in
     };
     return precision == IGA_FP32 ? run(float{}) : run(double{});
   });
+  if (rc) return rc;
+  MfInfo I{};
+  I.magic = kMfMagic;
+  I.dim = patch->dim;
+  I.degree = patch->degree;
+  I.precision = precision;
+  I.flags = flags;
+  I.coef_kind = coef_kind;
+  I.coef_const = coef_const;
+  I.n_e = 1;
+  for (int d = 0; d < patch->dim; ++d) I.n_e *= patch->n_elem[d];
+  I.bytes = iga_mf_workspace(patch, precision, flags);
+  // the header travels with the workspace (stream-ordered copy from the
+  // library's registry entry, which outlives the call)
+  {
+    std::lock_guard<std::mutex> lock(g_mf_mu);
+    auto& slot = g_mf[work];
+    slot = I;
+    if (hipMemcpyAsync(work, &slot, sizeof(MfInfo), hipMemcpyHostToDevice, st) != hipSuccess)
+      return IGA_ERR_HIP;
+  }
+  return IGA_OK;
 }
 
 namespace iga {
 
-int mf_apply_launch(const iga_patch_t* patch, int precision, int rational, const void* work,
-                    int64_t work_bytes, int n_free, const int32_t* freed,
-                    const int32_t* full_to_free, const double* x, double* y, const int* skip,
-                    hipStream_t st) {
-  if (!patch || !work || !freed || !full_to_free || !x || !y) return IGA_ERR_ARG;
-  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
+static int mf_launch(const iga_patch_t* patch, const MfInfo& I, const void* work, int n_free,
+                     const int32_t* freed, const int32_t* full_to_free, const double* x, double* y,
+                     const int* skip, int mode, double* parts, int* nparts, hipStream_t st) {
+  // gather blocks: one row per lane, at most kRedBlocks blocks (grid-stride)
+  const int gb = (int)std::max<int64_t>(1, std::min<int64_t>((n_free + 255) / 256, kRedBlocks));
+  if (nparts) *nparts = gb;
   return by_shape(patch->dim, patch->degree, [&](auto D, auto Pc) -> int {
     constexpr int DIM = decltype(D)::value, P = decltype(Pc)::value;
+    using S = MfShape<DIM, P>;
+    const MfLayout L = mf_layout<DIM, P>(*patch, I.precision, I.flags);
+    char* base = const_cast<char*>(reinterpret_cast<const char*>(work));
+    if (I.flags & IGA_MF_SEPARABLE) {
+      MfsArgs A{};
+      A.n_e = L.n_e;
+      A.skip = skip;
+      A.f2f = full_to_free;
+      for (int d = 0; d < DIM; ++d) {
+        A.tab[d] = base + L.tabs[d];
+        A.dtab[d] = base + L.dtabs[d];
+      }
+      A.coef = I.coef_kind == IGA_COEF_POINTS ? base + L.coef : nullptr;
+      A.coef_const = I.coef_const;
+      auto run = [&](auto ttag, auto ctag) -> int {
+        using T = decltype(ttag);
+        using TC = decltype(ctag);
+        T* rloc = reinterpret_cast<T*>(base + L.rloc);
+        // blocks of 64..256 lanes along direction 0, one run of a row each
+        const int n0 = patch->n_elem[0];
+        const int bw = std::min(256, (n0 + 63) / 64 * 64);
+        const int64_t blocks = L.n_e > 0 ? (L.n_e / n0) * ((n0 + bw - 1) / bw) : 0;
+        if (blocks > 0) {
+          if (mode == 0)
+            hipLaunchKernelGGL((k_mfs_apply<DIM, P, T, TC, 0>), dim3((unsigned)blocks), dim3(bw), 0,
+                               st, *patch, A, x, rloc);
+          else
+            hipLaunchKernelGGL((k_mfs_apply<DIM, P, T, TC, 1>), dim3((unsigned)blocks), dim3(bw), 0,
+                               st, *patch, A, x, rloc);
+          IGA_CHECK_LAUNCH();
+        }
+        if (n_free > 0) {
+          hipLaunchKernelGGL((k_mfs_gather<DIM, P, T>), dim3(gb), dim3(256), 0, st, *patch, n_free,
+                             L.n_e, freed, rloc, y, skip, x, parts);
+          IGA_CHECK_LAUNCH();
+        }
+        return IGA_OK;
+      };
+      if (I.precision == IGA_FP64) return run(double{}, double{});
+      if (I.precision == IGA_FP32) return run(float{}, float{});
+      return run(float{}, uint16_t{});
+    }
+    if (mode != 0) return (int)IGA_ERR_UNSUPPORTED;  // the diagonal is sum-factorised on separable patches
+    const bool rational = I.flags & IGA_MF_RATIONAL;
     auto run = [&](auto tag) -> int {
       using T = decltype(tag);
-      using S = MfShape<DIM, P>;
-      const MfLayout L = mf_layout<DIM, P>(*patch, sizeof(T), rational);
-      if (work_bytes < L.total) return IGA_ERR_SHAPE;
-      char* base = const_cast<char*>(reinterpret_cast<const char*>(work));
       MfArgs A;
       A.n_e = L.n_e;
       A.skip = skip;
@@ -649,23 +1236,43 @@ int mf_apply_launch(const iga_patch_t* patch, int precision, int rational, const
         IGA_CHECK_LAUNCH();
       }
       if (n_free > 0) {
-        hipLaunchKernelGGL((k_mf_gather<DIM, P, T, double>), dim3((n_free + 255) / 256), dim3(256), 0,
-                           st, *patch, n_free, freed, rloc, y, skip);
+        hipLaunchKernelGGL((k_mf_gather<DIM, P, T, double>), dim3(gb), dim3(256), 0, st, *patch,
+                           n_free, freed, rloc, y, skip, x, parts);
         IGA_CHECK_LAUNCH();
       }
       return IGA_OK;
     };
-    return precision == IGA_FP32 ? run(float{}) : run(double{});
+    return I.precision == IGA_FP32 ? run(float{}) : run(double{});
   });
+}
+
+int mf_apply_launch(const iga_patch_t* patch, const void* work, int64_t work_bytes, int n_free,
+                    const int32_t* freed, const int32_t* full_to_free, const double* x, double* y,
+                    const int* skip, double* parts, int* nparts, hipStream_t st) {
+  if (!patch || !work || !freed || !full_to_free || !x || !y) return IGA_ERR_ARG;
+  MfInfo I;
+  if (int rc = lookup(patch, work, work_bytes, &I)) return rc;
+  return mf_launch(patch, I, work, n_free, freed, full_to_free, x, y, skip, 0, parts, nparts, st);
 }
 
 }  // namespace iga
 
-extern "C" int iga_mf_apply(const iga_patch_t* patch, int32_t precision, int32_t rational,
-                            const void* work, int64_t work_bytes, int32_t n_free,
-                            const int32_t* freed, const int32_t* full_to_free, const void* x,
-                            void* y, void* stream) {
-  return iga::mf_apply_launch(patch, precision, rational, work, work_bytes, n_free, freed,
-                              full_to_free, reinterpret_cast<const double*>(x),
-                              reinterpret_cast<double*>(y), nullptr, (hipStream_t)stream);
+extern "C" int iga_mf_apply(const iga_patch_t* patch, const void* work, int64_t work_bytes,
+                            int32_t n_free, const int32_t* freed, const int32_t* full_to_free,
+                            const void* x, void* y, void* stream) {
+  return iga::mf_apply_launch(patch, work, work_bytes, n_free, freed, full_to_free,
+                              reinterpret_cast<const double*>(x), reinterpret_cast<double*>(y),
+                              nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int iga_mf_diag(const iga_patch_t* patch, const void* work, int64_t work_bytes,
+                           int32_t n_free, const int32_t* freed, const int32_t* full_to_free,
+                           double* diag, void* stream) {
+  if (!patch || !work || !freed || !full_to_free || !diag) return IGA_ERR_ARG;
+  MfInfo I;
+  if (int rc = lookup(patch, work, work_bytes, &I)) return rc;
+  if (!(I.flags & IGA_MF_SEPARABLE)) return IGA_ERR_UNSUPPORTED;
+  // the input vector is not read in diagonal mode; pass diag as a valid address
+  return iga::mf_launch(patch, I, work, n_free, freed, full_to_free, diag, diag, nullptr, 1,
+                        nullptr, nullptr, (hipStream_t)stream);
 }

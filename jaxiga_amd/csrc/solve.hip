@@ -408,8 +408,171 @@ extern "C" int iga_spmv(int32_t n, const int32_t* indptr, const int32_t* cols, c
   });
 }
 
-// workspace layout (doubles): state (8), r, z, p, q (4n), partials (5 * kRedBlocks)
-extern "C" int64_t iga_cg_workspace(int32_t n) { return 8 + 4 * (int64_t)n + 5 * kRedBlocks; }
+// workspace (doubles), large enough for both loops:
+//   reliable-update loop: state (8), r, z, p, q (4n), partials (5 kRedBlocks)
+//   single-reduction loop: state (16), r, u, p, s, w (5n), partials (6 kRedBlocks)
+extern "C" int64_t iga_cg_workspace(int32_t n) { return 16 + 5 * (int64_t)n + 6 * kRedBlocks; }
+
+// ---------------------------------------------------------------------------
+// Single-reduction Jacobi-PCG (Chronopoulos & Gear, 1989): the same iterates
+// as the textbook recurrence in exact arithmetic, with one reduction point
+// per iteration, so an iteration is the operator (w = A u with the w.u
+// partials fused in) plus one vector kernel:
+//   beta_i = gamma_i / gamma_{i-1},  alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,  u = D^-1 r
+// with gamma = r.u, delta = w.u; the stopping test ||r|| <= tol ||b|| is made
+// on r_i before iteration i, as in the textbook loop.
+// ---------------------------------------------------------------------------
+
+struct CG1Bufs {
+  CGState* st;
+  double* gam;   // [2] gamma by iteration parity
+  double* alp;   // [2] alpha by iteration parity
+  double* P_ru;  // [2][kRedBlocks]
+  double* P_rr;  // [2][kRedBlocks]
+  double* P_wu;  // [kRedBlocks]
+  double* P_bb;  // [kRedBlocks]
+};
+
+static CG1Bufs cg1_bufs(double* work, int32_t n) {
+  CG1Bufs B;
+  B.st = reinterpret_cast<CGState*>(work);
+  B.gam = work + 8;
+  B.alp = work + 10;
+  double* parts = work + 16 + 5 * (int64_t)n;
+  B.P_ru = parts;
+  B.P_rr = parts + 2 * kRedBlocks;
+  B.P_wu = parts + 4 * kRedBlocks;
+  B.P_bb = parts + 5 * kRedBlocks;
+  return B;
+}
+
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg1_init(int n, const double* __restrict__ b, const double* __restrict__ dinv, double* x,
+               double* r, double* u, double* p, double* s, CG1Bufs B) {
+  double ru = 0.0, bb = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double bi = b[i];
+    const double ui = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    u[i] = ui;
+    p[i] = 0.0;
+    s[i] = 0.0;
+    ru += bi * ui;
+    bb += bi * bi;
+  }
+  store_partial(B.P_ru, ru);
+  store_partial(B.P_rr, bb);
+  store_partial(B.P_bb, bb);
+}
+
+__global__ void k_cg1_init_fin(int nblk, CG1Bufs B, double tol2, int maxiter) {
+  const double bb = reduce_partials(B.P_bb, nblk);
+  if (threadIdx.x == 0) {
+    CGState* st = B.st;
+    st->bb = bb;
+    st->rr = bb;
+    st->tol2 = tol2;
+    st->iter = 0;
+    st->maxiter = maxiter;
+    st->done = (!(bb > tol2 * bb) || maxiter <= 0) ? 1 : 0;
+  }
+}
+
+// iteration `it` (host counter): test r_it, then the update. Every block
+// reduces the same partials in the same order, so every block takes the same
+// branch; block 0 records the state. gamma/alpha of iteration it are written
+// to slot it&1 and read by iteration it+1 from the other slot.
+__global__ void __launch_bounds__(kVecThreads)
+    k_cg1_update(int n, int it, int nblk_wu, int nblk, const double* __restrict__ dinv,
+                 const double* __restrict__ w, double* u, double* x, double* r, double* p,
+                 double* s, CG1Bufs B) {
+  CGState* st = B.st;
+  if (st->done) return;
+  const int par = it & 1;
+  const double gamma = reduce_partials(B.P_ru + par * kRedBlocks, nblk);
+  const double rho = reduce_partials(B.P_rr + par * kRedBlocks, nblk);
+  const double delta = reduce_partials(B.P_wu, nblk_wu);
+  if (!(rho > st->tol2 * st->bb) || it >= st->maxiter) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rho;
+      st->iter = it;
+      st->done = 1;
+    }
+    return;
+  }
+  const double beta = it == 0 ? 0.0 : gamma / B.gam[1 - par];
+  const double alpha = it == 0 ? gamma / delta : gamma / (delta - beta * gamma / B.alp[1 - par]);
+  double ru = 0.0, rr = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double pi = u[i] + beta * p[i];
+    const double si = w[i] + beta * s[i];
+    p[i] = pi;
+    s[i] = si;
+    x[i] += alpha * pi;
+    const double ri = r[i] - alpha * si;
+    const double ui = dinv[i] * ri;
+    r[i] = ri;
+    u[i] = ui;
+    ru += ri * ui;
+    rr += ri * ri;
+  }
+  store_partial(B.P_ru + (1 - par) * kRedBlocks, ru);
+  store_partial(B.P_rr + (1 - par) * kRedBlocks, rr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    B.gam[par] = gamma;
+    B.alp[par] = alpha;
+    st->rr = rho;
+    st->iter = it + 1;
+  }
+}
+
+// `op(u, w, P)` must produce w = A u and the w.u partials into P (returns the
+// number of partial blocks, or a negative status); a set done flag must make
+// it a no-op.
+template <typename Op>
+static int cg1_loop(int32_t n, const double* dinv, const double* b, double* x, double tol,
+                    int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
+                    double* out_bb, hipStream_t st, Op op) {
+  if (n == 0) {
+    if (out_iters) *out_iters = 0;
+    if (out_rr) *out_rr = 0.0;
+    if (out_bb) *out_bb = 0.0;
+    return IGA_OK;
+  }
+  CG1Bufs B = cg1_bufs(work, n);
+  double* r = work + 16;
+  double* u = r + n;
+  double* p = u + n;
+  double* s = p + n;
+  double* w = s + n;
+  const int vb = vec_blocks(n);
+  hipLaunchKernelGGL(k_cg1_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, dinv, x, r, u, p, s, B);
+  hipLaunchKernelGGL(k_cg1_init_fin, dim3(1), dim3(kVecThreads), 0, st, vb, B, tol * tol, maxiter);
+  IGA_CHECK_LAUNCH();
+  CGState host{};
+  const int batch = 16;
+  int launched = 0;
+  for (;;) {
+    if (hipMemcpyAsync(&host, B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess)
+      return IGA_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+    if (host.done) break;
+    for (int k = 0; k < batch; ++k, ++launched) {
+      const int nb = op(u, w, B.P_wu, B.st);
+      if (nb < 0) return nb;
+      hipLaunchKernelGGL(k_cg1_update, dim3(vb), dim3(kVecThreads), 0, st, n, launched, nb, vb,
+                         dinv, w, u, x, r, p, s, B);
+      IGA_CHECK_LAUNCH();
+    }
+  }
+  if (out_iters) *out_iters = host.iter;
+  if (out_rr) *out_rr = host.rr;
+  if (out_bb) *out_bb = host.bb;
+  return IGA_OK;
+}
+
 
 static CGBufs cg_bufs(double* work, int32_t n) {
   CGBufs B;
@@ -426,16 +589,6 @@ static CGBufs cg_bufs(double* work, int32_t n) {
 static int spmv_blocks(int32_t n, int lpr) {
   const int64_t b = ((int64_t)n * lpr + kSpmvThreads - 1) / kSpmvThreads;
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
-}
-
-// Dot-product partials p.q for an operator applied outside the fused SpMV.
-__global__ void __launch_bounds__(kVecThreads)
-    k_cg_dot(int n, const double* __restrict__ p, const double* __restrict__ q, CGBufs B) {
-  if (B.st->done) return;
-  double s = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    s += p[i] * q[i];
-  store_partial(B.P_pq, s);
 }
 
 // Residual replacement for CG on an inexact operator: r = b - A x with the
@@ -555,32 +708,47 @@ static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, do
   return IGA_OK;
 }
 
+// one CSR operator application on the device-resident state: q = A p and
+// the p.q partials into P (returns the number of partial blocks)
+template <typename TV>
+static int csr_apply(int32_t n, int lpr, int rb, const int32_t* indptr, const int32_t* cols,
+                     const TV* vals, const double* p, double* q, double* P, CGState* state,
+                     hipStream_t st) {
+  CGBufs V{};
+  V.st = state;
+  V.P_pq = P;
+  int rc = by_lpr(lpr, [&](auto L) {
+    constexpr int LPR = decltype(L)::value;
+    hipLaunchKernelGGL((k_cg_spmv<LPR, TV>), dim3(rb), dim3(kSpmvThreads), 0, st, n, indptr, cols,
+                       vals, p, q, V);
+    IGA_CHECK_LAUNCH();
+    return (int)IGA_OK;
+  });
+  return rc ? rc : rb;
+}
+
+// `single`: the single-reduction loop (two kernels per iteration), else the
+// textbook three-kernel loop
 static int cg_solve_csr(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
                         int32_t precision, const double* dinv, const double* b, double* x,
                         double tol, int32_t maxiter, double* work, int32_t* out_iters,
-                        double* out_rr, double* out_bb, hipStream_t st) {
+                        double* out_rr, double* out_bb, hipStream_t st, bool single = false) {
   int nnz = 0;
   if (n > 0)
     if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
   const int lpr = pick_lpr(n, nnz);
   const int rb = spmv_blocks(n, lpr);
+  auto op = [&](const double* p, double* q, double* P, CGState* state) -> int {
+    if (precision == IGA_FP32)
+      return csr_apply(n, lpr, rb, indptr, cols, reinterpret_cast<const float*>(vals), p, q, P,
+                       state, st);
+    return csr_apply(n, lpr, rb, indptr, cols, reinterpret_cast<const double*>(vals), p, q, P, state,
+                     st);
+  };
+  if (single)
+    return cg1_loop(n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st, op);
   return cg_loop(n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
-                 [&](const double* p, double* q, CGBufs B) -> int {
-                   int rc = by_lpr(lpr, [&](auto L) {
-                     constexpr int LPR = decltype(L)::value;
-                     if (precision == IGA_FP32)
-                       hipLaunchKernelGGL((k_cg_spmv<LPR, float>), dim3(rb), dim3(kSpmvThreads), 0,
-                                          st, n, indptr, cols, reinterpret_cast<const float*>(vals),
-                                          p, q, B);
-                     else
-                       hipLaunchKernelGGL((k_cg_spmv<LPR, double>), dim3(rb), dim3(kSpmvThreads), 0,
-                                          st, n, indptr, cols,
-                                          reinterpret_cast<const double*>(vals), p, q, B);
-                     IGA_CHECK_LAUNCH();
-                     return (int)IGA_OK;
-                   });
-                   return rc ? rc : rb;
-                 });
+                 [&](const double* p, double* q, CGBufs B) { return op(p, q, B.P_pq, B.st); });
 }
 
 extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
@@ -593,6 +761,16 @@ extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* col
                       out_rr, out_bb, (hipStream_t)stream);
 }
 
+extern "C" int iga_cg1_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
+                             const void* vals, int32_t precision, const double* dinv,
+                             const double* b, double* x, double tol, int32_t maxiter, double* work,
+                             int32_t* out_iters, double* out_rr, double* out_bb, void* stream) {
+  if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
+  if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
+  return cg_solve_csr(n, indptr, cols, vals, precision, dinv, b, x, tol, maxiter, work, out_iters,
+                      out_rr, out_bb, (hipStream_t)stream, true);
+}
+
 // ---------------------------------------------------------------------------
 // The same Jacobi-PCG for callers that cannot allocate: its workspace (the
 // solver state, r, z, p, q and the reduction partials) is the file-scope
@@ -603,8 +781,8 @@ extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* col
 // ---------------------------------------------------------------------------
 
 constexpr int64_t kNoAllocDoubles = int64_t(1) << 21;  // 16 MiB per GPU
-static_assert(kNoAllocDoubles >= 8 + 4 * int64_t(IGA_CG_NOALLOC_MAX_N) + 5 * kRedBlocks,
-              "scratch too small for IGA_CG_NOALLOC_MAX_N");
+static_assert(kNoAllocDoubles >= 16 + 5 * int64_t(IGA_CG_NOALLOC_MAX_N) + 6 * kRedBlocks,
+              "scratch too small for IGA_CG_NOALLOC_MAX_N (see iga_cg_workspace)");
 
 static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
@@ -695,24 +873,21 @@ extern "C" int iga_cg_solve_mixed(int32_t n, const int32_t* indptr, const int32_
       rep);
 }
 
-extern "C" int iga_cg_solve_mf(const iga_patch_t* patch, int32_t precision, int32_t rational,
-                               const void* mf_work, int64_t mf_work_bytes, int32_t n_free,
+extern "C" int iga_cg_solve_mf(const iga_patch_t* patch, const void* mf_work,
+                               int64_t mf_work_bytes, int32_t n_free,
                                const int32_t* freed, const int32_t* full_to_free,
                                const double* dinv, const double* b, double* x, double tol,
                                int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
                                double* out_bb, void* stream) {
   if (!patch || !mf_work || !dinv || !b || !x || !work) return IGA_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  const int vb = vec_blocks(n_free);
+  // textbook loop; the p.q partials come out of the operator's gather kernel
   return cg_loop(n_free, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
                  [&](const double* p, double* q, CGBufs B) -> int {
-                   int rc = mf_apply_launch(patch, precision, rational, mf_work, mf_work_bytes,
-                                            n_free, freed, full_to_free, p, q, &B.st->done, st);
-                   if (rc) return rc;
-                   hipLaunchKernelGGL(k_cg_dot, dim3(vb), dim3(kVecThreads), 0, st, n_free, p, q,
-                                      B);
-                   IGA_CHECK_LAUNCH();
-                   return vb;
+                   int nb = 0;
+                   int rc = mf_apply_launch(patch, mf_work, mf_work_bytes, n_free, freed,
+                                            full_to_free, p, q, &B.st->done, B.P_pq, &nb, st);
+                   return rc ? rc : nb;
                  });
 }
 

@@ -177,8 +177,11 @@ class CGResult:
     bb: float
 
 
-def cg_csr(pattern: CSRPattern, data, b, tol=1e-10, maxiter=None, dinv=None, workspace=None):
-    """Jacobi-PCG on the device; returns :class:`CGResult`."""
+def cg_csr(pattern: CSRPattern, data, b, tol=1e-10, maxiter=None, dinv=None, workspace=None,
+           single_reduction=False):
+    """Jacobi-PCG on the device; returns :class:`CGResult`. ``single_reduction``
+    runs the Chronopoulos-Gear form (two kernels per iteration, the same
+    iterates in exact arithmetic) instead of the textbook recurrence."""
     import torch
 
     L = lib()
@@ -193,14 +196,15 @@ def cg_csr(pattern: CSRPattern, data, b, tol=1e-10, maxiter=None, dinv=None, wor
     bb = _lib.c_double(0.0)
     maxiter = 10 * n if maxiter is None else int(maxiter)
     prec = _lib.IGA_FP32 if data.dtype == torch.float32 else _lib.IGA_FP64
-    _lib.check(L.iga_cg_solve(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d), _lib.ptr(data),
-                              prec, _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), float(tol), maxiter,
-                              _lib.ptr(workspace), _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
-                              _lib.ctypes.byref(bb), stream()), "iga_cg_solve")
+    fn = L.iga_cg1_solve if single_reduction else L.iga_cg_solve
+    _lib.check(fn(n, _lib.ptr(pattern.indptr_d), _lib.ptr(pattern.cols_d), _lib.ptr(data),
+                  prec, _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), float(tol), maxiter,
+                  _lib.ptr(workspace), _lib.ctypes.byref(it), _lib.ctypes.byref(rr),
+                  _lib.ctypes.byref(bb), stream()), "iga_cg_solve")
     return CGResult(x, it.value, rr.value, bb.value)
 
 
-NOALLOC_MAX_N = 500000  # IGA_CG_NOALLOC_MAX_N of the C ABI
+NOALLOC_MAX_N = 400000  # IGA_CG_NOALLOC_MAX_N of the C ABI
 
 
 def cg_csr_noalloc(pattern: CSRPattern, data, b, x, dinv, tol=1e-10, maxiter=None):

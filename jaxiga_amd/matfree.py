@@ -22,8 +22,33 @@ from jaxiga_amd import linear as LIN
 from jaxiga_amd.runtime import lib, stream
 
 
+def separable_map(space, rtol=1e-14):
+    """True when the patch map is a tensor product of 1-D maps: control-point
+    coordinate ``c`` depends only on the ``c``-th index (every refined box),
+    to a relative spread of ``rtol`` (knot insertion leaves rounding-level
+    differences)."""
+    dim = space.dim
+    if space.dim_phys != dim:
+        return False
+    n = tuple(space.patch.n_cp_per_dir)
+    X = np.asarray(space.cpts, dtype=np.float64).reshape(*n[::-1], dim)
+    scale = max(float(np.abs(X).max()), 1e-300)
+    for c in range(dim):
+        # index c is axis dim-1-c of the (i_{D-1}, .., i_0) layout
+        Xc = np.moveaxis(X[..., c], dim - 1 - c, -1).reshape(-1, n[c])
+        if np.abs(Xc - Xc[:1]).max() > rtol * scale:
+            return False
+    return True
+
+
 class MatrixFreeOperator:
-    """``y = K x`` on the free dofs of a scalar Poisson problem."""
+    """``y = K x`` on the free dofs of a scalar Poisson problem.
+
+    ``precision``: ``IGA_FP64`` or ``IGA_FP32`` arithmetic, or ``IGA_BF16``
+    (coefficient stored in BF16, FP32 arithmetic; separable patches only).
+    The geometry is always built in FP64. On a separable polynomial patch
+    (:func:`separable_map`) only the coefficient is stored per Gauss point
+    and the Jacobi diagonal is sum-factorised (:meth:`diagonal`)."""
 
     def __init__(self, ctx, params=None, precision=_lib.IGA_FP64, inputs=None):
         """``inputs`` (optional): the values of :func:`jaxiga_amd.methods.point_inputs`;
@@ -45,9 +70,14 @@ class MatrixFreeOperator:
         # the geometry factors always use the patch's own weights.
         w = np.asarray(space.wgts, dtype=np.float64)
         self.rational = int(w.size > 0 and np.ptp(w) > 1e-14 * np.abs(w).max())
+        self.separable = int(not self.rational and separable_map(space))
+        self.flags = (_lib.IGA_MF_RATIONAL if self.rational else 0) | \
+            (_lib.IGA_MF_SEPARABLE if self.separable else 0)
+        if precision == _lib.IGA_BF16 and not self.separable:
+            raise NotImplementedError("the BF16 matrix-free operator needs a separable polynomial patch")
         L = lib()
         st = space.device().struct
-        nbytes = int(L.iga_mf_workspace(st, precision, self.rational))
+        nbytes = int(L.iga_mf_workspace(st, precision, self.flags))
         if nbytes < 0:
             raise NotImplementedError(f"no matrix-free kernel for dim={space.dim}, degree={space.degree}")
         self.work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=ctx.f2f_d.device)
@@ -55,7 +85,7 @@ class MatrixFreeOperator:
         args = _lib.IgaAsmArgs()
         keep = (_bind_coefficient(args, inputs["coef"]) if inputs is not None
                 else _coefficient_args(ctx, params, args))
-        _lib.check(L.iga_mf_setup(st, ctx.rule(), precision, self.rational, args.coef_kind,
+        _lib.check(L.iga_mf_setup(st, ctx.rule(), precision, self.flags, args.coef_kind,
                                   args.coef_const, args.coef_points, _lib.ptr(self.work), nbytes,
                                   stream()), "iga_mf_setup")
         del keep
@@ -66,13 +96,27 @@ class MatrixFreeOperator:
 
         y = torch.empty_like(x) if out is None else out
         ctx = self.ctx
-        _lib.check(lib().iga_mf_apply(ctx.space.device().struct, self.precision, self.rational,
-                                      _lib.ptr(self.work), self.nbytes, self.n, _lib.ptr(ctx.free_d),
-                                      _lib.ptr(ctx.f2f_d), _lib.ptr(x), _lib.ptr(y), stream()),
-                   "iga_mf_apply")
+        _lib.check(lib().iga_mf_apply(ctx.space.device().struct, _lib.ptr(self.work), self.nbytes,
+                                      self.n, _lib.ptr(ctx.free_d), _lib.ptr(ctx.f2f_d), _lib.ptr(x),
+                                      _lib.ptr(y), stream()), "iga_mf_apply")
         return y
 
     __call__ = apply
+
+    def diagonal(self):
+        """``diag(K)`` on the free dofs (FP64), sum-factorised; separable
+        patches only (elsewhere the element pass gives it:
+        :func:`jaxiga_amd.methods.tangent_diagonal`)."""
+        import torch
+
+        if not self.separable:
+            raise NotImplementedError("the sum-factorised diagonal needs a separable polynomial patch")
+        ctx = self.ctx
+        d = torch.empty(self.n, dtype=torch.float64, device=ctx.f2f_d.device)
+        _lib.check(lib().iga_mf_diag(ctx.space.device().struct, _lib.ptr(self.work), self.nbytes,
+                                     self.n, _lib.ptr(ctx.free_d), _lib.ptr(ctx.f2f_d), _lib.ptr(d),
+                                     stream()), "iga_mf_diag")
+        return d
 
     def cg(self, b, tol=1e-10, maxiter=None, dinv=None, workspace=None):
         import torch
@@ -87,12 +131,11 @@ class MatrixFreeOperator:
         it, rr, bb = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0)
         maxiter = 10 * n if maxiter is None else int(maxiter)
         ctx = self.ctx
-        _lib.check(L.iga_cg_solve_mf(ctx.space.device().struct, self.precision, self.rational,
-                                     _lib.ptr(self.work), self.nbytes, n, _lib.ptr(ctx.free_d),
-                                     _lib.ptr(ctx.f2f_d), _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x),
-                                     float(tol), maxiter, _lib.ptr(workspace), _lib.ctypes.byref(it),
-                                     _lib.ctypes.byref(rr), _lib.ctypes.byref(bb), stream()),
-                   "iga_cg_solve_mf")
+        _lib.check(L.iga_cg_solve_mf(ctx.space.device().struct, _lib.ptr(self.work), self.nbytes, n,
+                                     _lib.ptr(ctx.free_d), _lib.ptr(ctx.f2f_d), _lib.ptr(dinv),
+                                     _lib.ptr(b), _lib.ptr(x), float(tol), maxiter, _lib.ptr(workspace),
+                                     _lib.ctypes.byref(it), _lib.ctypes.byref(rr), _lib.ctypes.byref(bb),
+                                     stream()), "iga_cg_solve_mf")
         return LIN.CGResult(x, it.value, rr.value, bb.value)
 
 
@@ -129,9 +172,14 @@ def matrix_free_solve_ctx(ctx, params, linear, precision=_lib.IGA_FP64):
     if linear.method != "cg":
         raise NotImplementedError(f"matrix-free method {linear.method!r} is not provided; use 'cg'")
     t0 = time.perf_counter()
-    _, b, diag = assemble(ctx, params, values=False, load=True,
-                          diagonal=linear.preconditioner == "jacobi")
+    jacobi = linear.preconditioner == "jacobi"
     op = MatrixFreeOperator(ctx, params, precision)
+    # the Jacobi diagonal: sum-factorised on separable patches, else from the
+    # element pass that also forms the load
+    _, b, diag = assemble(ctx, params, values=False, load=True,
+                          diagonal=jacobi and not op.separable)
+    if jacobi and op.separable:
+        diag = op.diagonal()
     dinv = inverse_guard(diag) if diag is not None else None
     LIN.SOLVE_COUNTER["forward"] += 1
     res = op.cg(b, linear.tol, linear.maxiter, dinv=dinv)

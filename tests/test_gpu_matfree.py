@@ -193,3 +193,105 @@ def test_matrix_free_3d_rational_weights_and_coefficient(gpu):
         err = np.abs(y - ref_y).max() / np.abs(ref_y).max()
         print(f"[mf-rational] precision {prec}: K v rel err {err:.2e}", flush=True)
         assert err < tol
+
+
+def _bf16_round(a):
+    """FP64 -> FP32 (nearest) -> BF16 (nearest even), as FP64 values."""
+    u = np.asarray(a, np.float64).astype(np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _grf_cube(ref, seed=20260901):
+    from jaxiga_amd import grf
+    from oracle import forms as OF
+
+    case = C.poisson_case("cube", 2, ref, source=lambda x, p: 1.0 + 0 * x[0])
+    ctx = M.build_context(case.problem)
+    a = grf.gauss_point_field(ctx, seed, 0.1, 1.0).exp_()
+    case.params = {"a0": OF.PointField(a.cpu().numpy())}
+    return case, ctx, a
+
+
+def test_bf16_rounding_helper_is_round_to_nearest_even():
+    """The test's own BF16 statement against values whose rounding is known."""
+    ulp = 2.0 ** -7  # BF16 spacing in [1, 2)
+    vals = np.array([1.0, 1.0 + ulp / 2, 1.0 + 1.5 * ulp, 1.0 + ulp / 2 + 2 ** -20, -3.0])
+    np.testing.assert_array_equal(_bf16_round(vals), [1.0, 1.0, 1.0 + 2 * ulp, 1.0 + ulp, -3.0])
+
+
+@pytest.mark.parametrize("geom,deg,ref", [("cube", 2, 3), ("square", 3, 3), ("cube", 1, 3)])
+def test_separable_patches_take_the_one_value_per_point_apply(gpu, geom, deg, ref):
+    """Boxes are tensor products of 1-D maps: the operator stores one value per
+    Gauss point, and its sum-factorised diagonal equals the oracle's
+    assembled diagonal."""
+    src = (lambda x, p: 1.0 + 0 * x[0]) if geom == "cube" else C.sin_source
+    a0 = (lambda x, p: 1.0 + x[0] + 2.0 * x[1] * x[-1])
+    case = C.poisson_case(geom, deg, ref, source=src, a0=a0)
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    ctx = M.build_context(case.problem, case.params)
+    op = MF.MatrixFreeOperator(ctx, case.params)
+    assert op.separable == 1 and op.rational == 0
+    v = np.random.default_rng(7).normal(size=K.shape[0])
+    y = op.apply(torch.as_tensor(v, device="cuda")).cpu().numpy()
+    assert np.abs(y - K @ v).max() / np.abs(K @ v).max() < 1e-12
+    d = op.diagonal().cpu().numpy()
+    np.testing.assert_allclose(d, K.diagonal(), rtol=1e-12)
+    d32 = MF.MatrixFreeOperator(ctx, case.params, precision=_lib.IGA_FP32).diagonal().cpu().numpy()
+    np.testing.assert_allclose(d32, K.diagonal(), rtol=5e-6)
+
+
+def test_bf16_coefficient_apply_and_refinement(gpu):
+    """BF16 storage of the GRF coefficient (FP32 arithmetic): K v equals the
+    oracle's operator assembled on the BF16-rounded coefficient to FP32
+    accuracy, differs from the exact one by the BF16 rounding only, and FP64
+    iterative refinement on the FP64 operator reaches 1e-10."""
+    import scipy.sparse.linalg as spla
+
+    from jaxiga_amd import precision as PR
+    from oracle import forms as OF
+
+    case, ctx, a = _grf_cube(3)
+    params = {"a0": jx.PointField(a)}
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    a16 = _bf16_round(case.params["a0"].values)
+    case16 = C.Case(case.name, case.kind, case.space, case.ospace, case.problem, case.odirichlet,
+                    case.oneumann, case.source, {"a0": OF.PointField(a16)})
+    _, pat16, data16, _ = _oracle_system(case16)
+    K16 = OA.to_scipy(pat16, data16)
+    v = np.random.default_rng(9).normal(size=K.shape[0])
+    op16 = MF.MatrixFreeOperator(ctx, params, precision=_lib.IGA_BF16)
+    y16 = op16.apply(torch.as_tensor(v, device="cuda")).cpu().numpy()
+    scale = np.abs(K16 @ v).max()
+    err16 = np.abs(y16 - K16 @ v).max() / scale
+    err_exact = np.abs(y16 - K @ v).max() / scale
+    print(f"[mf-bf16] vs rounded-coefficient operator {err16:.2e}, vs exact {err_exact:.2e}", flush=True)
+    assert err16 < 5e-6
+    assert 1e-5 < err_exact < 1e-2
+    op64 = MF.MatrixFreeOperator(ctx, params)
+    dinv = MF.inverse_guard(op64.diagonal())
+    rhs = torch.as_tensor(b, device="cuda")
+    res = PR.mf_mixed_solve(op64, op16, rhs, dinv, tol=1e-10, inner_tol=1e-3)
+    x = res.x.cpu().numpy()
+    assert np.linalg.norm(b - K @ x) <= 1e-10 * np.linalg.norm(b) * 1.01
+    x_ref = spla.spsolve(K.tocsc(), b)
+    assert np.abs(x - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+
+
+def test_apply_rejects_a_workspace_that_was_not_set_up(gpu):
+    case = C.poisson_case("cube", 2, 2, source=lambda x, p: 1.0 + 0 * x[0])
+    ctx = M.build_context(case.problem, case.params)
+    op = MF.MatrixFreeOperator(ctx, case.params)
+    x = torch.zeros(ctx.n_free, dtype=torch.float64, device="cuda")
+    from jaxiga_amd.runtime import lib, stream
+
+    # an address no workspace starts at (allocations are 512-byte aligned)
+    other = op.work.data_ptr() + 8
+    rc = lib().iga_mf_apply(ctx.space.device().struct, other, op.nbytes - 8, ctx.n_free,
+                            _lib.ptr(ctx.free_d), _lib.ptr(ctx.f2f_d), _lib.ptr(x), _lib.ptr(x), stream())
+    assert rc == _lib.IGA_ERR_ARG
+    rc = lib().iga_mf_apply(ctx.space.device().struct, _lib.ptr(op.work), op.nbytes - 256, ctx.n_free,
+                            _lib.ptr(ctx.free_d), _lib.ptr(ctx.f2f_d), _lib.ptr(x), _lib.ptr(x), stream())
+    assert rc == _lib.IGA_ERR_SHAPE

@@ -153,6 +153,26 @@ def test_cg_matches_oracle(gpu, make):
     assert rn <= 1.01e-10 * bn
 
 
+@pytest.mark.parametrize("make", SYSTEM_CASES[:3] + SYSTEM_CASES[5:6])
+def test_single_reduction_cg_matches_oracle(gpu, make):
+    """The Chronopoulos-Gear form: the textbook iterates in exact arithmetic;
+    in FP64 its recurrences round differently, so the count may move by two
+    on the ill-conditioned elasticity case; the solution and the independent
+    FP64 residual hold the same bars."""
+    case = make()
+    print(f"[cg1] {case.name}", flush=True)
+    dm, pat, data, b = _oracle_system(case)
+    K = OA.to_scipy(pat, data)
+    u_ref, it_ref = OSV.jacobi_pcg(lambda v: K @ v, b, K.diagonal(), tol=1e-10)
+    ctx = M.build_context(case.problem, case.params)
+    vals, rhs, _ = M.assemble(ctx, case.params)
+    res = jx.linear.cg_csr(ctx.assembly, vals, rhs, tol=1e-10, single_reduction=True)
+    assert abs(res.iterations - it_ref) <= 2, (res.iterations, it_ref)
+    assert rel(res.x.cpu().numpy(), u_ref) < 1e-8
+    rn, bn = jx.linear.residual_norms(ctx.assembly, vals, res.x, rhs)
+    assert rn <= 1.05e-10 * bn
+
+
 def test_truncated_cg_reports_non_convergence(gpu):
     case = C.poisson_case("square", 3, 3)
     with pytest.warns(RuntimeWarning, match="did not converge"):
