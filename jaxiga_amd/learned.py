@@ -35,6 +35,7 @@ from __future__ import annotations
 import dataclasses
 import math
 import time
+import weakref
 
 import numpy as np
 
@@ -315,6 +316,51 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     return op
 
 
+class CapturedSetup:
+    """The per-operator set-up of :func:`setup` (Galerkin levels, dense tail,
+    generator inference on its side stream) captured once in a HIP graph over
+    a persistent copy of the level-0 values: a new operator is one copy into
+    that buffer and one graph launch, with no host work per kernel."""
+
+    def __init__(self, ctx, gen, levels):
+        import torch
+
+        self.gen_ref = weakref.ref(gen)
+        self.vin = torch.empty(ctx.assembly.nnz, dtype=torch.float64, device=ctx.f2f_d.device)
+        self.vin.zero_()
+        # one eager run first: lazy buffers stay out of the graph
+        self.op = setup(ctx, self.vin, gen, levels)
+        self.stencils = list(self.op.stencils)  # the persistent buffers the graph writes
+        torch.cuda.synchronize()
+        cap = side_stream(self.vin.device, "setup-capture")
+        cap.wait_stream(torch.cuda.current_stream())
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=cap):
+            setup(ctx, self.vin, gen, levels)
+        torch.cuda.synchronize()
+
+    def run(self, vals64):
+        self.vin.copy_(vals64)
+        self.graph.replay()
+        # the operator may have been set up since without these smoothers
+        # (another generator, or none): point it at the graph's buffers again
+        self.op.set_stencils(self.stencils)
+        return self.op
+
+
+def captured_setup(ctx, vals64, gen: ConvNet, levels=LEARNED_LEVELS):
+    """:func:`setup` through the operator's captured set-up graph (captured
+    at first use per generator and level count)."""
+    op = MG.operator(MG.hierarchy(ctx), omega=OMEGA_COARSE, ks=KS)
+    if gen is None or not op.split_load():
+        return setup(ctx, vals64, gen, levels)
+    graphs = op.__dict__.setdefault("_setup_graphs", {})
+    cs = graphs.get(levels)
+    if cs is None or cs.gen_ref() is not gen:
+        cs = graphs[levels] = CapturedSetup(ctx, gen, levels)
+    return cs.run(vals64)
+
+
 _DEFAULT = {"generator": None}
 
 
@@ -349,9 +395,10 @@ class SolveResult:
 
 
 def solve(ctx, vals64, b, gen: ConvNet = None, tol=1e-10, maxiter=1000, graph_iters=8,
-          levels=LEARNED_LEVELS):
+          levels=LEARNED_LEVELS, captured=True):
     """Flexible CG with the learned multigrid preconditioner; Jacobi-CG is the
-    fallback and the independent FP64 residual the judge."""
+    fallback and the independent FP64 residual the judge. ``captured``: the
+    set-up runs from its captured graph (:func:`captured_setup`)."""
     import torch
 
     from jaxiga_amd import precision as PR
@@ -360,7 +407,7 @@ def solve(ctx, vals64, b, gen: ConvNet = None, tol=1e-10, maxiter=1000, graph_it
     t0 = time.perf_counter()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
-    op = setup(ctx, vals64, gen, levels)
+    op = captured_setup(ctx, vals64, gen, levels) if captured else setup(ctx, vals64, gen, levels)
     ev[1].record()
     # the plan recomputes ||b - K x|| in FP64 from x after the solve
     x, it, rr, bb, rn = op.fcg(b, tol, maxiter, graph_iters, true_residual=True)

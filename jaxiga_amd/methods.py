@@ -75,7 +75,7 @@ class MethodContext:
 
 
 def csr_pattern(space, dofmap, free_d, f2f_d):
-    key = (space.vec, dofmap.free.tobytes())
+    key = (space.vec, dofmap.key)
     pat = space._patterns.get(key)
     if pat is None:
         pat = LIN.build_pattern(space, free_d, f2f_d, len(dofmap.free))

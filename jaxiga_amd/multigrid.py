@@ -20,8 +20,6 @@ dense inverse.
 from __future__ import annotations
 
 import dataclasses
-import hashlib
-import weakref
 
 import numpy as np
 import scipy.sparse as sp
@@ -343,7 +341,6 @@ def tensor_coupling_ok(c_lo, pw, f_lo, rw):
     return True
 
 
-_FREE_KEYS = weakref.WeakKeyDictionary()
 
 # levels at or below this size take the dense tail (a 289-dof level: 0.3 MB
 # FP32 with padded rows, one product per cycle instead of about a dozen
@@ -356,13 +353,9 @@ DENSE_TAIL_COARSE, DENSE_TAIL_NEXT = 48, 128
 
 
 def _free_key(dofmap):
-    """Digest of a dof map's free set, computed once per dof map: contexts of
-    one space with the same free set share one hierarchy."""
-    k = _FREE_KEYS.get(dofmap)
-    if k is None:
-        k = hashlib.sha1(np.ascontiguousarray(dofmap.free).tobytes()).digest()
-        _FREE_KEYS[dofmap] = k
-    return k
+    """Digest of a dof map's free set: contexts of one space with the same
+    free set share one hierarchy."""
+    return dofmap.key
 
 
 def hierarchy(ctx, max_coarse=64):

@@ -27,6 +27,8 @@ on span ``e``.
 from __future__ import annotations
 
 import dataclasses
+import functools
+import hashlib
 import weakref
 
 import numpy as np
@@ -320,6 +322,15 @@ class DofMap:
     @property
     def n_free(self) -> int:
         return len(self.free)
+
+    @functools.cached_property
+    def key(self) -> bytes:
+        """Digest of the free set, from the (short) prescribed list: the free
+        dofs are the sorted complement of the prescribed ones in
+        ``range(n_dofs)``, so this identifies the free set without hashing it."""
+        h = hashlib.sha1(np.int64(self.n_dofs).tobytes())
+        h.update(np.ascontiguousarray(self.prescribed, dtype=np.int64).tobytes())
+        return h.digest()
 
     def lift(self, u_free):
         import torch

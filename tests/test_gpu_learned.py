@@ -236,6 +236,23 @@ def test_training_gains_over_untrained_smoothers(gpu):
     assert after.iterations < jac.iterations
     ref = jac.x.cpu().numpy()
     assert np.abs(after.x.cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+    # the captured set-up graph reproduces the eager set-up bitwise, replay
+    # after replay, with other operators and eager work in between
+    for seed in (99, 100, 99, 101):
+        c2, v2, r2 = _problem(6, seed)
+        LN.solve(c2, v2, r2, None)  # the untrained set-up of the same operator in between
+        op = LN.captured_setup(c2, v2, gen)
+        snap = [t.clone() for t in (op.tail, *op.vals32, *op.stencils) if t is not None]
+        _ = torch.ones(1000, device="cuda").sum().item()
+        op = LN.setup(c2, v2, gen)
+        eager = [t for t in (op.tail, *op.vals32, *op.stencils) if t is not None]
+        assert len(eager) == len(snap)
+        assert all(torch.equal(a, b) for a, b in zip(snap, eager)), seed
+        eag = LN.solve(c2, v2, r2, gen, captured=False)
+        LN.solve(c2, v2, r2, None)
+        cap = LN.solve(c2, v2, r2, gen, captured=True)
+        assert cap.iterations == eag.iterations and torch.equal(cap.x, eag.x)
+        assert cap.iterations < LN.solve(c2, v2, r2, None).iterations
 
 
 def test_drop_in_solve_with_multigrid_preconditioner_and_its_gradient(gpu):
