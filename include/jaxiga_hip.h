@@ -36,6 +36,8 @@
  *   iga_cg_solve_mixed        solvers/linear.py:324-351    (the same Jacobi-PCG, on an
  *                             FP32 matrix with FP64 residual replacement; the
  *                             reference runs FP64 only)
+ *   iga_point_forces          methods/galerkin.py:37-148 under jax.grad with
+ *                             respect to the control points (shape cotangent)
  *   iga_net_* / iga_nmse / iga_adam / iga_mg_* / iga_masked_spgemm /
  *   iga_stencil_* / iga_dense_inverse / iga_csr_f32 / iga_energy_loss / iga_fcg_mg
  *                             (no reference counterpart: the learned preconditioner,
@@ -322,6 +324,20 @@ int iga_csr_outer(int64_t nnz, const int32_t* rows, const int32_t* cols, const d
 int iga_field_points(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, int32_t vec,
                      const double* R, const double* dR, const int32_t* dofs, const double* u0,
                      const double* u1, double* val, double* grad, void* stream);
+
+/* Shape cotangent of the solve (replaces the reverse pass of JAX autodiff
+ * through the element kernels to the control points, exercised by the
+ * reference's tests/test_galerkin.py:233-262): point quantities pulled back
+ * to the control points,
+ *   G[dofs[e][k]][c] += sum_q ( sum_j P[e][q][c][j] dR[e][q][j][k]
+ *                               + R[e][q][k] xbar[e][q][c] )
+ * with R (n_e,n_q,n_l), dR (n_e,n_q,dim,n_l) as iga_basis_eval writes them,
+ * P (n_e,n_q,dim,dim), xbar (n_e,n_q,dim) or null (R may then be null),
+ * dofs (n_e,n_l) control-point indices, G (n_cp,dim) accumulated (FP64
+ * atomics: the summation order is not fixed).                              */
+int iga_point_forces(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, const double* R,
+                     const double* dR, const double* P, const double* xbar, const int32_t* dofs,
+                     double* G, void* stream);
 
 /* Learned networks on the control-point grid: 3x3 convolutions, float32 NCHW.
  * kind[l]   : 0 forward convolution (weights (cout, cin, 3, 3)),

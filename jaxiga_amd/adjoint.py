@@ -27,8 +27,13 @@ carries the lifting ``-K_fp g``) and ``lam`` zero on prescribed dofs:
 pass therefore costs exactly one adjoint solve (``SOLVE_COUNTER['adjoint']``),
 whatever the forward iteration did, and torch carries the point cotangents
 back to the parameters through the user's callables. Dirichlet values are
-resolved at set-up and are not differentiated (as in the reference); nor are
-control points or weights (geometry enters the kernels as fixed data). With
+resolved at set-up and are not differentiated (as in the reference).
+
+Control points (a space from ``FunctionSpace.replace(cpts=tensor)``) get the
+shape cotangent :meth:`_Run.shape_cotangent` from the same adjoint field:
+the first variation of the point functional under a motion of the map,
+pulled back to the control points by ``iga_point_forces``. Weights are not
+differentiated. With
 ``matrix_free=True`` both solves are the matrix-free CG (the symmetric
 operator is its own transpose, ``src/jaxiga/solvers/linear.py:436-442``).
 """
@@ -160,8 +165,9 @@ class _Run:
         self._adjoint = adjoint
         return res.x, {"iterations": res.iterations}, rn, bn
 
-    def backward(self, u_bar, needs):
-        """Cotangents of ``(coef, src, C, *tractions)`` (None where not needed)."""
+    def backward(self, u_bar, needs, shape=False):
+        """Cotangents of ``(coef, src, C, *tractions)`` (None where not needed),
+        and of the control points (``shape``) as an extra last item."""
         import torch
 
         ctx = self.ctx
@@ -203,7 +209,101 @@ class _Run:
             basis = evaluate(space, ps, outputs=("R", "x", "w"))
             v, _ = field_points(basis, lam, vec=space.vec, gradients=False)
             out[3 + k] = basis.w[..., None] * v[0]
+        if shape:
+            out.append(self.shape_cotangent(lam))
         return out
+
+    def shape_cotangent(self, lam):
+        """``d/dX [lam^T (F - K u)]`` over the control points ``(n_cp, dim)``.
+        Under a motion ``dX_k`` of control point k the Gauss points move by
+        ``R_k dX_k``, so ``dw = w div V`` and ``d grad(phi) = -grad(V)^T
+        grad(phi)``; the point functional ``w (f.lam - eps(lam):sigma(u))``
+        then varies by ``P : grad(R_k) + R_k xbar``,
+        ``P_cj = w [(f.lam - eps(lam):sigma(u)) delta_cj + sum_i (d_c lam_i
+        sigma_ij(u) + d_c u_i sigma_ij(lam))]`` (``sigma = a grad u`` for
+        Poisson), ``xbar`` the derivative through the explicit ``x`` of
+        callable coefficients and sources (torch autograd of the callables).
+        The pull-back to the control points is ``iga_point_forces``."""
+        import torch
+
+        ctx, space = self.ctx, self.ctx.space
+        dim, vec = space.dim, space.vec
+        basis = evaluate(space, gauss(space, ctx.n_q1), outputs=("R", "dR", "x", "w"))
+        val, g = field_points(basis, self.u, lam, vec)
+        w = basis.w
+        gu, gl, lq = g[0], g[1], val[1]  # (n_e, n_q, vec, dim), (n_e, n_q, vec)
+        src = self.inputs["src"]
+        if src is None:
+            fl = torch.zeros_like(w)
+        else:
+            f = torch.as_tensor(src, dtype=torch.float64, device=w.device).detach()
+            fl = (f * lq).sum(-1)
+        eye = torch.eye(dim, dtype=torch.float64, device=w.device)
+        if ctx.problem.kernel == "poisson":
+            coef = self.inputs["coef"]
+            a = torch.ones_like(w) if coef is None else torch.as_tensor(coef, dtype=torch.float64,
+                                                                          device=w.device).detach()
+            a = a.expand_as(w) if a.dim() == 0 else a
+            u1, l1 = gu[:, :, 0, :], gl[:, :, 0, :]
+            inner = (u1 * l1).sum(-1)
+            P = (fl - a * inner)[..., None, None] * eye
+            P = P + a[..., None, None] * (l1[..., :, None] * u1[..., None, :] + u1[..., :, None] * l1[..., None, :])
+        else:
+            C = torch.as_tensor(self.inputs["C"], dtype=torch.float64).detach().to(w.device)
+
+            def stress(gr):
+                sv = voigt(gr) @ C.T
+                if dim == 2:
+                    rows = [[sv[..., 0], sv[..., 2]], [sv[..., 2], sv[..., 1]]]
+                else:
+                    rows = [[sv[..., 0], sv[..., 5], sv[..., 4]], [sv[..., 5], sv[..., 1], sv[..., 3]],
+                            [sv[..., 4], sv[..., 3], sv[..., 2]]]
+                return torch.stack([torch.stack(r, -1) for r in rows], -2)
+
+            su, sl = stress(gu), stress(gl)
+            energy = (voigt(gl) * (voigt(gu) @ C.T)).sum(-1)
+            P = (fl - energy)[..., None, None] * eye
+            P = P + torch.einsum("eqic,eqij->eqcj", gl, su) + torch.einsum("eqic,eqij->eqcj", gu, sl)
+        P = (P * w[..., None, None]).contiguous()
+        xbar = self._explicit_x(basis, lq, gu, gl)
+        G = torch.zeros((space.n_scalar_basis, dim), dtype=torch.float64, device=w.device)
+        dofs = basis.__dict__.get("_dofs_dev")
+        _lib.check(lib().iga_point_forces(w.shape[0], w.shape[1], basis.dR.shape[-1], dim,
+                                          _lib.ptr(basis.R), _lib.ptr(basis.dR), _lib.ptr(P),
+                                          _lib.ptr(xbar), _lib.ptr(dofs), _lib.ptr(G), stream()),
+                   "iga_point_forces")
+        return G
+
+    def _explicit_x(self, basis, lq, gu, gl):
+        """``w d/dx (f(x).lam - a(x) grad(lam).grad(u))`` at the points through
+        the callables' own dependence on x (None when nothing depends on x)."""
+        import torch
+
+        from jaxiga_amd.methods import call_points
+
+        ctx, params = self.ctx, self.params
+        problem = ctx.problem
+        a0 = params.get("a0") if isinstance(params, dict) else None
+        from jaxiga_amd.forms import PointField
+
+        a_fn = a0 if (problem.kernel == "poisson" and callable(a0) and not isinstance(a0, PointField)) else None
+        f_fn = problem.source if callable(problem.source) else None
+        if a_fn is None and f_fn is None:
+            return None
+        x = basis.x.detach().clone().requires_grad_(True)
+        with torch.enable_grad():
+            S = torch.zeros((), dtype=torch.float64, device=x.device)
+            if f_fn is not None:
+                f = call_points(f_fn, x, params=params, vec=ctx.space.vec)
+                S = S + (basis.w[..., None] * f * lq).sum()
+            if a_fn is not None:
+                a = call_points(a_fn, x, params=params)[..., 0]
+                inner = (gu[:, :, 0, :] * gl[:, :, 0, :]).sum(-1)
+                S = S - (basis.w * a * inner).sum()
+            if not S.requires_grad:
+                return None
+            (xbar,) = torch.autograd.grad(S, x)
+        return xbar.contiguous()
 
 
 def _galerkin_fn():
@@ -211,14 +311,21 @@ def _galerkin_fn():
 
     class GalerkinSolve(torch.autograd.Function):
         @staticmethod
-        def forward(fc, run, coef, src, C, *tractions):
+        def forward(fc, run, cpts, coef, src, C, *tractions):
             fc.run = run
+            fc.cpts_meta = None if cpts is None else (cpts.shape, cpts.dtype, cpts.device)
             return run.forward()
 
         @staticmethod
         def backward(fc, u_bar):
-            grads = fc.run.backward(u_bar.detach().to(torch.float64).contiguous(), fc.needs_input_grad[1:])
-            return (None, *grads)
+            shape = bool(fc.needs_input_grad[1])
+            grads = fc.run.backward(u_bar.detach().to(torch.float64).contiguous(), fc.needs_input_grad[2:],
+                                    shape=shape)
+            g_cpts = None
+            if shape:
+                shp, dt, dev = fc.cpts_meta
+                g_cpts = grads.pop().to(device=dev, dtype=dt).reshape(shp)
+            return (None, g_cpts, *grads)
 
     return GalerkinSolve
 
@@ -239,6 +346,6 @@ def galerkin_solve(ctx, params, linear, inputs):
         return v if isinstance(v, torch.Tensor) and v.requires_grad else None
 
     tractions = inputs["tractions"] or []
-    u = _FN[0].apply(run, as_arg(inputs["coef"]), as_arg(inputs["src"]), as_arg(inputs["C"]),
-                     *[as_arg(t) for t in tractions])
+    u = _FN[0].apply(run, as_arg(getattr(ctx.space, "_cpts_t", None)), as_arg(inputs["coef"]),
+                     as_arg(inputs["src"]), as_arg(inputs["C"]), *[as_arg(t) for t in tractions])
     return u, run.stats

@@ -261,3 +261,71 @@ extern "C" int iga_field_points(int32_t n_e, int32_t n_q, int32_t n_l, int32_t d
   if (dim == 3 && vec == 3) return field_points_nf<3, 3>(nf, n_p, n_q, n_l, Ru, dRu, dofs, u0, u1, val, grad, st);
   return IGA_ERR_UNSUPPORTED;
 }
+
+// ---------------------------------------------------------------------------
+// Shape cotangent: point "forces" pulled back to the control points.
+//
+// A first variation of the geometry moves the control point k by dX_k; at
+// each Gauss point the map moves by R_k dX_k, so a point functional whose
+// derivative with respect to the point's position is xbar and whose
+// derivative with respect to the map's gradient is carried by the tensor P
+// (P[c][j] multiplies d(grad R_k)_j for a motion along c) pulls back to
+//   G[dofs[e][k]][c] += sum_q ( sum_j P[e][q][c][j] dR[e][q][j][k]
+//                               + R[e][q][k] xbar[e][q][c] ).
+// One thread per (element, local function); FP64 atomics into G (n_cp, DIM).
+// ---------------------------------------------------------------------------
+
+namespace iga {
+
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_point_forces(int64_t n_e, int n_q, int n_l, const double* __restrict__ R,
+                   const double* __restrict__ dR, const double* __restrict__ P,
+                   const double* __restrict__ xbar, const int32_t* __restrict__ dofs,
+                   double* __restrict__ G) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_e * n_l) return;
+  const int64_t e = t / n_l;
+  const int k = (int)(t % n_l);
+  double g[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) g[c] = 0.0;
+  for (int q = 0; q < n_q; ++q) {
+    const int64_t p = e * n_q + q;
+    double d[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) d[j] = dR[(p * DIM + j) * n_l + k];
+    const double r = xbar ? R[p * n_l + k] : 0.0;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      double s = xbar ? r * xbar[p * DIM + c] : 0.0;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) s += P[(p * DIM + c) * DIM + j] * d[j];
+      g[c] += s;
+    }
+  }
+  const int64_t cp = dofs[e * n_l + k];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) atomicAdd(G + cp * DIM + c, g[c]);
+}
+
+}  // namespace iga
+
+extern "C" int iga_point_forces(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, const double* R,
+                                const double* dR, const double* P, const double* xbar,
+                                const int32_t* dofs, double* G, void* stream) {
+  if (!dR || !P || !dofs || !G || (xbar && !R)) return IGA_ERR_ARG;
+  if (dim != 2 && dim != 3) return IGA_ERR_UNSUPPORTED;
+  const int64_t n = (int64_t)n_e * n_l;
+  if (n == 0) return IGA_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_point_forces<2>, dim3(blocks), dim3(256), 0, st, (int64_t)n_e, n_q, n_l, R,
+                       dR, P, xbar, dofs, G);
+  else
+    hipLaunchKernelGGL(k_point_forces<3>, dim3(blocks), dim3(256), 0, st, (int64_t)n_e, n_q, n_l, R,
+                       dR, P, xbar, dofs, G);
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
+}

@@ -135,6 +135,16 @@ class Problem:
             f"{type(self).__name__} defines no energy(); use a built-in problem "
             "(Poisson, LinearElasticity)")
 
+    def with_space(self, space) -> "Problem":
+        """The same problem on another space of the same connectivity (the
+        geometry changes, the physics and the resolved boundary dofs do not),
+        as ``Problem.with_space`` of the reference (``forms/problem.py:167-175``)
+        for shape derivatives."""
+        clone = object.__new__(type(self))
+        clone.__dict__.update(self.__dict__)
+        clone.space = space
+        return clone
+
     def resolve_dirichlet(self, params=None):
         """Sorted prescribed global dofs and their nodal values at control points."""
         space = self.space

@@ -570,6 +570,21 @@ def default_linear(problem, n_free=None) -> LIN.LinearOptions:
     return LIN.LinearOptions()
 
 
+def _check_shape_derivative(problem):
+    """What the shape cotangent covers: interior terms (stiffness, source,
+    coefficient) with constant Dirichlet values. Neumann loads move with the
+    boundary (measure and normal), and callable Dirichlet values are taken at
+    the control points that move; neither is differentiated here."""
+    if problem.neumann:
+        raise NotImplementedError("derivatives with respect to control points do not cover Neumann "
+                                  "loads; remove the Neumann conditions or keep the control points fixed")
+    for bc in problem.dirichlet:
+        if callable(bc.value):
+            raise NotImplementedError("derivatives with respect to control points need constant "
+                                      "Dirichlet values (a callable is evaluated at the moving control "
+                                      "points)")
+
+
 def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadrature=None,
                    chunk=None, **kwargs):
     import torch
@@ -587,7 +602,11 @@ def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadra
     # one a callable closes over), the solve is differentiated implicitly:
     # the reverse pass is one adjoint solve
     inputs = point_inputs(ctx, params) if torch.is_grad_enabled() else None
-    traced = inputs is not None and inputs_require_grad(inputs)
+    cpts_t = getattr(problem.space, "_cpts_t", None)
+    shape = inputs is not None and isinstance(cpts_t, torch.Tensor) and cpts_t.requires_grad
+    if shape:
+        _check_shape_derivative(problem)
+    traced = inputs is not None and (shape or inputs_require_grad(inputs))
     if linear.matrix_free and not traced:
         from jaxiga_amd.matfree import matrix_free_solve_ctx
 

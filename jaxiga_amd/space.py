@@ -134,6 +134,7 @@ class FunctionSpace:
         self.boundaries = self._classify_boundaries()
         self._device = None
         self._patterns = {}
+        self._cpts_t = None  # control points as a tensor carrying a gradient (replace)
 
     # -- reference-compatible properties -------------------------------------
     @property
@@ -163,6 +164,32 @@ class FunctionSpace:
                 C = np.kron(C, self.extraction_ops[d][self.elem_index[e, d]])
             out[e] = C
         return out
+
+    def replace(self, *, cpts=None):
+        """A copy on new control points (same knots, weights and connectivity),
+        the role ``dataclasses.replace(V, cpts=cpts)`` plays in the reference's
+        shape derivatives (``tests/test_galerkin.py:233-262``). A torch tensor
+        that requires grad makes :func:`jaxiga_amd.solve` differentiable in
+        the control points; the copy shares the CSR patterns (they depend on
+        the connectivity only)."""
+        import copy
+
+        import torch
+
+        new = copy.copy(self)
+        if cpts is not None:
+            if isinstance(cpts, torch.Tensor):
+                arr = cpts.detach().to("cpu", torch.float64).numpy().copy()
+                new._cpts_t = cpts
+            else:
+                arr = np.array(cpts, dtype=float)
+                new._cpts_t = None
+            if arr.shape != self.cpts.shape:
+                raise ValueError(f"control points of shape {arr.shape}, expected {self.cpts.shape}")
+            new.cpts = arr
+            new.patch = dataclasses.replace(self.patch, ctrl_pts=arr)
+        new._device = None
+        return new
 
     def elem_dofs_vec(self) -> np.ndarray:
         return expand_dofs(self.elem_dofs, self.vec)

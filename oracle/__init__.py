@@ -53,4 +53,5 @@ PINNING = {
     "multigrid.vcycle/fcg": "tests/test_oracle_multigrid.py (symmetry, exact two-level solve, dense solve)",
     "adjoint.csr_solve_vjp": "tests/test_oracle_adjoint.py (central finite differences of the direct solve in data and b)",
     "adjoint.element_cotangent/point_cotangents/boundary_cotangent": "tests/test_oracle_adjoint.py (finite differences in a0 at points, source amplitude, E, nu, traction; non-zero Dirichlet data)",
+    "adjoint.shape_cotangent": "tests/test_oracle_adjoint.py (central finite differences of direct solves on moved control points, Poisson with callable a0(x), f(x) and constant non-zero Dirichlet data, rational elasticity with a body force of x; planted mistakes caught)",
 }

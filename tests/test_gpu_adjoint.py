@@ -249,3 +249,95 @@ def test_elasticity_gradients_match_oracle_and_finite_differences(gpu):
         tp[i] += h
         tm[i] -= h
         np.testing.assert_allclose(g[i], (loss(*tp) - loss(*tm)) / (2 * h), rtol=1e-5)
+
+
+def test_shape_derivative_wrt_control_points_reference_case(gpu):
+    """The reference's tests/test_galerkin.py:233-262: compliance sum(u^2) of
+    a p=2, 2x2 unit-square Poisson solve (u = 0 on the boundary, f = 1,
+    a0 = 1) differentiated in the control points, rebuilt on the moved
+    points; finite differences at flat indices 5 and 11 (rtol 1e-4, atol
+    1e-8), one forward and one adjoint solve."""
+    V = jx.FunctionSpace(jx.primitives.quadrilateral(UNIT).elevate(2).refine(1))
+    base = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=everywhere)], source=lambda x, p: 1.0 + 0 * x[0])
+
+    def compliance(cpts):
+        problem = base.with_space(V.replace(cpts=cpts))
+        return (jx.solve(problem, params={"a0": 1.0}, linear=TIGHT).u ** 2).sum()
+
+    c0 = torch.tensor(V.cpts, dtype=torch.float64, requires_grad=True)
+    LIN.reset_solve_counter()
+    compliance(c0).backward()
+    assert LIN.SOLVE_COUNTER == {"forward": 1, "adjoint": 1}, LIN.SOLVE_COUNTER
+    g = c0.grad.reshape(-1).numpy()
+    assert np.any(np.abs(g) > 1e-12)
+    flat = torch.tensor(V.cpts, dtype=torch.float64).reshape(-1)
+    for i in (5, 11):
+        h = 1e-6
+        with torch.no_grad():
+            fp = flat.clone()
+            fp[i] += h
+            fm = flat.clone()
+            fm[i] -= h
+            fd = float(compliance(fp.reshape(V.cpts.shape)) - compliance(fm.reshape(V.cpts.shape))) / (2 * h)
+        print(f"[shape] index {i}: adjoint {g[i]:.9e} finite difference {fd:.9e}", flush=True)
+        np.testing.assert_allclose(g[i], fd, rtol=1e-4, atol=1e-8)
+
+
+def _shape_oracle(kind):
+    """Matched GPU and oracle problems for the shape cotangent: distorted quad
+    Poisson (a0 and source callables of x, constant non-zero Dirichlet
+    value) and plane-strain elasticity on the rational quarter annulus with a
+    body force of x."""
+    if kind == "poisson":
+        P = jx.primitives.quadrilateral(QUAD).elevate(2).refine(2)
+        O = OG.quadrilateral(QUAD).elevate(2).refine(2)
+        V, VO = jx.FunctionSpace(P), OS.function_space(O)
+        a0 = lambda x, p: 1.0 + x[0] ** 2 + 0.5 * x[1]  # noqa: E731
+        src = lambda x, p: 1.0 + x[0] * x[1]  # noqa: E731
+        prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.3, where=everywhere)], source=src)
+        return V, VO, prob, [OF.DirichletBC(0.3, everywhere)], src, {"a0": a0}, "stress"
+    P = jx.primitives.quarter_annulus(1.0, 2.0).elevate(2).refine(1)
+    O = OG.quarter_annulus(1.0, 2.0).elevate(2).refine(1)
+    V, VO = jx.FunctionSpace(P, vec=2), OS.function_space(O, vec=2)
+
+    def src(x, p):
+        m = xp(x)
+        return m.stack([0.2 + x[1], -1.0 + 0.3 * x[0] * x[1]])
+
+    prob = jx.LinearElasticity(V, plane="strain", dirichlet=[jx.DirichletBC(0.0, where="inner")], source=src)
+    return V, VO, prob, [OF.DirichletBC(0.0, "inner")], src, {"E": 1.0, "nu": 0.3}, "strain"
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_shape_cotangent_matches_oracle(gpu, kind):
+    """d(c.u)/dX against the oracle's restated shape cotangent (pinned by
+    finite differences of its own solves): 1e-7 relative to the cotangent's
+    scale (the oracle differentiates the callables by central differences)."""
+    V, VO, prob, odir, src, params, plane = _shape_oracle(kind)
+    rng = np.random.default_rng(8)
+    c = torch.as_tensor(rng.standard_normal(V.n_dofs), device="cuda")
+    X = torch.tensor(V.cpts, dtype=torch.float64, requires_grad=True)
+    sol = jx.solve(prob.with_space(V.replace(cpts=X)), params=params, linear=TIGHT)
+    (c * sol.u).sum().backward()
+    out = OP.solve(kind, VO, odir, source=src, params=params, plane=plane, method="direct")
+    from oracle import assembly as OA
+
+    K = OA.to_scipy(out["pattern"], out["data"])
+    import scipy.sparse.linalg as spla
+
+    cc = c.cpu().numpy()
+    lam = np.zeros(VO.n_dofs)
+    lam[out["dofmap"].free] = spla.spsolve(K.tocsc(), cc[out["dofmap"].free])
+    ref = OJ.shape_cotangent(kind, VO, OS.gauss_basis(VO), out["u"], lam, params, src, plane=plane)
+    err = rel(X.grad.numpy(), ref)
+    print(f"[shape] {kind}: rel. difference to the oracle {err:.2e}", flush=True)
+    assert err < 1e-7
+
+
+def test_shape_derivative_rejects_what_it_does_not_cover(gpu):
+    V = jx.FunctionSpace(jx.primitives.quadrilateral(UNIT).elevate(2).refine(1))
+    X = torch.tensor(V.cpts, dtype=torch.float64, requires_grad=True)
+    prob = jx.Poisson(V.replace(cpts=X), dirichlet=[jx.DirichletBC(lambda x, p: x[0], where=everywhere)],
+                      source=lambda x, p: 1.0 + 0 * x[0])
+    with pytest.raises(NotImplementedError, match="constant"):
+        jx.solve(prob, params={"a0": 1.0}, linear=TIGHT)

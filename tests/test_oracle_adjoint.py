@@ -106,3 +106,56 @@ def test_elasticity_cotangents_match_finite_differences():
     dt = np.stack([np.ones(bb.x.shape[:2]), 0.3 * bb.x[..., 1]], -1)
     np.testing.assert_allclose((tb * dt).sum(), _fd(lambda e: c @ _elastic(V, E, nu, t + e)["u"], 1e-5),
                                rtol=RTOL)
+
+
+def _shape_case(kind, X):
+    """An oracle solve on control points X: distorted Poisson quad (p=2, 4x4,
+    a0 and source callables of x, constant non-zero Dirichlet value) or
+    plane-strain elasticity on the rational quarter annulus (p=2, clamped
+    inner edge, body force callable of x)."""
+    if kind == "poisson":
+        base = G.quadrilateral([[0, 0], [1.2, 0.1], [1.0, 1.0], [-0.1, 0.9]]).elevate(2).refine(2)
+        V = S.function_space(G.Patch(X, base.weights, base.knots, base.degree, base.labels))
+        params = {"a0": lambda x, p: 1.0 + x[0] ** 2 + 0.5 * x[1]}
+        out = P.solve("poisson", V, [F.DirichletBC(0.3, everywhere)],
+                      source=lambda x, p: 1.0 + x[0] * x[1], params=params, method="direct")
+        return V, out, params, lambda x, p: 1.0 + x[0] * x[1]
+    base = G.quarter_annulus(1.0, 2.0).elevate(2).refine(1)
+    V = S.function_space(G.Patch(X, base.weights, base.knots, base.degree, base.labels), vec=2)
+    params = {"E": 1.0, "nu": 0.3}
+    src = lambda x, p: np.stack([0.2 + x[1], -1.0 + 0.3 * x[0] * x[1]])  # noqa: E731
+    out = P.solve("elasticity", V, [F.DirichletBC(0.0, "inner")], source=src, params=params,
+                  plane="strain", method="direct")
+    return V, out, params, src
+
+
+def _base_cpts(kind):
+    if kind == "poisson":
+        return G.quadrilateral([[0, 0], [1.2, 0.1], [1.0, 1.0], [-0.1, 0.9]]).elevate(2).refine(2).ctrl_pts
+    return G.quarter_annulus(1.0, 2.0).elevate(2).refine(1).ctrl_pts
+
+
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_shape_cotangent_matches_finite_differences(kind):
+    """d(c.u)/dX along random directions of the control points (interior and
+    boundary points both move) against central differences of the oracle's
+    direct solves on the moved patches."""
+    X0 = _base_cpts(kind)
+    V, out, params, src = _shape_case(kind, X0)
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal(V.n_dofs)
+    K = A.to_scipy(out["pattern"], out["data"])
+    lam_free = spla.spsolve(K.tocsc(), c[out["dofmap"].free])
+    lam = np.zeros(V.n_dofs)
+    lam[out["dofmap"].free] = lam_free
+    basis = S.gauss_basis(V)
+    Gc = AJ.shape_cotangent(kind, V, basis, out["u"], lam, params, src,
+                            plane="strain" if kind == "elasticity" else "stress")
+    for trial in range(3):
+        dX = rng.standard_normal(X0.shape)
+        h = 1e-6
+        fd = (c @ _shape_case(kind, X0 + h * dX)[1]["u"] - c @ _shape_case(kind, X0 - h * dX)[1]["u"]) / (2 * h)
+        np.testing.assert_allclose((Gc * dX).sum(), fd, rtol=1e-5)
