@@ -36,6 +36,8 @@
  *   iga_cg_solve_mixed        solvers/linear.py:324-351    (the same Jacobi-PCG, on an
  *                             FP32 matrix with FP64 residual replacement; the
  *                             reference runs FP64 only)
+ *   iga_neohooke              forms/library.py:206-225 (Hyperelasticity residual and
+ *                             tangent for the Newton path, methods/galerkin.py:337-355)
  *   iga_point_forces          methods/galerkin.py:37-148 under jax.grad with
  *                             respect to the control points (shape cotangent)
  *   iga_net_* / iga_nmse / iga_adam / iga_mg_* / iga_masked_spgemm /
@@ -338,6 +340,23 @@ int iga_field_points(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, int32_t
 int iga_point_forces(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, const double* R,
                      const double* dR, const double* P, const double* xbar, const int32_t* dofs,
                      double* G, void* stream);
+
+/* Compressible neo-Hookean element residuals and tangents at a displacement
+ * (replaces the residual and the Hessian of Hyperelasticity.energy,
+ * forms/library.py:206-225, that the reference's Newton method re-assembles
+ * every step, methods/galerkin.py:337-355, solvers/newton.py:27-145):
+ * psi = mu/2 (tr F^T F - d) - mu ln J + lam/2 (ln J)^2, F = I + grad u.
+ * R (n_e,n_q,n_l), dR (n_e,n_q,dim,n_l) and w (n_e,n_q) as iga_basis_eval
+ * writes them at the interior Gauss points; edofs (n_e,n_l) control points;
+ * u (n_cp*dim, component fastest) the full displacement; src (n_e,n_q,dim)
+ * body force at the points or null. rfree (n_free) += sum_q w (P:grad R_a
+ * - f.R_a) over the free dofs (atomics), and/or ke (n_e, n_l*dim, n_l*dim)
+ * the element tangents, row a*dim+i, the layout iga_assemble's gather pass
+ * (passes = 2) reads. dim 2 (p <= 4) or 3 (p <= 2).                        */
+int iga_neohooke(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, const double* R,
+                 const double* dR, const double* w, const int32_t* edofs, const double* u,
+                 const int32_t* full_to_free, double mu, double lam, const double* src,
+                 double* rfree, double* ke, void* stream);
 
 /* Learned networks on the control-point grid: 3x3 convolutions, float32 NCHW.
  * kind[l]   : 0 forward convolution (weights (cout, cin, 3, 3)),

@@ -22,6 +22,7 @@ __version__ = "0.1.0"
 from jaxiga_amd.evaluation import BasisData, evaluate
 from jaxiga_amd.forms import (
     DirichletBC,
+    Hyperelasticity,
     LinearElasticity,
     Neumann,
     PointField,
@@ -39,6 +40,7 @@ from jaxiga_amd.space import DofMap, FunctionSpace, expand_dofs
 __all__ = [
     "Patch", "primitives", "FunctionSpace", "DofMap", "expand_dofs", "PointSet", "gauss",
     "boundary_gauss", "BasisData", "evaluate", "Problem", "Poisson", "LinearElasticity",
+    "Hyperelasticity",
     "DirichletBC", "Neumann", "PointField", "materials", "LinearOptions", "solve", "Solution",
     "errornorm", "integrate", "norm",
 ]

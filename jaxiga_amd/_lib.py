@@ -158,6 +158,9 @@ _SIGNATURES = {
     "iga_inv_guard": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "iga_csr_outer": (c_int, [ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                               c_void_p]),
+    "iga_neohooke": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
+                             c_void_p]),
     "iga_point_forces": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     "iga_field_points": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -154,7 +154,8 @@ class Problem:
             if callable(bc.value):
                 vals = _per_point(bc.value(space.cpts[sdofs].T, params), len(sdofs))
             else:
-                vals = np.full((1, len(sdofs)), float(bc.value))
+                # one value, or one per component (forms/problem.py:203-206)
+                vals = np.repeat(np.atleast_1d(np.asarray(bc.value, float))[:, None], len(sdofs), 1)
             k = space.vec
             for c in comps:
                 row = vals[c] if vals.shape[0] == space.vec else vals[0]
@@ -201,6 +202,31 @@ class LinearElasticity(Problem):
 
     def C(self, params):
         return materials.constitutive(params["E"], params["nu"], self.space.dim, self.plane)
+
+
+class Hyperelasticity(Problem):
+    """Compressible neo-Hookean solid ``psi = mu/2 (tr F^T F - d) - mu ln J
+    + lam/2 (ln J)^2``, ``F = I + grad u`` (``forms/library.py:206-225``),
+    solved by Newton's method; ``params`` holds ``mu`` and ``lam`` or ``E``
+    and ``nu`` (converted by :meth:`materials.lame`)."""
+
+    is_linear = False
+    is_positive_definite = False
+    kernel = "neohooke"
+
+    def __init__(self, space, **kwargs):
+        if space.vec != space.dim:
+            raise ValueError(f"hyperelasticity needs vec == dim; got vec={space.vec}, dim={space.dim}. "
+                             f"Build the space with FunctionSpace(patch, vec={space.dim}).")
+        super().__init__(space, **kwargs)
+
+    @staticmethod
+    def lame_parameters(params):
+        """``(mu, lam)`` from ``params``."""
+        if "mu" in params:
+            return float(params["mu"]), float(params["lam"])
+        lam, mu = materials.lame(float(params["E"]), float(params["nu"]))
+        return mu, lam
 
 
 # ---------------------------------------------------------------------------

@@ -482,7 +482,7 @@ def element_workspace(ctx, precision=_lib.IGA_FP64):
     key = (ctx.problem.kernel, precision)
     ws = ctx._ke.get(key) if ctx._ke is not None else None
     if ws is None:
-        kind = _lib.IGA_ELASTICITY if ctx.problem.kernel == "elasticity" else _lib.IGA_POISSON
+        kind = _lib.IGA_ELASTICITY if ctx.problem.kernel in ("elasticity", "neohooke") else _lib.IGA_POISSON
         nbytes = int(lib().iga_assemble_workspace(ctx.space.device().struct, kind, precision))
         if nbytes < 0:
             raise NotImplementedError(f"no GPU assembly for dim={ctx.space.dim}, degree={ctx.space.degree}")
@@ -591,12 +591,20 @@ def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadra
 
     if not isinstance(problem, Problem) or problem.kernel is None:
         raise TypeError("Galerkin requires a problem with an element kernel "
-                        "(Poisson or LinearElasticity)")
-    if not problem.is_linear:
-        raise NotImplementedError("nonlinear Galerkin (Newton) is outside the supported hot path")
+                        "(Poisson, LinearElasticity or Hyperelasticity)")
     ctx = build_context(problem, params, quadrature, chunk)
     if linear is None:
         linear = default_linear(problem, ctx.n_free)
+    if not problem.is_linear:
+        if problem.kernel != "neohooke":
+            raise NotImplementedError(f"{type(problem).__name__}: the nonlinear Galerkin path covers "
+                                      "Hyperelasticity")
+        if linear.matrix_free:
+            raise NotImplementedError("the Newton path assembles its tangent; use matrix_free=False")
+        from jaxiga_amd.nonlinear import solve_nonlinear
+
+        u, stats = solve_nonlinear(ctx, params, linear, initial_guess, **kwargs)
+        return u, ctx, stats
     # with autograd on, the parameter-dependent values at points are
     # evaluated once here; if any carries a gradient (a tensor in params, or
     # one a callable closes over), the solve is differentiated implicitly:

@@ -21,7 +21,9 @@ pipeline   end-to-end ``solve`` for the configurations used by the tests
 grf        seeded random-Fourier-feature Gaussian random fields (inputs)
 learned    networks of the learned preconditioner (forward, exact gradients)
 multigrid  knot coarsening, Galerkin levels, stencil smoothers, V-cycle, FCG
-adjoint    reverse pass of the solve: CSR reverse rule, element and point cotangents
+adjoint    reverse pass of the solve: CSR reverse rule, element and point cotangents,
+           control-point (shape) cotangent
+nonlinear  neo-Hookean residual and tangent, Newton with backtracking
 
 Pinning status of every function is listed in ``PINNING`` below; each entry
 names the test module that pins it against closed forms, SciPy, or brute
@@ -53,5 +55,7 @@ PINNING = {
     "multigrid.vcycle/fcg": "tests/test_oracle_multigrid.py (symmetry, exact two-level solve, dense solve)",
     "adjoint.csr_solve_vjp": "tests/test_oracle_adjoint.py (central finite differences of the direct solve in data and b)",
     "adjoint.element_cotangent/point_cotangents/boundary_cotangent": "tests/test_oracle_adjoint.py (finite differences in a0 at points, source amplitude, E, nu, traction; non-zero Dirichlet data)",
+    "nonlinear.element_residual_tangent": "tests/test_oracle_nonlinear.py (central finite differences of the neo-Hookean energy and of the residual, with a body force)",
+    "nonlinear.newton": "tests/test_oracle_nonlinear.py (quadratic convergence and the prescribed top displacement on the reference's nonlinear case)",
     "adjoint.shape_cotangent": "tests/test_oracle_adjoint.py (central finite differences of direct solves on moved control points, Poisson with callable a0(x), f(x) and constant non-zero Dirichlet data, rational elasticity with a body force of x; planted mistakes caught)",
 }

@@ -177,7 +177,8 @@ def resolve_dirichlet(space, bcs, params=None):
         if callable(bc.value):
             vals = _per_point(bc.value(space.cpts[sdofs].T, params), len(sdofs))
         else:
-            vals = np.full((1, len(sdofs)), float(bc.value))
+            # a literal: one value, or one per component (forms/problem.py:203-206)
+            vals = np.repeat(np.atleast_1d(np.asarray(bc.value, float))[:, None], len(sdofs), 1)
         for k, sd in enumerate(sdofs):
             for c in comps:
                 g = int(space.vec * sd + c)
