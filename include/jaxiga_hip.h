@@ -36,6 +36,9 @@
  *   iga_cg_solve_mixed        solvers/linear.py:324-351    (the same Jacobi-PCG, on an
  *                             FP32 matrix with FP64 residual replacement; the
  *                             reference runs FP64 only)
+ *   iga_elem_pattern / iga_elem_basis / iga_elem_assemble
+ *                             space/function_space.py:23-38 (padded elements),
+ *                             methods/_common.py:125-264 (chunks, CSR pattern)
  *   iga_neohooke              forms/library.py:206-225 (Hyperelasticity residual and
  *                             tangent for the Newton path, methods/galerkin.py:337-355)
  *   iga_point_forces          methods/galerkin.py:37-148 under jax.grad with
@@ -340,6 +343,43 @@ int iga_field_points(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, int32_t
 int iga_point_forces(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, const double* R,
                      const double* dR, const double* P, const double* xbar, const int32_t* dofs,
                      double* G, void* stream);
+
+/* The element-list path: the padded-element contract of the reference's
+ * FunctionSpace (space/function_space.py:23-38: elem_dofs (n_e, n_lmax)
+ * padded with the sentinel n_scalar, extraction (n_e, n_lmax, (p+1)^dim)
+ * with zero rows under the padding) and its chunked evaluation
+ * (space/evaluation.py:256-302, methods/_common.py:125-160).
+ * iga_elem_pattern  (methods/_common.py:203-264, _build_csr_pattern): sorted
+ *   unique keys row*n_free + col over every (element, local row, local column)
+ *   with free dofs; first call with cols == NULL writes nnz to *out_nnz (the
+ *   stream is synchronised), a second call with the same arguments and
+ *   workspace fills indptr (n_free+1), cols/rows (nnz) and scatter (n_e,
+ *   n_lmax*vec, n_lmax*vec): the slot of each entry or nnz for padding and
+ *   prescribed dofs. Workspace: iga_elem_pattern_workspace bytes (-1 when
+ *   n_e (n_lmax vec)^2 exceeds 2^31 - 1).
+ * iga_elem_basis (space/evaluation.py:305-467 on a range of elements): R
+ *   (n_c,n_q,n_lmax) or null, dR (n_c,n_q,dim,n_lmax), x (n_c,n_q,dim) or
+ *   null, w (n_c,n_q) of elements lo .. lo+n_c-1 at the tensor Gauss points;
+ *   boxes (n_e, 2 dim): the parametric lower corners then upper corners.
+ * iga_elem_assemble (methods/galerkin.py:122-148 and 240-245 on a range of
+ *   elements): Poisson or elasticity tangents from that basis data,
+ *   scatter-added into args->vals through `scatter` (FP64 atomics), and the
+ *   source work and lifting -K_fp g into args->rhs; coefficient and source
+ *   per point are (n_c, n_q[, vec]) for the range. args: problem, vec,
+ *   coef_*, cmat, src_*, full_to_free, gfull, nnz, vals, rhs.              */
+int64_t iga_elem_pattern_workspace(int32_t n_e, int32_t n_lmax, int32_t vec);
+int iga_elem_pattern(int32_t n_e, int32_t n_lmax, int32_t vec, const int32_t* elem_dofs,
+                     int32_t n_scalar, const int32_t* full_to_free, int32_t n_free, void* work,
+                     int64_t work_bytes, int32_t* indptr, int32_t* cols, int32_t* rows,
+                     int32_t* scatter, int64_t* out_nnz, void* stream);
+int iga_elem_basis(int32_t dim, int32_t degree, int32_t n_lmax, int64_t lo, int64_t n_c,
+                   const iga_rule_t* rule, const double* ext, const int32_t* elem_dofs,
+                   int32_t n_scalar, const double* cpts, const double* wgts, const double* boxes,
+                   double* R, double* dR, double* x, double* w, void* stream);
+int iga_elem_assemble(int32_t dim, int32_t n_q, int32_t n_lmax, int64_t lo, int64_t n_c,
+                      const double* R, const double* dR, const double* w, const int32_t* elem_dofs,
+                      int32_t n_scalar, const iga_asm_args_t* args, const int32_t* scatter,
+                      void* stream);
 
 /* Compressible neo-Hookean element residuals and tangents at a displacement
  * (replaces the residual and the Hessian of Hyperelasticity.energy,

@@ -158,6 +158,14 @@ _SIGNATURES = {
     "iga_inv_guard": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "iga_csr_outer": (c_int, [ctypes.c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                               c_void_p]),
+    "iga_elem_pattern_workspace": (c_i64, [c_int, c_int, c_int]),
+    "iga_elem_pattern": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_i64,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_i64), c_void_p]),
+    "iga_elem_basis": (c_int, [c_int, c_int, c_int, c_i64, c_i64, ctypes.POINTER(IgaRule), c_void_p,
+                               c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p]),
+    "iga_elem_assemble": (c_int, [c_int, c_int, c_int, c_i64, c_i64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_int, ctypes.POINTER(IgaAsmArgs), c_void_p, c_void_p]),
     "iga_neohooke": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p,
                              c_void_p]),

@@ -592,6 +592,15 @@ def solve_galerkin(problem, params=None, linear=None, initial_guess=None, quadra
     if not isinstance(problem, Problem) or problem.kernel is None:
         raise TypeError("Galerkin requires a problem with an element kernel "
                         "(Poisson, LinearElasticity or Hyperelasticity)")
+    from jaxiga_amd import elements as EL
+
+    if chunk is not None or isinstance(problem.space, EL.ElementSpace):
+        # the padded-element contract and chunked assembly: the element-list path
+        if not problem.is_linear:
+            raise NotImplementedError("the element-list path covers linear problems")
+        if quadrature is not None:
+            raise NotImplementedError("the element-list path uses the p+1 Gauss rule")
+        return EL.solve(problem, params, linear, chunk)
     ctx = build_context(problem, params, quadrature, chunk)
     if linear is None:
         linear = default_linear(problem, ctx.n_free)

@@ -122,3 +122,23 @@ def test_point_callables_written_per_point_are_vectorised():
     assert float(call_points(lambda y, p: 2.5, x).unique()) == 2.5
     vec = call_points(lambda y, p: torch.tensor([0.0, -1.0], dtype=torch.float64), x, vec=2)
     assert vec.shape == (6, 4, 2) and float(vec[..., 1].unique()) == -1.0
+
+
+def test_element_space_validates_the_padding_contract():
+    """reference space/function_space.py:23-38: padding entries are the
+    sentinel n_scalar_basis and their extraction rows are zero."""
+    import jaxiga_amd as jx
+    from jaxiga_amd.elements import ElementSpace
+
+    V, _ = _unit_poisson()
+    es = ElementSpace.from_space(V, n_local_max=V.n_local + 2, permute_seed=0)
+    assert es.n_local_max == V.n_local + 2
+    assert (es.elem_dofs == V.n_scalar_basis).sum() == 2 * V.n_elems
+    bad = es.extraction.copy()
+    bad[es.elem_dofs == V.n_scalar_basis] = 1.0
+    with pytest.raises(ValueError, match="zero"):
+        ElementSpace(degree=V.degree, cpts=V.cpts, wgts=V.wgts, elem_dofs=es.elem_dofs, extraction=bad,
+                     elem_vertex=V.elem_vertex)
+    with pytest.raises(ValueError, match="n_local_max"):
+        ElementSpace.from_space(V, n_local_max=V.n_local - 1)
+    del jx
