@@ -93,5 +93,21 @@ def evaluate(space, ps, order: int = 1, outputs=("R", "dR", "x", "w", "normal", 
         int(wref is not None and wref.dim() == 1), _lib.ptr(sides), _lib.ptr(R), _lib.ptr(dR),
         _lib.ptr(x), _lib.ptr(w), _lib.ptr(normal), _lib.ptr(detJ), stream())
     _lib.check(rc, "iga_basis_eval")
-    return BasisData(R=R, dR=dR, x=x, dofs=space.elem_dofs[np.asarray(ps.elems)], w=w,
-                     normal=normal, detJ=detJ, tag=ps.tag)
+    return BasisData(R=R, dR=dR, x=x, dofs=_element_dofs(space, ps), w=w, normal=normal, detJ=detJ,
+                     tag=ps.tag)
+
+
+def _element_dofs(space, ps):
+    """``space.elem_dofs`` of the point set's elements, formed once per point
+    set and space (the full interior set is the table itself, not a copy)."""
+    cached = getattr(ps, "_dofs_cache", None)
+    if cached is not None and cached[0] is space.elem_dofs:
+        return cached[1]
+    elems = np.asarray(ps.elems)
+    n = space.elem_dofs.shape[0]
+    if len(elems) == n and elems[0] == 0 and np.array_equal(elems, np.arange(n)):
+        dofs = space.elem_dofs
+    else:
+        dofs = space.elem_dofs[elems]
+    object.__setattr__(ps, "_dofs_cache", (space.elem_dofs, dofs))
+    return dofs
