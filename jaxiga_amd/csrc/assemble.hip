@@ -257,39 +257,128 @@ __global__ void __launch_bounds__(256)
   double jac_ref = 1.0;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) jac_ref *= 0.5 * (pt.hi[d][ei[d]] - pt.lo[d][ei[d]]);
+  // weights and weighted control points of the element's functions
+  double* ctl = Mb + (kLdsCombine ? DIM * DIM * 256 : 0);  // [NL][DIM+1]
+  double* frm = ctl + NL * (DIM + 1);                        // [nq][1 + DIM + DIM*DIM]
+  double* wkq = frm + nq * (1 + DIM + DIM * DIM);            // [nq] w_q times the coefficient
+  constexpr int FR = 1 + DIM + DIM * DIM;
+  if (slot == 0 && active) {
+    ctl[lane * (DIM + 1)] = wgt;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) ctl[lane * (DIM + 1) + 1 + c] = wgt * Pc[c];
+  }
   __syncthreads();
 
-  // ---- rational basis and push-forward per Gauss point (FP64) -------------
-  for (int q = slot; q < nq; q += nslots) {
-    int qi[3] = {q % nq1, (q / nq1) % nq1, q / (nq1 * nq1)};
-    double N = 1.0, dN[DIM];
+  // ---- per Gauss point, one lane: weight function, geometry, J^-1 (FP64) ---
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+    const int qi[3] = {q % nq1, (q / nq1) % nq1, q / (nq1 * nq1)};
+    double W = 0.0, dW[DIM] = {}, X[DIM] = {}, Y[DIM][DIM] = {};
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) dN[k] = 1.0;
+    for (int f = 0; f < NL; ++f) {
+      const int fi[3] = {f % N1, (f / N1) % N1, f / (N1 * N1)};
+      double N = 1.0, dN[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      const int o = (d * nq1 + qi[d]) * N1 + ai[d];
-      const double v = N1t[o], dv = dN1t[o];
-      N *= v;
+      for (int k = 0; k < DIM; ++k) dN[k] = 1.0;
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) dN[k] *= (k == d) ? dv : v;
+      for (int d = 0; d < DIM; ++d) {
+        const int o = (d * nq1 + qi[d]) * N1 + fi[d];
+        const double v = N1t[o], dv = dN1t[o];
+        N *= v;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) dN[k] *= (k == d) ? dv : v;
+      }
+      const double* c = ctl + f * (DIM + 1);
+      W += N * c[0];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) dW[k] += dN[k] * c[0];
+#pragma unroll
+      for (int cc = 0; cc < DIM; ++cc) {
+        X[cc] += N * c[1 + cc];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) Y[k][cc] += dN[k] * c[1 + cc];
+      }
     }
-    PointGeo<DIM, NLP> g;
-    rational_pushforward<DIM, NLP>(active, N, dN, wgt, Pc, g);
-    double wq = jac_ref * fabs(g.detJ);
+    const double iW = 1.0 / W;
+    double xp[DIM], J[DIM][DIM], inv[DIM][DIM], det;
+#pragma unroll
+    for (int cc = 0; cc < DIM; ++cc) xp[cc] = X[cc] * iW;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k)
+#pragma unroll
+      for (int cc = 0; cc < DIM; ++cc) J[k][cc] = (Y[k][cc] - dW[k] * xp[cc]) * iW;
+    if constexpr (DIM == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      inv[0][0] = J[1][1] / det;
+      inv[0][1] = -J[0][1] / det;
+      inv[1][0] = -J[1][0] / det;
+      inv[1][1] = J[0][0] / det;
+    } else {
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      inv[0][0] = c00 / det;
+      inv[1][0] = c01 / det;
+      inv[2][0] = c02 / det;
+      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    double* fr = frm + q * FR;
+    fr[0] = W;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) fr[1 + k] = dW[k];
+#pragma unroll
+    for (int cc = 0; cc < DIM; ++cc)
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) fr[1 + DIM + cc * DIM + k] = inv[cc][k];
+    double wq = jac_ref * fabs(det);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) wq *= qw[qi[d]];
     double wk = wq;
     if constexpr (PROB == IGA_POISSON)
       wk *= a.coef_kind == IGA_COEF_POINTS ? coefp[(size_t)e * nq + q] : a.coef_const;
-    if (active) {
+    Wf[q] = wq;
+    wkq[q] = wk;
+  }
+  __syncthreads();
+
+  // ---- per (point, function): R and the physical gradients ---------------
+  for (int t = threadIdx.x; t < nq * NL; t += blockDim.x) {
+    const int q = t / NL, f = t - q * NL;
+    const int qi[3] = {q % nq1, (q / nq1) % nq1, q / (nq1 * nq1)};
+    const int fi[3] = {f % N1, (f / N1) % N1, f / (N1 * N1)};
+    double N = 1.0, dN[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        GA[(c * nqp + q) * NLT + lane] = (T)g.dR[c];
-        GW[(c * nqp + q) * NLT + lane] = (T)(wk * g.dR[c]);
-      }
-      Rs[q * NL + lane] = g.R;
+    for (int k = 0; k < DIM; ++k) dN[k] = 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int o = (d * nq1 + qi[d]) * N1 + fi[d];
+      const double v = N1t[o], dv = dN1t[o];
+      N *= v;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) dN[k] *= (k == d) ? dv : v;
     }
-    if (lane == 0) Wf[q] = wq;
+    const double* fr = frm + q * FR;
+    const double wa = ctl[f * (DIM + 1)];
+    const double iW = 1.0 / fr[0];
+    const double R = N * wa * iW;
+    double gp[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) gp[k] = (dN[k] * wa - R * fr[1 + k]) * iW;
+    const double wk = wkq[q];
+#pragma unroll
+    for (int cc = 0; cc < DIM; ++cc) {
+      double g = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) g += fr[1 + DIM + cc * DIM + k] * gp[k];
+      GA[(cc * nqp + q) * NLT + f] = (T)g;
+      GW[(cc * nqp + q) * NLT + f] = (T)(wk * g);
+    }
+    Rs[q * NL + f] = R;
   }
   __syncthreads();
 
@@ -746,6 +835,378 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
   }
 }
 
+// Sign flip of an MFMA operand by a mask (0 or the sign bit of the high
+// word), one integer operation instead of an FP64 multiply.
+__device__ __forceinline__ double flip_sign(double x, uint32_t m) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)m << 32));
+}
+__device__ __forceinline__ float flip_sign(float x, uint32_t m) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ m);
+}
+
+// Element pass for scalar Poisson on 3-D patches with p <= 2 and p + 1 Gauss
+// points per direction: one wavefront per element, four per workgroup, no
+// workgroup barrier. The FP64 geometry -- W, w x and their parametric
+// derivatives at every Gauss point, sixteen tensor sums -- is sum-factorised
+// in two stages: the first direction per (field, a1, a2), then the other two
+// per (field, point). With A = J^-1 / W, b = A dW / W and s = sqrt|w_q a(x_q)|,
+// the scaled physical gradient of R_i = w_i N_i / W is
+//   s grad R_i = (s A) (w_i dN_i) - (w_i N_i) (s b),
+// and K_e = sum_q sign(w_q a) (s grad R_i)(s grad R_j): both MFMA operands
+// are the same scaled gradient (up to the sign), so one point record (s A,
+// s b, the source weight over W, the point's table offsets) and seven
+// products per function and point give the operands. Lane l owns the
+// functions l % 16 + 16 u (u < TI) at the Gauss point 4 t + l / 16 in step t,
+// the operand layout of the 16x16x4 MFMA, so the gradients feed the matrix
+// cores from registers. K_e is symmetric: only the tiles I <= J are formed
+// (TI (TI + 1) / 2 of TI^2); they are placed (the off-diagonal ones twice)
+// in the wave's LDS, over the geometry, and streamed out row-major with
+// coalesced stores; the diagonal and the lifting read the staged matrix.
+template <int P, typename T, bool REG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+    k_element_w3(iga_patch_t pt, const double* __restrict__ qp, const double* __restrict__ qw,
+                 const AsmDev a, T* __restrict__ ke_all, int n_e) {
+  constexpr int N1 = P + 1, N2 = N1 * N1, NL = N2 * N1, NQ = NL;
+  constexpr int TI = (NL + 15) / 16;       // 16-function tiles
+  constexpr int NF = 16 * TI;              // padded function count
+  constexpr int NT = (NQ + 3) / 4;         // MFMA steps per gradient component
+  constexpr int NTILE = TI * (TI + 1) / 2;  // tiles I <= J
+  constexpr int WPB = 4;                   // elements (waves) per workgroup
+  // doubles per Gauss-point record: 16 used; the stride of 18 puts the four
+  // records read together (one per lane group) on different LDS banks
+  constexpr int REC = 18;
+  constexpr int T1 = 4 * N2 * N1 * 2 > NF * N1 * 2 ? 4 * N2 * N1 * 2 : NF * N1 * 2;
+  // one LDS buffer per wave (offsets in doubles, 16-byte aligned):
+  //   tab [dir][q][a][value|derivative]          1-D tables
+  //   ctl [4][NL]                                w, w x, w y, w z per function
+  //   t1  stage-1 sums [field][a1 + N1 a2][q0][v|d]; afterwards the weighted
+  //       first-direction factors w_f (N, dN)(q0) per padded function [f][q0][v|d]
+  //   geo [q][4 field + (value, d0, d1, d2)], then in place the point record
+  // and, once the products are done, the element matrix [NL][NL] over all of it
+  constexpr int O_TAB = 0;
+  constexpr int O_CTL = O_TAB + 3 * N1 * N1 * 2;
+  constexpr int O_T1 = (O_CTL + 4 * NL + 1) & ~1;
+  constexpr int O_GEO = O_T1 + T1;
+  constexpr int KE_D = (NL * NL * (int)sizeof(T) + 7) / 8;
+  constexpr int SZ = ((O_GEO + NT * 4 * REC > KE_D ? O_GEO + NT * 4 * REC : KE_D) + 1) & ~1;
+  __shared__ __align__(16) double s_buf[WPB][SZ];
+  __shared__ int s_fdof[WPB][NL];
+  __shared__ int s_gdof[WPB][NL];
+
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  const int e = blockIdx.x * WPB + w;
+  if (e >= n_e) return;
+  double* const tab = s_buf[w] + O_TAB;
+  double* const ctl = s_buf[w] + O_CTL;
+  double* const t1b = s_buf[w] + O_T1;
+  double* const geo = s_buf[w] + O_GEO;
+  auto TAB = [&](int d, int q, int f) { return tab + ((d * N1 + q) * N1 + f) * 2; };
+  const int64_t bi = blockIdx.y;
+  const double* coefp = a.coef_points ? a.coef_points + bi * n_e * NQ : nullptr;
+  const double* srcp = a.src_points ? a.src_points + bi * n_e * NQ : nullptr;
+  double* rhsp = a.rhs ? a.rhs + bi * a.n_free : nullptr;
+  double* diagp = a.diag ? a.diag + bi * a.n_free : nullptr;
+  T* kout = ke_all ? ke_all + (bi * n_e + e) * (int64_t)(NL * NL) : nullptr;
+  const int ei0 = e % pt.n_elem[0], ei1 = (e / pt.n_elem[0]) % pt.n_elem[1],
+            ei2 = e / (pt.n_elem[0] * pt.n_elem[1]);
+
+  // ---- every global read of the element, issued together ------------------
+  // table lane (direction td, function ta, point tq), meaningful for l < 3 N1^2;
+  // the kernel-argument arrays are selected, not indexed at run time
+  const int tl = l < 3 * N2 ? l : 0;
+  const int td = tl / N2, ta = (tl / N1) % N1, tq = tl % N1;
+  const int etd = td == 0 ? ei0 : td == 1 ? ei1 : ei2;
+  const double* ext = td == 0 ? pt.ext[0] : td == 1 ? pt.ext[1] : pt.ext[2];
+  const double* lo = td == 0 ? pt.lo[0] : td == 1 ? pt.lo[1] : pt.lo[2];
+  const double* hi = td == 0 ? pt.hi[0] : td == 1 ? pt.hi[1] : pt.hi[2];
+  double crow[N1];
+#pragma unroll
+  for (int b = 0; b < N1; ++b) crow[b] = ext[((size_t)etd * N1 + ta) * N1 + b];
+  const double xq = qp[tq];
+  const double hd = hi[etd] - lo[etd];
+  // function lane f = l (f0 + N1 f1 + N1^2 f2), meaningful for l < NL;
+  // regular patches: the first function of element i is i (no table read)
+  const int fl = l < NL ? l : NL - 1;
+  const int f0 = REG ? ei0 : pt.first[0][ei0], f1 = REG ? ei1 : pt.first[1][ei1],
+            f2 = REG ? ei2 : pt.first[2][ei2];
+  const int sdof = (f0 + fl % N1) + pt.n_cp[0] * ((f1 + (fl / N1) % N1) + pt.n_cp[1] * (f2 + fl / N2));
+  const double wt_r = pt.wgts[sdof];
+  const double px = pt.cpts[(size_t)sdof * 3], py = pt.cpts[(size_t)sdof * 3 + 1],
+               pz = pt.cpts[(size_t)sdof * 3 + 2];
+  const int fd_r = a.f2f[sdof];
+  // point lane q = l, meaningful for l < NQ
+  const int ql = l < NQ ? l : NQ - 1;
+  const double coef_r = *(coefp ? coefp + (size_t)e * NQ + ql : qw);
+  const double src_r = *(srcp ? srcp + (size_t)e * NQ + ql : qw);
+  const double jac_ref = 0.125 * (pt.hi[0][ei0] - pt.lo[0][ei0]) * (pt.hi[1][ei1] - pt.lo[1][ei1]) *
+                         (pt.hi[2][ei2] - pt.lo[2][ei2]);
+
+  // ---- 1-D extracted tables and the local functions -------------------------
+  if (l < 3 * N2) {
+    double B[N1], dB[N1], v = 0.0, dv = 0.0;
+    bernstein<P>(xq, B, dB);
+#pragma unroll
+    for (int b = 0; b < N1; ++b) {
+      v += crow[b] * B[b];
+      dv += crow[b] * dB[b];
+    }
+    double* tp = TAB(td, tq, ta);
+    tp[0] = v;
+    tp[1] = dv * (2.0 / hd);
+  }
+  if (l < NL) {
+    ctl[l] = wt_r;
+    ctl[NL + l] = wt_r * px;
+    ctl[2 * NL + l] = wt_r * py;
+    ctl[3 * NL + l] = wt_r * pz;
+    s_fdof[w][l] = fd_r;
+    s_gdof[w][l] = sdof;
+  }
+  wave_lds_sync();
+
+  // ---- geometry, stage 1: contract the first direction -----------------------
+  if (l < 4 * N2) {
+    const int f = l / N2, a12 = l % N2;
+    double c[N1];
+#pragma unroll
+    for (int a0 = 0; a0 < N1; ++a0) c[a0] = ctl[f * NL + a0 + N1 * a12];
+#pragma unroll
+    for (int q0 = 0; q0 < N1; ++q0) {
+      double sv = 0.0, sd = 0.0;
+#pragma unroll
+      for (int a0 = 0; a0 < N1; ++a0) {
+        sv += TAB(0, q0, a0)[0] * c[a0];
+        sd += TAB(0, q0, a0)[1] * c[a0];
+      }
+      t1b[((f * N2 + a12) * N1 + q0) * 2] = sv;
+      t1b[((f * N2 + a12) * N1 + q0) * 2 + 1] = sd;
+    }
+  }
+  wave_lds_sync();
+  // ---- stage 2: the other two directions, per (field, point) ----------------
+#pragma unroll
+  for (int it = l; it < 4 * NQ; it += 64) {
+    const int f = it / NQ, q = it % NQ;
+    const int q0 = q % N1, q1 = (q / N1) % N1, q2 = q / N2;
+    double v = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+    for (int a2 = 0; a2 < N1; ++a2) {
+      const double n2 = TAB(2, q2, a2)[0], dn2 = TAB(2, q2, a2)[1];
+#pragma unroll
+      for (int a1 = 0; a1 < N1; ++a1) {
+        const double n1 = TAB(1, q1, a1)[0], dn1 = TAB(1, q1, a1)[1];
+        const double* t1 = t1b + ((f * N2 + a1 + N1 * a2) * N1 + q0) * 2;
+        const double tv = t1[0], tdv = t1[1];
+        const double b = n1 * n2;
+        v += tv * b;
+        d0 += tdv * b;
+        d1 += tv * (dn1 * n2);
+        d2 += tv * (n1 * dn2);
+      }
+    }
+    double* gq = geo + q * REC + 4 * f;
+    gq[0] = v;
+    gq[1] = d0;
+    gq[2] = d1;
+    gq[3] = d2;
+  }
+  wave_lds_sync();
+  // ---- point records: s A, s b, the source weight over W, the point's table
+  // offsets and the sign of its stiffness weight; and the weighted
+  // first-direction factors of every function ----------------------------------
+  if (l < NT * 4) {
+    double rec[13];
+#pragma unroll
+    for (int t = 0; t < 13; ++t) rec[t] = 0.0;
+    uint32_t sign = 0;
+    const int qc = l < NQ ? l : 0;
+    double* gq = geo + l * REC;
+    if (l < NQ) {
+      double G[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) G[t] = gq[t];
+      const double W = G[0], iW = 1.0 / W;
+      // J[k][c] = d x_c / d xi_k
+      double J[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double xc = G[4 * (c + 1)] * iW;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) J[k][c] = (G[4 * (c + 1) + 1 + k] - xc * G[1 + k]) * iW;
+      }
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      const double wq = jac_ref * fabs(det) * qw[l % N1] * qw[(l / N1) % N1] * qw[l / N2];
+      const double wk = wq * (a.coef_kind == IGA_COEF_POINTS ? coef_r : a.coef_const);
+      sign = wk < 0.0 ? 0x80000000u : 0u;
+      // (s J^-1 / W)[c][k] at 3 c + k
+      const double sc = sqrt(fabs(wk)) * iW / det;
+      double A[3][3];
+      A[0][0] = c00 * sc;
+      A[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * sc;
+      A[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * sc;
+      A[1][0] = c01 * sc;
+      A[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * sc;
+      A[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * sc;
+      A[2][0] = c02 * sc;
+      A[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * sc;
+      A[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * sc;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rec[3 * c + k] = A[c][k];
+        rec[9 + c] = (A[c][0] * G[1] + A[c][1] * G[2] + A[c][2] * G[3]) * iW;
+      }
+      rec[12] = iW * (a.src_kind == IGA_SRC_POINTS  ? wq * src_r
+                      : a.src_kind == IGA_SRC_CONST ? wq * a.src_const[0]
+                                                    : 0.0);
+    }
+#pragma unroll
+    for (int t = 0; t < 13; ++t) gq[t] = rec[t];
+    // table offsets (in doubles) of the point in each direction, the sign
+    int* ofs = reinterpret_cast<int*>(gq + 13);
+    ofs[0] = (qc % N1) * 2;
+    ofs[1] = ((qc / N1) % N1) * N1 * 2;
+    ofs[2] = (qc / N2) * N1 * 2;
+    ofs[3] = (int)sign;
+  }
+#pragma unroll
+  for (int it = l; it < NF * N1; it += 64) {
+    const int f = it / N1, q0 = it % N1;
+    const int fc = f < NL ? f : 0;
+    const double wf = f < NL ? ctl[fc] : 0.0;
+    t1b[it * 2] = wf * TAB(0, q0, fc % N1)[0];
+    t1b[it * 2 + 1] = wf * TAB(0, q0, fc % N1)[1];
+  }
+  wave_lds_sync();
+
+  // ---- gradient products on the matrix cores --------------------------------
+  const int r16 = l & 15, g = l >> 4;
+  // the lane's functions: weighted first-direction factors, second- and
+  // third-direction tables
+  const double* fb[TI];
+  const double* tb1[TI];
+  const double* tb2[TI];
+#pragma unroll
+  for (int u = 0; u < TI; ++u) {
+    const int f = r16 + 16 * u;
+    const int fc = f < NL ? f : 0;
+    fb[u] = t1b + f * N1 * 2;
+    tb1[u] = TAB(1, 0, (fc / N1) % N1);
+    tb2[u] = TAB(2, 0, fc / N2);
+  }
+  typename Mfma<T>::acc_t acc[NTILE];
+#pragma unroll
+  for (int n = 0; n < NTILE; ++n) acc[n] = Mfma<T>::zero();
+  double ld[TI];
+#pragma unroll
+  for (int u = 0; u < TI; ++u) ld[u] = 0.0;
+  // with no negative stiffness weight in the element (the usual case) both
+  // operands are the same registers
+  auto products = [&](auto has_neg) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      // padded points (q >= NQ) carry a zero record: the operands vanish there
+      const double* rc = geo + (4 * t + g) * REC;
+      const int* ofs = reinterpret_cast<const int*>(rc + 13);
+      const int o0 = ofs[0], o1 = ofs[1], o2 = ofs[2];
+      const uint32_t sg = (uint32_t)ofs[3];
+      const double ws = rc[12];
+      T ga[TI][3], gw[TI][3];
+#pragma unroll
+      for (int u = 0; u < TI; ++u) {
+        const double a0 = fb[u][o0], e0 = fb[u][o0 + 1];
+        const double v1 = tb1[u][o1], e1 = tb1[u][o1 + 1];
+        const double v2 = tb2[u][o2], e2 = tb2[u][o2 + 1];
+        const double p0 = v1 * v2;
+        const double Nw = a0 * p0;  // w N
+        const double d0 = e0 * p0, d1 = a0 * (e1 * v2), d2 = a0 * (v1 * e2);  // w dN
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double gc =
+              rc[3 * c] * d0 + rc[3 * c + 1] * d1 + rc[3 * c + 2] * d2 - Nw * rc[9 + c];
+          ga[u][c] = (T)gc;
+          if constexpr (decltype(has_neg)::value) gw[u][c] = flip_sign(ga[u][c], sg);
+          else gw[u][c] = ga[u][c];
+        }
+        ld[u] += Nw * ws;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        int n = 0;
+#pragma unroll
+        for (int I = 0; I < TI; ++I)
+#pragma unroll
+          for (int J = I; J < TI; ++J, ++n) acc[n] = Mfma<T>::step(ga[I][c], gw[J][c], acc[n]);
+      }
+    }
+  };
+  const bool neg = l < NT * 4 && reinterpret_cast<const int*>(geo + (l < NT * 4 ? l : 0) * REC + 13)[3] != 0;
+  if (__ballot(neg) != 0)
+    products(std::true_type{});
+  else
+    products(std::false_type{});
+
+  // ---- load vector: sum the four point groups of each function --------------
+  if (rhsp && a.src_kind != IGA_SRC_NONE) {
+#pragma unroll
+    for (int u = 0; u < TI; ++u) {
+      ld[u] += __shfl_xor(ld[u], 16);
+      ld[u] += __shfl_xor(ld[u], 32);
+    }
+    if (g == 0) {
+#pragma unroll
+      for (int u = 0; u < TI; ++u) {
+        const int f = r16 + 16 * u;
+        if (f >= NL) continue;
+        const int r = s_fdof[w][f];
+        if (r >= 0) unsafeAtomicAdd(rhsp + r, ld[u]);
+      }
+    }
+  }
+
+  // ---- element matrix: staged in LDS, streamed out; diagonal, lifting --------
+  wave_lds_sync();  // every read of the geometry and tables precedes this
+  T* kl = reinterpret_cast<T*>(s_buf[w]);
+  {
+    int n = 0;
+#pragma unroll
+    for (int I = 0; I < TI; ++I)
+#pragma unroll
+      for (int J = I; J < TI; ++J, ++n) {
+        const int j = 16 * J + r16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * I + Mfma<T>::row(g, r);
+          if (i < NL && j < NL) {
+            kl[i * NL + j] = acc[n][r];
+            if (I != J) kl[j * NL + i] = acc[n][r];
+          }
+        }
+      }
+  }
+  wave_lds_sync();
+  if (kout)
+    for (int t = l; t < NL * NL; t += 64) kout[t] = kl[t];
+  if (diagp && l < NL) {
+    const int rr = s_fdof[w][l];
+    if (rr >= 0) unsafeAtomicAdd(diagp + rr, (double)kl[l * NL + l]);
+  }
+  const bool lift = rhsp != nullptr && a.gfull != nullptr;
+  if (lift && __ballot(l < NL && s_fdof[w][l < NL ? l : 0] < 0) != 0) {
+    for (int t = l; t < NL * NL; t += 64) {
+      const int i = t / NL, j = t - i * NL;
+      const int ri = s_fdof[w][i];
+      if (ri < 0 || s_fdof[w][j] >= 0) continue;
+      const double gv = a.gfull[s_gdof[w][j]];
+      if (gv != 0.0) unsafeAtomicAdd(rhsp + ri, -(double)kl[t] * gv);
+    }
+  }
+}
+
 // Pass 2: vals[k] = sum over the elements shared by row and column of the
 // matching element-matrix entries (fixed order -> bitwise reproducible).
 template <int DIM, int P, int PROB, typename T>
@@ -1197,7 +1658,8 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
   constexpr bool lds_combine = PROB == IGA_ELASTICITY && NLT == 16 && DIM == 2;
   const size_t lds = sizeof(double) * (2 * DIM * nq1 * N1 + nq * NL + nq) +
                      sizeof(T) * (2 * DIM * nqp * NLT) + sizeof(int) * (NL * 3 + NL + 2) +
-                     (lds_combine ? sizeof(double) * DIM * DIM * 256 : 0) + 16;
+                     (lds_combine ? sizeof(double) * DIM * DIM * 256 : 0) +
+                     sizeof(double) * (NL * (DIM + 1) + nq * (2 + DIM + DIM * DIM)) + 16;
   if (lds > 64 * 1024) return IGA_ERR_UNSUPPORTED;
   int n_e = 1;
   for (int d = 0; d < DIM; ++d) n_e *= pt.n_elem[d];
@@ -1229,6 +1691,19 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
         else
           hipLaunchKernelGGL((k_element_w<P, PROB, T, false>), grid, dim3(256), 0, st, pt, nq1,
                              rule.pts, rule.wts, ae, ke, n_e);
+        done = true;
+      }
+    }
+    if constexpr (DIM == 3 && P <= 2 && PROB == IGA_POISSON) {
+      // scalar Poisson, p + 1 Gauss points per direction: one wave per element
+      if (nq1 == P + 1 && a.vec == 1) {
+        const dim3 grid((unsigned)((n_e + 3) / 4), batch);
+        if (pt.regular)
+          hipLaunchKernelGGL((k_element_w3<P, T, true>), grid, dim3(256), 0, st, pt, rule.pts,
+                             rule.wts, ae, ke, n_e);
+        else
+          hipLaunchKernelGGL((k_element_w3<P, T, false>), grid, dim3(256), 0, st, pt, rule.pts,
+                             rule.wts, ae, ke, n_e);
         done = true;
       }
     }

@@ -59,75 +59,6 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-// Rational basis and physical gradients at one point for the lane holding
-// basis function i of a group of NLP lanes (lanes >= n_l pass active=false).
-// J[a][c] = d x_c / d xi_a; dR_phys solves J dR_phys = dR_param.
-template <int DIM, int NLP>
-struct PointGeo {
-  double R;
-  double dR[DIM];
-  double x[DIM];
-  double J[DIM][DIM];
-  double detJ;
-};
-
-template <int DIM, int NLP>
-__device__ __forceinline__ void rational_pushforward(bool active, double N, const double* dN,
-                                                     double wgt, const double* P,
-                                                     PointGeo<DIM, NLP>& g) {
-  const double M = active ? N * wgt : 0.0;
-  double dM[DIM];
-#pragma unroll
-  for (int k = 0; k < DIM; ++k) dM[k] = active ? dN[k] * wgt : 0.0;
-  double S0 = group_sum<NLP>(M);
-  double S1[DIM], X[DIM], Y[DIM][DIM];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) X[c] = group_sum<NLP>(M * P[c]);
-#pragma unroll
-  for (int k = 0; k < DIM; ++k) {
-    S1[k] = group_sum<NLP>(dM[k]);
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) Y[k][c] = group_sum<NLP>(dM[k] * P[c]);
-  }
-  const double iW = 1.0 / S0;
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) g.x[c] = X[c] * iW;
-#pragma unroll
-  for (int k = 0; k < DIM; ++k)
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) g.J[k][c] = (Y[k][c] - S1[k] * g.x[c]) * iW;
-  g.R = M * iW;
-  double dRp[DIM];
-#pragma unroll
-  for (int k = 0; k < DIM; ++k) dRp[k] = (dM[k] - g.R * S1[k]) * iW;
-  const auto& J = g.J;
-  if constexpr (DIM == 2) {
-    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double id = 1.0 / det;
-    g.detJ = det;
-    g.dR[0] = (J[1][1] * dRp[0] - J[0][1] * dRp[1]) * id;
-    g.dR[1] = (-J[1][0] * dRp[0] + J[0][0] * dRp[1]) * id;
-  } else {
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const double id = 1.0 / det;
-    g.detJ = det;
-    // inverse = adj(J)^T / det ; inv[c][k]
-    const double i00 = c00 * id, i10 = c01 * id, i20 = c02 * id;
-    const double i01 = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-    const double i11 = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-    const double i21 = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-    const double i02 = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-    const double i12 = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-    const double i22 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-    g.dR[0] = i00 * dRp[0] + i01 * dRp[1] + i02 * dRp[2];
-    g.dR[1] = i10 * dRp[0] + i11 * dRp[1] + i12 * dRp[2];
-    g.dR[2] = i20 * dRp[0] + i21 * dRp[1] + i22 * dRp[2];
-  }
-}
-
 // Block-wide FP64 sum (blockDim.x a multiple of 64, <= 1024); result valid in
 // thread 0.
 __device__ __forceinline__ double block_sum(double v, double* scratch32) {

@@ -3,10 +3,12 @@
 //
 // CG keeps every scalar on the device (no host round trip per iteration):
 // each iteration is three kernels -- SpMV fused with p.Ap, the x/r/z update
-// fused with r.z and r.r, and the direction update -- whose dot products are
-// reduced deterministically (fixed block order) by the last block to finish.
-// Kernels read a device `done` flag, so the host enqueues iterations in
-// batches and synchronises only once per batch.
+// fused with r.z and r.r, and the direction update. Each kernel writes one
+// partial sum per block; every block of the consuming kernel re-reduces the
+// partials in the same fixed order, so the scalars are deterministic and no
+// block waits for another. The single-reduction variant (Chronopoulos-Gear)
+// needs two kernels per iteration. Kernels read a device `done` flag, so the
+// host enqueues iterations in batches and synchronises only once per batch.
 #include <algorithm>
 #include <mutex>
 #include <type_traits>

@@ -2,8 +2,10 @@
 
 Bars: integer structure (free/prescribed sets, CSR indptr/columns) bit-exact;
 FP64 basis, matrix entries and load within 1e-12 relative to their scale
-(summation order differs: the GPU scatters with atomics, like the reference's
-XLA scatter-add); CG iterates within 1e-9 of the oracle's solution.
+(summation order differs: the GPU forms the matrix values by a fixed-order
+gather of element matrices and sums the load with atomics, where the oracle
+follows the reference's scatter-add); CG iterates within 1e-9 of the oracle's
+solution.
 """
 
 import numpy as np
@@ -82,6 +84,26 @@ def _oracle_system(case):
     return dm, pat, data, load[dm.free]
 
 
+def rational_cube(deg, ref=2):
+    """Unit cube with interior weights perturbed by up to +-30 % (rational
+    basis, curved parametrisation), a0 = 1 + x0 + 2 x1 x2, a varying source and
+    an affine Dirichlet value, so the lifting -K_fp g is non-zero."""
+    import dataclasses
+
+    P, O = C.patches("cube", deg, ref)
+    rng = np.random.default_rng(40 + deg)
+    w = O.weights * (1.0 + 0.3 * rng.uniform(-1.0, 1.0, size=O.weights.shape))
+    P = dataclasses.replace(P, weights=w)
+    O = dataclasses.replace(O, weights=w.copy())
+    a0 = lambda x, p: 1.0 + x[0] + 2.0 * x[1] * x[2]  # noqa: E731
+    src = lambda x, p: 1.0 + x[0] * x[1] - x[2]  # noqa: E731
+    g = lambda x, p: 0.5 + x[0] - 2.0 * x[1] + 3.0 * x[2]  # noqa: E731
+    V = jx.FunctionSpace(P)
+    prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(g, where=C.everywhere)], source=src)
+    return C.Case(f"poisson-cube-p{deg}-r{ref}-rational", "poisson", V, OS.function_space(O), prob,
+                  [C.OF.DirichletBC(g, C.everywhere)], [], src, {"a0": a0})
+
+
 SYSTEM_CASES = [
     lambda: C.poisson_case("square", 3, 3),
     lambda: C.poisson_case("square", 2, 3, a0=2.5),
@@ -90,6 +112,8 @@ SYSTEM_CASES = [
     lambda: C.poisson_case("square", 3, 2, g=lambda x, p: 1.0 + 2.0 * x[0] - 3.0 * x[1]),
     lambda: C.elasticity_case(3, 2, "strain"),
     lambda: C.elasticity_case(2, 2, "stress"),
+    lambda: rational_cube(2),
+    lambda: rational_cube(1, 3),
 ]
 
 
@@ -111,6 +135,21 @@ def test_assembled_system_matches_oracle(gpu, make):
     assert rel(rhs.cpu(), b) < TOL_FP64
     K = OA.to_scipy(pat, data)
     assert rel(diag.cpu(), K.diagonal()) < TOL_FP64
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_sign_changing_coefficient_assembly(gpu, deg):
+    """a0 = x0 - 0.4 changes sign inside the cube: the element matrices of
+    elements with negative stiffness weights take the signed operand path."""
+    case = rational_cube(deg)
+    case.params = {"a0": lambda x, p: x[0] - 0.4}
+    print(f"[system] {case.name} sign-changing a0", flush=True)
+    dm, pat, data, b = _oracle_system(case)
+    ctx = M.build_context(case.problem, case.params)
+    vals, rhs, diag = M.assemble(ctx, case.params, diagonal=True)
+    assert rel(vals.cpu(), data) < TOL_FP64
+    assert rel(rhs.cpu(), b) < TOL_FP64
+    assert rel(diag.cpu(), OA.to_scipy(pat, data).diagonal()) < TOL_FP64
 
 
 @pytest.mark.parametrize("split", [False, True])

@@ -17,13 +17,14 @@ from jaxiga_amd import precision as PR
 from oracle import assembly as OA
 from oracle import solver as OSV
 from tests import cases as C
-from tests.test_gpu_parity import _oracle_system
+from tests.test_gpu_parity import _oracle_system, rational_cube
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("make", [lambda: C.elasticity_case(3, 3, "strain"),
-                                  lambda: C.poisson_case("annulus", 3, 3)])
+                                  lambda: C.poisson_case("annulus", 3, 3),
+                                  lambda: rational_cube(2, 3)])
 def test_fp32_assembly_entries_and_refined_solution(gpu, make):
     case = make()
     print(f"[mixed-csr] {case.name}", flush=True)
