@@ -70,6 +70,14 @@ def elasticity_flops_per_element(n_q=16, n_s=3, n_lv=32):
     return 2 * n_q * (n_s * n_s * n_lv + n_s * n_lv * n_lv)
 
 
+def hbm_roofline(kernel, nbytes, ms, algorithmic, traffic=None):
+    """Roofline entry of a bandwidth-bound kernel: algorithmic bytes per launch
+    over its measured duration, against the HBM peak."""
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "kernel": kernel,
+            "kernel_ms": ms, "algorithmic": algorithmic, "frac": achieved / PEAK_HBM_GBS, "traffic": traffic}
+
+
 def event_ms(launch, batches=5, per=10):
     """Average duration (ms) of one launch: batches of ``per`` back-to-back
     launches on the current stream between two HIP events, each batch
@@ -377,9 +385,20 @@ def matfree_bench(refine=6, reps=5, log=print):
     bn = LIN.residual_norms(pattern, vals, b * 0, b)[1]
     rel = {k: LIN.residual_norms(pattern, vals, r.x, b)[0] / bn for k, r in
            (("fp64", mfr), ("fp32", ref32), ("bf16", ref16))}
+    traffic = measured_traffic(os.path.join(ROOT, "profiles", "matfree_cube_64.md"))
+    roof = {f"matfree_{k}": hbm_roofline(
+        f"k_mfs_apply + k_mfs_gather <3,2,{k}>", mf_bytes[k], t * 1e3,
+        f"{n_e} elements x {n_q} points x {coef_bytes[k]} B coefficient + x read and y written once "
+        f"({n} x 8 B each)", traffic.get(f"k_mfs_apply+k_mfs_gather<3,2,{k}>")) for k, t in t_apply.items()}
+    roof["csr_fp64"] = hbm_roofline(
+        "k_spmv<16,f64>", csr_bytes, t_csr * 1e3,
+        f"{pattern.nnz} nonzeros x (8 B value + 4 B column) + row pointers + x read and y written once",
+        measured_traffic(os.path.join(ROOT, "profiles", "spmv_csr.md")).get("k_spmv<16,f64,cube-64>"))
     out = {
         "workload": f"unit-cube Poisson, p=2, {2**refine}^3 elements, a=exp(GRF l=0.1 sigma=1), {n} free dofs, "
                     f"{pattern.nnz} nonzeros",
+        "roofline": roof["matfree_fp32"],
+        "roofline_apply": roof,
         "apply_seconds": {"csr_fp64": t_csr, **{f"matfree_{k}": t for k, t in t_apply.items()}},
         "apply_algorithmic_GBs": {"csr_fp64": csr_bytes / t_csr / 1e9,
                                   **{f"matfree_{k}": mf_bytes[k] / t / 1e9 for k, t in t_apply.items()}},
@@ -742,7 +761,7 @@ def main():
 
     # -- time to solve (rel. residual 1e-10): FP64 Jacobi-CG, and FP32 operator
     # -- with FP64 iterative refinement (outer residual on the FP64 operator)
-    solve = None
+    solve = spmv_roof = None
     if not args.no_solve:
         from jaxiga_amd import precision as PR
 
@@ -756,6 +775,16 @@ def main():
         torch.cuda.synchronize()
         t_solve = time.perf_counter() - t1
         rn, bn = LIN.residual_norms(pattern, v64, res.x, rhs)
+        # the CSR SpMV of the solve against HBM: FP64 values, column indices
+        # and row pointers read once, x read and y written once
+        xs = torch.randn(ctx.n_free, dtype=torch.float64, device="cuda")
+        t_sp = event_ms(lambda: LIN.spmv(pattern, v64, xs))
+        sp_bytes = pattern.nnz * 12 + (ctx.n_free + 1) * 4 + 2 * ctx.n_free * 8
+        spmv_roof = hbm_roofline(
+            "k_spmv<16,f64>", sp_bytes, t_sp,
+            f"{pattern.nnz} nonzeros x (8 B value + 4 B column) + {ctx.n_free + 1} row pointers x 4 B "
+            f"+ x read and y written once ({ctx.n_free} x 8 B each)",
+            measured_traffic(os.path.join(ROOT, "profiles", "spmv_csr.md")).get("k_spmv<16,f64,elasticity-256>"))
         # FP32 operator, FP64 vectors, residual replaced with the FP64 one
         PR.csr_mixed_cg(pattern, v64, v32, rhs, dinv, tol=1e-10, maxiter=50, workspace=work)
         torch.cuda.synchronize()
@@ -812,6 +841,7 @@ def main():
         "roofline": dominant,
         "roofline_secondary": secondary,
         "roofline_assembly": main_run["roofline_assembly"],
+        "roofline_spmv": spmv_roof,
         "cpu_baseline": cpu,
         "time_to_solve": solve,
         f"assembly_{other}": {k: side_run[k] for k in ("value", "ms_per_step", "dtype", "roofline",
