@@ -421,32 +421,101 @@ def matfree_bench(refine=6, reps=5, log=print):
     return out
 
 
-def batched_bench(n_instances=512, log=print):
+def batched_bench(n_instances=512, cpu=True, log=print):
     """cfg4: 512 uneven unit-square Poisson instances (p in {2,3}, 16..128
-    elements per side, a = exp(GRF), source 1): batched assembly plus one
-    persistent batched Jacobi-CG launch, tiles from the autotuner, replayed
-    from a captured graph; instances per second and the worst relative
-    residual."""
+    elements per side, a = exp(GRF), source 1): batched assembly plus the
+    batched Jacobi-CG, method and tiles from the autotuner (the segmented
+    solve eagerly, the persistent one replayed from a captured graph);
+    instances per second, the worst relative residual, the solve's CSR
+    traffic against HBM, and the oracle's per-instance time on the CPU."""
     import torch
 
     from jaxiga_amd import batched as BT
 
     bp = BT.BatchedPoisson(BT.cfg4_specs(n_instances))
     best, timings = bp.autotune()
-    graph = bp.capture(tol=1e-10)
-    graph.replay()
+    if bp.method == "persistent":
+        graph = bp.capture(tol=1e-10)
+        job = graph.replay
+    else:
+        def job():
+            bp.run(tol=1e-10)
+    job()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    graph.replay()
+    job()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # the solve alone (assembled operators resident)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    bp.solve(1e-10)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t1
     iters, rel = bp.results()
+    # algorithmic bytes of the solve: per instance and iteration the CSR
+    # (8 B value + 4 B column per nonzero, row pointers) and the 13 vector
+    # streams of one Jacobi-PCG iteration (FP64)
+    n_rows = bp.n_rows.astype(np.int64)
+    per_it = bp.nnz * 12 + (n_rows + 1) * 4 + 13 * n_rows * 8
+    solve_bytes = int((iters.astype(np.int64) * per_it).sum())
+    roof = hbm_roofline(f"batched Jacobi-CG ({bp.method})", solve_bytes, t_solve * 1e3,
+                        "sum over instances of iterations x (12 B per nonzero + row pointers + 13 x 8 B "
+                        "per row)")
     log(f"[bench] cfg4: {n_instances} instances in {dt * 1e3:.1f} ms ({n_instances / dt:.0f} instances/s), "
-        f"tiles {best}, max rel.res {float(rel.max()):.2e}, iterations {int(iters.min())}..{int(iters.max())}")
-    return {"workload": f"{n_instances} unit-square Poisson instances, p in {{2,3}}, 16..128 elements per side",
-            "seconds": dt, "instances_per_second": n_instances / dt, "lanes_per_row_threads": list(best),
-            "max_relative_residual": float(rel.max()), "iterations_min_max": [int(iters.min()), int(iters.max())],
-            "elements": bp.n_elements}
+        f"solve {t_solve * 1e3:.1f} ms ({roof['achieved']:.0f} GB/s), method {best}, "
+        f"max rel.res {float(rel.max()):.2e}, iterations {int(iters.min())}..{int(iters.max())}")
+    out = {"workload": f"{n_instances} unit-square Poisson instances, p in {{2,3}}, 16..128 elements per side",
+           "seconds": dt, "instances_per_second": n_instances / dt, "method_lanes_per_row_threads": list(best),
+           "solve_seconds": t_solve, "roofline": roof,
+           "max_relative_residual": float(rel.max()), "iterations_min_max": [int(iters.min()), int(iters.max())],
+           "elements": bp.n_elements}
+    if cpu:
+        out["cpu_baseline"] = batched_cpu_baseline(bp, n_instances)
+    return out
+
+
+def batched_cpu_baseline(bp, n_instances):
+    """The oracle's FP64 assembly (basis, element matrices, CSR, load) and
+    Jacobi-PCG to 1e-10 of one instance of each (degree, mesh) class of the
+    batch, one core, with the batch's coefficient values as input; the batch
+    time is the class times weighted by the class sizes."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import assembly as OA
+    from oracle import forms as OF
+    from oracle import geometry as OG
+    from oracle import solver as OSV
+    from oracle import space as OS
+
+    everywhere = lambda x: np.ones(x.shape[0], bool)  # noqa: E731
+    total, per_class = 0.0, {}
+    with threadpool_limits(1):
+        for g in bp.groups:
+            p, n_el = g.degree, g.n_el
+            coef = g.coef[0].cpu().numpy()
+            t0 = time.perf_counter()
+            O = OG.quadrilateral([[0, 0], [1, 0], [1, 1], [0, 1]]).elevate(p).refine(int(round(np.log2(n_el))))
+            VO = OS.function_space(O)
+            presc, vals = OF.resolve_dirichlet(VO, [OF.DirichletBC(0.0, everywhere)])
+            dm = OS.make_dofmap(VO.n_dofs, presc, vals)
+            basis = OS.gauss_basis(VO)
+            params = {"a0": OF.PointField(coef)}
+            Ke = OF.poisson_tangents(basis, params)
+            pat = OA.csr_pattern(VO.elem_dofs, VO.n_dofs, 1, dm.free)
+            data = OA.assemble_csr(pat, Ke)
+            gd = OF.gdofs(VO)
+            load = OA.global_load(VO.n_dofs, gd, OF.source_load(basis, lambda x, q: 1.0, params, 1))
+            b = (load - OA.lifting(VO.n_dofs, gd, Ke, dm))[dm.free]
+            K = OA.to_scipy(pat, data)
+            OSV.jacobi_pcg(lambda v: K @ v, b, K.diagonal(), tol=1e-10)
+            t = time.perf_counter() - t0
+            per_class[f"p{p}-n{n_el}"] = t
+            total += t * len(g.members)
+    return {"value": n_instances / total, "unit": "instances/s", "cores": 1, "kind": "port",
+            "sample": "oracle FP64 assembly + Jacobi-PCG to 1e-10 of one instance per (degree, mesh) class "
+                      "(" + ", ".join(f"{k} {v:.2f} s" for k, v in per_class.items()) + "), weighted by the "
+                      "class sizes of the batch"}
 
 
 def adjoint_bench(refine=8, reps=3, log=print):
@@ -742,7 +811,7 @@ def main():
         print(json.dumps({"adjoint": adj}), flush=True)
         return
     if args.extras_only:
-        print(json.dumps({"matfree": matfree_bench(), "batched": batched_bench()}), flush=True)
+        print(json.dumps({"matfree": matfree_bench(), "batched": batched_bench(cpu=not args.no_cpu)}), flush=True)
         return
     V, prob, params = build_problem()
     ctx = M.build_context(prob, params)
@@ -813,7 +882,8 @@ def main():
     learned = None if (args.no_learned or args.no_solve) else learned_pc_bench(
         n_train=args.learned_fields, steps=args.learned_steps,
         test_seeds=tuple(range(10_000, 10_000 + args.learned_seeds)), cpu=not args.no_cpu)
-    extras = {} if (args.no_extras or args.no_solve) else {"matfree": matfree_bench(), "batched": batched_bench()}
+    extras = {} if (args.no_extras or args.no_solve) else {"matfree": matfree_bench(),
+                                                           "batched": batched_bench(cpu=not args.no_cpu)}
     if not (args.no_extras or args.no_solve):
         adj = adjoint_bench()
         if not args.no_cpu:

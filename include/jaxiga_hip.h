@@ -51,6 +51,9 @@
  *                             solvers/linear.py:113-116)
  *   iga_batched_cg            (no reference counterpart: the reference batches by
  *                             vmap over params, tests/test_advanced.py:337-344)
+ *   iga_batched_cg_seg / _plan / _workspace
+ *                             (the same batched solve with every instance's rows
+ *                             spread over the GPU; no reference counterpart)
  *   iga_rff_eval              (no reference counterpart: seeded GRF inputs)
  */
 #ifndef JAXIGA_HIP_H
@@ -673,6 +676,29 @@ int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32
 int iga_batched_cg(int32_t n_inst, const void* insts, const int32_t* order,
                    int32_t lanes_per_row, int32_t threads, double tol, int32_t maxiter,
                    int32_t* out_iters, double* out_rel, void* stream);
+
+/* Segmented batched Jacobi-PCG, the same iteration and outputs as
+ * iga_batched_cg (FP64 recurrences, true-residual restarts, at most three),
+ * with the rows of every instance cut into blocks of rows_per_block rows,
+ * one 256-thread workgroup each: every kernel covers the blocks of all
+ * instances, so a large instance spreads over the whole GPU; three launches
+ * per iteration for the batch; a converged instance's blocks return at once.
+ * The host enqueues 16 iterations between reads of the active-instance count
+ * (not graph-capturable; concurrent calls are serialised).
+ * iga_batched_cg_seg_plan (host only) fills the block map from the host row
+ * counts n_rows[n_inst]: blk_inst[b] (instance of block b), blk_row0[b]
+ * (its first row) and inst_blk0[i] (first block of instance i, n_inst + 1
+ * entries); it returns the number of blocks (any output may be null to count
+ * only). The three arrays are passed to the solve in device memory.
+ * iga_batched_cg_seg_workspace: bytes of `work` (64-byte aligned). */
+int32_t iga_batched_cg_seg_plan(int32_t n_inst, const int32_t* n_rows, int32_t rows_per_block,
+                                int32_t* blk_inst, int32_t* blk_row0, int32_t* inst_blk0);
+int64_t iga_batched_cg_seg_workspace(int32_t n_inst, int32_t n_blocks);
+int iga_batched_cg_seg(int32_t n_inst, const void* insts, int32_t n_blocks,
+                       const int32_t* blk_inst, const int32_t* blk_row0,
+                       const int32_t* inst_blk0, int32_t rows_per_block,
+                       int32_t lanes_per_row, double tol, int32_t maxiter, void* work,
+                       int64_t work_bytes, int32_t* out_iters, double* out_rel, void* stream);
 
 /* Seeded random-Fourier-feature field g(x) = amp * sum_m cos(omega_m . x + phase_m)
  * at n points (n, dim) (device, FP64 throughout). */

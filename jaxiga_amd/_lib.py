@@ -217,6 +217,10 @@ _SIGNATURES = {
                            c_void_p]),
     "iga_batched_cg": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_double, c_int, c_void_p,
                                c_void_p, c_void_p]),
+    "iga_batched_cg_seg_plan": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "iga_batched_cg_seg_workspace": (c_i64, [c_int, c_int]),
+    "iga_batched_cg_seg": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                   c_double, c_int, c_void_p, c_i64, c_void_p, c_void_p, c_void_p]),
     "iga_rff_eval": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_double,
                              c_void_p, c_void_p]),
 }

@@ -8,12 +8,17 @@ degrees, mesh sizes and coefficient fields is grouped by shape (degree, mesh):
 * per shape, one element-pass launch and one gather launch assemble every
   instance of the group against the shared CSR pattern (``iga_assemble`` with
   ``batch > 1``), the diagonal coming out of the same pass;
-* one persistent launch (``iga_batched_cg``) solves all instances of the batch,
-  one workgroup per instance, largest first, FP64 Jacobi-PCG to ``tol``;
-* the lanes-per-row and threads-per-workgroup of the solve are chosen by a
-  small autotuner that times the candidates on the batch itself;
-* the whole batch pipeline can be captured in a hipGraph (via a torch CUDA
-  graph on a side stream) and replayed.
+* the batch is solved by FP64 Jacobi-PCG to ``tol`` in one of two ways:
+  ``segmented`` (``iga_batched_cg_seg``, the default): every instance's rows
+  are cut into blocks of ``ROWS_PER_BLOCK`` rows and each iteration is three
+  launches over the blocks of all instances, so the large instances stream
+  through the whole GPU; or ``persistent`` (``iga_batched_cg``): one
+  workgroup per instance runs its whole solve in one launch, largest first;
+* the method and its lanes per row (and threads per workgroup) are chosen by
+  a small autotuner that times the candidates on the batch itself;
+* the batch pipeline with the persistent solve can be captured in a hipGraph
+  (via a torch CUDA graph on a side stream) and replayed; the segmented solve
+  reads an active-instance count back between groups of iterations.
 """
 
 from __future__ import annotations
@@ -61,6 +66,7 @@ class _Group:
         prob = jx.Poisson(V, dirichlet=[jx.DirichletBC(0.0, where=lambda x: np.ones(len(x), bool))],
                           source=1.0)
         self.members = members
+        self.degree, self.n_el = degree, n_el
         self.ctx = M.build_context(prob)
         self.pattern = self.ctx.assembly
         B = len(members)
@@ -104,6 +110,8 @@ class _Group:
                                        stream()), "iga_inv_guard")
 
 
+ROWS_PER_BLOCK = 256
+
 INST_DTYPE = np.dtype([("n", "<i4"), ("pad", "<i4"), ("indptr", "<u8"), ("cols", "<u8"),
                        ("vals", "<u8"), ("dinv", "<u8"), ("b", "<u8"), ("x", "<u8"), ("work", "<u8")])
 
@@ -111,7 +119,7 @@ INST_DTYPE = np.dtype([("n", "<i4"), ("pad", "<i4"), ("indptr", "<u8"), ("cols",
 class BatchedPoisson:
     """A batch of uneven unit-square Poisson instances assembled and solved together."""
 
-    def __init__(self, specs, sigma=1.0, device="cuda"):
+    def __init__(self, specs, sigma=1.0, device="cuda", rows_per_block=ROWS_PER_BLOCK):
         import torch
 
         self.specs = specs
@@ -132,58 +140,90 @@ class BatchedPoisson:
         self.order = torch.as_tensor(np.argsort(-sizes, kind="stable").astype(np.int32), device=device)
         self.iters = torch.zeros(len(specs), dtype=torch.int32, device=device)
         self.rel = torch.zeros(len(specs), dtype=torch.float64, device=device)
-        self.lpr, self.threads = 16, 256
+        # segmented solve: block map and workspace
+        n_rows = np.ascontiguousarray(rec["n"], dtype=np.int32)
+        L = lib()
+        self.rows_per_block = int(rows_per_block)
+        nb = int(L.iga_batched_cg_seg_plan(len(specs), n_rows.ctypes.data, self.rows_per_block, None, None,
+                                           None))
+        if nb < 0:
+            raise RuntimeError(f"iga_batched_cg_seg_plan failed ({nb})")
+        blk_inst, blk_row0 = np.empty(nb, np.int32), np.empty(nb, np.int32)
+        inst_blk0 = np.empty(len(specs) + 1, np.int32)
+        L.iga_batched_cg_seg_plan(len(specs), n_rows.ctypes.data, self.rows_per_block, blk_inst.ctypes.data,
+                                  blk_row0.ctypes.data, inst_blk0.ctypes.data)
+        self.n_blocks = nb
+        self.seg_map = [torch.as_tensor(a, device=device) for a in (blk_inst, blk_row0, inst_blk0)]
+        self.seg_work = torch.empty(int(L.iga_batched_cg_seg_workspace(len(specs), nb)), dtype=torch.uint8,
+                                    device=device)
+        self.n_rows = n_rows
+        self.nnz = sizes
+        self.method, self.lpr, self.threads = "segmented", 8, 256
         self.n_elements = sum(g.ctx.space.n_elems * len(g.members) for g in self.groups)
 
     def assemble(self):
         for g in self.groups:
             g.assemble()
 
-    def solve(self, tol=1e-10, maxiter=20000, lpr=None, threads=None):
-        _lib.check(lib().iga_batched_cg(len(self.specs), _lib.ptr(self.insts), _lib.ptr(self.order),
-                                        lpr or self.lpr, threads or self.threads, float(tol), int(maxiter),
-                                        _lib.ptr(self.iters), _lib.ptr(self.rel), stream()),
-                   "iga_batched_cg")
+    def solve(self, tol=1e-10, maxiter=20000, lpr=None, threads=None, method=None):
+        method = method or self.method
+        if method == "segmented":
+            m = self.seg_map
+            _lib.check(lib().iga_batched_cg_seg(
+                len(self.specs), _lib.ptr(self.insts), self.n_blocks, _lib.ptr(m[0]), _lib.ptr(m[1]),
+                _lib.ptr(m[2]), self.rows_per_block, lpr or self.lpr, float(tol), int(maxiter),
+                _lib.ptr(self.seg_work), self.seg_work.numel(), _lib.ptr(self.iters), _lib.ptr(self.rel),
+                stream()), "iga_batched_cg_seg")
+        elif method == "persistent":
+            _lib.check(lib().iga_batched_cg(len(self.specs), _lib.ptr(self.insts), _lib.ptr(self.order),
+                                            lpr or self.lpr, threads or self.threads, float(tol),
+                                            int(maxiter), _lib.ptr(self.iters), _lib.ptr(self.rel),
+                                            stream()), "iga_batched_cg")
+        else:
+            raise ValueError(f"unknown batched solve method {method!r}")
 
-    def run(self, tol=1e-10):
+    def run(self, tol=1e-10, method=None):
         self.assemble()
-        self.solve(tol)
+        self.solve(tol, method=method)
 
     def autotune(self, candidates=None, probe_iters=40):
-        """Pick lanes-per-row and threads-per-workgroup of the batched solve by
-        timing ``probe_iters`` iterations of every candidate on this batch."""
+        """Pick the solve method and its lanes per row (and, for the
+        persistent solve, threads per workgroup) by timing ``probe_iters``
+        iterations of every candidate on this batch."""
         import torch
 
         if candidates is None:
-            candidates = list(itertools.product((8, 16, 32), (256, 512, 1024)))
+            candidates = [("segmented", lpr, 256) for lpr in (8, 16, 32)] + [
+                ("persistent", lpr, thr) for lpr, thr in itertools.product((8, 16, 32), (256, 512, 1024))]
         self.assemble()
         best, timings = None, {}
-        for lpr, thr in candidates:
-            self.solve(1e-30, probe_iters, lpr, thr)  # warm
+        for method, lpr, thr in candidates:
+            self.solve(1e-30, probe_iters, lpr, thr, method)  # warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            self.solve(1e-30, probe_iters, lpr, thr)
+            self.solve(1e-30, probe_iters, lpr, thr, method)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
-            timings[(lpr, thr)] = dt
+            timings[(method, lpr, thr)] = dt
             if best is None or dt < timings[best]:
-                best = (lpr, thr)
-        self.lpr, self.threads = best
+                best = (method, lpr, thr)
+        self.method, self.lpr, self.threads = best
         return best, timings
 
     def capture(self, tol=1e-10):
-        """A replayable graph of the whole batch (assembly + solve)."""
+        """A replayable graph of the whole batch (assembly + the persistent
+        solve, which needs no host synchronisation)."""
         import torch
 
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            self.run(tol)  # warm-up on the capture stream
+            self.run(tol, method="persistent")  # warm-up on the capture stream
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=side):
-            self.run(tol)
+            self.run(tol, method="persistent")
         return g
 
     def results(self):

@@ -1,5 +1,5 @@
-"""Batched assembly + persistent batched CG (the reference batches by vmap
-and checks batch == loop, tests/test_advanced.py:337-344).
+"""Batched assembly + batched CG, segmented and persistent (the reference
+batches by vmap and checks batch == loop, tests/test_advanced.py:337-344).
 
 * Every (degree, mesh) class of the 512-instance workload -- p in {2, 3} x
   {16, 32, 64, 128} elements per side, exp(GRF) coefficients -- against the
@@ -9,6 +9,8 @@ and checks batch == loop, tests/test_advanced.py:337-344).
   its scale (CG to 1e-12), the oracle's FP64 residual of the GPU solution
   <= 2e-12.
 * The full 512-instance batch: every independent FP64 residual <= 1e-10.
+* All of the above for both solve methods (segmented: every instance's rows
+  spread over the GPU; persistent: one workgroup per instance).
 * Every instance of an uneven batch matches the single-instance path.
 * The autotuner picks a valid candidate; hipGraph replays reproduce the eager
   result (the load vector is summed with atomics, so replays agree to
@@ -56,10 +58,14 @@ def _single(spec, sigma=1.0):
     return ctx, vals, rhs
 
 
-def test_batch_matches_single_instances(gpu):
+METHODS = ["segmented", "persistent"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_batch_matches_single_instances(gpu, method):
     specs = _specs()
     bp = BT.BatchedPoisson(specs)
-    bp.run(tol=1e-12)
+    bp.run(tol=1e-12, method=method)
     torch.cuda.synchronize()
     iters, rel = bp.results()
     assert (rel <= 1e-12).all(), rel
@@ -81,9 +87,12 @@ def test_batch_matches_single_instances(gpu):
 def test_autotune_and_graph_replay(gpu):
     specs = _specs()
     bp = BT.BatchedPoisson(specs)
-    best, timings = bp.autotune(candidates=[(8, 256), (16, 256), (16, 512), (32, 1024)], probe_iters=10)
-    assert best in timings and len(timings) == 4
-    bp.run(tol=1e-10)
+    cands = [("segmented", 8, 256), ("segmented", 16, 256), ("persistent", 8, 256), ("persistent", 16, 512),
+             ("persistent", 32, 1024)]
+    best, timings = bp.autotune(candidates=cands, probe_iters=10)
+    assert best in timings and len(timings) == 5
+    bp.lpr, bp.threads = 16, 512
+    bp.run(tol=1e-10, method="persistent")
     torch.cuda.synchronize()
     eager = [g.x.clone() for g in bp.groups]
     it_eager = bp.iters.clone()
@@ -130,17 +139,18 @@ def _oracle_instance(spec, a_gpu, sigma=1.0, n_check=4000):
     return pat, data, load[dm.free]
 
 
-def test_every_batch_class_matches_oracle(gpu):
+@pytest.mark.parametrize("method", METHODS)
+def test_every_batch_class_matches_oracle(gpu, method):
     rng = np.random.default_rng(11)
     specs = [BT.InstanceSpec(p, n, float(rng.uniform(0.05, 0.2)), 5000 + 10 * p + n)
              for p in (2, 3) for n in (16, 32, 64, 128)]
     bp = BT.BatchedPoisson(specs)
-    bp.run(tol=1e-12)
+    bp.run(tol=1e-12, method=method)
     torch.cuda.synchronize()
     iters, rel = bp.results()
     assert (rel <= 1e-12).all(), rel
     for i, s in enumerate(specs):
-        print(f"[batched-oracle] instance {i} p={s.degree} n={s.n_el} ell={s.ell:.3f}", flush=True)
+        print(f"[batched-oracle] {method} instance {i} p={s.degree} n={s.n_el} ell={s.ell:.3f}", flush=True)
         g, k = bp.instance(i)
         pat, data, b = _oracle_instance(s, g.coef[k].cpu().numpy())
         np.testing.assert_array_equal(g.pattern.indptr, pat.indptr)
@@ -158,12 +168,13 @@ def test_every_batch_class_matches_oracle(gpu):
         assert np.linalg.norm(b - K @ x) <= 2e-12 * np.linalg.norm(b)
 
 
-def test_full_batch_independent_residuals(gpu):
+@pytest.mark.parametrize("method", METHODS)
+def test_full_batch_independent_residuals(gpu, method):
     specs = BT.cfg4_specs(512)
     bp = BT.BatchedPoisson(specs)
     bp.lpr, bp.threads = 8, 1024
-    print("[batched-512] run", flush=True)
-    bp.run(tol=1e-10)
+    print(f"[batched-512] run {method}", flush=True)
+    bp.run(tol=1e-10, method=method)
     torch.cuda.synchronize()
     iters, rel = bp.results()
     assert (rel <= 1e-10).all(), float(rel.max())
