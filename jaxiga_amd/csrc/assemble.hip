@@ -527,7 +527,8 @@ __global__ void __launch_bounds__(256)
 template <int P, int PROB, typename T, bool REG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_element_w(iga_patch_t pt, int nq1, const double* __restrict__ qp,
-                const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all, int n_e) {
+                const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all, int n_e,
+                int e_begin, int e_end) {
   constexpr int N1 = P + 1;
   static_assert(N1 <= 4, "at most four functions per direction");
   constexpr int NL = N1 * N1;
@@ -546,8 +547,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
-  const int e = blockIdx.x * WPB + w;
-  if (e >= n_e) return;
+  // elements [e_begin, e_end) of the n_e of the patch
+  const int e = e_begin + blockIdx.x * WPB + w;
+  if (e >= e_end) return;
   const int64_t bi = blockIdx.y;
   const int nq = nq1 * nq1;
   const double* coefp = a.coef_points ? a.coef_points + bi * n_e * nq : nullptr;
@@ -1457,7 +1459,7 @@ __global__ void __launch_bounds__(256)
 // Write-out as in the row form, once per free component row.
 template <int DIM, int P, int PROB, typename T, bool REG>
 __global__ void __launch_bounds__(256)
-    k_gather_points(iga_patch_t pt, int32_t n_points, const int32_t* __restrict__ indptr,
+    k_gather_points(iga_patch_t pt, int32_t p_begin, int32_t p_end, const int32_t* __restrict__ indptr,
                     const int32_t* __restrict__ f2f, const T* __restrict__ ke_all,
                     T* __restrict__ vals_all, double* __restrict__ diag_all, int32_t n_free,
                     int64_t nnz, int64_t n_e) {
@@ -1475,8 +1477,9 @@ __global__ void __launch_bounds__(256)
   constexpr int CHUNKS = (BOXV + kWave - 1) / kWave;
   __shared__ double box[WAVES][GROUPS * VEC * BOXV];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
-  const int pnt = blockIdx.x * WAVES + wave;
-  if (pnt >= n_points) return;
+  // control points [p_begin, p_end)
+  const int pnt = p_begin + blockIdx.x * WAVES + wave;
+  if (pnt >= p_end) return;
   const int64_t b = blockIdx.y;
   const T* ke = ke_all + b * n_e * NLV * NLV;
   T* vals = vals_all + b * nnz;
@@ -1679,20 +1682,40 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
   const bool row_gather = args.vals && args.nnz > 0 && args.indptr && args.n_free > 0;
   AsmDev ae = a;
   if (row_gather) ae.diag = nullptr;
+  int n_pts = 1;
+  for (int d = 0; d < DIM; ++d) n_pts *= pt.n_cp[d];
+  // planar, at most 4 x 4 functions and Gauss points: one wave per element
+  const bool planar_w = DIM == 2 && P <= 3 && nq1 <= 4 && a.vec == S::VEC;
+  auto element_w = [&](int e_begin, int e_end, hipStream_t s) {
+    if constexpr (DIM == 2 && P <= 3) {
+      const dim3 grid((unsigned)((e_end - e_begin + 3) / 4), batch);
+      if (pt.regular)
+        hipLaunchKernelGGL((k_element_w<P, PROB, T, true>), grid, dim3(256), 0, s, pt, nq1,
+                           rule.pts, rule.wts, ae, ke, n_e, e_begin, e_end);
+      else
+        hipLaunchKernelGGL((k_element_w<P, PROB, T, false>), grid, dim3(256), 0, s, pt, nq1,
+                           rule.pts, rule.wts, ae, ke, n_e, e_begin, e_end);
+    }
+  };
+  const bool point_gather = row_gather && args.vec == S::VEC;
+  auto gather_points = [&](int p_begin, int p_end, hipStream_t s) {
+    const dim3 grid((unsigned)((p_end - p_begin + 3) / 4), batch);
+    if (pt.regular)
+      hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, true>), grid, dim3(256), 0, s, pt,
+                         p_begin, p_end, args.indptr, args.full_to_free, ke,
+                         reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
+                         (int64_t)n_e);
+    else
+      hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, false>), grid, dim3(256), 0, s, pt,
+                         p_begin, p_end, args.indptr, args.full_to_free, ke,
+                         reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
+                         (int64_t)n_e);
+  };
   if (passes & 1) {
     bool done = false;
-    if constexpr (DIM == 2 && P <= 3) {
-      // planar, at most 4 x 4 functions and Gauss points: one wave per element
-      if (nq1 <= 4 && a.vec == S::VEC) {
-        const dim3 grid((unsigned)((n_e + 3) / 4), batch);
-        if (pt.regular)
-          hipLaunchKernelGGL((k_element_w<P, PROB, T, true>), grid, dim3(256), 0, st, pt, nq1,
-                             rule.pts, rule.wts, ae, ke, n_e);
-        else
-          hipLaunchKernelGGL((k_element_w<P, PROB, T, false>), grid, dim3(256), 0, st, pt, nq1,
-                             rule.pts, rule.wts, ae, ke, n_e);
-        done = true;
-      }
+    if (planar_w) {
+      element_w(0, n_e, st);
+      done = true;
     }
     if constexpr (DIM == 3 && P <= 2 && PROB == IGA_POISSON) {
       // scalar Poisson, p + 1 Gauss points per direction: one wave per element
@@ -1713,20 +1736,8 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
     IGA_CHECK_LAUNCH();
   }
   if ((passes & 2) && args.vals && args.nnz > 0) {
-    int n_pts = 1;
-    for (int d = 0; d < DIM; ++d) n_pts *= pt.n_cp[d];
-    if (row_gather && args.vec == S::VEC) {
-      dim3 grid((unsigned)((n_pts + 3) / 4), batch);
-      if (pt.regular)
-        hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, true>), grid, dim3(256), 0, st, pt,
-                           n_pts, args.indptr, args.full_to_free, ke,
-                           reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
-                           (int64_t)n_e);
-      else
-        hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, false>), grid, dim3(256), 0, st, pt,
-                           n_pts, args.indptr, args.full_to_free, ke,
-                           reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
-                           (int64_t)n_e);
+    if (point_gather) {
+      gather_points(0, n_pts, st);
     } else if (row_gather) {
       const unsigned blocks = (unsigned)((args.n_free + 3) / 4);
       hipLaunchKernelGGL((k_gather_rows<DIM, P, PROB, T>), dim3(blocks, batch), dim3(256), 0, st,
