@@ -78,6 +78,35 @@ def hbm_roofline(kernel, nbytes, ms, algorithmic, traffic=None):
             "kernel_ms": ms, "algorithmic": algorithmic, "frac": achieved / PEAK_HBM_GBS, "traffic": traffic}
 
 
+def graph_ms(launch, per=20, reps=5):
+    """Average device time (ms) of one launch for calls too short for eager
+    timing (the host's enqueue time per call would exceed the kernel's):
+    ``per`` calls captured once in a graph, the replay timed between HIP
+    events on the current stream, the median over ``reps`` replays."""
+    import torch
+
+    launch()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        for _ in range(per):
+            launch()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        g.replay()
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / per)
+    del g
+    return statistics.median(times)
+
+
 def event_ms(launch, batches=5, per=10):
     """Average duration (ms) of one launch: batches of ``per`` back-to-back
     launches on the current stream between two HIP events, each batch
@@ -345,15 +374,7 @@ def matfree_bench(refine=6, reps=5, log=print):
     x = torch.randn(ctx.n_free, dtype=torch.float64, device="cuda")
 
     def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        ev[0].record()
-        for _ in range(reps):
-            fn()
-        ev[1].record()
-        torch.cuda.synchronize()
-        return ev[0].elapsed_time(ev[1]) / reps * 1e-3
+        return graph_ms(fn) * 1e-3
 
     t_csr = timed(lambda: LIN.spmv(pattern, vals, x))
     y_csr = LIN.spmv(pattern, vals, x)
@@ -847,13 +868,35 @@ def main():
         # the CSR SpMV of the solve against HBM: FP64 values, column indices
         # and row pointers read once, x read and y written once
         xs = torch.randn(ctx.n_free, dtype=torch.float64, device="cuda")
-        t_sp = event_ms(lambda: LIN.spmv(pattern, v64, xs))
+        t_sp = graph_ms(lambda: LIN.spmv(pattern, v64, xs))
         sp_bytes = pattern.nnz * 12 + (ctx.n_free + 1) * 4 + 2 * ctx.n_free * 8
         spmv_roof = hbm_roofline(
             "k_spmv<16,f64>", sp_bytes, t_sp,
             f"{pattern.nnz} nonzeros x (8 B value + 4 B column) + {ctx.n_free + 1} row pointers x 4 B "
             f"+ x read and y written once ({ctx.n_free} x 8 B each)",
             measured_traffic(os.path.join(ROOT, "profiles", "spmv_csr.md")).get("k_spmv<16,f64,elasticity-256>"))
+        # the matrix-free linear-elasticity operator of the same problem
+        # (rational patch: per Gauss point sqrt(w|det J|) J^-1 and W, grad W)
+        from jaxiga_amd import matfree as MF
+
+        y_csr = LIN.spmv(pattern, v64, xs)
+        mf_el = {"csr_spmv_ms": t_sp}
+        ops = {}
+        for name, prec, tb in (("fp64", _lib.IGA_FP64, 8), ("fp32", _lib.IGA_FP32, 4)):
+            op = ops[name] = MF.MatrixFreeOperator(ctx, params, prec)
+            t_ap = graph_ms(lambda op=op: op.apply(xs))
+            err = float((op.apply(xs) - y_csr).abs().max() / y_csr.abs().max())
+            ap_bytes = n_e * 16 * (4 + 3) * tb + 2 * ctx.n_free * 8
+            mf_el[name] = {"apply_ms": t_ap, "rel_err_vs_csr_fp64": err, "roofline": hbm_roofline(
+                f"k_mf_apply_el + k_mf_gather <2,3,{name}>", ap_bytes, t_ap,
+                f"{n_e} elements x 16 points x 7 factors x {tb} B (J^-1 scaled, W, grad W) + x read "
+                f"and y written once")}
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res_mf = ops["fp64"].cg(rhs, 1e-10, maxiter=20 * ctx.n_free, dinv=dinv)
+        torch.cuda.synchronize()
+        mf_el["solve_fp64_jacobi_cg"] = {"seconds": time.perf_counter() - t1, "iterations": res_mf.iterations}
+        del ops
         # FP32 operator, FP64 vectors, residual replaced with the FP64 one
         PR.csr_mixed_cg(pattern, v64, v32, rhs, dinv, tol=1e-10, maxiter=50, workspace=work)
         torch.cuda.synchronize()
@@ -873,11 +916,15 @@ def main():
                  "fp32_operator_fp64_refinement": {"seconds": refined.seconds, "outer": refined.outer,
                                                    "inner_iterations": refined.total_inner,
                                                    "relative_residual_fp64": rn_r / bn},
-                 "fp32_vs_fp64_stiffness_max_rel": stiff_err}
+                 "fp32_vs_fp64_stiffness_max_rel": stiff_err,
+                 "matrix_free_elasticity": mf_el}
         print(f"[bench] jacobi-cg fp64 {t_solve:.3f} s, {res.iterations} it, rel.res {rn / bn:.2e}; "
               f"fp32 + reliable updates {mixed.seconds:.3f} s, {mixed.iterations} it, "
               f"rel.res {rn_m / bn:.2e}, displacement diff {disp_err:.2e}; fp32 + refinement "
-              f"{refined.seconds:.3f} s ({refined.outer} outer, {refined.total_inner} inner)", flush=True)
+              f"{refined.seconds:.3f} s ({refined.outer} outer, {refined.total_inner} inner); matrix-free "
+              f"apply fp64 {mf_el['fp64']['apply_ms']:.4f} ms, fp32 {mf_el['fp32']['apply_ms']:.4f} ms "
+              f"(CSR {t_sp:.4f} ms), matrix-free CG {mf_el['solve_fp64_jacobi_cg']['seconds']:.3f} s "
+              f"({mf_el['solve_fp64_jacobi_cg']['iterations']} it)", flush=True)
 
     learned = None if (args.no_learned or args.no_solve) else learned_pc_bench(
         n_train=args.learned_fields, steps=args.learned_steps,

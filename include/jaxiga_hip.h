@@ -27,6 +27,8 @@
  *   iga_residual_norms        solvers/diagnostics.py:8-19  (independent ||Au-b||)
  *   iga_mf_setup/iga_mf_apply methods/galerkin.py:207-218 (tangent_matvec,
  *                             sum-factorised instead of a JVP)
+ *   iga_mf_setup_elasticity   methods/galerkin.py:207-218 (tangent_matvec of
+ *                             LinearElasticity, sum-factorised)
  *   iga_mf_diag               methods/galerkin.py:221-237 (tangent_diagonal)
  *   iga_cg_solve_mf           solvers/linear.py:414-442    (matrix_free_solve, cg)
  *   iga_diff_norms            solvers/diagnostics.py:8-19  (matrix-free diagnostics)
@@ -289,10 +291,21 @@ int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols, co
  *                    methods/galerkin.py:221-237).                          */
 #define IGA_MF_RATIONAL 1
 #define IGA_MF_SEPARABLE 2
+#define IGA_MF_ELASTICITY 4
 int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision, int32_t flags);
 int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
                  int32_t flags, int32_t coef_kind, double coef_const, const double* coef_points,
                  void* work, int64_t work_bytes, void* stream);
+/* Linear elasticity (vec = dim) on the general layout: set-up stores
+ * sqrt(w |det J|) J^-1 per Gauss point (and W, grad W when rational); cmat is
+ * the Voigt matrix (3x3 in 2-D, 6x6 in 3-D, engineering shear, row-major,
+ * host memory). The workspace size is iga_mf_workspace(patch, precision,
+ * flags | IGA_MF_ELASTICITY); iga_mf_apply and iga_cg_solve_mf then act on
+ * the vector free dofs (full dof = vec * control point + component). Replaces
+ * tangent_matvec for LinearElasticity (methods/galerkin.py:207-218). */
+int iga_mf_setup_elasticity(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
+                            int32_t flags, const double* cmat, void* work, int64_t work_bytes,
+                            void* stream);
 int iga_mf_apply(const iga_patch_t* patch, const void* work, int64_t work_bytes, int32_t n_free,
                  const int32_t* freed, const int32_t* full_to_free, const void* x, void* y,
                  void* stream);

@@ -25,7 +25,7 @@ IGA_COEF_CONST, IGA_COEF_POINTS = 0, 1
 IGA_SRC_NONE, IGA_SRC_CONST, IGA_SRC_POINTS = 0, 1, 2
 IGA_FP64, IGA_FP32, IGA_BF16 = 0, 1, 2
 IGA_OK, IGA_ERR_SHAPE, IGA_ERR_UNSUPPORTED, IGA_ERR_HIP, IGA_ERR_ARG = 0, -1, -2, -3, -4
-IGA_MF_RATIONAL, IGA_MF_SEPARABLE = 1, 2
+IGA_MF_RATIONAL, IGA_MF_SEPARABLE, IGA_MF_ELASTICITY = 1, 2, 4
 
 STATUS = {-1: "inconsistent shapes", -2: "unsupported degree/dimension/quadrature",
           -3: "HIP runtime error", -4: "null pointer or invalid argument"}
@@ -147,6 +147,8 @@ _SIGNATURES = {
     "iga_mf_workspace": (c_i64, [ctypes.POINTER(IgaPatch), c_int, c_int]),
     "iga_mf_setup": (c_int, [ctypes.POINTER(IgaPatch), ctypes.POINTER(IgaRule), c_int, c_int, c_int,
                              c_double, c_void_p, c_void_p, c_i64, c_void_p]),
+    "iga_mf_setup_elasticity": (c_int, [ctypes.POINTER(IgaPatch), ctypes.POINTER(IgaRule), c_int, c_int,
+                                        c_void_p, c_void_p, c_i64, c_void_p]),
     "iga_mf_apply": (c_int, [ctypes.POINTER(IgaPatch), c_void_p, c_i64, c_int, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_void_p]),
     "iga_mf_diag": (c_int, [ctypes.POINTER(IgaPatch), c_void_p, c_i64, c_int, c_void_p, c_void_p,

@@ -1,5 +1,6 @@
 // Matrix-free, sum-factorised operator apply y = K v for the scalar Poisson
-// form on a single tensor-product NURBS patch (gfx950).
+// form and for linear elasticity on a single tensor-product NURBS patch
+// (gfx950).
 //
 // Setup (k_mf_geometry), once per operator: at every Gauss point of every
 // element, the symmetric factor G = w a J^{-T} J^{-1} (w: quadrature weight
@@ -22,6 +23,11 @@
 // are both separable sums, u_h = U / W and grad u_h = (grad U - u_h grad W)/W;
 // the test side carries the transposed correction r_i = w_i [ grad N_i . f/W
 // - N_i (grad W . f)/W^2 ] with f = G grad u_h.
+//
+// Linear elasticity (k_mf_apply_el): the same contractions per displacement
+// component; set-up stores M = sqrt(w |det J|) J^-1 per point instead of G,
+// and the flux of component c is T_ck = sum_x S_cx M_xk with S the stress
+// tensor of sigma = C eps(grad u) (Voigt, engineering shear).
 #include <algorithm>
 #include <mutex>
 #include <unordered_map>
@@ -62,7 +68,7 @@ __device__ __forceinline__ int gsym(int a, int b) {
 // setup: geometric factors at the Gauss points
 // ---------------------------------------------------------------------------
 
-template <int DIM, int P, typename T>
+template <int DIM, int P, typename T, bool ELAS>
 __global__ void __launch_bounds__(256)
     k_mf_geometry(iga_patch_t pt, const double* __restrict__ qp, const double* __restrict__ qw,
                   int coef_kind, double coef_const, const double* __restrict__ coef_points,
@@ -146,15 +152,23 @@ __global__ void __launch_bounds__(256)
     inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
     inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
   }
-  const double coef = coef_kind == IGA_COEF_POINTS ? coef_points[e * NQ + q] : coef_const;
-  const double wa = jac * fabs(det) * coef;
-  // G[a][b] = wa * sum_c inv[c][a] inv[c][b]; layout [e][g][q]
-  for (int a = 0; a < DIM; ++a)
-    for (int b = a; b < DIM; ++b) {
-      double s = 0.0;
-      for (int c = 0; c < DIM; ++c) s += inv[c][a] * inv[c][b];
-      geo[(e * NG + gsym<DIM>(a, b)) * NQ + q] = (T)(wa * s);
-    }
+  if constexpr (ELAS) {
+    // M[x][k] = sqrt(w |det J|) (J^-1)[x][k]; layout [e][x * DIM + k][q]
+    const double sc = sqrt(jac * fabs(det));
+    for (int xx = 0; xx < DIM; ++xx)
+      for (int k = 0; k < DIM; ++k)
+        geo[(e * DIM * DIM + xx * DIM + k) * NQ + q] = (T)(sc * inv[xx][k]);
+  } else {
+    const double coef = coef_kind == IGA_COEF_POINTS ? coef_points[e * NQ + q] : coef_const;
+    const double wa = jac * fabs(det) * coef;
+    // G[a][b] = wa * sum_c inv[c][a] inv[c][b]; layout [e][g][q]
+    for (int a = 0; a < DIM; ++a)
+      for (int b = a; b < DIM; ++b) {
+        double s = 0.0;
+        for (int c = 0; c < DIM; ++c) s += inv[c][a] * inv[c][b];
+        geo[(e * NG + gsym<DIM>(a, b)) * NQ + q] = (T)(wa * s);
+      }
+  }
   if (wgeo) {
     wgeo[(e * NW + 0) * NQ + q] = (T)W;
     for (int k = 0; k < DIM; ++k) wgeo[(e * NW + 1 + k) * NQ + q] = (T)dW[k];
@@ -478,10 +492,324 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Voigt matrix of the elastic material (row-major NS x NS, NS = 3 or 6)
+struct MfElastic {
+  double c[36];
+};
+
+// Linear elasticity, vec = DIM: one group of lanes per element as in
+// k_mf_apply. The forward contractions run component by component into
+// per-component gradient (and, rational, value) buffers; at each Gauss point
+// one lane forms H = P M^T (P_ck = d u_c / d xi_k, M = sqrt(w |det J|) J^-1),
+// the engineering strain, sigma = C eps and the flux T_ck = sum_x S_cx M_xk;
+// the transposed contractions then run component by component. Local
+// results are stored [e][i][component], the dof order.
+template <int DIM, int P, typename T, bool RATIONAL>
+__global__ void __launch_bounds__(256)
+    k_mf_apply_el(iga_patch_t pt, MfArgs A, const MfElastic cm, const T* __restrict__ geo,
+                  const T* __restrict__ wgeo, const double* __restrict__ v, T* __restrict__ rloc) {
+  if (A.skip && *A.skip) return;
+  using S = MfShape<DIM, P>;
+  constexpr int N1 = S::N1, Q1 = S::Q1, NL = S::NL, NQ = S::NQ, NW = S::NW;
+  constexpr int VEC = DIM, NM = DIM * DIM, NS = DIM == 2 ? 3 : 6;
+  constexpr int LANES = S::LANES;
+  constexpr int EPB = S::BLOCK / LANES;  // elements per block
+  constexpr int BUF = NQ > NL ? (NQ > N1 * N1 * Q1 ? NQ : N1 * N1 * Q1) : NL;
+  constexpr int NWK = DIM == 2 ? 2 : 5;      // work buffers of the contractions
+  constexpr int GB = VEC + NWK;              // first per-component point buffer
+  constexpr int NBUF = GB + VEC * (DIM + 1);  // u[c], work, then [c][k] (k = DIM: value)
+  __shared__ T tabs[EPB][DIM][2][Q1 * N1];
+  __shared__ T buf[EPB][NBUF][BUF + 1];
+  __shared__ T wloc[EPB][NL];
+  __shared__ double cmat[NS * NS];
+
+  if (threadIdx.x < NS * NS) cmat[threadIdx.x] = cm.c[threadIdx.x];
+  const int le = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  const int64_t e = (int64_t)blockIdx.x * EPB + le;
+  const bool live = e < A.n_e;
+  int ei[3] = {0, 0, 0};
+  if (live) {
+    int64_t rem = e;
+    for (int d = 0; d < DIM; ++d) {
+      ei[d] = (int)(rem % pt.n_elem[d]);
+      rem /= pt.n_elem[d];
+    }
+  }
+  for (int t = lane; t < DIM * Q1 * N1; t += LANES) {
+    const int d = t / (Q1 * N1), o = t % (Q1 * N1);
+    T tv = 0, dv = 0;
+    if (live) {
+      tv = reinterpret_cast<const T*>(A.tab[d])[(size_t)ei[d] * Q1 * N1 + o];
+      dv = reinterpret_cast<const T*>(A.dtab[d])[(size_t)ei[d] * Q1 * N1 + o];
+    }
+    tabs[le][d][0][o] = tv;
+    tabs[le][d][1][o] = dv;
+  }
+  if (lane < NL) {
+    T wv = 1;
+    T val[VEC];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) val[c] = 0;
+    if (live) {
+      const int ai[3] = {lane % N1, (lane / N1) % N1, lane / (N1 * N1)};
+      int sdof = 0, stride = 1;
+      for (int d = 0; d < DIM; ++d) {
+        sdof += (pt.first[d][ei[d]] + ai[d]) * stride;
+        stride *= pt.n_cp[d];
+      }
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) {
+        const int f = A.f2f[(size_t)VEC * sdof + c];
+        val[c] = f >= 0 ? (T)v[f] : (T)0;
+      }
+      if (RATIONAL) wv = (T)pt.wgts[sdof];
+    }
+    wloc[le][lane] = wv;
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) buf[le][c][lane] = RATIONAL ? val[c] * wv : val[c];
+  }
+  __syncthreads();
+  const T* B0 = tabs[le][0][0];
+  const T* D0 = tabs[le][0][1];
+  const T* B1 = tabs[le][1][0];
+  const T* D1 = tabs[le][1][1];
+  auto G = [&](int c, int k) { return buf[le][GB + c * (DIM + 1) + k]; };
+
+  // ---- forward: parametric gradient (and value) of every component ---------
+  for (int c = 0; c < VEC; ++c) {
+    const T* u = buf[le][c];
+    if constexpr (DIM == 2) {
+      T* A0 = buf[le][VEC];
+      T* A0d = buf[le][VEC + 1];
+      {
+        const int ext[3] = {N1, N1, 1};
+        if (lane < Q1 * N1) {
+          const int x = lane % Q1, y = lane / Q1;
+          A0[lane] = contract(u, B0, 0, N1, x, y, 0, ext);
+          A0d[lane] = contract(u, D0, 0, N1, x, y, 0, ext);
+        }
+      }
+      __syncthreads();
+      {
+        const int ext[3] = {Q1, N1, 1};
+        if (lane < NQ) {
+          const int x = lane % Q1, y = lane / Q1;
+          G(c, 0)[lane] = contract(A0d, B1, 1, N1, x, y, 0, ext);
+          G(c, 1)[lane] = contract(A0, D1, 1, N1, x, y, 0, ext);
+          G(c, 2)[lane] = RATIONAL ? contract(A0, B1, 1, N1, x, y, 0, ext) : (T)0;
+        }
+      }
+      __syncthreads();
+    } else {
+      const T* B2 = tabs[le][2][0];
+      const T* D2 = tabs[le][2][1];
+      T* A0 = buf[le][VEC];
+      T* A0d = buf[le][VEC + 1];
+      {
+        const int ext[3] = {N1, N1, N1};
+        if (lane < Q1 * N1 * N1) {
+          const int x = lane % Q1, y = (lane / Q1) % N1, z = lane / (Q1 * N1);
+          A0[lane] = contract(u, B0, 0, N1, x, y, z, ext);
+          A0d[lane] = contract(u, D0, 0, N1, x, y, z, ext);
+        }
+      }
+      __syncthreads();
+      T* A1 = buf[le][VEC + 2];
+      T* A1x = buf[le][VEC + 3];
+      T* A1y = buf[le][VEC + 4];
+      {
+        const int ext[3] = {Q1, N1, N1};
+        if (lane < Q1 * Q1 * N1) {
+          const int x = lane % Q1, y = (lane / Q1) % Q1, z = lane / (Q1 * Q1);
+          A1[lane] = contract(A0, B1, 1, N1, x, y, z, ext);
+          A1x[lane] = contract(A0d, B1, 1, N1, x, y, z, ext);
+          A1y[lane] = contract(A0, D1, 1, N1, x, y, z, ext);
+        }
+      }
+      __syncthreads();
+      {
+        const int ext[3] = {Q1, Q1, N1};
+        if (lane < NQ) {
+          const int x = lane % Q1, y = (lane / Q1) % Q1, z = lane / (Q1 * Q1);
+          G(c, 0)[lane] = contract(A1x, B2, 2, N1, x, y, z, ext);
+          G(c, 1)[lane] = contract(A1y, B2, 2, N1, x, y, z, ext);
+          G(c, 2)[lane] = contract(A1, D2, 2, N1, x, y, z, ext);
+          G(c, 3)[lane] = RATIONAL ? contract(A1, B2, 2, N1, x, y, z, ext) : (T)0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- at the Gauss points: strain, stress, parametric flux ------------------
+  if (lane < NQ) {
+    const T* m = geo + (size_t)(live ? e : 0) * NM * NQ;
+    T M[DIM][DIM];
+#pragma unroll
+    for (int xx = 0; xx < DIM; ++xx)
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) M[xx][k] = m[(xx * DIM + k) * NQ + lane];
+    T Pg[VEC][DIM];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c)
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) Pg[c][k] = G(c, k)[lane];
+    T W = 1, dW[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) dW[k] = 0;
+    if (RATIONAL) {
+      const T* wg = wgeo + (size_t)(live ? e : 0) * NW * NQ;
+      W = wg[lane];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) dW[k] = wg[(1 + k) * NQ + lane];
+#pragma unroll
+      for (int c = 0; c < VEC; ++c) {
+        const T uh = G(c, DIM)[lane] / W;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) Pg[c][k] = (Pg[c][k] - uh * dW[k]) / W;
+      }
+    }
+    // H[c][x] = sum_k M[x][k] P[c][k]
+    T H[VEC][DIM];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c)
+#pragma unroll
+      for (int xx = 0; xx < DIM; ++xx) {
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s += M[xx][k] * Pg[c][k];
+        H[c][xx] = s;
+      }
+    // engineering strain, stress (Voigt), symmetric stress tensor
+    T eps[NS], sig[NS], St[DIM][DIM];
+    if constexpr (DIM == 2) {
+      eps[0] = H[0][0];
+      eps[1] = H[1][1];
+      eps[2] = H[0][1] + H[1][0];
+    } else {
+      eps[0] = H[0][0];
+      eps[1] = H[1][1];
+      eps[2] = H[2][2];
+      eps[3] = H[1][2] + H[2][1];
+      eps[4] = H[0][2] + H[2][0];
+      eps[5] = H[0][1] + H[1][0];
+    }
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+      T s = 0;
+#pragma unroll
+      for (int b = 0; b < NS; ++b) s += (T)cmat[a * NS + b] * eps[b];
+      sig[a] = s;
+    }
+    if constexpr (DIM == 2) {
+      St[0][0] = sig[0];
+      St[1][1] = sig[1];
+      St[0][1] = St[1][0] = sig[2];
+    } else {
+      St[0][0] = sig[0];
+      St[1][1] = sig[1];
+      St[2][2] = sig[2];
+      St[1][2] = St[2][1] = sig[3];
+      St[0][2] = St[2][0] = sig[4];
+      St[0][1] = St[1][0] = sig[5];
+    }
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) {
+      T Tf[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        T s = 0;
+#pragma unroll
+        for (int xx = 0; xx < DIM; ++xx) s += St[c][xx] * M[xx][k];
+        Tf[k] = s;
+      }
+      T sc = 0;
+      if (RATIONAL) {
+        T dot = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) dot += dW[k] * Tf[k];
+        sc = -dot / (W * W);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) Tf[k] /= W;
+      }
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) G(c, k)[lane] = Tf[k];
+      G(c, DIM)[lane] = sc;
+    }
+  }
+  __syncthreads();
+
+  // ---- transposed contractions, component by component ----------------------
+  for (int c = 0; c < VEC; ++c) {
+    T r = 0;
+    if constexpr (DIM == 2) {
+      T* C0 = buf[le][VEC];
+      T* C1 = buf[le][VEC + 1];
+      {
+        const int ext[3] = {Q1, Q1, 1};
+        if (lane < Q1 * N1) {
+          const int x = lane % Q1, y = lane / Q1;
+          C0[lane] = contract_t(G(c, 0), B1, 1, Q1, N1, x, y, 0, ext);
+          T c1 = contract_t(G(c, 1), D1, 1, Q1, N1, x, y, 0, ext);
+          if (RATIONAL) c1 += contract_t(G(c, 2), B1, 1, Q1, N1, x, y, 0, ext);
+          C1[lane] = c1;
+        }
+      }
+      __syncthreads();
+      if (lane < NL) {
+        const int ext[3] = {Q1, N1, 1};
+        const int x = lane % N1, y = lane / N1;
+        r = contract_t(C0, D0, 0, Q1, N1, x, y, 0, ext) + contract_t(C1, B0, 0, Q1, N1, x, y, 0, ext);
+      }
+    } else {
+      const T* B2 = tabs[le][2][0];
+      const T* D2 = tabs[le][2][1];
+      T* C0 = buf[le][VEC];
+      T* C1 = buf[le][VEC + 1];
+      T* C2 = buf[le][VEC + 2];
+      {
+        const int ext[3] = {Q1, Q1, Q1};
+        if (lane < Q1 * Q1 * N1) {
+          const int x = lane % Q1, y = (lane / Q1) % Q1, z = lane / (Q1 * Q1);
+          C0[lane] = contract_t(G(c, 0), B2, 2, Q1, N1, x, y, z, ext);
+          C1[lane] = contract_t(G(c, 1), B2, 2, Q1, N1, x, y, z, ext);
+          T c2 = contract_t(G(c, 2), D2, 2, Q1, N1, x, y, z, ext);
+          if (RATIONAL) c2 += contract_t(G(c, 3), B2, 2, Q1, N1, x, y, z, ext);
+          C2[lane] = c2;
+        }
+      }
+      __syncthreads();
+      T* E0 = buf[le][VEC + 3];
+      T* E1 = buf[le][VEC + 4];
+      {
+        const int ext[3] = {Q1, Q1, N1};
+        if (lane < Q1 * N1 * N1) {
+          const int x = lane % Q1, y = (lane / Q1) % N1, z = lane / (Q1 * N1);
+          E0[lane] = contract_t(C0, B1, 1, Q1, N1, x, y, z, ext);
+          E1[lane] = contract_t(C1, D1, 1, Q1, N1, x, y, z, ext) +
+                     contract_t(C2, B1, 1, Q1, N1, x, y, z, ext);
+        }
+      }
+      __syncthreads();
+      if (lane < NL) {
+        const int ext[3] = {Q1, N1, N1};
+        const int x = lane % N1, y = (lane / N1) % N1, z = lane / (N1 * N1);
+        r = contract_t(E0, D0, 0, Q1, N1, x, y, z, ext) + contract_t(E1, B0, 0, Q1, N1, x, y, z, ext);
+      }
+    }
+    if (lane < NL && live) {
+      if (RATIONAL) r *= wloc[le][lane];
+      rloc[((size_t)e * NL + lane) * VEC + c] = r;
+    }
+    // the work buffers are written again by the next component
+    __syncthreads();
+  }
+}
+
 // y[r] = sum of the local results of the elements supporting free dof r;
 // grid-stride over at most kRedBlocks blocks, with the x.y partials (one per
 // block) when `parts` is set
-template <int DIM, int P, typename T, typename TO>
+template <int DIM, int P, typename T, typename TO, int VEC>
 __global__ void __launch_bounds__(256)
     k_mf_gather(iga_patch_t pt, int n_free, const int32_t* __restrict__ freed,
                 const T* __restrict__ rloc, TO* __restrict__ y, const int* skip,
@@ -490,7 +818,10 @@ __global__ void __launch_bounds__(256)
   constexpr int N1 = P + 1, NL = ipow(N1, DIM);
   double dot = 0.0;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n_free; r += gridDim.x * blockDim.x) {
-    int s = freed[r];
+    // free dof -> (control point, component); rloc is [e][i][component]
+    const int full = freed[r];
+    int s = full / VEC;
+    const int comp = full - s * VEC;
     int id[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int d = 0; d < DIM; ++d) {
       id[d] = s % pt.n_cp[d];
@@ -510,7 +841,7 @@ __global__ void __launch_bounds__(256)
             e += ee[d] * estride;
             estride *= pt.n_elem[d];
           }
-          sum += (double)rloc[(size_t)e * NL + li];
+          sum += (double)rloc[((size_t)e * NL + li) * VEC + comp];
         }
     y[r] = (TO)sum;
     if (parts) dot += sum * x[r];
@@ -967,9 +1298,11 @@ struct MfInfo {
   int32_t magic, dim, degree, precision, flags, coef_kind;
   int64_t n_e, bytes;
   double coef_const;
+  double cmat[36];  // elasticity: the Voigt matrix
 };
 constexpr int32_t kMfMagic = 0x4d465731;  // "MFW1"
-constexpr int64_t kMfHeader = 256;
+constexpr int64_t kMfHeader = 512;
+static_assert(sizeof(MfInfo) <= kMfHeader, "the workspace header holds MfInfo");
 
 std::mutex g_mf_mu;
 std::unordered_map<const void*, MfInfo> g_mf;
@@ -1004,14 +1337,15 @@ MfLayout mf_layout(const iga_patch_t& pt, int precision, int flags) {
     }
     L.coef = take(L.n_e * (cs == 2 ? (S::NQ + 1) / 2 * 4 : S::NQ * cs));
   } else {
-    L.geo = take(L.n_e * S::NG * S::NQ * ts);
+    const int ng = (flags & IGA_MF_ELASTICITY) ? DIM * DIM : S::NG;
+    L.geo = take(L.n_e * ng * S::NQ * ts);
     L.wgeo = (flags & IGA_MF_RATIONAL) ? take(L.n_e * S::NW * S::NQ * ts) : -1;
     for (int d = 0; d < DIM; ++d) {
       L.tabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * ts);
       L.dtabs[d] = take((int64_t)pt.n_elem[d] * S::Q1 * S::N1 * ts);
     }
   }
-  L.rloc = take(L.n_e * S::NL * ts);
+  L.rloc = take(L.n_e * S::NL * ((flags & IGA_MF_ELASTICITY) ? DIM : 1) * ts);
   L.total = off;
   return L;
 }
@@ -1032,8 +1366,10 @@ int by_shape(int dim, int degree, F f) {
 // valid combinations: FP64 / FP32 with either layout (rational only in the
 // general one), BF16 (BF16 coefficient, FP32 arithmetic) separable only
 int check_flags(int precision, int flags) {
-  if (flags & ~(IGA_MF_RATIONAL | IGA_MF_SEPARABLE)) return IGA_ERR_ARG;
+  if (flags & ~(IGA_MF_RATIONAL | IGA_MF_SEPARABLE | IGA_MF_ELASTICITY)) return IGA_ERR_ARG;
   if ((flags & IGA_MF_RATIONAL) && (flags & IGA_MF_SEPARABLE)) return IGA_ERR_ARG;
+  // elasticity: the general layout, FP64 or FP32
+  if ((flags & IGA_MF_ELASTICITY) && (flags & IGA_MF_SEPARABLE)) return IGA_ERR_UNSUPPORTED;
   if (precision == IGA_FP64 || precision == IGA_FP32) return IGA_OK;
   if (precision == IGA_BF16 && (flags & IGA_MF_SEPARABLE)) return IGA_OK;
   return IGA_ERR_UNSUPPORTED;
@@ -1064,10 +1400,9 @@ extern "C" int64_t iga_mf_workspace(const iga_patch_t* patch, int32_t precision,
   return out;
 }
 
-extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
-                            int32_t flags, int32_t coef_kind, double coef_const,
-                            const double* coef_points, void* work, int64_t work_bytes,
-                            void* stream) {
+static int mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
+                    int32_t flags, int32_t coef_kind, double coef_const, const double* coef_points,
+                    const double* cmat, void* work, int64_t work_bytes, void* stream) {
   if (!patch || !rule || !work) return IGA_ERR_ARG;
   if (rule->n_q1 != patch->degree + 1) return IGA_ERR_UNSUPPORTED;
   if (int rc = check_flags(precision, flags)) return rc;
@@ -1116,10 +1451,16 @@ extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, in
     auto run = [&](auto tag) -> int {
       using T = decltype(tag);
       if (pts > 0) {
-        hipLaunchKernelGGL((k_mf_geometry<DIM, P, T>), dim3((unsigned)((pts + 255) / 256)), dim3(256),
-                           0, st, *patch, rule->pts, rule->wts, coef_kind, coef_const, coef_points,
-                           reinterpret_cast<T*>(base + L.geo),
-                           rational ? reinterpret_cast<T*>(base + L.wgeo) : nullptr, L.n_e);
+        if (flags & IGA_MF_ELASTICITY)
+          hipLaunchKernelGGL((k_mf_geometry<DIM, P, T, true>), dim3((unsigned)((pts + 255) / 256)),
+                             dim3(256), 0, st, *patch, rule->pts, rule->wts, coef_kind, coef_const,
+                             coef_points, reinterpret_cast<T*>(base + L.geo),
+                             rational ? reinterpret_cast<T*>(base + L.wgeo) : nullptr, L.n_e);
+        else
+          hipLaunchKernelGGL((k_mf_geometry<DIM, P, T, false>), dim3((unsigned)((pts + 255) / 256)),
+                             dim3(256), 0, st, *patch, rule->pts, rule->wts, coef_kind, coef_const,
+                             coef_points, reinterpret_cast<T*>(base + L.geo),
+                             rational ? reinterpret_cast<T*>(base + L.wgeo) : nullptr, L.n_e);
         IGA_CHECK_LAUNCH();
       }
       for (int d = 0; d < DIM; ++d) {
@@ -1143,6 +1484,10 @@ extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, in
   I.flags = flags;
   I.coef_kind = coef_kind;
   I.coef_const = coef_const;
+  if (cmat) {
+    const int ns = patch->dim == 2 ? 3 : 6;
+    for (int t = 0; t < ns * ns; ++t) I.cmat[t] = cmat[t];
+  }
   I.n_e = 1;
   for (int d = 0; d < patch->dim; ++d) I.n_e *= patch->n_elem[d];
   I.bytes = iga_mf_workspace(patch, precision, flags);
@@ -1156,6 +1501,23 @@ extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, in
       return IGA_ERR_HIP;
   }
   return IGA_OK;
+}
+
+extern "C" int iga_mf_setup(const iga_patch_t* patch, const iga_rule_t* rule, int32_t precision,
+                            int32_t flags, int32_t coef_kind, double coef_const,
+                            const double* coef_points, void* work, int64_t work_bytes,
+                            void* stream) {
+  if (flags & IGA_MF_ELASTICITY) return IGA_ERR_ARG;  // iga_mf_setup_elasticity
+  return mf_setup(patch, rule, precision, flags, coef_kind, coef_const, coef_points, nullptr, work,
+                  work_bytes, stream);
+}
+
+extern "C" int iga_mf_setup_elasticity(const iga_patch_t* patch, const iga_rule_t* rule,
+                                       int32_t precision, int32_t flags, const double* cmat,
+                                       void* work, int64_t work_bytes, void* stream) {
+  if (!cmat) return IGA_ERR_ARG;
+  return mf_setup(patch, rule, precision, flags | IGA_MF_ELASTICITY, IGA_COEF_CONST, 1.0, nullptr,
+                  cmat, work, work_bytes, stream);
 }
 
 namespace iga {
@@ -1212,6 +1574,42 @@ static int mf_launch(const iga_patch_t* patch, const MfInfo& I, const void* work
     }
     if (mode != 0) return (int)IGA_ERR_UNSUPPORTED;  // the diagonal is sum-factorised on separable patches
     const bool rational = I.flags & IGA_MF_RATIONAL;
+    if (I.flags & IGA_MF_ELASTICITY) {
+      auto run_el = [&](auto tag) -> int {
+        using T = decltype(tag);
+        MfArgs A;
+        A.n_e = L.n_e;
+        A.skip = skip;
+        A.f2f = full_to_free;
+        for (int d = 0; d < 3; ++d) {
+          A.tab[d] = d < DIM ? base + L.tabs[d] : nullptr;
+          A.dtab[d] = d < DIM ? base + L.dtabs[d] : nullptr;
+        }
+        MfElastic cm;
+        for (int t = 0; t < 36; ++t) cm.c[t] = I.cmat[t];
+        T* rloc = reinterpret_cast<T*>(base + L.rloc);
+        constexpr int EPB = S::BLOCK / S::LANES;
+        const int64_t blocks = (L.n_e + EPB - 1) / EPB;
+        if (blocks > 0) {
+          if (rational)
+            hipLaunchKernelGGL((k_mf_apply_el<DIM, P, T, true>), dim3((unsigned)blocks), dim3(S::BLOCK),
+                               0, st, *patch, A, cm, reinterpret_cast<const T*>(base + L.geo),
+                               reinterpret_cast<const T*>(base + L.wgeo), x, rloc);
+          else
+            hipLaunchKernelGGL((k_mf_apply_el<DIM, P, T, false>), dim3((unsigned)blocks), dim3(S::BLOCK),
+                               0, st, *patch, A, cm, reinterpret_cast<const T*>(base + L.geo), nullptr,
+                               x, rloc);
+          IGA_CHECK_LAUNCH();
+        }
+        if (n_free > 0) {
+          hipLaunchKernelGGL((k_mf_gather<DIM, P, T, double, DIM>), dim3(gb), dim3(256), 0, st,
+                             *patch, n_free, freed, rloc, y, skip, x, parts);
+          IGA_CHECK_LAUNCH();
+        }
+        return IGA_OK;
+      };
+      return I.precision == IGA_FP32 ? run_el(float{}) : run_el(double{});
+    }
     auto run = [&](auto tag) -> int {
       using T = decltype(tag);
       MfArgs A;
@@ -1236,7 +1634,7 @@ static int mf_launch(const iga_patch_t* patch, const MfInfo& I, const void* work
         IGA_CHECK_LAUNCH();
       }
       if (n_free > 0) {
-        hipLaunchKernelGGL((k_mf_gather<DIM, P, T, double>), dim3(gb), dim3(256), 0, st, *patch,
+        hipLaunchKernelGGL((k_mf_gather<DIM, P, T, double, 1>), dim3(gb), dim3(256), 0, st, *patch,
                            n_free, freed, rloc, y, skip, x, parts);
         IGA_CHECK_LAUNCH();
       }

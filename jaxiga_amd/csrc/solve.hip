@@ -11,6 +11,7 @@
 // host enqueues iterations in batches and synchronises only once per batch.
 #include <algorithm>
 #include <mutex>
+#include <unordered_map>
 #include <type_traits>
 #include <vector>
 
@@ -394,13 +395,36 @@ static int nnz_of(int n, const int32_t* indptr, hipStream_t st, int* nnz) {
   return IGA_OK;
 }
 
+// Lanes per row of a pattern, read from its row pointers once per (indptr,
+// n) and remembered, so a repeated product issues no host round trip (and
+// can be captured in a graph). The choice only tunes speed: a stale entry
+// for a reused address gives the same product.
+static int spmv_lpr(int n, const int32_t* indptr, hipStream_t st, int* lpr) {
+  static std::mutex mu;
+  static std::unordered_map<const int32_t*, std::pair<int, int>> seen;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = seen.find(indptr);
+    if (it != seen.end() && it->second.first == n) {
+      *lpr = it->second.second;
+      return IGA_OK;
+    }
+  }
+  int nnz = 0;
+  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
+  *lpr = pick_lpr(n, nnz);
+  std::lock_guard<std::mutex> lock(mu);
+  seen[indptr] = {n, *lpr};
+  return IGA_OK;
+}
+
 extern "C" int iga_spmv(int32_t n, const int32_t* indptr, const int32_t* cols, const double* vals,
                         const double* x, double* y, void* stream) {
   if (n == 0) return IGA_OK;
   hipStream_t st = (hipStream_t)stream;
-  int nnz = 0;
-  if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
-  return by_lpr(pick_lpr(n, nnz), [&](auto L) {
+  int lpr = 8;
+  if (int rc = spmv_lpr(n, indptr, st, &lpr)) return rc;
+  return by_lpr(lpr, [&](auto L) {
     constexpr int LPR = decltype(L)::value;
     const long long threads = (long long)n * LPR;
     hipLaunchKernelGGL(k_spmv<LPR>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, n,

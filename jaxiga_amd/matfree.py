@@ -1,4 +1,5 @@
-"""Matrix-free operator and solve (sum-factorised, GPU).
+"""Matrix-free operator and solve (sum-factorised, GPU): scalar Poisson and
+linear elasticity.
 
 Mirrors the matrix-free branch of the reference: ``tangent_matvec``
 (``methods/galerkin.py:207-218``), ``tangent_diagonal`` (:221-237),
@@ -42,7 +43,8 @@ def separable_map(space, rtol=1e-14):
 
 
 class MatrixFreeOperator:
-    """``y = K x`` on the free dofs of a scalar Poisson problem.
+    """``y = K x`` on the free dofs of a scalar Poisson or a linear elasticity
+    problem.
 
     ``precision``: ``IGA_FP64`` or ``IGA_FP32`` arithmetic, or ``IGA_BF16``
     (coefficient stored in BF16, FP32 arithmetic; separable patches only).
@@ -58,8 +60,11 @@ class MatrixFreeOperator:
         from jaxiga_amd.methods import _bind_coefficient, _coefficient_args
 
         space = ctx.space
-        if ctx.problem.kernel != "poisson" or space.vec != 1:
-            raise NotImplementedError("the matrix-free operator covers the scalar Poisson form")
+        kernel = ctx.problem.kernel
+        self.elastic = kernel == "elasticity"
+        if not ((kernel == "poisson" and space.vec == 1) or (self.elastic and space.vec == space.dim)):
+            raise NotImplementedError("the matrix-free operator covers the scalar Poisson form and "
+                                      "linear elasticity")
         if ctx.n_q1 != space.degree[0] + 1:
             raise NotImplementedError("the matrix-free operator uses the p+1 Gauss rule")
         self.ctx = ctx
@@ -70,11 +75,14 @@ class MatrixFreeOperator:
         # the geometry factors always use the patch's own weights.
         w = np.asarray(space.wgts, dtype=np.float64)
         self.rational = int(w.size > 0 and np.ptp(w) > 1e-14 * np.abs(w).max())
-        self.separable = int(not self.rational and separable_map(space))
+        # elasticity uses the general layout (per-point inverse Jacobians)
+        self.separable = int(not self.rational and not self.elastic and separable_map(space))
         self.flags = (_lib.IGA_MF_RATIONAL if self.rational else 0) | \
-            (_lib.IGA_MF_SEPARABLE if self.separable else 0)
+            (_lib.IGA_MF_SEPARABLE if self.separable else 0) | \
+            (_lib.IGA_MF_ELASTICITY if self.elastic else 0)
         if precision == _lib.IGA_BF16 and not self.separable:
-            raise NotImplementedError("the BF16 matrix-free operator needs a separable polynomial patch")
+            raise NotImplementedError("the BF16 matrix-free operator needs a separable polynomial patch "
+                                      "(scalar Poisson)")
         L = lib()
         st = space.device().struct
         nbytes = int(L.iga_mf_workspace(st, precision, self.flags))
@@ -82,13 +90,21 @@ class MatrixFreeOperator:
             raise NotImplementedError(f"no matrix-free kernel for dim={space.dim}, degree={space.degree}")
         self.work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=ctx.f2f_d.device)
         self.nbytes = nbytes
-        args = _lib.IgaAsmArgs()
-        keep = (_bind_coefficient(args, inputs["coef"]) if inputs is not None
-                else _coefficient_args(ctx, params, args))
-        _lib.check(L.iga_mf_setup(st, ctx.rule(), precision, self.flags, args.coef_kind,
-                                  args.coef_const, args.coef_points, _lib.ptr(self.work), nbytes,
-                                  stream()), "iga_mf_setup")
-        del keep
+        if self.elastic:
+            C = inputs["C"] if inputs is not None else ctx.problem.C(params)
+            C = C.detach().cpu().numpy() if isinstance(C, torch.Tensor) else np.asarray(C, float)
+            C = np.ascontiguousarray(C, dtype=np.float64)
+            _lib.check(L.iga_mf_setup_elasticity(st, ctx.rule(), precision, self.flags, C.ctypes.data,
+                                                 _lib.ptr(self.work), nbytes, stream()),
+                       "iga_mf_setup_elasticity")
+        else:
+            args = _lib.IgaAsmArgs()
+            keep = (_bind_coefficient(args, inputs["coef"]) if inputs is not None
+                    else _coefficient_args(ctx, params, args))
+            _lib.check(L.iga_mf_setup(st, ctx.rule(), precision, self.flags, args.coef_kind,
+                                      args.coef_const, args.coef_points, _lib.ptr(self.work), nbytes,
+                                      stream()), "iga_mf_setup")
+            del keep
         self.n = ctx.n_free
 
     def apply(self, x, out=None):
