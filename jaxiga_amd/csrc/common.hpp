@@ -166,6 +166,36 @@ __device__ __forceinline__ double reduce_partials(const double* parts, int n) {
   return r;
 }
 
+// reduce_partials in two halves, so that a kernel can issue the loads of the
+// partials together with its other loads and reduce them later: this thread's
+// share (at most K partials per thread, n <= K * blockDim.x), then the sum of
+// the shares. The summation order is reduce_partials' own, so both give the
+// same bits.
+template <int K>
+__device__ __forceinline__ double partials_share(const double* parts, int n) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * blockDim.x;
+    v[k] = i < n ? parts[i] : 0.0;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) s += v[k];
+  return s;
+}
+
+__device__ __forceinline__ double partials_total(double s) {
+  __shared__ double scratch[16];
+  __shared__ double result;
+  s = block_sum(s, scratch);
+  if (threadIdx.x == 0) result = s;
+  __syncthreads();
+  const double r = result;
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ void store_partial(double* parts, double v) {
   __shared__ double scratch[16];
   v = block_sum(v, scratch);

@@ -386,6 +386,41 @@ __device__ __forceinline__ bool gate_closed(const Gate& g) {
   return g.flag && *g.flag;
 }
 
+// gate_closed with its loads issued early (gate_load, beside the kernel's
+// other loads) and the test made later (gate_test): one trip to memory less
+struct GateVals {
+  double share, tol2, bb;
+  int flag;
+};
+
+// (blocks of 256 threads: at most kRedBlocks / 256 partials per thread)
+__device__ __forceinline__ GateVals gate_load(const Gate& g) {
+  GateVals v{0.0, 0.0, 0.0, 0};
+  if (g.rr) {
+    v.share = partials_share<kRedBlocks / 256>(g.rr, g.n_rr);
+    v.tol2 = g.st->tol2;
+    v.bb = g.st->bb;
+  } else if (g.flag) {
+    v.flag = *g.flag;
+  }
+  return v;
+}
+
+__device__ __forceinline__ bool gate_test(const Gate& g, const GateVals& v) {
+  if (g.rr) {
+    const double rr = partials_total(v.share);
+    const bool done = !(rr > v.tol2 * v.bb);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+      g.st->rr = rr;
+      g.st->done = done ? 1 : 0;
+      if (!done) g.st->iter += 1;
+      if (g.clear_pending) g.st->pending = 0;
+    }
+    return done;
+  }
+  return v.flag != 0;
+}
+
 // Optional dot product fused into the kernel that writes a vector z:
 // parts[block] = sum over the block's entries of z_i y_i (FP64).
 struct DotOut {
@@ -864,30 +899,6 @@ __device__ __forceinline__ T tap_sum(const T (&a)[kGN], const V (*v)[LD], int ly
   return acc;
 }
 
-// (P x_c) at fine point (r, c) from the coarse values staged in cs (origin
-// (cr0, cc0)); weights off the band are zero and indices are clamped to the
-// staging, so no access leaves it
-__device__ __forceinline__ float prolong_at(const GBand& br, const GBand& bc, int r, int c,
-                                            const float (*cs)[kCoarseSpan], int cr0, int cc0) {
-  const int ka = br.c_lo[r] - cr0, kb = bc.c_lo[c] - cc0;
-  float wr[kKPF], wc[kKPF];
-#pragma unroll
-  for (int a = 0; a < kKPF; ++a) {
-    wr[a] = br.pw[r * kKPF + a];
-    wc[a] = bc.pw[c * kKPF + a];
-  }
-  float v = 0.f;
-#pragma unroll
-  for (int a = 0; a < kKPF; ++a) {
-    const int ia = min(max(ka + a, 0), kCoarseSpan - 1);
-    float s = 0.f;
-#pragma unroll
-    for (int b = 0; b < kKPF; ++b) s += wc[b] * cs[ia][min(max(kb + b, 0), kCoarseSpan - 1)];
-    v += wr[a] * s;
-  }
-  return v;
-}
-
 // stage x_c over coarse rows [cr0, cr0 + 24) x cols [cc0, cc0 + 24), all
 // loads in flight together
 __device__ __forceinline__ void stage_coarse(float (*cs)[kCoarseSpan], const float* __restrict__ xc,
@@ -909,18 +920,50 @@ __device__ __forceinline__ void stage_coarse(float (*cs)[kCoarseSpan], const flo
   }
 }
 
-// coarse tile of level l+1 from the fine residual r_f of level l
+// coarse tile of level l+1 from the fine residual r_f of level l. Every load
+// that depends on no other (the skip flag, the staging origin, the band rows
+// and scalings of the halo points, the tile point's taps) is issued first, in
+// one group, and the fine residual is staged behind them: two trips to memory.
 __global__ void __launch_bounds__(kGTh)
     k_g_restrict(int hf, int wf, GLevel gc, GBand br, GBand bc, const float* __restrict__ rf,
                  float* __restrict__ bo, float* __restrict__ xo, float* __restrict__ ro,
                  const int* __restrict__ skip) {
-  if (skipped(skip)) return;
   __shared__ float fs[kFineSpan][kFineSpan];
   __shared__ float bs[kGH1][kGH1];
   __shared__ float xs[kGH1][kGH1];
+  constexpr int NH = passes(kGH1 * kGH1);
   const int nc = gc.h * gc.w;
   const int R0 = blockIdx.y * kGT, C0 = blockIdx.x * kGT;
+  const int skipv = skip ? *skip : 0;
   const int fr0 = br.f_lo[max(R0 - kGR, 0)], fc0 = bc.f_lo[max(C0 - kGR, 0)];
+  int fa[NH], fb[NH];
+  float wr[NH][kKFC], wc[NH][kKFC], dh[NH];
+  bool hin[NH];
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const int R = R0 - kGR + ly, C = C0 - kGR + lx;
+    hin[k] = t < kGH1 * kGH1 && on_grid(R, C, gc.h, gc.w);
+    const int Rc = hin[k] ? R : 0, Cc = hin[k] ? C : 0;
+    fa[k] = br.f_lo[Rc];
+    fb[k] = bc.f_lo[Cc];
+#pragma unroll
+    for (int a = 0; a < kKFC; ++a) {
+      wr[k][a] = br.rw[Rc * kKFC + a];
+      wc[k][a] = bc.rw[Cc * kKFC + a];
+    }
+    dh[k] = gc.dsq[Rc * gc.w + Cc];
+  }
+  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+  const int R = R0 + ty, C = C0 + tx;
+  const bool in = on_grid(R, C, gc.h, gc.w);
+  const int i = in ? R * gc.w + C : 0;
+  float a[kGN];
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = gc.A[(size_t)o * nc + i];
+  __builtin_amdgcn_sched_barrier(0);
+  if (skipv) return;
   {
     constexpr int N = kFineSpan * kFineSpan, NP = passes(N);
     float v[NP];
@@ -928,9 +971,9 @@ __global__ void __launch_bounds__(kGTh)
     for (int k = 0; k < NP; ++k) {
       const int t = threadIdx.x + k * kGTh;
       const int rr = fr0 + t / kFineSpan, cc = fc0 + t % kFineSpan;
-      const bool in = t < N && on_grid(rr, cc, hf, wf);
-      v[k] = rf[in ? rr * wf + cc : 0];
-      v[k] = in ? v[k] : 0.f;
+      const bool fin = t < N && on_grid(rr, cc, hf, wf);
+      v[k] = rf[fin ? rr * wf + cc : 0];
+      v[k] = fin ? v[k] : 0.f;
     }
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -940,41 +983,25 @@ __global__ void __launch_bounds__(kGTh)
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < passes(kGH1 * kGH1); ++k) {
+  for (int k = 0; k < NH; ++k) {
     const int t = threadIdx.x + k * kGTh;
     const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
-    const int R = R0 - kGR + ly, C = C0 - kGR + lx;
-    const bool in = t < kGH1 * kGH1 && on_grid(R, C, gc.h, gc.w);
-    const int Rc = in ? R : 0, Cc = in ? C : 0;
-    const int fa = br.f_lo[Rc] - fr0, fb = bc.f_lo[Cc] - fc0;
-    float wr[kKFC], wc[kKFC];
-#pragma unroll
-    for (int a = 0; a < kKFC; ++a) {
-      wr[a] = br.rw[Rc * kKFC + a];
-      wc[a] = bc.rw[Cc * kKFC + a];
-    }
-    const float d = gc.dsq[Rc * gc.w + Cc];
+    const int ka = fa[k] - fr0, kb = fb[k] - fc0;
     float bv = 0.f;
 #pragma unroll
-    for (int a = 0; a < kKFC; ++a) {
-      const int ia = min(max(fa + a, 0), kFineSpan - 1);
+    for (int q = 0; q < kKFC; ++q) {
+      const int ia = min(max(ka + q, 0), kFineSpan - 1);
       float s = 0.f;
 #pragma unroll
-      for (int q = 0; q < kKFC; ++q) s += wc[q] * fs[ia][min(max(fb + q, 0), kFineSpan - 1)];
-      bv += wr[a] * s;
+      for (int u = 0; u < kKFC; ++u) s += wc[k][u] * fs[ia][min(max(kb + u, 0), kFineSpan - 1)];
+      bv += wr[k][q] * s;
     }
     if (t < kGH1 * kGH1) {
-      bs[ly][lx] = in ? bv : 0.f;
-      xs[ly][lx] = in ? gc.omega * d * d * bv : 0.f;
+      bs[ly][lx] = hin[k] ? bv : 0.f;
+      xs[ly][lx] = hin[k] ? gc.omega * dh[k] * dh[k] * bv : 0.f;
     }
   }
   __syncthreads();
-  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
-  const int R = R0 + ty, C = C0 + tx;
-  const bool in = on_grid(R, C, gc.h, gc.w);
-  const int i = in ? R * gc.w + C : 0;
-  float a[kGN];
-  load_taps(gc.A, nc, i, a);
   const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
   if (!in) return;
   const float bv = bs[ty + kGR][tx + kGR];
@@ -984,38 +1011,80 @@ __global__ void __launch_bounds__(kGTh)
 }
 
 // x_t = M (P^T r_f) on the tail level: every block restricts the whole
-// (small) fine residual into LDS, then one wavefront per row of M
+// (small) fine residual into LDS, then one wavefront per row of M. The row of
+// M, the first fine values and the band rows of the first coarse points are
+// loaded in one group with the skip flag.
 template <int NV>
 __global__ void __launch_bounds__(kGTh)
     k_g_tail(int hf, int wf, int hc, int wc, GBand br, GBand bc, const float* __restrict__ rf,
              const float* __restrict__ M, int ld, float* __restrict__ xo,
              const int* __restrict__ skip) {
-  if (skipped(skip)) return;
   __shared__ float fs[kTailFine];
   __shared__ float bs[kTailCoarse];
+  constexpr int NF = 8, NJ = 2;  // fine values and coarse points per thread in the first group
   const int nf = hf * wf, nc = hc * wc;
   const int i = blockIdx.x * 4 + threadIdx.x / kWave, l = threadIdx.x % kWave;
-  // this wavefront's row of M, in flight while the block restricts
+  const int skipv = skip ? *skip : 0;
   const float4* row = reinterpret_cast<const float4*>(M + (size_t)min(i, nc - 1) * ld);
   float4 m[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) m[v] = row[l + kWave * v];
-  for (int t = threadIdx.x; t < nf; t += kGTh) fs[t] = rf[t];
-  __syncthreads();
-  for (int J = threadIdx.x; J < nc; J += kGTh) {
+  float fv[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    fv[k] = rf[min(t, nf - 1)];
+  }
+  int fa[NJ], fb[NJ];
+  float wr[NJ][kKFC], wcl[NJ][kKFC];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int J = min((int)threadIdx.x + j * kGTh, nc - 1);
     const int R = J / wc, C = J - R * wc;
-    const int fa = br.f_lo[R], fb = bc.f_lo[C];
+    fa[j] = br.f_lo[R];
+    fb[j] = bc.f_lo[C];
+#pragma unroll
+    for (int a = 0; a < kKFC; ++a) {
+      wr[j][a] = br.rw[R * kKFC + a];
+      wcl[j][a] = bc.rw[C * kKFC + a];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (skipv) return;
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    if (t < nf) fs[t] = fv[k];
+  }
+  for (int t = threadIdx.x + NF * kGTh; t < nf; t += kGTh) fs[t] = rf[t];
+  __syncthreads();
+  // P^T r_f at coarse point J from the band rows (ja, war) x (jb, wac)
+  auto restrict_at = [&](int ja, int jb, const float(&war)[kKFC], const float(&wac)[kKFC]) {
     float bv = 0.f;
 #pragma unroll
     for (int a = 0; a < kKFC; ++a) {
-      const float wa = br.rw[R * kKFC + a];
-      const int ra = min(fa + a, hf - 1);
+      const int ra = min(ja + a, hf - 1);
       float s = 0.f;
 #pragma unroll
-      for (int q = 0; q < kKFC; ++q) s += bc.rw[C * kKFC + q] * fs[ra * wf + min(fb + q, wf - 1)];
-      bv += wa * s;
+      for (int q = 0; q < kKFC; ++q) s += wac[q] * fs[ra * wf + min(jb + q, wf - 1)];
+      bv += war[a] * s;
     }
-    bs[J] = bv;
+    return bv;
+  };
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int J = threadIdx.x + j * kGTh;
+    if (J < nc) bs[J] = restrict_at(fa[j], fb[j], wr[j], wcl[j]);
+  }
+  for (int J = threadIdx.x + NJ * kGTh; J < nc; J += kGTh) {
+    const int R = J / wc, C = J - R * wc;
+    float war[kKFC], wac[kKFC];
+#pragma unroll
+    for (int a = 0; a < kKFC; ++a) {
+      war[a] = br.rw[R * kKFC + a];
+      wac[a] = bc.rw[C * kKFC + a];
+    }
+    bs[J] = restrict_at(br.f_lo[R], bc.f_lo[C], war, wac);
   }
   __syncthreads();
   if (i >= nc) return;
@@ -1032,37 +1101,67 @@ __global__ void __launch_bounds__(kGTh)
 }
 
 // fine tile of level l >= 1: x = x_pre + P x_c on the halo, then one damped
-// Jacobi step x + w D^-1 (b - A x) on the tile
+// Jacobi step x + w D^-1 (b - A x) on the tile. As in k_g_restrict, every
+// independent load goes out in the first group; the coarse values follow.
 __global__ void __launch_bounds__(kGTh)
     k_g_prolong(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ xc,
                 const float* __restrict__ xpre, const float* __restrict__ bf,
                 float* __restrict__ xo, const int* __restrict__ skip) {
-  if (skipped(skip)) return;
   __shared__ float cs[kCoarseSpan][kCoarseSpan];
   __shared__ float xs[kGH1][kGH1];
+  constexpr int NH = passes(kGH1 * kGH1);
   const int n = g.h * g.w;
   const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
+  const int skipv = skip ? *skip : 0;
   const int cr0 = br.c_lo[max(r0 - kGR, 0)], cc0 = bc.c_lo[max(c0 - kGR, 0)];
-  stage_coarse(cs, xc, hc, wc, cr0, cc0);
-  __syncthreads();
+  float xp[NH], wr[NH][kKPF], wcl[NH][kKPF];
+  int ka[NH], kb[NH];
+  bool hin[NH];
 #pragma unroll
-  for (int k = 0; k < passes(kGH1 * kGH1); ++k) {
+  for (int k = 0; k < NH; ++k) {
     const int t = threadIdx.x + k * kGTh;
     const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
     const int rr = r0 - kGR + ly, cc = c0 - kGR + lx;
-    const bool in = t < kGH1 * kGH1 && on_grid(rr, cc, g.h, g.w);
-    const int rc = in ? rr : 0, ccc = in ? cc : 0;
-    const float v = xpre[rc * g.w + ccc] + prolong_at(br, bc, rc, ccc, cs, cr0, cc0);
-    if (t < kGH1 * kGH1) xs[ly][lx] = in ? v : 0.f;
+    hin[k] = t < kGH1 * kGH1 && on_grid(rr, cc, g.h, g.w);
+    const int rc = hin[k] ? rr : 0, ccc = hin[k] ? cc : 0;
+    xp[k] = xpre[rc * g.w + ccc];
+    ka[k] = br.c_lo[rc];
+    kb[k] = bc.c_lo[ccc];
+#pragma unroll
+    for (int a = 0; a < kKPF; ++a) {
+      wr[k][a] = br.pw[rc * kKPF + a];
+      wcl[k][a] = bc.pw[ccc * kKPF + a];
+    }
   }
-  __syncthreads();
   const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
   const int rr = r0 + ty, cc = c0 + tx;
   const bool in = on_grid(rr, cc, g.h, g.w);
   const int i = in ? rr * g.w + cc : 0;
   float a[kGN];
-  load_taps(g.A, n, i, a);
+#pragma unroll
+  for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + i];
   const float d = g.dsq[i], bi = bf[i];
+  __builtin_amdgcn_sched_barrier(0);
+  if (skipv) return;
+  stage_coarse(cs, xc, hc, wc, cr0, cc0);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+    const int ia0 = ka[k] - cr0, ib0 = kb[k] - cc0;
+    float v = 0.f;
+#pragma unroll
+    for (int p = 0; p < kKPF; ++p) {
+      const int ia = min(max(ia0 + p, 0), kCoarseSpan - 1);
+      float s2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < kKPF; ++q) s2 += wcl[k][q] * cs[ia][min(max(ib0 + q, 0), kCoarseSpan - 1)];
+      v += wr[k][p] * s2;
+    }
+    if (t < kGH1 * kGH1) xs[ly][lx] = hin[k] ? xp[k] + v : 0.f;
+  }
+  __syncthreads();
   const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
   if (!in) return;
   xo[i] = xs[ty + kGR][tx + kGR] + g.omega * d * d * (bi - ax);
@@ -1142,8 +1241,9 @@ __global__ void __launch_bounds__(kGTh)
   const float dt = g.dsq[T.it], bt = b[T.it];
   BoxLoad<true, float> bl;
   if (STW) bl.load(b, g.dsq, g.h, g.w, T.r0, T.c0);
+  const GateVals gv = gate_load(gate);
   __builtin_amdgcn_sched_barrier(0);
-  if (gate_closed(gate)) return;
+  if (gate_test(gate, gv)) return;
   float v;
   if (STW) {
     bl.store(ub);
@@ -1186,18 +1286,23 @@ __global__ void __launch_bounds__(kGTh)
     for (int o = 0; o < kGN; ++o) a[o] = W[(size_t)o * n + T.it];
   const float dt = g.dsq[T.it];
   const double pt = f.p[T.it], xt = f.x[T.it];
+  const double ro = f.r_old[T.it], qo = f.q[T.it];
   BoxLoad<false, double> rl, ql;
   BoxLoad<false, float> dl;
   rl.load(f.r_old, nullptr, g.h, g.w, T.r0, T.c0);
   ql.load(f.q, nullptr, g.h, g.w, T.r0, T.c0);
   dl.load(g.dsq, nullptr, g.h, g.w, T.r0, T.c0);
+  // the state and the product's partials in the same group
+  const int done = f.st->done, pending = f.st->pending;
+  const double sc = partials_share<kRedBlocks / kGTh>(f.P_c, f.nb);
+  const double sb = partials_share<kRedBlocks / kGTh>(f.P_b, f.nb);
   __builtin_amdgcn_sched_barrier(0);
-  if (f.st->done) return;
-  const bool pend = f.st->pending != 0;
+  if (done) return;
+  const bool pend = pending != 0;
   double alpha = 0.0, pq = 0.0;
   if (pend) {
-    pq = reduce_partials(f.P_c, f.nb);
-    alpha = reduce_partials(f.P_b, f.nb) / pq;
+    pq = partials_total(sc);
+    alpha = partials_total(sb) / pq;
   }
 #pragma unroll
   for (int k = 0; k < BoxLoad<false, float>::NP; ++k) {
@@ -1218,7 +1323,6 @@ __global__ void __launch_bounds__(kGTh)
   }
   // the FP64 new residual of the tile point
   {
-    const double ro = f.r_old[T.it], qo = f.q[T.it];
     const double rn = ro - alpha * qo;
     if (T.in) {
       f.r_new[T.it] = rn;
@@ -1268,7 +1372,6 @@ __global__ void __launch_bounds__(kGTh) k_fcg_flush(int h, int w, FcgFuse f) {
 __global__ void __launch_bounds__(kGTh)
     k_g0_resid(GLevel g, const float* __restrict__ x, const float* __restrict__ b,
                float* __restrict__ r, Gate gate) {
-  if (gate_closed(gate)) return;
   __shared__ float xb[kL0HH][kL0HW];
   const int n = g.h * g.w;
   const L0Tile T = l0_tile(g.h, g.w);
@@ -1278,7 +1381,9 @@ __global__ void __launch_bounds__(kGTh)
   const float bt = b[T.it];
   BoxLoad<false, float> bl;
   bl.load(x, nullptr, g.h, g.w, T.r0, T.c0);
+  const GateVals gv = gate_load(gate);
   __builtin_amdgcn_sched_barrier(0);
+  if (gate_test(gate, gv)) return;
   bl.store(xb);
   __syncthreads();
   const float ax = tap_sum<kL0HW>(a, xb, T.ty + kGR, T.tx + kGR);
@@ -1291,7 +1396,6 @@ __global__ void __launch_bounds__(kGTh)
     k_g0_prolong(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ xc,
                  const float* __restrict__ x0, const float* __restrict__ b0,
                  float* __restrict__ xo, float* __restrict__ uo, const int* __restrict__ skip) {
-  if (skipped(skip)) return;
   __shared__ float cs[kL0CH][kL0CW];
   __shared__ float xb[kL0HH][kL0HW];
   __shared__ int rlo[kL0HH], clo[kL0HW];
@@ -1318,7 +1422,9 @@ __global__ void __launch_bounds__(kGTh)
   BoxLoad<false, float> xl;
   xl.load(x0, nullptr, g.h, g.w, T.r0, T.c0);
   const int cr0 = br.c_lo[max(T.r0 - kGR, 0)], cc0 = bc.c_lo[max(T.c0 - kGR, 0)];
+  const int skipv = skip ? *skip : 0;
   __builtin_amdgcn_sched_barrier(0);
+  if (skipv) return;
   {
     constexpr int N = kL0CH * kL0CW, NP = passes(N);
     float v[NP];
@@ -1377,7 +1483,6 @@ __global__ void __launch_bounds__(kGTh)
     k_g0_smooth_t(GLevel g, const float* __restrict__ W, const float* __restrict__ x,
                   const float* __restrict__ u, float* __restrict__ z,
                   const int* __restrict__ skip, DotOut dot) {
-  if (skipped(skip)) return;
   __shared__ float ubx[kL0HH][kL0HW];
   const int n = g.h * g.w;
   const L0Tile T = l0_tile(g.h, g.w);
@@ -1397,7 +1502,9 @@ __global__ void __launch_bounds__(kGTh)
   const double yt = dot.parts ? dot.y[T.it] : 0.0;
   BoxLoad<false, float> bl;
   if (STW) bl.load(u, nullptr, g.h, g.w, T.r0, T.c0);
+  const int skipv = skip ? *skip : 0;
   __builtin_amdgcn_sched_barrier(0);
+  if (skipv) return;
   float acc;
   if (STW) {
     bl.store(ubx);
@@ -1437,9 +1544,12 @@ __global__ void __launch_bounds__(kGTh)
   BoxLoad<false, double> pl;
   zl.load(z32, nullptr, h, w, T.r0, T.c0);
   pl.load(pp, nullptr, h, w, T.r0, T.c0);
+  const int done = st->done;
+  const double pq_prev = st->pq_prev;
+  const double sa = partials_share<kRedBlocks / kGTh>(P_a, nblk_a);
   __builtin_amdgcn_sched_barrier(0);
-  if (st->done) return;
-  const double beta = -reduce_partials(P_a, nblk_a) / st->pq_prev;
+  if (done) return;
+  const double beta = -partials_total(sa) / pq_prev;
 #pragma unroll
   for (int k = 0; k < BoxLoad<false, float>::NP; ++k) {
     const int t = threadIdx.x + k * kGTh;
@@ -1789,13 +1899,9 @@ __device__ __forceinline__ double band_w(const double (&w)[kKFC], int k) {
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw) {
-  const int l = a.t + blockIdx.y;
-  if (l >= a.lc) return;
+__device__ __forceinline__ void tail_g_entry(const TailArgs& a, const TailWork& tw, int l, int e) {
   const TailLevel& L = a.lv[l];
   const TailLevel& C = a.lv[l + 1];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= L.n * C.n) return;
   const int i = e / C.n, c = e - i * C.n;
   const int ir = i / L.w, ic = i - ir * L.w, cr = c / C.w, cc = c - cr * C.w;
   const int flo1 = L.f_lo[1][cr], flo0 = L.f_lo[0][cc];
@@ -1818,12 +1924,21 @@ __global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw) {
   a.work[tw.goff[l] + e] = band_w(w1, ir - flo1) * band_w(w0, ic - flo0) - mi * s;
 }
 
+// G_l for the levels l0 + blockIdx.y (below lc)
+__global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw, int l0) {
+  const int l = l0 + blockIdx.y;
+  if (l >= a.lc) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.lv[l].n * a.lv[l + 1].n) return;
+  tail_g_entry(a, tw, l, e);
+}
+
 // one block: the coarse inverse (Gauss-Jordan in LDS, SPD without pivoting)
 // and V_l = G_l V_{l+1} G_l^T + 2M - M A M for l = lc-1 .. t+1; V_{t+1} ends
 // at the front of the work
 constexpr int kTailLds = 4800;  // doubles of LDS: the elimination, then V_{l+1}, G_l, T
 
-__global__ void __launch_bounds__(1024) k_tail_small(TailArgs a, TailWork tw) {
+__device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWork& tw) {
   __shared__ double M[kTailLds];
   __shared__ double fac[kGjLds];
   const TailLevel& C = a.lv[a.lc];
@@ -1917,6 +2032,8 @@ __global__ void __launch_bounds__(1024) k_tail_small(TailArgs a, TailWork tw) {
     for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) Va[e] = Vc[e];
   }
 }
+
+__global__ void __launch_bounds__(1024) k_tail_small(TailArgs a, TailWork tw) { tail_small_body(a, tw); }
 
 // W = V_{t+1} G_t^T (n_{t+1} x n_t), one thread per entry
 __global__ void __launch_bounds__(256) k_tail_w(TailArgs a, TailWork tw) {
@@ -2417,13 +2534,21 @@ extern "C" int64_t iga_mg_tail_dense_doubles(const iga_mg_setup_t* s, int32_t le
   return tail_dense_doubles(*s, level);
 }
 
-// the dense tail of level t from the tensor set-up (after the Galerkin passes)
-static int tail_dense(const iga_mg_setup_t& s, hipStream_t st) {
+static int tail_dense_check(const iga_mg_setup_t& s) {
   const iga_mg_setup_grid_t& G = *s.grid;
   const int t = G.tail_level, lc = s.n_levels - 1;
   if (t < 1 || t >= lc || !G.tail || !G.tail_work || !s.coarse_inv) return IGA_ERR_ARG;
   if (s.level[lc].n > kGjLds || s.level[t + 1].n > 128 || s.level[t].n > kDenseMaxRows)
     return IGA_ERR_UNSUPPORTED;
+  return IGA_OK;
+}
+
+// the dense tail of level t from the tensor set-up (after the Galerkin passes)
+static int tail_dense(const iga_mg_setup_t& s, hipStream_t st) {
+  const iga_mg_setup_grid_t& G = *s.grid;
+  const int t = G.tail_level, lc = s.n_levels - 1;
+  const int rc = tail_dense_check(s);
+  if (rc) return rc;
   TailArgs a{};
   a.t = t;
   a.lc = lc;
@@ -2446,7 +2571,7 @@ static int tail_dense(const iga_mg_setup_t& s, hipStream_t st) {
   int gmax = 1;
   for (int l = t; l < lc; ++l) gmax = std::max(gmax, s.level[l].n * s.level[l + 1].n);
   const int nt = s.level[t].n, nc = s.level[t + 1].n, ld = dense_ld(nt);
-  hipLaunchKernelGGL(k_tail_g, dim3((gmax + 255) / 256, lc - t), dim3(256), 0, st, a, tw);
+  hipLaunchKernelGGL(k_tail_g, dim3((gmax + 255) / 256, lc - t), dim3(256), 0, st, a, tw, t);
   hipLaunchKernelGGL(k_tail_small, dim3(1), dim3(1024), 0, st, a, tw);
   hipLaunchKernelGGL(k_tail_w, dim3((nc * nt + 255) / 256), dim3(256), 0, st, a, tw);
   hipLaunchKernelGGL(k_tail_v, dim3((ld + 255) / 256, nt), dim3(256), 0, st, a, tw, G.tail, ld);
