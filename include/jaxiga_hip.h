@@ -586,6 +586,22 @@ typedef struct iga_mg_setup_grid {
    * may run bit 0, start work that reads the level-0 features on another
    * stream, and run bit 1 beside it. */
   int32_t stages;
+  /* optional (gal_multi != 0): coarse levels formed up to three at a time.
+   * Level m (1 <= m < n_levels) is formed from level gal_src[m] (m-3 <= src
+   * < m) through the product of the transfers in between, whose coarse-column
+   * band is gal_f_lo[m][d], gal_rw[m][d] (rows of 6, 14 or 30 FP64 values for
+   * one, two or three levels); the levels formed from one source are the
+   * consecutive levels below it and take one pass-0 and one pass-1 launch in
+   * all. gal_work: per launch the sum over its levels of 49 h_src w_m doubles,
+   * then 32 max_l w_l zeros; a level that is the source of levels two or
+   * three down carries 32 zeros past the end of its op64 stencil. The caller
+   * checks that the wide bands' neighbour offsets stay inside the LDS rows
+   * the passes stage them in (16 and 28 zeros around the 14 and 30 values). */
+  int32_t gal_multi;
+  int32_t gal_src[16];
+  const int32_t* gal_f_lo[16][2];
+  const double* gal_rw[16][2];
+  double* gal_work;
 } iga_mg_setup_grid_t;
 
 typedef struct iga_mg_setup {

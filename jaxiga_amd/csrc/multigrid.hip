@@ -1706,16 +1706,20 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
 //          K[(r, c), (r + dr, c')] P0[c', C + dC]
 //   pass 1 (across rows):     K_c[(R, C), (dR, dC)] = sum_{r, r'} P1[r, R]
 //          K'[(r, C), (r', C + dC)] P1[r', R + dR]
-// Each thread keeps u[j] = sum_c P[c, C] K[c, c'] for the 12 points c' of
-// its window (c' = f_lo[C] - 3 + j), then forms the 7 outputs of its offset
-// line from u and the coarse-column bands of its neighbours.
+// Each thread keeps u[j] = sum_c P[c, C] K[c, c'] for the KF + 6 points c'
+// of its window (c' = f_lo[C] - 3 + j), then forms the 7 outputs of its
+// offset line from u and the coarse-column bands of its neighbours. KF is
+// the band width (fine values per coarse column, the row stride of rw): 6
+// for one level's transfer; a coarse level two or three levels down is
+// formed straight from a finer one through the product of the transfers in
+// between, whose bands are 14 and 30 wide. The passes are the same: P^T K P
+// with P = P_s ... P_{m-1} equals the level-by-level products.
 // ---------------------------------------------------------------------------
-
-constexpr int kGalWin = kKFC + 2 * kGR;  // 12
 
 // out[d] = sum_j u[j] P[f0 - 3 + j, C + d - 3] for d < 7 (zero for a coarse
 // index off [0, nc)); P[f, C2] = rw[C2][f - f_lo[C2]] inside the band
-__device__ __forceinline__ void gal_close(const double (&u)[kGalWin], int f0, int C, int nc,
+template <int KF>
+__device__ __forceinline__ void gal_close(const double (&u)[KF + 2 * kGR], int f0, int C, int nc,
                                           const int32_t* __restrict__ f_lo,
                                           const double* __restrict__ rw, double (&out)[kGS]) {
 #pragma unroll
@@ -1726,20 +1730,78 @@ __device__ __forceinline__ void gal_close(const double (&u)[kGalWin], int f0, in
     const int off = f0 - kGR - f_lo[Cc];
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < kGalWin; ++j) {
+    for (int j = 0; j < KF + 2 * kGR; ++j) {
       const int k = j + off;
-      const double wv = rw[Cc * kKFC + min(max(k, 0), kKFC - 1)];
-      s += (k >= 0 && k < kKFC) ? u[j] * wv : 0.0;
+      const double wv = rw[Cc * KF + min(max(k, 0), KF - 1)];
+      s += (k >= 0 && k < KF) ? u[j] * wv : 0.0;
     }
     out[d] = in ? s : 0.0;
   }
+}
+
+// Wide bands (two or three levels): the band of the close step is staged in
+// LDS with gal_pad zeros before and after every column's KF values, so the
+// KF + 6 values a neighbour contributes are read at consecutive addresses
+// from one base, with no test per value (the host checks that the offsets
+// stay inside the padding); the window's operator reads are unclamped (the
+// band is zero past a column's last fine point, and the fine stencils carry
+// kGalTail zero values past their end), consecutive per stencil offset.
+constexpr int kGalTail = 32;
+constexpr int kBufWord3 = 0x00020000;  // raw buffer descriptor, dword 3 (32-bit data)
+constexpr int kGalLdsBand = 4096;  // doubles of LDS for the padded band
+
+template <int KF>
+constexpr int gal_pad() {
+  return KF <= kKFC ? 9 : KF <= 14 ? 16 : 28;
+}
+
+template <int KF>
+constexpr int gal_row() {
+  return KF + 2 * gal_pad<KF>();
+}
+
+template <int KF>
+__device__ __forceinline__ void gal_close_lds(const double (&u)[KF + 2 * kGR], int f0, int C, int nc,
+                                              const int32_t* __restrict__ f_lo,
+                                              const double* __restrict__ s_rw, double (&out)[kGS]) {
+  int base[kGS];
+#pragma unroll
+  for (int d = 0; d < kGS; ++d) {
+    const int C2 = C + d - kGR;
+    const int Cc = C2 >= 0 && C2 < nc ? C2 : 0;
+    // inside the staged band (off-grid neighbours read any values, dropped)
+    base[d] = min(max(Cc * gal_row<KF>() + gal_pad<KF>() + (f0 - kGR - f_lo[Cc]), 0),
+                  nc * gal_row<KF>() - (KF + 2 * kGR));
+    out[d] = 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < KF + 2 * kGR; ++j) {
+#pragma unroll
+    for (int d = 0; d < kGS; ++d) out[d] += u[j] * s_rw[base[d] + j];
+    if (KF > kKFC && j % 6 == 5) __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int d = 0; d < kGS; ++d)
+    if (C + d - kGR < 0 || C + d - kGR >= nc) out[d] = 0.0;
+}
+
+// the padded band of n coarse columns into LDS (whole workgroup)
+template <int KF>
+__device__ __forceinline__ void gal_stage_band(const double* __restrict__ rw, int n,
+                                               double* __restrict__ s_rw) {
+  constexpr int RS = gal_row<KF>();
+  for (int i = threadIdx.x; i < n * RS; i += blockDim.x) {
+    const int c = i / RS, k = i - c * RS - gal_pad<KF>();
+    s_rw[i] = k >= 0 && k < KF ? rw[c * KF + k] : 0.0;
+  }
+  __syncthreads();
 }
 
 // One 1-D Galerkin step of a level (see above): the arguments of both passes.
 struct GalLevel {
   int hf, wf, hc, wc;
   const int32_t* f_lo[2];
-  const double* rw[2];
+  const double* rw[2];  // rows of KF values
   const double* K;  // fine FP64 stencil
   double* Kp;       // intermediate
   double* S64;      // coarse FP64 stencil
@@ -1753,25 +1815,43 @@ struct GalLevel {
   int regular;  // every coarse CSR row is exactly its clipped 7 x 7 window
 };
 
-// pass 0 entry t = (dr, r, C), C fastest
-__device__ __forceinline__ void gal_pass0_entry(const GalLevel& g, int64_t t) {
+// pass 0 entry t = (dr, r, C), C fastest; WIDE: the band of the close step
+// is the padded LDS copy s_rw
+template <int KF, bool WIDE>
+__device__ __forceinline__ void gal_pass0_entry(const GalLevel& g, int64_t t, const double* s_rw) {
   const int64_t np = (int64_t)g.hf * g.wc;
   const int C = (int)(t % g.wc), r = (int)((t / g.wc) % g.hf), dr = (int)(t / np) - kGR;
   const int64_t nf = (int64_t)g.hf * g.wf;
   const int f0 = g.f_lo[0][C];
-  double u[kGalWin];
+  double u[KF + 2 * kGR];
 #pragma unroll
-  for (int j = 0; j < kGalWin; ++j) u[j] = 0.0;
+  for (int j = 0; j < KF + 2 * kGR; ++j) u[j] = 0.0;
+  if constexpr (WIDE) {
+    const double* row = g.K + (int64_t)(dr + kGR) * kGS * nf + (int64_t)r * g.wf + f0;
 #pragma unroll
-  for (int a = 0; a < kKFC; ++a) {
-    const double wa = g.rw[0][C * kKFC + a];
-    const int64_t idx = (int64_t)r * g.wf + min(f0 + a, g.wf - 1);
+    for (int dc = -kGR; dc <= kGR; ++dc) {
+      const double* kr = row + (int64_t)(dc + kGR) * nf;
 #pragma unroll
-    for (int dc = -kGR; dc <= kGR; ++dc)
-      u[a + dc + kGR] += wa * g.K[((dr + kGR) * kGS + dc + kGR) * nf + idx];
+      for (int a = 0; a < KF; ++a) u[a + dc + kGR] += g.rw[0][C * KF + a] * kr[a];
+      // wide bands: one offset's reads in flight at a time (all of them
+      // would not fit the registers)
+      if (KF > kKFC) __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < KF; ++a) {
+      const double wa = g.rw[0][C * KF + a];
+      const int64_t idx = (int64_t)r * g.wf + min(f0 + a, g.wf - 1);
+#pragma unroll
+      for (int dc = -kGR; dc <= kGR; ++dc)
+        u[a + dc + kGR] += wa * g.K[((dr + kGR) * kGS + dc + kGR) * nf + idx];
+    }
   }
   double out[kGS];
-  gal_close(u, f0, C, g.wc, g.f_lo[0], g.rw[0], out);
+  if constexpr (WIDE)
+    gal_close_lds<KF>(u, f0, C, g.wc, g.f_lo[0], s_rw, out);
+  else
+    gal_close<KF>(u, f0, C, g.wc, g.f_lo[0], g.rw[0], out);
 #pragma unroll
   for (int d = 0; d < kGS; ++d) g.Kp[((dr + kGR) * kGS + d) * np + (int64_t)r * g.wc + C] = out[d];
 }
@@ -1779,23 +1859,45 @@ __device__ __forceinline__ void gal_pass0_entry(const GalLevel& g, int64_t t) {
 // pass 1 entry t = (dC, R, C), C fastest: the coarse stencil line (., dC) of
 // point (R, C) in FP64 (and FP32) stencil layout, its CSR values and, for
 // dC = 0, the diagonal scalings (1 where the diagonal is zero)
-__device__ __forceinline__ void gal_pass1_entry(const GalLevel& g, int64_t t) {
+template <int KF, bool WIDE>
+__device__ __forceinline__ void gal_pass1_entry(const GalLevel& g, int64_t t, const double* s_rw) {
   const int64_t nc = (int64_t)g.hc * g.wc, np = (int64_t)g.hf * g.wc;
   const int C = (int)(t % g.wc), R = (int)((t / g.wc) % g.hc), dC = (int)(t / nc) - kGR;
   const int f0 = g.f_lo[1][R];
-  double u[kGalWin];
+  double u[KF + 2 * kGR];
 #pragma unroll
-  for (int j = 0; j < kGalWin; ++j) u[j] = 0.0;
+  for (int j = 0; j < KF + 2 * kGR; ++j) u[j] = 0.0;
+  if constexpr (WIDE) {
+    // buffer reads: the lane's offset in a register, each value's distance
+    // from it (uniform) in a scalar register (the host keeps the
+    // intermediate below 2 GiB)
+    const __amdgpu_buffer_rsrc_t kp = __builtin_amdgcn_make_buffer_rsrc(g.Kp, (short)0, 0x7fffffff, kBufWord3);
+    const int voff = (int)(((dC + kGR) * np + (int64_t)f0 * g.wc + C) * 8);
 #pragma unroll
-  for (int a = 0; a < kKFC; ++a) {
-    const double wa = g.rw[1][R * kKFC + a];
-    const int64_t idx = (int64_t)min(f0 + a, g.hf - 1) * g.wc + C;
+    for (int dr = -kGR; dr <= kGR; ++dr) {
 #pragma unroll
-    for (int dr = -kGR; dr <= kGR; ++dr)
-      u[a + dr + kGR] += wa * g.Kp[((dr + kGR) * kGS + dC + kGR) * np + idx];
+      for (int a = 0; a < KF; ++a) {
+        const int soff = (int)((((int64_t)(dr + kGR) * kGS) * np + (int64_t)a * g.wc) * 8);
+        const double v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(kp, voff, soff, 0));
+        u[a + dr + kGR] += g.rw[1][R * KF + a] * v;
+      }
+      if (KF > kKFC) __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < KF; ++a) {
+      const double wa = g.rw[1][R * KF + a];
+      const int64_t idx = (int64_t)min(f0 + a, g.hf - 1) * g.wc + C;
+#pragma unroll
+      for (int dr = -kGR; dr <= kGR; ++dr)
+        u[a + dr + kGR] += wa * g.Kp[((dr + kGR) * kGS + dC + kGR) * np + idx];
+    }
   }
   double out[kGS];
-  gal_close(u, f0, R, g.hc, g.f_lo[1], g.rw[1], out);
+  if constexpr (WIDE)
+    gal_close_lds<KF>(u, f0, R, g.hc, g.f_lo[1], s_rw, out);
+  else
+    gal_close<KF>(u, f0, R, g.hc, g.f_lo[1], g.rw[1], out);
   const int I = R * g.wc + C;
   const bool cin = C + dC >= 0 && C + dC < g.wc;
   const int lo = g.indptr[I], hi = g.indptr[I + 1];
@@ -1832,12 +1934,60 @@ __device__ __forceinline__ void gal_pass1_entry(const GalLevel& g, int64_t t) {
 
 __global__ void __launch_bounds__(256) k_gal_pass0(GalLevel g) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (int64_t)kGS * g.hf * g.wc) gal_pass0_entry(g, t);
+  if (t < (int64_t)kGS * g.hf * g.wc) gal_pass0_entry<kKFC, false>(g, t, nullptr);
 }
 
 __global__ void __launch_bounds__(256) k_gal_pass1(GalLevel g) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (int64_t)kGS * g.hc * g.wc) gal_pass1_entry(g, t);
+  if (t < (int64_t)kGS * g.hc * g.wc) gal_pass1_entry<kKFC, false>(g, t, nullptr);
+}
+
+// Band widths of the transfer from a level to the levels one, two and three
+// below it, and the passes that form up to three coarse levels from one fine
+// level in one launch each (g[k] the level k + 1 below; blockIdx.y = NT-1-k,
+// so the blocks of the widest bands are dispatched first)
+constexpr int kGalKF[3] = {kKFC, 14, 30};
+
+struct GalMulti {
+  GalLevel g[3];
+};
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_gal_multi0(GalMulti m) {
+  __shared__ double s_rw[kGalLdsBand];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = NT - 1 - (int)blockIdx.y;
+  if (k == 0) {
+    const GalLevel& g = m.g[0];
+    if (t < (int64_t)kGS * g.hf * g.wc) gal_pass0_entry<kGalKF[0], false>(g, t, nullptr);
+  } else if (NT > 1 && k == 1) {
+    const GalLevel& g = m.g[1];
+    gal_stage_band<kGalKF[1]>(g.rw[0], g.wc, s_rw);
+    if (t < (int64_t)kGS * g.hf * g.wc) gal_pass0_entry<kGalKF[1], true>(g, t, s_rw);
+  } else if (NT > 2 && k == 2) {
+    const GalLevel& g = m.g[2];
+    gal_stage_band<kGalKF[2]>(g.rw[0], g.wc, s_rw);
+    if (t < (int64_t)kGS * g.hf * g.wc) gal_pass0_entry<kGalKF[2], true>(g, t, s_rw);
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) k_gal_multi1(GalMulti m) {
+  __shared__ double s_rw[kGalLdsBand];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = NT - 1 - (int)blockIdx.y;
+  if (k == 0) {
+    const GalLevel& g = m.g[0];
+    if (t < (int64_t)kGS * g.hc * g.wc) gal_pass1_entry<kGalKF[0], false>(g, t, nullptr);
+  } else if (NT > 1 && k == 1) {
+    const GalLevel& g = m.g[1];
+    gal_stage_band<kGalKF[1]>(g.rw[1], g.hc, s_rw);
+    if (t < (int64_t)kGS * g.hc * g.wc) gal_pass1_entry<kGalKF[1], true>(g, t, s_rw);
+  } else if (NT > 2 && k == 2) {
+    const GalLevel& g = m.g[2];
+    gal_stage_band<kGalKF[2]>(g.rw[1], g.hc, s_rw);
+    if (t < (int64_t)kGS * g.hc * g.wc) gal_pass1_entry<kGalKF[2], true>(g, t, s_rw);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1938,6 +2088,53 @@ __global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw, int l0)
 // at the front of the work
 constexpr int kTailLds = 4800;  // doubles of LDS: the elimination, then V_{l+1}, G_l, T
 
+// Gauss-Jordan on [A | I] for n <= kGjWave in one wavefront: lane j holds
+// column j in registers; step k scales row k by the pivot (lane k's entry
+// k) and subtracts M[i][k] (lane k's entry i, read before its own update)
+// times row k from every other row i -- the operations of the elimination in
+// LDS below, in the same order, without a barrier per step. Writes the
+// inverse (FP64 into inv64, FP32 into inv32), row-major n x n.
+constexpr int kGjWave = kWave / 2;
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ void gj_wave(const TailLevel& C, int n, double* __restrict__ inv64,
+                                        float* __restrict__ inv32) {
+  const int j = threadIdx.x % kWave;
+  double col[kGjWave];
+#pragma unroll
+  for (int i = 0; i < kGjWave; ++i)
+    col[i] = (i < n && j < 2 * n) ? (j < n ? a_entry(C, i, j) : (j == n + i ? 1.0 : 0.0)) : 0.0;
+#pragma unroll
+  for (int k = 0; k < kGjWave; ++k) {
+    if (k < n) {
+      const double piv = readlane_f64(col[k], k);
+      col[k] /= piv;
+#pragma unroll
+      for (int i = 0; i < kGjWave; ++i) {
+        if (i != k && i < n) {
+          const double f = readlane_f64(col[i], k);
+          col[i] -= f * col[k];
+        }
+      }
+    }
+  }
+  if (j >= n && j < 2 * n) {
+#pragma unroll
+    for (int i = 0; i < kGjWave; ++i) {
+      if (i < n) {
+        inv64[i * n + j - n] = col[i];
+        inv32[i * n + j - n] = (float)col[i];
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWork& tw) {
   __shared__ double M[kTailLds];
   __shared__ double fac[kGjLds];
@@ -1946,30 +2143,36 @@ __device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWor
   double* Va = a.work;
   double* Vb = a.work + tw.vmax;
   double* T = a.work + 2 * tw.vmax;
-  for (int e = threadIdx.x; e < n * n2; e += blockDim.x) {
-    const int i = e / n2, j = e - i * n2;
-    M[e] = j < n ? a_entry(C, i, j) : (j == n + i ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    const double piv = M[k * n2 + k];
+  if (n <= kGjWave) {
+    // one wavefront: lane j holds column j of [A | I], no barrier per step
+    if (threadIdx.x < kWave) gj_wave(C, n, Va, a.coarse_inv);
     __syncthreads();
-    for (int j = threadIdx.x; j < n2; j += blockDim.x) M[k * n2 + j] /= piv;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = M[i * n2 + k];
-    __syncthreads();
+  } else {
     for (int e = threadIdx.x; e < n * n2; e += blockDim.x) {
       const int i = e / n2, j = e - i * n2;
-      if (i != k) M[e] -= fac[i] * M[k * n2 + j];
+      M[e] = j < n ? a_entry(C, i, j) : (j == n + i ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+      const double piv = M[k * n2 + k];
+      __syncthreads();
+      for (int j = threadIdx.x; j < n2; j += blockDim.x) M[k * n2 + j] /= piv;
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = M[i * n2 + k];
+      __syncthreads();
+      for (int e = threadIdx.x; e < n * n2; e += blockDim.x) {
+        const int i = e / n2, j = e - i * n2;
+        if (i != k) M[e] -= fac[i] * M[k * n2 + j];
+      }
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      const int i = e / n, j = e - i * n;
+      Va[e] = M[i * n2 + n + j];
+      a.coarse_inv[e] = (float)M[i * n2 + n + j];
     }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e / n, j = e - i * n;
-    Va[e] = M[i * n2 + n + j];
-    a.coarse_inv[e] = (float)M[i * n2 + n + j];
-  }
-  __syncthreads();
   double* Vc = Va;
   double* Vn = Vb;
   for (int l = a.lc - 1; l > a.t; --l) {
@@ -1991,13 +2194,27 @@ __device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWor
         st2[e] = s;
       }
       __syncthreads();
-      for (int e = threadIdx.x; e < nl * nl; e += blockDim.x) {
-        const int i = e / nl, j = e - i * nl;
-        double s = 0.0;
-        for (int k = 0; k < nc; ++k) s += st2[i * nc + k] * sg[j * nc + k];
-        const double mi = a.omega * L.dinv[i], mj = a.omega * L.dinv[j];
-        s += (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * mj;
-        Vn[e] = s;
+      // the global reads of a thread's entries (operator entry, diagonal
+      // scalings) are issued together before the products
+      constexpr int kPer = 8;
+      for (int e0 = threadIdx.x; e0 < nl * nl; e0 += kPer * blockDim.x) {
+        double lin[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const int e = min(e0 + q * (int)blockDim.x, nl * nl - 1);
+          const int i = e / nl, j = e - i * nl;
+          const double mi = a.omega * L.dinv[i], mj = a.omega * L.dinv[j];
+          lin[q] = (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * mj;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const int e = e0 + q * (int)blockDim.x;
+          if (e >= nl * nl) break;
+          const int i = e / nl, j = e - i * nl;
+          double s = 0.0;
+          for (int k = 0; k < nc; ++k) s += st2[i * nc + k] * sg[j * nc + k];
+          Vn[e] = s + lin[q];
+        }
       }
       __syncthreads();
       double* tmp = Vc;
@@ -2611,6 +2828,23 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     }
     if (s->radius != kGR) return IGA_ERR_UNSUPPORTED;
     if (G->stages < 0 || G->stages > 3) return IGA_ERR_ARG;
+    if (G->gal_multi) {
+      if (!G->gal_work) return IGA_ERR_ARG;
+      for (int m = 1; m < L; ++m) {
+        const int src = G->gal_src[m];
+        if (src < 0 || src >= m || m - src > 3 || !G->gal_f_lo[m][0] || !G->gal_f_lo[m][1] ||
+            !G->gal_rw[m][0] || !G->gal_rw[m][1])
+          return IGA_ERR_ARG;
+        // the levels of one source are consecutive below it
+        if (m - src > 1 && G->gal_src[m - 1] != src) return IGA_ERR_ARG;
+        // wide bands are staged whole in LDS; the intermediate of a launch
+        // is addressed with 32-bit byte offsets
+        const int row = m - src == 1 ? 0 : m - src == 2 ? gal_row<kGalKF[1]>() : gal_row<kGalKF[2]>();
+        if ((int64_t)std::max(s->level[m].h, s->level[m].w) * row > kGalLdsBand ||
+            (int64_t)kGN * s->level[src].h * s->level[m].w * 8 * 3 >= INT32_MAX)
+          return IGA_ERR_UNSUPPORTED;
+      }
+    }
   }
   const int stages = G && G->stages ? G->stages : 3;
   if (G && (stages & 1)) {
@@ -2661,13 +2895,53 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
       g.regular = G->regular[l + 1];
       return g;
     };
-    for (int l = 0; l + 1 < L; ++l) {
-      const iga_mg_setup_level_t& F = s->level[l];
-      const iga_mg_setup_level_t& C = s->level[l + 1];
-      const int64_t n0 = (int64_t)kGS * F.h * C.w, n1 = (int64_t)kGS * C.n;
-      const GalLevel g = gal(l);
-      hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, g);
-      hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, g);
+    if (G->gal_multi) {
+      // the levels below each source level, up to three per launch pair
+      for (int src = 0; src + 1 < L;) {
+        GalMulti m{};
+        int nt = 0;
+        int64_t b0 = 0, b1 = 0, woff = 0;
+        while (nt < 3 && src + nt + 1 < L && G->gal_src[src + nt + 1] == src) {
+          const int tl = src + nt + 1;
+          const iga_mg_setup_level_t& F = s->level[src];
+          const iga_mg_setup_level_t& C = s->level[tl];
+          GalLevel g = gal(tl - 1);
+          for (int d = 0; d < 2; ++d) {
+            g.f_lo[d] = G->gal_f_lo[tl][d];
+            g.rw[d] = G->gal_rw[tl][d];
+          }
+          g.hf = F.h;
+          g.wf = F.w;
+          g.K = G->op64[src];
+          g.Kp = G->gal_work + woff;
+          woff += (int64_t)kGN * F.h * C.w;
+          b0 = std::max(b0, ((int64_t)kGS * F.h * C.w + 255) / 256);
+          b1 = std::max(b1, ((int64_t)kGS * C.n + 255) / 256);
+          m.g[nt++] = g;
+        }
+        if (nt == 0) return IGA_ERR_ARG;
+        const dim3 g0((unsigned)b0, nt), g1((unsigned)b1, nt);
+        if (nt == 1) {
+          hipLaunchKernelGGL(k_gal_multi0<1>, g0, dim3(256), 0, st, m);
+          hipLaunchKernelGGL(k_gal_multi1<1>, g1, dim3(256), 0, st, m);
+        } else if (nt == 2) {
+          hipLaunchKernelGGL(k_gal_multi0<2>, g0, dim3(256), 0, st, m);
+          hipLaunchKernelGGL(k_gal_multi1<2>, g1, dim3(256), 0, st, m);
+        } else {
+          hipLaunchKernelGGL(k_gal_multi0<3>, g0, dim3(256), 0, st, m);
+          hipLaunchKernelGGL(k_gal_multi1<3>, g1, dim3(256), 0, st, m);
+        }
+        src += nt;
+      }
+    } else {
+      for (int l = 0; l + 1 < L; ++l) {
+        const iga_mg_setup_level_t& F = s->level[l];
+        const iga_mg_setup_level_t& C = s->level[l + 1];
+        const int64_t n0 = (int64_t)kGS * F.h * C.w, n1 = (int64_t)kGS * C.n;
+        const GalLevel g = gal(l);
+        hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, g);
+        hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, g);
+      }
     }
   } else if (!G) {
     for (int l = 0; l + 1 < L; ++l) {

@@ -302,6 +302,7 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     # the generator reads only level 0's features: it runs on a second
     # stream beside the Galerkin levels and the dense tail, which are
     # enqueued first so that the host's work on the generator overlaps them
+    # (captured in a graph with the generator first, the set-up is slower)
     op.load(vals64, stage=1)
     main = torch.cuda.current_stream()
     level0 = torch.cuda.Event()

@@ -212,6 +212,60 @@ class Hierarchy:
         self._bands["tensor"] = out
         return out
 
+    def multi_bands(self, depth=None):
+        """Schedule and device bands of the multi-level Galerkin set-up: level
+        m is formed from level ``src[m] = D floor((m - 1) / D)`` (D = ``depth``
+        levels per launch, GAL_DEPTH by default) through the product of the
+        1-D transfers in between (bands of GAL_KF[m - src] values per coarse
+        column); ``(src, [(f_lo, rw64) per direction] per level, work
+        doubles)``, or None when a product's band or coupling does not fit
+        (the set-up then forms the levels one by one)."""
+        import torch
+
+        D = GAL_DEPTH if depth is None else depth
+        if ("multi", D) in self._bands:
+            return self._bands[("multi", D)]
+        out = None
+        lv = self.levels
+        if self.tensor_bands() is not None:
+            dev = lv[0].indptr.device
+            src, bands, work = [0] * len(lv), [None] * len(lv), 0
+            ok = 1 <= D <= 3
+            for m in range(1, len(lv) if ok else 0):
+                s = D * ((m - 1) // D)
+                kf = GAL_KF[m - s]
+                per_dir = []
+                # wide bands are staged in LDS; the intermediates of a launch
+                # are addressed with 32-bit byte offsets
+                if (m - s > 1 and max(lv[m].h, lv[m].w) * (kf + 2 * GAL_PAD[kf]) > GAL_LDS_BAND) or \
+                        3 * (2 * GRID_RADIUS + 1) ** 2 * lv[s].h * lv[m].w * 8 >= 2 ** 31:
+                    ok = False
+                    break
+                for d in range(2):
+                    T = np.asarray(lv[s].P1[d], float)
+                    for l in range(s + 1, m):
+                        T = T @ np.asarray(lv[l].P1[d], float)
+                    T[np.abs(T) < 1e-14] = 0.0
+                    # one level down: the narrow pass reads no padded rows
+                    tb = wide_band(T, kf, pad=m - s > 1)
+                    if tb is None:
+                        ok = False
+                        break
+                    per_dir.append(tuple(torch.as_tensor(a, device=dev) for a in tb))
+                if not ok:
+                    break
+                src[m], bands[m] = s, per_dir
+            if ok:
+                # the intermediates of a launch, and zeros past them for the
+                # unclamped reads of the last one
+                for s in range(0, len(lv) - 1, D):
+                    work = max(work, sum((2 * GRID_RADIUS + 1) ** 2 * lv[s].h * lv[m].w
+                                         for m in range(s + 1, min(s + D + 1, len(lv)))))
+                work += GAL_TAIL * max(l.w for l in lv)
+                out = (src, bands, work)
+        self._bands[("multi", D)] = out
+        return out
+
     def grid_bands(self, tail_level):
         """Device band tables of the transfers of levels ``0 .. tail_level-1``
         for the grid V-cycle, or None when a level's operator reaches beyond
@@ -282,6 +336,58 @@ def band_tables(T):
     if np.any(np.diff(c_lo) < 0) or np.any(np.diff(f_lo) < 0):
         return None
     return c_lo, pw, f_lo, rw
+
+
+# band widths of the product of one, two and three 1-D transfers (fine values
+# per coarse column; dyadic cubic transfers need 5, 13 and 29), the zeros the
+# passes stage around each column's band in LDS, the LDS they stage it in
+# (doubles), and the zero values past the end of every FP64 stencil that the
+# passes' unclamped reads may reach
+GAL_KF = (0, KFC, 14, 30)
+GAL_DEPTH = 3  # coarse levels formed per launch pair
+GAL_PAD = {KFC: 9, 14: 16, 30: 28}
+GAL_LDS_BAND = 4096
+GAL_TAIL = 32
+
+
+def wide_band(T, kf, pad=True):
+    """Coarse-column band ``(f_lo, rw64)`` of a 1-D transfer (fine x coarse)
+    with ``kf`` values per column, for the tensor Galerkin passes; None if a
+    column is wider, first indices decrease, the coupling of the passes
+    (every fine point of column C moved by up to GRID_RADIUS points lies only
+    in columns within GRID_RADIUS of C) does not hold, or (``pad``) the
+    neighbour offsets leave the padded LDS rows."""
+    T = np.asarray(T, float)
+    nf, nc = T.shape
+    f_lo = np.zeros(nc, np.int32)
+    rw = np.zeros((nc, kf), np.float64)
+    for c in range(nc):
+        nz = np.flatnonzero(T[:, c])
+        if len(nz):
+            if nz[-1] - nz[0] >= kf:
+                return None
+            f_lo[c] = nz[0]
+        k = min(kf, nf - f_lo[c])
+        rw[c, :k] = T[f_lo[c]: f_lo[c] + k, c]
+    if np.any(np.diff(f_lo) < 0):
+        return None
+    # the close step reads KF + 6 consecutive padded band values of each
+    # neighbour column from f_lo[C] - 3 - f_lo[C2]
+    for dC in range(-GRID_RADIUS, GRID_RADIUS + 1) if pad else ():
+        C = np.arange(max(0, -dC), min(nc, nc - dC))
+        off = f_lo[C] - GRID_RADIUS - f_lo[C + dC]
+        if len(off) and (off.min() < -GAL_PAD[kf] or off.max() > GAL_PAD[kf] - 2 * GRID_RADIUS):
+            return None
+    nzr = T != 0.0
+    for C in range(nc):
+        rows = np.flatnonzero(nzr[:, C])
+        if not len(rows):
+            continue
+        g = np.arange(max(rows.min() - GRID_RADIUS, 0), min(rows.max() + GRID_RADIUS, nf - 1) + 1)
+        cols = np.flatnonzero(nzr[g].any(axis=0))
+        if len(cols) and np.abs(cols - C).max() > GRID_RADIUS:
+            return None
+    return f_lo, rw
 
 
 def band_spans_fit(c_lo, pw, f_lo, rw, level0_dir=None, tile=16, halo=3):
@@ -453,7 +559,7 @@ class MGOperator:
         if self._bands is not None:
             nst = (2 * GRID_RADIUS + 1) ** 2
             self.grid_ops = [torch.empty(nst * hier.levels[l].n, **f32) for l in range(self.tail_level)]
-            self.grid_op64 = torch.empty(nst * n0, **f64)  # the flexible-CG operator
+            self.grid_op64 = torch.zeros(nst * n0 + GAL_TAIL, **f64)  # the flexible-CG operator
             # level 0's stencil divided by its diagonal, written with the layouts
             self.grid_feats = torch.empty(nst * n0, **f32)
             g = _lib.IgaMgGrid()
@@ -475,7 +581,9 @@ class MGOperator:
         if tb is not None:
             nst = (2 * GRID_RADIUS + 1) ** 2
             lv = hier.levels
-            self.grid_s64 = [self.grid_op64] + [torch.empty(nst * l.n, **f64) for l in lv[1:]]
+            # zeros past each FP64 stencil: the multi-level passes read up to
+            # GAL_TAIL values beyond the last grid point
+            self.grid_s64 = [self.grid_op64] + [torch.zeros(nst * l.n + GAL_TAIL, **f64) for l in lv[1:]]
             self._gal_work = torch.empty(nst * max(lv[l].h * lv[l + 1].w for l in range(len(lv) - 1)), **f64)
             sg = _lib.IgaMgSetupGrid()
             for l in range(len(lv)):
@@ -487,6 +595,18 @@ class MGOperator:
                     sg.rw[l][d] = per_dir[d][1].data_ptr()
             sg.feats = self.grid_feats.data_ptr()
             sg.work = self._gal_work.data_ptr()
+            # coarse levels up to three at a time from one finer level
+            mb = hier.multi_bands()
+            if mb is not None:
+                src, bands, nwork = mb
+                self._gal_multi_work = torch.zeros(nwork, **f64)
+                sg.gal_multi = 1
+                sg.gal_work = self._gal_multi_work.data_ptr()
+                for m in range(1, len(lv)):
+                    sg.gal_src[m] = src[m]
+                    for d in range(2):
+                        sg.gal_f_lo[m][d] = bands[m][d][0].data_ptr()
+                        sg.gal_rw[m][d] = bands[m][d][1].data_ptr()
             for l in range(len(lv)):
                 sg.regular[l] = 1 if lv[l].regular else 0
             # the dense tail straight from the stencils (small levels below it)
