@@ -136,6 +136,12 @@ enum iga_prec { IGA_FP64 = 0, IGA_FP32 = 1, IGA_BF16 = 2 };
  * indptr     : (n_free + 1) CSR row pointers, optional: with it (and n_free)
  *              the gather pass runs one wavefront per row over coalesced
  *              element-matrix rows; without it, one thread per nonzero
+ * ke_full    : 0: the element-matrix workspace layout is the library's own
+ *              (FP64 on planar p <= 3 patches with indptr: the upper blocks of
+ *              each symmetric element matrix only, packed by rows; otherwise
+ *              full rows); both passes must then get the same arguments.
+ *              1: full rows (n_e, n_l*vec, n_l*vec), rows a*vec + i -- for a
+ *              gather pass over element matrices the caller wrote.
  * vals, rhs and diag may each be null.                                    */
 typedef struct iga_asm_args {
   int32_t problem;
@@ -163,6 +169,7 @@ typedef struct iga_asm_args {
   int32_t batch;
   int32_t n_free;
   const int32_t* indptr;
+  int32_t ke_full;
 } iga_asm_args_t;
 
 /* bytes of element-matrix workspace iga_assemble needs for vals, or -1 */

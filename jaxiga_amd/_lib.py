@@ -52,7 +52,7 @@ class IgaAsmArgs(ctypes.Structure):
         ("full_to_free", c_void_p), ("freed", c_void_p), ("gfull", c_void_p), ("nnz", c_i64),
         ("rows", c_void_p), ("cols", c_void_p), ("vals", c_void_p), ("ke_work", c_void_p),
         ("ke_work_bytes", c_i64), ("rhs", c_void_p), ("diag", c_void_p), ("passes", c_int),
-        ("batch", c_int), ("n_free", c_int), ("indptr", c_void_p),
+        ("batch", c_int), ("n_free", c_int), ("indptr", c_void_p), ("ke_full", c_int),
     ]
 
 

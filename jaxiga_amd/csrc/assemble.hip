@@ -512,6 +512,18 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Packed symmetric element matrix: the blocks (i, j), i <= j, of the NL x NL
+// block matrix (VEC x VEC values each, row-major), stored by block rows --
+// row i holds blocks i .. NL-1 -- so block (i, j) is the
+// i NL - i (i - 1) / 2 + (j - i)-th. Block (j, i) of the matrix is the
+// transpose of the stored (i, j): the element matrices are symmetric, so
+// the element pass writes about half of them and the gather reads half.
+__host__ __device__ constexpr int sym_ke_values(int nl, int vec) { return nl * (nl + 1) / 2 * vec * vec; }
+
+__device__ __forceinline__ int sym_block(int i, int j, int nl) {
+  return i * nl - (i * (i - 1)) / 2 + (j - i);
+}
+
 // Element pass, one wavefront per element, for planar patches with at most
 // four functions and four Gauss points per direction. Lane l owns the local
 // function i = l % 16 (i0 = i % 4, i1 = i / 4, padded to 4 x 4) and, in step
@@ -524,7 +536,7 @@ __global__ void __launch_bounds__(256)
 // Eight waves per SIMD: the kernel fits 64 registers (the accumulators in
 // architectural VGPRs) without spilling; left to itself the compiler takes
 // 88 (64 + 24 accumulation registers), which caps residency at five waves.
-template <int P, int PROB, typename T, bool REG>
+template <int P, int PROB, typename T, bool REG, bool SYM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
     k_element_w(iga_patch_t pt, int nq1, const double* __restrict__ qp,
                 const double* __restrict__ qw, const AsmDev a, T* __restrict__ ke_all, int n_e,
@@ -556,7 +568,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
   const double* srcp = a.src_points ? a.src_points + bi * n_e * nq * VEC : nullptr;
   double* rhsp = a.rhs ? a.rhs + bi * a.n_free : nullptr;
   double* diagp = a.diag ? a.diag + bi * a.n_free : nullptr;
-  T* kout = ke_all ? ke_all + (bi * n_e + e) * (int64_t)(NLV * NLV) : nullptr;
+  constexpr int KE = SYM ? sym_ke_values(NL, VEC) : NLV * NLV;  // stored values per element
+  T* kout = ke_all ? ke_all + (bi * n_e + e) * (int64_t)KE : nullptr;
   const int ei0 = e % pt.n_elem[0], ei1 = e / pt.n_elem[0];
 
   // ---- every global read of the element, issued together ------------------
@@ -803,14 +816,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
       v[0][0] = acc[0][r];
     }
     if (kout) {
+      if constexpr (SYM) {
+        // upper blocks only, packed by rows
+        if (ic <= jc) {
+          T* dst = kout + (size_t)sym_block(ic, jc, NL) * (VEC * VEC);
+          if constexpr (VEC == 2) {
+            using T2 = T __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<T2*>(dst) = T2{v[0][0], v[0][1]};
+            *reinterpret_cast<T2*>(dst + 2) = T2{v[1][0], v[1][1]};
+          } else {
+            *dst = v[0][0];
+          }
+        }
+      } else {
 #pragma unroll
-      for (int aa = 0; aa < VEC; ++aa) {
-        T* dst = kout + (size_t)(ic * VEC + aa) * NLV + jc * VEC;
-        if constexpr (VEC == 2) {
-          using T2 = T __attribute__((ext_vector_type(2)));
-          *reinterpret_cast<T2*>(dst) = T2{v[aa][0], v[aa][1]};
-        } else {
-          *dst = v[aa][0];
+        for (int aa = 0; aa < VEC; ++aa) {
+          T* dst = kout + (size_t)(ic * VEC + aa) * NLV + jc * VEC;
+          if constexpr (VEC == 2) {
+            using T2 = T __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<T2*>(dst) = T2{v[aa][0], v[aa][1]};
+          } else {
+            *dst = v[aa][0];
+          }
         }
       }
     }
@@ -1457,7 +1484,7 @@ __global__ void __launch_bounds__(256)
 // support); on a regular patch the support tables are closed forms, so no
 // dependent read stands between the point index and the element matrices.
 // Write-out as in the row form, once per free component row.
-template <int DIM, int P, int PROB, typename T, bool REG>
+template <int DIM, int P, int PROB, typename T, bool REG, bool SYM>
 __global__ void __launch_bounds__(256)
     k_gather_points(iga_patch_t pt, int32_t p_begin, int32_t p_end, const int32_t* __restrict__ indptr,
                     const int32_t* __restrict__ f2f, const T* __restrict__ ke_all,
@@ -1481,7 +1508,9 @@ __global__ void __launch_bounds__(256)
   const int pnt = p_begin + blockIdx.x * WAVES + wave;
   if (pnt >= p_end) return;
   const int64_t b = blockIdx.y;
-  const T* ke = ke_all + b * n_e * NLV * NLV;
+  constexpr int NL = S::NL;
+  constexpr int KE = SYM ? sym_ke_values(NL, VEC) : NLV * NLV;  // stored values per element
+  const T* ke = ke_all + b * n_e * KE;
   T* vals = vals_all + b * nnz;
   double* mybox = box[wave];
   for (int t = lane; t < GROUPS * VEC * BOXV; t += kWave) mybox[t] = 0.0;
@@ -1561,15 +1590,30 @@ __global__ void __launch_bounds__(256)
       ostride *= BW;
     }
     sofs[sb] = ok ? o * VEC : -1;
-    // rows li*VEC .. li*VEC + VEC-1 of the element matrix are contiguous.
     // The reads are unconditional (an absent element reads the first
     // supporting one, skipped when adding): a conditional read would make
     // the compiler wait for each one before issuing the next.
-    const T* krow = ke + (size_t)e * NLV * NLV + (size_t)li * VEC * NLV;
+    if constexpr (SYM) {
+      // block (li, f) of the row: stored as (li, f) when li <= f, else as
+      // the transpose of (f, li)
+      const T* kel = ke + (size_t)e * KE;
 #pragma unroll
-    for (int u = 0; u < CPL; ++u) {
-      const int v = gl + u * NLVP;
-      vbuf[sb][u] = krow[v < VALS ? v : VALS - 1];
+      for (int u = 0; u < CPL; ++u) {
+        const int v = min(gl + u * NLVP, VALS - 1);
+        const int rc = v / NLV, c = v - rc * NLV;
+        const int f = c / VEC, cj = c - f * VEC;
+        const bool up = li <= f;
+        const int blk = up ? sym_block(li, f, NL) : sym_block(f, li, NL);
+        vbuf[sb][u] = kel[blk * (VEC * VEC) + (up ? rc * VEC + cj : cj * VEC + rc)];
+      }
+    } else {
+      // rows li*VEC .. li*VEC + VEC-1 of the element matrix are contiguous
+      const T* krow = ke + (size_t)e * NLV * NLV + (size_t)li * VEC * NLV;
+#pragma unroll
+      for (int u = 0; u < CPL; ++u) {
+        const int v = gl + u * NLVP;
+        vbuf[sb][u] = krow[v < VALS ? v : VALS - 1];
+      }
     }
   };
   auto accumulate = [&](int sb) {
@@ -1606,6 +1650,15 @@ __global__ void __launch_bounds__(256)
       colf[ch] = ok ? fv : -1;
     }
   }
+  // the rows' CSR extents, in flight with the element-matrix reads
+  // (unconditional: a prescribed row reads row 0's, unused)
+  int k0r[VEC], k1r[VEC];
+#pragma unroll
+  for (int rc = 0; rc < VEC; ++rc) {
+    const int row = rowc[rc] >= 0 ? rowc[rc] : 0;
+    k0r[rc] = indptr[row];
+    k1r[rc] = indptr[row + 1];
+  }
   uint64_t mask[CHUNKS];
 #pragma unroll
   for (int ch = 0; ch < CHUNKS; ++ch) mask[ch] = __ballot(colf[ch] >= 0);
@@ -1627,7 +1680,7 @@ __global__ void __launch_bounds__(256)
   for (int rc = 0; rc < VEC; ++rc) {
     const int row = rowc[rc];
     if (row < 0) continue;
-    const int k0 = indptr[row], k1 = indptr[row + 1];
+    const int k0 = k0r[rc], k1 = k1r[rc];
     int base = 0;
 #pragma unroll
     for (int ch = 0; ch < CHUNKS; ++ch) {
@@ -1686,30 +1739,45 @@ static int launch_assemble(const iga_patch_t& pt, const iga_rule_t& rule, const 
   for (int d = 0; d < DIM; ++d) n_pts *= pt.n_cp[d];
   // planar, at most 4 x 4 functions and Gauss points: one wave per element
   const bool planar_w = DIM == 2 && P <= 3 && nq1 <= 4 && a.vec == S::VEC;
+  const bool point_gather = row_gather && args.vec == S::VEC;
+  // in FP64 the planar element pass and the point gather share the packed
+  // symmetric layout of the element matrices (both passes see the same
+  // arguments): half the bytes written and read. In FP32 the gather's reads
+  // of the transposed blocks, 16 B each from as many rows, cost more than
+  // the halved bytes save (measured), so the rows stay whole.
+  const bool sym = std::is_same_v<T, double> && planar_w && point_gather && !args.ke_full;
   auto element_w = [&](int e_begin, int e_end, hipStream_t s) {
     if constexpr (DIM == 2 && P <= 3) {
       const dim3 grid((unsigned)((e_end - e_begin + 3) / 4), batch);
-      if (pt.regular)
-        hipLaunchKernelGGL((k_element_w<P, PROB, T, true>), grid, dim3(256), 0, s, pt, nq1,
-                           rule.pts, rule.wts, ae, ke, n_e, e_begin, e_end);
-      else
-        hipLaunchKernelGGL((k_element_w<P, PROB, T, false>), grid, dim3(256), 0, s, pt, nq1,
-                           rule.pts, rule.wts, ae, ke, n_e, e_begin, e_end);
+#define IGA_EW(REG, SYM)                                                                          \
+  hipLaunchKernelGGL((k_element_w<P, PROB, T, REG, SYM>), grid, dim3(256), 0, s, pt, nq1, rule.pts, \
+                     rule.wts, ae, ke, n_e, e_begin, e_end)
+      if (pt.regular) {
+        if (sym) IGA_EW(true, true);
+        else IGA_EW(true, false);
+      } else {
+        if (sym) IGA_EW(false, true);
+        else IGA_EW(false, false);
+      }
+#undef IGA_EW
     }
   };
-  const bool point_gather = row_gather && args.vec == S::VEC;
   auto gather_points = [&](int p_begin, int p_end, hipStream_t s) {
     const dim3 grid((unsigned)((p_end - p_begin + 3) / 4), batch);
-    if (pt.regular)
-      hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, true>), grid, dim3(256), 0, s, pt,
-                         p_begin, p_end, args.indptr, args.full_to_free, ke,
-                         reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
-                         (int64_t)n_e);
-    else
-      hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, false>), grid, dim3(256), 0, s, pt,
-                         p_begin, p_end, args.indptr, args.full_to_free, ke,
-                         reinterpret_cast<T*>(args.vals), args.diag, args.n_free, args.nnz,
-                         (int64_t)n_e);
+#define IGA_GP(REG, SYM)                                                                           \
+  hipLaunchKernelGGL((k_gather_points<DIM, P, PROB, T, REG, SYM>), grid, dim3(256), 0, s, pt, p_begin, \
+                     p_end, args.indptr, args.full_to_free, ke, reinterpret_cast<T*>(args.vals),     \
+                     args.diag, args.n_free, args.nnz, (int64_t)n_e)
+    if constexpr (DIM == 2 && P <= 3) {
+      if (sym) {
+        if (pt.regular) IGA_GP(true, true);
+        else IGA_GP(false, true);
+        return;
+      }
+    }
+    if (pt.regular) IGA_GP(true, false);
+    else IGA_GP(false, false);
+#undef IGA_GP
   };
   if (passes & 1) {
     bool done = false;

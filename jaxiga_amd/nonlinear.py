@@ -78,6 +78,7 @@ class NeoHookeanOperator:
         args.indptr = pat.indptr_d.data_ptr()
         args.ke_work, args.ke_work_bytes = self.ke.data_ptr(), self.ke.numel()
         args.passes = 2
+        args.ke_full = 1  # k_neohooke writes the tangents as full rows
         self.args = args
         self.u_full = torch.zeros(space.n_dofs, dtype=torch.float64, device=dev)
         self.presc = torch.as_tensor(ctx.dofmap.prescribed, dtype=torch.long, device=dev)
