@@ -45,14 +45,6 @@ struct AsmDev {
   float ecd32[16];  // planar E^cd_ab rounded to FP32 (FP32 combination)
 };
 
-// Orders the LDS accesses of one wavefront: lanes exchange data through LDS
-// with no workgroup barrier (LDS executes a wave's operations in order; the
-// fence keeps the compiler from moving accesses across this point).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 

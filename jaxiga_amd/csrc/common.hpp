@@ -16,6 +16,14 @@ namespace iga {
 
 constexpr int kWave = 64;
 
+// Orders the LDS accesses of one wavefront: lanes exchange data through LDS
+// with no workgroup barrier (LDS executes a wave's operations in order; the
+// fence keeps the compiler from moving accesses across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 __host__ __device__ constexpr int pow2ceil(int v) {
   int p = 1;

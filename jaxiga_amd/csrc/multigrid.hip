@@ -2009,27 +2009,24 @@ struct TailLevel {
 };
 
 struct TailArgs {
-  int t, lc;  // tail level, coarsest level
+  int t, lc;    // tail level, coarsest level
+  int gj_in_g;  // the coarse inverse is formed in k_tail_g (n_lc <= kGjWave)
   double omega;
   TailLevel lv[16];
   double* work;        // scratch (see tail_work_doubles)
   float* coarse_inv;   // FP32 inverse of the coarsest level
 };
 
-// P_l[f][c] for fine grid point f and coarse point c of the next level
-__device__ __forceinline__ double p_entry(const TailLevel& L, int wc, int f, int c) {
-  const int fr = f / L.w, fc = f - fr * L.w, cr = c / wc, cc = c - cr * wc;
-  const int a = fr - L.f_lo[1][cr], b = fc - L.f_lo[0][cc];
-  if (a < 0 || a >= kKFC || b < 0 || b >= kKFC) return 0.0;
-  return L.rw[1][cr * kKFC + a] * L.rw[0][cc * kKFC + b];
-}
-
-// A_l[i][j] from the stencil (0 off the 7 x 7 neighbourhood)
+// A_l[i][j] from the stencil (0 off the 7 x 7 neighbourhood), i, j < n. The
+// read is unconditional, from a clamped offset: a read under a condition is
+// waited for before the next one is issued.
 __device__ __forceinline__ double a_entry(const TailLevel& L, int i, int j) {
   const int ir = i / L.w, ic = i - ir * L.w, jr = j / L.w, jc = j - jr * L.w;
   const int dr = jr - ir, dc = jc - ic;
-  if (dr < -kGR || dr > kGR || dc < -kGR || dc > kGR) return 0.0;
-  return L.op[(size_t)((dr + kGR) * kGS + dc + kGR) * L.n + i];
+  const bool in = dr >= -kGR && dr <= kGR && dc >= -kGR && dc <= kGR;
+  const int o = (min(max(dr, -kGR), kGR) + kGR) * kGS + min(max(dc, -kGR), kGR) + kGR;
+  const double v = L.op[(size_t)o * L.n + i];
+  return in ? v : 0.0;
 }
 
 // Scratch of the dense tail (doubles): V ping-pong (vmax each), T (vmax), W
@@ -2074,54 +2071,41 @@ __device__ __forceinline__ void tail_g_entry(const TailArgs& a, const TailWork& 
   a.work[tw.goff[l] + e] = band_w(w1, ir - flo1) * band_w(w0, ic - flo0) - mi * s;
 }
 
-// G_l for the levels l0 + blockIdx.y (below lc)
-__global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw, int l0) {
-  const int l = l0 + blockIdx.y;
-  if (l >= a.lc) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.lv[l].n * a.lv[l + 1].n) return;
-  tail_g_entry(a, tw, l, e);
-}
-
-// one block: the coarse inverse (Gauss-Jordan in LDS, SPD without pivoting)
-// and V_l = G_l V_{l+1} G_l^T + 2M - M A M for l = lc-1 .. t+1; V_{t+1} ends
-// at the front of the work
-constexpr int kTailLds = 4800;  // doubles of LDS: the elimination, then V_{l+1}, G_l, T
-
 // Gauss-Jordan on [A | I] for n <= kGjWave in one wavefront: lane j holds
-// column j in registers; step k scales row k by the pivot (lane k's entry
-// k) and subtracts M[i][k] (lane k's entry i, read before its own update)
-// times row k from every other row i -- the operations of the elimination in
-// LDS below, in the same order, without a barrier per step. Writes the
-// inverse (FP64 into inv64, FP32 into inv32), row-major n x n.
+// column j in registers; in step k lane k puts its column (the multipliers
+// M[i][k]) into LDS, every lane reads it back as broadcasts, scales its
+// entry k by the pivot and subtracts M[i][k] times it from every other
+// entry i -- the operations of the elimination in LDS below, in the same
+// order, without a workgroup barrier per step. Rows i >= n are zero and
+// stay zero. Writes the inverse (FP64 into inv64, FP32 into inv32),
+// row-major n x n; s_col: kGjWave doubles of LDS.
 constexpr int kGjWave = kWave / 2;
 
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
 __device__ __forceinline__ void gj_wave(const TailLevel& C, int n, double* __restrict__ inv64,
-                                        float* __restrict__ inv32) {
+                                        float* __restrict__ inv32, double* __restrict__ s_col) {
   const int j = threadIdx.x % kWave;
   double col[kGjWave];
 #pragma unroll
-  for (int i = 0; i < kGjWave; ++i)
-    col[i] = (i < n && j < 2 * n) ? (j < n ? a_entry(C, i, j) : (j == n + i ? 1.0 : 0.0)) : 0.0;
+  for (int i = 0; i < kGjWave; ++i) {
+    const double av = a_entry(C, min(i, n - 1), min(j, n - 1));
+    col[i] = (i < n && j < 2 * n) ? (j < n ? av : (j == n + i ? 1.0 : 0.0)) : 0.0;
+  }
 #pragma unroll
   for (int k = 0; k < kGjWave; ++k) {
     if (k < n) {
-      const double piv = readlane_f64(col[k], k);
-      col[k] /= piv;
+      if (j == k) {
 #pragma unroll
-      for (int i = 0; i < kGjWave; ++i) {
-        if (i != k && i < n) {
-          const double f = readlane_f64(col[i], k);
-          col[i] -= f * col[k];
-        }
+        for (int i = 0; i < kGjWave; ++i) s_col[i] = col[i];
       }
+      wave_lds_sync();
+      double m[kGjWave];
+#pragma unroll
+      for (int i = 0; i < kGjWave; ++i) m[i] = s_col[i];
+      col[k] /= m[k];
+#pragma unroll
+      for (int i = 0; i < kGjWave; ++i)
+        if (i != k) col[i] -= m[i] * col[k];
+      wave_lds_sync();
     }
   }
   if (j >= n && j < 2 * n) {
@@ -2135,6 +2119,68 @@ __device__ __forceinline__ void gj_wave(const TailLevel& C, int n, double* __res
   }
 }
 
+// G_l for the levels l0 + blockIdx.y (below lc); with a.gj_in_g, the slice
+// blockIdx.y = lc - l0 is the coarse inverse (V_lc at the front of the work
+// and the FP32 copy), formed by one wavefront beside the products
+__global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw, int l0) {
+  __shared__ double s_st[kGN * kGjWave];
+  __shared__ double s_col[kGjWave];
+  const int l = l0 + blockIdx.y;
+  if (l == a.lc && a.gj_in_g) {
+    if (blockIdx.x != 0) return;
+    const TailLevel& C = a.lv[a.lc];
+    for (int e = threadIdx.x; e < kGN * C.n; e += blockDim.x) s_st[e] = C.op[e];
+    __syncthreads();
+    TailLevel Cs = C;
+    Cs.op = s_st;
+    if (threadIdx.x < kWave) gj_wave(Cs, C.n, a.work, a.coarse_inv, s_col);
+    return;
+  }
+  if (l >= a.lc) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.lv[l].n * a.lv[l + 1].n) return;
+  tail_g_entry(a, tw, l, e);
+}
+
+// one block: the coarse inverse (Gauss-Jordan, SPD without pivoting: in one
+// wavefront up to 32 rows, in LDS above) and V_l = G_l V_{l+1} G_l^T + 2M -
+// M A M for l = lc-1 .. t+1 (on the matrix cores when the operands fit the
+// LDS); V_{t+1} ends at the front of the work
+constexpr int kTailLds = 4800;  // doubles of LDS: the elimination, then V_{l+1}, G_l, T
+
+// C(i, j) = sum_k A(i, k) B(k, j) for i < m, j < n, k < kk inside one
+// workgroup on the matrix cores: each wave takes 16 x 16 tiles in turn; in
+// step k0 lane l holds A(i0 + l % 16, k0 + l / 16) and B(k0 + l / 16,
+// j0 + l % 16), and its results are the rows i0 + l / 16 + 4 r (r < 4) of
+// column j0 + l % 16 (v_mfma_f64_16x16x4f64). Operands outside the ranges
+// are zero; every result in range is handed to out(i, j, value).
+using d4 = double __attribute__((ext_vector_type(4)));
+
+template <class FA, class FB, class FO>
+__device__ __forceinline__ void tile_dgemm(int m, int n, int kk, FA A, FB B, FO out) {
+  const int nw = blockDim.x / kWave, l = threadIdx.x % kWave;
+  const int tm = (m + 15) / 16, tn = (n + 15) / 16;
+  for (int t = threadIdx.x / kWave; t < tm * tn; t += nw) {
+    const int i0 = (t / tn) * 16, j0 = (t % tn) * 16;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < kk; k0 += 4) {
+      const int k = k0 + l / 16, i = i0 + l % 16, j = j0 + l % 16;
+      const double av = i < m && k < kk ? A(i, k) : 0.0;
+      const double bv = j < n && k < kk ? B(k, j) : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + l / 16 + 4 * r, j = j0 + l % 16;
+      if (i < m && j < n) out(i, j, acc[r]);
+    }
+  }
+}
+
+// tiles per wave whose operator entries are read at once in the V_l
+// product (16 waves: two cover V_l up to 80 x 80, more take further rounds)
+constexpr int kTailTilesPerWave = 2;
+
 __device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWork& tw) {
   __shared__ double M[kTailLds];
   __shared__ double fac[kGjLds];
@@ -2143,10 +2189,8 @@ __device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWor
   double* Va = a.work;
   double* Vb = a.work + tw.vmax;
   double* T = a.work + 2 * tw.vmax;
-  if (n <= kGjWave) {
-    // one wavefront: lane j holds column j of [A | I], no barrier per step
-    if (threadIdx.x < kWave) gj_wave(C, n, Va, a.coarse_inv);
-    __syncthreads();
+  if (a.gj_in_g) {
+    // formed by k_tail_g
   } else {
     for (int e = threadIdx.x; e < n * n2; e += blockDim.x) {
       const int i = e / n2, j = e - i * n2;
@@ -2187,33 +2231,46 @@ __device__ __forceinline__ void tail_small_body(const TailArgs& a, const TailWor
       for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) sv[e] = Vc[e];
       for (int e = threadIdx.x; e < nl * nc; e += blockDim.x) sg[e] = g[e];
       __syncthreads();
-      for (int e = threadIdx.x; e < nl * nc; e += blockDim.x) {
-        const int i = e / nc, k = e - i * nc;
-        double s = 0.0;
-        for (int m = 0; m < nc; ++m) s += sg[i * nc + m] * sv[m * nc + k];
-        st2[e] = s;
-      }
+      // T = G V, then V_l = T G^T + 2M - MAM, both on the matrix cores
+      // (16 x 16 tiles, v_mfma_f64_16x16x4f64, operands from LDS)
+      tile_dgemm(nl, nc, nc, [&](int i, int m) { return sg[i * nc + m]; },
+                 [&](int m, int k) { return sv[m * nc + k]; },
+                 [&](int i, int k, double v) { st2[i * nc + k] = v; });
       __syncthreads();
-      // the global reads of a thread's entries (operator entry, diagonal
-      // scalings) are issued together before the products
-      constexpr int kPer = 8;
-      for (int e0 = threadIdx.x; e0 < nl * nl; e0 += kPer * blockDim.x) {
-        double lin[kPer];
+      // the global reads of a lane's entries (operator entry, diagonal
+      // scalings) are issued for all its tiles before the products
+      const int tn = (nl + 15) / 16, nw = blockDim.x / kWave, wv = threadIdx.x / kWave;
+      const int ln = threadIdx.x % kWave;
+      for (int u0 = 0; wv + u0 * nw < tn * tn; u0 += kTailTilesPerWave) {
+        double lin[kTailTilesPerWave][4];
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-          const int e = min(e0 + q * (int)blockDim.x, nl * nl - 1);
-          const int i = e / nl, j = e - i * nl;
-          const double mi = a.omega * L.dinv[i], mj = a.omega * L.dinv[j];
-          lin[q] = (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * mj;
+        for (int u = 0; u < kTailTilesPerWave; ++u) {
+          const int t = min(wv + (u0 + u) * nw, tn * tn - 1);
+          const int i0 = (t / tn) * 16, j = min((t % tn) * 16 + ln % 16, nl - 1);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = min(i0 + ln / 16 + 4 * r, nl - 1);
+            const double mi = a.omega * L.dinv[i], mj = a.omega * L.dinv[j];
+            lin[u][r] = (i == j ? 2.0 * mi : 0.0) - mi * a_entry(L, i, j) * mj;
+          }
         }
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-          const int e = e0 + q * (int)blockDim.x;
-          if (e >= nl * nl) break;
-          const int i = e / nl, j = e - i * nl;
-          double s = 0.0;
-          for (int k = 0; k < nc; ++k) s += st2[i * nc + k] * sg[j * nc + k];
-          Vn[e] = s + lin[q];
+        for (int u = 0; u < kTailTilesPerWave; ++u) {
+          const int t = wv + (u0 + u) * nw;
+          if (t >= tn * tn) break;
+          const int i0 = (t / tn) * 16, j0 = (t % tn) * 16;
+          d4 acc = {0.0, 0.0, 0.0, 0.0};
+          for (int k0 = 0; k0 < nc; k0 += 4) {
+            const int k = k0 + ln / 16, i = i0 + ln % 16, j = j0 + ln % 16;
+            const double av = i < nl && k < nc ? st2[i * nc + k] : 0.0;
+            const double bv = j < nl && k < nc ? sg[j * nc + k] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = i0 + ln / 16 + 4 * r, j = j0 + ln % 16;
+            if (i < nl && j < nl) Vn[i * nl + j] = acc[r] + lin[u][r];
+          }
         }
       }
       __syncthreads();
@@ -2788,7 +2845,9 @@ static int tail_dense(const iga_mg_setup_t& s, hipStream_t st) {
   int gmax = 1;
   for (int l = t; l < lc; ++l) gmax = std::max(gmax, s.level[l].n * s.level[l + 1].n);
   const int nt = s.level[t].n, nc = s.level[t + 1].n, ld = dense_ld(nt);
-  hipLaunchKernelGGL(k_tail_g, dim3((gmax + 255) / 256, lc - t), dim3(256), 0, st, a, tw, t);
+  // a small coarse inverse is formed beside the G products
+  a.gj_in_g = s.level[lc].n <= kGjWave;
+  hipLaunchKernelGGL(k_tail_g, dim3((gmax + 255) / 256, lc - t + a.gj_in_g), dim3(256), 0, st, a, tw, t);
   hipLaunchKernelGGL(k_tail_small, dim3(1), dim3(1024), 0, st, a, tw);
   hipLaunchKernelGGL(k_tail_w, dim3((nc * nt + 255) / 256), dim3(256), 0, st, a, tw);
   hipLaunchKernelGGL(k_tail_v, dim3((ld + 255) / 256, nt), dim3(256), 0, st, a, tw, G.tail, ld);
