@@ -609,6 +609,11 @@ typedef struct iga_mg_setup_grid {
   const int32_t* gal_f_lo[16][2];
   const double* gal_rw[16][2];
   double* gal_work;
+  /* optional: device memory holding the address of the level-0 values; the
+   * level-0 stage then reads them through it (vals0 of iga_mg_load only has
+   * to be non-null), so that a captured graph of the set-up takes new
+   * values by one 8-byte write instead of a copy of the values. */
+  const double* const* vals0_slot;
 } iga_mg_setup_grid_t;
 
 typedef struct iga_mg_setup {

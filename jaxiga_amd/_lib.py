@@ -88,7 +88,8 @@ class IgaMgSetupGrid(ctypes.Structure):
                 ("op32", c_void_p * 16), ("feats", c_void_p), ("work", c_void_p),
                 ("tail_level", c_int), ("omega", c_double), ("tail", c_void_p), ("tail_work", c_void_p),
                 ("regular", c_int * 16), ("stages", c_int), ("gal_multi", c_int), ("gal_src", c_int * 16),
-                ("gal_f_lo", (c_void_p * 2) * 16), ("gal_rw", (c_void_p * 2) * 16), ("gal_work", c_void_p)]
+                ("gal_f_lo", (c_void_p * 2) * 16), ("gal_rw", (c_void_p * 2) * 16), ("gal_work", c_void_p),
+                ("vals0_slot", c_void_p)]
 
 
 class IgaMgSetup(ctypes.Structure):

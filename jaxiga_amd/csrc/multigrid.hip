@@ -1584,6 +1584,7 @@ struct GridOpsArgs {
   const float* vals[16];
   float* op[16];
   const double* vals64;
+  const double* const* vals64_slot;  // optional: device memory holding vals64
   double* op64;
   float* feats;  // level 0: K[i, i + off] / K[i, i] (FP32), with op64
   // level 0 from vals64 alone (vals[0] null): its CSR copies and scalings
@@ -1607,6 +1608,7 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
   const int i0 = blockIdx.x * kOpsRows;
   if (i0 >= n) return;
   const int i1 = min(i0 + kOpsRows, n);
+  const double* vals64 = a.vals64_slot ? *a.vals64_slot : a.vals64;
   const int32_t* cols = a.cols[l];
   const float* vals = a.vals[l];  // null: the FP32 values from vals64 (level 0)
   const int lo = a.indptr[l][i0], cnt = a.indptr[l][i1] - lo;
@@ -1623,7 +1625,7 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     for (int k = 0; k < NP; ++k) {
       const int t = min((int)threadIdx.x + k * kGTh, max(cnt - 1, 0));
       c[k] = cols[lo + t];
-      v64[k] = f64 ? a.vals64[lo + t] : 0.0;
+      v64[k] = f64 ? vals64[lo + t] : 0.0;
       v[k] = prep ? (float)v64[k] : vals[lo + t];
     }
 #pragma unroll
@@ -1641,7 +1643,7 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     }
   } else if (prep) {
     for (int t = threadIdx.x; t < cnt; t += kGTh) {
-      const double v64 = a.vals64[lo + t];
+      const double v64 = vals64[lo + t];
       if (a.csr64) a.csr64[lo + t] = v64;
       a.csr32[lo + t] = (float)v64;
     }
@@ -1652,7 +1654,7 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     const int i = i0 + threadIdx.x;
     const int q0 = lo + sp[threadIdx.x], q1 = lo + sp[threadIdx.x + 1];
     const int k = lower_bound(cols, q0, q1, i);
-    const double d = (k < q1 && cols[k] == i) ? a.vals64[k] : 0.0;
+    const double d = (k < q1 && cols[k] == i) ? vals64[k] : 0.0;
     const double di = d != 0.0 ? 1.0 / d : 1.0;
     a.dinv[i] = di;
     a.dsq[i] = (float)sqrt(fabs(di));
@@ -1679,7 +1681,7 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
       double v64 = 0.0;
       for (int q = lo + s0; q < lo + e0; ++q)
         if (cols[q] == j) {
-          v64 = f64 ? a.vals64[q] : 0.0;
+          v64 = f64 ? vals64[q] : 0.0;
           v32 = prep ? (float)v64 : vals[q];
         }
       return v64;
@@ -2916,7 +2918,10 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     a.indptr[0] = V.indptr;
     a.cols[0] = V.cols;
     a.vals[0] = nullptr;  // from vals0: the CSR copies and scalings too
-    a.csr64 = V.vals64 != vals0 ? V.vals64 : nullptr;
+    // read through the slot when given (the values' address is then known
+    // only to the device: always copy them)
+    a.vals64_slot = G->vals0_slot;
+    a.csr64 = G->vals0_slot || V.vals64 != vals0 ? V.vals64 : nullptr;
     a.csr32 = V.vals32;
     a.dinv = V.dinv;
     a.dsq = V.dsq;

@@ -554,6 +554,31 @@ __global__ void __launch_bounds__(kVecThreads)
   }
 }
 
+// Two pinned host copies of the solver state and their events, per host
+// thread and device (created at the first solve there, kept for the
+// process), so that reading the state of one batch overlaps the next.
+struct StatePipe {
+  CGState* host = nullptr;  // [2], pinned
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static int state_pipe(StatePipe** out) {
+  constexpr int kMaxDev = 64;
+  thread_local StatePipe pipes[kMaxDev];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return IGA_ERR_HIP;
+  StatePipe& P = pipes[dev];
+  if (!P.host) {
+    void* h = nullptr;
+    if (hipHostMalloc(&h, 2 * sizeof(CGState), hipHostMallocDefault) != hipSuccess) return IGA_ERR_HIP;
+    for (int i = 0; i < 2; ++i)
+      if (hipEventCreateWithFlags(&P.ev[i], hipEventDisableTiming) != hipSuccess) return IGA_ERR_HIP;
+    P.host = static_cast<CGState*>(h);
+  }
+  *out = &P;
+  return IGA_OK;
+}
+
 // `op(u, w, P)` must produce w = A u and the w.u partials into P (returns the
 // number of partial blocks, or a negative status); a set done flag must make
 // it a no-op.
@@ -577,14 +602,15 @@ static int cg1_loop(int32_t n, const double* dinv, const double* b, double* x, d
   hipLaunchKernelGGL(k_cg1_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, dinv, x, r, u, p, s, B);
   hipLaunchKernelGGL(k_cg1_init_fin, dim3(1), dim3(kVecThreads), 0, st, vb, B, tol * tol, maxiter);
   IGA_CHECK_LAUNCH();
-  CGState host{};
-  const int batch = 16;
+  // The host keeps one batch queued ahead of the state it reads: batch k+1
+  // is enqueued before the state after batch k is awaited, so the GPU never
+  // drains between batches (after convergence the batch in flight is
+  // early-exit launches). The state copies go to pinned host slots.
+  StatePipe* pipe = nullptr;
+  if (int rc = state_pipe(&pipe)) return rc;
+  const int batch = 8;
   int launched = 0;
-  for (;;) {
-    if (hipMemcpyAsync(&host, B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess)
-      return IGA_ERR_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
-    if (host.done) break;
+  auto enqueue = [&](int slot) -> int {
     for (int k = 0; k < batch; ++k, ++launched) {
       const int nb = op(u, w, B.P_wu, B.st);
       if (nb < 0) return nb;
@@ -592,6 +618,23 @@ static int cg1_loop(int32_t n, const double* dinv, const double* b, double* x, d
                          dinv, w, u, x, r, p, s, B);
       IGA_CHECK_LAUNCH();
     }
+    if (hipMemcpyAsync(&pipe->host[slot], B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipEventRecord(pipe->ev[slot], st) != hipSuccess)
+      return IGA_ERR_HIP;
+    return IGA_OK;
+  };
+  // the state after the initialisation
+  if (hipMemcpyAsync(&pipe->host[0], B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipEventRecord(pipe->ev[0], st) != hipSuccess)
+    return IGA_ERR_HIP;
+  CGState host{};
+  for (int k = 0;; ++k) {
+    // batch k+1 (slot (k+1)&1) queued behind the state of slot k&1
+    if (int rc = enqueue((k + 1) & 1)) return rc;
+    if (hipEventSynchronize(pipe->ev[k & 1]) != hipSuccess) return IGA_ERR_HIP;
+    host = pipe->host[k & 1];
+    if (host.done) break;
   }
   if (out_iters) *out_iters = host.iter;
   if (out_rr) *out_rr = host.rr;

@@ -287,9 +287,11 @@ def smoother_stencils(gen: ConvNet, op: MG.MGOperator, levels=LEARNED_LEVELS):
     return out
 
 
-def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
+def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=None):
     """Multigrid operator with the generator's smoothers on the ``levels``
-    finest levels and damped Jacobi elsewhere (everywhere when ``gen`` is None)."""
+    finest levels and damped Jacobi elsewhere (everywhere when ``gen`` is None).
+    ``vals_slot``: level 0's values are read through this one-element int64
+    device tensor holding their address (see :meth:`MGOperator.load`)."""
     import torch
 
     hier = MG.hierarchy(ctx)
@@ -303,7 +305,7 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
     # stream beside the Galerkin levels and the dense tail, which are
     # enqueued first so that the host's work on the generator overlaps them
     # (captured in a graph with the generator first, the set-up is slower)
-    op.load(vals64, stage=1)
+    op.load(vals64, stage=1, vals_slot=vals_slot)
     main = torch.cuda.current_stream()
     level0 = torch.cuda.Event()
     level0.record(main)
@@ -319,29 +321,33 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS):
 
 class CapturedSetup:
     """The per-operator set-up of :func:`setup` (Galerkin levels, dense tail,
-    generator inference on its side stream) captured once in a HIP graph over
-    a persistent copy of the level-0 values: a new operator is one copy into
-    that buffer and one graph launch, with no host work per kernel."""
+    generator inference on its side stream) captured once in a HIP graph that
+    reads the level-0 values through a device slot holding their address: a
+    new operator is one 8-byte write and one graph launch, with no copy of
+    the values and no host work per kernel."""
 
     def __init__(self, ctx, gen, levels):
         import torch
 
         self.gen_ref = weakref.ref(gen)
-        self.vin = torch.empty(ctx.assembly.nnz, dtype=torch.float64, device=ctx.f2f_d.device)
-        self.vin.zero_()
+        dev = ctx.f2f_d.device
+        vin = torch.zeros(ctx.assembly.nnz, dtype=torch.float64, device=dev)
+        self.slot = torch.full((1,), vin.data_ptr(), dtype=torch.int64, device=dev)
         # one eager run first: lazy buffers stay out of the graph
-        self.op = setup(ctx, self.vin, gen, levels)
+        self.op = setup(ctx, vin, gen, levels)
         self.stencils = list(self.op.stencils)  # the persistent buffers the graph writes
         torch.cuda.synchronize()
-        cap = side_stream(self.vin.device, "setup-capture")
+        cap = side_stream(dev, "setup-capture")
         cap.wait_stream(torch.cuda.current_stream())
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph, stream=cap):
-            setup(ctx, self.vin, gen, levels)
+            setup(ctx, vin, gen, levels, vals_slot=self.slot)
         torch.cuda.synchronize()
 
     def run(self, vals64):
-        self.vin.copy_(vals64)
+        # the graph reads vals64 through the slot: it must stay alive (and
+        # unchanged) until the set-up has run
+        self.slot.fill_(vals64.data_ptr())
         self.graph.replay()
         # the operator may have been set up since without these smoothers
         # (another generator, or none): point it at the graph's buffers again

@@ -649,7 +649,7 @@ class MGOperator:
         """Whether :meth:`load` can run in two stages (the tensor set-up)."""
         return self._tensor is not None
 
-    def load(self, vals64, stencils=None, stage=0):
+    def load(self, vals64, stencils=None, stage=0, vals_slot=None):
         """Write a new level-0 operator (FP64 CSR values) and its smoothers:
         one library call forms every Galerkin level, the FP32 copies, the
         diagonal scalings and the coarsest inverse, a second the dense matrix
@@ -658,13 +658,22 @@ class MGOperator:
         With the tensor set-up, ``stage=1`` writes level 0 only (its stencil
         layouts, the generator features, copies and scalings) and ``stage=2``
         the rest, so work that reads the features can run on another stream
-        beside stage 2; ``stage=0`` does both."""
+        beside stage 2; ``stage=0`` does both. ``vals_slot`` (tensor set-up):
+        a one-element int64 device tensor holding the address of the values,
+        which level 0 is then read through (``vals64`` is not read)."""
         if stage and not self.split_load():
             raise ValueError("a staged load needs the tensor set-up")
+        if vals_slot is not None and self._tensor is None:
+            raise ValueError("reading the values through a slot needs the tensor set-up")
         s = self._setup_struct()
         if self._tensor is not None:
             self._tensor.stages = stage
-        _lib.check(lib().iga_mg_load(s, _lib.ptr(vals64), stream()), "iga_mg_load")
+            self._tensor.vals0_slot = vals_slot.data_ptr() if vals_slot is not None else None
+        try:
+            _lib.check(lib().iga_mg_load(s, _lib.ptr(vals64), stream()), "iga_mg_load")
+        finally:
+            if self._tensor is not None:
+                self._tensor.vals0_slot = None
         if stage == 1:
             return self
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
