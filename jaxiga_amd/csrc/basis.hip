@@ -309,7 +309,10 @@ static int launch_basis(const iga_patch_t& pt, int n_e, const int32_t* elems, in
     IGA_CHECK_LAUNCH();
     return (int)IGA_OK;
   };
-  auto fits = [&](int epb) { return epb * nq <= 256 && L::total(epb, nq, basis) <= 48 * 1024; };
+  // and, for few elements (a boundary), enough blocks to spread over the GPU
+  auto fits = [&](int epb) {
+    return epb * nq <= 256 && L::total(epb, nq, basis) <= 48 * 1024 && (epb == 1 || n_e / epb >= 512);
+  };
   if (fits(32)) return go(std::integral_constant<int, 32>{});
   if (fits(16)) return go(std::integral_constant<int, 16>{});
   if (fits(8)) return go(std::integral_constant<int, 8>{});
