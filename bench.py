@@ -705,9 +705,8 @@ def assembly_run(ctx, params, prec_name, steps, warmup):
     n_e = ctx.space.n_elems
     # arguments and output buffers are prepared once; every step recomputes
     # everything (basis, element matrices, gather, load with its boundary
-    # basis). The element pass and the gather run on this stream, the
-    # Neumann work overlaps them on a side stream; the timing events are
-    # created before the timed region.
+    # basis), all on this stream; the timing events are created before the
+    # timed region.
     job = M.prepare_assembly(ctx, params, precision=precision)
 
     def step():

@@ -25,7 +25,7 @@ from jaxiga_amd import linear as LIN
 from jaxiga_amd.evaluation import evaluate
 from jaxiga_amd.forms import PointField, Problem
 from jaxiga_amd.pointset import boundary_gauss, gauss, n_points, tensor_rule
-from jaxiga_amd.runtime import device, lib, side_stream, stream
+from jaxiga_amd.runtime import device, lib, stream
 from jaxiga_amd.space import make_dofmap
 
 CG_CROSSOVER = 3000
@@ -274,7 +274,6 @@ class AssemblyLaunch:
     diag: object
     keep: list
     tractions: list | None = None  # Neumann tractions given by the caller
-    events: list | None = None  # (fork, join) events of the Neumann side stream, reused
 
     def launch(self, ctx, passes=0):
         self.args.passes = passes
@@ -297,41 +296,25 @@ def assemble(ctx: MethodContext, params=None, *, values=True, load=True, diagona
 
 
 def run_assembly(ctx, params, job, *, load=True, passes=(0,), on_pass=None):
-    """Launch a prepared assembly on the current stream. The Neumann work
-    (boundary basis, traction, load scatter) runs on a side stream alongside
-    the element pass -- both add into the load with atomics -- and the
-    current stream waits for it before returning. ``passes`` splits the
-    iga_assemble call (``(1, 2)``: element pass, then gather) and
-    ``on_pass(p)`` runs after each launch (e.g. to record a timing event)."""
-    import torch
-
-    neumann = load and job.rhs is not None and bool(ctx.problem.neumann)
-    cur = torch.cuda.current_stream()
-    if neumann:
-        # the side stream starts from the same point as the element pass; the
-        # fork and join events are created once per prepared job
-        if job.events is None:
-            job.events = [torch.cuda.Event(), torch.cuda.Event()]
-        fork, join = job.events
-        side = side_stream(job.rhs.device, "neumann")
-        fork.record(cur)
-        side.wait_event(fork)
+    """Launch a prepared assembly on the current stream: the passes, then the
+    Neumann work (boundary basis, traction, load scatter), which adds into the
+    load with atomics beside the element pass's source work. ``passes``
+    splits the iga_assemble call (``(1, 2)``: element pass, then gather) and
+    ``on_pass(p)`` runs after each launch (e.g. to record a timing event).
+    The Neumann work runs on the same stream: on a side stream beside the
+    element pass it cost more in cross-stream synchronisation and contention
+    than its two small kernels take in sequence (measured)."""
     for p in passes:
         job.launch(ctx, passes=p)
         if on_pass is not None:
             on_pass(p)
-    if neumann:
-        # issued once every pass is queued: the GPU runs the element pass and
-        # the gather back to back while the host prepares the boundary work
-        with torch.cuda.stream(side):
-            neumann_load(ctx, params, job.rhs, job.tractions)
-        join.record(side)
-        cur.wait_event(join)
+    if load and job.rhs is not None and bool(ctx.problem.neumann):
+        neumann_load(ctx, params, job.rhs, job.tractions)
 
 
 class CapturedAssembly:
-    """A full assembly (load zeroed, element pass with the Neumann work
-    forked to a side stream, gather) captured in HIP graphs over persistent
+    """A full assembly (load zeroed, element pass, gather, Neumann work)
+    captured in HIP graphs over persistent
     buffers. ``replay()`` recomputes everything -- basis, element matrices,
     CSR values, load -- with one graph launch per part and no host work;
     ``vals``, ``rhs`` and ``diag`` hold the results. With ``split`` the
