@@ -724,7 +724,8 @@ struct NoReplace {
 template <typename Apply, typename Replace = NoReplace>
 static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, double tol,
                    int32_t maxiter, double* work, int32_t* out_iters, double* out_rr,
-                   double* out_bb, hipStream_t st, Apply apply, Replace replace = Replace{}) {
+                   double* out_bb, hipStream_t st, Apply apply, Replace replace = Replace{},
+                   bool queued = false) {
   if (n == 0) {
     if (out_iters) *out_iters = 0;
     if (out_rr) *out_rr = 0.0;
@@ -740,6 +741,44 @@ static int cg_loop(int32_t n, const double* dinv, const double* b, double* x, do
   hipLaunchKernelGGL(k_cg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, dinv, x, r, z, p, B);
   hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kVecThreads), 0, st, vb, B, tol * tol, maxiter);
   IGA_CHECK_LAUNCH();
+  if (queued && replace.every <= 0) {
+    // one batch queued ahead of the state the host reads (see cg1_loop);
+    // after convergence the batch in flight is early-exit launches
+    StatePipe* pipe = nullptr;
+    if (int rc = state_pipe(&pipe)) return rc;
+    int launched = 0;
+    auto enqueue = [&](int slot) -> int {
+      for (int k = 0; k < 8; ++k, ++launched) {
+        const int par = launched & 1;
+        const int pb = apply(p, q, B);
+        if (pb < 0) return pb;
+        hipLaunchKernelGGL(k_cg_update, dim3(vb), dim3(kVecThreads), 0, st, n, pb, vb, par, dinv,
+                           p, q, x, r, z, B);
+        hipLaunchKernelGGL(k_cg_dir, dim3(vb), dim3(kVecThreads), 0, st, n, vb, par, z, p, B);
+        IGA_CHECK_LAUNCH();
+      }
+      if (hipMemcpyAsync(&pipe->host[slot], B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) !=
+              hipSuccess ||
+          hipEventRecord(pipe->ev[slot], st) != hipSuccess)
+        return IGA_ERR_HIP;
+      return IGA_OK;
+    };
+    if (hipMemcpyAsync(&pipe->host[0], B.st, sizeof(CGState), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipEventRecord(pipe->ev[0], st) != hipSuccess)
+      return IGA_ERR_HIP;
+    CGState host{};
+    for (int k = 0;; ++k) {
+      if (int rc = enqueue((k + 1) & 1)) return rc;
+      if (hipEventSynchronize(pipe->ev[k & 1]) != hipSuccess) return IGA_ERR_HIP;
+      host = pipe->host[k & 1];
+      if (host.done) break;
+    }
+    if (out_iters) *out_iters = host.iter;
+    if (out_rr) *out_rr = host.rr;
+    if (out_bb) *out_bb = host.bb;
+    return IGA_OK;
+  }
   CGState host{};
   const int batch = 16;
   int launched = 0;
@@ -801,7 +840,8 @@ static int csr_apply(int32_t n, int lpr, int rb, const int32_t* indptr, const in
 static int cg_solve_csr(int32_t n, const int32_t* indptr, const int32_t* cols, const void* vals,
                         int32_t precision, const double* dinv, const double* b, double* x,
                         double tol, int32_t maxiter, double* work, int32_t* out_iters,
-                        double* out_rr, double* out_bb, hipStream_t st, bool single = false) {
+                        double* out_rr, double* out_bb, hipStream_t st, bool single = false,
+                        bool queued = false) {
   int nnz = 0;
   if (n > 0)
     if (int rc = nnz_of(n, indptr, st, &nnz)) return rc;
@@ -817,7 +857,8 @@ static int cg_solve_csr(int32_t n, const int32_t* indptr, const int32_t* cols, c
   if (single)
     return cg1_loop(n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st, op);
   return cg_loop(n, dinv, b, x, tol, maxiter, work, out_iters, out_rr, out_bb, st,
-                 [&](const double* p, double* q, CGBufs B) { return op(p, q, B.P_pq, B.st); });
+                 [&](const double* p, double* q, CGBufs B) { return op(p, q, B.P_pq, B.st); },
+                 NoReplace{}, queued);
 }
 
 extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
@@ -827,7 +868,7 @@ extern "C" int iga_cg_solve(int32_t n, const int32_t* indptr, const int32_t* col
   if (!indptr || !cols || !vals || !dinv || !b || !x || !work) return IGA_ERR_ARG;
   if (precision != IGA_FP64 && precision != IGA_FP32) return IGA_ERR_UNSUPPORTED;
   return cg_solve_csr(n, indptr, cols, vals, precision, dinv, b, x, tol, maxiter, work, out_iters,
-                      out_rr, out_bb, (hipStream_t)stream);
+                      out_rr, out_bb, (hipStream_t)stream, false, true);
 }
 
 extern "C" int iga_cg1_solve(int32_t n, const int32_t* indptr, const int32_t* cols,
@@ -957,7 +998,8 @@ extern "C" int iga_cg_solve_mf(const iga_patch_t* patch, const void* mf_work,
                    int rc = mf_apply_launch(patch, mf_work, mf_work_bytes, n_free, freed,
                                             full_to_free, p, q, &B.st->done, B.P_pq, &nb, st);
                    return rc ? rc : nb;
-                 });
+                 },
+                 NoReplace{}, true);
 }
 
 extern "C" int iga_residual_norms(int32_t n, const int32_t* indptr, const int32_t* cols,
