@@ -694,10 +694,14 @@ int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* i
  * (independent of the recursion), read back with the solver state;
  * x_out (optional): n doubles receiving x (the plan's x is overwritten by
  * its next run): written by the last kernel of every replay on the grid
- * cycle, else copied on the stream behind the iterations. A run may use any
+ * cycle, else copied on the stream behind the iterations;
+ * b_in (optional): n doubles of the right-hand side, read by the run's first
+ * kernel and kept in the plan's b (which the plan then writes: b must not
+ * be read-only memory); null: the plan's b as it stands. A run may use any
  * stream, also one other than the plan's capture stream. */
 int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter, int32_t* out_iters,
-                   double* out_rr, double* out_bb, double* out_res, double* x_out, void* stream);
+                   double* out_rr, double* out_bb, double* out_res, double* x_out,
+                   const double* b_in, void* stream);
 void iga_fcg_mg_destroy(iga_fcg_plan_t* plan);
 int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr, const int32_t* cols,
                const double* vals, const double* b, double* x, double tol, int32_t maxiter,

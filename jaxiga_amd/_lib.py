@@ -214,7 +214,7 @@ _SIGNATURES = {
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                 ctypes.POINTER(c_void_p)]),
     "iga_fcg_mg_run": (c_int, [c_void_p, c_double, c_int, P_int, P_dbl, P_dbl, P_dbl, c_void_p,
-                               c_void_p]),
+                               c_void_p, c_void_p]),
     "iga_fcg_mg_destroy": (None, [c_void_p]),
     "iga_fcg_mg": (c_int, [ctypes.POINTER(IgaMg), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_double, c_int, c_int, c_void_p, c_void_p, P_int, P_dbl, P_dbl,

@@ -371,18 +371,26 @@ struct Gate {
   int clear_pending;  // block 0 also clears st->pending (the update was applied)
 };
 
-__device__ __forceinline__ bool gate_closed(const Gate& g) {
-  if (g.rr) {
-    const double rr = reduce_partials(g.rr, g.n_rr);
-    const bool done = !(rr > g.st->tol2 * g.st->bb);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
-      g.st->rr = rr;
-      g.st->done = done ? 1 : 0;
-      if (!done) g.st->iter += 1;
-      if (g.clear_pending) g.st->pending = 0;
-    }
-    return done;
+// A negative st->bb (set by an init that skips its own reduction) means the
+// partials the first gate reduces are those of r = b: that gate records them
+// as ||b||^2. Blocks that read st->bb while block 0 writes it see either the
+// sentinel or the same value, so every block makes the same test.
+__device__ __forceinline__ bool gate_decide(const Gate& g, double rr, double tol2, double bb) {
+  const bool first = bb < 0.0;
+  bb = first ? rr : bb;
+  const bool done = !(rr > tol2 * bb);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+    if (first) g.st->bb = bb;
+    g.st->rr = rr;
+    g.st->done = done ? 1 : 0;
+    if (!done) g.st->iter += 1;
+    if (g.clear_pending) g.st->pending = 0;
   }
+  return done;
+}
+
+__device__ __forceinline__ bool gate_closed(const Gate& g) {
+  if (g.rr) return gate_decide(g, reduce_partials(g.rr, g.n_rr), g.st->tol2, g.st->bb);
   return g.flag && *g.flag;
 }
 
@@ -407,17 +415,7 @@ __device__ __forceinline__ GateVals gate_load(const Gate& g) {
 }
 
 __device__ __forceinline__ bool gate_test(const Gate& g, const GateVals& v) {
-  if (g.rr) {
-    const double rr = partials_total(v.share);
-    const bool done = !(rr > v.tol2 * v.bb);
-    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
-      g.st->rr = rr;
-      g.st->done = done ? 1 : 0;
-      if (!done) g.st->iter += 1;
-      if (g.clear_pending) g.st->pending = 0;
-    }
-    return done;
-  }
+  if (g.rr) return gate_decide(g, partials_total(v.share), v.tol2, v.bb);
   return v.flag != 0;
 }
 
@@ -840,6 +838,8 @@ __global__ void __launch_bounds__(256)
 //   k_g_restrict    level l+1: b = P^T r_l, x = w D^-1 b, r = b - A x
 //   k_g_tail        level t: x_t = M P^T r_{t-1} (dense tail)
 //   k_g_prolong     level l: x += P x_{l+1}, x += w D^-1 (b - A x)
+//   k_g_prolong_p   level t-2: the same from level t-1, whose values it forms
+//                   from the tail first (the prolongation into t-1 folded in)
 //   k_g0_prolong    level 0: x = x0 + P x1, u = D^-1/2 (b0 - A0 x)
 //   k_g0_smooth_t   level 0: z = x + D^-1/2 T_W^T u (+ z . q)
 // Values off the grid are zero. A level's radius-3 operator reaches a 22 x 22
@@ -1100,71 +1100,249 @@ __global__ void __launch_bounds__(kGTh)
   if (l == 0) xo[i] = s;
 }
 
-// fine tile of level l >= 1: x = x_pre + P x_c on the halo, then one damped
-// Jacobi step x + w D^-1 (b - A x) on the tile. As in k_g_restrict, every
-// independent load goes out in the first group; the coarse values follow.
+// The first group of loads of a fine tile of level l >= 1 (the halo's
+// x_pre and band rows, the tile point's taps, scaling and b) and, once the
+// coarse values are staged in cs, the rest: x = x_pre + P x_c on the halo,
+// then one damped Jacobi step x + w D^-1 (b - A x) on the tile.
+struct ProlongTile {
+  static constexpr int NH = passes(kGH1 * kGH1);
+  float xp[NH], wr[NH][kKPF], wcl[NH][kKPF];
+  int ka[NH], kb[NH];
+  bool hin[NH];
+  float a[kGN];
+  float d, bi;
+  int i;
+  bool in;
+
+  __device__ __forceinline__ void load(const GLevel& g, const GBand& br, const GBand& bc,
+                                       const float* __restrict__ xpre,
+                                       const float* __restrict__ bf, int r0, int c0) {
+    const int n = g.h * g.w;
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+      const int rr = r0 - kGR + ly, cc = c0 - kGR + lx;
+      hin[k] = t < kGH1 * kGH1 && on_grid(rr, cc, g.h, g.w);
+      const int rc = hin[k] ? rr : 0, ccc = hin[k] ? cc : 0;
+      xp[k] = xpre[rc * g.w + ccc];
+      ka[k] = br.c_lo[rc];
+      kb[k] = bc.c_lo[ccc];
+#pragma unroll
+      for (int q = 0; q < kKPF; ++q) {
+        wr[k][q] = br.pw[rc * kKPF + q];
+        wcl[k][q] = bc.pw[ccc * kKPF + q];
+      }
+    }
+    const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+    const int rr = r0 + ty, cc = c0 + tx;
+    in = on_grid(rr, cc, g.h, g.w);
+    i = in ? rr * g.w + cc : 0;
+#pragma unroll
+    for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + i];
+    d = g.dsq[i];
+    bi = bf[i];
+  }
+
+  // cs: the coarse values from (cr0, cc0), staged and synchronised
+  __device__ __forceinline__ void finish(const GLevel& g, const float (*cs)[kCoarseSpan],
+                                         float (*xs)[kGH1], int cr0, int cc0,
+                                         float* __restrict__ xo) const {
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+      const int t = threadIdx.x + k * kGTh;
+      const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
+      const int ia0 = ka[k] - cr0, ib0 = kb[k] - cc0;
+      float v = 0.f;
+#pragma unroll
+      for (int p = 0; p < kKPF; ++p) {
+        const int ia = min(max(ia0 + p, 0), kCoarseSpan - 1);
+        float s2 = 0.f;
+#pragma unroll
+        for (int q = 0; q < kKPF; ++q)
+          s2 += wcl[k][q] * cs[ia][min(max(ib0 + q, 0), kCoarseSpan - 1)];
+        v += wr[k][p] * s2;
+      }
+      if (t < kGH1 * kGH1) xs[ly][lx] = hin[k] ? xp[k] + v : 0.f;
+    }
+    __syncthreads();
+    const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
+    const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
+    if (!in) return;
+    xo[i] = xs[ty + kGR][tx + kGR] + g.omega * d * d * (bi - ax);
+  }
+};
+
+// fine tile of level l >= 1 from the coarse values of level l + 1. As in
+// k_g_restrict, every independent load goes out in the first group; the
+// coarse values follow.
 __global__ void __launch_bounds__(kGTh)
     k_g_prolong(GLevel g, int hc, int wc, GBand br, GBand bc, const float* __restrict__ xc,
                 const float* __restrict__ xpre, const float* __restrict__ bf,
                 float* __restrict__ xo, const int* __restrict__ skip) {
   __shared__ float cs[kCoarseSpan][kCoarseSpan];
   __shared__ float xs[kGH1][kGH1];
-  constexpr int NH = passes(kGH1 * kGH1);
-  const int n = g.h * g.w;
   const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
   const int skipv = skip ? *skip : 0;
   const int cr0 = br.c_lo[max(r0 - kGR, 0)], cc0 = bc.c_lo[max(c0 - kGR, 0)];
-  float xp[NH], wr[NH][kKPF], wcl[NH][kKPF];
-  int ka[NH], kb[NH];
-  bool hin[NH];
-#pragma unroll
-  for (int k = 0; k < NH; ++k) {
-    const int t = threadIdx.x + k * kGTh;
-    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
-    const int rr = r0 - kGR + ly, cc = c0 - kGR + lx;
-    hin[k] = t < kGH1 * kGH1 && on_grid(rr, cc, g.h, g.w);
-    const int rc = hin[k] ? rr : 0, ccc = hin[k] ? cc : 0;
-    xp[k] = xpre[rc * g.w + ccc];
-    ka[k] = br.c_lo[rc];
-    kb[k] = bc.c_lo[ccc];
-#pragma unroll
-    for (int a = 0; a < kKPF; ++a) {
-      wr[k][a] = br.pw[rc * kKPF + a];
-      wcl[k][a] = bc.pw[ccc * kKPF + a];
-    }
-  }
-  const int ty = threadIdx.x / kGT, tx = threadIdx.x % kGT;
-  const int rr = r0 + ty, cc = c0 + tx;
-  const bool in = on_grid(rr, cc, g.h, g.w);
-  const int i = in ? rr * g.w + cc : 0;
-  float a[kGN];
-#pragma unroll
-  for (int o = 0; o < kGN; ++o) a[o] = g.A[(size_t)o * n + i];
-  const float d = g.dsq[i], bi = bf[i];
+  ProlongTile T;
+  T.load(g, br, bc, xpre, bf, r0, c0);
   __builtin_amdgcn_sched_barrier(0);
   if (skipv) return;
   stage_coarse(cs, xc, hc, wc, cr0, cc0);
   __syncthreads();
+  T.finish(g, cs, xs, cr0, cc0, xo);
+}
+
+// The prolongation from the tail into level p = t - 1 and its damped Jacobi
+// post-smoothing, folded into the prolongation into level g = p - 1: a tile
+// block of level g forms the level-p values its coarse window needs,
+//   y = x_p + P_p x_t                    (the window and a border of kGR),
+//   x_p' = y + w D_p^-1 (b_p - A_p y)    (the window),
+// in LDS, then finishes as k_g_prolong with x_p' as the coarse values. The
+// window is the coarse rows and columns the tile's halo reaches (at most
+// kCoarseSpan, as for k_g_prolong's staging).
+constexpr int kPBor = kCoarseSpan + 2 * kGR;  // the window with its border
+constexpr int kPTail = 512;                   // tail values staged
+
+struct ProlongP {
+  GLevel g, p;     // levels g and p
+  int ht, wt;      // level t
+  GBand brg, bcg;  // g -> p (rows, columns)
+  GBand brp, bcp;  // p -> t
+};
+
+__global__ void __launch_bounds__(kGTh)
+    k_g_prolong_p(ProlongP a, const float* __restrict__ xt, const float* __restrict__ xp_pre,
+                  const float* __restrict__ bp, const float* __restrict__ xpre,
+                  const float* __restrict__ bf, float* __restrict__ xo,
+                  const int* __restrict__ skip) {
+  __shared__ float cs[kCoarseSpan][kCoarseSpan];  // x_p' (zero off the window and the grid)
+  __shared__ float ys[kPBor][kPBor];              // y
+  __shared__ float xs[kGH1][kGH1];
+  __shared__ float ts[kPTail];                    // x_t
+  __shared__ int ylo[2][kPBor];                   // tail band rows of the bordered window
+  __shared__ float ypw[2][kPBor][kKPF];
+  constexpr int NT = kPTail / kGTh;
+  constexpr int NB = passes(kPBor * kPBor);
+  const GLevel& P = a.p;
+  const int np = P.h * P.w, nt = a.ht * a.wt;
+  const int r0 = blockIdx.y * kGT, c0 = blockIdx.x * kGT;
+  const int skipv = skip ? *skip : 0;
+  // the window: coarse rows cr0 .. cr0 + wh - 1 and columns cc0 .. cc0 + ww - 1
+  const int cr0 = a.brg.c_lo[max(r0 - kGR, 0)], cc0 = a.bcg.c_lo[max(c0 - kGR, 0)];
+  const int cr1 = a.brg.c_lo[min(r0 + kGT + kGR - 1, a.g.h - 1)] + kKPF - 1;
+  const int cc1 = a.bcg.c_lo[min(c0 + kGT + kGR - 1, a.g.w - 1)] + kKPF - 1;
+  ProlongTile T;
+  T.load(a.g, a.brg, a.bcg, xpre, bf, r0, c0);
+  float tv[NT];
 #pragma unroll
-  for (int k = 0; k < NH; ++k) {
+  for (int k = 0; k < NT; ++k) tv[k] = xt[min((int)threadIdx.x + k * kGTh, nt - 1)];
+  __builtin_amdgcn_sched_barrier(0);
+  if (skipv) return;
+  const int wh = min(min(cr1, P.h - 1) - cr0 + 1, kCoarseSpan);
+  const int ww = min(min(cc1, P.w - 1) - cc0 + 1, kCoarseSpan);
+  const int bh = wh + 2 * kGR, bw = ww + 2 * kGR;
+  // second group: the tail band rows of the bordered window's rows (threads
+  // 0 .. bh-1) and columns (threads 64 .. 64+bw-1), x_p on it, and the
+  // scaling, b_p and taps of this thread's first window point
+  int yl = 0;
+  float yw[kKPF] = {0.f, 0.f, 0.f, 0.f};
+  const bool is_row = (int)threadIdx.x < bh;
+  const bool is_col = (int)threadIdx.x >= 64 && (int)threadIdx.x < 64 + bw;
+  if (is_row || is_col) {
+    const int q = is_row ? min(max(cr0 - kGR + (int)threadIdx.x, 0), P.h - 1)
+                         : min(max(cc0 - kGR + (int)threadIdx.x - 64, 0), P.w - 1);
+    const GBand& B = is_row ? a.brp : a.bcp;
+    yl = B.c_lo[q];
+#pragma unroll
+    for (int k = 0; k < kKPF; ++k) yw[k] = B.pw[q * kKPF + k];
+  }
+  float xv[NB];
+  bool xin[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
     const int t = threadIdx.x + k * kGTh;
-    const int ly = min(t / kGH1, kGH1 - 1), lx = t % kGH1;
-    const int ia0 = ka[k] - cr0, ib0 = kb[k] - cc0;
-    float v = 0.f;
+    const int rr = cr0 - kGR + t / max(bw, 1), cc = cc0 - kGR + t % max(bw, 1);
+    xin[k] = t < bh * bw && on_grid(rr, cc, P.h, P.w);
+    xv[k] = xp_pre[xin[k] ? rr * P.w + cc : 0];
+  }
+  const int nw = wh * ww;
+  auto win_point = [&](int t, int& K) {
+    const int rr = cr0 + t / max(ww, 1), cc = cc0 + t % max(ww, 1);
+    const bool on = t < nw && on_grid(rr, cc, P.h, P.w);
+    K = on ? rr * P.w + cc : 0;
+    return on;
+  };
+  int K0;
+  const bool on0 = win_point(threadIdx.x, K0);
+  float pa[kGN];
 #pragma unroll
-    for (int p = 0; p < kKPF; ++p) {
-      const int ia = min(max(ia0 + p, 0), kCoarseSpan - 1);
-      float s2 = 0.f;
+  for (int o = 0; o < kGN; ++o) pa[o] = P.A[(size_t)o * np + K0];
+  const float pd0 = P.dsq[K0], pb0 = bp[K0];
+  __builtin_amdgcn_sched_barrier(0);
+  if (is_row) {
+    ylo[0][threadIdx.x] = yl;
 #pragma unroll
-      for (int q = 0; q < kKPF; ++q) s2 += wcl[k][q] * cs[ia][min(max(ib0 + q, 0), kCoarseSpan - 1)];
-      v += wr[k][p] * s2;
-    }
-    if (t < kGH1 * kGH1) xs[ly][lx] = hin[k] ? xp[k] + v : 0.f;
+    for (int k = 0; k < kKPF; ++k) ypw[0][threadIdx.x][k] = yw[k];
+  }
+  if (is_col) {
+    ylo[1][threadIdx.x - 64] = yl;
+#pragma unroll
+    for (int k = 0; k < kKPF; ++k) ypw[1][threadIdx.x - 64][k] = yw[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    if (t < nt) ts[t] = tv[k];
   }
   __syncthreads();
-  const float ax = tap_sum<kGH1>(a, xs, ty + kGR, tx + kGR);
-  if (!in) return;
-  xo[i] = xs[ty + kGR][tx + kGR] + g.omega * d * d * (bi - ax);
+  // y on the bordered window
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int t = threadIdx.x + k * kGTh;
+    if (t < bh * bw) {
+      const int br = t / bw, bc = t - br * bw;
+      float v = 0.f;
+#pragma unroll
+      for (int p = 0; p < kKPF; ++p) {
+        const int ra = min(ylo[0][br] + p, a.ht - 1);
+        float s2 = 0.f;
+#pragma unroll
+        for (int q = 0; q < kKPF; ++q) s2 += ypw[1][bc][q] * ts[ra * a.wt + min(ylo[1][bc] + q, a.wt - 1)];
+        v += ypw[0][br][p] * s2;
+      }
+      ys[br][bc] = xin[k] ? xv[k] + v : 0.f;
+    }
+  }
+  __syncthreads();
+  // x_p' on the window (zero elsewhere in cs)
+  auto smooth_at = [&](int t, int K, float d, float b, const float(&av)[kGN]) {
+    const int wr = t / ww, wc = t - wr * ww;
+    float s = 0.f;
+#pragma unroll
+    for (int dy = -kGR; dy <= kGR; ++dy)
+#pragma unroll
+      for (int dx = -kGR; dx <= kGR; ++dx)
+        s += av[(dy + kGR) * kGS + dx + kGR] * ys[wr + kGR + dy][wc + kGR + dx];
+    return ys[wr + kGR][wc + kGR] + P.omega * d * d * (b - s);
+  };
+  for (int t = threadIdx.x; t < kCoarseSpan * kCoarseSpan; t += kGTh)
+    cs[t / kCoarseSpan][t % kCoarseSpan] = 0.f;
+  __syncthreads();
+  if (on0) {
+    const int t = threadIdx.x;
+    cs[t / ww][t % ww] = smooth_at(t, K0, pd0, pb0, pa);
+  }
+  for (int t = threadIdx.x + kGTh; t < nw; t += kGTh) {
+    int K;
+    if (!win_point(t, K)) continue;
+#pragma unroll
+    for (int o = 0; o < kGN; ++o) pa[o] = P.A[(size_t)o * np + K];
+    cs[t / ww][t % ww] = smooth_at(t, K, P.dsq[K], bp[K], pa);
+  }
+  __syncthreads();
+  T.finish(a.g, cs, xs, cr0, cc0, xo);
 }
 
 // ---------------------------------------------------------------------------
@@ -1598,9 +1776,10 @@ constexpr int kOpsRows = 64;
 constexpr int kOpsMax = kOpsRows * kGN;
 
 __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
+  // the staged segment: columns and values (FP32 levels' values widened
+  // exactly, so one array serves both)
   __shared__ int sc[kOpsMax];
-  __shared__ float sv[kOpsMax];
-  __shared__ double sv64[kOpsMax];
+  __shared__ double sv[kOpsMax];
   __shared__ int sp[kOpsRows + 1];
   const int l = blockIdx.y;
   if (l >= a.n_levels) return;
@@ -1619,14 +1798,12 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
   if (fits && cnt > 0) {
     constexpr int NP = passes(kOpsMax);
     int c[NP];
-    float v[NP];
-    double v64[NP];
+    double v[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
       const int t = min((int)threadIdx.x + k * kGTh, max(cnt - 1, 0));
       c[k] = cols[lo + t];
-      v64[k] = f64 ? vals64[lo + t] : 0.0;
-      v[k] = prep ? (float)v64[k] : vals[lo + t];
+      v[k] = prep || f64 ? vals64[lo + t] : (double)vals[lo + t];
     }
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
@@ -1634,10 +1811,9 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
       if (t < cnt) {
         sc[t] = c[k];
         sv[t] = v[k];
-        sv64[t] = v64[k];
         if (prep) {
-          if (a.csr64) a.csr64[lo + t] = v64[k];
-          a.csr32[lo + t] = v[k];
+          if (a.csr64) a.csr64[lo + t] = v[k];
+          a.csr32[lo + t] = (float)v[k];
         }
       }
     }
@@ -1649,12 +1825,35 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     }
   }
   __syncthreads();
+  // row i's entry at grid offset (ey, ex): at its computed slot of the
+  // staged segment (the in-grid neighbours in row-major order), else
+  // searched in the CSR row (0 off the grid or absent); FP32 in v32
+  auto entry = [&](int i, int ey, int ex, float& v32) -> double {
+    v32 = 0.f;
+    const int il = i - i0, r = i / w, c = i - r * w;
+    if (!on_grid(r + ey, c + ex, h, w)) return 0.0;
+    const int s0 = sp[il], e0 = sp[il + 1];
+    const int dy0 = max(-kGR, -r), dx0 = max(-kGR, -c);
+    const int ncol = min(kGR, w - 1 - c) - dx0 + 1;
+    const int j = i + ey * w + ex;
+    const int k = s0 + (ey - dy0) * ncol + (ex - dx0);
+    if (fits && k < e0 && sc[k] == j) {
+      v32 = (float)sv[k];
+      return f64 || prep ? sv[k] : 0.0;
+    }
+    double v64 = 0.0;
+    for (int q = lo + s0; q < lo + e0; ++q)
+      if (cols[q] == j) {
+        v64 = f64 || prep ? vals64[q] : 0.0;
+        v32 = prep ? (float)v64 : vals[q];
+      }
+    return v64;
+  };
   if (prep && (int)threadIdx.x < i1 - i0) {
     // the row's scalings: 1 / K_ii (1 where zero or absent), D^-1/2
     const int i = i0 + threadIdx.x;
-    const int q0 = lo + sp[threadIdx.x], q1 = lo + sp[threadIdx.x + 1];
-    const int k = lower_bound(cols, q0, q1, i);
-    const double d = (k < q1 && cols[k] == i) ? vals64[k] : 0.0;
+    float dummy;
+    const double d = entry(i, 0, 0, dummy);
     const double di = d != 0.0 ? 1.0 / d : 1.0;
     a.dinv[i] = di;
     a.dsq[i] = (float)sqrt(fabs(di));
@@ -1663,37 +1862,15 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
     const int il = t % kOpsRows, o = t / kOpsRows;
     const int i = i0 + il;
     if (i >= i1) continue;
-    const int r = i / w, c = i - r * w;
     const int dy = o / kGS - kGR, dx = o % kGS - kGR;
-    const int s0 = sp[il], e0 = sp[il + 1];
-    const int dy0 = max(-kGR, -r), dx0 = max(-kGR, -c);
-    const int ncol = min(kGR, w - 1 - c) - dx0 + 1;
-    // the FP64 entry at grid offset (ey, ex) of row i (0 off the grid or absent)
-    auto entry = [&](int ey, int ex, float& v32) -> double {
-      v32 = 0.f;
-      if (!on_grid(r + ey, c + ex, h, w)) return 0.0;
-      const int j = i + ey * w + ex;
-      const int k = s0 + (ey - dy0) * ncol + (ex - dx0);
-      if (fits && k < e0 && sc[k] == j) {
-        v32 = sv[k];
-        return sv64[k];
-      }
-      double v64 = 0.0;
-      for (int q = lo + s0; q < lo + e0; ++q)
-        if (cols[q] == j) {
-          v64 = f64 ? vals64[q] : 0.0;
-          v32 = prep ? (float)v64 : vals[q];
-        }
-      return v64;
-    };
     float v;
-    const double v64 = entry(dy, dx, v);
+    const double v64 = entry(i, dy, dx, v);
     a.op[l][(size_t)o * n + i] = v;
     if (f64) {
       a.op64[(size_t)o * n + i] = v64;
       if (a.feats) {
         float dummy;
-        double d = entry(0, 0, dummy);
+        double d = entry(i, 0, 0, dummy);
         if (d == 0.0) d = 1.0;
         a.feats[(size_t)o * n + i] = (float)(v64 / d);
       }
@@ -2456,10 +2633,23 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
     }
     if (rc) return rc;
   }
+  // the prolongation into level t - 1 joins the one into level t - 2 when
+  // that is a tile level (t >= 3) and the tail fits the staging
+  const bool fuse_prolong = t >= 3 && mg.level[t].n <= kPTail;
   for (int l = t - 1; l >= 1; --l) {
-    const iga_mg_level_t& C = mg.level[l + 1];
-    hipLaunchKernelGGL(k_g_prolong, tiles_of(mg.level[l]), dim3(kGTh), 0, st, glevel(mg, l), C.h,
-                       C.w, gband(mg, l, 1), gband(mg, l, 0), XO[l + 1], X[l], B[l], XO[l], skip);
+    if (fuse_prolong && l == t - 1) continue;
+    if (fuse_prolong && l == t - 2) {
+      const ProlongP pp{glevel(mg, l),       glevel(mg, t - 1),  mg.level[t].h,
+                        mg.level[t].w,       gband(mg, l, 1),    gband(mg, l, 0),
+                        gband(mg, t - 1, 1), gband(mg, t - 1, 0)};
+      hipLaunchKernelGGL(k_g_prolong_p, tiles_of(mg.level[l]), dim3(kGTh), 0, st, pp, XO[t],
+                         X[t - 1], B[t - 1], X[l], B[l], XO[l], skip);
+    } else {
+      const iga_mg_level_t& C = mg.level[l + 1];
+      hipLaunchKernelGGL(k_g_prolong, tiles_of(mg.level[l]), dim3(kGTh), 0, st, glevel(mg, l),
+                         C.h, C.w, gband(mg, l, 1), gband(mg, l, 0), XO[l + 1], X[l], B[l], XO[l],
+                         skip);
+    }
     IGA_CHECK_LAUNCH();
   }
   const iga_mg_level_t& L1 = mg.level[1];
@@ -2642,18 +2832,45 @@ struct FBufs {
   double* P_d;  // [kRedBlocks] r.r
 };
 
+// the solver state at the start of a run, written by k_fcg_init when `fold`
+// (then the first gate takes ||b||^2 from its r.r partials: no check kernel)
+struct FInit {
+  int fold;
+  double tol2;
+  int maxiter;
+};
+
+// x = 0, r = b, the first "previous" direction zero, the b.b partials; with
+// b_in, b is read from there and kept in b (the run's copy of the caller's
+// right-hand side)
 __global__ void __launch_bounds__(kVecThreads)
-    k_fcg_init(int n, const double* __restrict__ b, double* x, double* r, float* r32, double* p1,
-               double* q1, FBufs F, double* x_out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) F.st->x_out = x_out;
+    k_fcg_init(int n, const double* __restrict__ b_in, double* __restrict__ b, double* x,
+               double* r, float* r32, double* p1, double* q1, FBufs F, double* x_out, FInit ini) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    FState* st = F.st;
+    st->x_out = x_out;
+    if (ini.fold) {
+      st->bb = -1.0;
+      st->rr = 0.0;
+      st->tol2 = ini.tol2;
+      st->maxiter = ini.maxiter;
+      st->iter = 0;
+      st->pq_prev = 1.0;
+      st->pending = 0;
+      st->done = 0;
+    }
+  }
+  const double* src = b_in ? b_in : b;
   double bb = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double bi = src[i];
+    if (b_in) b[i] = bi;
     x[i] = 0.0;
-    r[i] = b[i];
-    r32[i] = (float)b[i];
+    r[i] = bi;
+    r32[i] = (float)bi;
     p1[i] = 0.0;  // the "previous" direction of the first iteration (beta = 0)
     q1[i] = 0.0;
-    bb += b[i] * b[i];
+    bb += bi * bi;
   }
   store_partial(F.P_d, bb);
 }
@@ -2741,6 +2958,20 @@ __global__ void __launch_bounds__(kVecThreads)
 __global__ void k_fcg_resid_fin(int nblk, FBufs F) {
   const double s = reduce_partials(F.P_b, nblk);
   if (threadIdx.x == 0) F.st->res2 = s;
+}
+
+// the end-of-batch test of k_fcg_check (not at init) and the residual sum
+// of k_fcg_resid_fin in one kernel
+__global__ void k_fcg_check_res(int nblk_d, int nblk_b, FBufs F) {
+  FState* st = F.st;
+  const double rr = reduce_partials(F.P_d, nblk_d);
+  const double s = reduce_partials(F.P_b, nblk_b);
+  if (threadIdx.x == 0) {
+    st->pending = 0;
+    st->rr = rr;
+    st->done = !(rr > st->tol2 * st->bb) ? 1 : 0;
+    st->res2 = s;
+  }
 }
 
 __global__ void __launch_bounds__(kVecThreads)
@@ -3287,7 +3518,7 @@ struct iga_fcg_plan {
   const int32_t* indptr;
   const int32_t* cols;
   const double* vals;
-  const double* b;
+  double* b;  // written by a run given b_in
   double* x;
   FBufs F;
   double *r, *P[2], *Q[2];
@@ -3357,13 +3588,29 @@ struct iga_fcg_plan {
     return IGA_OK;
   }
 
+  // grid cycle, end of a replay: the waiting update, the FP64 residual of x
+  // and the test (one kernel for the test and the residual's sum)
+  int check_residual(int par, hipStream_t st) {
+    const FcgFuse fz = fuse(1 - par, R[par]);  // R[par] in place
+    hipLaunchKernelGGL(k_fcg_flush, row_tiles(mg.level[0]), dim3(kGTh), 0, st, mg.level[0].h,
+                       mg.level[0].w, fz);
+    by_lpr(lpr, [&](auto L) {
+      constexpr int LPR = decltype(L)::value;
+      hipLaunchKernelGGL(k_fcg_resid<LPR>, dim3(rb), dim3(kVecThreads), 0, st, n, indptr, cols,
+                         vals, x, b, F);
+      return 0;
+    });
+    hipLaunchKernelGGL(k_fcg_check_res, dim3(1), dim3(kVecThreads), 0, st, nt0, rb, F);
+    IGA_CHECK_LAUNCH();
+    return IGA_OK;
+  }
+
   int capture_one(FcgGraph& out, int iters, int par0, hipStream_t st) {
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return IGA_ERR_HIP;
     int rc = 0;
     for (int k = 0; k < iters && !rc; ++k) rc = iteration((par0 + k) & 1, st);
-    if (!rc) rc = check((par0 + iters - 1) & 1, st);
     // grid cycle: the FP64 residual of x is part of every replay
-    if (!rc && grid64) rc = true_residual(st);
+    if (!rc) rc = grid64 ? check_residual((par0 + iters - 1) & 1, st) : check((par0 + iters - 1) & 1, st);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc || e != hipSuccess) {
@@ -3423,20 +3670,25 @@ struct iga_fcg_plan {
   }
 
   int run(double tol, int maxiter, int32_t* out_iters, double* out_rr, double* out_bb,
-          double* out_res, double* x_out, hipStream_t st) {
+          double* out_res, double* x_out, const double* b_in, hipStream_t st) {
+    const bool graphs = gk[1].exec != nullptr;
+    // with graphs the init writes the state itself and the first gate
+    // records ||b||^2 (one kernel less); it runs at least one iteration
+    const bool fold = graphs && maxiter > 0;
     // the first iteration (parity 0) reads R[1] on the grid cycle
     // on the grid cycle the flush at the end of every replay writes x_out
-    hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b, x, grid64 ? R[1] : r,
-                       r32, P[1], Q[1], F, grid64 ? x_out : nullptr);
-    hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol,
-                       maxiter);
+    hipLaunchKernelGGL(k_fcg_init, dim3(vb), dim3(kVecThreads), 0, st, n, b_in, b, x,
+                       grid64 ? R[1] : r, r32, P[1], Q[1], F, grid64 ? x_out : nullptr,
+                       FInit{fold ? 1 : 0, tol * tol, maxiter});
+    if (!fold)
+      hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(kVecThreads), 0, st, vb, F, 1, tol * tol,
+                         maxiter);
     IGA_CHECK_LAUNCH();
     FState host{};
     int launched = 0, status = IGA_OK;
     // the FP64 residual is recomputed after every batch when asked for, so
     // the look that finds convergence also reads it
     bool res_fresh = false;
-    const bool graphs = gk[1].exec != nullptr;
     // iterations before the first look: the last run's count (four on the
     // first run), then one per look; never more than maxiter in all
     int next = last_iters > 0 ? last_iters : 4;
@@ -3511,7 +3763,8 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   P->indptr = indptr;
   P->cols = cols;
   P->vals = vals;
-  P->b = b;
+  // a run writes b only when given b_in (iga_fcg_mg passes none)
+  P->b = const_cast<double*>(b);
   P->x = x;
   P->F.st = reinterpret_cast<FState*>(work);
   P->r = work + 8;
@@ -3550,9 +3803,10 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
 
 extern "C" int iga_fcg_mg_run(iga_fcg_plan_t* plan, double tol, int32_t maxiter,
                               int32_t* out_iters, double* out_rr, double* out_bb,
-                              double* out_res, double* x_out, void* stream) {
+                              double* out_res, double* x_out, const double* b_in, void* stream) {
   if (!plan) return IGA_ERR_ARG;
-  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, out_res, x_out, (hipStream_t)stream);
+  return plan->run(tol, maxiter, out_iters, out_rr, out_bb, out_res, x_out, b_in,
+                   (hipStream_t)stream);
 }
 
 extern "C" void iga_fcg_mg_destroy(iga_fcg_plan_t* plan) { delete plan; }
@@ -3576,7 +3830,7 @@ extern "C" int iga_fcg_mg(const iga_mg_t* mg, int32_t n, const int32_t* indptr,
   int rc = iga_fcg_mg_plan(mg, n, nnz, indptr, cols, vals, b, x, work, work32, graph_iters, stream,
                            &plan);
   if (rc) return rc;
-  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, nullptr, nullptr, st);
+  rc = plan->run(tol, maxiter, out_iters, out_rr, out_bb, nullptr, nullptr, nullptr, st);
   delete plan;
   return rc;
 }

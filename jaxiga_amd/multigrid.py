@@ -820,12 +820,15 @@ class MGOperator:
 
         it, rr, bb, res = _lib.c_int(0), _lib.c_double(0.0), _lib.c_double(0.0), _lib.c_double(-1.0)
         plan = self._plan(graph_iters)  # captured once; replayed on the current stream
-        self.b.copy_(b)
+        if b.dtype != torch.float64 or b.device != self.b.device or b.numel() != self.b.numel():
+            raise ValueError("fcg: b must be a float64 vector of the level-0 size on the operator's device")
+        b = b.contiguous()
         x = torch.empty_like(self.x)  # filled by the run, behind the iterations
+        # the run's first kernel reads b and keeps it in self.b
         _lib.check(lib().iga_fcg_mg_run(plan.handle, float(tol), int(maxiter), _lib.ctypes.byref(it),
                                         _lib.ctypes.byref(rr), _lib.ctypes.byref(bb),
                                         _lib.ctypes.byref(res) if true_residual else None,
-                                        _lib.ptr(x), stream()),
+                                        _lib.ptr(x), _lib.ptr(b), stream()),
                    "iga_fcg_mg_run")
         if true_residual:
             return x, it.value, rr.value, bb.value, res.value
