@@ -2251,45 +2251,51 @@ __device__ __forceinline__ void tail_g_entry(const TailArgs& a, const TailWork& 
 }
 
 // Gauss-Jordan on [A | I] for n <= kGjWave in one wavefront: lane j holds
-// column j in registers; in step k lane k puts its column (the multipliers
-// M[i][k]) into LDS, every lane reads it back as broadcasts, scales its
-// entry k by the pivot and subtracts M[i][k] times it from every other
-// entry i -- the operations of the elimination in LDS below, in the same
-// order, without a workgroup barrier per step. Rows i >= n are zero and
-// stay zero. Writes the inverse (FP64 into inv64, FP32 into inv32),
-// row-major n x n; s_col: kGjWave doubles of LDS.
+// column j in registers; in step k every lane reads lane k's column (the
+// multipliers M[i][k]) as scalars (readlane), scales its entry k by the
+// pivot's reciprocal and subtracts M[i][k] times it from every other entry
+// i -- the operations of the elimination in LDS below, in the same order,
+// with no LDS traffic or barrier per step. Rows i >= n are zero and stay
+// zero.
+// Writes the inverse (FP64 into inv64, FP32 into inv32), row-major n x n.
 constexpr int kGjWave = kWave / 2;
 
-__device__ __forceinline__ void gj_wave(const TailLevel& C, int n, double* __restrict__ inv64,
-                                        float* __restrict__ inv32, double* __restrict__ s_col) {
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int NR>
+__device__ __forceinline__ void gj_wave_n(const TailLevel& C, int n, double* __restrict__ inv64,
+                                          float* __restrict__ inv32) {
   const int j = threadIdx.x % kWave;
-  double col[kGjWave];
+  double col[NR];
 #pragma unroll
-  for (int i = 0; i < kGjWave; ++i) {
+  for (int i = 0; i < NR; ++i) {
     const double av = a_entry(C, min(i, n - 1), min(j, n - 1));
     col[i] = (i < n && j < 2 * n) ? (j < n ? av : (j == n + i ? 1.0 : 0.0)) : 0.0;
   }
 #pragma unroll
-  for (int k = 0; k < kGjWave; ++k) {
+  for (int k = 0; k < NR; ++k) {
     if (k < n) {
-      if (j == k) {
+      double m[NR];
 #pragma unroll
-        for (int i = 0; i < kGjWave; ++i) s_col[i] = col[i];
-      }
-      wave_lds_sync();
-      double m[kGjWave];
+      for (int i = 0; i < NR; ++i) m[i] = lane_value(col[i], k);
+      // the pivot's reciprocal (two Newton steps on the hardware estimate)
+      double r = __builtin_amdgcn_rcp(m[k]);
+      r = fma(fma(-m[k], r, 1.0), r, r);
+      r = fma(fma(-m[k], r, 1.0), r, r);
+      col[k] *= r;
 #pragma unroll
-      for (int i = 0; i < kGjWave; ++i) m[i] = s_col[i];
-      col[k] /= m[k];
-#pragma unroll
-      for (int i = 0; i < kGjWave; ++i)
+      for (int i = 0; i < NR; ++i)
         if (i != k) col[i] -= m[i] * col[k];
-      wave_lds_sync();
     }
   }
   if (j >= n && j < 2 * n) {
 #pragma unroll
-    for (int i = 0; i < kGjWave; ++i) {
+    for (int i = 0; i < NR; ++i) {
       if (i < n) {
         inv64[i * n + j - n] = col[i];
         inv32[i * n + j - n] = (float)col[i];
@@ -2298,21 +2304,44 @@ __device__ __forceinline__ void gj_wave(const TailLevel& C, int n, double* __res
   }
 }
 
+// the register arrays sized to the level (the step loop is unrolled: rows
+// beyond n would cost instructions in every step)
+__device__ __forceinline__ void gj_wave(const TailLevel& C, int n, double* __restrict__ inv64,
+                                        float* __restrict__ inv32) {
+  if (n <= 9)
+    gj_wave_n<9>(C, n, inv64, inv32);
+  else if (n <= 16)
+    gj_wave_n<16>(C, n, inv64, inv32);
+  else if (n <= 25)
+    gj_wave_n<25>(C, n, inv64, inv32);
+  else
+    gj_wave_n<kGjWave>(C, n, inv64, inv32);
+}
+
 // G_l for the levels l0 + blockIdx.y (below lc); with a.gj_in_g, the slice
 // blockIdx.y = lc - l0 is the coarse inverse (V_lc at the front of the work
 // and the FP32 copy), formed by one wavefront beside the products
 __global__ void __launch_bounds__(256) k_tail_g(TailArgs a, TailWork tw, int l0) {
   __shared__ double s_st[kGN * kGjWave];
-  __shared__ double s_col[kGjWave];
   const int l = l0 + blockIdx.y;
   if (l == a.lc && a.gj_in_g) {
     if (blockIdx.x != 0) return;
     const TailLevel& C = a.lv[a.lc];
-    for (int e = threadIdx.x; e < kGN * C.n; e += blockDim.x) s_st[e] = C.op[e];
+    // every load of the staging in flight together (n_lc <= kGjWave)
+    constexpr int NS = (kGN * kGjWave + 255) / 256;
+    const int ns = kGN * C.n;
+    double v[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) v[k] = C.op[min((int)threadIdx.x + k * 256, ns - 1)];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int e = threadIdx.x + k * 256;
+      if (e < ns) s_st[e] = v[k];
+    }
     __syncthreads();
     TailLevel Cs = C;
     Cs.op = s_st;
-    if (threadIdx.x < kWave) gj_wave(Cs, C.n, a.work, a.coarse_inv, s_col);
+    if (threadIdx.x < kWave) gj_wave(Cs, C.n, a.work, a.coarse_inv);
     return;
   }
   if (l >= a.lc) return;
@@ -2352,6 +2381,37 @@ __device__ __forceinline__ void tile_dgemm(int m, int n, int kk, FA A, FB B, FO 
     for (int r = 0; r < 4; ++r) {
       const int i = i0 + l / 16 + 4 * r, j = j0 + l % 16;
       if (i < m && j < n) out(i, j, acc[r]);
+    }
+  }
+}
+
+// tile_dgemm with every k-step's operands of a tile read (from clamped
+// indices) before its chain of matrix-core products: the reads overlap
+// instead of each product waiting for its own. kk <= 4 KS.
+template <int KS, class FA, class FB, class FO>
+__device__ __forceinline__ void tile_dgemm_pre(int m, int n, int kk, FA A, FB B, FO out) {
+  const int nw = blockDim.x / kWave, l = threadIdx.x % kWave;
+  const int tm = (m + 15) / 16, tn = (n + 15) / 16;
+  for (int t = threadIdx.x / kWave; t < tm * tn; t += nw) {
+    const int i0 = (t / tn) * 16, j0 = (t % tn) * 16;
+    const int i = i0 + l % 16, j = j0 + l % 16;
+    const int ic = min(i, m - 1), jc = min(j, n - 1);
+    double av[KS], bv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + l / 16, kc = min(k, kk - 1);
+      av[s] = A(ic, kc);
+      bv[s] = B(kc, jc);
+      av[s] = i < m && k < kk ? av[s] : 0.0;
+      bv[s] = j < n && k < kk ? bv[s] : 0.0;
+    }
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int io = i0 + l / 16 + 4 * r;
+      if (io < m && j < n) out(io, j, acc[r]);
     }
   }
 }
@@ -2522,6 +2582,151 @@ __global__ void __launch_bounds__(256) k_tail_v(TailArgs a, TailWork tw, float* 
     out = (float)s;
   }
   tail[(size_t)i * ld + j] = out;
+}
+
+// V_t in one launch when the coarse inverse sits two levels below t (n_{t+1}
+// small): every block forms V_{t+1} = G_{t+1} V_{t+2} G_{t+1}^T + 2M - MAM
+// in LDS, then the columns of W = V_{t+1} G_t^T its tile needs, then its
+// kTFT x kTFT tile of V_t = G_t W + 2M - MAM, all on the FP64 matrix cores
+// (the products of k_tail_small, k_tail_w and k_tail_v, repeated per block
+// where they are small). Blocks past n_t write the zero columns up to ld.
+// LDS: V_{t+1}, then one region holding V_{t+2}, G_{t+1} and G_{t+1}V_{t+2}
+// while V_{t+1} is formed and the G_t rows of the tile, W and the tile's
+// 2M - MAM afterwards (those are read into registers first).
+constexpr int kTFT = 32;           // tile of V_t
+constexpr int kTFTh = 512;         // threads per block
+constexpr int kTFLds = 16384;      // doubles of LDS
+constexpr int kTFMaxN1 = 96;       // n_{t+1}
+constexpr int kTFRowVals = (kTFT * kTFMaxN1 + kTFTh - 1) / kTFTh;  // G_t values per thread
+constexpr int kTFLin = kTFT * kTFT / kTFTh;                         // tile entries per thread
+
+__global__ void __launch_bounds__(kTFTh)
+    k_tail_fused(TailArgs a, TailWork tw, float* __restrict__ tail, int ld) {
+  __shared__ double lds[kTFLds];
+  const int t = a.t;
+  const TailLevel& Lt = a.lv[t];
+  const TailLevel& L1 = a.lv[t + 1];
+  const int nt = Lt.n, n1 = L1.n, n2 = a.lv[t + 2].n;
+  const int i0 = blockIdx.y * kTFT, j0 = blockIdx.x * kTFT;
+  if (j0 >= nt) {
+    for (int e = threadIdx.x; e < kTFT * kTFT; e += kTFTh) {
+      const int i = i0 + e / kTFT, j = j0 + e % kTFT;
+      if (i < nt && j < ld) tail[(size_t)i * ld + j] = 0.f;
+    }
+    return;
+  }
+  double* sV1 = lds;             // n1 x n1
+  double* sV2 = lds + n1 * n1;   // n2 x n2
+  double* sG1 = sV2 + n2 * n2;   // n1 x n2
+  double* sT1 = sG1 + n1 * n2;   // n1 x n2
+  double* sGi = lds + n1 * n1;   // kTFT x n1 (after V_{t+1}): G_t rows i0 ..
+  double* sGj = sGi + kTFT * n1; // kTFT x n1: G_t rows j0 ..
+  double* sW = sGj + kTFT * n1;  // n1 x kTFT
+  double* sLin = sW + n1 * kTFT; // kTFT x kTFT
+  const double* V2 = a.work;     // the coarse inverse, at the front
+  const double* G1 = a.work + tw.goff[t + 1];
+  const double* Gt = a.work + tw.goff[t];
+  // the tile's G_t rows and 2M - MAM into registers, beside the loads of
+  // the first products' operands into LDS
+  double gi[kTFRowVals], gj[kTFRowVals], lin[kTFLin];
+#pragma unroll
+  for (int k = 0; k < kTFRowVals; ++k) {
+    const int e = threadIdx.x + k * kTFTh;
+    const int r = e / n1, m = e - r * n1;
+    const bool ok = e < kTFT * n1;
+    gi[k] = Gt[(size_t)(ok && i0 + r < nt ? i0 + r : 0) * n1 + (ok ? m : 0)];
+    gj[k] = Gt[(size_t)(ok && j0 + r < nt ? j0 + r : 0) * n1 + (ok ? m : 0)];
+    gi[k] = ok && i0 + r < nt ? gi[k] : 0.0;
+    gj[k] = ok && j0 + r < nt ? gj[k] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < kTFLin; ++k) {
+    const int e = threadIdx.x + k * kTFTh;
+    const int i = min(i0 + e / kTFT, nt - 1), j = min(j0 + e % kTFT, nt - 1);
+    const double mi = a.omega * Lt.dinv[i], mj = a.omega * Lt.dinv[j];
+    lin[k] = (i == j ? 2.0 * mi : 0.0) - mi * a_entry(Lt, i, j) * mj;
+  }
+  // V_{t+2}, G_{t+1} and level t+1's stencil and scalings: one group of
+  // loads, then LDS (the stencil and scalings behind G_{t+1} V_{t+2})
+  double* sOp = sT1 + n1 * n2;   // kGN x n1
+  double* sDi = sOp + kGN * n1;  // n1
+  {
+    constexpr int N2 = (kGjWave * kGjWave + kTFTh - 1) / kTFTh;
+    constexpr int N12 = (kTFMaxN1 * kGjWave + kTFTh - 1) / kTFTh;
+    constexpr int NO = (kGN * kTFMaxN1 + kTFTh - 1) / kTFTh;
+    double v2[N2], g1[N12], op[NO], di;
+#pragma unroll
+    for (int k = 0; k < N2; ++k) v2[k] = V2[min((int)threadIdx.x + k * kTFTh, n2 * n2 - 1)];
+#pragma unroll
+    for (int k = 0; k < N12; ++k) g1[k] = G1[min((int)threadIdx.x + k * kTFTh, n1 * n2 - 1)];
+#pragma unroll
+    for (int k = 0; k < NO; ++k) op[k] = L1.op[min((int)threadIdx.x + k * kTFTh, kGN * n1 - 1)];
+    di = L1.dinv[min((int)threadIdx.x, n1 - 1)];
+#pragma unroll
+    for (int k = 0; k < N2; ++k) {
+      const int e = threadIdx.x + k * kTFTh;
+      if (e < n2 * n2) sV2[e] = v2[k];
+    }
+#pragma unroll
+    for (int k = 0; k < N12; ++k) {
+      const int e = threadIdx.x + k * kTFTh;
+      if (e < n1 * n2) sG1[e] = g1[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+      const int e = threadIdx.x + k * kTFTh;
+      if (e < kGN * n1) sOp[e] = op[k];
+    }
+    if ((int)threadIdx.x < n1) sDi[threadIdx.x] = di;
+  }
+  __syncthreads();
+  {
+    TailLevel S1 = L1;
+    S1.op = sOp;
+    for (int e = threadIdx.x; e < n1 * n1; e += kTFTh) {
+      const int i = e / n1, j = e - i * n1;
+      const double mi = a.omega * sDi[i], mj = a.omega * sDi[j];
+      sV1[e] = (i == j ? 2.0 * mi : 0.0) - mi * a_entry(S1, i, j) * mj;
+    }
+  }
+  tile_dgemm_pre<kGjWave / 4>(n1, n2, n2, [&](int i, int k) { return sG1[i * n2 + k]; },
+                              [&](int k, int j) { return sV2[k * n2 + j]; },
+                              [&](int i, int j, double v) { sT1[i * n2 + j] = v; });
+  __syncthreads();
+  tile_dgemm_pre<kGjWave / 4>(n1, n1, n2, [&](int i, int k) { return sT1[i * n2 + k]; },
+             [&](int k, int j) { return sG1[j * n2 + k]; },
+             [&](int i, int j, double v) { sV1[i * n1 + j] += v; });
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kTFRowVals; ++k) {
+    const int e = threadIdx.x + k * kTFTh;
+    if (e < kTFT * n1) {
+      sGi[e] = gi[k];
+      sGj[e] = gj[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kTFLin; ++k) sLin[threadIdx.x + k * kTFTh] = lin[k];
+  __syncthreads();
+  // W[m][c] = sum_k V_{t+1}[m][k] G_t[j0 + c][k]
+  tile_dgemm_pre<kTFMaxN1 / 4>(n1, kTFT, n1, [&](int i, int k) { return sV1[i * n1 + k]; },
+             [&](int k, int j) { return sGj[j * n1 + k]; },
+             [&](int i, int j, double v) { sW[i * kTFT + j] = v; });
+  __syncthreads();
+  tile_dgemm_pre<kTFMaxN1 / 4>(kTFT, kTFT, n1, [&](int i, int k) { return sGi[i * n1 + k]; },
+             [&](int k, int j) { return sW[k * kTFT + j]; },
+             [&](int r, int c, double v) {
+               const int i = i0 + r, j = j0 + c;
+               if (i < nt) tail[(size_t)i * ld + j] = j < nt ? (float)(v + sLin[r * kTFT + c]) : 0.f;
+             });
+}
+
+// whether k_tail_fused forms V_t (see there)
+static bool tail_fused_fits(const iga_mg_setup_t& s, int t, int lc, bool gj_in_g) {
+  if (!gj_in_g || lc != t + 2) return false;
+  const int64_t n1 = s.level[t + 1].n, n2 = s.level[t + 2].n;
+  return n1 <= kTFMaxN1 &&
+         n1 * n1 + std::max(n2 * n2 + 2 * n1 * n2 + kGN * n1 + n1, 3 * kTFT * n1 + kTFT * kTFT) <= kTFLds;
 }
 
 static inline GLevel glevel(const iga_mg_t& mg, int l) {
@@ -3081,9 +3286,14 @@ static int tail_dense(const iga_mg_setup_t& s, hipStream_t st) {
   // a small coarse inverse is formed beside the G products
   a.gj_in_g = s.level[lc].n <= kGjWave;
   hipLaunchKernelGGL(k_tail_g, dim3((gmax + 255) / 256, lc - t + a.gj_in_g), dim3(256), 0, st, a, tw, t);
-  hipLaunchKernelGGL(k_tail_small, dim3(1), dim3(1024), 0, st, a, tw);
-  hipLaunchKernelGGL(k_tail_w, dim3((nc * nt + 255) / 256), dim3(256), 0, st, a, tw);
-  hipLaunchKernelGGL(k_tail_v, dim3((ld + 255) / 256, nt), dim3(256), 0, st, a, tw, G.tail, ld);
+  if (tail_fused_fits(s, t, lc, a.gj_in_g)) {
+    hipLaunchKernelGGL(k_tail_fused, dim3(ld / kTFT, (nt + kTFT - 1) / kTFT), dim3(kTFTh), 0, st, a,
+                       tw, G.tail, ld);
+  } else {
+    hipLaunchKernelGGL(k_tail_small, dim3(1), dim3(1024), 0, st, a, tw);
+    hipLaunchKernelGGL(k_tail_w, dim3((nc * nt + 255) / 256), dim3(256), 0, st, a, tw);
+    hipLaunchKernelGGL(k_tail_v, dim3((ld + 255) / 256, nt), dim3(256), 0, st, a, tw, G.tail, ld);
+  }
   IGA_CHECK_LAUNCH();
   return IGA_OK;
 }
