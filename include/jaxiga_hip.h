@@ -681,10 +681,11 @@ int64_t iga_fcg_workspace(int32_t n);
  * vals, work, and everything the iga_mg_t points to); each run re-initialises
  * the solver state on the device and replays the graphs, so one plan serves
  * every right-hand side written into b and every operator whose values are
- * rewritten in place. A run replays as many iterations as the plan's last
- * run needed before it first reads the convergence flag, then one per read,
- * and never more than maxiter in all. The plan owns only the
- * graphs; destroy it before the buffers are freed. */
+ * rewritten in place. A run replays as many iterations as the plan's recent
+ * runs (the last eight) most often needed before it first reads the
+ * convergence flag, then one per read, and never more than maxiter in all.
+ * The plan owns only the graphs; destroy it before the buffers are freed.
+ * b is written by a run given b_in (iga_fcg_mg_run). */
 typedef struct iga_fcg_plan iga_fcg_plan_t;
 int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const int32_t* indptr,
                     const int32_t* cols, const double* vals, const double* b, double* x,

@@ -3738,7 +3738,24 @@ struct iga_fcg_plan {
   static constexpr int kMaxGraph = 16;
   FcgGraph gk[kMaxGraph + 1];  // gk[k]: k iterations from parity 0
   FcgGraph g1odd;              // one iteration from parity 1
-  int last_iters = 0;
+  // iteration counts of the last kHist runs (newest at hist[0]); the first
+  // replay of a run takes the most frequent one (ties: the newer)
+  static constexpr int kHist = 8;
+  int hist[kHist] = {};
+  int n_hist = 0;
+
+  int predicted_iters() const {
+    int best = 0, best_count = 0;
+    for (int a = 0; a < n_hist; ++a) {
+      int c = 0;
+      for (int b = 0; b < n_hist; ++b) c += hist[b] == hist[a];
+      if (c > best_count) {
+        best = hist[a];
+        best_count = c;
+      }
+    }
+    return best;
+  }
 
   // the waiting update of the iteration before one of parity par, applied
   // from R[1 - par] into r_new (R[par] inside a cycle, in place at a flush)
@@ -3899,9 +3916,10 @@ struct iga_fcg_plan {
     // the FP64 residual is recomputed after every batch when asked for, so
     // the look that finds convergence also reads it
     bool res_fresh = false;
-    // iterations before the first look: the last run's count (four on the
-    // first run), then one per look; never more than maxiter in all
-    int next = last_iters > 0 ? last_iters : 4;
+    // iterations before the first look: the most frequent count of the
+    // recent runs (four on the first run), then one per look; never more
+    // than maxiter in all
+    int next = n_hist > 0 && predicted_iters() > 0 ? predicted_iters() : 4;
     for (bool look = false;; look = true) {
       if (look && (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) !=
                        hipSuccess ||
@@ -3939,7 +3957,11 @@ struct iga_fcg_plan {
         break;
       }
     }
-    if (status == IGA_OK) last_iters = host.iter;
+    if (status == IGA_OK) {
+      for (int k = kHist - 1; k > 0; --k) hist[k] = hist[k - 1];
+      hist[0] = host.iter;
+      n_hist = std::min(n_hist + 1, kHist);
+    }
     if (out_iters) *out_iters = host.iter;
     if (out_rr) *out_rr = host.rr;
     if (out_bb) *out_bb = host.bb;
