@@ -493,6 +493,9 @@ typedef struct iga_mg_level {
   const int32_t* pt_ptr;
   const int32_t* pt_cols;
   const float* pt_vals;
+  /* optional (level 0, grid cycle): an FP16 copy of `stencil` (binary16,
+   * same layout) that the grid cycle's level-0 smoothers read in its place */
+  const void* stencil16;
 } iga_mg_level_t;
 
 /* Grid form of a hierarchy (optional). The free dofs of every level form an

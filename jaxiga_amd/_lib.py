@@ -71,7 +71,7 @@ class IgaMgLevel(ctypes.Structure):
         ("cols", c_void_p), ("vals", c_void_p), ("dsq", c_void_p), ("stencil", c_void_p),
         ("omega", ctypes.c_float), ("p_nnz", c_int), ("pt_nnz", c_int), ("p_ptr", c_void_p),
         ("p_cols", c_void_p), ("p_vals", c_void_p), ("pt_ptr", c_void_p), ("pt_cols", c_void_p),
-        ("pt_vals", c_void_p),
+        ("pt_vals", c_void_p), ("stencil16", c_void_p),
     ]
 
 

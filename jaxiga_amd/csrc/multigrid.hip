@@ -1405,9 +1405,9 @@ struct BoxLoad {
 
 // x0 = S b0: D^-1/2 T_W (D^-1/2 b0), or w D^-1 b0 without a stencil; the
 // first kernel of a cycle (gated)
-template <bool STW>
+template <bool STW, typename WT = float>
 __global__ void __launch_bounds__(kGTh)
-    k_g0_smooth(GLevel g, const float* __restrict__ W, const float* __restrict__ b,
+    k_g0_smooth(GLevel g, const WT* __restrict__ W, const float* __restrict__ b,
                 float* __restrict__ x, Gate gate) {
   __shared__ float ub[kL0HH][kL0HW];
   const int n = g.h * g.w;
@@ -1452,9 +1452,9 @@ struct FcgFuse {
 };
 
 // the update (when one waits), then x0 = S b0 with b0 = r32 on the box
-template <bool STW>
+template <bool STW, typename WT = float>
 __global__ void __launch_bounds__(kGTh)
-    k_g0_smooth_upd(GLevel g, const float* __restrict__ W, FcgFuse f, float* __restrict__ x0) {
+    k_g0_smooth_upd(GLevel g, const WT* __restrict__ W, FcgFuse f, float* __restrict__ x0) {
   __shared__ float ub[kL0HH][kL0HW];
   const int n = g.h * g.w;
   const L0Tile T = l0_tile(g.h, g.w);
@@ -1656,9 +1656,9 @@ __global__ void __launch_bounds__(kGTh)
 
 // z = x + D^-1/2 T_W^T u (or x + w D^-1/2 u), u = D^-1/2 r, with the partial
 // of z . dot.y per tile; the transposed stencil reads the neighbours' weights
-template <bool STW>
+template <bool STW, typename WT = float>
 __global__ void __launch_bounds__(kGTh)
-    k_g0_smooth_t(GLevel g, const float* __restrict__ W, const float* __restrict__ x,
+    k_g0_smooth_t(GLevel g, const WT* __restrict__ W, const float* __restrict__ x,
                   const float* __restrict__ u, float* __restrict__ z,
                   const int* __restrict__ skip, DotOut dot) {
   __shared__ float ubx[kL0HH][kL0HW];
@@ -2786,21 +2786,31 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
   }
   const int* skip = gate.flag;
   const iga_mg_level_t& L0 = mg.level[0];
+  // the FP16 copy of the level-0 smoother, when given, in place of the stencil
+  const _Float16* W16 = L0.stencil ? static_cast<const _Float16*>(L0.stencil16) : nullptr;
   const GLevel g0 = glevel(mg, 0);
   const dim3 t0 = row_tiles(L0);
   if (fz) {
     // flexible CG: the waiting update first, its convergence test (the
     // gate) in the residual kernel
-    if (L0.stencil)
+    if (W16)
+      hipLaunchKernelGGL((k_g0_smooth_upd<true, _Float16>), t0, dim3(kGTh), 0, st, g0, W16, *fz,
+                         X[0]);
+    else if (L0.stencil)
       hipLaunchKernelGGL(k_g0_smooth_upd<true>, t0, dim3(kGTh), 0, st, g0, L0.stencil, *fz, X[0]);
     else
-      hipLaunchKernelGGL(k_g0_smooth_upd<false>, t0, dim3(kGTh), 0, st, g0, nullptr, *fz, X[0]);
+      hipLaunchKernelGGL((k_g0_smooth_upd<false, float>), t0, dim3(kGTh), 0, st, g0, nullptr, *fz,
+                         X[0]);
     hipLaunchKernelGGL(k_g0_resid, t0, dim3(kGTh), 0, st, g0, X[0], b0, Rr[0], gate);
   } else {
-    if (L0.stencil)
+    if (W16)
+      hipLaunchKernelGGL((k_g0_smooth<true, _Float16>), t0, dim3(kGTh), 0, st, g0, W16, b0, X[0],
+                         gate);
+    else if (L0.stencil)
       hipLaunchKernelGGL(k_g0_smooth<true>, t0, dim3(kGTh), 0, st, g0, L0.stencil, b0, X[0], gate);
     else
-      hipLaunchKernelGGL(k_g0_smooth<false>, t0, dim3(kGTh), 0, st, g0, nullptr, b0, X[0], gate);
+      hipLaunchKernelGGL((k_g0_smooth<false, float>), t0, dim3(kGTh), 0, st, g0, nullptr, b0, X[0],
+                         gate);
     hipLaunchKernelGGL(k_g0_resid, t0, dim3(kGTh), 0, st, g0, X[0], b0, Rr[0],
                        Gate{skip, nullptr, 0, nullptr, 0});
   }
@@ -2863,12 +2873,15 @@ static int vcycle_grid(const iga_mg_t& mg, const float* b0, float* z, const Gate
   const dim3 tg = t0;
   const bool fuse = dot.parts && (int)(tg.x * tg.y) <= kRedBlocks;
   const DotOut d = fuse ? dot : DotOut{nullptr, nullptr};
-  if (L0.stencil)
+  if (W16)
+    hipLaunchKernelGGL((k_g0_smooth_t<true, _Float16>), tg, dim3(kGTh), 0, st, g0, W16, XO[0],
+                       B[0], z, skip, d);
+  else if (L0.stencil)
     hipLaunchKernelGGL(k_g0_smooth_t<true>, tg, dim3(kGTh), 0, st, g0, L0.stencil, XO[0], B[0], z,
                        skip, d);
   else
-    hipLaunchKernelGGL(k_g0_smooth_t<false>, tg, dim3(kGTh), 0, st, g0, nullptr, XO[0], B[0], z,
-                       skip, d);
+    hipLaunchKernelGGL((k_g0_smooth_t<false, float>), tg, dim3(kGTh), 0, st, g0, nullptr, XO[0],
+                       B[0], z, skip, d);
   IGA_CHECK_LAUNCH();
   if (fuse) {
     *dot_blocks = (int)(tg.x * tg.y);

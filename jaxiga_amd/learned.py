@@ -299,7 +299,9 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=Non
     if gen is None or not op.split_load():
         op.load(vals64)
         if gen is not None:
-            op.set_stencils(smoother_stencils(gen, op, levels))
+            # the grid cycle's level-0 smoothers read an FP16 copy (the
+            # generator's own arithmetic is FP16 on the matrix cores)
+            op.set_stencils(smoother_stencils(gen, op, levels), half="copy")
         return op
     # the generator reads only level 0's features: it runs on a second
     # stream beside the Galerkin levels and the dense tail, which are
@@ -315,7 +317,7 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=Non
     with torch.cuda.stream(side):
         stencils = smoother_stencils(gen, op, levels)
     main.wait_stream(side)
-    op.set_stencils(stencils)
+    op.set_stencils(stencils, half="copy")
     return op
 
 
@@ -351,7 +353,7 @@ class CapturedSetup:
         self.graph.replay()
         # the operator may have been set up since without these smoothers
         # (another generator, or none): point it at the graph's buffers again
-        self.op.set_stencils(self.stencils)
+        self.op.set_stencils(self.stencils, half="ready")
         return self.op
 
 

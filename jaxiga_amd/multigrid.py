@@ -532,6 +532,9 @@ class MGOperator:
         self.coarse_inv = torch.empty(c.n * c.n, **f32)
         self._stencil_buf = {}
         self.stencils = [None] * hier.n_levels
+        # FP16 copy of level 0's stencil for the grid cycle (see set_stencils)
+        self._stencil16 = None
+        self._half = False
         n0 = hier.levels[0].n
         self.b = torch.empty(n0, **f64)
         self.x = torch.empty(n0, **f64)
@@ -721,7 +724,14 @@ class MGOperator:
             self._stencil_buf[l] = buf
         return buf
 
-    def set_stencils(self, stencils):
+    def set_stencils(self, stencils, half=None):
+        """Smoother stencils per level (None: damped Jacobi), copied into the
+        persistent buffers unless they are those buffers. ``half``: the grid
+        cycle's level-0 smoothers read an FP16 copy of level 0's stencil --
+        "copy" writes it (on the stream), "ready" takes the one written last
+        (by a captured set-up that made it); None: FP32 only."""
+        import torch
+
         new = []
         for l, s in enumerate(list(stencils) + [None] * (self.hier.n_levels - len(stencils))):
             if s is None:
@@ -733,6 +743,15 @@ class MGOperator:
             new.append(buf)
         if [x is None for x in new] != [x is None for x in self.stencils]:
             self._struct = None
+        half_on = (half == "copy" or (half == "ready" and self._stencil16 is not None)) and \
+            new[0] is not None and self._grid is not None
+        if half_on and half == "copy":
+            if self._stencil16 is None:
+                self._stencil16 = torch.empty_like(new[0], dtype=torch.float16)
+            self._stencil16.copy_(new[0])
+        if half_on != self._half:
+            self._struct = None
+        self._half = half_on
         was = self.tail_active()
         self.stencils = new
         if self.tail_active() and not was:
@@ -745,7 +764,7 @@ class MGOperator:
             return self._struct
         # one descriptor per pattern of levels with stencils (their buffers
         # are persistent), built once
-        key = (tuple(s is None for s in self.stencils), self.tail_active())
+        key = (tuple(s is None for s in self.stencils), self.tail_active(), self._half)
         cached = self.__dict__.setdefault("_structs", {}).get(key)
         if cached is not None:
             self._struct = cached
@@ -760,6 +779,8 @@ class MGOperator:
             L.indptr, L.cols = lv.indptr.data_ptr(), lv.cols.data_ptr()
             L.vals, L.dsq = self.vals32[l].data_ptr(), self.dsq[l].data_ptr()
             L.stencil = self.stencils[l].data_ptr() if self.stencils[l] is not None else None
+            if l == 0 and self._half:
+                L.stencil16 = self._stencil16.data_ptr()
             L.omega = self.omega
             if lv.p_dev is not None:
                 ip, cc, _, v32 = lv.p_dev
@@ -789,7 +810,7 @@ class MGOperator:
         return x
 
     def _plan(self, graph_iters):
-        key = (tuple(s is None for s in self.stencils), self.tail_active(), graph_iters)
+        key = (tuple(s is None for s in self.stencils), self.tail_active(), self._half, graph_iters)
         plan = self._plans.get(key)
         if plan is None:
             import torch

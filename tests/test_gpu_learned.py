@@ -186,6 +186,19 @@ def test_grid_vcycle_matches_oracle(gpu, refine):
         xr = OM.vcycle(ref, st, b, omega=op.omega)
         print(f"[grid] refine {refine} stencil {st is not None}: rel {_rel(x, xr):.2e}", flush=True)
         assert _rel(x, xr) < 1e-4
+    # the FP16 copy of the level-0 stencil that the learned set-up hands the
+    # grid cycle: the same cycle with the stencil rounded to binary16 (from
+    # its FP32 buffer, as the copy is made)
+    op = LN.setup(ctx, vals)
+    op.set_stencils([torch.as_tensor(s0, dtype=torch.float32, device="cuda")], half="copy")
+    assert op.grid_active() and op.struct().level[0].stencil16
+    x = op.vcycle(torch.as_tensor(b, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+    s16 = s0.astype(np.float32).astype(np.float16).astype(np.float64)
+    xr16 = OM.vcycle(ref, [s16] + [None] * (len(ref) - 2), b, omega=op.omega)
+    xr32 = OM.vcycle(ref, [s0] + [None] * (len(ref) - 2), b, omega=op.omega)
+    print(f"[grid fp16] refine {refine}: rel {_rel(x, xr16):.2e} (vs the FP32 stencil {_rel(x, xr32):.2e})",
+          flush=True)
+    assert _rel(x, xr16) < 1e-4
 
 
 def test_multigrid_fcg_converges_and_matches_jacobi_cg(gpu):
