@@ -3748,6 +3748,7 @@ struct iga_fcg_plan {
   double* R[2];  // grid cycle: residual ping-pong (an iteration of parity k writes R[k])
   int nt0;       // row tiles of level 0 (grid cycle: partials of r.r, p.r, p.q)
   float *r32, *z32;
+  FState* host_pinned = nullptr;  // the state as the host reads it
   static constexpr int kMaxGraph = 16;
   FcgGraph gk[kMaxGraph + 1];  // gk[k]: k iterations from parity 0
   FcgGraph g1odd;              // one iteration from parity 1
@@ -3934,12 +3935,15 @@ struct iga_fcg_plan {
     // than maxiter in all
     int next = n_hist > 0 && predicted_iters() > 0 ? predicted_iters() : 4;
     for (bool look = false;; look = true) {
-      if (look && (hipMemcpyAsync(&host, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) !=
+      // the state is read into pinned memory (an asynchronous copy; a
+      // pageable destination takes a staged, blocking one)
+      if (look && (hipMemcpyAsync(host_pinned, F.st, sizeof(FState), hipMemcpyDeviceToHost, st) !=
                        hipSuccess ||
                    hipStreamSynchronize(st) != hipSuccess)) {
         status = IGA_ERR_HIP;
         break;
       }
+      if (look) host = *host_pinned;
       if ((look && host.done) || launched >= maxiter) {
         if (!look) continue;
         if (!out_res || res_fresh) break;
@@ -3983,6 +3987,7 @@ struct iga_fcg_plan {
   }
 
   ~iga_fcg_plan() {
+    if (host_pinned) (void)hipHostFree(host_pinned);
     for (int k = 0; k <= kMaxGraph + 1; ++k) {
       FcgGraph* g = k <= kMaxGraph ? &gk[k] : &g1odd;
       if (g->exec) (void)hipGraphExecDestroy(g->exec);
@@ -4002,6 +4007,11 @@ extern "C" int iga_fcg_mg_plan(const iga_mg_t* mg, int32_t n, int32_t nnz, const
   if (!indptr || !cols || !vals || !b || !x || !work || !work32 || n <= 0) return IGA_ERR_ARG;
   if (mg->level[0].n != n) return IGA_ERR_SHAPE;
   iga_fcg_plan* P = new iga_fcg_plan();
+  if (hipHostMalloc(reinterpret_cast<void**>(&P->host_pinned), sizeof(FState)) != hipSuccess) {
+    P->host_pinned = nullptr;
+    delete P;
+    return IGA_ERR_HIP;
+  }
   P->mg = *mg;
   P->n = n;
   P->nnz = nnz;

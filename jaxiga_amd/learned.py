@@ -371,6 +371,7 @@ def captured_setup(ctx, vals64, gen: ConvNet, levels=LEARNED_LEVELS):
 
 
 _DEFAULT = {"generator": None}
+_SETUP_EVENTS = {}
 
 
 def use_generator(gen):
@@ -412,9 +413,13 @@ def solve(ctx, vals64, b, gen: ConvNet = None, tol=1e-10, maxiter=1000, graph_it
 
     from jaxiga_amd import precision as PR
 
+    # the timing events of the set-up, made once per device (creating them
+    # is host work in front of the first launch)
+    ev = _SETUP_EVENTS.get(b.device)
+    if ev is None:
+        ev = _SETUP_EVENTS[b.device] = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
     op = captured_setup(ctx, vals64, gen, levels) if captured else setup(ctx, vals64, gen, levels)
     ev[1].record()
