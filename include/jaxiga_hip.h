@@ -592,9 +592,10 @@ typedef struct iga_mg_setup_grid {
   int32_t regular[16];
   /* which part of the set-up a call runs: bit 0 the level-0 stencil layouts,
    * features, copies and scalings; bit 1 the Galerkin levels, the dense tail
-   * and the coarse inverse (which read what bit 0 wrote); 0 = both. A caller
-   * may run bit 0, start work that reads the level-0 features on another
-   * stream, and run bit 1 beside it. */
+   * and the coarse inverse (which read what bit 0 wrote); 0 = both. Bit 1's
+   * work may also run as two calls: bit 2 its first launch, bit 3 the rest.
+   * A caller may run bit 0, start work that reads the level-0 features on
+   * another stream, and run bit 1 (or bits 2 and 3) beside it. */
   int32_t stages;
   /* optional (gal_multi != 0): coarse levels formed up to three at a time.
    * Level m (1 <= m < n_levels) is formed from level gal_src[m] (m-3 <= src

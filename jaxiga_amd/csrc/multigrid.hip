@@ -3342,7 +3342,7 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
         return IGA_ERR_ARG;
     }
     if (s->radius != kGR) return IGA_ERR_UNSUPPORTED;
-    if (G->stages < 0 || G->stages > 3) return IGA_ERR_ARG;
+    if (G->stages < 0 || G->stages > 15) return IGA_ERR_ARG;
     if (G->gal_multi) {
       if (!G->gal_work) return IGA_ERR_ARG;
       for (int m = 1; m < L; ++m) {
@@ -3387,7 +3387,11 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     a.feats = G->feats;
     hipLaunchKernelGGL(k_grid_ops, dim3((V.n + kOpsRows - 1) / kOpsRows, 1), dim3(kGTh), 0, st, a);
   }
-  if (G && (stages & 2)) {
+  // stage 2 whole (bit 1), or its first launch (bit 2) and the rest (bit 3)
+  const bool gal_first = (stages & 2) || (stages & 4);
+  const bool gal_rest = (stages & 2) || (stages & 8);
+  bool first_launch = true;
+  if (G && (gal_first || gal_rest)) {
     auto gal = [&](int l) {
       const iga_mg_setup_level_t& F = s->level[l];
       const iga_mg_setup_level_t& C = s->level[l + 1];
@@ -3439,15 +3443,17 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
         }
         if (nt == 0) return IGA_ERR_ARG;
         const dim3 g0((unsigned)b0, nt), g1((unsigned)b1, nt);
+        const bool p0 = first_launch ? gal_first : gal_rest;
+        first_launch = false;
         if (nt == 1) {
-          hipLaunchKernelGGL(k_gal_multi0<1>, g0, dim3(256), 0, st, m);
-          hipLaunchKernelGGL(k_gal_multi1<1>, g1, dim3(256), 0, st, m);
+          if (p0) hipLaunchKernelGGL(k_gal_multi0<1>, g0, dim3(256), 0, st, m);
+          if (gal_rest) hipLaunchKernelGGL(k_gal_multi1<1>, g1, dim3(256), 0, st, m);
         } else if (nt == 2) {
-          hipLaunchKernelGGL(k_gal_multi0<2>, g0, dim3(256), 0, st, m);
-          hipLaunchKernelGGL(k_gal_multi1<2>, g1, dim3(256), 0, st, m);
+          if (p0) hipLaunchKernelGGL(k_gal_multi0<2>, g0, dim3(256), 0, st, m);
+          if (gal_rest) hipLaunchKernelGGL(k_gal_multi1<2>, g1, dim3(256), 0, st, m);
         } else {
-          hipLaunchKernelGGL(k_gal_multi0<3>, g0, dim3(256), 0, st, m);
-          hipLaunchKernelGGL(k_gal_multi1<3>, g1, dim3(256), 0, st, m);
+          if (p0) hipLaunchKernelGGL(k_gal_multi0<3>, g0, dim3(256), 0, st, m);
+          if (gal_rest) hipLaunchKernelGGL(k_gal_multi1<3>, g1, dim3(256), 0, st, m);
         }
         src += nt;
       }
@@ -3457,8 +3463,11 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
         const iga_mg_setup_level_t& C = s->level[l + 1];
         const int64_t n0 = (int64_t)kGS * F.h * C.w, n1 = (int64_t)kGS * C.n;
         const GalLevel g = gal(l);
-        hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, g);
-        hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, g);
+        if (first_launch ? gal_first : gal_rest)
+          hipLaunchKernelGGL(k_gal_pass0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, st, g);
+        first_launch = false;
+        if (gal_rest)
+          hipLaunchKernelGGL(k_gal_pass1, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, g);
       }
     }
   } else if (!G) {
@@ -3471,7 +3480,7 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     }
   }
   IGA_CHECK_LAUNCH();
-  if (!(stages & 2)) return IGA_OK;
+  if (!gal_rest) return IGA_OK;
   // with the tensor set-up and a tail level, the dense tail (and with it the
   // coarse inverse) comes straight from the stencils
   if (G && G->tail_level > 0) return tail_dense(*s, st);

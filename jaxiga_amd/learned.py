@@ -304,18 +304,22 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=Non
             op.set_stencils(smoother_stencils(gen, op, levels), half="copy")
         return op
     # the generator reads only level 0's features: it runs on a second
-    # stream beside the Galerkin levels and the dense tail, which are
-    # enqueued first so that the host's work on the generator overlaps them
-    # (captured in a graph with the generator first, the set-up is slower)
+    # stream beside the Galerkin levels and the dense tail. The launches are
+    # enqueued (and captured) in the order the GPU can start them: level 0,
+    # the first (longest) Galerkin launch, the generator, the rest -- a graph
+    # replay submits its kernels in that order, so the generator's first one
+    # is not behind the whole Galerkin chain, nor the chain behind the
+    # generator's
     op.load(vals64, stage=1, vals_slot=vals_slot)
     main = torch.cuda.current_stream()
     level0 = torch.cuda.Event()
     level0.record(main)
-    op.load(vals64, stage=2)
+    op.load(vals64, stage=4)
     side = side_stream(op.b.device, "generator")
     side.wait_event(level0)
     with torch.cuda.stream(side):
         stencils = smoother_stencils(gen, op, levels)
+    op.load(vals64, stage=8)
     main.wait_stream(side)
     op.set_stencils(stencils, half="copy")
     return op

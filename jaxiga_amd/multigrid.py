@@ -661,9 +661,11 @@ class MGOperator:
         With the tensor set-up, ``stage=1`` writes level 0 only (its stencil
         layouts, the generator features, copies and scalings) and ``stage=2``
         the rest, so work that reads the features can run on another stream
-        beside stage 2; ``stage=0`` does both. ``vals_slot`` (tensor set-up):
-        a one-element int64 device tensor holding the address of the values,
-        which level 0 is then read through (``vals64`` is not read)."""
+        beside stage 2; ``stage=0`` does both. Stage 2 may also run as
+        ``stage=4`` (its first launch) and ``stage=8`` (the rest).
+        ``vals_slot`` (tensor set-up): a one-element int64 device tensor
+        holding the address of the values, which level 0 is then read
+        through (``vals64`` is not read)."""
         if stage and not self.split_load():
             raise ValueError("a staged load needs the tensor set-up")
         if vals_slot is not None and self._tensor is None:
@@ -677,7 +679,7 @@ class MGOperator:
         finally:
             if self._tensor is not None:
                 self._tensor.vals0_slot = None
-        if stage == 1:
+        if stage in (1, 4):
             return self
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
         self._form_tail()
