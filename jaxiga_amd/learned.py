@@ -319,9 +319,11 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=Non
     side.wait_event(level0)
     with torch.cuda.stream(side):
         stencils = smoother_stencils(gen, op, levels)
+        # the grid cycle's FP16 copy, made on the generator's stream
+        half = op.write_half(stencils[0])
     op.load(vals64, stage=8)
     main.wait_stream(side)
-    op.set_stencils(stencils, half="copy")
+    op.set_stencils(stencils, half=half)
     return op
 
 

@@ -726,6 +726,20 @@ class MGOperator:
             self._stencil_buf[l] = buf
         return buf
 
+    def write_half(self, stencil0):
+        """Write the FP16 copy of level 0's stencil (on the current stream)
+        for the grid cycle; returns the ``half`` argument that hands it to
+        :meth:`set_stencils` ("ready"), or None where the grid cycle does not
+        apply."""
+        import torch
+
+        if stencil0 is None or self._grid is None:
+            return None
+        if self._stencil16 is None:
+            self._stencil16 = torch.empty(stencil0.numel(), dtype=torch.float16, device=stencil0.device)
+        self._stencil16.copy_(stencil0.reshape(-1))
+        return "ready"
+
     def set_stencils(self, stencils, half=None):
         """Smoother stencils per level (None: damped Jacobi), copied into the
         persistent buffers unless they are those buffers. ``half``: the grid
