@@ -593,9 +593,11 @@ typedef struct iga_mg_setup_grid {
   /* which part of the set-up a call runs: bit 0 the level-0 stencil layouts,
    * features, copies and scalings; bit 1 the Galerkin levels, the dense tail
    * and the coarse inverse (which read what bit 0 wrote); 0 = both. Bit 1's
-   * work may also run as two calls: bit 2 its first launch, bit 3 the rest.
-   * A caller may run bit 0, start work that reads the level-0 features on
-   * another stream, and run bit 1 (or bits 2 and 3) beside it. */
+   * work may also run as two calls: bit 2 its first launch, bit 3 the rest;
+   * and bit 0's as bit 4 (without level 0's CSR copies) and bit 5 (the copies
+   * alone, which only the flexible CG's FP64 residual reads). A caller may run
+   * bit 0, start work that reads the level-0 features on another stream, and
+   * run bit 1 (or bits 2 and 3) beside it. */
   int32_t stages;
   /* optional (gal_multi != 0): coarse levels formed up to three at a time.
    * Level m (1 <= m < n_levels) is formed from level gal_src[m] (m-3 <= src

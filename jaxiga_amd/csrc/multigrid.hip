@@ -233,6 +233,20 @@ __global__ void k_level0_prep(int n, int64_t nnz, const int32_t* __restrict__ in
   }
 }
 
+// level 0's CSR copies (FP64 when csr64, FP32) from vals, or from the
+// address in *slot when given
+__global__ void k_csr0_copies(int64_t nnz, const double* __restrict__ vals,
+                              const double* const* __restrict__ slot, double* __restrict__ csr64,
+                              float* __restrict__ csr32) {
+  const double* v = slot ? *slot : vals;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double x = v[k];
+    if (csr64) csr64[k] = x;
+    csr32[k] = (float)x;
+  }
+}
+
 __global__ void k_copy_d(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -1811,13 +1825,13 @@ __global__ void __launch_bounds__(kGTh) k_grid_ops(GridOpsArgs a) {
       if (t < cnt) {
         sc[t] = c[k];
         sv[t] = v[k];
-        if (prep) {
+        if (prep && a.csr32) {
           if (a.csr64) a.csr64[lo + t] = v[k];
           a.csr32[lo + t] = (float)v[k];
         }
       }
     }
-  } else if (prep) {
+  } else if (prep && a.csr32) {
     for (int t = threadIdx.x; t < cnt; t += kGTh) {
       const double v64 = vals64[lo + t];
       if (a.csr64) a.csr64[lo + t] = v64;
@@ -3342,7 +3356,7 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
         return IGA_ERR_ARG;
     }
     if (s->radius != kGR) return IGA_ERR_UNSUPPORTED;
-    if (G->stages < 0 || G->stages > 15) return IGA_ERR_ARG;
+    if (G->stages < 0 || G->stages > 63) return IGA_ERR_ARG;
     if (G->gal_multi) {
       if (!G->gal_work) return IGA_ERR_ARG;
       for (int m = 1; m < L; ++m) {
@@ -3362,7 +3376,16 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     }
   }
   const int stages = G && G->stages ? G->stages : 3;
-  if (G && (stages & 1)) {
+  // level 0 with its CSR copies (bit 0), without them (bit 4), or the
+  // copies alone (bit 5)
+  if (G && (stages & 32)) {
+    const iga_mg_setup_level_t& V = s->level[0];
+    const int64_t nnz = V.nnz;
+    hipLaunchKernelGGL(k_csr0_copies, dim3((unsigned)std::min<int64_t>(blocks_for(nnz), 8192)),
+                       dim3(256), 0, st, nnz, vals0, G->vals0_slot,
+                       G->vals0_slot || V.vals64 != vals0 ? V.vals64 : nullptr, V.vals32);
+  }
+  if (G && (stages & (1 | 16))) {
     // level 0 in stencil layout (FP32, FP64, features) from its CSR
     const iga_mg_setup_level_t& V = s->level[0];
     GridOpsArgs a{};
@@ -3375,8 +3398,9 @@ extern "C" int iga_mg_load(const iga_mg_setup_t* s, const double* vals0, void* s
     // read through the slot when given (the values' address is then known
     // only to the device: always copy them)
     a.vals64_slot = G->vals0_slot;
-    a.csr64 = G->vals0_slot || V.vals64 != vals0 ? V.vals64 : nullptr;
-    a.csr32 = V.vals32;
+    const bool copies = (stages & 1) != 0;
+    a.csr64 = copies && (G->vals0_slot || V.vals64 != vals0) ? V.vals64 : nullptr;
+    a.csr32 = copies ? V.vals32 : nullptr;
     a.dinv = V.dinv;
     a.dsq = V.dsq;
     float* op32_0 = G->op32[0];

@@ -310,7 +310,9 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=Non
     # replay submits its kernels in that order, so the generator's first one
     # is not behind the whole Galerkin chain, nor the chain behind the
     # generator's
-    op.load(vals64, stage=1, vals_slot=vals_slot)
+    # level 0's CSR copies (read only by the flexible CG's FP64 residual)
+    # are written on the generator's stream too, off the Galerkin chain
+    op.load(vals64, stage=16, vals_slot=vals_slot)
     main = torch.cuda.current_stream()
     level0 = torch.cuda.Event()
     level0.record(main)
@@ -321,6 +323,7 @@ def setup(ctx, vals64, gen: ConvNet = None, levels=LEARNED_LEVELS, vals_slot=Non
         stencils = smoother_stencils(gen, op, levels)
         # the grid cycle's FP16 copy, made on the generator's stream
         half = op.write_half(stencils[0])
+        op.load(vals64, stage=32, vals_slot=vals_slot)
     op.load(vals64, stage=8)
     main.wait_stream(side)
     op.set_stencils(stencils, half=half)

@@ -662,7 +662,9 @@ class MGOperator:
         layouts, the generator features, copies and scalings) and ``stage=2``
         the rest, so work that reads the features can run on another stream
         beside stage 2; ``stage=0`` does both. Stage 2 may also run as
-        ``stage=4`` (its first launch) and ``stage=8`` (the rest).
+        ``stage=4`` (its first launch) and ``stage=8`` (the rest), stage 1 as
+        ``stage=16`` (without level 0's CSR copies) and ``stage=32`` (the
+        copies alone).
         ``vals_slot`` (tensor set-up): a one-element int64 device tensor
         holding the address of the values, which level 0 is then read
         through (``vals64`` is not read)."""
@@ -679,7 +681,7 @@ class MGOperator:
         finally:
             if self._tensor is not None:
                 self._tensor.vals0_slot = None
-        if stage in (1, 4):
+        if stage in (1, 4, 16, 32):
             return self
         self.set_stencils(stencils if stencils is not None else [None] * self.hier.n_levels)
         self._form_tail()
