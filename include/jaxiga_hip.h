@@ -431,7 +431,8 @@ int iga_neohooke(int32_t n_e, int32_t n_q, int32_t n_l, int32_t dim, const doubl
  * relu[l]   : 1 applies max(0, .) after the bias (and after the skip add)
  * H/W[0]    : input size; H/W[l+1]: output size of layer l
  * precision : 0 FP32 throughout; 1 FP16 operands (FP32 accumulation) for the
- *             stride-1 convolutions of the forward pass (inference)        */
+ *             stride-1 convolutions of the forward pass (inference); 2 as 1
+ *             with those layers' FP16 weights already in act               */
 typedef struct iga_net {
   int32_t n_layers;
   int32_t kind[16], cin[16], cout[16], stride[16], src[16], skip[16];
@@ -455,6 +456,11 @@ int iga_net_forward(const iga_net_t* net, int32_t batch, const float* params, fl
  * the last layer to `output` (the other layers use their slots of act) */
 int iga_net_infer(const iga_net_t* net, int32_t batch, const float* params, float* act,
                   const float* input, float* output, void* stream);
+/* packs the FP16 weights of the FP16 layers (precision 1 or 2) into act: a
+ * precision-1 call does it itself each time; after this call, inference at
+ * precision 2 with the same act skips it until params change */
+int iga_net_pack16(const iga_net_t* net, int32_t batch, const float* params, float* act,
+                   void* stream);
 /* backward: `gact` (same layout as act) holds d loss/d output in the output
  * slot on entry; on exit `grads` (parameter layout) and the input slot hold
  * d loss / d parameters (weights and biases) and d loss / d input */

@@ -183,6 +183,7 @@ _SIGNATURES = {
     "iga_net_forward": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p]),
     "iga_net_infer": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_void_p]),
+    "iga_net_pack16": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p]),
     "iga_net_backward": (c_int, [ctypes.POINTER(IgaNet), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p]),
     "iga_nmse": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -453,7 +453,8 @@ struct NetPlan {
 // FP16 matrix-core convolution: stride-1 forward layers of at most
 // kConvMaxC channels, when the descriptor asks for FP16 operands
 static bool f16_layer(const iga_net_t& net, int l, int Ci) {
-  return net.precision == 1 && net.kind[l] == 0 && net.stride[l] == 1 && Ci <= kConvMaxC &&
+  return (net.precision == 1 || net.precision == 2) && net.kind[l] == 0 && net.stride[l] == 1 &&
+         Ci <= kConvMaxC &&
          net.cout[l] <= kConvMaxC &&
          conv3_lds_bytes((Ci + 15) / 16 * 16, (net.cout[l] + 15) / 16 * 16) <= 64 * 1024;
 }
@@ -535,8 +536,10 @@ static int net_forward(const iga_net_t& net, int batch, const float* params, flo
         const int Cp = (Ci + 15) / 16 * 16, Cop = (Co + 15) / 16 * 16;
         _Float16* w16 = reinterpret_cast<_Float16*>(act + P.w16_off[l]);
         const int nw = Cop * 9 * Cp;
-        hipLaunchKernelGGL(k_w16_pack, dim3((nw + 255) / 256), dim3(256), 0, st, Ci, Cp, Co, Cop, w,
-                           w16);
+        // precision 2: packed already (iga_net_pack16 into the same act)
+        if (net.precision != 2)
+          hipLaunchKernelGGL(k_w16_pack, dim3((nw + 255) / 256), dim3(256), 0, st, Ci, Cp, Co, Cop,
+                             w, w16);
         const size_t lds = 0;  // static LDS (conv3_lds_bytes)
         switch (Cop / 16) {
           case 1:
@@ -670,6 +673,23 @@ extern "C" int iga_net_forward(const iga_net_t* net, int32_t batch, const float*
                                float* act, void* stream) {
   if (!net || !params || !act) return IGA_ERR_ARG;
   return net_forward(*net, batch, params, act, (hipStream_t)stream);
+}
+
+extern "C" int iga_net_pack16(const iga_net_t* net, int32_t batch, const float* params,
+                              float* act, void* stream) {
+  if (!net || !params || !act || net->n_layers < 1) return IGA_ERR_ARG;
+  const NetPlan P = plan(*net, batch);
+  hipStream_t st = (hipStream_t)stream;
+  for (int l = 0; l < net->n_layers; ++l) {
+    if (P.w16_off[l] < 0) continue;
+    const int Ci = in_C(*net, net->src[l]), Co = net->cout[l];
+    const int Cp = (Ci + 15) / 16 * 16, Cop = (Co + 15) / 16 * 16;
+    const int nw = Cop * 9 * Cp;
+    hipLaunchKernelGGL(k_w16_pack, dim3((nw + 255) / 256), dim3(256), 0, st, Ci, Cp, Co, Cop,
+                       params + net->woff[l], reinterpret_cast<_Float16*>(act + P.w16_off[l]));
+  }
+  IGA_CHECK_LAUNCH();
+  return IGA_OK;
 }
 
 extern "C" int iga_net_infer(const iga_net_t* net, int32_t batch, const float* params, float* act,

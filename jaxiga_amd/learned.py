@@ -165,25 +165,50 @@ class ConvNet:
         Ho, Wo = st.H[len(self.arch)], st.W[len(self.arch)]
         return act[out_off: out_off + B * cout * Ho * Wo].view(B, cout, Ho, Wo), act
 
+    def weights_version(self):
+        """Changes whenever the parameters may have: Adam steps (written
+        through the library), in-place torch writes, a new tensor."""
+        return (self.step, self.params.data_ptr(), self.params._version)
+
+    def _act(self, H, W, precision, device):
+        import torch
+
+        act = self.__dict__.setdefault("_acts", {}).get((H, W, precision))
+        if act is None:
+            st = self.struct(H, W, precision)
+            act = torch.empty(int(lib().iga_net_act_floats(st, 1)), dtype=torch.float32, device=device)
+            self._acts[(H, W, precision)] = act
+        return act
+
+    def pack16(self, H, W):
+        """Pack the FP16 weights that inference at (H, W) reads (into the
+        activation buffer kept for that size), on the current stream."""
+        st = self.struct(H, W, 1)
+        act = self._act(H, W, 1, self.params.device)
+        _lib.check(lib().iga_net_pack16(st, 1, _lib.ptr(self.params), _lib.ptr(act), stream()),
+                   "iga_net_pack16")
+        self.__dict__.setdefault("_packed", {})[(H, W)] = self.weights_version()
+
     def infer(self, x, out, precision=1):
         """Inference of one sample without copies: input ``x (c_in, H, W)`` and
         output ``out`` (``(c_out, H, W)`` floats) are read and written in place;
-        the hidden activations use a buffer kept per input size."""
-        import torch
-
+        the hidden activations use a buffer kept per input size. With FP16
+        operands (``precision=1``) the packed FP16 weights are kept with that
+        buffer and packed again only when the parameters have changed."""
         C, H, W = x.shape[-3:]
         if C != self.c_in:
             raise ValueError(f"network expects {self.c_in} input channels, got {C}")
         st = self.struct(H, W, precision)
-        L = lib()
-        act = self._acts.get((H, W, precision)) if hasattr(self, "_acts") else None
-        if act is None:
-            act = torch.empty(int(L.iga_net_act_floats(st, 1)), dtype=torch.float32, device=x.device)
-            self.__dict__.setdefault("_acts", {})[(H, W, precision)] = act
+        act = self._act(H, W, precision, x.device)
         if out.numel() != self.arch[-1][3] * st.H[len(self.arch)] * st.W[len(self.arch)]:
             raise ValueError("output buffer size does not match the network")
-        _lib.check(L.iga_net_infer(st, 1, _lib.ptr(self.params), _lib.ptr(act), _lib.ptr(x),
-                                   _lib.ptr(out), stream()), "iga_net_infer")
+        packed = self.__dict__.setdefault("_packed", {})
+        if precision == 1 and packed.get((H, W)) == self.weights_version():
+            st = self.struct(H, W, 2)  # the same plan, without the packing launches
+        _lib.check(lib().iga_net_infer(st, 1, _lib.ptr(self.params), _lib.ptr(act), _lib.ptr(x),
+                                       _lib.ptr(out), stream()), "iga_net_infer")
+        if precision == 1:
+            packed[(H, W)] = self.weights_version()
         return out
 
     def backward(self, act, gy, B, H, W):
@@ -354,8 +379,18 @@ class CapturedSetup:
         with torch.cuda.graph(self.graph, stream=cap):
             setup(ctx, vin, gen, levels, vals_slot=self.slot)
         torch.cuda.synchronize()
+        # the eager run packed the FP16 weights, so the captured inference
+        # reads them packed (see ConvNet.infer)
+        self.packed = gen.weights_version()
 
     def run(self, vals64):
+        # the graph's inference reads the FP16 weights packed when it was
+        # captured: pack them again if the parameters have changed since
+        gen = self.gen_ref()
+        if gen.weights_version() != self.packed:
+            lv = self.op.hier.levels[0]
+            gen.pack16(lv.h, lv.w)
+            self.packed = gen.weights_version()
         # the graph reads vals64 through the slot: it must stay alive (and
         # unchanged) until the set-up has run
         self.slot.fill_(vals64.data_ptr())

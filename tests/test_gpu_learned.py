@@ -68,6 +68,21 @@ def test_network_forward_backward_match_oracle(gpu, kind):
     # inference with FP16 operands on the matrix cores (FP32 accumulation)
     y16, _ = net.forward(torch.as_tensor(x, dtype=torch.float32, device="cuda"), precision=1)
     assert _rel(y16.double().cpu().numpy(), y_ref) < 1e-2
+    if kind == "generator":
+        # single-sample inference keeps its packed FP16 weights: the second
+        # call (no packing) equals the first, and a change of the weights is
+        # packed again before the next call
+        x0 = torch.as_tensor(x[0], dtype=torch.float32, device="cuda").contiguous()
+        n_out = arch[-1][3] * 19 * 21
+        outs = [torch.empty(n_out, device="cuda") for _ in range(3)]
+        net.infer(x0, outs[0])
+        net.infer(x0, outs[1])
+        assert torch.equal(outs[0], outs[1])
+        net.params.mul_(0.5)
+        net.infer(x0, outs[2])
+        y_half, _ = OL.forward(arch, {k: 0.5 * v for k, v in params.items()}, x[:1])
+        assert _rel(outs[2].double().cpu().numpy(), y_half.reshape(-1)) < 1e-2
+        assert _rel(outs[2].double().cpu().numpy(), outs[0].double().cpu().numpy()) > 0.1
 
 
 def _problem(refine, seed, sigma=1.5, ell=0.1):
